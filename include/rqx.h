@@ -1,0 +1,140 @@
+/*
+ * rqx.h -- C ABI of librqx.so, the gfx950 (MI355X) implementation of the
+ * SQD subspace Pauli-sum action and its helpers.
+ *
+ * Conventions
+ *   - States are uint64 bitstrings: bit q is qubit q.  The subspace index is
+ *     the sorted, unique list of real states; rows [n_states, n_total) of every
+ *     vector are filler slots (the reference fills them with 0xFF rows,
+ *     rqutils/sqd.py:473-478) and always receive 0.
+ *   - Pauli terms are grouped by X mask (groups sorted ascending by xmask,
+ *     rqutils/paulis/symplectic.py:98).  Group j owns terms
+ *     [grp_off[j], grp_off[j+1]) with Z masks zmask[] and phased coefficients
+ *     coef[] = alpha * (-i)^(popcount(xmask & zmask) & 3)
+ *     (symplectic.py:109-111), stored as interleaved (re, im) doubles.
+ *   - The action is  y[i] = sum_j C_j(s_i) * x[rank(s_i ^ xmask_j)],
+ *     C_j(s) = sum_k coef_k * (-1)^popcount(s & zmask_k)   (sqd.py:65-86).
+ *   - Vector pointers are device pointers owned by the caller; every call is
+ *     asynchronous on `stream` (a hipStream_t, NULL = default stream).
+ *   - Return value: RQX_OK or a negative status; rqx_last_error() gives the
+ *     message (thread local).  No C++ exception crosses this ABI.
+ *   - A plan may be used by one host thread / stream at a time.
+ */
+#ifndef RQX_H
+#define RQX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct rqx_plan rqx_plan;
+typedef void *rqx_stream_t;
+
+enum {
+    RQX_OK = 0,
+    RQX_ERR_ARG = -1,     /* invalid argument (sizes, ordering, null pointer) */
+    RQX_ERR_HIP = -2,     /* HIP runtime error */
+    RQX_ERR_DTYPE = -3,   /* real vector dtype requested for a complex Hamiltonian */
+    RQX_ERR_LIMIT = -4    /* size beyond a documented limit */
+};
+
+enum { RQX_F64 = 0, RQX_C128 = 1, RQX_F32 = 2, RQX_C64 = 3 };
+
+typedef struct {
+    int32_t n_qubits;
+    int32_t split_bits;     /* L: states are segmented by s >> L, searched by s & (2^L-1) */
+    int64_t n_states;       /* real (unique) states */
+    int64_t n_total;        /* vector length incl. filler slots */
+    int32_t n_groups;       /* J */
+    int32_t n_classes;      /* distinct xmask >> L */
+    int64_t n_terms;        /* T */
+    int32_t n_segments;     /* distinct s >> L */
+    int32_t n_tiles;        /* work tiles of <= 256 rows within one segment */
+    int32_t max_segment;    /* largest segment (rows) */
+    int32_t has_diagonal;   /* group 0 has xmask == 0 */
+    int32_t is_real;        /* every coefficient is real */
+    int32_t max_group_terms;
+} rqx_plan_info;
+
+/* Library version (major*10000 + minor*100 + patch). */
+int rqx_version(void);
+
+/* Last error message of the calling thread ("" if none). */
+const char *rqx_last_error(void);
+
+/*
+ * Build a plan: upload the grouped term tables and the sorted subspace index
+ * to the current device, choose the segment split and precompute the x == 0
+ * group's diagonal.  All input pointers are HOST pointers.
+ * split_bits < 0 selects the split with the host cost model.
+ * Replaces the setup half of run_sqd (rqutils/sqd.py:341-351: uniquified
+ * states + the J x N xsource cache) -- no J x N cache is built.
+ * Limits: 1 <= n_qubits <= 64, n_total < 2^31 (sqd.py:140-145).
+ */
+int rqx_plan_create(rqx_plan **out, int32_t n_qubits,
+                    const uint64_t *xmask, const int64_t *grp_off, int32_t n_groups,
+                    const uint64_t *zmask, const double *coef, int64_t n_terms,
+                    const uint64_t *sorted_states, int64_t n_states, int64_t n_total,
+                    int32_t split_bits, rqx_stream_t stream);
+
+int rqx_plan_destroy(rqx_plan *plan);
+
+int rqx_plan_get_info(const rqx_plan *plan, rqx_plan_info *out);
+
+/*
+ * idx[j * n_total + i] = rank of s_i ^ xmask_j among the real states, or -1
+ * (also -1 for filler rows).  Replaces get_xsource (rqutils/sqd.py:485-548)
+ * for all groups, i.e. the scan at sqd.py:347-351.  idx: device, J*n_total int32.
+ * vals (optional, may be NULL): device, J*n_total complex interleaved doubles;
+ * receives the matrix element C_j(s_i) where idx >= 0 and 0 elsewhere, i.e.
+ * the (columns, elements) pair that hproj scans for (rqutils/sqd.py:300-307).
+ */
+int rqx_lookup(const rqx_plan *plan, int32_t *idx, double *vals, rqx_stream_t stream);
+
+/*
+ * diag[i] = C_0(s_i) for the x == 0 group (complex, interleaved doubles,
+ * n_total entries, 0 on filler rows and when no such group exists).
+ * Replaces get_diagonal(z[0], c[0], states_u) (rqutils/sqd.py:603-628,410).
+ */
+int rqx_diagonal(const rqx_plan *plan, double *diag, rqx_stream_t stream);
+
+/*
+ * y = H_S x for one vector of length n_total (device pointers, dtype one of
+ * RQX_F64 / RQX_C128 / RQX_F32 / RQX_C64; F32/C64 accumulate in FP32).
+ * Replaces apply_h_x_cached + make_x_cached_matvec on one device
+ * (rqutils/sqd.py:694-709,746-751) and every other cache strategy
+ * (sqd.py:643-743): all give the same numbers.  x and y must not alias.
+ */
+int rqx_matvec(const rqx_plan *plan, const void *x, void *y, int32_t dtype, rqx_stream_t stream);
+
+/*
+ * counts[0] = structural nonzeros (row, group) hits incl. the diagonal,
+ * counts[1] = (row, group) pairs resolved at the segment level (the fine probes),
+ * counts[2] = segment-level class lookups.  counts: device, 3 x int64 (overwritten).
+ */
+int rqx_count(const rqx_plan *plan, int64_t *counts, rqx_stream_t stream);
+
+/*
+ * Lowest eigenpair of a small Hermitian matrix (n = 2 or 3), row-major,
+ * complex interleaved, device pointers.  out = [theta, 0, k0re, k0im, ...].
+ * Replaces eigenpair_2x2 / eigenpair_3x3 (rqutils/ground_locg.py:467-514),
+ * same formulas, with a fallback when the cross-product eigenvector vanishes.
+ */
+int rqx_eigenpair_small(const double *mat, int32_t n, double *out, rqx_stream_t stream);
+
+/*
+ * Sort and deduplicate n uint64 keys on the device: out[0..*n_unique) is the
+ * sorted unique list.  keys/out: device arrays of n entries (may not alias);
+ * n_unique: host pointer (the call synchronises `stream`).
+ * Replaces uniquify_states (rqutils/sqd.py:445-482) on uint64 keys.
+ */
+int rqx_sort_unique_u64(const uint64_t *keys, int64_t n, uint64_t *out, int64_t *n_unique,
+                        rqx_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RQX_H */

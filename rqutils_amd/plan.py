@@ -1,0 +1,201 @@
+"""Device-resident SQD plan: term tables + subspace index behind ``librqx``.
+
+A ``Plan`` replaces the setup half of ``run_sqd`` (``rqutils/sqd.py:341-356``):
+the uniquified states and the X-grouped term tables are uploaded once, and
+``Plan.matvec`` is the matrix-free action ``y = H_S x`` that
+``make_x_cached_matvec`` returns on one device (``sqd.py:746-751``).  No
+``J x N`` source-index cache is built: sources are searched on the fly.
+"""
+import ctypes
+import numpy as np
+import torch
+
+from rqutils_amd import _capi
+from rqutils_amd.paulis.symplectic import TermTables
+
+_DTYPE_CODES = {
+    torch.float64: _capi.RQX_F64,
+    torch.complex128: _capi.RQX_C128,
+    torch.float32: _capi.RQX_F32,
+    torch.complex64: _capi.RQX_C64,
+}
+
+
+def _require_cuda():
+    if not torch.cuda.is_available():
+        raise RuntimeError('rqutils_amd needs a ROCm GPU (torch.cuda.is_available() is False)')
+
+
+class Plan:
+    """Term tables + sorted uint64 subspace index resident on one GPU.
+
+    Args:
+        tables: X-grouped terms (``TermTables``).
+        sorted_states: sorted, unique uint64 states (numpy array or CPU tensor).
+        n_total: vector length (>= number of states); rows past the states are
+            filler slots that always receive 0 (``sqd.py:473-478``).
+        device: CUDA device (default: current).
+        split_bits: segment split L (default: chosen by the library's cost model).
+    """
+
+    def __init__(self, tables: TermTables, sorted_states, n_total=None, device=None, split_bits=-1):
+        _require_cuda()
+        lib = _capi.lib()
+        self.tables = tables
+        states = np.ascontiguousarray(np.asarray(sorted_states, dtype=np.uint64).reshape(-1))
+        self.n_states = int(states.shape[0])
+        self.n_total = int(self.n_states if n_total is None else n_total)
+        if self.n_total < self.n_states:
+            raise ValueError('n_total smaller than the number of states')
+        self.device = torch.device('cuda', torch.cuda.current_device() if device is None else
+                                   torch.device(device).index or 0)
+        xmask = np.ascontiguousarray(tables.xmask, dtype=np.uint64)
+        grp_off = np.ascontiguousarray(tables.grp_off, dtype=np.int64)
+        zmask = np.ascontiguousarray(tables.zmask, dtype=np.uint64)
+        coef = np.ascontiguousarray(tables.coef, dtype=np.complex128).view(np.float64)
+        handle = ctypes.c_void_p()
+        with torch.cuda.device(self.device):
+            stream = _capi.current_stream(self.device)
+            rc = lib.rqx_plan_create(ctypes.byref(handle), int(tables.num_qubits),
+                                     _capi.ptr(xmask), _capi.ptr(grp_off), int(tables.num_groups),
+                                     _capi.ptr(zmask), _capi.ptr(coef), int(tables.num_terms),
+                                     _capi.ptr(states), self.n_states, self.n_total, int(split_bits),
+                                     stream)
+        _capi.check(rc)
+        self._handle = handle
+        self._states = states
+        info = _capi.PlanInfo()
+        _capi.check(lib.rqx_plan_get_info(self._handle, ctypes.byref(info)))
+        self.info = info.as_dict()
+        self.is_real = bool(self.info['is_real'])
+
+    @classmethod
+    def full_space(cls, tables: TermTables, device=None):
+        """Plan over the complete ``2^n`` computational basis (``symplectic.py:130-149``)."""
+        n = tables.num_qubits
+        if n > 30:
+            raise ValueError('full-space action needs num_qubits <= 30')
+        return cls(tables, np.arange(1 << n, dtype=np.uint64), device=device)
+
+    # ------------------------------------------------------------------ lifetime
+    def close(self):
+        handle = getattr(self, '_handle', None)
+        if handle is not None and handle.value:
+            lib = _capi.lib()
+            with torch.cuda.device(self.device):
+                torch.cuda.current_stream(self.device).synchronize()
+                rc = lib.rqx_plan_destroy(handle)
+            self._handle = None
+            _capi.check(rc)
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # pragma: no cover - interpreter shutdown
+            pass
+
+    @property
+    def states(self):
+        return self._states
+
+    # ------------------------------------------------------------------ kernels
+    def _vec(self, x):
+        if not isinstance(x, torch.Tensor):
+            raise TypeError('Plan.matvec expects a torch tensor on the plan device')
+        if x.device != self.device:
+            raise ValueError(f'vector is on {x.device}, plan on {self.device}')
+        if x.dim() != 1 or x.shape[0] != self.n_total:
+            raise ValueError(f'vector must have shape ({self.n_total},), got {tuple(x.shape)}')
+        if x.dtype not in _DTYPE_CODES:
+            raise ValueError(f'unsupported dtype {x.dtype}')
+        return x.contiguous()
+
+    def matvec(self, x, out=None):
+        """``y = H_S x`` (device tensor of dtype float64/complex128/float32/complex64)."""
+        x = self._vec(x)
+        if out is None:
+            out = torch.empty_like(x)
+        elif out.dtype != x.dtype or out.shape != x.shape or out.device != x.device or not out.is_contiguous():
+            raise ValueError('out must be a contiguous tensor like x')
+        stream = _capi.current_stream(self.device)
+        _capi.check(_capi.lib().rqx_matvec(self._handle, x.data_ptr(), out.data_ptr(),
+                                           _DTYPE_CODES[x.dtype], stream))
+        return out
+
+    __call__ = matvec
+
+    def lookup(self, values=False):
+        """``(J, n_total)`` int32 source indices (-1 where absent), ``get_xsource`` for all groups.
+
+        With ``values=True`` also returns the ``(J, n_total)`` complex128 matrix
+        elements ``C_j(s_i)`` at the hits (0 elsewhere).
+        """
+        shape = (self.info['n_groups'], self.n_total)
+        out = torch.empty(shape, dtype=torch.int32, device=self.device)
+        vals = torch.empty(shape, dtype=torch.complex128, device=self.device) if values else None
+        stream = _capi.current_stream(self.device)
+        _capi.check(_capi.lib().rqx_lookup(self._handle, out.data_ptr() if out.numel() else None,
+                                           vals.data_ptr() if (vals is not None and vals.numel()) else None,
+                                           stream))
+        return (out, vals) if values else out
+
+    def diagonal(self):
+        """Complex ``C_0(s)`` of the x == 0 group for every row (0 on filler rows)."""
+        out = torch.empty(self.n_total, dtype=torch.complex128, device=self.device)
+        stream = _capi.current_stream(self.device)
+        _capi.check(_capi.lib().rqx_diagonal(self._handle, out.data_ptr() if out.numel() else None, stream))
+        return out
+
+    def count(self):
+        """Structural nonzeros, segment-level probes and class lookups of one matvec."""
+        out = torch.zeros(3, dtype=torch.int64, device=self.device)
+        stream = _capi.current_stream(self.device)
+        _capi.check(_capi.lib().rqx_count(self._handle, out.data_ptr(), stream))
+        hits, probes, lookups = (int(v) for v in out.cpu())
+        return {'hits': hits, 'probes': probes, 'class_lookups': lookups}
+
+    def algorithmic_bytes(self, dtype, hits=None):
+        """Per-matvec algorithmic bytes of the HBM model.
+
+        ``8 N + 8 P + s_v H + s_v N + T (8 + s_c) + 12 J`` with ``P = N J'``
+        (every (row, non-diagonal group) pair is resolved through the two-level
+        index), ``H`` structural nonzeros incl. the diagonal, ``s_v`` the vector
+        element bytes and ``s_c`` the coefficient bytes read per term.
+        """
+        s_v = torch.empty(0, dtype=dtype).element_size()
+        s_c = 16 if dtype in (torch.float64, torch.complex128) else 8
+        n = self.n_states
+        J = self.info['n_groups']
+        Jp = J - (1 if self.info['has_diagonal'] else 0)
+        T = self.info['n_terms']
+        if hits is None:
+            hits = self.count()['hits']
+        return 8 * n + 8 * n * Jp + s_v * hits + s_v * n + T * (8 + s_c) + 12 * J
+
+
+def sort_unique_u64(keys: torch.Tensor):
+    """Sorted unique copy of a uint64 device tensor (``uniquify_states``, ``sqd.py:445-482``)."""
+    _require_cuda()
+    if keys.dtype not in (torch.uint64, torch.int64):
+        raise ValueError('keys must be a 64-bit integer tensor')
+    keys = keys.contiguous()
+    out = torch.empty_like(keys)
+    n_unique = ctypes.c_int64(0)
+    stream = _capi.current_stream(keys.device)
+    with torch.cuda.device(keys.device):
+        _capi.check(_capi.lib().rqx_sort_unique_u64(keys.data_ptr() if keys.numel() else None, keys.numel(),
+                                                    out.data_ptr() if out.numel() else None,
+                                                    ctypes.byref(n_unique), stream))
+    return out[:n_unique.value]
+
+
+def eigenpair_small(mat: torch.Tensor):
+    """Lowest eigenpair of a 2x2 / 3x3 Hermitian device matrix -> (theta, kappa) device tensors."""
+    n = mat.shape[0]
+    m = mat.to(torch.complex128).contiguous()
+    out = torch.empty(2 + 2 * n, dtype=torch.float64, device=mat.device)
+    stream = _capi.current_stream(mat.device)
+    _capi.check(_capi.lib().rqx_eigenpair_small(m.data_ptr(), n, out.data_ptr(), stream))
+    theta = out[0]
+    kappa = torch.view_as_complex(out[2:].reshape(n, 2))
+    return theta, kappa

@@ -1,0 +1,177 @@
+"""GPU LOCG and the drop-in sqd / run_sqd / hproj / matmul entry points against the oracle."""
+import sys
+
+import numpy as np
+import pytest
+import scipy.linalg
+import torch
+
+from oracle import dense, locgref, sqdref
+from rqutils_amd import workloads
+from rqutils_amd.ground_locg import ground_locg
+from rqutils_amd.paulis.symplectic import PauliSumXZ
+from rqutils_amd.plan import Plan
+from rqutils_amd.sqd import SQDOptions, hproj, run_sqd, sqd
+
+pytestmark = pytest.mark.gpu
+DEV = 'cuda:0'
+
+
+def _say(msg):
+    print(f'[case] {msg}', flush=True)
+    sys.stdout.flush()
+
+
+def _tfim(nq, hx):
+    paulis, coeffs = [], []
+    for q in range(nq):
+        s = ['I'] * nq
+        s[q] = 'Z'
+        s[(q + 1) % nq] = 'Z'
+        paulis.append(''.join(s))
+        coeffs.append(-1.)
+    for q in range(nq):
+        s = ['I'] * nq
+        s[q] = 'X'
+        paulis.append(''.join(s))
+        coeffs.append(hx)
+    return paulis, coeffs
+
+
+def _tfim_e0(nq, hx):
+    k = np.pi * (2 * np.arange(nq) + 1) / nq
+    return -np.sum(np.sqrt(1. + hx * hx - 2. * hx * np.cos(k)))
+
+
+def _random_problem(seed, n=10, t=80, ns=400):
+    rng = np.random.default_rng(seed)
+    codes = rng.choice(4, size=(t, n), p=(0.5, 0.2, 0.15, 0.15))
+    paulis = [''.join('IXYZ'[c] for c in row) for row in codes]
+    coeffs = rng.standard_normal(t)
+    keys = np.sort(rng.choice(1 << n, size=ns, replace=False).astype(np.uint64))
+    return paulis, coeffs, keys
+
+
+def test_locg_tfim_closed_form():
+    _say('locg tfim')
+    nq, hx = 12, 0.8
+    h = PauliSumXZ.from_paulisum(_tfim(nq, hx))
+    plan = Plan.full_space(h.tables)
+    x0 = torch.from_numpy(np.random.default_rng(0).standard_normal(1 << nq)).to(DEV)
+    theta, vec, niter = ground_locg(plan.matvec, x0, maxiter=1000)
+    assert abs(float(theta) - _tfim_e0(nq, hx)) < 1e-9 * nq
+    assert abs(torch.linalg.vector_norm(vec).item() - 1) < 1e-10
+
+
+def test_locg_matches_oracle_trajectory():
+    _say('locg vs oracle')
+    paulis, coeffs, keys = _random_problem(1)
+    h = PauliSumXZ.from_paulisum((paulis, coeffs))
+    plan = Plan(h.tables, keys)
+    hs = dense.projected_matrix(paulis, coeffs, keys)
+    x0 = np.random.default_rng(2).standard_normal(400) + 1j * np.random.default_rng(3).standard_normal(400)
+    theta_o, vec_o, it_o = locgref.ground_locg(lambda v: hs @ v, x0, maxiter=40, tol=1e-300)
+    theta_g, vec_g, it_g = ground_locg(plan.matvec, torch.from_numpy(x0).to(DEV), maxiter=40, tol=1e-300)
+    assert it_o == it_g == 40
+    w = scipy.linalg.eigvalsh(hs)[0]
+    # same iteration sequence: the 40-iteration Rayleigh values agree far below the remaining error
+    assert abs(float(theta_g) - theta_o) < 1e-9 * abs(w)
+    theta, vec, _ = ground_locg(plan.matvec, torch.from_numpy(x0).to(DEV), maxiter=3000)
+    assert abs(float(theta) - w) < 1e-9 * abs(w)
+
+
+def test_locg_orth_and_dense_matrix():
+    _say('locg orth/dense')
+    nq, hx = 8, 1.3
+    hmat = dense.full_matrix(*_tfim(nq, hx), nq).real
+    w, v = np.linalg.eigh(hmat)
+    x0 = np.random.default_rng(4).standard_normal(1 << nq)
+    theta, vec, _ = ground_locg(torch.from_numpy(hmat).to(DEV), torch.from_numpy(x0).to(DEV),
+                                orth=torch.from_numpy(v[:, 0]).to(DEV), maxiter=2000)
+    assert abs(float(theta) - w[1]) < 1e-8 * abs(w[1])
+    # integer start index with a dense matrix
+    theta, vec, _ = ground_locg(torch.from_numpy(hmat).to(DEV), 3, maxiter=2000)
+    assert abs(float(theta) - w[0]) < 1e-8 * abs(w[0])
+
+
+def test_sqd_end_to_end_matches_dense_eigh():
+    """The method of tests/sqd.ipynb: dense eigh of the projected Hamiltonian."""
+    _say('sqd end to end')
+    rng = np.random.default_rng(5)
+    nq, nsample, nterm = 10, 3000, 100
+    states = rng.choice(2, size=(nsample, nq)).astype(np.uint8)
+    paulis = [''.join('IXYZ'[ip] for ip in row) for row in rng.choice(4, size=(nterm, nq))]
+    coeffs = rng.uniform(-1., 1., size=nterm)
+    eigval, eigvec, basis = sqd((paulis, coeffs), states, options={'locg_maxiter': 3000})
+    hmat = dense.full_matrix(paulis, coeffs, nq)
+    states_u = np.unique(states, axis=0)
+    idx = np.sum(states_u * (1 << np.arange(nq)[::-1])[None, :], axis=1)
+    hp = hmat[idx][:, idx]
+    w, vv = np.linalg.eigh(hp)
+    assert np.array_equal(basis, states_u)
+    assert abs(eigval - w[0]) < 1e-8 * abs(w[0])
+    assert abs(abs(np.vdot(vv[:, 0], eigvec)) - 1) < 1e-6
+    # energy only
+    e = sqd((paulis, coeffs), states, options=SQDOptions(return_eigvec=False, locg_maxiter=3000))
+    assert abs(e - w[0]) < 1e-8 * abs(w[0])
+
+
+def test_run_sqd_packed_with_fillers():
+    _say('run_sqd packed + fillers')
+    paulis, coeffs, keys = _random_problem(6, n=11, t=60, ns=500)
+    states = ((keys[:, None] >> np.arange(10, -1, -1, dtype=np.uint64)) & np.uint64(1)).astype(np.uint8)
+    states = np.concatenate([states, states[:50]])       # duplicates
+    states_p = sqdref.pack_states(states)
+    h = PauliSumXZ.from_paulisum((paulis, coeffs), force_real=True, add_padding=True)
+    e, vec, states_u, dim = run_sqd(h, states_p, 600, options=SQDOptions(locg_maxiter=3000))
+    assert dim == 500
+    want_u = sqdref.uniquify_states(states_p, 600)
+    assert np.array_equal(states_u, want_u)
+    assert vec.shape == (600,) and torch.count_nonzero(vec[500:]) == 0
+    hs = dense.projected_matrix(paulis, coeffs, keys)
+    w = scipy.linalg.eigvalsh(hs)[0]
+    assert abs(float(e) - w) < 1e-8 * abs(w)
+    # vinit rule: one-hot at argmin of the diagonal
+    x, z, c, _ = (h.x, h.z, h.c, h.num_qubits)
+    assert sqdref.vinit(x, z, c, want_u) == int(np.argmin(np.real(np.diag(hs))))
+
+
+def test_hproj_matches_dense():
+    _say('hproj')
+    paulis, coeffs, keys = _random_problem(7, n=9, t=50, ns=200)
+    states = ((keys[:, None] >> np.arange(8, -1, -1, dtype=np.uint64)) & np.uint64(1)).astype(np.uint8)
+    m = hproj((paulis, coeffs), states[::-1])
+    hs = dense.projected_matrix(paulis, coeffs, keys)
+    assert np.allclose(m.toarray(), hs, rtol=0, atol=1e-12)
+
+
+def test_matmul_full_space_matches_dense():
+    _say('matmul')
+    rng = np.random.default_rng(8)
+    codes = rng.choice(4, size=(40, 8))
+    paulis = [''.join('IXYZ'[c] for c in row) for row in codes]
+    coeffs = rng.standard_normal(40)
+    h = PauliSumXZ.from_paulisum((paulis, coeffs))
+    v = rng.standard_normal((256, 3)) + 1j * rng.standard_normal((256, 3))
+    got = h.matmul(v)
+    assert np.allclose(got, dense.full_matrix(paulis, coeffs, 8) @ v, atol=1e-12)
+
+
+def test_locg_jw_matches_eigsh():
+    _say('locg jw small vs eigsh')
+    import scipy.sparse.linalg
+    w = workloads.config1(n_alpha=40, n_beta=40)
+    plan = Plan(w.tables, w.states)
+    cols, vals = plan.lookup(values=True)
+    n = plan.n_states
+    hit = (cols >= 0).cpu().numpy()
+    rows = np.nonzero(hit)[1]
+    data = vals.cpu().numpy()[hit].real
+    cc = cols.cpu().numpy()[hit]
+    import scipy.sparse
+    hs = scipy.sparse.csr_matrix((data, (rows, cc)), shape=(n, n))
+    e_ref = scipy.sparse.linalg.eigsh(hs, k=1, which='SA', tol=1e-12)[0][0]
+    diag = plan.diagonal().real
+    theta, vec, niter = ground_locg(plan.matvec, torch.argmin(diag), maxiter=2000,
+                                   vspace=(n, torch.float64, plan.device))
+    assert abs(float(theta) - e_ref) < 1e-9 * abs(e_ref)

@@ -1,0 +1,261 @@
+"""Parity of the gfx950 lookup / matvec kernels with the oracle (GPU)."""
+import os
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import dense, sqdref
+from rqutils_amd import workloads
+from rqutils_amd.paulis.symplectic import PauliSumXZ, TermTables
+from rqutils_amd.plan import Plan, eigenpair_small, sort_unique_u64
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'golden')
+DEV = 'cuda:0'
+
+# tolerances written per dtype: FP64 paths must agree to 1e-12 relative (max norm)
+RTOL64 = 1e-12
+RTOL32 = 2e-5
+
+
+def _say(msg):
+    print(f'[case] {msg}', flush=True)
+    sys.stdout.flush()
+
+
+def _groups(t: TermTables):
+    return [(t.zmask[t.grp_off[j]:t.grp_off[j + 1]], t.coef[t.grp_off[j]:t.grp_off[j + 1]])
+            for j in range(t.num_groups)]
+
+
+def _oracle_lookup(t, keys, n_total):
+    out = np.full((t.num_groups, n_total), -1, dtype=np.int64)
+    for j, xm in enumerate(t.xmask):
+        out[j, :keys.shape[0]] = sqdref.xsource_searchsorted(keys, xm)
+    return out
+
+
+def _rel(a, b):
+    return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
+
+
+@pytest.fixture(scope='module')
+def cfg0():
+    return workloads.config0()
+
+
+def test_lookup_bit_exact_cfg0(cfg0):
+    _say('lookup cfg0')
+    plan = Plan(cfg0.tables, cfg0.states)
+    got = plan.lookup().cpu().numpy()
+    want = _oracle_lookup(cfg0.tables, cfg0.states, plan.n_total)
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize('split', [-1, 0, 7, 13])
+def test_matvec_c128_cfg0_all_splits(cfg0, split):
+    _say(f'matvec c128 cfg0 split={split}')
+    plan = Plan(cfg0.tables, cfg0.states, split_bits=split)
+    rng = np.random.default_rng(2)
+    v = rng.standard_normal(plan.n_total) + 1j * rng.standard_normal(plan.n_total)
+    y = plan.matvec(torch.from_numpy(v).to(DEV)).cpu().numpy()
+    want = sqdref.matvec_u64(cfg0.tables.xmask, _groups(cfg0.tables), cfg0.states, v)
+    assert _rel(y, want) < RTOL64
+
+
+def test_matvec_bitwise_identical_across_splits(cfg0):
+    _say('matvec split invariance')
+    rng = np.random.default_rng(3)
+    v = torch.from_numpy(rng.standard_normal(8192) + 1j * rng.standard_normal(8192)).to(DEV)
+    ys = [Plan(cfg0.tables, cfg0.states, split_bits=s).matvec(v) for s in (0, 5, 11, 13)]
+    for y in ys[1:]:
+        assert torch.equal(y, ys[0])
+
+
+def test_matvec_c64_cfg0(cfg0):
+    _say('matvec c64 cfg0')
+    plan = Plan(cfg0.tables, cfg0.states)
+    rng = np.random.default_rng(4)
+    v = rng.standard_normal(plan.n_total) + 1j * rng.standard_normal(plan.n_total)
+    y = plan.matvec(torch.from_numpy(v.astype(np.complex64)).to(DEV)).cpu().numpy()
+    want = sqdref.matvec_u64(cfg0.tables.xmask, _groups(cfg0.tables), cfg0.states, v.astype(np.complex64))
+    assert _rel(y, want) < RTOL32
+
+
+def test_real_dtype_rejected_for_complex_hamiltonian(cfg0):
+    plan = Plan(cfg0.tables, cfg0.states)
+    with pytest.raises(ValueError, match='complex'):
+        plan.matvec(torch.zeros(plan.n_total, dtype=torch.float64, device=DEV))
+
+
+@pytest.fixture(scope='module')
+def jw_small():
+    return workloads.config1(n_alpha=60, n_beta=50)
+
+
+def test_lookup_and_matvec_jw_small(jw_small):
+    _say('jw small lookup/matvec')
+    t, keys = jw_small.tables, jw_small.states
+    plan = Plan(t, keys, n_total=keys.shape[0] + 17)
+    assert plan.is_real
+    got = plan.lookup().cpu().numpy()
+    assert np.array_equal(got, _oracle_lookup(t, keys, plan.n_total))
+    rng = np.random.default_rng(5)
+    v = rng.standard_normal(plan.n_total)
+    v[keys.shape[0]:] = rng.standard_normal(17)      # filler inputs must not leak
+    want = np.zeros(plan.n_total, dtype=np.complex128)
+    want[:keys.shape[0]] = sqdref.matvec_u64(t.xmask, _groups(t), keys, v[:keys.shape[0]])
+    y64 = plan.matvec(torch.from_numpy(v).to(DEV)).cpu().numpy()
+    assert _rel(y64, want.real) < RTOL64
+    assert np.all(y64[keys.shape[0]:] == 0)
+    yc = plan.matvec(torch.from_numpy(v.astype(np.complex128)).to(DEV)).cpu().numpy()
+    assert _rel(yc, want) < RTOL64
+    y32 = plan.matvec(torch.from_numpy(v.astype(np.float32)).to(DEV)).cpu().numpy()
+    assert _rel(y32, want.real) < RTOL32
+    d = plan.diagonal().cpu().numpy()
+    z0, c0 = _groups(t)[0]
+    assert np.allclose(d[:keys.shape[0]], sqdref.diagonal_u64(z0, c0, keys), rtol=0, atol=1e-12)
+    assert np.all(d[keys.shape[0]:] == 0)
+
+
+@pytest.mark.parametrize('name', ['small_10q', 'filler_12q'])
+def test_golden_fixtures(name):
+    _say(f'golden {name}')
+    f = np.load(os.path.join(HERE, f'{name}.npz'))
+    h = PauliSumXZ.from_paulisum(([str(p) for p in f['paulis']], f['coeffs']), add_padding=True)
+    plan = Plan(h.tables, f['keys'], n_total=int(f['states_size']))
+    n = f['keys'].shape[0]
+    assert np.array_equal(plan.lookup().cpu().numpy()[:, :n], f['xsource'])
+    y = plan.matvec(torch.from_numpy(f['v']).to(DEV)).cpu().numpy()
+    assert _rel(y, f['y']) < RTOL64
+
+
+def test_dense_projection_small():
+    _say('dense projection')
+    rng = np.random.default_rng(8)
+    n = 10
+    codes = rng.choice(4, size=(80, n), p=(0.5, 0.2, 0.15, 0.15))
+    paulis = [''.join('IXYZ'[c] for c in row) for row in codes]
+    coeffs = rng.standard_normal(80)
+    keys = np.sort(rng.choice(1 << n, size=400, replace=False).astype(np.uint64))
+    h = PauliSumXZ.from_paulisum((paulis, coeffs))
+    plan = Plan(h.tables, keys)
+    hs = dense.projected_matrix(paulis, coeffs, keys)
+    eye = torch.eye(400, dtype=torch.complex128, device=DEV)
+    cols = torch.stack([plan.matvec(eye[:, k].contiguous()) for k in range(400)], dim=1).cpu().numpy()
+    assert np.allclose(cols, hs, rtol=0, atol=1e-12 * np.abs(hs).max())
+
+
+def test_edge_cases():
+    _say('edge cases')
+    t = TermTables.from_masks(np.array([0, 3], np.uint64), np.array([1, 2], np.uint64), np.array([0.5, -1.]), 4)
+    # one state: only the diagonal survives
+    plan = Plan(t, np.array([5], np.uint64), n_total=3)
+    y = plan.matvec(torch.tensor([2., 7., 7.], dtype=torch.float64, device=DEV)).cpu().numpy()
+    want_d = 0.5 * (1 - 2 * (bin(5 & 1).count('1') & 1))
+    assert np.allclose(y, [want_d * 2., 0, 0])
+    # empty subspace
+    plan = Plan(t, np.zeros(0, np.uint64), n_total=4)
+    y = plan.matvec(torch.ones(4, dtype=torch.complex128, device=DEV))
+    assert torch.count_nonzero(y) == 0
+    # no diagonal group
+    t2 = TermTables.from_masks(np.array([1, 6], np.uint64), np.array([0, 4], np.uint64), np.array([1., 2.]), 3)
+    keys = np.arange(8, dtype=np.uint64)
+    plan = Plan(t2, keys)
+    assert plan.info['has_diagonal'] == 0
+    v = np.arange(8, dtype=np.float64) + 1
+    want = sqdref.matvec_u64(t2.xmask, _groups(t2), keys, v).real
+    assert np.allclose(plan.matvec(torch.from_numpy(v).to(DEV)).cpu().numpy(), want)
+    # no terms at all
+    t3 = TermTables.from_masks(np.zeros(0, np.uint64), np.zeros(0, np.uint64), np.zeros(0), 5)
+    plan = Plan(t3, keys)
+    assert torch.count_nonzero(plan.matvec(torch.ones(8, dtype=torch.float64, device=DEV))) == 0
+
+
+def test_plan_argument_errors():
+    t = TermTables.from_masks(np.array([0], np.uint64), np.array([1], np.uint64), np.array([1.]), 4)
+    with pytest.raises(ValueError, match='sorted'):
+        Plan(t, np.array([3, 1], np.uint64))
+    with pytest.raises(ValueError, match='beyond'):
+        Plan(t, np.array([1, 99], np.uint64))
+    plan = Plan(t, np.array([1, 2], np.uint64))
+    with pytest.raises(ValueError, match='shape'):
+        plan.matvec(torch.zeros(3, dtype=torch.float64, device=DEV))
+
+
+def test_sort_unique():
+    _say('sort unique')
+    rng = np.random.default_rng(9)
+    keys = rng.integers(0, 1 << 40, size=100_000, dtype=np.uint64)
+    keys[::7] = keys[0]
+    got = sort_unique_u64(torch.from_numpy(keys.view(np.int64)).to(DEV)).cpu().numpy().view(np.uint64)
+    assert np.array_equal(got, np.unique(keys))
+    big = rng.integers(0, 1 << 63, size=1000, dtype=np.uint64) | np.uint64(1 << 63)
+    got = sort_unique_u64(torch.from_numpy(big.view(np.int64)).to(DEV)).cpu().numpy().view(np.uint64)
+    assert np.array_equal(got, np.unique(big))
+
+
+@pytest.mark.parametrize('n', [2, 3])
+def test_eigenpair_small_matches_eigh(n):
+    rng = np.random.default_rng(20 + n)
+    for k in range(40):
+        a = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+        if k % 4 == 1:
+            a = a.real.astype(complex)
+        a = (a + a.conj().T) / 2
+        if k % 4 == 2:
+            a = np.diag(np.diag(a).real).astype(complex)
+        theta, kappa = eigenpair_small(torch.from_numpy(a).to(DEV))
+        theta, kappa = float(theta), kappa.cpu().numpy()
+        w = np.linalg.eigvalsh(a)
+        assert abs(theta - w[0]) < 1e-10 * max(1, abs(w).max())
+        assert abs(np.linalg.norm(kappa) - 1) < 1e-10
+        assert np.linalg.norm(a @ kappa - theta * kappa) < 1e-8
+    # degenerate lowest eigenvalue and a multiple of the identity
+    for a in (np.diag([1., 1., 3.])[:n, :n], 2. * np.eye(n)):
+        theta, kappa = eigenpair_small(torch.from_numpy(a.astype(complex)).to(DEV))
+        kappa = kappa.cpu().numpy()
+        assert abs(float(theta) - np.linalg.eigvalsh(a)[0]) < 1e-12
+        assert abs(np.linalg.norm(kappa) - 1) < 1e-10
+        assert np.linalg.norm(a @ kappa - float(theta) * kappa) < 1e-10
+
+
+def test_hermitian_and_linear_cfg1_full():
+    """Size-independent properties at the full 1M-determinant shape."""
+    _say('hermiticity cfg1 full')
+    w = workloads.config1()
+    plan = Plan(w.tables, w.states)
+    rng = np.random.default_rng(6)
+    u = torch.from_numpy(rng.standard_normal(plan.n_total)).to(DEV)
+    v = torch.from_numpy(rng.standard_normal(plan.n_total)).to(DEV)
+    hu, hv = plan.matvec(u), plan.matvec(v)
+    lhs, rhs = torch.dot(u, hv).item(), torch.dot(hu, v).item()
+    assert abs(lhs - rhs) < 1e-10 * (abs(lhs) + 1)
+    hw = plan.matvec(2.5 * u - 0.5 * v)
+    assert torch.allclose(hw, 2.5 * hu - 0.5 * hv, rtol=0, atol=1e-11 * hu.abs().max().item())
+    # diagonal vs oracle on a sample of rows, and the counts line up
+    d = plan.diagonal().real.cpu().numpy()
+    z0, c0 = _groups(w.tables)[0]
+    sample = np.linspace(0, plan.n_states - 1, 2000).astype(np.int64)
+    assert np.allclose(d[sample], sqdref.diagonal_u64(z0, c0, w.states[sample]).real, atol=1e-12)
+    cnt = plan.count()
+    assert cnt['hits'] > plan.n_states
+    # row sample of the matvec against the oracle (only the sampled rows' sources are needed)
+    e = torch.zeros(plan.n_total, dtype=torch.float64, device=DEV)
+    e[12345] = 1.
+    col = plan.matvec(e).cpu().numpy()      # column 12345 of H_S (H is symmetric)
+    t = w.tables
+    keys = w.states
+    s = keys[12345]
+    want = np.zeros(plan.n_total)
+    for j in range(t.num_groups):
+        tgt = s ^ t.xmask[j]
+        pos = np.searchsorted(keys, tgt)
+        if pos < keys.shape[0] and keys[pos] == tgt:
+            zs, cs = t.zmask[t.grp_off[j]:t.grp_off[j + 1]], t.coef[t.grp_off[j]:t.grp_off[j + 1]]
+            # H[pos, 12345] = C_j(s_pos) for the group mapping s_pos -> s
+            val = np.sum(cs * (1. - 2. * (np.bitwise_count(keys[pos] & zs) & 1)))
+            want[pos] += val.real
+    assert _rel(col, want) < RTOL64

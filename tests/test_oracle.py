@@ -1,0 +1,212 @@
+"""Pins of the CPU oracle (no GPU): closed forms, brute force and textbook matrices."""
+import numpy as np
+import pytest
+import scipy.linalg
+import scipy.sparse.linalg
+
+from oracle import dense, locgref, sqdref, terms
+
+
+def _random_paulis(rng, n, t, probs=(0.4, 0.2, 0.2, 0.2)):
+    codes = rng.choice(4, size=(t, n), p=probs)
+    return [''.join('IXYZ'[c] for c in row) for row in codes]
+
+
+def _ops_from_tables(x, groups, n):
+    """sum_j sum_k c_jk Z^z X^x as a dense 2^n matrix (a = b ^ x, sign from the row a)."""
+    dim = 1 << n
+    b = np.arange(dim, dtype=np.uint64)
+    out = np.zeros((dim, dim), dtype=np.complex128)
+    for xm, (zs, cs) in zip(x, groups):
+        a = b ^ np.uint64(xm)
+        for zk, ck in zip(zs, cs):
+            sign = 1. - 2. * (np.bitwise_count(a & np.uint64(zk)) & 1)
+            out[a.astype(np.int64), b.astype(np.int64)] += ck * sign
+    return out
+
+
+def test_single_qubit_textbook_matrices():
+    sx = np.array([[0, 1], [1, 0]], dtype=complex)
+    sy = np.array([[0, -1j], [1j, 0]])
+    sz = np.diag([1., -1.]).astype(complex)
+    for ch, mat in (('X', sx), ('Y', sy), ('Z', sz), ('I', np.eye(2))):
+        assert np.allclose(dense.full_matrix([ch], [1.], 1), mat)
+    # Y = -i Z X: the (-i)^{x.z} phase of the symplectic form
+    assert np.allclose(sy, -1j * sz @ sx)
+
+
+def test_two_qubit_order_is_big_endian():
+    # 'XI': X on qubit 1 (the most significant bit of the basis index)
+    sx = np.array([[0, 1], [1, 0]], dtype=complex)
+    assert np.allclose(dense.full_matrix(['XI'], [1.], 2), np.kron(sx, np.eye(2)))
+    assert np.allclose(dense.full_matrix(['IX'], [1.], 2), np.kron(np.eye(2), sx))
+
+
+@pytest.mark.parametrize('padded', [False, True])
+@pytest.mark.parametrize('seed', [0, 1, 2])
+def test_term_tables_reproduce_dense_pauli_sum(seed, padded):
+    rng = np.random.default_rng(seed)
+    n = 5
+    paulis = _random_paulis(rng, n, 40)
+    paulis += paulis[:5]          # duplicates are summed
+    coeffs = rng.standard_normal(len(paulis))
+    x, z, c, nq = terms.from_paulisum_tuple(paulis, coeffs, add_padding=padded)
+    assert nq == n
+    xm, groups = terms.tables_as_u64(x, z, c, n, padded)
+    assert np.all(np.diff(xm.astype(np.int64)) > 0)          # groups ascending by x
+    got = _ops_from_tables(xm, groups, n)
+    want = dense.full_matrix(paulis, coeffs, n)
+    assert np.allclose(got, want, atol=1e-12)
+    # a plausible mistake, the conjugate phase (+i)^{x.z}, must be caught
+    bad = [(zs, cs * (1j ** (2 * (np.bitwise_count(np.uint64(xj) & zs) & 1))))
+           for xj, (zs, cs) in zip(xm, groups)]
+    if any(np.any(np.bitwise_count(np.uint64(xj) & zs) & 1) for xj, (zs, _) in zip(xm, groups)):
+        assert not np.allclose(_ops_from_tables(xm, bad, n), want, atol=1e-9)
+
+
+def test_real_coefficients_are_real_dtype():
+    x, z, c, _ = terms.from_paulisum_tuple(['XX', 'YY', 'ZI'], [1., 2., 3.])
+    assert c.dtype == np.float64
+    x, z, c, _ = terms.from_paulisum_tuple(['XY', 'ZI'], [1., 3.])
+    assert c.dtype == np.complex128
+
+
+def test_packed_xsource_matches_searchsorted_and_bruteforce():
+    rng = np.random.default_rng(5)
+    n = 11
+    keys = np.sort(rng.choice(1 << n, size=300, replace=False).astype(np.uint64))
+    states = ((keys[:, None] >> np.arange(n - 1, -1, -1, dtype=np.uint64)) & np.uint64(1)).astype(np.uint8)
+    states_p = sqdref.pack_states(states)
+    lookup = {int(k): i for i, k in enumerate(keys)}
+    for xm in [0, 1, 5, 0x400, 0x7ff, int(rng.integers(1, 1 << n))]:
+        xbits = ((np.uint64(xm) >> np.arange(n - 1, -1, -1, dtype=np.uint64)) & np.uint64(1)).astype(np.uint8)
+        xsig = np.packbits(np.pad(xbits, (1, 0)))
+        got = sqdref.get_xsource(xsig, states_p)
+        ss = sqdref.xsource_searchsorted(keys, xm)
+        brute = np.array([lookup.get(int(k) ^ xm, -1) for k in keys])
+        assert np.array_equal(got, ss)
+        assert np.array_equal(ss, brute)
+
+
+def test_uniquify_fills_with_255():
+    rng = np.random.default_rng(3)
+    states = rng.integers(0, 2, size=(50, 6)).astype(np.uint8)
+    states_p = sqdref.pack_states(states)
+    u = sqdref.uniquify_states(states_p, 80)
+    nu = np.unique(states, axis=0).shape[0]
+    assert sqdref.subspace_dim(u) == nu
+    assert np.all(u[nu:] == 255)
+    assert np.array_equal(u[:nu], sqdref.pack_states(np.unique(states, axis=0)))
+
+
+@pytest.mark.parametrize('seed', [0, 1])
+def test_oracle_matvec_matches_dense_projection(seed):
+    rng = np.random.default_rng(seed)
+    n = 9
+    paulis = _random_paulis(rng, n, 60)
+    coeffs = rng.standard_normal(60)
+    keys = np.sort(rng.choice(1 << n, size=200, replace=False).astype(np.uint64))
+    states = ((keys[:, None] >> np.arange(n - 1, -1, -1, dtype=np.uint64)) & np.uint64(1)).astype(np.uint8)
+    v = rng.standard_normal(200) + 1j * rng.standard_normal(200)
+    hs = dense.projected_matrix(paulis, coeffs, keys)
+    want = hs @ v
+    # packed restatement of the reference (strategy (1, 0))
+    x, z, c, _ = terms.from_paulisum_tuple(paulis, coeffs, add_padding=True)
+    states_u = sqdref.uniquify_states(sqdref.pack_states(states), 200)
+    xs = sqdref.xsources_all(x, states_u)
+    got = sqdref.apply_h_x_cached(v, xs, z, c, states_u)
+    assert np.allclose(got, want, rtol=0, atol=1e-12 * np.abs(want).max())
+    # uint64 form
+    xm, groups = terms.tables_as_u64(x, z, c, n, True)
+    got2 = sqdref.matvec_u64(xm, groups, keys, v)
+    assert np.allclose(got2, want, rtol=0, atol=1e-12 * np.abs(want).max())
+    assert np.allclose(hs, hs.conj().T)
+
+
+def _rand_herm(rng, n, complex_=True):
+    a = rng.standard_normal((n, n)) + (1j * rng.standard_normal((n, n)) if complex_ else 0)
+    return (a + np.conjugate(a).T) / 2
+
+
+@pytest.mark.parametrize('n', [2, 3])
+@pytest.mark.parametrize('complex_', [False, True])
+def test_eigenpairs_match_eigh(n, complex_):
+    rng = np.random.default_rng(10 * n + complex_)
+    fn = locgref.eigenpair_2x2 if n == 2 else locgref.eigenpair_3x3
+    for _ in range(50):
+        a = _rand_herm(rng, n, complex_)
+        lam, vec = fn(a)
+        w = np.linalg.eigvalsh(a)
+        assert abs(lam - w[0]) < 1e-10 * max(1, abs(w).max())
+        assert abs(np.linalg.norm(vec) - 1) < 1e-10
+        assert np.linalg.norm(a @ vec - lam * vec) < 1e-8
+
+
+def test_tfim_closed_form_matches_eigh():
+    for nq, hx in ((6, 0.5), (8, 1.0), (7, 1.7)):
+        e_closed = _tfim_e0(nq, hx)
+        paulis, coeffs = _tfim_paulis(nq, hx)
+        w = np.linalg.eigvalsh(dense.full_matrix(paulis, coeffs, nq))
+        assert abs(e_closed - w[0]) < 1e-10 * nq
+
+
+def _tfim_e0(nq, hx):
+    k = np.pi * (2 * np.arange(nq) + 1) / nq
+    return -np.sum(np.sqrt(1. + hx * hx - 2. * hx * np.cos(k)))
+
+
+def _tfim_paulis(nq, hx):
+    """Periodic chain -sum Z_i Z_{i+1} + hx sum X_i (ground_locg_demo.py:15-60)."""
+    paulis, coeffs = [], []
+    for q in range(nq):
+        s = ['I'] * nq
+        s[q] = 'Z'
+        s[(q + 1) % nq] = 'Z'
+        paulis.append(''.join(s))
+        coeffs.append(-1.)
+    for q in range(nq):
+        s = ['I'] * nq
+        s[q] = 'X'
+        paulis.append(''.join(s))
+        coeffs.append(hx)
+    return paulis, coeffs
+
+
+def test_locg_oracle_tfim_and_eigsh():
+    nq, hx = 10, 0.9
+    paulis, coeffs = _tfim_paulis(nq, hx)
+    keys = np.arange(1 << nq, dtype=np.uint64)
+    hs = scipy.sparse.csr_matrix(dense.projected_matrix(paulis, coeffs, keys).real)
+    rng = np.random.default_rng(1)
+    x0 = rng.standard_normal(1 << nq)
+    theta, vec, niter = locgref.ground_locg(lambda v: hs @ v, x0, maxiter=500)
+    assert abs(theta - _tfim_e0(nq, hx)) < 1e-9 * nq
+    w = scipy.sparse.linalg.eigsh(hs, k=1, which='SA')[0][0]
+    assert abs(theta - w) < 1e-9 * nq
+    assert abs(np.linalg.norm(vec) - 1) < 1e-10
+
+
+def test_locg_oracle_random_projected_complex():
+    rng = np.random.default_rng(7)
+    n = 9
+    paulis = _random_paulis(rng, n, 80)
+    coeffs = rng.standard_normal(80)
+    keys = np.sort(rng.choice(1 << n, size=300, replace=False).astype(np.uint64))
+    hs = dense.projected_matrix(paulis, coeffs, keys)
+    x0 = rng.standard_normal(300) + 1j * rng.standard_normal(300)
+    trace = []
+    theta, vec, niter = locgref.ground_locg(lambda v: hs @ v, x0, maxiter=2000, trace=trace)
+    w = scipy.linalg.eigvalsh(hs)[0]
+    assert abs(theta - w) < 1e-8 * abs(w)
+    assert np.all(np.diff(trace) <= 1e-9 * abs(w))     # Rayleigh-Ritz values never increase
+
+
+def test_locg_oracle_orth_deflation_gives_first_excited():
+    nq, hx = 8, 1.3
+    paulis, coeffs = _tfim_paulis(nq, hx)
+    h = dense.full_matrix(paulis, coeffs, nq).real
+    w, v = np.linalg.eigh(h)
+    rng = np.random.default_rng(2)
+    theta, vec, _ = locgref.ground_locg(lambda x: h @ x, rng.standard_normal(1 << nq), orth=(v[:, 0],),
+                                        maxiter=2000)
+    assert abs(theta - w[1]) < 1e-8 * abs(w[1])
