@@ -124,6 +124,9 @@ int rqx_count(const rqx_plan *plan, int64_t *counts, rqx_stream_t stream);
  */
 int rqx_eigenpair_small(const double *mat, int32_t n, double *out, rqx_stream_t stream);
 
+/* The same routine on HOST pointers (runs on the calling CPU thread). */
+int rqx_eigenpair_small_host(const double *mat, int32_t n, double *out);
+
 /*
  * Sort and deduplicate n uint64 keys on the device: out[0..*n_unique) is the
  * sorted unique list.  keys/out: device arrays of n entries (may not alias);

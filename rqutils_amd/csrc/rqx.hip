@@ -146,7 +146,7 @@ __global__ __launch_bounds__(TILE) void k_tiles(const KParams p, const void *__r
                                                 void *__restrict__ yout) {
     using V = typename VT<R, CPLX>::T;
     using CC = typename C2<R>::T;
-    __shared__ uint32_t s_keys[SEG_CAP];
+    extern __shared__ uint32_t s_keys[];      // staged target segment (max_segment keys)
     __shared__ int32_t s_tseg[TILE];
 
     const int tid = threadIdx.x;
@@ -191,14 +191,7 @@ __global__ __launch_bounds__(TILE) void k_tiles(const KParams p, const void *__r
             uint32_t g0 = p.cls_goff[cb + cc];
             const uint32_t g1 = p.cls_goff[cb + cc + 1];
             if (p.has_diag && g0 == 0) g0 = 1;
-            for (uint32_t g = g0; g < g1; ++g) {
-                const uint32_t tl = slo ^ p.g_xlo[g];
-                uint32_t base = 0, len = tn;
-                while (len > 1) {
-                    const uint32_t half = len >> 1;
-                    base = (s_keys[base + half] <= tl) ? base + half : base;
-                    len -= half;
-                }
+            auto handle = [&](uint32_t g, uint32_t base, uint32_t tl) {
                 const bool hit = active && (s_keys[base] == tl);
                 if constexpr (MODE == MODE_MATVEC) {
                     if (hit) fma_acc(acc, group_coef<R>(p, g, s), x[ta + base]);
@@ -212,6 +205,36 @@ __global__ __launch_bounds__(TILE) void k_tiles(const KParams p, const void *__r
                     n_hit += hit;
                     n_probe += active;
                 }
+            };
+            uint32_t g = g0;
+            // four independent searches in lock step: the dependent LDS loads of
+            // one search overlap with those of the other three
+            for (; g + 4 <= g1; g += 4) {
+                const uint32_t t0 = slo ^ p.g_xlo[g], t1 = slo ^ p.g_xlo[g + 1];
+                const uint32_t t2 = slo ^ p.g_xlo[g + 2], t3 = slo ^ p.g_xlo[g + 3];
+                uint32_t b0 = 0, b1 = 0, b2 = 0, b3 = 0, len = tn;
+                while (len > 1) {
+                    const uint32_t half = len >> 1;
+                    b0 = (s_keys[b0 + half] <= t0) ? b0 + half : b0;
+                    b1 = (s_keys[b1 + half] <= t1) ? b1 + half : b1;
+                    b2 = (s_keys[b2 + half] <= t2) ? b2 + half : b2;
+                    b3 = (s_keys[b3 + half] <= t3) ? b3 + half : b3;
+                    len -= half;
+                }
+                handle(g, b0, t0);
+                handle(g + 1, b1, t1);
+                handle(g + 2, b2, t2);
+                handle(g + 3, b3, t3);
+            }
+            for (; g < g1; ++g) {
+                const uint32_t tl = slo ^ p.g_xlo[g];
+                uint32_t base = 0, len = tn;
+                while (len > 1) {
+                    const uint32_t half = len >> 1;
+                    base = (s_keys[base + half] <= tl) ? base + half : base;
+                    len -= half;
+                }
+                handle(g, base, tl);
             }
             __syncthreads();          // before the next target overwrites s_keys
         }
@@ -260,62 +283,150 @@ __global__ void k_diag(const uint64_t *__restrict__ keys, int64_t n, const uint6
 struct cd {
     double r, i;
 };
-__device__ inline cd cadd(cd a, cd b) { return {a.r + b.r, a.i + b.i}; }
-__device__ inline cd csub(cd a, cd b) { return {a.r - b.r, a.i - b.i}; }
-__device__ inline cd cmul(cd a, cd b) { return {a.r * b.r - a.i * b.i, a.r * b.i + a.i * b.r}; }
-__device__ inline cd cconj(cd a) { return {a.r, -a.i}; }
-__device__ inline double cabs2(cd a) { return a.r * a.r + a.i * a.i; }
-__device__ inline cd cdiv(cd a, cd b) {
+__host__ __device__ inline cd csub(cd a, cd b) { return {a.r - b.r, a.i - b.i}; }
+__host__ __device__ inline cd cmul(cd a, cd b) { return {a.r * b.r - a.i * b.i, a.r * b.i + a.i * b.r}; }
+__host__ __device__ inline cd cconj(cd a) { return {a.r, -a.i}; }
+__host__ __device__ inline double cabs2(cd a) { return a.r * a.r + a.i * a.i; }
+__host__ __device__ inline cd cdiv(cd a, cd b) {
     const double d = cabs2(b);
     return {(a.r * b.r + a.i * b.i) / d, (a.i * b.r - a.r * b.i) / d};
 }
-
-__device__ inline void cross3(const cd *a, const cd *b, cd *o) {
+__host__ __device__ inline void cross3(const cd *a, const cd *b, cd *o) {
     o[0] = csub(cmul(a[1], b[2]), cmul(a[2], b[1]));
     o[1] = csub(cmul(a[2], b[0]), cmul(a[0], b[2]));
     o[2] = csub(cmul(a[0], b[1]), cmul(a[1], b[0]));
 }
+__host__ __device__ inline double norm2_3(const cd *a) { return cabs2(a[0]) + cabs2(a[1]) + cabs2(a[2]); }
 
-__global__ void k_eigen_small(const double *__restrict__ m, int n, double *__restrict__ out) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// eigenvector of the 3x3 Hermitian A for eigenvalue lam: conjugated cross product
+// of two columns of A - lam I (ground_locg.py:506-513)
+__host__ __device__ inline void vec3_at(const cd (*A)[3], double lam, cd *v) {
+    cd col[3][3];
+    double fro2 = 0.0;
+    for (int c = 0; c < 3; ++c)
+        for (int r = 0; r < 3; ++r) {
+            col[c][r] = (r == c) ? cd{A[r][c].r - lam, A[r][c].i} : A[r][c];
+            fro2 += cabs2(col[c][r]);
+        }
+    cd w12[3], w01[3], w02[3];
+    cross3(col[1], col[2], w12);
+    cross3(col[0], col[1], w01);
+    cross3(col[0], col[2], w02);
+    const double n12 = norm2_3(w12), n01 = norm2_3(w01), n02 = norm2_3(w02);
+    const double nmax = fmax(n12, fmax(n01, n02));
+    cd w[3];
+    double nw;
+    if (nmax > 1e-28 * fro2 * fro2 && isfinite(nmax)) {
+        // the reference's column pair (1, 2) unless it is ill-conditioned
+        // relative to the best pair (then its direction is rounding noise)
+        const cd *src = w12;
+        nw = n12;
+        if (!(n12 >= 1e-6 * nmax)) {
+            if (n01 >= n02) {
+                src = w01;
+                nw = n01;
+            } else {
+                src = w02;
+                nw = n02;
+            }
+        }
+        for (int k = 0; k < 3; ++k) w[k] = src[k];
+    } else {
+        // eigenvalue of multiplicity >= 2: a vector orthogonal to the dominant column
+        double best = -1.0;
+        int bc = 0;
+        for (int c = 0; c < 3; ++c) {
+            const double nc = norm2_3(col[c]);
+            if (nc > best) {
+                best = nc;
+                bc = c;
+            }
+        }
+        if (best > 1e-28 * fro2 && best > 0.0) {
+            int kmin = 0;
+            for (int k = 1; k < 3; ++k)
+                if (cabs2(col[bc][k]) < cabs2(col[bc][kmin])) kmin = k;
+            cd e[3] = {{0, 0}, {0, 0}, {0, 0}};
+            e[kmin] = {1.0, 0.0};
+            cross3(col[bc], e, w);
+        } else {
+            w[0] = {1.0, 0.0};
+            w[1] = {0.0, 0.0};
+            w[2] = {0.0, 0.0};
+        }
+        nw = norm2_3(w);
+    }
+    const double inv = 1.0 / sqrt(nw);
+    for (int k = 0; k < 3; ++k) v[k] = {w[k].r * inv, -w[k].i * inv};
+}
+
+__host__ __device__ inline double rayleigh(const cd (*A)[3], const cd *v, int n) {
+    double acc = 0.0;
+    for (int r = 0; r < n; ++r)
+        for (int c = 0; c < n; ++c) acc += cmul(cconj(v[r]), cmul(A[r][c], v[c])).r;
+    return acc;
+}
+
+// eigenvector of the 2x2 Hermitian A for eigenvalue lam: the reference's mediant
+// form [(d1 - lam + conj(b)) / (d0 - lam + b), -1] (ground_locg.py:474) unless its
+// denominator is negligible, else the null vector of the larger row of A - lam I
+__host__ __device__ inline void vec2_at(const cd (*A)[3], double lam, cd *v) {
+    const double d0 = A[0][0].r, d1 = A[1][1].r;
+    const cd b = A[1][0];
+    const cd num = {d1 - lam + b.r, -b.i};
+    const cd den = {d0 - lam + b.r, b.i};
+    const double scale = (d0 - lam) * (d0 - lam) + (d1 - lam) * (d1 - lam) + cabs2(b);
+    if (cabs2(den) > 1e-24 * scale && scale > 0.0) {
+        const cd v0 = cdiv(num, den);
+        const double nrm = sqrt(cabs2(v0) + 1.0);
+        if (isfinite(nrm)) {
+            v[0] = {v0.r / nrm, v0.i / nrm};
+            v[1] = {-1.0 / nrm, 0.0};
+            return;
+        }
+    }
+    const cd a1 = {d1 - lam, 0.0}, mb = {-b.r, -b.i};
+    const cd a2 = cconj(b), m0 = {-(d0 - lam), 0.0};
+    cd w0 = a1, w1 = mb;
+    if (cabs2(a2) + cabs2(m0) > cabs2(a1) + cabs2(mb)) {
+        w0 = a2;
+        w1 = m0;
+    }
+    const double nrm = sqrt(cabs2(w0) + cabs2(w1));
+    if (nrm > 0.0) {
+        v[0] = {w0.r / nrm, w0.i / nrm};
+        v[1] = {w1.r / nrm, w1.i / nrm};
+    } else {
+        v[0] = {1.0, 0.0};
+        v[1] = {0.0, 0.0};
+    }
+}
+
+// Lowest eigenpair of a 2x2 / 3x3 Hermitian matrix m (row-major, complex
+// interleaved); out = [lambda, 0, v0re, v0im, ...].
+__host__ __device__ inline void eigen_small(const double *m, int n, double *out) {
     cd A[3][3];
     for (int r = 0; r < n; ++r)
         for (int c = 0; c < n; ++c) A[r][c] = {m[2 * (r * n + c)], m[2 * (r * n + c) + 1]};
     double lam;
     cd v[3] = {{0, 0}, {0, 0}, {0, 0}};
     if (n == 2) {
-        // ground_locg.py:467-478
+        // ground_locg.py:467-478: eigenvalue from trace/determinant, mediant-form vector
         const double d0 = A[0][0].r, d1 = A[1][1].r;
-        const cd b = A[1][0];
-        const double det = d0 * d1 - cabs2(b);
+        const double det = d0 * d1 - cabs2(A[1][0]);
         const double tr = d0 + d1;
-        lam = (tr - sqrt(tr * tr - 4.0 * det)) * 0.5;
-        const cd num = {d1 - lam + b.r, -b.i};
-        const cd den = {d0 - lam + b.r, b.i};
-        cd v0 = cdiv(num, den);
-        double nrm = sqrt(cabs2(v0) + 1.0);
-        if (cabs2(den) > 0.0 && isfinite(nrm)) {
-            v[0] = {v0.r / nrm, v0.i / nrm};
-            v[1] = {-1.0 / nrm, 0.0};
-        } else {
-            // null vector of a row of A - lam I (mediant denominator vanished)
-            const cd a1 = {d1 - lam, 0.0}, mb = {-b.r, -b.i};
-            const cd a2 = cconj(b), m0 = {-(d0 - lam), 0.0};
-            cd w0 = a1, w1 = mb;
-            if (cabs2(a2) + cabs2(m0) > cabs2(a1) + cabs2(mb)) {
-                w0 = a2;
-                w1 = m0;
-            }
-            nrm = sqrt(cabs2(w0) + cabs2(w1));
-            if (nrm > 0.0) {
-                v[0] = {w0.r / nrm, w0.i / nrm};
-                v[1] = {w1.r / nrm, w1.i / nrm};
-            } else {
-                v[0] = {1.0, 0.0};
-            }
+        lam = (tr - sqrt(fmax(tr * tr - 4.0 * det, 0.0))) * 0.5;
+        vec2_at(A, lam, v);
+        for (int it = 0; it < 2; ++it) {   // Rayleigh-quotient refinement
+            const double rq = rayleigh(A, v, 2);
+            if (!isfinite(rq)) break;
+            lam = rq;
+            vec2_at(A, lam, v);
         }
     } else {
-        // ground_locg.py:481-514 (J. Kopp, Int. J. Mod. Phys. C 19, 523 (2008))
+        // ground_locg.py:481-514 (J. Kopp, Int. J. Mod. Phys. C 19, 523 (2008)):
+        // characteristic polynomial, trigonometric root, eigenvector as the
+        // conjugated cross product of two columns of A - lam I
         const double d[3] = {A[0][0].r, A[1][1].r, A[2][2].r};
         const double mo[3] = {cabs2(A[1][0]), cabs2(A[2][0]), cabs2(A[2][1])};
         const double c2 = -(d[0] + d[1] + d[2]);
@@ -326,60 +437,17 @@ __global__ void k_eigen_small(const double *__restrict__ m, int n, double *__res
         const double p = c2 * c2 - 3.0 * c1;
         const double q = -13.5 * c0 - c2 * c2 * c2 + 4.5 * c2 * c1;
         const double phi =
-            atan2(sqrt(27.0 * (0.25 * c1 * c1 * (p - c1) + c0 * (q + 6.75 * c0))), q) / 3.0;
+            atan2(sqrt(fmax(0.0, 27.0 * (0.25 * c1 * c1 * (p - c1) + c0 * (q + 6.75 * c0)))), q) / 3.0;
         const double cphi = cos(phi), sphi = sin(phi), s3 = sqrt(3.0);
         const double xmin = fmin(fmin(2.0 * cphi, -cphi - s3 * sphi), -cphi + s3 * sphi);
-        lam = sqrt(p) / 3.0 * xmin - c2 / 3.0;
-        cd col[3][3];
-        for (int c = 0; c < 3; ++c)
-            for (int r = 0; r < 3; ++r) col[c][r] = (r == c) ? cd{A[r][c].r - lam, A[r][c].i} : A[r][c];
-        cd w[3];
-        cross3(col[1], col[2], w);
-        double nw = cabs2(w[0]) + cabs2(w[1]) + cabs2(w[2]);
-        const double scale = (cabs2(col[1][0]) + cabs2(col[1][1]) + cabs2(col[1][2])) *
-                             (cabs2(col[2][0]) + cabs2(col[2][1]) + cabs2(col[2][2]));
-        if (!(nw > 1e-28 * scale) || !isfinite(nw)) {
-            // fallback: best of the other column pairs, then a vector orthogonal
-            // to the dominant column (eigenvalue of multiplicity >= 2)
-            const int pairs[2][2] = {{0, 1}, {0, 2}};
-            for (int pi = 0; pi < 2; ++pi) {
-                cd w2[3];
-                cross3(col[pairs[pi][0]], col[pairs[pi][1]], w2);
-                const double n2 = cabs2(w2[0]) + cabs2(w2[1]) + cabs2(w2[2]);
-                if (n2 > nw && isfinite(n2)) {
-                    nw = n2;
-                    w[0] = w2[0];
-                    w[1] = w2[1];
-                    w[2] = w2[2];
-                }
-            }
-            double best = -1.0;
-            int bc = 0;
-            for (int c = 0; c < 3; ++c) {
-                const double nc = cabs2(col[c][0]) + cabs2(col[c][1]) + cabs2(col[c][2]);
-                if (nc > best) {
-                    best = nc;
-                    bc = c;
-                }
-            }
-            if (!(nw > 1e-28 * best * best)) {
-                if (best > 0.0) {
-                    int kmin = 0;
-                    for (int k = 1; k < 3; ++k)
-                        if (cabs2(col[bc][k]) < cabs2(col[bc][kmin])) kmin = k;
-                    cd e[3] = {{0, 0}, {0, 0}, {0, 0}};
-                    e[kmin] = {1.0, 0.0};
-                    cross3(col[bc], e, w);
-                } else {
-                    w[0] = {1.0, 0.0};
-                    w[1] = {0.0, 0.0};
-                    w[2] = {0.0, 0.0};
-                }
-                nw = cabs2(w[0]) + cabs2(w[1]) + cabs2(w[2]);
-            }
+        lam = sqrt(fmax(p, 0.0)) / 3.0 * xmin - c2 / 3.0;
+        vec3_at(A, lam, v);
+        for (int it = 0; it < 2; ++it) {   // Rayleigh-quotient refinement
+            const double rq = rayleigh(A, v, 3);
+            if (!isfinite(rq)) break;
+            lam = rq;
+            vec3_at(A, lam, v);
         }
-        const double inv = 1.0 / sqrt(nw);
-        for (int k = 0; k < 3; ++k) v[k] = {w[k].r * inv, -w[k].i * inv};
     }
     out[0] = lam;
     out[1] = 0.0;
@@ -387,6 +455,10 @@ __global__ void k_eigen_small(const double *__restrict__ m, int n, double *__res
         out[2 + 2 * k] = v[k].r;
         out[3 + 2 * k] = v[k].i;
     }
+}
+
+__global__ void k_eigen_small(const double *__restrict__ m, int n, double *__restrict__ out) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) eigen_small(m, n, out);
 }
 
 // ---------------------------------------------------------------- host: index build
@@ -512,6 +584,9 @@ struct rqx_plan {
 };
 
 namespace {
+
+// dynamic LDS of the tile kernel: one staged segment of uint32 low keys
+size_t seg_lds_bytes(const rqx_plan *p) { return static_cast<size_t>(std::max(p->max_seg, 1)) * 4; }
 
 KParams make_params(const rqx_plan *p, bool fp64) {
     KParams k{};
@@ -806,7 +881,7 @@ int rqx_lookup(const rqx_plan *p, int32_t *idx, double *vals, rqx_stream_t strea
         KParams k = make_params(p, true);
         k.idx = idx;
         k.vals = reinterpret_cast<double2 *>(vals);
-        hipLaunchKernelGGL((k_tiles<MODE_LOOKUP, double, true>), dim3(p->n_tiles), dim3(TILE), 0, st, k,
+        hipLaunchKernelGGL((k_tiles<MODE_LOOKUP, double, true>), dim3(p->n_tiles), dim3(TILE), seg_lds_bytes(p), st, k,
                            nullptr, nullptr);
         HIP_TRY(hipGetLastError());
     }
@@ -845,11 +920,12 @@ int rqx_matvec(const rqx_plan *p, const void *x, void *y, int32_t dtype, rqx_str
     KParams k = make_params(p, fp64);
     if (p->n_tiles > 0) {
         const dim3 grid(p->n_tiles), block(TILE);
+        const size_t lds = seg_lds_bytes(p);
         switch (dtype) {
-            case RQX_F64: hipLaunchKernelGGL((k_tiles<MODE_MATVEC, double, false>), grid, block, 0, st, k, x, y); break;
-            case RQX_C128: hipLaunchKernelGGL((k_tiles<MODE_MATVEC, double, true>), grid, block, 0, st, k, x, y); break;
-            case RQX_F32: hipLaunchKernelGGL((k_tiles<MODE_MATVEC, float, false>), grid, block, 0, st, k, x, y); break;
-            default: hipLaunchKernelGGL((k_tiles<MODE_MATVEC, float, true>), grid, block, 0, st, k, x, y); break;
+            case RQX_F64: hipLaunchKernelGGL((k_tiles<MODE_MATVEC, double, false>), grid, block, lds, st, k, x, y); break;
+            case RQX_C128: hipLaunchKernelGGL((k_tiles<MODE_MATVEC, double, true>), grid, block, lds, st, k, x, y); break;
+            case RQX_F32: hipLaunchKernelGGL((k_tiles<MODE_MATVEC, float, false>), grid, block, lds, st, k, x, y); break;
+            default: hipLaunchKernelGGL((k_tiles<MODE_MATVEC, float, true>), grid, block, lds, st, k, x, y); break;
         }
         HIP_TRY(hipGetLastError());
     }
@@ -876,7 +952,7 @@ int rqx_count(const rqx_plan *p, int64_t *counts, rqx_stream_t stream) {
     if (p->n_tiles > 0) {
         KParams k = make_params(p, true);
         k.counts = reinterpret_cast<unsigned long long *>(counts);
-        hipLaunchKernelGGL((k_tiles<MODE_COUNT, double, true>), dim3(p->n_tiles), dim3(TILE), 0, st, k, nullptr,
+        hipLaunchKernelGGL((k_tiles<MODE_COUNT, double, true>), dim3(p->n_tiles), dim3(TILE), seg_lds_bytes(p), st, k, nullptr,
                            nullptr);
         HIP_TRY(hipGetLastError());
     }
@@ -889,6 +965,14 @@ int rqx_eigenpair_small(const double *mat, int32_t n, double *out, rqx_stream_t 
     if (!mat || !out) return fail(RQX_ERR_ARG, "NULL argument");
     hipLaunchKernelGGL(k_eigen_small, dim3(1), dim3(64), 0, static_cast<hipStream_t>(stream), mat, n, out);
     HIP_TRY(hipGetLastError());
+    return RQX_OK;
+}
+
+int rqx_eigenpair_small_host(const double *mat, int32_t n, double *out) {
+    g_err.clear();
+    if (n != 2 && n != 3) return fail(RQX_ERR_ARG, "n must be 2 or 3");
+    if (!mat || !out) return fail(RQX_ERR_ARG, "NULL argument");
+    eigen_small(mat, n, out);
     return RQX_OK;
 }
 

@@ -164,10 +164,11 @@ def test_locg_jw_matches_eigsh():
     plan = Plan(w.tables, w.states)
     cols, vals = plan.lookup(values=True)
     n = plan.n_states
-    hit = (cols >= 0).cpu().numpy()
-    rows = np.nonzero(hit)[1]
-    data = vals.cpu().numpy()[hit].real
-    cc = cols.cpu().numpy()[hit]
+    hit = cols >= 0
+    rows = torch.nonzero(hit, as_tuple=True)[1].cpu().numpy()
+    cc = cols[hit].cpu().numpy()
+    data = vals[hit].real.cpu().numpy()
+    del cols, vals, hit
     import scipy.sparse
     hs = scipy.sparse.csr_matrix((data, (rows, cc)), shape=(n, n))
     e_ref = scipy.sparse.linalg.eigsh(hs, k=1, which='SA', tol=1e-12)[0][0]

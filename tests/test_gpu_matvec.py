@@ -150,7 +150,9 @@ def test_dense_projection_small():
 
 def test_edge_cases():
     _say('edge cases')
-    t = TermTables.from_masks(np.array([0, 3], np.uint64), np.array([1, 2], np.uint64), np.array([0.5, -1.]), 4)
+    # x=3, z=3 is a YY-type term: phase (-i)^2 = -1 keeps the Hamiltonian real
+    t = TermTables.from_masks(np.array([0, 3], np.uint64), np.array([1, 3], np.uint64), np.array([0.5, -1.]), 4)
+    assert t.is_real
     # one state: only the diagonal survives
     plan = Plan(t, np.array([5], np.uint64), n_total=3)
     y = plan.matvec(torch.tensor([2., 7., 7.], dtype=torch.float64, device=DEV)).cpu().numpy()
@@ -161,7 +163,8 @@ def test_edge_cases():
     y = plan.matvec(torch.ones(4, dtype=torch.complex128, device=DEV))
     assert torch.count_nonzero(y) == 0
     # no diagonal group
-    t2 = TermTables.from_masks(np.array([1, 6], np.uint64), np.array([0, 4], np.uint64), np.array([1., 2.]), 3)
+    t2 = TermTables.from_masks(np.array([1, 6], np.uint64), np.array([0, 6], np.uint64), np.array([1., 2.]), 3)
+    assert t2.is_real
     keys = np.arange(8, dtype=np.uint64)
     plan = Plan(t2, keys)
     assert plan.info['has_diagonal'] == 0
