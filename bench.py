@@ -66,6 +66,59 @@ def cpu_baseline(w, alg_bytes, budget_s):
     }
 
 
+def locg_time_to_solution(plan, complex_, maxiter, mixed_tol, target=1e-8):
+    """Time to ``target`` relative ground-energy error: plain FP64 LOCG vs mixed precision.
+
+    Both start from the reference's one-hot vector at the argmin of the x = 0
+    diagonal (``sqd.py:406-426``).  The plain run follows the reference
+    sequence (4 matvecs per iteration) to its own stopping rule; its final
+    energy is E*, and its time-to-target is the first iteration within
+    ``target`` of E*.  The mixed run stops by its own rule (no knowledge of
+    E*); its time is the whole call, its error is measured against E*.
+    """
+    import torch
+    from rqutils_amd.ground_locg import ground_locg
+    from rqutils_amd.locg_mixed import ground_locg_mixed
+
+    hi = torch.complex128 if complex_ else torch.float64
+    n = plan.n_total
+    diag = plan.diagonal().real[:plan.n_states]
+    v0 = int(torch.argmin(diag))
+    x0 = torch.zeros(n, dtype=hi, device=plan.device)
+    x0[v0] = 1
+    trace = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    theta, _, niter = ground_locg(plan.matvec, x0.clone(), maxiter=maxiter,
+                                  callback=lambda k, th: trace.append((time.perf_counter() - t0, th)))
+    torch.cuda.synchronize()
+    t_plain_total = time.perf_counter() - t0
+    e_star = float(theta)
+    t_plain = next((t for t, th in trace if abs(th - e_star) <= target * abs(e_star)), t_plain_total)
+    log(f'plain LOCG: E*={e_star:.12f} in {niter} iterations, {t_plain_total:.2f} s '
+        f'(within {target:g} after {t_plain:.2f} s)')
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res = ground_locg_mixed(plan.matvec, plan.matvec, x0.clone(), maxiter=maxiter, tol=mixed_tol, record=True)
+    torch.cuda.synchronize()
+    t_mixed = time.perf_counter() - t0
+    err = abs(float(res.theta) - e_star) / abs(e_star)
+    log(f'mixed LOCG: E={float(res.theta):.12f} rel err {err:.2e} in {t_mixed:.2f} s '
+        f'({res.niter} iterations, {res.matvecs_lo} low + {res.matvecs_hi} FP64 matvecs)')
+    lo_name = 'c64' if complex_ else 'f32'
+    hi_name = 'c128' if complex_ else 'f64'
+    return {
+        'e_star': e_star,
+        'target_rel_err': target,
+        'plain': {'dtype': hi_name, 'iterations': niter, 'matvecs': 4 * niter + 3,
+                  'time_to_target_s': round(t_plain, 4), 'total_s': round(t_plain_total, 4)},
+        'mixed': {'dtype': f'{lo_name} matvec, {hi_name} vectors/RR', 'iterations': res.niter,
+                  'matvecs_low': res.matvecs_lo, 'matvecs_fp64': res.matvecs_hi,
+                  'time_s': round(t_mixed, 4), 'rel_err_vs_plain': err, 'stop_rel_residual': mixed_tol},
+        'speedup': round(t_plain / t_mixed, 3),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -76,6 +129,10 @@ def main():
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--cpu-budget', type=float, default=20.0)
     ap.add_argument('--split-bits', type=int, default=-1)
+    ap.add_argument('--no-locg', action='store_true')
+    ap.add_argument('--locg-real', action='store_true', help='float64/float32 LOCG instead of complex')
+    ap.add_argument('--locg-maxiter', type=int, default=1000)
+    ap.add_argument('--mixed-tol', type=float, default=3e-6)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -166,6 +223,8 @@ def main():
             'traffic': traffic,
         },
     }
+    if not args.no_locg:
+        result['locg'] = locg_time_to_solution(plan, not args.locg_real, args.locg_maxiter, args.mixed_tol)
     if not args.no_cpu:
         log(f'cpu baseline (budget {args.cpu_budget} s) ...')
         result['cpu_baseline'] = cpu_baseline(w, alg, args.cpu_budget)

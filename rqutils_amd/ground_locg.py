@@ -92,7 +92,9 @@ def ground_locg(
     maxiter: int = 1000,
     tol: Optional[float] = None,
     vspace=None,
-    log_level: int = logging.WARNING
+    log_level: int = logging.WARNING,
+    *,
+    callback=None
 ):
     r"""Single-vector LOBPCG (``ground_locg.py:128-162``).
 
@@ -107,6 +109,8 @@ def ground_locg(
         tol: Convergence condition (default: machine epsilon of the vector dtype).
         vspace: ``(dimension, dtype[, device])`` of the vector space.
         log_level: Verbosity level.
+        callback: optional ``callback(niter, theta)`` after every main-loop
+            iteration (``theta`` a Python float; keyword-only extension).
 
     Returns:
         The smallest eigenvalue (0-dim tensor), its eigenvector and the number
@@ -115,11 +119,11 @@ def ground_locg(
     if not isinstance(orth, tuple):
         orth = (orth,)
     if callable(mat) and not isinstance(mat, (torch.Tensor, np.ndarray)):
-        return _ground_locg_callable(mat, xinit, args, orth, maxiter, tol, vspace, log_level)
-    return _ground_locg_matrix(mat, xinit, orth, maxiter, tol, log_level)
+        return _ground_locg_callable(mat, xinit, args, orth, maxiter, tol, vspace, log_level, callback)
+    return _ground_locg_matrix(mat, xinit, orth, maxiter, tol, log_level, callback)
 
 
-def _ground_locg_matrix(mat, xinit, orth, maxiter, tol, log_level):
+def _ground_locg_matrix(mat, xinit, orth, maxiter, tol, log_level, callback=None):
     """Dense-matrix path (``ground_locg.py:165-185``)."""
     if isinstance(xinit, torch.Tensor) and xinit.is_cuda:
         device = xinit.device
@@ -135,7 +139,7 @@ def _ground_locg_matrix(mat, xinit, orth, maxiter, tol, log_level):
     def matvec(x):
         return torch.mv(m, x)
 
-    return _ground_locg_callable(matvec, xinit, (), orth, maxiter, tol, vspace, log_level)
+    return _ground_locg_callable(matvec, xinit, (), orth, maxiter, tol, vspace, log_level, callback)
 
 
 def _is_index(xinit):
@@ -148,7 +152,7 @@ def _is_index(xinit):
     return False
 
 
-def _ground_locg_callable(matvec, xinit, args, orth, maxiter, tol, vspace, log_level):
+def _ground_locg_callable(matvec, xinit, args, orth, maxiter, tol, vspace, log_level, callback=None):
     """``ground_locg.py:197-423``."""
     if _is_index(xinit):
         ndim, dtype = vspace[0], torch_dtype(vspace[1])
@@ -251,6 +255,8 @@ def _ground_locg_callable(matvec, xinit, args, orth, maxiter, tol, vspace, log_l
         xcurr, ycurr, rcurr = xnext, ynext, rnext
         niter += 1
         converged = bool(conv_t)
+        if callback is not None:
+            callback(niter, float(theta))
         if log_level <= logging.DEBUG:
             LOG.debug('Residual %s, reltol %s, converged: %s', float(norm_rnext), float(reltol), converged)
 

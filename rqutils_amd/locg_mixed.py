@@ -1,0 +1,150 @@
+r"""Mixed-precision single-vector LOCG: low-precision matvec, FP64 everything else.
+
+No counterpart in the reference (``rqutils/ground_locg.py`` runs every step in
+the vector dtype and spends four matvecs per iteration, ``:256,346``).  Here:
+
+* the vectors ``x`` (Ritz vector), ``y`` (previous direction) and ``p``
+  (residual direction) are FP64 / complex128, and so are their images
+  ``Ax, Ay, Ap``, which are carried through every linear combination, so an
+  iteration needs exactly one new matvec (``A p``);
+* that matvec runs in FP32 / complex64 (``matvec_lo``) in the first phase;
+* ``[x, y, p]`` is orthonormalised in FP64 (classical Gram-Schmidt, two
+  passes), the 3x3 Rayleigh-Ritz matrix ``V^H A V`` is formed from the carried
+  images and solved by the same analytic eigen routine as the plain path;
+* once the FP64 residual of the low phase reaches ``switch_tol`` (relative to
+  ``|theta|``) or stops improving, ``Ax`` and ``Ay`` are recomputed with the
+  FP64 matvec (``matvec_hi``) and the iteration continues in FP64 until
+  ``tol``: the returned energy is a Rayleigh quotient of FP64 images.
+
+Per iteration: 1 matvec, about 25 FP64 vector passes, 1 host synchronisation
+every ``check_every`` iterations.
+"""
+import time
+
+import torch
+
+from rqutils_amd.plan import eigenpair_small
+
+
+def _dot(a, b):
+    return torch.vdot(a, b) if a.is_complex() else torch.dot(a, b)
+
+
+def _nrm(v):
+    return torch.linalg.vector_norm(v)
+
+
+class MixedLOCGResult:
+    def __init__(self, theta, x, niter, matvecs_lo, matvecs_hi, history):
+        self.theta = theta
+        self.x = x
+        self.niter = niter
+        self.matvecs_lo = matvecs_lo
+        self.matvecs_hi = matvecs_hi
+        self.history = history     # list of (seconds, theta, rel_residual, phase)
+
+    def __iter__(self):
+        return iter((self.theta, self.x, self.niter))
+
+
+def ground_locg_mixed(matvec_hi, matvec_lo, xinit, maxiter=1000, tol=1e-6, switch_tol=3e-5,
+                      check_every=1, stagnation=8, record=False):
+    """Ground eigenpair with a low-precision bulk matvec and FP64 refinement.
+
+    Args:
+        matvec_hi: ``v -> A v`` in the vector dtype (float64 / complex128).
+        matvec_lo: ``v -> A v`` in low precision (float32 / complex64), or None
+            to run the FP64 phase only.
+        xinit: start vector (float64 / complex128 tensor).
+        maxiter: total iteration limit.
+        tol: stop when ``|A x - theta x| <= tol |theta|`` with FP64 images.
+        switch_tol: leave the low-precision phase at this relative residual.
+        check_every: iterations between host-side convergence checks.
+        stagnation: leave the low phase after this many checks without a new
+            best residual.
+        record: keep ``(seconds, theta, rel_residual, phase)`` per check.
+
+    Returns:
+        ``MixedLOCGResult`` (iterable as ``theta, x, niter``).
+    """
+    hi = xinit.dtype
+    if hi not in (torch.float64, torch.complex128):
+        raise ValueError('xinit must be float64 or complex128')
+    lo = {torch.float64: torch.float32, torch.complex128: torch.complex64}[hi]
+    t_start = time.perf_counter()
+    counts = {'lo': 0, 'hi': 0}
+
+    def mv(v, phase):
+        counts[phase] += 1
+        if phase == 'lo':
+            return matvec_lo(v.to(lo)).to(hi)
+        return matvec_hi(v)
+
+    phase = 'lo' if matvec_lo is not None else 'hi'
+    x = xinit / _nrm(xinit)
+    ax = mv(x, phase)
+    y = ay = None
+    history = []
+    best = float('inf')
+    since_best = 0
+    theta = None
+    niter = 0
+    while True:
+        theta = _dot(x, ax).real
+        r = ax - theta.to(hi) * x
+        if niter % check_every == 0 or niter >= maxiter:
+            rel = float(_nrm(r) / theta.abs())
+            if record:
+                history.append((time.perf_counter() - t_start, float(theta), rel, phase))
+            if phase == 'lo':
+                if rel < best * 0.98:
+                    best, since_best = rel, 0
+                else:
+                    since_best += 1
+                if rel <= switch_tol or since_best >= stagnation or niter >= maxiter:
+                    phase = 'hi'
+                    ax = mv(x, 'hi')
+                    if y is not None:
+                        ay = mv(y, 'hi')
+                    continue
+            elif rel <= tol or niter >= maxiter:
+                break
+        # residual direction, orthonormal to x and y (two CGS passes)
+        p = r
+        for _ in range(2):
+            p = p - x * _dot(x, p)
+            if y is not None:
+                p = p - y * _dot(y, p)
+        pn = _nrm(p)
+        p = p / pn
+        ap = mv(p, phase)
+        # Rayleigh-Ritz on [x, y, p] from the carried images
+        basis = (x, p) if y is None else (x, y, p)
+        images = (ax, ap) if y is None else (ax, ay, ap)
+        n = len(basis)
+        k = torch.empty((n, n), dtype=torch.complex128, device=x.device)
+        for a in range(n):
+            for b in range(a, n):
+                val = 0.5 * (_dot(basis[a], images[b]) + _dot(images[a], basis[b]))
+                k[a, b] = val
+                k[b, a] = val.conj() if torch.is_tensor(val) and val.is_complex() else val
+        _, c = eigenpair_small(k)
+        if not x.is_complex():
+            c = c.real
+        c = c.to(hi)
+        xn = sum(ci * v for ci, v in zip(c, basis))
+        axn = sum(ci * v for ci, v in zip(c, images))
+        d = sum(ci * v for ci, v in zip(c[1:], basis[1:]))
+        ad = sum(ci * v for ci, v in zip(c[1:], images[1:]))
+        s = _nrm(xn)
+        x, ax = xn / s, axn / s
+        proj = _dot(x, d)
+        d = d - x * proj
+        ad = ad - ax * proj
+        dn = _nrm(d)
+        if float(dn) > 1e-14:
+            y, ay = d / dn, ad / dn
+        else:                      # x is already an eigenvector of the subspace
+            y = ay = None
+        niter += 1
+    return MixedLOCGResult(theta, x, niter, counts['lo'], counts['hi'], history)

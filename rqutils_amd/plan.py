@@ -190,12 +190,19 @@ def sort_unique_u64(keys: torch.Tensor):
 
 
 def eigenpair_small(mat: torch.Tensor):
-    """Lowest eigenpair of a 2x2 / 3x3 Hermitian device matrix -> (theta, kappa) device tensors."""
+    """Lowest eigenpair of a 2x2 / 3x3 Hermitian matrix -> (theta, kappa) tensors on its device.
+
+    A device matrix runs the one-thread HIP kernel on the current stream; a CPU
+    matrix runs the same routine on the calling thread.
+    """
     n = mat.shape[0]
     m = mat.to(torch.complex128).contiguous()
     out = torch.empty(2 + 2 * n, dtype=torch.float64, device=mat.device)
-    stream = _capi.current_stream(mat.device)
-    _capi.check(_capi.lib().rqx_eigenpair_small(m.data_ptr(), n, out.data_ptr(), stream))
+    if mat.is_cuda:
+        stream = _capi.current_stream(mat.device)
+        _capi.check(_capi.lib().rqx_eigenpair_small(m.data_ptr(), n, out.data_ptr(), stream))
+    else:
+        _capi.check(_capi.lib().rqx_eigenpair_small_host(m.data_ptr(), n, out.data_ptr()))
     theta = out[0]
     kappa = torch.view_as_complex(out[2:].reshape(n, 2))
     return theta, kappa
