@@ -47,8 +47,8 @@ class MixedLOCGResult:
         return iter((self.theta, self.x, self.niter))
 
 
-def ground_locg_mixed(matvec_hi, matvec_lo, xinit, maxiter=1000, tol=1e-6, switch_tol=3e-5,
-                      check_every=1, stagnation=8, record=False):
+def ground_locg_mixed(matvec_hi, matvec_lo, xinit, maxiter=1000, tol=1e-6, switch_tol=1e-5,
+                      check_every=1, stagnation=25, record=False):
     """Ground eigenpair with a low-precision bulk matvec and FP64 refinement.
 
     Args:
