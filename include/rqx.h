@@ -117,6 +117,34 @@ int rqx_matvec(const rqx_plan *plan, const void *x, void *y, int32_t dtype, rqx_
 int rqx_count(const rqx_plan *plan, int64_t *counts, rqx_stream_t stream);
 
 /*
+ * grad[k] = sum_i Re( conj(gy_i) * (P_k x)_i ) for every term k (plan term
+ * order), i.e. the gradient of Re<gy, H x> with respect to the unphased real
+ * coefficient alpha_k of term k, where (P_k x)_i = (-i)^{x.z_k}
+ * (-1)^{popcount(s_i & z_k)} x[rank(s_i ^ x_k)].  x, gy: device vectors of the
+ * given dtype; grad: device, n_terms doubles (overwritten).  No counterpart in
+ * the reference (JAX autodiff cannot differentiate get_diagonal's while_loop,
+ * rqutils/sqd.py:611-628, in reverse mode); it is the backward of
+ * rqx_matvec with respect to the coefficients.
+ */
+int rqx_grad_coef(const rqx_plan *plan, const void *x, const void *gy, double *grad, int32_t dtype,
+                  rqx_stream_t stream);
+
+/*
+ * out[k] = Re <psi| P_k |psi> (unnormalised) for every term: the Hellmann-Feynman
+ * derivative dE/dalpha_k of the ground energy at a normalised ground vector psi.
+ * Same layout as rqx_grad_coef (it is rqx_grad_coef(psi, psi)).
+ */
+int rqx_expect_terms(const rqx_plan *plan, const void *psi, double *out, int32_t dtype, rqx_stream_t stream);
+
+/*
+ * Replace the phased coefficients of every term (device pointer, n_terms complex
+ * interleaved doubles, plan term order) and rebuild the x == 0 diagonal, so a
+ * plan can follow a changing coefficient vector (autograd, parameter scans).
+ * is_real must state whether every new coefficient is real.
+ */
+int rqx_plan_update_coef(rqx_plan *plan, const double *coef, int32_t is_real, rqx_stream_t stream);
+
+/*
  * Lowest eigenpair of a small Hermitian matrix (n = 2 or 3), row-major,
  * complex interleaved, device pointers.  out = [theta, 0, k0re, k0im, ...].
  * Replaces eigenpair_2x2 / eigenpair_3x3 (rqutils/ground_locg.py:467-514),

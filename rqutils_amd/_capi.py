@@ -26,7 +26,8 @@ RQX_C64 = 3
 EXPORTS = (
     'rqx_version', 'rqx_last_error', 'rqx_plan_create', 'rqx_plan_destroy', 'rqx_plan_get_info',
     'rqx_lookup', 'rqx_diagonal', 'rqx_matvec', 'rqx_count', 'rqx_eigenpair_small',
-    'rqx_eigenpair_small_host', 'rqx_sort_unique_u64',
+    'rqx_eigenpair_small_host', 'rqx_sort_unique_u64', 'rqx_grad_coef', 'rqx_expect_terms',
+    'rqx_plan_update_coef',
 )
 
 
@@ -83,6 +84,9 @@ _SIGS = {
     'rqx_count': (_i32, [_vp, _vp, _vp]),
     'rqx_eigenpair_small': (_i32, [_vp, _i32, _vp, _vp]),
     'rqx_eigenpair_small_host': (_i32, [_vp, _i32, _vp]),
+    'rqx_grad_coef': (_i32, [_vp, _vp, _vp, _vp, _i32, _vp]),
+    'rqx_expect_terms': (_i32, [_vp, _vp, _vp, _i32, _vp]),
+    'rqx_plan_update_coef': (_i32, [_vp, _vp, _i32, _vp]),
     'rqx_sort_unique_u64': (_i32, [_vp, _i64, _vp, ctypes.POINTER(_i64), _vp]),
 }
 

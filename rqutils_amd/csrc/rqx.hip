@@ -50,7 +50,7 @@ int fail(int code, const std::string &msg) {
             return fail(RQX_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
-enum { MODE_MATVEC = 0, MODE_LOOKUP = 1, MODE_COUNT = 2 };
+enum { MODE_MATVEC = 0, MODE_LOOKUP = 1, MODE_COUNT = 2, MODE_GRAD = 3 };
 
 struct KParams {
     const uint64_t *keys;
@@ -69,6 +69,9 @@ struct KParams {
     int32_t *idx;                  // lookup output
     double2 *vals;                 // lookup output (optional): matrix element C_j(s_i)
     unsigned long long *counts;    // count output
+    const uint64_t *g_x;           // full X mask per group
+    const void *gy;                // gradient mode: upstream gradient (same dtype as x)
+    double *grad;                  // gradient mode: per-term output (accumulated)
     int64_t n_total;
     uint32_t lomask;
     int32_t n_classes;
@@ -141,6 +144,49 @@ __device__ inline typename C2<R>::T group_coef(const KParams &p, uint32_t g, uin
     return acc;
 }
 
+// sum over the 64 lanes of a wave (every lane receives the total)
+template <typename R>
+__device__ inline R wave_sum(R v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// real and imaginary parts of conj(g) * v for real or complex entries
+__device__ inline void conj_mul(double g, double v, double &re, double &im) { re = g * v; im = 0.0; }
+__device__ inline void conj_mul(float g, float v, float &re, float &im) { re = g * v; im = 0.f; }
+__device__ inline void conj_mul(double2 g, double2 v, double &re, double &im) {
+    re = g.x * v.x + g.y * v.y;
+    im = g.x * v.y - g.y * v.x;
+}
+__device__ inline void conj_mul(float2 g, float2 v, float &re, float &im) {
+    re = g.x * v.x + g.y * v.y;
+    im = g.x * v.y - g.y * v.x;
+}
+
+// Re((-i)^ph * (re + i im))
+template <typename R>
+__device__ inline R re_phase(int ph, R re, R im) {
+    return (ph == 0) ? re : (ph == 1) ? im : (ph == 2) ? -re : -im;
+}
+
+// Gradient of Re<gy, H x> with respect to the unphased coefficient of every
+// term of group g: sum over rows of Re(conj(gy_i) (-i)^{x.z} (-1)^{s_i.z} x[src]).
+// Called by every lane of the wave; lanes without a hit contribute 0.
+template <typename R>
+__device__ inline void grad_group(const KParams &p, uint32_t g, uint64_t s, bool hit, R wre, R wim) {
+    if (__ballot(hit) == 0) return;
+    const uint64_t xg = p.g_x[g];
+    const uint32_t k1 = p.g_toff[g + 1];
+    for (uint32_t k = p.g_toff[g]; k < k1; ++k) {
+        const uint64_t zk = p.z[k];
+        const int ph = __popcll(xg & zk) & 3;
+        R val = hit ? re_phase(ph, wre, wim) : R(0);
+        if (__popcll(s & zk) & 1) val = -val;
+        val = wave_sum(val);
+        if ((threadIdx.x & 63) == 0) atomicAdd(&p.grad[k], static_cast<double>(val));
+    }
+}
+
 template <int MODE, typename R, bool CPLX>
 __global__ __launch_bounds__(TILE) void k_tiles(const KParams p, const void *__restrict__ xin,
                                                 void *__restrict__ yout) {
@@ -161,7 +207,16 @@ __global__ __launch_bounds__(TILE) void k_tiles(const KParams p, const void *__r
 
     V acc = vzero(static_cast<V *>(nullptr));
     unsigned long long n_hit = 0, n_probe = 0;
-    if (p.has_diag && active) {
+    V gv = vzero(static_cast<V *>(nullptr));
+    if constexpr (MODE == MODE_GRAD) {
+        if (active) gv = static_cast<const V *>(p.gy)[i];
+        if (p.has_diag) {
+            R wre = R(0), wim = R(0);
+            if (active) conj_mul(gv, x[i], wre, wim);
+            grad_group<R>(p, 0u, s, active, wre, wim);
+        }
+    }
+    if (p.has_diag && active && MODE != MODE_GRAD) {
         if constexpr (MODE == MODE_MATVEC) {
             const CC d = static_cast<const CC *>(p.diag)[i];
             fma_acc(acc, d, x[i]);
@@ -201,6 +256,10 @@ __global__ __launch_bounds__(TILE) void k_tiles(const KParams p, const void *__r
                         p.idx[o] = static_cast<int32_t>(ta + base);
                         if (p.vals) p.vals[o] = group_coef<double>(p, g, s);
                     }
+                } else if constexpr (MODE == MODE_GRAD) {
+                    R wre = R(0), wim = R(0);
+                    if (hit) conj_mul(gv, x[ta + base], wre, wim);
+                    grad_group<R>(p, g, s, hit, wre, wim);
                 } else {
                     n_hit += hit;
                     n_probe += active;
@@ -260,6 +319,11 @@ template <typename V>
 __global__ void k_fill(V *__restrict__ y, int64_t begin, int64_t end, V value) {
     const int64_t i = begin + static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i < end) y[i] = value;
+}
+
+__global__ void k_narrow(const double2 *__restrict__ in, float2 *__restrict__ out, int64_t n) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = make_float2(static_cast<float>(in[i].x), static_cast<float>(in[i].y));
 }
 
 // diag[i] = C_0(s_i) for group 0 (x == 0), in double and float copies
@@ -562,6 +626,7 @@ struct rqx_plan {
     int is_real = 1;
     int max_seg = 0;
     int max_group_terms = 0;
+    uint32_t diag_terms = 0;   // terms of the x == 0 group (group 0)
     int64_t n_states = 0, n_total = 0, n_terms = 0;
     int32_t n_groups = 0, n_classes = 0, n_segs = 0, n_tiles = 0;
     // device arrays (one allocation, carved)
@@ -574,6 +639,7 @@ struct rqx_plan {
     uint32_t *cls_goff = nullptr;
     uint32_t *g_xlo = nullptr;
     uint32_t *g_toff = nullptr;
+    uint64_t *g_x = nullptr;
     uint64_t *z = nullptr;
     double2 *coef64 = nullptr;
     float2 *coef32 = nullptr;
@@ -598,6 +664,7 @@ KParams make_params(const rqx_plan *p, bool fp64) {
     k.cls_goff = p->cls_goff;
     k.g_xlo = p->g_xlo;
     k.g_toff = p->g_toff;
+    k.g_x = p->g_x;
     k.z = p->z;
     k.coef = fp64 ? static_cast<const void *>(p->coef64) : static_cast<const void *>(p->coef32);
     k.diag = fp64 ? static_cast<const void *>(p->diag64) : static_cast<const void *>(p->diag32);
@@ -679,6 +746,7 @@ int rqx_plan_create(rqx_plan **out, int32_t n_qubits, const uint64_t *xmask, con
         if (coef[2 * k + 1] != 0.0) p->is_real = 0;
     for (int32_t j = 0; j < n_groups; ++j)
         p->max_group_terms = std::max<int>(p->max_group_terms, static_cast<int>(grp_off[j + 1] - grp_off[j]));
+    p->diag_terms = p->has_diag ? static_cast<uint32_t>(grp_off[1]) : 0u;
 
     // ---- choose the split L
     std::vector<int8_t> hd(static_cast<size_t>(n_states), 64);
@@ -755,7 +823,7 @@ int rqx_plan_create(rqx_plan **out, int32_t n_qubits, const uint64_t *xmask, con
                  o_gtoff = carve(g_toff.size() * 4), o_z = carve(static_cast<size_t>(n_terms) * 8),
                  o_c64 = carve(static_cast<size_t>(n_terms) * 16), o_c32 = carve(static_cast<size_t>(n_terms) * 8),
                  o_d64 = carve(n * 16), o_d32 = carve(n * 8), o_trow = carve(tile_row0.size() * 4),
-                 o_tseg = carve(tile_seg.size() * 4);
+                 o_tseg = carve(tile_seg.size() * 4), o_gx = carve(static_cast<size_t>(n_groups) * 8);
     hipError_t e = hipMalloc(&p->block, off);
     if (e != hipSuccess) {
         delete p;
@@ -770,6 +838,7 @@ int rqx_plan_create(rqx_plan **out, int32_t n_qubits, const uint64_t *xmask, con
     p->cls_goff = reinterpret_cast<uint32_t *>(b + o_clsgoff);
     p->g_xlo = reinterpret_cast<uint32_t *>(b + o_gxlo);
     p->g_toff = reinterpret_cast<uint32_t *>(b + o_gtoff);
+    p->g_x = reinterpret_cast<uint64_t *>(b + o_gx);
     p->z = reinterpret_cast<uint64_t *>(b + o_z);
     p->coef64 = reinterpret_cast<double2 *>(b + o_c64);
     p->coef32 = reinterpret_cast<float2 *>(b + o_c32);
@@ -797,7 +866,8 @@ int rqx_plan_create(rqx_plan **out, int32_t n_qubits, const uint64_t *xmask, con
         up(p->coef32, c32.data(), static_cast<size_t>(n_terms) * 8),
         up(p->tile_row0, tile_row0.data(), tile_row0.size() * 4),
         up(p->tile_seg, tile_seg.data(), tile_seg.size() * 4),
-        hipSuccess, hipSuccess, hipSuccess};
+        up(p->g_x, xmask, static_cast<size_t>(n_groups) * 8),
+        hipSuccess, hipSuccess};
     for (hipError_t ei : errs)
         if (ei != hipSuccess) {
             (void)hipFree(p->block);
@@ -956,6 +1026,64 @@ int rqx_count(const rqx_plan *p, int64_t *counts, rqx_stream_t stream) {
                            nullptr);
         HIP_TRY(hipGetLastError());
     }
+    return RQX_OK;
+}
+
+int rqx_grad_coef(const rqx_plan *p, const void *x, const void *gy, double *grad, int32_t dtype,
+                  rqx_stream_t stream) {
+    g_err.clear();
+    if (int rc = check_plan(p)) return rc;
+    if (p->n_terms == 0) return RQX_OK;
+    if (!grad) return fail(RQX_ERR_ARG, "grad is NULL");
+    if (p->n_states > 0 && (!x || !gy)) return fail(RQX_ERR_ARG, "x or gy is NULL");
+    if (dtype < RQX_F64 || dtype > RQX_C64) return fail(RQX_ERR_ARG, "unknown dtype");
+    const bool cplx = (dtype == RQX_C128 || dtype == RQX_C64);
+    if (!cplx && !p->is_real)
+        return fail(RQX_ERR_DTYPE, "real vector dtype requested but the Hamiltonian has complex coefficients");
+    const bool fp64 = (dtype == RQX_F64 || dtype == RQX_C128);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const unsigned fb = static_cast<unsigned>((p->n_terms + 255) / 256);
+    hipLaunchKernelGGL(k_fill<double>, dim3(fb), dim3(256), 0, st, grad, int64_t(0), p->n_terms, 0.0);
+    HIP_TRY(hipGetLastError());
+    if (p->n_tiles > 0) {
+        KParams k = make_params(p, fp64);
+        k.gy = gy;
+        k.grad = grad;
+        const dim3 grid(p->n_tiles), block(TILE);
+        const size_t lds = seg_lds_bytes(p);
+        switch (dtype) {
+            case RQX_F64: hipLaunchKernelGGL((k_tiles<MODE_GRAD, double, false>), grid, block, lds, st, k, x, nullptr); break;
+            case RQX_C128: hipLaunchKernelGGL((k_tiles<MODE_GRAD, double, true>), grid, block, lds, st, k, x, nullptr); break;
+            case RQX_F32: hipLaunchKernelGGL((k_tiles<MODE_GRAD, float, false>), grid, block, lds, st, k, x, nullptr); break;
+            default: hipLaunchKernelGGL((k_tiles<MODE_GRAD, float, true>), grid, block, lds, st, k, x, nullptr); break;
+        }
+        HIP_TRY(hipGetLastError());
+    }
+    return RQX_OK;
+}
+
+int rqx_expect_terms(const rqx_plan *p, const void *psi, double *out, int32_t dtype, rqx_stream_t stream) {
+    return rqx_grad_coef(p, psi, psi, out, dtype, stream);
+}
+
+int rqx_plan_update_coef(rqx_plan *p, const double *coef, int32_t is_real, rqx_stream_t stream) {
+    g_err.clear();
+    if (int rc = check_plan(p)) return rc;
+    if (p->n_terms == 0) return RQX_OK;
+    if (!coef) return fail(RQX_ERR_ARG, "coef is NULL");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    HIP_TRY(hipMemcpyAsync(p->coef64, coef, static_cast<size_t>(p->n_terms) * 16, hipMemcpyDeviceToDevice, st));
+    const unsigned tb = static_cast<unsigned>((p->n_terms + 255) / 256);
+    hipLaunchKernelGGL(k_narrow, dim3(tb), dim3(256), 0, st, p->coef64, p->coef32, p->n_terms);
+    HIP_TRY(hipGetLastError());
+    if (p->n_states > 0) {
+        const unsigned blocks = static_cast<unsigned>((p->n_states + 255) / 256);
+        const uint32_t k1 = p->has_diag ? p->diag_terms : 0u;
+        hipLaunchKernelGGL(k_diag, dim3(blocks), dim3(256), 0, st, p->keys, p->n_states, p->z, p->coef64, 0u, k1,
+                           p->diag64, p->diag32);
+        HIP_TRY(hipGetLastError());
+    }
+    p->is_real = is_real ? 1 : 0;
     return RQX_OK;
 }
 

@@ -76,12 +76,27 @@ class TermTables:
         return self.coef * (1j ** ph)
 
     @classmethod
-    def from_masks(cls, xmask, zmask, alpha, num_qubits, force_real=False, order=None):
+    def from_masks(cls, xmask, zmask, alpha, num_qubits, force_real=False, order=None, return_index=False):
         """Group raw (x, z, alpha) terms; duplicates are summed, zero sums dropped.
 
         Groups follow ascending x; within a group terms follow ascending z
         (the order ``np.unique`` gives Pauli strings with I < X < Y < Z).
+        With ``return_index`` also returns, for every input term, its position
+        in the tables (-1 if it was dropped); duplicates share a position.
         """
+        if return_index:
+            n_in = np.asarray(xmask).reshape(-1).shape[0]
+            tagged = cls.from_masks(xmask, zmask, alpha, num_qubits, force_real=force_real, order=order)
+            # locate every input (x, z) among the table terms
+            xin = np.asarray(xmask, dtype=np.uint64).reshape(-1)
+            zin = np.asarray(zmask, dtype=np.uint64).reshape(-1)
+            tx, tz = tagged.term_xmask(), tagged.zmask
+            keys_t = np.stack([tx, tz], axis=1).view([('x', np.uint64), ('z', np.uint64)]).reshape(-1)
+            keys_i = np.stack([xin, zin], axis=1).view([('x', np.uint64), ('z', np.uint64)]).reshape(-1)
+            pos = np.searchsorted(keys_t, keys_i) if keys_t.shape[0] else np.zeros(n_in, np.int64)
+            pos_c = np.minimum(pos, max(keys_t.shape[0] - 1, 0))
+            found = (keys_t.shape[0] > 0) & (keys_t[pos_c] == keys_i) if keys_t.shape[0] else np.zeros(n_in, bool)
+            return tagged, np.where(found, pos_c, -1).astype(np.int64)
         xmask = np.asarray(xmask, dtype=np.uint64).reshape(-1)
         zmask = np.asarray(zmask, dtype=np.uint64).reshape(-1)
         alpha = np.asarray(alpha).reshape(-1)

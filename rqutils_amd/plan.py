@@ -146,6 +146,55 @@ class Plan:
         _capi.check(_capi.lib().rqx_diagonal(self._handle, out.data_ptr() if out.numel() else None, stream))
         return out
 
+    def grad_coef(self, x, gy, out=None):
+        """``g_k = Re <gy, P_k x>``: gradient of ``Re <gy, H x>`` w.r.t. the unphased coefficients.
+
+        Terms follow ``self.tables`` order; returns a float64 device tensor.
+        """
+        x = self._vec(x)
+        gy = self._vec(gy)
+        if gy.dtype != x.dtype:
+            raise ValueError('x and gy must have the same dtype')
+        T = self.info['n_terms']
+        if out is None:
+            out = torch.empty(T, dtype=torch.float64, device=self.device)
+        stream = _capi.current_stream(self.device)
+        _capi.check(_capi.lib().rqx_grad_coef(self._handle, x.data_ptr(), gy.data_ptr(),
+                                              out.data_ptr() if T else None, _DTYPE_CODES[x.dtype], stream))
+        return out
+
+    def expect_terms(self, psi):
+        """``Re <psi|P_k|psi> / <psi|psi>`` per term: Hellmann-Feynman ``dE/dalpha_k``."""
+        psi = self._vec(psi)
+        T = self.info['n_terms']
+        out = torch.empty(T, dtype=torch.float64, device=self.device)
+        stream = _capi.current_stream(self.device)
+        _capi.check(_capi.lib().rqx_expect_terms(self._handle, psi.data_ptr(), out.data_ptr() if T else None,
+                                                 _DTYPE_CODES[psi.dtype], stream))
+        nrm2 = torch.linalg.vector_norm(psi).to(torch.float64) ** 2
+        return out / nrm2
+
+    def term_phases(self):
+        """``(-i)^{popcount(x_k & z_k)}`` per term (complex128 device tensor)."""
+        if getattr(self, '_phases', None) is None:
+            xm = self.tables.term_xmask()
+            ph = np.bitwise_count(xm & self.tables.zmask).astype(np.int64) & 3
+            table = np.array([1., -1j, -1., 1j])
+            self._phases = torch.from_numpy(table[ph]).to(self.device)
+        return self._phases
+
+    def set_alphas(self, alpha):
+        """Use ``alpha`` (real, one per term, tables order) as the unphased coefficients."""
+        alpha = torch.as_tensor(alpha, device=self.device).to(torch.float64).reshape(-1)
+        if alpha.shape[0] != self.info['n_terms']:
+            raise ValueError('alpha must have one entry per term')
+        coef = (alpha.to(torch.complex128) * self.term_phases()).contiguous()
+        is_real = bool(torch.all(self.term_phases().imag == 0).item()) if coef.numel() else True
+        stream = _capi.current_stream(self.device)
+        _capi.check(_capi.lib().rqx_plan_update_coef(self._handle, torch.view_as_real(coef).data_ptr()
+                                                     if coef.numel() else None, int(is_real), stream))
+        self.is_real = is_real
+
     def count(self):
         """Structural nonzeros, segment-level probes and class lookups of one matvec."""
         out = torch.zeros(3, dtype=torch.int64, device=self.device)
