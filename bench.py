@@ -66,7 +66,7 @@ def cpu_baseline(w, alg_bytes, budget_s):
     }
 
 
-def locg_time_to_solution(plan, complex_, maxiter, mixed_tol, target=1e-8):
+def locg_time_to_solution(plan, complex_, maxiter, mixed_tol, energy_tol, target=1e-8):
     """Time to ``target`` relative ground-energy error: plain FP64 LOCG vs mixed precision.
 
     Both start from the reference's one-hot vector at the argmin of the x = 0
@@ -99,7 +99,8 @@ def locg_time_to_solution(plan, complex_, maxiter, mixed_tol, target=1e-8):
         f'(within {target:g} after {t_plain:.2f} s)')
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    res = ground_locg_mixed(plan.matvec, plan.matvec, x0.clone(), maxiter=maxiter, tol=mixed_tol, record=True)
+    res = ground_locg_mixed(plan.matvec, plan.matvec, x0.clone(), maxiter=maxiter, tol=mixed_tol,
+                            energy_tol=energy_tol, record=True)
     torch.cuda.synchronize()
     t_mixed = time.perf_counter() - t0
     err = abs(float(res.theta) - e_star) / abs(e_star)
@@ -114,7 +115,8 @@ def locg_time_to_solution(plan, complex_, maxiter, mixed_tol, target=1e-8):
                   'time_to_target_s': round(t_plain, 4), 'total_s': round(t_plain_total, 4)},
         'mixed': {'dtype': f'{lo_name} matvec, {hi_name} vectors/RR', 'iterations': res.niter,
                   'matvecs_low': res.matvecs_lo, 'matvecs_fp64': res.matvecs_hi,
-                  'time_s': round(t_mixed, 4), 'rel_err_vs_plain': err, 'stop_rel_residual': mixed_tol},
+                  'time_s': round(t_mixed, 4), 'rel_err_vs_plain': err, 'stop_rel_residual': mixed_tol,
+                  'stop_energy_tail': energy_tol},
         'speedup': round(t_plain / t_mixed, 3),
     }
 
@@ -132,7 +134,8 @@ def main():
     ap.add_argument('--no-locg', action='store_true')
     ap.add_argument('--locg-real', action='store_true', help='float64/float32 LOCG instead of complex')
     ap.add_argument('--locg-maxiter', type=int, default=1000)
-    ap.add_argument('--mixed-tol', type=float, default=3e-6)
+    ap.add_argument('--mixed-tol', type=float, default=1e-6)
+    ap.add_argument('--energy-tol', type=float, default=3e-10)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -224,7 +227,8 @@ def main():
         },
     }
     if not args.no_locg:
-        result['locg'] = locg_time_to_solution(plan, not args.locg_real, args.locg_maxiter, args.mixed_tol)
+        result['locg'] = locg_time_to_solution(plan, not args.locg_real, args.locg_maxiter, args.mixed_tol,
+                                               args.energy_tol)
     if not args.no_cpu:
         log(f'cpu baseline (budget {args.cpu_budget} s) ...')
         result['cpu_baseline'] = cpu_baseline(w, alg, args.cpu_budget)

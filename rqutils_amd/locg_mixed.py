@@ -47,8 +47,9 @@ class MixedLOCGResult:
         return iter((self.theta, self.x, self.niter))
 
 
-def ground_locg_mixed(matvec_hi, matvec_lo, xinit, maxiter=1000, tol=1e-6, switch_tol=1e-5,
-                      check_every=1, stagnation=25, record=False):
+def ground_locg_mixed(matvec_hi, matvec_lo, xinit, maxiter=1000, tol=1e-6, switch_tol=1e-6,
+                      check_every=1, stagnation=100, record=False, energy_tol=None, energy_checks=3,
+                      switch_energy=1e-9):
     """Ground eigenpair with a low-precision bulk matvec and FP64 refinement.
 
     Args:
@@ -63,6 +64,10 @@ def ground_locg_mixed(matvec_hi, matvec_lo, xinit, maxiter=1000, tol=1e-6, switc
         stagnation: leave the low phase after this many checks without a new
             best residual.
         record: keep ``(seconds, theta, rel_residual, phase)`` per check.
+        energy_tol: if set, also stop in the FP64 phase once the remaining
+            decrease of theta, estimated from the last three decreases as a
+            geometric tail (Aitken), and the last decrease are both below
+            ``energy_tol * |theta|`` for ``energy_checks`` consecutive checks.
 
     Returns:
         ``MixedLOCGResult`` (iterable as ``theta, x, niter``).
@@ -89,6 +94,9 @@ def ground_locg_mixed(matvec_hi, matvec_lo, xinit, maxiter=1000, tol=1e-6, switc
     since_best = 0
     theta = None
     niter = 0
+    hi_thetas = []
+    calm = 0
+    force_switch = False
     while True:
         theta = _dot(x, ax).real
         r = ax - theta.to(hi) * x
@@ -96,12 +104,29 @@ def ground_locg_mixed(matvec_hi, matvec_lo, xinit, maxiter=1000, tol=1e-6, switc
             rel = float(_nrm(r) / theta.abs())
             if record:
                 history.append((time.perf_counter() - t_start, float(theta), rel, phase))
+            if energy_tol is not None:
+                th = float(theta)
+                hi_thetas.append(th)
+                if len(hi_thetas) >= 3:
+                    d0 = hi_thetas[-3] - hi_thetas[-2]
+                    d1 = hi_thetas[-2] - hi_thetas[-1]
+                    # the low phase's theta carries FP32 image errors: compare with a
+                    # looser limit there and hand over to the FP64 phase when calm
+                    lim = (energy_tol if phase == 'hi' else max(energy_tol, 30 * switch_energy)) * abs(th)
+                    tail = d1 * d1 / (d0 - d1) if d0 > d1 >= 0 else float('inf')
+                    calm = calm + 1 if (0 <= d1 <= lim and tail <= lim) else 0
+                    if calm >= energy_checks:
+                        if phase == 'hi':
+                            break
+                        force_switch = True
+                        hi_thetas.clear()
+                        calm = 0
             if phase == 'lo':
                 if rel < best * 0.98:
                     best, since_best = rel, 0
                 else:
                     since_best += 1
-                if rel <= switch_tol or since_best >= stagnation or niter >= maxiter:
+                if rel <= switch_tol or since_best >= stagnation or niter >= maxiter or force_switch:
                     phase = 'hi'
                     ax = mv(x, 'hi')
                     if y is not None:
