@@ -56,7 +56,30 @@ typedef struct {
     int32_t has_diagonal;   /* group 0 has xmask == 0 */
     int32_t is_real;        /* every coefficient is real */
     int32_t max_group_terms;
+    int32_t n_problems;     /* 1, or the number of problems of a batched plan */
+    int32_t persistent;     /* matvec launches persistent workgroups on a tile queue */
+    int32_t blocks_per_cu;  /* persistent grid: workgroups per CU */
+    int32_t n_cu;           /* compute units of the device */
 } rqx_plan_info;
+
+/* One problem of a (batched) plan: its rows, terms and groups inside the plan. */
+typedef struct {
+    int64_t row_offset;     /* first row of the problem in the plan's vectors */
+    int64_t n_states;
+    int64_t term_offset;    /* first term in the plan's term order (rqx_grad_coef) */
+    int64_t n_terms;
+    int32_t group_offset;
+    int32_t n_groups;
+    int32_t n_qubits;
+    int32_t split_bits;
+    int32_t n_segments;
+    int32_t n_classes;
+    int32_t n_tiles;
+    int32_t max_segment;
+    int32_t has_diagonal;
+    int32_t is_real;
+    double cost;            /* cost-model estimate of one matvec (arbitrary units) */
+} rqx_problem_info;
 
 /* Library version (major*10000 + minor*100 + patch). */
 int rqx_version(void);
@@ -79,9 +102,35 @@ int rqx_plan_create(rqx_plan **out, int32_t n_qubits,
                     const uint64_t *sorted_states, int64_t n_states, int64_t n_total,
                     int32_t split_bits, rqx_stream_t stream);
 
+/*
+ * Batched plan over n_problems independent (Hamiltonian, subspace) pairs, each
+ * given as in rqx_plan_create (arrays of HOST pointers, one entry per problem;
+ * no filler rows: n_total = sum of n_states).  Every kernel of the plan then
+ * covers all problems in one launch: vectors are the concatenation of the
+ * problems' rows, term outputs the concatenation of their terms.  The matvec
+ * runs persistent workgroups that take tiles from a queue, largest first.
+ * No counterpart in the reference (one Hamiltonian per run_sqd call,
+ * rqutils/sqd.py:317-323).
+ */
+int rqx_plan_create_batch(rqx_plan **out, int32_t n_problems, const int32_t *n_qubits,
+                          const uint64_t *const *xmask, const int64_t *const *grp_off, const int32_t *n_groups,
+                          const uint64_t *const *zmask, const double *const *coef, const int64_t *n_terms,
+                          const uint64_t *const *sorted_states, const int64_t *n_states, int32_t split_bits,
+                          rqx_stream_t stream);
+
 int rqx_plan_destroy(rqx_plan *plan);
 
 int rqx_plan_get_info(const rqx_plan *plan, rqx_plan_info *out);
+
+/* Offsets and index statistics of problem k (0 <= k < n_problems). */
+int rqx_plan_get_problem(const rqx_plan *plan, int32_t k, rqx_problem_info *out);
+
+/*
+ * Launch shape of the tile kernels: persistent (work queue) or one workgroup
+ * per tile, and the persistent grid in workgroups per CU (0 = occupancy limit).
+ * Used by the autotuner.
+ */
+int rqx_plan_set_launch(rqx_plan *plan, int32_t persistent, int32_t blocks_per_cu);
 
 /*
  * idx[j * n_total + i] = rank of s_i ^ xmask_j among the real states, or -1
@@ -154,6 +203,28 @@ int rqx_eigenpair_small(const double *mat, int32_t n, double *out, rqx_stream_t 
 
 /* The same routine on HOST pointers (runs on the calling CPU thread). */
 int rqx_eigenpair_small_host(const double *mat, int32_t n, double *out);
+
+/* count independent eigenproblems: mats count x n x n, out count x (2 + 2n) (device). */
+int rqx_eigenpair_batch(const double *mats, int32_t count, int32_t n, double *out, rqx_stream_t stream);
+
+/*
+ * Per-problem inner products over a plan's vectors:
+ * out[2 * (k * n_pairs + q) + {0, 1}] = sum over rows of problem k of conj(a_q) * b_q.
+ * a, b: HOST arrays of n_pairs (<= 9) DEVICE vector pointers; out: device doubles.
+ * The batched counterpart of the jnp.sum(conj(u) * v) reductions of
+ * rqutils/ground_locg.py:259-262,445.
+ */
+int rqx_batch_dots(const rqx_plan *plan, int32_t n_pairs, const void *const *a, const void *const *b, double *out,
+                   int32_t dtype, rqx_stream_t stream);
+
+/*
+ * out[i] = sum_q coef[k * n_in + q] * v_q[i] for the rows i of problem k
+ * (coef: device, complex interleaved doubles; real vectors use the real part).
+ * v: HOST array of n_in (<= 4) DEVICE vector pointers; out may be one of them.
+ * The batched counterpart of the vector updates of rqutils/ground_locg.py:336-341.
+ */
+int rqx_batch_combine(const rqx_plan *plan, int32_t n_in, const void *const *v, const double *coef, void *out,
+                      int32_t dtype, rqx_stream_t stream);
 
 /*
  * Sort and deduplicate n uint64 keys on the device: out[0..*n_unique) is the

@@ -27,7 +27,8 @@ EXPORTS = (
     'rqx_version', 'rqx_last_error', 'rqx_plan_create', 'rqx_plan_destroy', 'rqx_plan_get_info',
     'rqx_lookup', 'rqx_diagonal', 'rqx_matvec', 'rqx_count', 'rqx_eigenpair_small',
     'rqx_eigenpair_small_host', 'rqx_sort_unique_u64', 'rqx_grad_coef', 'rqx_expect_terms',
-    'rqx_plan_update_coef',
+    'rqx_plan_update_coef', 'rqx_plan_create_batch', 'rqx_plan_get_problem', 'rqx_plan_set_launch',
+    'rqx_eigenpair_batch', 'rqx_batch_dots', 'rqx_batch_combine',
 )
 
 
@@ -58,6 +59,33 @@ class PlanInfo(ctypes.Structure):
         ('has_diagonal', ctypes.c_int32),
         ('is_real', ctypes.c_int32),
         ('max_group_terms', ctypes.c_int32),
+        ('n_problems', ctypes.c_int32),
+        ('persistent', ctypes.c_int32),
+        ('blocks_per_cu', ctypes.c_int32),
+        ('n_cu', ctypes.c_int32),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class ProblemInfo(ctypes.Structure):
+    _fields_ = [
+        ('row_offset', ctypes.c_int64),
+        ('n_states', ctypes.c_int64),
+        ('term_offset', ctypes.c_int64),
+        ('n_terms', ctypes.c_int64),
+        ('group_offset', ctypes.c_int32),
+        ('n_groups', ctypes.c_int32),
+        ('n_qubits', ctypes.c_int32),
+        ('split_bits', ctypes.c_int32),
+        ('n_segments', ctypes.c_int32),
+        ('n_classes', ctypes.c_int32),
+        ('n_tiles', ctypes.c_int32),
+        ('max_segment', ctypes.c_int32),
+        ('has_diagonal', ctypes.c_int32),
+        ('is_real', ctypes.c_int32),
+        ('cost', ctypes.c_double),
     ]
 
     def as_dict(self):
@@ -87,6 +115,13 @@ _SIGS = {
     'rqx_grad_coef': (_i32, [_vp, _vp, _vp, _vp, _i32, _vp]),
     'rqx_expect_terms': (_i32, [_vp, _vp, _vp, _i32, _vp]),
     'rqx_plan_update_coef': (_i32, [_vp, _vp, _i32, _vp]),
+    'rqx_plan_create_batch': (_i32, [ctypes.POINTER(_vp), _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                                     _i32, _vp]),
+    'rqx_plan_get_problem': (_i32, [_vp, _i32, ctypes.POINTER(ProblemInfo)]),
+    'rqx_plan_set_launch': (_i32, [_vp, _i32, _i32]),
+    'rqx_eigenpair_batch': (_i32, [_vp, _i32, _i32, _vp, _vp]),
+    'rqx_batch_dots': (_i32, [_vp, _i32, _vp, _vp, _vp, _i32, _vp]),
+    'rqx_batch_combine': (_i32, [_vp, _i32, _vp, _vp, _vp, _i32, _vp]),
     'rqx_sort_unique_u64': (_i32, [_vp, _i64, _vp, ctypes.POINTER(_i64), _vp]),
 }
 

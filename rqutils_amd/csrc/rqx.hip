@@ -24,6 +24,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <memory>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -52,6 +53,18 @@ int fail(int code, const std::string &msg) {
 
 enum { MODE_MATVEC = 0, MODE_LOOKUP = 1, MODE_COUNT = 2, MODE_GRAD = 3 };
 
+// One problem of a plan (a batched plan holds several).  Row, segment, class,
+// group and term indices in the plan arrays are absolute; the descriptor gives
+// the problem's own ranges.
+struct ProbDesc {
+    uint32_t seg_base, n_segs;      // segments [seg_base, seg_base + n_segs)
+    uint32_t cls_base, n_classes;   // classes  [cls_base, cls_base + n_classes)
+    uint32_t grp_base;              // first group (the x == 0 group when has_diag)
+    uint32_t lomask;                // low-key mask 2^L - 1 of the problem's split
+    int32_t has_diag;
+    int32_t pad;
+};
+
 struct KParams {
     const uint64_t *keys;
     const uint32_t *lo;
@@ -66,6 +79,9 @@ struct KParams {
     const void *diag;      // double2[N] or float2[N]
     const uint32_t *tile_row0;
     const uint32_t *tile_seg;
+    const uint32_t *tile_prob;     // problem of each tile
+    const ProbDesc *prob;
+    unsigned int *queue;           // work-queue counter (persistent launch)
     int32_t *idx;                  // lookup output
     double2 *vals;                 // lookup output (optional): matrix element C_j(s_i)
     unsigned long long *counts;    // count output
@@ -73,10 +89,7 @@ struct KParams {
     const void *gy;                // gradient mode: upstream gradient (same dtype as x)
     double *grad;                  // gradient mode: per-term output (accumulated)
     int64_t n_total;
-    uint32_t lomask;
-    int32_t n_classes;
-    int32_t n_segs;
-    int32_t has_diag;
+    uint32_t n_tiles;
 };
 
 template <typename R> struct C2;
@@ -187,21 +200,19 @@ __device__ inline void grad_group(const KParams &p, uint32_t g, uint64_t s, bool
     }
 }
 
+// One tile: <= 256 consecutive rows of one segment, one row per thread.
 template <int MODE, typename R, bool CPLX>
-__global__ __launch_bounds__(TILE) void k_tiles(const KParams p, const void *__restrict__ xin,
-                                                void *__restrict__ yout) {
+__device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s_keys, int32_t *s_tseg,
+                                    const void *__restrict__ xin, void *__restrict__ yout) {
     using V = typename VT<R, CPLX>::T;
     using CC = typename C2<R>::T;
-    extern __shared__ uint32_t s_keys[];      // staged target segment (max_segment keys)
-    __shared__ int32_t s_tseg[TILE];
-
     const int tid = threadIdx.x;
-    const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
+    const ProbDesc pd = p.prob[p.tile_prob[tile]];
     const uint32_t seg = p.tile_seg[tile];
     const uint32_t i = p.tile_row0[tile] + tid;
     const bool active = i < p.seg_off[seg + 1];
     const uint64_t s = active ? p.keys[i] : 0ull;
-    const uint32_t slo = static_cast<uint32_t>(s) & p.lomask;
+    const uint32_t slo = static_cast<uint32_t>(s) & pd.lomask;
     const uint64_t shi = p.seg_hi[seg];
     const V *x = static_cast<const V *>(xin);
 
@@ -210,32 +221,36 @@ __global__ __launch_bounds__(TILE) void k_tiles(const KParams p, const void *__r
     V gv = vzero(static_cast<V *>(nullptr));
     if constexpr (MODE == MODE_GRAD) {
         if (active) gv = static_cast<const V *>(p.gy)[i];
-        if (p.has_diag) {
+        if (pd.has_diag) {
             R wre = R(0), wim = R(0);
             if (active) conj_mul(gv, x[i], wre, wim);
-            grad_group<R>(p, 0u, s, active, wre, wim);
+            grad_group<R>(p, pd.grp_base, s, active, wre, wim);
         }
     }
-    if (p.has_diag && active && MODE != MODE_GRAD) {
+    if (pd.has_diag && active && MODE != MODE_GRAD) {
         if constexpr (MODE == MODE_MATVEC) {
             const CC d = static_cast<const CC *>(p.diag)[i];
             fma_acc(acc, d, x[i]);
         } else if constexpr (MODE == MODE_LOOKUP) {
-            p.idx[i] = static_cast<int32_t>(i);
-            if (p.vals) p.vals[i] = static_cast<const double2 *>(p.diag)[i];
+            p.idx[static_cast<size_t>(pd.grp_base) * p.n_total + i] = static_cast<int32_t>(i);
+            if (p.vals) p.vals[static_cast<size_t>(pd.grp_base) * p.n_total + i] = static_cast<const double2 *>(p.diag)[i];
         } else {
             n_hit += 1;
         }
     }
 
-    for (int cb = 0; cb < p.n_classes; cb += TILE) {
+    const int n_classes = static_cast<int>(pd.n_classes);
+    for (int cb = 0; cb < n_classes; cb += TILE) {
         const int c = cb + tid;
         int t = -1;
-        if (c < p.n_classes) t = find_u64(p.seg_hi, p.n_segs, shi ^ p.cls_hi[c]);
+        if (c < n_classes) {
+            t = find_u64(p.seg_hi + pd.seg_base, static_cast<int>(pd.n_segs), shi ^ p.cls_hi[pd.cls_base + c]);
+            if (t >= 0) t += static_cast<int>(pd.seg_base);
+        }
         __syncthreads();              // readers of the previous chunk are done
         s_tseg[tid] = t;
         __syncthreads();
-        const int cn = min(TILE, p.n_classes - cb);
+        const int cn = min(TILE, n_classes - cb);
         for (int cc = 0; cc < cn; ++cc) {
             const int ts = __builtin_amdgcn_readfirstlane(s_tseg[cc]);
             if (ts < 0) continue;
@@ -243,9 +258,10 @@ __global__ __launch_bounds__(TILE) void k_tiles(const KParams p, const void *__r
             const uint32_t tn = p.seg_off[ts + 1] - ta;
             for (uint32_t k = tid; k < tn; k += TILE) s_keys[k] = p.lo[ta + k];
             __syncthreads();
-            uint32_t g0 = p.cls_goff[cb + cc];
-            const uint32_t g1 = p.cls_goff[cb + cc + 1];
-            if (p.has_diag && g0 == 0) g0 = 1;
+            const uint32_t cls = pd.cls_base + cb + cc;
+            uint32_t g0 = p.cls_goff[cls];
+            const uint32_t g1 = p.cls_goff[cls + 1];
+            if (pd.has_diag && g0 == pd.grp_base) g0 += 1;
             auto handle = [&](uint32_t g, uint32_t base, uint32_t tl) {
                 const bool hit = active && (s_keys[base] == tl);
                 if constexpr (MODE == MODE_MATVEC) {
@@ -311,9 +327,37 @@ __global__ __launch_bounds__(TILE) void k_tiles(const KParams p, const void *__r
             atomicAdd(&p.counts[0], n_hit);
             atomicAdd(&p.counts[1], n_probe);
         }
-        if (tid == 0) atomicAdd(&p.counts[2], static_cast<unsigned long long>(p.n_classes));
+        if (tid == 0) atomicAdd(&p.counts[2], static_cast<unsigned long long>(pd.n_classes));
     }
 }
+
+// One workgroup per tile; tiles of one XCD are contiguous.
+template <int MODE, typename R, bool CPLX>
+__global__ __launch_bounds__(TILE) void k_tiles(const KParams p, const void *__restrict__ xin,
+                                                void *__restrict__ yout) {
+    extern __shared__ uint32_t s_keys[];      // staged target segment (max_segment keys)
+    __shared__ int32_t s_tseg[TILE];
+    process_tile<MODE, R, CPLX>(p, xcd_remap(blockIdx.x, gridDim.x), s_keys, s_tseg, xin, yout);
+}
+
+// Persistent workgroups pull tiles from a queue (tiles are stored largest
+// first); every workgroup leaves when the queue is exhausted.
+template <int MODE, typename R, bool CPLX>
+__global__ __launch_bounds__(TILE) void k_tiles_queue(const KParams p, const void *__restrict__ xin,
+                                                      void *__restrict__ yout) {
+    extern __shared__ uint32_t s_keys[];
+    __shared__ int32_t s_tseg[TILE];
+    __shared__ uint32_t s_tile;
+    for (;;) {
+        if (threadIdx.x == 0) s_tile = atomicAdd(p.queue, 1u);
+        __syncthreads();
+        const uint32_t tile = s_tile;
+        __syncthreads();              // every thread has read s_tile before the next claim
+        if (tile >= p.n_tiles) break;
+        process_tile<MODE, R, CPLX>(p, tile, s_keys, s_tseg, xin, yout);
+    }
+}
+
 
 template <typename V>
 __global__ void k_fill(V *__restrict__ y, int64_t begin, int64_t end, V value) {
@@ -525,6 +569,95 @@ __global__ void k_eigen_small(const double *__restrict__ m, int n, double *__res
     if (threadIdx.x == 0 && blockIdx.x == 0) eigen_small(m, n, out);
 }
 
+// ---------------------------------------------------------------- batched vector kernels
+// Vectors of a batched plan are the concatenation of every problem's rows
+// (rows[k] .. rows[k+1]); these kernels run one LOCG vector step for all
+// problems at once.  Grid: (chunks, problems).
+constexpr int MAXV = 9;
+
+struct PtrPack {
+    const void *a[MAXV];
+    const void *b[MAXV];
+};
+
+__device__ inline void load_c(const double &v, double &re, double &im) { re = v; im = 0.0; }
+__device__ inline void load_c(const float &v, double &re, double &im) { re = v; im = 0.0; }
+__device__ inline void load_c(const double2 &v, double &re, double &im) { re = v.x; im = v.y; }
+__device__ inline void load_c(const float2 &v, double &re, double &im) { re = v.x; im = v.y; }
+__device__ inline void store_c(double &o, double re, double) { o = re; }
+__device__ inline void store_c(float &o, double re, double) { o = static_cast<float>(re); }
+__device__ inline void store_c(double2 &o, double re, double im) { o = make_double2(re, im); }
+__device__ inline void store_c(float2 &o, double re, double im) { o = make_float2(static_cast<float>(re), static_cast<float>(im)); }
+
+// out[k * NP + q] += sum_{i in problem k} conj(a_q[i]) * b_q[i]   (complex double)
+template <typename V, int NP>
+__global__ __launch_bounds__(256) void k_batch_dots(PtrPack pk, const int64_t *__restrict__ rows,
+                                                    double *__restrict__ out) {
+    __shared__ double s_red[4][2 * NP];
+    const int k = blockIdx.y;
+    const int64_t r0 = rows[k], r1 = rows[k + 1];
+    double re[NP], im[NP];
+#pragma unroll
+    for (int q = 0; q < NP; ++q) re[q] = im[q] = 0.0;
+    for (int64_t i = r0 + static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x; i < r1;
+         i += static_cast<int64_t>(gridDim.x) * 256) {
+#pragma unroll
+        for (int q = 0; q < NP; ++q) {
+            double ar, ai, br, bi;
+            load_c(static_cast<const V *>(pk.a[q])[i], ar, ai);
+            load_c(static_cast<const V *>(pk.b[q])[i], br, bi);
+            re[q] += ar * br + ai * bi;
+            im[q] += ar * bi - ai * br;
+        }
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < NP; ++q) {
+        const double sr = wave_sum(re[q]), si = wave_sum(im[q]);
+        if (lane == 0) {
+            s_red[wave][2 * q] = sr;
+            s_red[wave][2 * q + 1] = si;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 2 * NP) {
+        const double v = s_red[0][threadIdx.x] + s_red[1][threadIdx.x] + s_red[2][threadIdx.x] + s_red[3][threadIdx.x];
+        if (v != 0.0) atomicAdd(&out[2 * static_cast<int64_t>(k) * NP + threadIdx.x], v);
+    }
+}
+
+// out[i] = sum_q coef[k * NV + q] * v_q[i] for the rows i of problem k (coef complex double;
+// only the real part is used for real vectors).  out may be one of the inputs.
+template <typename V, int NV>
+__global__ __launch_bounds__(256) void k_batch_combine(PtrPack pk, const int64_t *__restrict__ rows,
+                                                       const double *__restrict__ coef, void *out) {
+    const int k = blockIdx.y;
+    const int64_t r0 = rows[k], r1 = rows[k + 1];
+    double cr[NV], ci[NV];
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+        cr[q] = coef[2 * (static_cast<int64_t>(k) * NV + q)];
+        ci[q] = coef[2 * (static_cast<int64_t>(k) * NV + q) + 1];
+    }
+    for (int64_t i = r0 + static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x; i < r1;
+         i += static_cast<int64_t>(gridDim.x) * 256) {
+        double sr = 0.0, si = 0.0;
+#pragma unroll
+        for (int q = 0; q < NV; ++q) {
+            double vr, vi;
+            load_c(static_cast<const V *>(pk.a[q])[i], vr, vi);
+            sr += cr[q] * vr - ci[q] * vi;
+            si += cr[q] * vi + ci[q] * vr;
+        }
+        store_c(static_cast<V *>(out)[i], sr, si);
+    }
+}
+
+__global__ void k_eigen_batch(const double *__restrict__ mats, int count, int n, double *__restrict__ out) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < count) eigen_small(mats + static_cast<int64_t>(k) * 2 * n * n, n, out + static_cast<int64_t>(k) * (2 + 2 * n));
+}
+
 // ---------------------------------------------------------------- host: index build
 inline int highest_diff_bit(uint64_t a, uint64_t b) { return 63 - __builtin_clzll(a ^ b); }
 
@@ -618,17 +751,30 @@ double estimate_cost(const Segmentation &sg, bool has_diag) {
 
 }  // namespace
 
+struct ProbHost {            // host record of one problem of a plan
+    int64_t row_base = 0, n_states = 0;
+    int64_t term_base = 0, n_terms = 0;
+    int32_t grp_base = 0, n_groups = 0;
+    int32_t seg_base = 0, n_segs = 0;
+    int32_t cls_base = 0, n_classes = 0;
+    int32_t n_tiles = 0, max_seg = 0, L = 0, n_qubits = 0;
+    int32_t has_diag = 0, max_group_terms = 0;
+    uint32_t diag_terms = 0;   // terms of the x == 0 group
+    int32_t is_real = 1;
+    double cost = 0.0;         // cost-model estimate of one matvec
+};
+
 struct rqx_plan {
     int device = 0;
-    int n_qubits = 0;
-    int L = 0;
-    int has_diag = 0;
     int is_real = 1;
     int max_seg = 0;
-    int max_group_terms = 0;
-    uint32_t diag_terms = 0;   // terms of the x == 0 group (group 0)
+    int persistent = 0;        // launch the work-queue kernel
+    int blocks_per_cu = 0;     // persistent grid: blocks per CU (0 = occupancy limit)
+    int occupancy = 1;         // resident tile workgroups per CU
+    int n_cu = 1;
     int64_t n_states = 0, n_total = 0, n_terms = 0;
     int32_t n_groups = 0, n_classes = 0, n_segs = 0, n_tiles = 0;
+    std::vector<ProbHost> probs;
     // device arrays (one allocation, carved)
     void *block = nullptr;
     uint64_t *keys = nullptr;
@@ -647,9 +793,27 @@ struct rqx_plan {
     float2 *diag32 = nullptr;
     uint32_t *tile_row0 = nullptr;
     uint32_t *tile_seg = nullptr;
+    uint32_t *tile_prob = nullptr;
+    ProbDesc *prob = nullptr;
+    int64_t *prob_rows = nullptr;   // K + 1 row offsets
+    unsigned int *queue = nullptr;
 };
 
 namespace {
+
+size_t align_up(size_t v) { return (v + 255) & ~size_t(255); }
+
+struct ProblemIn {
+    int32_t n_qubits;
+    const uint64_t *xmask;
+    const int64_t *grp_off;
+    int32_t n_groups;
+    const uint64_t *zmask;
+    const double *coef;
+    int64_t n_terms;
+    const uint64_t *states;
+    int64_t n_states;
+};
 
 // dynamic LDS of the tile kernel: one staged segment of uint32 low keys
 size_t seg_lds_bytes(const rqx_plan *p) { return static_cast<size_t>(std::max(p->max_seg, 1)) * 4; }
@@ -670,15 +834,13 @@ KParams make_params(const rqx_plan *p, bool fp64) {
     k.diag = fp64 ? static_cast<const void *>(p->diag64) : static_cast<const void *>(p->diag32);
     k.tile_row0 = p->tile_row0;
     k.tile_seg = p->tile_seg;
+    k.tile_prob = p->tile_prob;
+    k.prob = p->prob;
+    k.queue = p->queue;
     k.n_total = p->n_total;
-    k.lomask = (p->L >= 32) ? 0xffffffffu : ((1u << p->L) - 1u);
-    k.n_classes = p->n_classes;
-    k.n_segs = p->n_segs;
-    k.has_diag = p->has_diag;
+    k.n_tiles = static_cast<uint32_t>(p->n_tiles);
     return k;
 }
-
-size_t align_up(size_t v) { return (v + 255) & ~size_t(255); }
 
 int check_plan(const rqx_plan *p) {
     if (!p) return fail(RQX_ERR_ARG, "plan is NULL");
@@ -690,145 +852,239 @@ int check_plan(const rqx_plan *p) {
     return RQX_OK;
 }
 
-}  // namespace
-
-extern "C" {
-
-int rqx_version(void) { return 100; }
-
-const char *rqx_last_error(void) { return g_err.c_str(); }
-
-int rqx_plan_create(rqx_plan **out, int32_t n_qubits, const uint64_t *xmask, const int64_t *grp_off,
-                    int32_t n_groups, const uint64_t *zmask, const double *coef, int64_t n_terms,
-                    const uint64_t *sorted_states, int64_t n_states, int64_t n_total, int32_t split_bits,
-                    rqx_stream_t stream) {
-    g_err.clear();
-    if (!out) return fail(RQX_ERR_ARG, "out is NULL");
-    *out = nullptr;
-    if (n_qubits < 1 || n_qubits > 64) return fail(RQX_ERR_ARG, "n_qubits must be in [1, 64]");
-    if (n_groups < 0 || n_terms < 0 || n_states < 0 || n_total < n_states)
-        return fail(RQX_ERR_ARG, "negative size or n_total < n_states");
-    if (n_total > 0x7fffffffLL) return fail(RQX_ERR_LIMIT, "n_total must be < 2^31");
-    if (n_terms > 0x7fffffffLL) return fail(RQX_ERR_LIMIT, "n_terms must be < 2^31");
-    if (n_groups > 0 && (!xmask || !grp_off)) return fail(RQX_ERR_ARG, "xmask/grp_off is NULL");
-    if (n_terms > 0 && (!zmask || !coef)) return fail(RQX_ERR_ARG, "zmask/coef is NULL");
-    if (n_states > 0 && !sorted_states) return fail(RQX_ERR_ARG, "sorted_states is NULL");
-    const uint64_t qmask = (n_qubits == 64) ? ~0ull : ((1ull << n_qubits) - 1ull);
-    if (n_groups > 0) {
-        if (grp_off[0] != 0 || grp_off[n_groups] != n_terms)
-            return fail(RQX_ERR_ARG, "grp_off must start at 0 and end at n_terms");
-        for (int32_t j = 0; j < n_groups; ++j) {
-            if (grp_off[j + 1] < grp_off[j]) return fail(RQX_ERR_ARG, "grp_off must be non-decreasing");
-            if (xmask[j] & ~qmask) return fail(RQX_ERR_ARG, "xmask has bits beyond n_qubits");
-            if (j > 0 && xmask[j] <= xmask[j - 1])
-                return fail(RQX_ERR_ARG, "xmask must be strictly ascending (one group per X mask)");
-        }
-    } else if (n_terms != 0) {
-        return fail(RQX_ERR_ARG, "terms given without groups");
-    }
-    for (int64_t k = 0; k < n_terms; ++k)
-        if (zmask[k] & ~qmask) return fail(RQX_ERR_ARG, "zmask has bits beyond n_qubits");
-    for (int64_t i = 0; i < n_states; ++i) {
-        if (sorted_states[i] & ~qmask) return fail(RQX_ERR_ARG, "state has bits beyond n_qubits");
-        if (i > 0 && sorted_states[i] <= sorted_states[i - 1])
-            return fail(RQX_ERR_ARG, "states must be sorted ascending and unique");
-    }
-
-    rqx_plan *p = new rqx_plan();
-    HIP_TRY(hipGetDevice(&p->device));
-    p->n_qubits = n_qubits;
-    p->n_states = n_states;
-    p->n_total = n_total;
-    p->n_terms = n_terms;
-    p->n_groups = n_groups;
-    p->has_diag = (n_groups > 0 && xmask[0] == 0) ? 1 : 0;
-    for (int64_t k = 0; k < n_terms; ++k)
-        if (coef[2 * k + 1] != 0.0) p->is_real = 0;
-    for (int32_t j = 0; j < n_groups; ++j)
-        p->max_group_terms = std::max<int>(p->max_group_terms, static_cast<int>(grp_off[j + 1] - grp_off[j]));
-    p->diag_terms = p->has_diag ? static_cast<uint32_t>(grp_off[1]) : 0u;
-
-    // ---- choose the split L
-    std::vector<int8_t> hd(static_cast<size_t>(n_states), 64);
-    for (int64_t i = 1; i < n_states; ++i) hd[i] = static_cast<int8_t>(highest_diff_bit(sorted_states[i - 1], sorted_states[i]));
-    const int Lmax_bits = std::min(32, n_qubits);
-    Segmentation best;
-    if (split_bits >= 0) {
-        if (split_bits > Lmax_bits) {
-            delete p;
-            return fail(RQX_ERR_ARG, "split_bits must be <= min(32, n_qubits)");
-        }
-        if (max_segment(hd, split_bits) > SEG_CAP) {
-            delete p;
-            return fail(RQX_ERR_LIMIT, "split_bits gives a segment larger than " + std::to_string(SEG_CAP) + " rows");
-        }
-        best.L = split_bits;
-        build_segments(best, sorted_states, n_states, xmask, n_groups);
+template <int MODE, typename R, bool CPLX>
+hipError_t launch_tiles(const rqx_plan *p, const KParams &k, const void *x, void *y, hipStream_t st) {
+    if (p->n_tiles == 0) return hipSuccess;
+    const size_t lds = seg_lds_bytes(p);
+    if (p->persistent) {
+        hipLaunchKernelGGL(k_fill<unsigned int>, dim3(1), dim3(64), 0, st, p->queue, int64_t(0), int64_t(1), 0u);
+        const int bpc = p->blocks_per_cu > 0 ? p->blocks_per_cu : p->occupancy;
+        const int grid = std::max(1, std::min(p->n_tiles, p->n_cu * bpc));
+        hipLaunchKernelGGL((k_tiles_queue<MODE, R, CPLX>), dim3(grid), dim3(TILE), lds, st, k, x, y);
     } else {
-        // largest feasible L (max segment is non-decreasing in L), then the
-        // cheapest of the candidates below it by the sampled cost model
-        int lo_l = 0, hi_l = Lmax_bits;
-        while (lo_l < hi_l) {
-            const int mid = (lo_l + hi_l + 1) / 2;
-            if (max_segment(hd, mid) <= SEG_CAP) lo_l = mid; else hi_l = mid - 1;
+        hipLaunchKernelGGL((k_tiles<MODE, R, CPLX>), dim3(p->n_tiles), dim3(TILE), lds, st, k, x, y);
+    }
+    return hipGetLastError();
+}
+
+template <int MODE>
+hipError_t launch_dtype(const rqx_plan *p, const KParams &k, const void *x, void *y, int32_t dtype, hipStream_t st) {
+    switch (dtype) {
+        case RQX_F64: return launch_tiles<MODE, double, false>(p, k, x, y, st);
+        case RQX_C128: return launch_tiles<MODE, double, true>(p, k, x, y, st);
+        case RQX_F32: return launch_tiles<MODE, float, false>(p, k, x, y, st);
+        default: return launch_tiles<MODE, float, true>(p, k, x, y, st);
+    }
+}
+
+int validate_problem(const ProblemIn &q, int k) {
+    const std::string tag = "problem " + std::to_string(k) + ": ";
+    if (q.n_qubits < 1 || q.n_qubits > 64) return fail(RQX_ERR_ARG, tag + "n_qubits must be in [1, 64]");
+    if (q.n_groups < 0 || q.n_terms < 0 || q.n_states < 0) return fail(RQX_ERR_ARG, tag + "negative size");
+    if (q.n_groups > 0 && (!q.xmask || !q.grp_off)) return fail(RQX_ERR_ARG, tag + "xmask/grp_off is NULL");
+    if (q.n_terms > 0 && (!q.zmask || !q.coef)) return fail(RQX_ERR_ARG, tag + "zmask/coef is NULL");
+    if (q.n_states > 0 && !q.states) return fail(RQX_ERR_ARG, tag + "sorted_states is NULL");
+    const uint64_t qmask = (q.n_qubits == 64) ? ~0ull : ((1ull << q.n_qubits) - 1ull);
+    if (q.n_groups > 0) {
+        if (q.grp_off[0] != 0 || q.grp_off[q.n_groups] != q.n_terms)
+            return fail(RQX_ERR_ARG, tag + "grp_off must start at 0 and end at n_terms");
+        for (int32_t j = 0; j < q.n_groups; ++j) {
+            if (q.grp_off[j + 1] < q.grp_off[j]) return fail(RQX_ERR_ARG, tag + "grp_off must be non-decreasing");
+            if (q.xmask[j] & ~qmask) return fail(RQX_ERR_ARG, tag + "xmask has bits beyond n_qubits");
+            if (j > 0 && q.xmask[j] <= q.xmask[j - 1])
+                return fail(RQX_ERR_ARG, tag + "xmask must be strictly ascending (one group per X mask)");
         }
-        const int Lfeas = lo_l;
-        bool first = true;
-        for (int L = Lfeas; L >= std::max(0, Lfeas - 8); --L) {
-            Segmentation cand;
-            cand.L = L;
-            build_segments(cand, sorted_states, n_states, xmask, n_groups);
-            cand.cost = estimate_cost(cand, p->has_diag);
-            if (first || cand.cost < best.cost) {
-                best = std::move(cand);
-                first = false;
+    } else if (q.n_terms != 0) {
+        return fail(RQX_ERR_ARG, tag + "terms given without groups");
+    }
+    for (int64_t t = 0; t < q.n_terms; ++t)
+        if (q.zmask[t] & ~qmask) return fail(RQX_ERR_ARG, tag + "zmask has bits beyond n_qubits");
+    for (int64_t i = 0; i < q.n_states; ++i) {
+        if (q.states[i] & ~qmask) return fail(RQX_ERR_ARG, tag + "state has bits beyond n_qubits");
+        if (i > 0 && q.states[i] <= q.states[i - 1])
+            return fail(RQX_ERR_ARG, tag + "states must be sorted ascending and unique");
+    }
+    return RQX_OK;
+}
+
+// Split choice: largest L whose segments fit SEG_CAP, then the cheapest of the
+// candidates below it by the sampled cost model (or the caller's L).
+int choose_split(const ProblemIn &q, int32_t split_bits, bool has_diag, Segmentation &best, int k) {
+    std::vector<int8_t> hd(static_cast<size_t>(q.n_states), 64);
+    for (int64_t i = 1; i < q.n_states; ++i) hd[i] = static_cast<int8_t>(highest_diff_bit(q.states[i - 1], q.states[i]));
+    const int Lmax_bits = std::min(32, q.n_qubits);
+    const std::string tag = "problem " + std::to_string(k) + ": ";
+    if (split_bits >= 0) {
+        if (split_bits > Lmax_bits) return fail(RQX_ERR_ARG, tag + "split_bits must be <= min(32, n_qubits)");
+        if (max_segment(hd, split_bits) > SEG_CAP)
+            return fail(RQX_ERR_LIMIT, tag + "split_bits gives a segment larger than " + std::to_string(SEG_CAP) + " rows");
+        best.L = split_bits;
+        build_segments(best, q.states, q.n_states, q.xmask, q.n_groups);
+        best.cost = estimate_cost(best, has_diag);
+        return RQX_OK;
+    }
+    int lo_l = 0, hi_l = Lmax_bits;
+    while (lo_l < hi_l) {
+        const int mid = (lo_l + hi_l + 1) / 2;
+        if (max_segment(hd, mid) <= SEG_CAP) lo_l = mid; else hi_l = mid - 1;
+    }
+    bool first = true;
+    for (int L = lo_l; L >= std::max(0, lo_l - 8); --L) {
+        Segmentation cand;
+        cand.L = L;
+        build_segments(cand, q.states, q.n_states, q.xmask, q.n_groups);
+        cand.cost = estimate_cost(cand, has_diag);
+        if (first || cand.cost < best.cost) {
+            best = std::move(cand);
+            first = false;
+        }
+    }
+    return RQX_OK;
+}
+
+int build_plan(rqx_plan **out, const std::vector<ProblemIn> &in, int64_t n_total, int32_t split_bits,
+               int persistent, hipStream_t st) {
+    *out = nullptr;
+    const int K = static_cast<int>(in.size());
+    int64_t N = 0, T = 0, J = 0;
+    for (int k = 0; k < K; ++k) {
+        if (int rc = validate_problem(in[k], k)) return rc;
+        N += in[k].n_states;
+        T += in[k].n_terms;
+        J += in[k].n_groups;
+    }
+    if (n_total < 0) n_total = N;
+    if (n_total < N) return fail(RQX_ERR_ARG, "n_total < n_states");
+    if (n_total > 0x7fffffffLL) return fail(RQX_ERR_LIMIT, "n_total must be < 2^31");
+    if (T > 0x7fffffffLL || J > 0x7fffffffLL) return fail(RQX_ERR_LIMIT, "terms and groups must be < 2^31");
+
+    std::unique_ptr<rqx_plan> p(new rqx_plan());
+    HIP_TRY(hipGetDevice(&p->device));
+    HIP_TRY(hipDeviceGetAttribute(&p->n_cu, hipDeviceAttributeMultiprocessorCount, p->device));
+    p->persistent = persistent;
+    p->n_states = N;
+    p->n_total = n_total;
+    p->n_terms = T;
+    p->n_groups = static_cast<int32_t>(J);
+
+    // host arrays of the whole plan
+    std::vector<uint32_t> lo(static_cast<size_t>(N)), seg_off_all, cls_goff_all, g_xlo, g_toff, tile_row0, tile_seg,
+        tile_prob;
+    std::vector<uint64_t> keys(static_cast<size_t>(N)), seg_hi_all, cls_hi_all, g_x, zall;
+    std::vector<double> c64;
+    std::vector<float2> c32;
+    std::vector<ProbDesc> desc(static_cast<size_t>(std::max(K, 1)));
+    std::vector<int64_t> prob_rows(static_cast<size_t>(K) + 1, 0);
+    std::vector<double> tile_cost;
+    int64_t row_base = 0, term_base = 0;
+    int32_t grp_base = 0;
+    for (int k = 0; k < K; ++k) {
+        const ProblemIn &q = in[k];
+        ProbHost ph;
+        ph.n_qubits = q.n_qubits;
+        ph.row_base = row_base;
+        ph.n_states = q.n_states;
+        ph.term_base = term_base;
+        ph.n_terms = q.n_terms;
+        ph.grp_base = grp_base;
+        ph.n_groups = q.n_groups;
+        ph.has_diag = (q.n_groups > 0 && q.xmask[0] == 0) ? 1 : 0;
+        ph.diag_terms = ph.has_diag ? static_cast<uint32_t>(q.grp_off[1]) : 0u;
+        for (int64_t t = 0; t < q.n_terms; ++t)
+            if (q.coef[2 * t + 1] != 0.0) ph.is_real = 0;
+        for (int32_t j = 0; j < q.n_groups; ++j)
+            ph.max_group_terms = std::max<int>(ph.max_group_terms, static_cast<int>(q.grp_off[j + 1] - q.grp_off[j]));
+        Segmentation sg;
+        if (int rc = choose_split(q, split_bits, ph.has_diag, sg, k)) return rc;
+        ph.L = sg.L;
+        ph.max_seg = sg.max_seg;
+        ph.cost = sg.cost;
+        ph.seg_base = static_cast<int32_t>(seg_hi_all.size());
+        ph.n_segs = static_cast<int32_t>(sg.seg_hi.size());
+        ph.cls_base = static_cast<int32_t>(cls_hi_all.size());
+        ph.n_classes = static_cast<int32_t>(sg.cls_hi.size());
+        const uint32_t lomask = (sg.L >= 32) ? 0xffffffffu : ((1u << sg.L) - 1u);
+        for (int64_t i = 0; i < q.n_states; ++i) {
+            keys[row_base + i] = q.states[i];
+            lo[row_base + i] = static_cast<uint32_t>(q.states[i]) & lomask;
+        }
+        for (int32_t m = 0; m < ph.n_segs; ++m) {
+            seg_hi_all.push_back(sg.seg_hi[m]);
+            seg_off_all.push_back(static_cast<uint32_t>(row_base + sg.seg_off[m]));
+            for (uint32_t r = sg.seg_off[m]; r < sg.seg_off[m + 1]; r += TILE) {
+                tile_row0.push_back(static_cast<uint32_t>(row_base + r));
+                tile_seg.push_back(static_cast<uint32_t>(ph.seg_base + m));
+                tile_prob.push_back(static_cast<uint32_t>(k));
+                ++ph.n_tiles;
             }
         }
-    }
-    p->L = best.L;
-    p->max_seg = best.max_seg;
-    p->n_segs = static_cast<int32_t>(best.seg_hi.size());
-    p->n_classes = static_cast<int32_t>(best.cls_hi.size());
-
-    std::vector<uint32_t> tile_row0, tile_seg;
-    for (int32_t m = 0; m < p->n_segs; ++m)
-        for (uint32_t r = best.seg_off[m]; r < best.seg_off[m + 1]; r += TILE) {
-            tile_row0.push_back(r);
-            tile_seg.push_back(static_cast<uint32_t>(m));
+        for (int32_t c = 0; c < ph.n_classes; ++c) {
+            cls_hi_all.push_back(sg.cls_hi[c]);
+            cls_goff_all.push_back(static_cast<uint32_t>(grp_base + sg.cls_goff[c]));
         }
+        for (int32_t j = 0; j < q.n_groups; ++j) {
+            g_xlo.push_back(static_cast<uint32_t>(q.xmask[j]) & lomask);
+            g_x.push_back(q.xmask[j]);
+            g_toff.push_back(static_cast<uint32_t>(term_base + q.grp_off[j]));
+        }
+        for (int64_t t = 0; t < q.n_terms; ++t) {
+            zall.push_back(q.zmask[t]);
+            c64.push_back(q.coef[2 * t]);
+            c64.push_back(q.coef[2 * t + 1]);
+            c32.push_back(make_float2(static_cast<float>(q.coef[2 * t]), static_cast<float>(q.coef[2 * t + 1])));
+        }
+        for (int32_t t = 0; t < ph.n_tiles; ++t) tile_cost.push_back(ph.n_tiles ? ph.cost / ph.n_tiles : 0.0);
+        desc[k] = ProbDesc{static_cast<uint32_t>(ph.seg_base), static_cast<uint32_t>(ph.n_segs),
+                           static_cast<uint32_t>(ph.cls_base), static_cast<uint32_t>(ph.n_classes),
+                           static_cast<uint32_t>(ph.grp_base), lomask, ph.has_diag, 0};
+        p->max_seg = std::max(p->max_seg, ph.max_seg);
+        p->is_real = p->is_real && ph.is_real;
+        row_base += q.n_states;
+        term_base += q.n_terms;
+        grp_base += q.n_groups;
+        prob_rows[k + 1] = row_base;
+        p->probs.push_back(ph);
+    }
+    // segment and class ranges end with their sentinels
+    seg_off_all.push_back(static_cast<uint32_t>(N));
+    cls_goff_all.push_back(static_cast<uint32_t>(J));
+    g_toff.push_back(static_cast<uint32_t>(T));
+    p->n_segs = static_cast<int32_t>(seg_hi_all.size());
+    p->n_classes = static_cast<int32_t>(cls_hi_all.size());
     p->n_tiles = static_cast<int32_t>(tile_row0.size());
-
-    const uint32_t lomask = (p->L >= 32) ? 0xffffffffu : ((1u << p->L) - 1u);
-    std::vector<uint32_t> lo(static_cast<size_t>(n_states));
-    for (int64_t i = 0; i < n_states; ++i) lo[i] = static_cast<uint32_t>(sorted_states[i]) & lomask;
-    std::vector<uint32_t> g_xlo(static_cast<size_t>(n_groups)), g_toff(static_cast<size_t>(n_groups) + 1);
-    for (int32_t j = 0; j < n_groups; ++j) g_xlo[j] = static_cast<uint32_t>(xmask[j]) & lomask;
-    for (int32_t j = 0; j <= n_groups; ++j) g_toff[j] = n_groups ? static_cast<uint32_t>(grp_off[j]) : 0u;
-    if (n_groups == 0) g_toff.assign(1, 0u);
-    std::vector<float2> c32(static_cast<size_t>(n_terms));
-    for (int64_t k = 0; k < n_terms; ++k)
-        c32[k] = make_float2(static_cast<float>(coef[2 * k]), static_cast<float>(coef[2 * k + 1]));
+    if (persistent && p->n_tiles > 1) {
+        // longest tiles first (LPT order for the work queue)
+        std::vector<uint32_t> order(tile_row0.size());
+        for (size_t t = 0; t < order.size(); ++t) order[t] = static_cast<uint32_t>(t);
+        std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return tile_cost[a] > tile_cost[b]; });
+        std::vector<uint32_t> r0(order.size()), sg(order.size()), pr(order.size());
+        for (size_t t = 0; t < order.size(); ++t) {
+            r0[t] = tile_row0[order[t]];
+            sg[t] = tile_seg[order[t]];
+            pr[t] = tile_prob[order[t]];
+        }
+        tile_row0.swap(r0);
+        tile_seg.swap(sg);
+        tile_prob.swap(pr);
+    }
 
     // ---- one device block, carved
-    const size_t n = static_cast<size_t>(n_states);
     size_t off = 0;
     auto carve = [&](size_t bytes) {
         const size_t o = off;
         off += align_up(std::max<size_t>(bytes, 1));
         return o;
     };
-    const size_t o_keys = carve(n * 8), o_lo = carve(n * 4), o_seghi = carve(best.seg_hi.size() * 8),
-                 o_segoff = carve(best.seg_off.size() * 4), o_clshi = carve(best.cls_hi.size() * 8),
-                 o_clsgoff = carve(std::max<size_t>(best.cls_goff.size(), 1) * 4), o_gxlo = carve(g_xlo.size() * 4),
-                 o_gtoff = carve(g_toff.size() * 4), o_z = carve(static_cast<size_t>(n_terms) * 8),
-                 o_c64 = carve(static_cast<size_t>(n_terms) * 16), o_c32 = carve(static_cast<size_t>(n_terms) * 8),
-                 o_d64 = carve(n * 16), o_d32 = carve(n * 8), o_trow = carve(tile_row0.size() * 4),
-                 o_tseg = carve(tile_seg.size() * 4), o_gx = carve(static_cast<size_t>(n_groups) * 8);
-    hipError_t e = hipMalloc(&p->block, off);
-    if (e != hipSuccess) {
-        delete p;
-        return fail(RQX_ERR_HIP, std::string("hipMalloc(plan): ") + hipGetErrorString(e));
-    }
+    const size_t n = static_cast<size_t>(N);
+    const size_t o_keys = carve(n * 8), o_lo = carve(n * 4), o_seghi = carve(seg_hi_all.size() * 8),
+                 o_segoff = carve(seg_off_all.size() * 4), o_clshi = carve(cls_hi_all.size() * 8),
+                 o_clsgoff = carve(cls_goff_all.size() * 4), o_gxlo = carve(g_xlo.size() * 4),
+                 o_gtoff = carve(g_toff.size() * 4), o_gx = carve(g_x.size() * 8), o_z = carve(zall.size() * 8),
+                 o_c64 = carve(c64.size() * 8), o_c32 = carve(c32.size() * 8), o_d64 = carve(n * 16),
+                 o_d32 = carve(n * 8), o_trow = carve(tile_row0.size() * 4), o_tseg = carve(tile_seg.size() * 4),
+                 o_tprob = carve(tile_prob.size() * 4), o_desc = carve(desc.size() * sizeof(ProbDesc)),
+                 o_prows = carve(prob_rows.size() * 8), o_queue = carve(sizeof(unsigned int));
+    HIP_TRY(hipMalloc(&p->block, off));
     char *b = static_cast<char *>(p->block);
     p->keys = reinterpret_cast<uint64_t *>(b + o_keys);
     p->lo = reinterpret_cast<uint32_t *>(b + o_lo);
@@ -846,57 +1102,101 @@ int rqx_plan_create(rqx_plan **out, int32_t n_qubits, const uint64_t *xmask, con
     p->diag32 = reinterpret_cast<float2 *>(b + o_d32);
     p->tile_row0 = reinterpret_cast<uint32_t *>(b + o_trow);
     p->tile_seg = reinterpret_cast<uint32_t *>(b + o_tseg);
+    p->tile_prob = reinterpret_cast<uint32_t *>(b + o_tprob);
+    p->prob = reinterpret_cast<ProbDesc *>(b + o_desc);
+    p->prob_rows = reinterpret_cast<int64_t *>(b + o_prows);
+    p->queue = reinterpret_cast<unsigned int *>(b + o_queue);
+    rqx_plan *raw = p.get();
+    auto release = [&]() { (void)hipFree(raw->block); };
 
-    hipStream_t st = static_cast<hipStream_t>(stream);
     auto up = [&](void *dst, const void *src, size_t bytes) -> hipError_t {
         if (bytes == 0) return hipSuccess;
         return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st);
     };
-    hipError_t errs[16] = {
-        up(p->keys, sorted_states, n * 8),
-        up(p->lo, lo.data(), n * 4),
-        up(p->seg_hi, best.seg_hi.data(), best.seg_hi.size() * 8),
-        up(p->seg_off, best.seg_off.data(), best.seg_off.size() * 4),
-        up(p->cls_hi, best.cls_hi.data(), best.cls_hi.size() * 8),
-        up(p->cls_goff, best.cls_goff.data(), best.cls_goff.size() * 4),
-        up(p->g_xlo, g_xlo.data(), g_xlo.size() * 4),
-        up(p->g_toff, g_toff.data(), g_toff.size() * 4),
-        up(p->z, zmask, static_cast<size_t>(n_terms) * 8),
-        up(p->coef64, coef, static_cast<size_t>(n_terms) * 16),
-        up(p->coef32, c32.data(), static_cast<size_t>(n_terms) * 8),
-        up(p->tile_row0, tile_row0.data(), tile_row0.size() * 4),
-        up(p->tile_seg, tile_seg.data(), tile_seg.size() * 4),
-        up(p->g_x, xmask, static_cast<size_t>(n_groups) * 8),
-        hipSuccess, hipSuccess};
+    const hipError_t errs[] = {
+        up(p->keys, keys.data(), n * 8), up(p->lo, lo.data(), n * 4),
+        up(p->seg_hi, seg_hi_all.data(), seg_hi_all.size() * 8), up(p->seg_off, seg_off_all.data(), seg_off_all.size() * 4),
+        up(p->cls_hi, cls_hi_all.data(), cls_hi_all.size() * 8), up(p->cls_goff, cls_goff_all.data(), cls_goff_all.size() * 4),
+        up(p->g_xlo, g_xlo.data(), g_xlo.size() * 4), up(p->g_toff, g_toff.data(), g_toff.size() * 4),
+        up(p->g_x, g_x.data(), g_x.size() * 8), up(p->z, zall.data(), zall.size() * 8),
+        up(p->coef64, c64.data(), c64.size() * 8), up(p->coef32, c32.data(), c32.size() * 8),
+        up(p->tile_row0, tile_row0.data(), tile_row0.size() * 4), up(p->tile_seg, tile_seg.data(), tile_seg.size() * 4),
+        up(p->tile_prob, tile_prob.data(), tile_prob.size() * 4), up(p->prob, desc.data(), desc.size() * sizeof(ProbDesc)),
+        up(p->prob_rows, prob_rows.data(), prob_rows.size() * 8)};
     for (hipError_t ei : errs)
         if (ei != hipSuccess) {
-            (void)hipFree(p->block);
-            delete p;
+            release();
             return fail(RQX_ERR_HIP, std::string("hipMemcpyAsync(plan): ") + hipGetErrorString(ei));
         }
-    if (n > 0) {
-        const int threads = 256;
-        const unsigned blocks = static_cast<unsigned>((n + threads - 1) / threads);
-        const uint32_t k1 = p->has_diag ? static_cast<uint32_t>(grp_off[1]) : 0u;
-        hipLaunchKernelGGL(k_diag, dim3(blocks), dim3(threads), 0, st, p->keys, static_cast<int64_t>(n), p->z,
-                           p->coef64, 0u, k1, p->diag64, p->diag32);
-        e = hipGetLastError();
-        if (e == hipSuccess) e = hipStreamSynchronize(st);
-        if (e != hipSuccess) {
-            (void)hipFree(p->block);
-            delete p;
-            return fail(RQX_ERR_HIP, std::string("k_diag: ") + hipGetErrorString(e));
-        }
-    } else {
-        e = hipStreamSynchronize(st);
-        if (e != hipSuccess) {
-            (void)hipFree(p->block);
-            delete p;
-            return fail(RQX_ERR_HIP, std::string("plan upload: ") + hipGetErrorString(e));
-        }
+    for (const ProbHost &ph : p->probs) {
+        if (ph.n_states == 0) continue;
+        const unsigned blocks = static_cast<unsigned>((ph.n_states + 255) / 256);
+        const uint32_t k0 = static_cast<uint32_t>(ph.term_base);
+        hipLaunchKernelGGL(k_diag, dim3(blocks), dim3(256), 0, st, p->keys + ph.row_base, ph.n_states, p->z, p->coef64,
+                           k0, k0 + ph.diag_terms, p->diag64 + ph.row_base, p->diag32 + ph.row_base);
     }
-    *out = p;
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e == hipSuccess) {
+        int occ = 0;
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &occ, reinterpret_cast<const void *>(&k_tiles_queue<MODE_MATVEC, double, true>), TILE, seg_lds_bytes(raw));
+        p->occupancy = std::max(occ, 1);
+    }
+    if (e != hipSuccess) {
+        release();
+        return fail(RQX_ERR_HIP, std::string("plan build: ") + hipGetErrorString(e));
+    }
+    *out = p.release();
     return RQX_OK;
+}
+
+int check_dtype(const rqx_plan *p, int32_t dtype) {
+    if (dtype < RQX_F64 || dtype > RQX_C64) return fail(RQX_ERR_ARG, "unknown dtype");
+    const bool cplx = (dtype == RQX_C128 || dtype == RQX_C64);
+    if (!cplx && !p->is_real)
+        return fail(RQX_ERR_DTYPE, "real vector dtype requested but the Hamiltonian has complex coefficients");
+    return RQX_OK;
+}
+
+bool is_fp64(int32_t dtype) { return dtype == RQX_F64 || dtype == RQX_C128; }
+
+}  // namespace
+
+extern "C" {
+
+int rqx_version(void) { return 200; }
+
+const char *rqx_last_error(void) { return g_err.c_str(); }
+
+int rqx_plan_create(rqx_plan **out, int32_t n_qubits, const uint64_t *xmask, const int64_t *grp_off,
+                    int32_t n_groups, const uint64_t *zmask, const double *coef, int64_t n_terms,
+                    const uint64_t *sorted_states, int64_t n_states, int64_t n_total, int32_t split_bits,
+                    rqx_stream_t stream) {
+    g_err.clear();
+    if (!out) return fail(RQX_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    if (n_total < n_states) return fail(RQX_ERR_ARG, "negative size or n_total < n_states");
+    std::vector<ProblemIn> in = {{n_qubits, xmask, grp_off, n_groups, zmask, coef, n_terms, sorted_states, n_states}};
+    return build_plan(out, in, n_total, split_bits, 0, static_cast<hipStream_t>(stream));
+}
+
+int rqx_plan_create_batch(rqx_plan **out, int32_t n_problems, const int32_t *n_qubits,
+                          const uint64_t *const *xmask, const int64_t *const *grp_off, const int32_t *n_groups,
+                          const uint64_t *const *zmask, const double *const *coef, const int64_t *n_terms,
+                          const uint64_t *const *sorted_states, const int64_t *n_states, int32_t split_bits,
+                          rqx_stream_t stream) {
+    g_err.clear();
+    if (!out) return fail(RQX_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    if (n_problems < 1) return fail(RQX_ERR_ARG, "n_problems must be >= 1");
+    if (!n_qubits || !xmask || !grp_off || !n_groups || !zmask || !coef || !n_terms || !sorted_states || !n_states)
+        return fail(RQX_ERR_ARG, "NULL argument array");
+    std::vector<ProblemIn> in;
+    for (int k = 0; k < n_problems; ++k)
+        in.push_back({n_qubits[k], xmask[k], grp_off[k], n_groups[k], zmask[k], coef[k], n_terms[k], sorted_states[k],
+                      n_states[k]});
+    return build_plan(out, in, -1, split_bits, 1, static_cast<hipStream_t>(stream));
 }
 
 int rqx_plan_destroy(rqx_plan *plan) {
@@ -915,8 +1215,9 @@ int rqx_plan_destroy(rqx_plan *plan) {
 int rqx_plan_get_info(const rqx_plan *p, rqx_plan_info *out) {
     g_err.clear();
     if (!p || !out) return fail(RQX_ERR_ARG, "NULL argument");
-    out->n_qubits = p->n_qubits;
-    out->split_bits = p->L;
+    const ProbHost &p0 = p->probs.front();
+    out->n_qubits = p0.n_qubits;
+    out->split_bits = p0.L;
     out->n_states = p->n_states;
     out->n_total = p->n_total;
     out->n_groups = p->n_groups;
@@ -925,9 +1226,47 @@ int rqx_plan_get_info(const rqx_plan *p, rqx_plan_info *out) {
     out->n_segments = p->n_segs;
     out->n_tiles = p->n_tiles;
     out->max_segment = p->max_seg;
-    out->has_diagonal = p->has_diag;
+    out->has_diagonal = p0.has_diag;
     out->is_real = p->is_real;
-    out->max_group_terms = p->max_group_terms;
+    int mg = 0;
+    for (const ProbHost &ph : p->probs) mg = std::max(mg, ph.max_group_terms);
+    out->max_group_terms = mg;
+    out->n_problems = static_cast<int32_t>(p->probs.size());
+    out->persistent = p->persistent;
+    out->blocks_per_cu = p->blocks_per_cu > 0 ? p->blocks_per_cu : p->occupancy;
+    out->n_cu = p->n_cu;
+    return RQX_OK;
+}
+
+int rqx_plan_get_problem(const rqx_plan *p, int32_t k, rqx_problem_info *out) {
+    g_err.clear();
+    if (!p || !out) return fail(RQX_ERR_ARG, "NULL argument");
+    if (k < 0 || k >= static_cast<int32_t>(p->probs.size())) return fail(RQX_ERR_ARG, "problem index out of range");
+    const ProbHost &ph = p->probs[k];
+    out->row_offset = ph.row_base;
+    out->n_states = ph.n_states;
+    out->term_offset = ph.term_base;
+    out->n_terms = ph.n_terms;
+    out->group_offset = ph.grp_base;
+    out->n_groups = ph.n_groups;
+    out->n_qubits = ph.n_qubits;
+    out->split_bits = ph.L;
+    out->n_segments = ph.n_segs;
+    out->n_classes = ph.n_classes;
+    out->n_tiles = ph.n_tiles;
+    out->max_segment = ph.max_seg;
+    out->has_diagonal = ph.has_diag;
+    out->is_real = ph.is_real;
+    out->cost = ph.cost;
+    return RQX_OK;
+}
+
+int rqx_plan_set_launch(rqx_plan *p, int32_t persistent, int32_t blocks_per_cu) {
+    g_err.clear();
+    if (!p) return fail(RQX_ERR_ARG, "plan is NULL");
+    if (blocks_per_cu < 0 || blocks_per_cu > 64) return fail(RQX_ERR_ARG, "blocks_per_cu must be in [0, 64]");
+    p->persistent = persistent ? 1 : 0;
+    p->blocks_per_cu = blocks_per_cu;
     return RQX_OK;
 }
 
@@ -947,13 +1286,11 @@ int rqx_lookup(const rqx_plan *p, int32_t *idx, double *vals, rqx_stream_t strea
             HIP_TRY(hipGetLastError());
         }
     }
-    if (p->n_tiles > 0 && p->n_groups > 0) {
+    if (p->n_groups > 0) {
         KParams k = make_params(p, true);
         k.idx = idx;
         k.vals = reinterpret_cast<double2 *>(vals);
-        hipLaunchKernelGGL((k_tiles<MODE_LOOKUP, double, true>), dim3(p->n_tiles), dim3(TILE), seg_lds_bytes(p), st, k,
-                           nullptr, nullptr);
-        HIP_TRY(hipGetLastError());
+        HIP_TRY((launch_tiles<MODE_LOOKUP, double, true>(p, k, nullptr, nullptr, st)));
     }
     return RQX_OK;
 }
@@ -970,7 +1307,7 @@ int rqx_diagonal(const rqx_plan *p, double *diag, rqx_stream_t stream) {
                            make_double2(0.0, 0.0));
         HIP_TRY(hipGetLastError());
     }
-    if (p->has_diag && p->n_states > 0)
+    if (p->n_states > 0)
         HIP_TRY(hipMemcpyAsync(d, p->diag64, static_cast<size_t>(p->n_states) * 16, hipMemcpyDeviceToDevice, st));
     return RQX_OK;
 }
@@ -981,24 +1318,10 @@ int rqx_matvec(const rqx_plan *p, const void *x, void *y, int32_t dtype, rqx_str
     if (p->n_total == 0) return RQX_OK;
     if (!x || !y) return fail(RQX_ERR_ARG, "x or y is NULL");
     if (x == y) return fail(RQX_ERR_ARG, "x and y must not alias");
-    if (dtype < RQX_F64 || dtype > RQX_C64) return fail(RQX_ERR_ARG, "unknown dtype");
-    const bool cplx = (dtype == RQX_C128 || dtype == RQX_C64);
-    if (!cplx && !p->is_real)
-        return fail(RQX_ERR_DTYPE, "real vector dtype requested but the Hamiltonian has complex coefficients");
-    const bool fp64 = (dtype == RQX_F64 || dtype == RQX_C128);
+    if (int rc = check_dtype(p, dtype)) return rc;
     hipStream_t st = static_cast<hipStream_t>(stream);
-    KParams k = make_params(p, fp64);
-    if (p->n_tiles > 0) {
-        const dim3 grid(p->n_tiles), block(TILE);
-        const size_t lds = seg_lds_bytes(p);
-        switch (dtype) {
-            case RQX_F64: hipLaunchKernelGGL((k_tiles<MODE_MATVEC, double, false>), grid, block, lds, st, k, x, y); break;
-            case RQX_C128: hipLaunchKernelGGL((k_tiles<MODE_MATVEC, double, true>), grid, block, lds, st, k, x, y); break;
-            case RQX_F32: hipLaunchKernelGGL((k_tiles<MODE_MATVEC, float, false>), grid, block, lds, st, k, x, y); break;
-            default: hipLaunchKernelGGL((k_tiles<MODE_MATVEC, float, true>), grid, block, lds, st, k, x, y); break;
-        }
-        HIP_TRY(hipGetLastError());
-    }
+    const KParams k = make_params(p, is_fp64(dtype));
+    HIP_TRY(launch_dtype<MODE_MATVEC>(p, k, x, y, dtype, st));
     if (p->n_total > p->n_states) {
         const int64_t cnt = p->n_total - p->n_states;
         const unsigned blocks = static_cast<unsigned>((cnt + 255) / 256);
@@ -1018,14 +1341,11 @@ int rqx_count(const rqx_plan *p, int64_t *counts, rqx_stream_t stream) {
     if (int rc = check_plan(p)) return rc;
     if (!counts) return fail(RQX_ERR_ARG, "counts is NULL");
     hipStream_t st = static_cast<hipStream_t>(stream);
-    HIP_TRY(hipMemsetAsync(counts, 0, 3 * sizeof(int64_t), st));
-    if (p->n_tiles > 0) {
-        KParams k = make_params(p, true);
-        k.counts = reinterpret_cast<unsigned long long *>(counts);
-        hipLaunchKernelGGL((k_tiles<MODE_COUNT, double, true>), dim3(p->n_tiles), dim3(TILE), seg_lds_bytes(p), st, k, nullptr,
-                           nullptr);
-        HIP_TRY(hipGetLastError());
-    }
+    hipLaunchKernelGGL(k_fill<int64_t>, dim3(1), dim3(64), 0, st, counts, int64_t(0), int64_t(3), int64_t(0));
+    HIP_TRY(hipGetLastError());
+    KParams k = make_params(p, true);
+    k.counts = reinterpret_cast<unsigned long long *>(counts);
+    HIP_TRY((launch_tiles<MODE_COUNT, double, true>(p, k, nullptr, nullptr, st)));
     return RQX_OK;
 }
 
@@ -1036,29 +1356,15 @@ int rqx_grad_coef(const rqx_plan *p, const void *x, const void *gy, double *grad
     if (p->n_terms == 0) return RQX_OK;
     if (!grad) return fail(RQX_ERR_ARG, "grad is NULL");
     if (p->n_states > 0 && (!x || !gy)) return fail(RQX_ERR_ARG, "x or gy is NULL");
-    if (dtype < RQX_F64 || dtype > RQX_C64) return fail(RQX_ERR_ARG, "unknown dtype");
-    const bool cplx = (dtype == RQX_C128 || dtype == RQX_C64);
-    if (!cplx && !p->is_real)
-        return fail(RQX_ERR_DTYPE, "real vector dtype requested but the Hamiltonian has complex coefficients");
-    const bool fp64 = (dtype == RQX_F64 || dtype == RQX_C128);
+    if (int rc = check_dtype(p, dtype)) return rc;
     hipStream_t st = static_cast<hipStream_t>(stream);
     const unsigned fb = static_cast<unsigned>((p->n_terms + 255) / 256);
     hipLaunchKernelGGL(k_fill<double>, dim3(fb), dim3(256), 0, st, grad, int64_t(0), p->n_terms, 0.0);
     HIP_TRY(hipGetLastError());
-    if (p->n_tiles > 0) {
-        KParams k = make_params(p, fp64);
-        k.gy = gy;
-        k.grad = grad;
-        const dim3 grid(p->n_tiles), block(TILE);
-        const size_t lds = seg_lds_bytes(p);
-        switch (dtype) {
-            case RQX_F64: hipLaunchKernelGGL((k_tiles<MODE_GRAD, double, false>), grid, block, lds, st, k, x, nullptr); break;
-            case RQX_C128: hipLaunchKernelGGL((k_tiles<MODE_GRAD, double, true>), grid, block, lds, st, k, x, nullptr); break;
-            case RQX_F32: hipLaunchKernelGGL((k_tiles<MODE_GRAD, float, false>), grid, block, lds, st, k, x, nullptr); break;
-            default: hipLaunchKernelGGL((k_tiles<MODE_GRAD, float, true>), grid, block, lds, st, k, x, nullptr); break;
-        }
-        HIP_TRY(hipGetLastError());
-    }
+    KParams k = make_params(p, is_fp64(dtype));
+    k.gy = gy;
+    k.grad = grad;
+    HIP_TRY(launch_dtype<MODE_GRAD>(p, k, x, nullptr, dtype, st));
     return RQX_OK;
 }
 
@@ -1076,14 +1382,128 @@ int rqx_plan_update_coef(rqx_plan *p, const double *coef, int32_t is_real, rqx_s
     const unsigned tb = static_cast<unsigned>((p->n_terms + 255) / 256);
     hipLaunchKernelGGL(k_narrow, dim3(tb), dim3(256), 0, st, p->coef64, p->coef32, p->n_terms);
     HIP_TRY(hipGetLastError());
-    if (p->n_states > 0) {
-        const unsigned blocks = static_cast<unsigned>((p->n_states + 255) / 256);
-        const uint32_t k1 = p->has_diag ? p->diag_terms : 0u;
-        hipLaunchKernelGGL(k_diag, dim3(blocks), dim3(256), 0, st, p->keys, p->n_states, p->z, p->coef64, 0u, k1,
-                           p->diag64, p->diag32);
+    for (const ProbHost &ph : p->probs) {
+        if (ph.n_states == 0) continue;
+        const unsigned blocks = static_cast<unsigned>((ph.n_states + 255) / 256);
+        const uint32_t k0 = static_cast<uint32_t>(ph.term_base);
+        hipLaunchKernelGGL(k_diag, dim3(blocks), dim3(256), 0, st, p->keys + ph.row_base, ph.n_states, p->z, p->coef64,
+                           k0, k0 + ph.diag_terms, p->diag64 + ph.row_base, p->diag32 + ph.row_base);
         HIP_TRY(hipGetLastError());
     }
     p->is_real = is_real ? 1 : 0;
+    return RQX_OK;
+}
+
+
+}  // extern "C"
+
+namespace {
+unsigned batch_chunks(const rqx_plan *p) {
+    int64_t mx = 0;
+    for (const ProbHost &ph : p->probs) mx = std::max(mx, ph.n_states);
+    return static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>(1024, (mx + 2047) / 2048)));
+}
+
+template <typename V, int NP>
+void dots_launch(const rqx_plan *p, const PtrPack &pk, double *out, hipStream_t st) {
+    hipLaunchKernelGGL((k_batch_dots<V, NP>), dim3(batch_chunks(p), p->probs.size()), dim3(256), 0, st, pk,
+                       p->prob_rows, out);
+}
+
+template <typename V>
+int dots_dispatch(const rqx_plan *p, int np, const PtrPack &pk, double *out, hipStream_t st) {
+    switch (np) {
+        case 1: dots_launch<V, 1>(p, pk, out, st); break;
+        case 2: dots_launch<V, 2>(p, pk, out, st); break;
+        case 3: dots_launch<V, 3>(p, pk, out, st); break;
+        case 4: dots_launch<V, 4>(p, pk, out, st); break;
+        case 5: dots_launch<V, 5>(p, pk, out, st); break;
+        case 6: dots_launch<V, 6>(p, pk, out, st); break;
+        case 7: dots_launch<V, 7>(p, pk, out, st); break;
+        case 8: dots_launch<V, 8>(p, pk, out, st); break;
+        default: dots_launch<V, 9>(p, pk, out, st); break;
+    }
+    HIP_TRY(hipGetLastError());
+    return RQX_OK;
+}
+
+template <typename V, int NV>
+void combine_launch(const rqx_plan *p, const PtrPack &pk, const double *coef, void *out, hipStream_t st) {
+    hipLaunchKernelGGL((k_batch_combine<V, NV>), dim3(batch_chunks(p), p->probs.size()), dim3(256), 0, st, pk,
+                       p->prob_rows, coef, out);
+}
+
+template <typename V>
+int combine_dispatch(const rqx_plan *p, int nv, const PtrPack &pk, const double *coef, void *out, hipStream_t st) {
+    switch (nv) {
+        case 1: combine_launch<V, 1>(p, pk, coef, out, st); break;
+        case 2: combine_launch<V, 2>(p, pk, coef, out, st); break;
+        case 3: combine_launch<V, 3>(p, pk, coef, out, st); break;
+        default: combine_launch<V, 4>(p, pk, coef, out, st); break;
+    }
+    HIP_TRY(hipGetLastError());
+    return RQX_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int rqx_batch_dots(const rqx_plan *p, int32_t n_pairs, const void *const *a, const void *const *b, double *out,
+                   int32_t dtype, rqx_stream_t stream) {
+    g_err.clear();
+    if (int rc = check_plan(p)) return rc;
+    if (n_pairs < 1 || n_pairs > MAXV) return fail(RQX_ERR_ARG, "n_pairs must be in [1, 9]");
+    if (!a || !b || !out) return fail(RQX_ERR_ARG, "NULL argument");
+    if (dtype < RQX_F64 || dtype > RQX_C64) return fail(RQX_ERR_ARG, "unknown dtype");
+    PtrPack pk{};
+    for (int q = 0; q < n_pairs; ++q) {
+        if (!a[q] || !b[q]) return fail(RQX_ERR_ARG, "NULL vector pointer");
+        pk.a[q] = a[q];
+        pk.b[q] = b[q];
+    }
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const int64_t n_out = 2 * static_cast<int64_t>(p->probs.size()) * n_pairs;
+    hipLaunchKernelGGL(k_fill<double>, dim3(static_cast<unsigned>((n_out + 255) / 256)), dim3(256), 0, st, out,
+                       int64_t(0), n_out, 0.0);
+    HIP_TRY(hipGetLastError());
+    switch (dtype) {
+        case RQX_F64: return dots_dispatch<double>(p, n_pairs, pk, out, st);
+        case RQX_C128: return dots_dispatch<double2>(p, n_pairs, pk, out, st);
+        case RQX_F32: return dots_dispatch<float>(p, n_pairs, pk, out, st);
+        default: return dots_dispatch<float2>(p, n_pairs, pk, out, st);
+    }
+}
+
+int rqx_batch_combine(const rqx_plan *p, int32_t n_in, const void *const *v, const double *coef, void *out,
+                      int32_t dtype, rqx_stream_t stream) {
+    g_err.clear();
+    if (int rc = check_plan(p)) return rc;
+    if (n_in < 1 || n_in > 4) return fail(RQX_ERR_ARG, "n_in must be in [1, 4]");
+    if (!v || !coef || !out) return fail(RQX_ERR_ARG, "NULL argument");
+    if (dtype < RQX_F64 || dtype > RQX_C64) return fail(RQX_ERR_ARG, "unknown dtype");
+    PtrPack pk{};
+    for (int q = 0; q < n_in; ++q) {
+        if (!v[q]) return fail(RQX_ERR_ARG, "NULL vector pointer");
+        pk.a[q] = v[q];
+    }
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    switch (dtype) {
+        case RQX_F64: return combine_dispatch<double>(p, n_in, pk, coef, out, st);
+        case RQX_C128: return combine_dispatch<double2>(p, n_in, pk, coef, out, st);
+        case RQX_F32: return combine_dispatch<float>(p, n_in, pk, coef, out, st);
+        default: return combine_dispatch<float2>(p, n_in, pk, coef, out, st);
+    }
+}
+
+int rqx_eigenpair_batch(const double *mats, int32_t count, int32_t n, double *out, rqx_stream_t stream) {
+    g_err.clear();
+    if (n != 2 && n != 3) return fail(RQX_ERR_ARG, "n must be 2 or 3");
+    if (count < 0) return fail(RQX_ERR_ARG, "count < 0");
+    if (count == 0) return RQX_OK;
+    if (!mats || !out) return fail(RQX_ERR_ARG, "NULL argument");
+    hipLaunchKernelGGL(k_eigen_batch, dim3(static_cast<unsigned>((count + 63) / 64)), dim3(64), 0,
+                       static_cast<hipStream_t>(stream), mats, count, n, out);
+    HIP_TRY(hipGetLastError());
     return RQX_OK;
 }
 

@@ -1,0 +1,196 @@
+r"""Batched single-vector LOCG: every problem of a ``BatchPlan`` iterates in lock step.
+
+No counterpart in the reference (``ground_locg`` solves one problem per call,
+``rqutils/ground_locg.py:128-162``).  The step is the mixed-precision LOCG of
+``locg_mixed`` with every per-problem scalar kept on the device:
+
+* one batched matvec per iteration (one launch for all problems, persistent
+  tile queue), complex64 / float32 in the bulk phase, the vector dtype in the
+  refinement phase;
+* per-problem inner products in one fused pass (``rqx_batch_dots``), vector
+  updates with per-problem coefficients (``rqx_batch_combine``), and the 3x3
+  Rayleigh-Ritz eigenpairs of all problems in one launch;
+* converged problems (relative residual <= ``tol``) are frozen: their
+  coefficients become (1, 0, 0) so their vectors stop changing;
+* no host synchronisation inside a step, so the step is captured once in a
+  HIP graph (``torch.cuda.CUDAGraph``) and replayed; convergence is polled
+  every ``check_every`` steps.
+"""
+import time
+
+import torch
+
+from rqutils_amd.plan import BatchPlan, eigenpair_batch
+
+_LO = {torch.float64: torch.float32, torch.complex128: torch.complex64}
+
+
+class BatchLOCG:
+    """Persistent buffers + one LOCG step for all problems of a ``BatchPlan``."""
+
+    def __init__(self, plan: BatchPlan, x0: torch.Tensor, tol=1e-7, switch_tol=1e-5, mixed=True):
+        if x0.dtype not in _LO:
+            raise ValueError('x0 must be float64 or complex128')
+        if x0.shape != (plan.n_total,) or x0.device != plan.device:
+            raise ValueError('x0 must be a device vector over all problems')
+        self.plan = plan
+        self.K = plan.n_problems
+        self.hi = x0.dtype
+        self.lo = _LO[x0.dtype]
+        self.tol = tol
+        self.switch_tol = switch_tol
+        dev = plan.device
+        z = lambda: torch.zeros_like(x0)  # noqa: E731
+        self.x, self.ax, self.y, self.ay, self.p, self.ap, self.r = (z() for _ in range(7))
+        self.p_lo = torch.zeros(plan.n_total, dtype=self.lo, device=dev)
+        self.ap_lo = torch.zeros_like(self.p_lo)
+        self.active = torch.ones(self.K, dtype=torch.bool, device=dev)
+        self.theta = torch.zeros(self.K, dtype=torch.float64, device=dev)
+        self.rel = torch.full((self.K,), float('inf'), dtype=torch.float64, device=dev)
+        self.one = torch.ones(self.K, dtype=torch.complex128, device=dev)
+        self.zero = torch.zeros(self.K, dtype=torch.complex128, device=dev)
+        self.phase = 'lo' if mixed else 'hi'
+        self.x.copy_(x0)
+        self._normalize(self.x, None)
+        self._apply(self.x, self.ax)
+        self._first_step()
+
+    # ------------------------------------------------------------------ helpers
+    def _apply(self, v, out):
+        if self.phase == 'lo':
+            self.p_lo.copy_(v)
+            self.plan.matvec(self.p_lo, out=self.ap_lo)
+            out.copy_(self.ap_lo)
+        else:
+            self.plan.matvec(v, out=out)
+
+    def _normalize(self, v, av):
+        n2 = self.plan.dots([(v, v)])[:, 0].real
+        inv = torch.where(n2 > 0, 1.0 / torch.sqrt(n2), torch.zeros_like(n2)).to(torch.complex128)
+        self.plan.combine([v], inv[:, None], v)
+        if av is not None:
+            self.plan.combine([av], inv[:, None], av)
+
+    def _residual(self):
+        """theta, r = Ax - theta x, relative residual, p = normalised r orthogonal to x and y."""
+        plan = self.plan
+        th = plan.dots([(self.x, self.ax)])[:, 0].real
+        self.theta.copy_(th)
+        plan.combine([self.ax, self.x], torch.stack([self.one, -th.to(torch.complex128)], 1), self.r)
+        rn = torch.sqrt(plan.dots([(self.r, self.r)])[:, 0].real)
+        self.rel.copy_(rn / th.abs().clamp_min(1e-300))
+        self.active &= self.rel > self.tol
+        self.p.copy_(self.r)
+        for _ in range(2):
+            d = plan.dots([(self.x, self.p), (self.y, self.p)])
+            plan.combine([self.p, self.x, self.y], torch.stack([self.one, -d[:, 0], -d[:, 1]], 1), self.p)
+        self._normalize(self.p, None)
+
+    def _update(self, c):
+        """x <- c0 x + c1 y + c2 p (and images); y <- normalised (c1 y + c2 p) orthogonal to x."""
+        plan = self.plan
+        keep = self.active[:, None]
+        ident = torch.stack([self.one, self.zero, self.zero], 1)
+        c = torch.where(keep, c, ident)
+        plan.combine([self.y, self.p], c[:, 1:], self.y)
+        plan.combine([self.ay, self.ap], c[:, 1:], self.ay)
+        plan.combine([self.x, self.y], torch.stack([c[:, 0], self.one], 1), self.x)
+        plan.combine([self.ax, self.ay], torch.stack([c[:, 0], self.one], 1), self.ax)
+        n2 = plan.dots([(self.x, self.x)])[:, 0].real
+        inv = (1.0 / torch.sqrt(n2)).to(torch.complex128)
+        plan.combine([self.x], inv[:, None], self.x)
+        plan.combine([self.ax], inv[:, None], self.ax)
+        pr = plan.dots([(self.x, self.y)])[:, 0]
+        plan.combine([self.y, self.x], torch.stack([self.one, -pr], 1), self.y)
+        plan.combine([self.ay, self.ax], torch.stack([self.one, -pr], 1), self.ay)
+        self._normalize(self.y, self.ay)
+
+    def _first_step(self):
+        """Rayleigh-Ritz on [x, p] (no previous direction yet)."""
+        self._residual()
+        self._apply(self.p, self.ap)
+        d = self.plan.dots([(self.x, self.ax), (self.x, self.ap), (self.p, self.ax), (self.p, self.ap)])
+        m = torch.empty((self.K, 2, 2), dtype=torch.complex128, device=self.plan.device)
+        m[:, 0, 0] = d[:, 0].real
+        m[:, 1, 1] = d[:, 3].real
+        m[:, 0, 1] = 0.5 * (d[:, 1] + d[:, 2].conj())
+        m[:, 1, 0] = m[:, 0, 1].conj()
+        _, c2 = eigenpair_batch(m)
+        if not self.x.is_complex():
+            c2 = c2.real.to(torch.complex128)
+        c = torch.stack([c2[:, 0], self.zero, c2[:, 1]], 1)
+        self._update(c)
+
+    def step(self):
+        """One LOCG iteration for every problem (no host synchronisation)."""
+        plan = self.plan
+        self._residual()
+        self._apply(self.p, self.ap)
+        d = plan.dots([(self.x, self.ax), (self.x, self.ay), (self.x, self.ap),
+                       (self.y, self.ax), (self.y, self.ay), (self.y, self.ap),
+                       (self.p, self.ax), (self.p, self.ay), (self.p, self.ap)]).reshape(self.K, 3, 3)
+        m = 0.5 * (d + d.conj().transpose(1, 2))
+        _, c = eigenpair_batch(m)
+        if not self.x.is_complex():
+            c = c.real.to(torch.complex128)
+        self._update(c)
+
+    def to_high_precision(self):
+        """Switch to the vector dtype: recompute the carried images exactly."""
+        self.phase = 'hi'
+        self._apply(self.x, self.ax)
+        self._apply(self.y, self.ay)
+
+
+def batch_ground_locg(plan: BatchPlan, x0, tol=1e-7, switch_tol=1e-5, maxiter=1000, check_every=10,
+                      use_graph=True, mixed=True, record=False):
+    """Ground eigenpairs of every problem of ``plan``.
+
+    Returns ``(theta (K,), x, iterations, info)`` where ``info`` holds per-problem
+    relative residuals, the iteration of the precision switch and timings.
+    """
+    t0 = time.perf_counter()
+    solver = BatchLOCG(plan, x0, tol=tol, switch_tol=switch_tol, mixed=mixed)
+    graphs = {}
+
+    def run_step():
+        if not use_graph:
+            solver.step()
+            return
+        g = graphs.get(solver.phase)
+        if g is None:
+            # warm up on a side stream (allocator), then capture one step
+            s = torch.cuda.Stream(device=plan.device)
+            s.wait_stream(torch.cuda.current_stream(plan.device))
+            with torch.cuda.stream(s):
+                solver.step()
+            torch.cuda.current_stream(plan.device).wait_stream(s)
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g):
+                solver.step()
+            graphs[solver.phase] = g
+            return
+        g.replay()
+
+    it = 1
+    switch_at = None
+    history = []
+    while it < maxiter:
+        for _ in range(check_every):
+            run_step()
+            it += 1
+        active = int(solver.active.sum().item())
+        worst = float(solver.rel[solver.active].max().item()) if active else 0.0
+        if record:
+            history.append((time.perf_counter() - t0, it, active, worst, solver.phase))
+        if solver.phase == 'lo' and (worst <= solver.switch_tol or active == 0 or it >= maxiter // 2):
+            solver.to_high_precision()
+            solver.active.fill_(True)
+            switch_at = it
+            continue
+        if solver.phase == 'hi' and active == 0:
+            break
+    torch.cuda.synchronize(plan.device)
+    info = {'rel_residual': solver.rel.clone(), 'switch_iteration': switch_at, 'iterations': it,
+            'seconds': time.perf_counter() - t0, 'history': history, 'graph': use_graph}
+    return solver.theta.clone(), solver.x, it, info

@@ -64,10 +64,24 @@ class Plan:
         _capi.check(rc)
         self._handle = handle
         self._states = states
+        self._load_info()
+
+    def _load_info(self):
+        lib = _capi.lib()
         info = _capi.PlanInfo()
         _capi.check(lib.rqx_plan_get_info(self._handle, ctypes.byref(info)))
         self.info = info.as_dict()
         self.is_real = bool(self.info['is_real'])
+        self.problems = []
+        for k in range(self.info['n_problems']):
+            pi = _capi.ProblemInfo()
+            _capi.check(lib.rqx_plan_get_problem(self._handle, k, ctypes.byref(pi)))
+            self.problems.append(pi.as_dict())
+
+    def set_launch(self, persistent, blocks_per_cu=0):
+        """Tile kernel launch shape: persistent work queue or one workgroup per tile."""
+        _capi.check(_capi.lib().rqx_plan_set_launch(self._handle, int(bool(persistent)), int(blocks_per_cu)))
+        self._load_info()
 
     @classmethod
     def full_space(cls, tables: TermTables, device=None):
@@ -177,8 +191,10 @@ class Plan:
     def term_phases(self):
         """``(-i)^{popcount(x_k & z_k)}`` per term (complex128 device tensor)."""
         if getattr(self, '_phases', None) is None:
-            xm = self.tables.term_xmask()
-            ph = np.bitwise_count(xm & self.tables.zmask).astype(np.int64) & 3
+            tabs = self.tables if isinstance(self.tables, (list, tuple)) else [self.tables]
+            xm = np.concatenate([t.term_xmask() for t in tabs])
+            zm = np.concatenate([t.zmask for t in tabs])
+            ph = np.bitwise_count(xm & zm).astype(np.int64) & 3
             table = np.array([1., -1j, -1., 1j])
             self._phases = torch.from_numpy(table[ph]).to(self.device)
         return self._phases
@@ -220,6 +236,118 @@ class Plan:
         if hits is None:
             hits = self.count()['hits']
         return 8 * n + 8 * n * Jp + s_v * hits + s_v * n + T * (8 + s_c) + 12 * J
+
+
+class BatchPlan(Plan):
+    """Many (Hamiltonian, subspace) problems resident together; every kernel covers all of them.
+
+    Vectors are the concatenation of the problems' rows (``row_offsets``); term
+    outputs (``grad_coef``) the concatenation of their terms.  The matvec runs
+    persistent workgroups on a tile queue ordered largest tile first.
+
+    Args:
+        problems: sequence of ``(TermTables, sorted uint64 states)``.
+    """
+
+    def __init__(self, problems, device=None, split_bits=-1):
+        _require_cuda()
+        lib = _capi.lib()
+        self.tables = [t for t, _ in problems]
+        K = len(problems)
+        if K < 1:
+            raise ValueError('a batch needs at least one problem')
+        self.device = torch.device('cuda', torch.cuda.current_device() if device is None else
+                                   torch.device(device).index or 0)
+        keep = []
+
+        def arr(a, dtype):
+            a = np.ascontiguousarray(np.asarray(a, dtype=dtype).reshape(-1))
+            keep.append(a)
+            return a.ctypes.data if a.size else None
+
+        ptrs = {k: (ctypes.c_void_p * K)() for k in ('x', 'g', 'z', 'c', 's')}
+        nq = (ctypes.c_int32 * K)()
+        ng = (ctypes.c_int32 * K)()
+        nt = (ctypes.c_int64 * K)()
+        ns = (ctypes.c_int64 * K)()
+        self._states = []
+        for k, (t, states) in enumerate(problems):
+            st = np.ascontiguousarray(np.asarray(states, dtype=np.uint64).reshape(-1))
+            self._states.append(st)
+            ptrs['x'][k] = arr(t.xmask, np.uint64)
+            ptrs['g'][k] = arr(t.grp_off, np.int64)
+            ptrs['z'][k] = arr(t.zmask, np.uint64)
+            ptrs['c'][k] = arr(np.ascontiguousarray(t.coef, dtype=np.complex128).view(np.float64), np.float64)
+            ptrs['s'][k] = st.ctypes.data if st.size else None
+            nq[k], ng[k], nt[k], ns[k] = t.num_qubits, t.num_groups, t.num_terms, st.shape[0]
+        handle = ctypes.c_void_p()
+        with torch.cuda.device(self.device):
+            stream = _capi.current_stream(self.device)
+            rc = lib.rqx_plan_create_batch(ctypes.byref(handle), K, nq, ptrs['x'], ptrs['g'], ng, ptrs['z'],
+                                           ptrs['c'], nt, ptrs['s'], ns, int(split_bits), stream)
+        _capi.check(rc)
+        self._handle = handle
+        self._load_info()
+        self.n_states = int(self.info['n_states'])
+        self.n_total = int(self.info['n_total'])
+        self.n_problems = K
+        offs = [p['row_offset'] for p in self.problems] + [self.n_total]
+        self.row_offsets = torch.tensor(offs, dtype=torch.int64, device=self.device)
+        self.row_problem = torch.repeat_interleave(
+            torch.arange(K, device=self.device), self.row_offsets[1:] - self.row_offsets[:-1])
+
+    @property
+    def states(self):
+        return np.concatenate(self._states)
+
+    def lookup(self, values=False):
+        raise NotImplementedError('lookup tables are per problem: build a Plan for one problem')
+
+    def algorithmic_bytes(self, dtype, hits=None):
+        """Sum over problems of the per-matvec byte model (see ``Plan.algorithmic_bytes``)."""
+        s_v = torch.empty(0, dtype=dtype).element_size()
+        s_c = 16 if dtype in (torch.float64, torch.complex128) else 8
+        if hits is None:
+            hits = self.count()['hits']
+        total = s_v * hits
+        for p in self.problems:
+            n, J, T = p['n_states'], p['n_groups'], p['n_terms']
+            jp = J - (1 if p['has_diagonal'] else 0)
+            total += 8 * n + 8 * n * jp + s_v * n + T * (8 + s_c) + 12 * J
+        return total
+
+    # ---------------------------------------------------------------- batched vector ops
+    def dots(self, pairs):
+        """Per-problem ``sum conj(a) * b`` for each ``(a, b)`` pair -> (K, len(pairs)) complex128."""
+        n = len(pairs)
+        a_ptrs = (ctypes.c_void_p * n)(*[a.data_ptr() for a, _ in pairs])
+        b_ptrs = (ctypes.c_void_p * n)(*[b.data_ptr() for _, b in pairs])
+        out = torch.empty((self.n_problems, n, 2), dtype=torch.float64, device=self.device)
+        dt = pairs[0][0].dtype
+        stream = _capi.current_stream(self.device)
+        _capi.check(_capi.lib().rqx_batch_dots(self._handle, n, a_ptrs, b_ptrs, out.data_ptr(),
+                                               _DTYPE_CODES[dt], stream))
+        return torch.view_as_complex(out)
+
+    def combine(self, vectors, coef, out):
+        """``out = sum_q coef[:, q] * vectors[q]`` per problem (coef: (K, n) complex128, device)."""
+        n = len(vectors)
+        v_ptrs = (ctypes.c_void_p * n)(*[v.data_ptr() for v in vectors])
+        c = torch.view_as_real(coef.to(torch.complex128).contiguous()).contiguous()
+        stream = _capi.current_stream(self.device)
+        _capi.check(_capi.lib().rqx_batch_combine(self._handle, n, v_ptrs, c.data_ptr(), out.data_ptr(),
+                                                  _DTYPE_CODES[out.dtype], stream))
+        return out
+
+
+def eigenpair_batch(mats: torch.Tensor):
+    """Lowest eigenpairs of ``(K, n, n)`` Hermitian device matrices -> (theta (K,), kappa (K, n))."""
+    K, n = mats.shape[0], mats.shape[1]
+    m = mats.to(torch.complex128).contiguous()
+    out = torch.empty((K, 2 + 2 * n), dtype=torch.float64, device=mats.device)
+    stream = _capi.current_stream(mats.device)
+    _capi.check(_capi.lib().rqx_eigenpair_batch(m.data_ptr(), K, n, out.data_ptr(), stream))
+    return out[:, 0], torch.view_as_complex(out[:, 2:].reshape(K, n, 2).contiguous())
 
 
 def sort_unique_u64(keys: torch.Tensor):

@@ -1,0 +1,150 @@
+"""Batched plans: one launch for many problems equals the per-problem kernels (GPU)."""
+import sys
+
+import numpy as np
+import pytest
+import scipy.sparse
+import scipy.sparse.linalg
+import torch
+
+from rqutils_amd import workloads
+from rqutils_amd.autotune import autotune
+from rqutils_amd.locg_batch import batch_ground_locg
+from rqutils_amd.paulis.symplectic import TermTables
+from rqutils_amd.plan import BatchPlan, Plan, eigenpair_batch
+
+pytestmark = pytest.mark.gpu
+DEV = 'cuda:0'
+
+
+def _say(msg):
+    print(f'[case] {msg}', flush=True)
+    sys.stdout.flush()
+
+
+def _random_problem(seed, n, t, ns, probs=(0.55, 0.15, 0.15, 0.15)):
+    rng = np.random.default_rng(seed)
+    codes = workloads.random_pauli_codes(n, t, probs, rng)
+    xm, zm = workloads.codes_to_masks(codes)
+    tables = TermTables.from_masks(xm, zm, rng.standard_normal(t), n)
+    states = workloads.random_subspace(n, ns, rng)
+    return tables, states
+
+
+@pytest.fixture(scope='module')
+def problems():
+    out = [_random_problem(1, 10, 60, 500), _random_problem(2, 12, 120, 2000),
+           _random_problem(3, 16, 300, 3000, probs=(0.7, 0.1, 0.1, 0.1)), _random_problem(4, 9, 20, 50)]
+    w = workloads.config1(n_alpha=30, n_beta=25)
+    out.append((w.tables, w.states))
+    return out
+
+
+def _csr(plan):
+    cols, vals = plan.lookup(values=True)
+    n = plan.n_states
+    hit = cols >= 0
+    rows = torch.nonzero(hit, as_tuple=True)[1].cpu().numpy()
+    return scipy.sparse.csr_matrix((vals[hit].cpu().numpy(), (rows, cols[hit].cpu().numpy())), shape=(n, n))
+
+
+def test_batch_matvec_equals_single_plans(problems):
+    _say('batch matvec == single')
+    bp = BatchPlan(problems)
+    assert bp.info['n_problems'] == len(problems) and bp.info['persistent'] == 1
+    rng = np.random.default_rng(7)
+    x = torch.from_numpy(rng.standard_normal(bp.n_total) + 1j * rng.standard_normal(bp.n_total)).to(DEV)
+    for dt in (torch.complex128, torch.complex64):
+        y = bp.matvec(x.to(dt))
+        for k, (t, s) in enumerate(problems):
+            a, n = bp.problems[k]['row_offset'], bp.problems[k]['n_states']
+            yk = Plan(t, s).matvec(x[a:a + n].to(dt).contiguous())
+            assert torch.equal(y[a:a + n], yk), k
+    # the same plan launched one workgroup per tile gives identical numbers
+    y_q = bp.matvec(x)
+    bp.set_launch(False)
+    assert torch.equal(bp.matvec(x), y_q)
+
+
+def test_batch_grad_equals_single(problems):
+    _say('batch grad == single')
+    bp = BatchPlan(problems)
+    rng = np.random.default_rng(8)
+    x = torch.from_numpy(rng.standard_normal(bp.n_total) + 1j * rng.standard_normal(bp.n_total)).to(DEV)
+    gy = torch.from_numpy(rng.standard_normal(bp.n_total) + 1j * rng.standard_normal(bp.n_total)).to(DEV)
+    g = bp.grad_coef(x, gy)
+    for k, (t, s) in enumerate(problems):
+        pk = bp.problems[k]
+        a, n, ta, tn = pk['row_offset'], pk['n_states'], pk['term_offset'], pk['n_terms']
+        gk = Plan(t, s).grad_coef(x[a:a + n].contiguous(), gy[a:a + n].contiguous())
+        assert torch.allclose(g[ta:ta + tn], gk, rtol=1e-12, atol=1e-12 * gk.abs().max().item())
+
+
+def test_batch_vector_kernels(problems):
+    _say('batch dots/combine/eigen')
+    bp = BatchPlan(problems)
+    rng = np.random.default_rng(9)
+    vs = [torch.from_numpy(rng.standard_normal(bp.n_total) + 1j * rng.standard_normal(bp.n_total)).to(DEV)
+          for _ in range(3)]
+    d = bp.dots([(vs[0], vs[1]), (vs[2], vs[2])])
+    coef = torch.from_numpy(rng.standard_normal((bp.n_problems, 3)) + 1j * rng.standard_normal((bp.n_problems, 3))).to(DEV)
+    out = torch.empty_like(vs[0])
+    bp.combine(vs, coef, out)
+    for k in range(bp.n_problems):
+        a, n = bp.problems[k]['row_offset'], bp.problems[k]['n_states']
+        sl = slice(a, a + n)
+        assert abs(d[k, 0] - torch.vdot(vs[0][sl], vs[1][sl])) < 1e-10 * n
+        assert abs(d[k, 1] - torch.vdot(vs[2][sl], vs[2][sl])) < 1e-10 * n
+        want = coef[k, 0] * vs[0][sl] + coef[k, 1] * vs[1][sl] + coef[k, 2] * vs[2][sl]
+        assert torch.allclose(out[sl], want, rtol=1e-13, atol=1e-12)
+    # in-place combine
+    v0 = vs[0].clone()
+    bp.combine([v0, vs[1]], coef[:, :2], v0)
+    for k in range(bp.n_problems):
+        a, n = bp.problems[k]['row_offset'], bp.problems[k]['n_states']
+        sl = slice(a, a + n)
+        assert torch.allclose(v0[sl], coef[k, 0] * vs[0][sl] + coef[k, 1] * vs[1][sl], rtol=1e-13, atol=1e-12)
+    m = torch.from_numpy(rng.standard_normal((50, 3, 3)) + 1j * rng.standard_normal((50, 3, 3)))
+    m = (m + m.conj().transpose(1, 2)) / 2
+    th, c = eigenpair_batch(m.to(DEV))
+    w = np.linalg.eigvalsh(m.numpy())[:, 0]
+    assert np.allclose(th.cpu().numpy(), w, atol=1e-10)
+    res = torch.linalg.vector_norm(m.to(DEV) @ c[:, :, None] - th[:, None, None] * c[:, :, None], dim=(1, 2))
+    assert float(res.max()) < 1e-8
+
+
+@pytest.mark.parametrize('use_graph', [False, True])
+def test_batch_locg_matches_eigsh(problems, use_graph):
+    _say(f'batch locg graph={use_graph}')
+    bp = BatchPlan(problems)
+    rng = np.random.default_rng(10)
+    x0 = torch.from_numpy(rng.standard_normal(bp.n_total) + 1j * rng.standard_normal(bp.n_total)).to(DEV)
+    theta, x, it, info = batch_ground_locg(bp, x0, tol=1e-7, maxiter=3000, check_every=10, use_graph=use_graph)
+    for k, (t, s) in enumerate(problems):
+        h = _csr(Plan(t, s))
+        e = scipy.sparse.linalg.eigsh(h, k=1, which='SA', tol=1e-12)[0][0] if h.shape[0] > 2 else \
+            np.linalg.eigvalsh(h.toarray())[0]
+        assert abs(float(theta[k]) - e) < 1e-8 * max(1., abs(e)), (k, float(theta[k]), e)
+    assert info['switch_iteration'] is not None
+
+
+def test_batch_graph_replays_match_eager(problems):
+    """The captured step replayed N times gives the eager trajectory."""
+    _say('graph vs eager')
+    bp = BatchPlan(problems[:3])
+    rng = np.random.default_rng(11)
+    x0 = torch.from_numpy(rng.standard_normal(bp.n_total) + 1j * rng.standard_normal(bp.n_total)).to(DEV)
+    th_e, _, _, _ = batch_ground_locg(bp, x0, tol=0.0, maxiter=41, check_every=20, use_graph=False, mixed=False)
+    th_g, _, _, _ = batch_ground_locg(bp, x0, tol=0.0, maxiter=41, check_every=20, use_graph=True, mixed=False)
+    assert torch.allclose(th_e, th_g, rtol=1e-10, atol=1e-12)
+
+
+def test_autotune_picks_a_valid_plan():
+    _say('autotune')
+    w = workloads.config1(n_alpha=200, n_beta=200)
+    plan, results = autotune(w.tables, w.states, reps=2)
+    assert results and results[0]['ms'] <= results[-1]['ms']
+    assert plan.info['split_bits'] == results[0]['split_bits']
+    ref = Plan(w.tables, w.states)
+    x = torch.randn(plan.n_total, dtype=torch.float64, device=DEV)
+    assert torch.allclose(plan.matvec(x), ref.matvec(x), rtol=0, atol=1e-12)
