@@ -217,7 +217,7 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
     const V *x = static_cast<const V *>(xin);
 
     V acc = vzero(static_cast<V *>(nullptr));
-    unsigned long long n_hit = 0, n_probe = 0;
+    unsigned long long n_hit = 0, n_probe = 0, n_look = 0;
     V gv = vzero(static_cast<V *>(nullptr));
     if constexpr (MODE == MODE_GRAD) {
         if (active) gv = static_cast<const V *>(p.gy)[i];
@@ -246,6 +246,7 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
         if (c < n_classes) {
             t = find_u64(p.seg_hi + pd.seg_base, static_cast<int>(pd.n_segs), shi ^ p.cls_hi[pd.cls_base + c]);
             if (t >= 0) t += static_cast<int>(pd.seg_base);
+            if constexpr (MODE == MODE_COUNT) n_look += 1;
         }
         __syncthreads();              // readers of the previous chunk are done
         s_tseg[tid] = t;
@@ -318,16 +319,18 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
     if constexpr (MODE == MODE_MATVEC) {
         if (active) static_cast<V *>(yout)[i] = acc;
     } else if constexpr (MODE == MODE_COUNT) {
-        // wave reduction, one atomic per wave
+        // wave reduction, one atomic per wave and counter (the class lookups are
+        // counted per thread like the rest: no block-leader branch at the tail)
         for (int off = 32; off > 0; off >>= 1) {
             n_hit += __shfl_down(n_hit, off, 64);
             n_probe += __shfl_down(n_probe, off, 64);
+            n_look += __shfl_down(n_look, off, 64);
         }
         if ((tid & 63) == 0) {
             atomicAdd(&p.counts[0], n_hit);
             atomicAdd(&p.counts[1], n_probe);
+            atomicAdd(&p.counts[2], n_look);
         }
-        if (tid == 0) atomicAdd(&p.counts[2], static_cast<unsigned long long>(pd.n_classes));
     }
 }
 
@@ -348,13 +351,19 @@ __global__ __launch_bounds__(TILE) void k_tiles_queue(const KParams p, const voi
     extern __shared__ uint32_t s_keys[];
     __shared__ int32_t s_tseg[TILE];
     __shared__ uint32_t s_tile;
-    for (;;) {
+    // a workgroup can claim at most n_tiles + 1 times (the last claim is past the end)
+    for (uint32_t claim = 0; claim <= p.n_tiles; ++claim) {
         if (threadIdx.x == 0) s_tile = atomicAdd(p.queue, 1u);
         __syncthreads();
-        const uint32_t tile = s_tile;
+        // readfirstlane makes the claimed tile a scalar: the loop exit below is then
+        // a uniform branch, so the compiler keeps every lane of every wave on the
+        // same path back to the claim (a divergent-looking exit let it park lane 0
+        // outside the loop while the other lanes re-read the previous claim)
+        const uint32_t tile = __builtin_amdgcn_readfirstlane(s_tile);
         __syncthreads();              // every thread has read s_tile before the next claim
         if (tile >= p.n_tiles) break;
         process_tile<MODE, R, CPLX>(p, tile, s_keys, s_tseg, xin, yout);
+        __syncthreads();              // the whole workgroup leaves the tile before the next claim
     }
 }
 

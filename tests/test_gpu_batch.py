@@ -66,6 +66,24 @@ def test_batch_matvec_equals_single_plans(problems):
     assert torch.equal(bp.matvec(x), y_q)
 
 
+def test_persistent_workgroups_take_several_tiles(problems):
+    """More tiles than persistent workgroups: every mode equals the one-tile-per-workgroup launch."""
+    _say('persistent multi-tile')
+    w = workloads.config1(n_alpha=300, n_beta=300)
+    bp = BatchPlan(list(problems) + [(w.tables, w.states)])
+    bp.set_launch(True, 1)                       # grid = number of CUs
+    assert bp.info['n_tiles'] > 2 * bp.info['n_cu']
+    rng = np.random.default_rng(12)
+    x = torch.from_numpy(rng.standard_normal(bp.n_total) + 1j * rng.standard_normal(bp.n_total)).to(DEV)
+    gy = torch.from_numpy(rng.standard_normal(bp.n_total) + 1j * rng.standard_normal(bp.n_total)).to(DEV)
+    c_q, y_q, g_q = bp.count(), bp.matvec(x), bp.grad_coef(x, gy)
+    bp.set_launch(False)
+    c_t, y_t, g_t = bp.count(), bp.matvec(x), bp.grad_coef(x, gy)
+    assert c_q == c_t
+    assert torch.equal(y_q, y_t)
+    assert torch.allclose(g_q, g_t, rtol=1e-12, atol=1e-12 * g_t.abs().max().item())
+
+
 def test_batch_grad_equals_single(problems):
     _say('batch grad == single')
     bp = BatchPlan(problems)
