@@ -121,6 +121,112 @@ def locg_time_to_solution(plan, complex_, maxiter, mixed_tol, energy_tol, target
     }
 
 
+def _event_ms(fn, reps):
+    import torch
+    fn()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(torch.cuda.current_stream())
+    for _ in range(reps):
+        fn()
+    e1.record(torch.cuda.current_stream())
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def extra_large_matvec(reps=3):
+    """25M-determinant JW matvec (40 spin-orbitals, 5000 x 5000), complex64."""
+    import numpy as np
+    import torch
+    from rqutils_amd import workloads
+    from rqutils_amd.plan import Plan
+    w = workloads.config4()
+    t0 = time.perf_counter()
+    plan = Plan(w.tables, w.states)
+    torch.cuda.synchronize()
+    setup = time.perf_counter() - t0
+    hits = plan.count()['hits']
+    alg = plan.algorithmic_bytes(torch.complex64, hits=hits)
+    rng = np.random.default_rng(4002)
+    x = torch.from_numpy(rng.standard_normal(plan.n_total) + 1j * rng.standard_normal(plan.n_total)).to(
+        plan.device, torch.complex64)
+    y = torch.empty_like(x)
+    ms = _event_ms(lambda: plan.matvec(x, out=y), reps)
+    log(f'25M matvec: {ms:.1f} ms, {alg / ms / 1e6:.0f} GB/s algorithmic')
+    out = {'workload': w.name, 'n_states': plan.n_states, 'n_groups': plan.info['n_groups'],
+           'n_terms': plan.info['n_terms'], 'hits_per_matvec': hits, 'split_bits': plan.info['split_bits'],
+           'dtype': 'c64', 'ms_per_matvec': round(ms, 3), 'algorithmic_bytes_per_matvec': alg,
+           'achieved_GBs': round(alg / ms / 1e6, 1), 'frac_of_8TBs': round(alg / ms / 1e6 / HBM_PEAK_GBS, 4),
+           'setup_s': round(setup, 2)}
+    del plan
+    torch.cuda.empty_cache()
+    return out
+
+
+def extra_batch(maxiter=1000):
+    """64 independent 40-qubit random Pauli sums in one batched plan: matvec + batched LOCG."""
+    import numpy as np
+    import torch
+    from rqutils_amd import workloads
+    from rqutils_amd.plan import BatchPlan
+    from rqutils_amd.locg_batch import batch_ground_locg
+    probs = [workloads.config2_problem(i) for i in range(64)]
+    t0 = time.perf_counter()
+    bp = BatchPlan([(p.tables, p.states) for p in probs])
+    torch.cuda.synchronize()
+    setup = time.perf_counter() - t0
+    rng = np.random.default_rng(2002)
+    x = torch.from_numpy(rng.standard_normal(bp.n_total) + 1j * rng.standard_normal(bp.n_total)).to(bp.device)
+    x64 = x.to(torch.complex64)
+    y = torch.empty_like(x64)
+    ms_q = _event_ms(lambda: bp.matvec(x64, out=y), 5)
+    bp.set_launch(False)
+    ms_t = _event_ms(lambda: bp.matvec(x64, out=y), 5)
+    bp.set_launch(True)
+    theta, _, it, info = batch_ground_locg(bp, x, tol=1e-7, maxiter=maxiter, check_every=10, use_graph=True)
+    rel = info['rel_residual']
+    log(f'batch: matvec {ms_q:.1f} ms (queue) / {ms_t:.1f} ms (per tile); LOCG {info["seconds"]:.2f} s, '
+        f'{int((rel <= 1e-7).sum())}/64 converged')
+    out = {'problems': 64, 'sum_states': bp.n_states, 'sum_terms': bp.info['n_terms'], 'tiles': bp.info['n_tiles'],
+           'matvec_ms_work_queue': round(ms_q, 3), 'matvec_ms_one_block_per_tile': round(ms_t, 3),
+           'locg_seconds': round(info['seconds'], 3), 'locg_iterations': it,
+           'converged_rel_residual_1e-7': int((rel <= 1e-7).sum()), 'worst_rel_residual': float(rel.max()),
+           'precision_switch_iteration': info['switch_iteration'], 'hip_graph': True,
+           'setup_s': round(setup, 2), 'dtype': 'c64 matvec, c128 vectors/RR'}
+    del bp
+    torch.cuda.empty_cache()
+    return out
+
+
+def extra_grad(reps=5):
+    """36-qubit, 10k-term random Pauli sum on 1M states: forward, backward and Hellmann-Feynman."""
+    import numpy as np
+    import torch
+    from rqutils_amd import workloads
+    from rqutils_amd.plan import Plan
+    w = workloads.config3()
+    plan = Plan(w.tables, w.states)
+    rng = np.random.default_rng(3003)
+    n = plan.n_total
+    x = torch.from_numpy(rng.standard_normal(n) + 1j * rng.standard_normal(n)).to(plan.device)
+    gy = torch.from_numpy(rng.standard_normal(n) + 1j * rng.standard_normal(n)).to(plan.device)
+    out = {'workload': w.name, 'n_terms': plan.info['n_terms'], 'dtype': 'c128'}
+    out['matvec_ms'] = round(_event_ms(lambda: plan.matvec(x), reps), 3)
+    out['grad_x_ms'] = out['matvec_ms']
+    out['grad_coef_ms'] = round(_event_ms(lambda: plan.grad_coef(x, gy), reps), 3)
+    out['hellmann_feynman_ms'] = round(_event_ms(lambda: plan.expect_terms(x), reps), 3)
+    # identity check at full size: sum_k alpha_k g_k = Re <gy, H x>
+    alpha = torch.from_numpy(w.tables.alphas().real.copy()).to(plan.device)
+    g = plan.grad_coef(x, gy)
+    lhs, rhs = torch.dot(alpha, g).item(), torch.vdot(gy, plan.matvec(x)).real.item()
+    out['identity_rel_err'] = abs(lhs - rhs) / max(abs(rhs), 1e-300)
+    log(f'grad: matvec {out["matvec_ms"]} ms, grad_coef {out["grad_coef_ms"]} ms')
+    del plan
+    torch.cuda.empty_cache()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -136,6 +242,8 @@ def main():
     ap.add_argument('--locg-maxiter', type=int, default=1000)
     ap.add_argument('--mixed-tol', type=float, default=1e-6)
     ap.add_argument('--energy-tol', type=float, default=3e-10)
+    ap.add_argument('--no-extras', action='store_true',
+                    help='skip the 25M matvec, the 64-problem batch and the gradient timings')
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -229,6 +337,11 @@ def main():
     if not args.no_locg:
         result['locg'] = locg_time_to_solution(plan, not args.locg_real, args.locg_maxiter, args.mixed_tol,
                                                args.energy_tol)
+    if not args.no_extras:
+        del y, x
+        torch.cuda.empty_cache()
+        result['extras'] = {'jw25m_matvec': extra_large_matvec(), 'batch64_locg': extra_batch(),
+                            'grad36q': extra_grad()}
     if not args.no_cpu:
         log(f'cpu baseline (budget {args.cpu_budget} s) ...')
         result['cpu_baseline'] = cpu_baseline(w, alg, args.cpu_budget)
