@@ -135,6 +135,60 @@ def _event_ms(fn, reps):
     return e0.elapsed_time(e1) / reps
 
 
+def cache_leg(plan, dtype, reps=10):
+    """Build the hit cache for ``dtype``'s precision and time the cached matvec against its bytes."""
+    import numpy as np
+    import torch
+    precision = 64 if dtype in (torch.float64, torch.complex128) else 32
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    info = plan.build_cache(precision)
+    torch.cuda.synchronize()
+    t_build = time.perf_counter() - t0
+    rng = np.random.default_rng(5005)
+    x = torch.from_numpy(rng.standard_normal(plan.n_total) + 1j * rng.standard_normal(plan.n_total)).to(plan.device)
+    x = (x if dtype.is_complex else x.real).to(dtype).contiguous()
+    y = torch.empty_like(x)
+    ms = _event_ms(lambda: plan.matvec(x, out=y), reps)
+    plan.use_cache(False)
+    y_free = plan.matvec(x)
+    plan.use_cache(True)
+    identical = bool(torch.equal(plan.matvec(x), y_free))
+    nbytes = plan.cached_bytes(dtype)
+    gbs = nbytes / ms / 1e6
+    name = {torch.complex64: 'c64', torch.complex128: 'c128', torch.float32: 'f32', torch.float64: 'f64'}[dtype]
+    log(f'cached {name} matvec: {ms:.3f} ms, cache {info["bytes"] / 1e9:.2f} GB built in {t_build:.2f} s, '
+        f'{gbs:.0f} GB/s of cache+vector bytes, identical to matrix-free: {identical}')
+    return {'dtype': name, 'ms_per_matvec': round(ms, 4), 'cache_bytes': info['bytes'], 'stored_hits': info['nnz'],
+            'entries_incl_padding': info['entries'], 'build_s': round(t_build, 3), 'bytes_per_matvec': nbytes,
+            'achieved_GBs': round(gbs, 1), 'frac_of_8TBs': round(gbs / HBM_PEAK_GBS, 4),
+            'identical_to_matrix_free': identical}
+
+
+def locg_cached(plan, complex_, maxiter, mixed_tol, energy_tol, e_star):
+    """Mixed LOCG with both matvecs read from hit caches (cache build included in the time)."""
+    import torch
+    from rqutils_amd.locg_mixed import ground_locg_mixed
+    hi = torch.complex128 if complex_ else torch.float64
+    diag = plan.diagonal().real[:plan.n_states]
+    x0 = torch.zeros(plan.n_total, dtype=hi, device=plan.device)
+    x0[int(torch.argmin(diag))] = 1
+    plan.drop_cache()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    plan.build_cache(32)
+    plan.build_cache(64)
+    torch.cuda.synchronize()
+    t_build = time.perf_counter() - t0
+    res = ground_locg_mixed(plan.matvec, plan.matvec, x0, maxiter=maxiter, tol=mixed_tol, energy_tol=energy_tol)
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    err = abs(float(res.theta) - e_star) / abs(e_star)
+    log(f'cached mixed LOCG: rel err {err:.2e} in {t:.2f} s incl. {t_build:.2f} s cache build ({res.niter} iterations)')
+    return {'time_s': round(t, 4), 'cache_build_s': round(t_build, 4), 'iterations': res.niter,
+            'matvecs_low': res.matvecs_lo, 'matvecs_fp64': res.matvecs_hi, 'rel_err_vs_plain': err}
+
+
 def extra_large_matvec(reps=3):
     """25M-determinant JW matvec (40 spin-orbitals, 5000 x 5000), complex64."""
     import numpy as np
@@ -154,11 +208,16 @@ def extra_large_matvec(reps=3):
     y = torch.empty_like(x)
     ms = _event_ms(lambda: plan.matvec(x, out=y), reps)
     log(f'25M matvec: {ms:.1f} ms, {alg / ms / 1e6:.0f} GB/s algorithmic')
+    del x, y
+    try:
+        cached = cache_leg(plan, torch.complex64, reps=5)
+    except Exception as exc:  # the cache may not fit next to other allocations
+        cached = {'error': str(exc)}
     out = {'workload': w.name, 'n_states': plan.n_states, 'n_groups': plan.info['n_groups'],
            'n_terms': plan.info['n_terms'], 'hits_per_matvec': hits, 'split_bits': plan.info['split_bits'],
            'dtype': 'c64', 'ms_per_matvec': round(ms, 3), 'algorithmic_bytes_per_matvec': alg,
            'achieved_GBs': round(alg / ms / 1e6, 1), 'frac_of_8TBs': round(alg / ms / 1e6 / HBM_PEAK_GBS, 4),
-           'setup_s': round(setup, 2)}
+           'setup_s': round(setup, 2), 'cached': cached}
     del plan
     torch.cuda.empty_cache()
     return out
@@ -242,6 +301,7 @@ def main():
     ap.add_argument('--locg-maxiter', type=int, default=1000)
     ap.add_argument('--mixed-tol', type=float, default=1e-6)
     ap.add_argument('--energy-tol', type=float, default=3e-10)
+    ap.add_argument('--no-cache', action='store_true', help='skip the hit-cache matvec and LOCG legs')
     ap.add_argument('--no-extras', action='store_true',
                     help='skip the 25M matvec, the 64-problem batch and the gradient timings')
     args = ap.parse_args()
@@ -337,6 +397,15 @@ def main():
     if not args.no_locg:
         result['locg'] = locg_time_to_solution(plan, not args.locg_real, args.locg_maxiter, args.mixed_tol,
                                                args.energy_tol)
+    if not args.no_cache:
+        result['cache'] = {'matvec': cache_leg(plan, dt)}
+        if not args.no_locg:
+            loc = locg_cached(plan, not args.locg_real, args.locg_maxiter, args.mixed_tol, args.energy_tol,
+                              result['locg']['e_star'])
+            loc['speedup_vs_plain_time_to_target'] = round(result['locg']['plain']['time_to_target_s'] /
+                                                           loc['time_s'], 3)
+            result['cache']['locg_mixed'] = loc
+        plan.drop_cache()
     if not args.no_extras:
         del y, x
         torch.cuda.empty_cache()

@@ -194,6 +194,37 @@ int rqx_expect_terms(const rqx_plan *plan, const void *psi, double *out, int32_t
 int rqx_plan_update_coef(rqx_plan *plan, const double *coef, int32_t is_real, rqx_stream_t stream);
 
 /*
+ * Hit cache: the reference's cache_strategy (1, 2) (rqutils/sqd.py:343-407: source
+ * indices per X group from get_xsource, coefficient sums per group from
+ * get_diagonal, applied by apply_h_xz_cached, sqd.py:731-743), compacted to the
+ * hits only.  Stored as sliced ELL: 64 rows per slice, entry k of row r at
+ * off[r / 64] + 64 k + r % 64, each slice padded to its widest row.  Entries are
+ * written in the order the matrix-free kernel accumulates them and with the same
+ * values, so rqx_matvec gives identical results with or without the cache.
+ *
+ * rqx_plan_build_cache: precision 32 (used by RQX_F32 / RQX_C64 matvecs) or 64
+ * (RQX_F64 / RQX_C128); values are real when the Hamiltonian is.  Allocates
+ * (device memory; synchronous); RQX_ERR_LIMIT if it would need more than
+ * max_bytes (max_bytes <= 0: 90% of the free device memory).  Rebuilding
+ * replaces a cache of the same precision.  rqx_plan_update_coef rewrites the
+ * values on its stream, or releases the cache if the coefficients change
+ * between real and complex.
+ */
+int rqx_plan_build_cache(rqx_plan *plan, int32_t precision, int64_t max_bytes, rqx_stream_t stream);
+
+/* Release the cache of one precision (32 or 64), or both (0).  Synchronises the device. */
+int rqx_plan_drop_cache(rqx_plan *plan, int32_t precision);
+
+/*
+ * out[0..6) = {built, stored hits (nnz), entries incl. padding, device bytes,
+ * slices, complex values} for the cache of one precision.
+ */
+int rqx_plan_cache_info(const rqx_plan *plan, int32_t precision, int64_t *out);
+
+/* Whether rqx_matvec uses a built cache (default 1); 0 forces the matrix-free kernel. */
+int rqx_plan_use_cache(rqx_plan *plan, int32_t enable);
+
+/*
  * Lowest eigenpair of a small Hermitian matrix (n = 2 or 3), row-major,
  * complex interleaved, device pointers.  out = [theta, 0, k0re, k0im, ...].
  * Replaces eigenpair_2x2 / eigenpair_3x3 (rqutils/ground_locg.py:467-514),

@@ -28,7 +28,8 @@ EXPORTS = (
     'rqx_lookup', 'rqx_diagonal', 'rqx_matvec', 'rqx_count', 'rqx_eigenpair_small',
     'rqx_eigenpair_small_host', 'rqx_sort_unique_u64', 'rqx_grad_coef', 'rqx_expect_terms',
     'rqx_plan_update_coef', 'rqx_plan_create_batch', 'rqx_plan_get_problem', 'rqx_plan_set_launch',
-    'rqx_eigenpair_batch', 'rqx_batch_dots', 'rqx_batch_combine',
+    'rqx_eigenpair_batch', 'rqx_batch_dots', 'rqx_batch_combine', 'rqx_plan_build_cache',
+    'rqx_plan_drop_cache', 'rqx_plan_cache_info', 'rqx_plan_use_cache',
 )
 
 
@@ -119,6 +120,10 @@ _SIGS = {
                                      _i32, _vp]),
     'rqx_plan_get_problem': (_i32, [_vp, _i32, ctypes.POINTER(ProblemInfo)]),
     'rqx_plan_set_launch': (_i32, [_vp, _i32, _i32]),
+    'rqx_plan_build_cache': (_i32, [_vp, _i32, _i64, _vp]),
+    'rqx_plan_drop_cache': (_i32, [_vp, _i32]),
+    'rqx_plan_cache_info': (_i32, [_vp, _i32, _vp]),
+    'rqx_plan_use_cache': (_i32, [_vp, _i32]),
     'rqx_eigenpair_batch': (_i32, [_vp, _i32, _i32, _vp, _vp]),
     'rqx_batch_dots': (_i32, [_vp, _i32, _vp, _vp, _vp, _i32, _vp]),
     'rqx_batch_combine': (_i32, [_vp, _i32, _vp, _vp, _vp, _i32, _vp]),

@@ -51,7 +51,11 @@ int fail(int code, const std::string &msg) {
             return fail(RQX_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
-enum { MODE_MATVEC = 0, MODE_LOOKUP = 1, MODE_COUNT = 2, MODE_GRAD = 3 };
+enum { MODE_MATVEC = 0, MODE_LOOKUP = 1, MODE_COUNT = 2, MODE_GRAD = 3, MODE_ROWCOUNT = 4, MODE_FILL = 5 };
+
+// hit cache: sliced ELL, 64 rows per slice (one wave), entry k of row r at
+// off[r / 64] + k * 64 + r % 64, padded to the slice's widest row with (r, 0)
+constexpr int SLICE = 64;
 
 // One problem of a plan (a batched plan holds several).  Row, segment, class,
 // group and term indices in the plan arrays are absolute; the descriptor gives
@@ -88,6 +92,11 @@ struct KParams {
     const uint64_t *g_x;           // full X mask per group
     const void *gy;                // gradient mode: upstream gradient (same dtype as x)
     double *grad;                  // gradient mode: per-term output (accumulated)
+    uint32_t *rowcnt;              // row-count mode: non-diagonal hits per row
+    int32_t *c_col;                // fill mode: cache column indices
+    void *c_val;                   // fill mode: cache values (R, or complex R)
+    const int64_t *c_off;          // fill mode: first entry of each slice
+    const uint32_t *c_width;       // fill mode: entries per row of each slice
     int64_t n_total;
     uint32_t n_tiles;
 };
@@ -234,7 +243,7 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
         } else if constexpr (MODE == MODE_LOOKUP) {
             p.idx[static_cast<size_t>(pd.grp_base) * p.n_total + i] = static_cast<int32_t>(i);
             if (p.vals) p.vals[static_cast<size_t>(pd.grp_base) * p.n_total + i] = static_cast<const double2 *>(p.diag)[i];
-        } else {
+        } else if constexpr (MODE == MODE_COUNT) {
             n_hit += 1;
         }
     }
@@ -277,6 +286,17 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
                     R wre = R(0), wim = R(0);
                     if (hit) conj_mul(gv, x[ta + base], wre, wim);
                     grad_group<R>(p, g, s, hit, wre, wim);
+                } else if constexpr (MODE == MODE_ROWCOUNT) {
+                    n_hit += hit;
+                } else if constexpr (MODE == MODE_FILL) {
+                    if (hit) {
+                        const int64_t slot = p.c_off[i / SLICE] + static_cast<int64_t>(n_hit) * SLICE + (i % SLICE);
+                        p.c_col[slot] = static_cast<int32_t>(ta + base);
+                        const CC cg = group_coef<R>(p, g, s);
+                        if constexpr (CPLX) static_cast<CC *>(p.c_val)[slot] = cg;
+                        else static_cast<R *>(p.c_val)[slot] = cg.x;
+                        n_hit += 1;
+                    }
                 } else {
                     n_hit += hit;
                     n_probe += active;
@@ -318,6 +338,18 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
 
     if constexpr (MODE == MODE_MATVEC) {
         if (active) static_cast<V *>(yout)[i] = acc;
+    } else if constexpr (MODE == MODE_ROWCOUNT) {
+        if (active) p.rowcnt[i] = static_cast<uint32_t>(n_hit);
+    } else if constexpr (MODE == MODE_FILL) {
+        if (active) {
+            const uint32_t w = p.c_width[i / SLICE];
+            for (uint32_t k = static_cast<uint32_t>(n_hit); k < w; ++k) {
+                const int64_t slot = p.c_off[i / SLICE] + static_cast<int64_t>(k) * SLICE + (i % SLICE);
+                p.c_col[slot] = static_cast<int32_t>(i);
+                if constexpr (CPLX) static_cast<CC *>(p.c_val)[slot] = CC{R(0), R(0)};
+                else static_cast<R *>(p.c_val)[slot] = R(0);
+            }
+        }
     } else if constexpr (MODE == MODE_COUNT) {
         // wave reduction, one atomic per wave and counter (the class lookups are
         // counted per thread like the rest: no block-leader branch at the tail)
@@ -662,6 +694,43 @@ __global__ __launch_bounds__(256) void k_batch_combine(PtrPack pk, const int64_t
     }
 }
 
+// real coefficient (hit cache of a real Hamiltonian): acc += c * v
+__device__ inline void fma_acc(double &acc, double c, double v) { acc = fma(c, v, acc); }
+__device__ inline void fma_acc(float &acc, float c, float v) { acc = fmaf(c, v, acc); }
+__device__ inline void fma_acc(double2 &acc, double c, double2 v) {
+    acc.x = fma(c, v.x, acc.x);
+    acc.y = fma(c, v.y, acc.y);
+}
+__device__ inline void fma_acc(float2 &acc, float c, float2 v) {
+    acc.x = fmaf(c, v.x, acc.x);
+    acc.y = fmaf(c, v.y, acc.y);
+}
+
+// y = H x from the hit cache: one wave per 64-row slice, one row per lane,
+// entries read lane-contiguously; the same accumulation sequence as k_tiles
+// (diagonal first, then hits in group order), so results are identical.
+template <typename R, bool CPLXV, bool CPLXC>
+__global__ __launch_bounds__(256) void k_spmv_sell(const int32_t *__restrict__ col, const void *__restrict__ val,
+                                                   const int64_t *__restrict__ off, const uint32_t *__restrict__ width,
+                                                   const void *__restrict__ diag, int64_t n_states,
+                                                   const void *__restrict__ xin, void *__restrict__ yout) {
+    using V = typename VT<R, CPLXV>::T;
+    using CC = typename C2<R>::T;
+    using CV = typename std::conditional<CPLXC, CC, R>::type;
+    const int64_t row = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (row >= n_states) return;
+    const int64_t slice = row / SLICE;
+    const int lane = static_cast<int>(row % SLICE);
+    const V *x = static_cast<const V *>(xin);
+    const CV *v = static_cast<const CV *>(val);
+    V acc = vzero(static_cast<V *>(nullptr));
+    if (diag) fma_acc(acc, static_cast<const CC *>(diag)[row], x[row]);   // NULL: no problem has an x = 0 group
+    const uint32_t w = width[slice];
+    int64_t e = off[slice] + lane;
+    for (uint32_t k = 0; k < w; ++k, e += SLICE) fma_acc(acc, v[e], x[col[e]]);
+    static_cast<V *>(yout)[row] = acc;
+}
+
 __global__ void k_eigen_batch(const double *__restrict__ mats, int count, int n, double *__restrict__ out) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k < count) eigen_small(mats + static_cast<int64_t>(k) * 2 * n * n, n, out + static_cast<int64_t>(k) * (2 + 2 * n));
@@ -773,7 +842,20 @@ struct ProbHost {            // host record of one problem of a plan
     double cost = 0.0;         // cost-model estimate of one matvec
 };
 
+struct HitCache {            // sliced-ELL cache of the hits (column, C_j(s_i)) in one precision
+    void *block = nullptr;
+    int32_t *col = nullptr;
+    void *val = nullptr;
+    int64_t *off = nullptr;
+    uint32_t *width = nullptr;
+    int64_t entries = 0, nnz = 0, n_slices = 0, bytes = 0;
+    int cplx = 0;
+    bool built = false;
+};
+
 struct rqx_plan {
+    HitCache cache[2];         // [0]: FP32 values, [1]: FP64 values
+    int use_cache = 1;
     int device = 0;
     int is_real = 1;
     int max_seg = 0;
@@ -1170,6 +1252,44 @@ int check_dtype(const rqx_plan *p, int32_t dtype) {
 
 bool is_fp64(int32_t dtype) { return dtype == RQX_F64 || dtype == RQX_C128; }
 
+int cache_slot(int32_t precision) { return precision == 64 ? 1 : precision == 32 ? 0 : -1; }
+
+size_t cache_val_bytes(int slot, int cplx) { return (slot == 1 ? 8u : 4u) * (cplx ? 2u : 1u); }
+
+hipError_t free_cache(HitCache &c) {
+    hipError_t e = hipSuccess;
+    if (c.block) e = hipFree(c.block);
+    c = HitCache{};
+    return e;
+}
+
+// (re)write columns and values of a built cache from the current coefficients
+hipError_t fill_cache(const rqx_plan *p, int slot, hipStream_t st) {
+    const HitCache &c = p->cache[slot];
+    KParams k = make_params(p, slot == 1);
+    k.c_col = c.col;
+    k.c_val = c.val;
+    k.c_off = c.off;
+    k.c_width = c.width;
+    if (slot == 1)
+        return c.cplx ? launch_tiles<MODE_FILL, double, true>(p, k, nullptr, nullptr, st)
+                      : launch_tiles<MODE_FILL, double, false>(p, k, nullptr, nullptr, st);
+    return c.cplx ? launch_tiles<MODE_FILL, float, true>(p, k, nullptr, nullptr, st)
+                  : launch_tiles<MODE_FILL, float, false>(p, k, nullptr, nullptr, st);
+}
+
+template <typename R, bool CPLXV>
+hipError_t spmv_launch(const rqx_plan *p, const HitCache &c, const void *diag, const void *x, void *y, hipStream_t st) {
+    const unsigned blocks = static_cast<unsigned>((p->n_states + 255) / 256);
+    if (c.cplx)
+        hipLaunchKernelGGL((k_spmv_sell<R, CPLXV, true>), dim3(blocks), dim3(256), 0, st, c.col, c.val, c.off, c.width,
+                           diag, p->n_states, x, y);
+    else
+        hipLaunchKernelGGL((k_spmv_sell<R, CPLXV, false>), dim3(blocks), dim3(256), 0, st, c.col, c.val, c.off,
+                           c.width, diag, p->n_states, x, y);
+    return hipGetLastError();
+}
+
 }  // namespace
 
 extern "C" {
@@ -1214,7 +1334,11 @@ int rqx_plan_destroy(rqx_plan *plan) {
     int prev = 0;
     HIP_TRY(hipGetDevice(&prev));
     HIP_TRY(hipSetDevice(plan->device));
-    const hipError_t e = hipFree(plan->block);
+    hipError_t e = hipFree(plan->block);
+    for (HitCache &c : plan->cache) {
+        const hipError_t ec = free_cache(c);
+        if (e == hipSuccess) e = ec;
+    }
     (void)hipSetDevice(prev);
     delete plan;
     if (e != hipSuccess) return fail(RQX_ERR_HIP, std::string("hipFree(plan): ") + hipGetErrorString(e));
@@ -1329,8 +1453,24 @@ int rqx_matvec(const rqx_plan *p, const void *x, void *y, int32_t dtype, rqx_str
     if (x == y) return fail(RQX_ERR_ARG, "x and y must not alias");
     if (int rc = check_dtype(p, dtype)) return rc;
     hipStream_t st = static_cast<hipStream_t>(stream);
-    const KParams k = make_params(p, is_fp64(dtype));
-    HIP_TRY(launch_dtype<MODE_MATVEC>(p, k, x, y, dtype, st));
+    const int slot = is_fp64(dtype) ? 1 : 0;
+    const HitCache &c = p->cache[slot];
+    if (p->use_cache && c.built && p->n_states > 0) {
+        bool any_diag = false;
+        for (const ProbHost &ph : p->probs) any_diag = any_diag || ph.has_diag;
+        const void *diag = !any_diag ? nullptr
+                         : slot == 1 ? static_cast<const void *>(p->diag64)
+                                     : static_cast<const void *>(p->diag32);
+        switch (dtype) {
+            case RQX_F64: HIP_TRY((spmv_launch<double, false>(p, c, diag, x, y, st))); break;
+            case RQX_C128: HIP_TRY((spmv_launch<double, true>(p, c, diag, x, y, st))); break;
+            case RQX_F32: HIP_TRY((spmv_launch<float, false>(p, c, diag, x, y, st))); break;
+            default: HIP_TRY((spmv_launch<float, true>(p, c, diag, x, y, st))); break;
+        }
+    } else {
+        const KParams k = make_params(p, is_fp64(dtype));
+        HIP_TRY(launch_dtype<MODE_MATVEC>(p, k, x, y, dtype, st));
+    }
     if (p->n_total > p->n_states) {
         const int64_t cnt = p->n_total - p->n_states;
         const unsigned blocks = static_cast<unsigned>((cnt + 255) / 256);
@@ -1400,6 +1540,129 @@ int rqx_plan_update_coef(rqx_plan *p, const double *coef, int32_t is_real, rqx_s
         HIP_TRY(hipGetLastError());
     }
     p->is_real = is_real ? 1 : 0;
+    // cached hits keep their columns; values are rewritten on the stream, or the
+    // cache is released when the value type changes (real <-> complex)
+    for (int slot = 0; slot < 2; ++slot) {
+        HitCache &c = p->cache[slot];
+        if (!c.built) continue;
+        if (c.cplx != (p->is_real ? 0 : 1)) {
+            HIP_TRY(hipStreamSynchronize(st));
+            HIP_TRY(free_cache(c));
+        } else {
+            HIP_TRY(fill_cache(p, slot, st));
+        }
+    }
+    return RQX_OK;
+}
+
+int rqx_plan_build_cache(rqx_plan *p, int32_t precision, int64_t max_bytes, rqx_stream_t stream) {
+    g_err.clear();
+    if (int rc = check_plan(p)) return rc;
+    const int slot = cache_slot(precision);
+    if (slot < 0) return fail(RQX_ERR_ARG, "precision must be 32 or 64");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    HIP_TRY(hipStreamSynchronize(st));
+    HIP_TRY(free_cache(p->cache[slot]));
+    if (p->n_states == 0) return RQX_OK;
+    const int64_t n = p->n_states;
+    if (n > INT32_MAX) return fail(RQX_ERR_LIMIT, "hit cache columns are int32: at most 2^31 - 1 rows");
+    const int64_t n_slices = (n + SLICE - 1) / SLICE;
+
+    // pass 1: non-diagonal hits per row
+    uint32_t *rowcnt = nullptr;
+    HIP_TRY(hipMalloc(&rowcnt, static_cast<size_t>(n) * 4));
+    KParams k = make_params(p, true);
+    k.rowcnt = rowcnt;
+    hipError_t e = launch_tiles<MODE_ROWCOUNT, double, true>(p, k, nullptr, nullptr, st);
+    std::vector<uint32_t> cnt(static_cast<size_t>(n));
+    if (e == hipSuccess) e = hipMemcpyAsync(cnt.data(), rowcnt, static_cast<size_t>(n) * 4, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    const hipError_t ef = hipFree(rowcnt);
+    if (e == hipSuccess) e = ef;
+    if (e != hipSuccess) return fail(RQX_ERR_HIP, std::string("cache row count: ") + hipGetErrorString(e));
+
+    // slice widths and offsets
+    std::vector<uint32_t> width(static_cast<size_t>(n_slices), 0u);
+    std::vector<int64_t> off(static_cast<size_t>(n_slices), 0);
+    int64_t entries = 0, nnz = 0;
+    for (int64_t s = 0; s < n_slices; ++s) {
+        uint32_t w = 0;
+        for (int64_t r = s * SLICE; r < std::min(n, (s + 1) * SLICE); ++r) {
+            w = std::max(w, cnt[static_cast<size_t>(r)]);
+            nnz += cnt[static_cast<size_t>(r)];
+        }
+        width[static_cast<size_t>(s)] = w;
+        off[static_cast<size_t>(s)] = entries;
+        entries += static_cast<int64_t>(w) * SLICE;
+    }
+    const int cplx = p->is_real ? 0 : 1;
+    const size_t vb = cache_val_bytes(slot, cplx);
+    const size_t o_col = 0, o_val = align_up(static_cast<size_t>(entries) * 4),
+                 o_off = o_val + align_up(static_cast<size_t>(entries) * vb),
+                 o_w = o_off + align_up(static_cast<size_t>(n_slices) * 8),
+                 total = o_w + align_up(static_cast<size_t>(n_slices) * 4);
+    size_t free_b = 0, total_b = 0;
+    HIP_TRY(hipMemGetInfo(&free_b, &total_b));
+    const int64_t limit = max_bytes > 0 ? max_bytes : static_cast<int64_t>(free_b / 10 * 9);
+    if (static_cast<int64_t>(total) > limit)
+        return fail(RQX_ERR_LIMIT, "hit cache needs " + std::to_string(total) + " bytes, limit " +
+                                       std::to_string(limit));
+
+    HitCache c;
+    HIP_TRY(hipMalloc(&c.block, total));
+    char *b = static_cast<char *>(c.block);
+    c.col = reinterpret_cast<int32_t *>(b + o_col);
+    c.val = b + o_val;
+    c.off = reinterpret_cast<int64_t *>(b + o_off);
+    c.width = reinterpret_cast<uint32_t *>(b + o_w);
+    c.entries = entries;
+    c.nnz = nnz;
+    c.n_slices = n_slices;
+    c.bytes = static_cast<int64_t>(total);
+    c.cplx = cplx;
+    c.built = true;
+    e = hipMemcpyAsync(c.off, off.data(), off.size() * 8, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(c.width, width.data(), width.size() * 4, hipMemcpyHostToDevice, st);
+    p->cache[slot] = c;
+    // pass 2: columns and values in the order the tile kernel accumulates them
+    if (e == hipSuccess) e = fill_cache(p, slot, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);   // host vectors above are released on return
+    if (e != hipSuccess) {
+        (void)free_cache(p->cache[slot]);
+        return fail(RQX_ERR_HIP, std::string("cache fill: ") + hipGetErrorString(e));
+    }
+    return RQX_OK;
+}
+
+int rqx_plan_drop_cache(rqx_plan *p, int32_t precision) {
+    g_err.clear();
+    if (int rc = check_plan(p)) return rc;
+    if (precision != 0 && cache_slot(precision) < 0) return fail(RQX_ERR_ARG, "precision must be 0, 32 or 64");
+    HIP_TRY(hipDeviceSynchronize());
+    for (int slot = 0; slot < 2; ++slot)
+        if (precision == 0 || cache_slot(precision) == slot) HIP_TRY(free_cache(p->cache[slot]));
+    return RQX_OK;
+}
+
+int rqx_plan_cache_info(const rqx_plan *p, int32_t precision, int64_t *out) {
+    g_err.clear();
+    if (!p || !out) return fail(RQX_ERR_ARG, "NULL argument");
+    const int slot = cache_slot(precision);
+    if (slot < 0) return fail(RQX_ERR_ARG, "precision must be 32 or 64");
+    const HitCache &c = p->cache[slot];
+    out[0] = c.built ? 1 : 0;
+    out[1] = c.nnz;
+    out[2] = c.entries;
+    out[3] = c.bytes;
+    out[4] = c.n_slices;
+    out[5] = c.cplx;
+    return RQX_OK;
+}
+
+int rqx_plan_use_cache(rqx_plan *p, int32_t enable) {
+    g_err.clear();
+    if (!p) return fail(RQX_ERR_ARG, "plan is NULL");
+    p->use_cache = enable ? 1 : 0;
     return RQX_OK;
 }
 

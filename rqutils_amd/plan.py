@@ -237,6 +237,57 @@ class Plan:
             hits = self.count()['hits']
         return 8 * n + 8 * n * Jp + s_v * hits + s_v * n + T * (8 + s_c) + 12 * J
 
+    # ------------------------------------------------------------------ hit cache
+    def build_cache(self, precision=64, max_bytes=0):
+        """Store every hit (source row, ``C_j(s_i)``) once: ``cache_strategy=(1, 2)``.
+
+        The reference caches source indices and coefficient sums for all ``J x N``
+        (group, row) pairs (``sqd.py:343-407``); this cache keeps only the hits,
+        as sliced ELL (64 rows per slice).  ``precision`` 64 serves float64 and
+        complex128 matvecs, 32 float32 and complex64.  Later ``matvec`` calls of
+        that precision read the cache and return the same values as the
+        matrix-free kernel.  Raises ``RqxError`` when the cache needs more than
+        ``max_bytes`` (0: 90% of the free device memory).
+        """
+        stream = _capi.current_stream(self.device)
+        _capi.check(_capi.lib().rqx_plan_build_cache(self._handle, int(precision), int(max_bytes), stream))
+        return self.cache_info(precision)
+
+    def drop_cache(self, precision=0):
+        """Release the cache of one precision (32 or 64) or both (0)."""
+        _capi.check(_capi.lib().rqx_plan_drop_cache(self._handle, int(precision)))
+
+    def cache_info(self, precision=64):
+        """``built``, stored hits, entries incl. padding, device bytes, slices, complex values."""
+        out = np.zeros(6, dtype=np.int64)
+        _capi.check(_capi.lib().rqx_plan_cache_info(self._handle, int(precision), _capi.ptr(out)))
+        keys = ('built', 'nnz', 'entries', 'bytes', 'slices', 'complex')
+        info = {k: int(v) for k, v in zip(keys, out)}
+        info['built'] = bool(info['built'])
+        info['complex'] = bool(info['complex'])
+        return info
+
+    def use_cache(self, enable=True):
+        """Let ``matvec`` read a built cache (default) or force the matrix-free kernel."""
+        _capi.check(_capi.lib().rqx_plan_use_cache(self._handle, int(bool(enable))))
+
+    def cached_bytes(self, dtype):
+        """Bytes one cached matvec must move: the cache, the diagonal, ``x`` once and ``y``.
+
+        ``entries (4 + s_val) + 12 slices + N (s_d + 2 s_v)`` with ``s_val`` the
+        stored value bytes, ``s_d`` the diagonal entry bytes (complex) and
+        ``s_v`` the vector element bytes; gathers of ``x`` past its first read
+        are counted as cache hits.
+        """
+        precision = 64 if dtype in (torch.float64, torch.complex128) else 32
+        info = self.cache_info(precision)
+        if not info['built']:
+            raise ValueError(f'no {precision}-bit cache is built')
+        s_v = torch.empty(0, dtype=dtype).element_size()
+        s_val = (8 if precision == 64 else 4) * (2 if info['complex'] else 1)
+        s_d = 16 if precision == 64 else 8
+        return info['entries'] * (4 + s_val) + 12 * info['slices'] + self.n_states * (s_d + 2 * s_v)
+
 
 class BatchPlan(Plan):
     """Many (Hamiltonian, subspace) problems resident together; every kernel covers all of them.

@@ -1,0 +1,146 @@
+"""Hit cache (sliced ELL of the hits): cached matvec equals the matrix-free kernel bit for bit (GPU)."""
+import ctypes
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import sqdref
+from rqutils_amd import _capi, workloads
+from rqutils_amd.paulis.symplectic import TermTables
+from rqutils_amd.plan import BatchPlan, Plan
+
+pytestmark = pytest.mark.gpu
+DEV = 'cuda:0'
+RTOL64 = 1e-12
+
+
+def _say(msg):
+    print(f'[case] {msg}', flush=True)
+    sys.stdout.flush()
+
+
+def _random_problem(seed, n, t, ns, probs=(0.55, 0.15, 0.15, 0.15), drop_diagonal=False):
+    rng = np.random.default_rng(seed)
+    codes = workloads.random_pauli_codes(n, t, probs, rng)
+    xm, zm = workloads.codes_to_masks(codes)
+    if drop_diagonal:
+        keep = xm != 0
+        xm, zm = xm[keep], zm[keep]
+    tables = TermTables.from_masks(xm, zm, rng.standard_normal(xm.shape[0]), n)
+    states = workloads.random_subspace(n, ns, rng)
+    return tables, states
+
+
+@pytest.fixture(scope='module')
+def cases():
+    w = workloads.config1(n_alpha=40, n_beta=30)
+    return {
+        'complex12': _random_problem(2, 12, 120, 2000),
+        'real_jw': (w.tables, w.states),
+        'no_diagonal': _random_problem(5, 11, 80, 1500, drop_diagonal=True),
+        'wide16': _random_problem(3, 16, 300, 3000, probs=(0.7, 0.1, 0.1, 0.1)),
+    }
+
+
+def _dtypes(plan):
+    return (torch.complex128, torch.complex64) if not plan.is_real else \
+        (torch.float64, torch.complex128, torch.float32, torch.complex64)
+
+
+def _vec(n, dtype, seed):
+    rng = np.random.default_rng(seed)
+    v = torch.from_numpy(rng.standard_normal(n) + 1j * rng.standard_normal(n)).to(DEV)
+    return (v.real if not dtype.is_complex else v).to(dtype).contiguous()
+
+
+def _check_identical(plan, seed=11):
+    for dt in _dtypes(plan):
+        x = _vec(plan.n_total, dt, seed)
+        plan.use_cache(False)
+        y_free = plan.matvec(x)
+        plan.use_cache(True)
+        y_cache = plan.matvec(x)
+        torch.cuda.synchronize()
+        assert torch.equal(y_cache, y_free), dt
+
+
+@pytest.mark.parametrize('name', ['complex12', 'real_jw', 'no_diagonal', 'wide16'])
+def test_cached_matvec_identical(cases, name):
+    _say(f'cache {name}')
+    tables, states = cases[name]
+    n_total = len(states) + 37          # filler rows past the states
+    plan = Plan(tables, states, n_total=n_total)
+    hits = plan.count()['hits']
+    for precision in (64, 32):
+        info = plan.build_cache(precision)
+        assert info['built'] and info['complex'] == (not plan.is_real)
+        assert info['nnz'] == hits - (plan.n_states if plan.info['has_diagonal'] else 0)
+        assert info['entries'] >= info['nnz'] and info['entries'] % 64 == 0
+    _check_identical(plan)
+    # the cached values are the oracle's matrix elements
+    x = _vec(plan.n_total, torch.complex128, 3)
+    y = plan.matvec(x).cpu().numpy()
+    groups = [(tables.zmask[tables.grp_off[j]:tables.grp_off[j + 1]],
+               tables.coef[tables.grp_off[j]:tables.grp_off[j + 1]]) for j in range(tables.num_groups)]
+    want = sqdref.matvec_u64(tables.xmask, groups, np.asarray(states, dtype=np.uint64), x.cpu().numpy())
+    assert np.max(np.abs(y[:plan.n_states] - want)) <= RTOL64 * max(np.max(np.abs(want)), 1e-300)
+    assert np.all(y[plan.n_states:] == 0)
+    plan.drop_cache(64)
+    assert not plan.cache_info(64)['built'] and plan.cache_info(32)['built']
+    plan.drop_cache()
+    assert not plan.cache_info(32)['built']
+
+
+def test_cached_batch_matvec_identical(cases):
+    _say('cache batch')
+    bp = BatchPlan([cases[k] for k in ('complex12', 'real_jw', 'no_diagonal', 'wide16')])
+    assert bp.info['persistent'] == 1
+    bp.build_cache(64)
+    bp.build_cache(32)
+    _check_identical(bp)
+    bp.set_launch(False)            # fill pass by one workgroup per tile gives the same cache
+    bp.build_cache(64)
+    _check_identical(bp, seed=12)
+
+
+def test_set_alphas_rewrites_cached_values(cases):
+    _say('cache set_alphas')
+    tables, states = cases['complex12']
+    plan = Plan(tables, states)
+    plan.build_cache(64)
+    plan.build_cache(32)
+    rng = np.random.default_rng(9)
+    plan.set_alphas(torch.from_numpy(rng.standard_normal(plan.info['n_terms'])).to(DEV))
+    assert plan.cache_info(64)['built']
+    _check_identical(plan, seed=13)
+
+
+def test_real_to_complex_coefficients_release_the_cache(cases):
+    _say('cache real->complex')
+    tables, states = cases['real_jw']
+    plan = Plan(tables, states)
+    assert plan.is_real
+    plan.build_cache(64)
+    rng = np.random.default_rng(4)
+    coef = torch.from_numpy(rng.standard_normal(plan.info['n_terms']) +
+                            1j * rng.standard_normal(plan.info['n_terms'])).to(DEV).contiguous()
+    stream = _capi.current_stream(plan.device)
+    _capi.check(_capi.lib().rqx_plan_update_coef(plan._handle, torch.view_as_real(coef).data_ptr(), 0, stream))
+    plan.is_real = False
+    assert not plan.cache_info(64)['built']
+    plan.build_cache(64)
+    assert plan.cache_info(64)['complex']
+    _check_identical(plan, seed=14)
+
+
+def test_cache_limit_is_an_error(cases):
+    _say('cache limit')
+    tables, states = cases['wide16']
+    plan = Plan(tables, states)
+    with pytest.raises(_capi.RqxError):
+        plan.build_cache(64, max_bytes=4096)
+    assert not plan.cache_info(64)['built']
+    x = _vec(plan.n_total, torch.complex128, 5)
+    assert torch.isfinite(plan.matvec(x)).all()
