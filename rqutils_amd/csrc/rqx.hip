@@ -706,9 +706,34 @@ __device__ inline void fma_acc(float2 &acc, float c, float2 v) {
     acc.y = fmaf(c, v.y, acc.y);
 }
 
+// entries per batch of the cached matvec: the column and value loads of a batch,
+// then its gathers of x, are all in flight together (16 while a batch's
+// registers stay small, 8 for 16-byte vector entries or complex values)
+template <typename V, typename CV> constexpr int spmv_unroll() {
+#ifdef SPMV_UNROLL
+    return SPMV_UNROLL;
+#else
+    return (sizeof(V) + sizeof(CV) <= 12) ? 16 : 8;
+#endif
+}
+
+// workgroup -> 256-row block of the cached matvec: contiguous blocks per XCD, so
+// one XCD's L2 sees the gathers of neighbouring rows
+__device__ inline uint32_t spmv_block() { return xcd_remap(blockIdx.x, gridDim.x); }
+
+template <typename T> __device__ inline T stream_load(const T *p) { return __builtin_nontemporal_load(p); }
+__device__ inline float2 stream_load(const float2 *p) {
+    return make_float2(__builtin_nontemporal_load(&p->x), __builtin_nontemporal_load(&p->y));
+}
+__device__ inline double2 stream_load(const double2 *p) {
+    return make_double2(__builtin_nontemporal_load(&p->x), __builtin_nontemporal_load(&p->y));
+}
+
 // y = H x from the hit cache: one wave per 64-row slice, one row per lane,
-// entries read lane-contiguously; the same accumulation sequence as k_tiles
-// (diagonal first, then hits in group order), so results are identical.
+// entries read lane-contiguously (and marked non-temporal: they are read once
+// per matvec, x is the data worth keeping in L2); the same accumulation
+// sequence as k_tiles (diagonal first, then hits in group order), so results
+// are identical.
 template <typename R, bool CPLXV, bool CPLXC>
 __global__ __launch_bounds__(256) void k_spmv_sell(const int32_t *__restrict__ col, const void *__restrict__ val,
                                                    const int64_t *__restrict__ off, const uint32_t *__restrict__ width,
@@ -717,17 +742,42 @@ __global__ __launch_bounds__(256) void k_spmv_sell(const int32_t *__restrict__ c
     using V = typename VT<R, CPLXV>::T;
     using CC = typename C2<R>::T;
     using CV = typename std::conditional<CPLXC, CC, R>::type;
-    const int64_t row = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    constexpr int U = spmv_unroll<V, CV>();
+    const int64_t row = static_cast<int64_t>(spmv_block()) * blockDim.x + threadIdx.x;
     if (row >= n_states) return;
+    // a wave's 64 rows are one slice (256-thread blocks start on slice bounds)
     const int64_t slice = row / SLICE;
     const int lane = static_cast<int>(row % SLICE);
     const V *x = static_cast<const V *>(xin);
     const CV *v = static_cast<const CV *>(val);
     V acc = vzero(static_cast<V *>(nullptr));
     if (diag) fma_acc(acc, static_cast<const CC *>(diag)[row], x[row]);   // NULL: no problem has an x = 0 group
-    const uint32_t w = width[slice];
-    int64_t e = off[slice] + lane;
-    for (uint32_t k = 0; k < w; ++k, e += SLICE) fma_acc(acc, v[e], x[col[e]]);
+    const uint32_t w = __builtin_amdgcn_readfirstlane(width[slice]);
+    const int64_t e0 = off[slice];
+    const int32_t *c = col + e0 + lane;
+    const CV *vv = v + e0 + lane;
+    uint32_t k = 0;
+    for (; k + U <= w; k += U) {
+        int32_t ci[U];
+        CV cv[U];
+        V xv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            ci[u] = stream_load(c + u * SLICE);
+            cv[u] = stream_load(vv + u * SLICE);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) xv[u] = x[ci[u]];
+#pragma unroll
+        for (int u = 0; u < U; ++u) fma_acc(acc, cv[u], xv[u]);
+        c += U * SLICE;
+        vv += U * SLICE;
+    }
+    for (; k < w; ++k) {
+        fma_acc(acc, stream_load(vv), x[stream_load(c)]);
+        c += SLICE;
+        vv += SLICE;
+    }
     static_cast<V *>(yout)[row] = acc;
 }
 

@@ -12,8 +12,10 @@ sorted subspace :math:`S` and a vector :math:`v`
 with the term 0 when :math:`s_i \oplus x_j \notin S`.  States are uint64
 bitstrings (bit q = qubit q); the gfx950 kernel (``librqx``) searches the
 flipped states in a two-level sorted index instead of building the reference's
-``J x N`` source-index cache, so every ``cache_strategy`` gives the same numbers
-and needs no extra memory.  Filler slots (``states_size`` beyond the unique
+``J x N`` source-index cache.  ``cache_strategy`` (1, *) (the reference default)
+additionally stores the hits once, source row and coefficient sum, in a
+compacted device cache when it fits (``Plan.build_cache``); every strategy
+gives the same numbers.  Filler slots (``states_size`` beyond the unique
 count) are the reference's 0xFF rows; they stay decoupled and hold 0.
 """
 from collections.abc import Sequence
@@ -28,6 +30,7 @@ import torch
 from scipy.sparse import coo_array, csr_array
 
 from rqutils_amd.paulis.symplectic import PauliSumXZ, masks_to_packed, packed_to_masks
+from rqutils_amd._capi import RqxError
 from rqutils_amd.plan import Plan, sort_unique_u64
 from rqutils_amd.ground_locg import ground_locg
 
@@ -42,7 +45,8 @@ class SQDOptions:
     return_eigvec: bool = True
     """Whether to return the eigenvector (coefficients and unique state bitstrings)."""
     cache_strategy: tuple[int, int] = (1, 0)
-    """Accepted for compatibility; every strategy runs the same on-the-fly kernel."""
+    """(1, *): cache the hits (source rows and coefficient sums) on the device when they fit
+    in its free memory; (0, *): matrix-free.  The results are identical."""
     shard_bundle: int = 1
     """Accepted for compatibility (single-GPU plan)."""
     locg_maxiter: int = 1000
@@ -189,6 +193,14 @@ def make_plan(hamiltonian: PauliSumXZ, keys, states_size=None, device=None, spli
 def _run(hamiltonian, keys, states_size, options, log_level=logging.INFO):
     plan = make_plan(hamiltonian, keys, states_size)
     tables = hamiltonian.tables
+    if options.cache_strategy[0] == 1 and plan.n_states > 0:
+        # sqd.py:343-351 caches the source index of every (group, row) pair; the
+        # hit cache holds the hits with their coefficient sums (strategy (1, 2))
+        try:
+            info = plan.build_cache(64)
+            LOG.debug('Hit cache: %d hits, %d bytes', info['nnz'], info['bytes'])
+        except RqxError as exc:
+            LOG.info('Hit cache does not fit (%s); using the matrix-free kernel', exc)
     dtype = torch.float64 if tables.is_real else torch.complex128
     # vinit (sqd.py:406-426): argmin of the x = 0 group's diagonal over real rows
     if tables.has_diagonal and plan.n_states > 0:

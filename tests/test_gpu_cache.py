@@ -1,5 +1,4 @@
 """Hit cache (sliced ELL of the hits): cached matvec equals the matrix-free kernel bit for bit (GPU)."""
-import ctypes
 import sys
 
 import numpy as np
@@ -144,3 +143,24 @@ def test_cache_limit_is_an_error(cases):
     assert not plan.cache_info(64)['built']
     x = _vec(plan.n_total, torch.complex128, 5)
     assert torch.isfinite(plan.matvec(x)).all()
+
+
+def test_run_sqd_cache_strategies_identical(cases):
+    """cache_strategy (1, *) runs on the hit cache, (0, *) matrix-free: the same eigenpair bit for bit."""
+    _say('cache run_sqd strategies')
+    from rqutils_amd.paulis.symplectic import PauliSumXZ, masks_to_packed
+    from rqutils_amd.sqd import SQDOptions, run_sqd
+    rng = np.random.default_rng(21)
+    codes = workloads.random_pauli_codes(11, 70, (0.55, 0.15, 0.15, 0.15), rng)
+    paulis = workloads.codes_to_strings(codes)
+    coeffs = rng.standard_normal(len(paulis))
+    keys = workloads.random_subspace(11, 700, rng)
+    h = PauliSumXZ.from_paulisum((paulis, coeffs), force_real=True, add_padding=True)
+    states_p = masks_to_packed(keys, 11, padded=True)
+    out = {}
+    for cs in ((0, 0), (1, 0), (1, 2)):
+        out[cs] = run_sqd(h, states_p, 720, options=SQDOptions(cache_strategy=cs, locg_maxiter=2000))
+    for cs in ((1, 0), (1, 2)):
+        assert float(out[cs][0]) == float(out[(0, 0)][0])
+        assert torch.equal(out[cs][1], out[(0, 0)][1])
+    assert out[(1, 0)][3] == 700
