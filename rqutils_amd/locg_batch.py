@@ -20,15 +20,15 @@ import time
 
 import torch
 
-from rqutils_amd.plan import BatchPlan, eigenpair_batch
+from rqutils_amd.plan import Plan, eigenpair_batch
 
 _LO = {torch.float64: torch.float32, torch.complex128: torch.complex64}
 
 
 class BatchLOCG:
-    """Persistent buffers + one LOCG step for all problems of a ``BatchPlan``."""
+    """Persistent buffers + one LOCG step for all problems of a plan (``BatchPlan`` or single ``Plan``)."""
 
-    def __init__(self, plan: BatchPlan, x0: torch.Tensor, tol=1e-7, switch_tol=1e-5, mixed=True):
+    def __init__(self, plan: Plan, x0: torch.Tensor, tol=1e-7, switch_tol=1e-5, mixed=True):
         if x0.dtype not in _LO:
             raise ValueError('x0 must be float64 or complex128')
         if x0.shape != (plan.n_total,) or x0.device != plan.device:
@@ -142,9 +142,18 @@ class BatchLOCG:
         self._apply(self.y, self.ay)
 
 
-def batch_ground_locg(plan: BatchPlan, x0, tol=1e-7, switch_tol=1e-5, maxiter=1000, check_every=10,
-                      use_graph=True, mixed=True, record=False):
-    """Ground eigenpairs of every problem of ``plan``.
+def batch_ground_locg(plan: Plan, x0, tol=1e-7, switch_tol=1e-5, maxiter=1000, check_every=10,
+                      use_graph=True, mixed=True, record=False, energy_tol=None, energy_checks=2,
+                      switch_energy=1e-9):
+    """Ground eigenpairs of every problem of ``plan`` (a ``BatchPlan``, or a ``Plan`` as one problem).
+
+    A problem stops (its vectors freeze) at relative residual ``tol``, or, with
+    ``energy_tol``, once its theta, polled every ``check_every`` steps, has
+    settled: the last decrease and the geometric (Aitken) estimate of the
+    remaining decrease are both below ``energy_tol * |theta|`` at
+    ``energy_checks`` consecutive polls.  In the low-precision phase the same
+    test with the limit ``max(energy_tol, 30 switch_energy)`` (theta carries
+    FP32 image errors there) hands every problem over to the FP64 phase.
 
     Returns ``(theta (K,), x, iterations, info)`` where ``info`` holds per-problem
     relative residuals, the iteration of the precision switch and timings.
@@ -175,20 +184,42 @@ def batch_ground_locg(plan: BatchPlan, x0, tol=1e-7, switch_tol=1e-5, maxiter=10
     it = 1
     switch_at = None
     history = []
+    thetas = []
+    calm = torch.zeros(solver.K, dtype=torch.int64)
     while it < maxiter:
         for _ in range(check_every):
             run_step()
             it += 1
-        active = int(solver.active.sum().item())
+        act = solver.active.cpu()
+        active = int(act.sum())
         worst = float(solver.rel[solver.active].max().item()) if active else 0.0
         if record:
             history.append((time.perf_counter() - t0, it, active, worst, solver.phase))
-        if solver.phase == 'lo' and (worst <= solver.switch_tol or active == 0 or it >= maxiter // 2):
-            solver.to_high_precision()
-            solver.active.fill_(True)
-            switch_at = it
+        settled = torch.zeros(solver.K, dtype=torch.bool)
+        if energy_tol is not None:
+            th = solver.theta.cpu()
+            thetas.append(th)
+            if len(thetas) >= 3:
+                d0, d1 = thetas[-3] - thetas[-2], thetas[-2] - thetas[-1]
+                scale = energy_tol if solver.phase == 'hi' else max(energy_tol, 30 * switch_energy)
+                lim = scale * th.abs()
+                tail = torch.where((d0 > d1) & (d1 >= 0), d1 * d1 / (d0 - d1), torch.full_like(d1, float('inf')))
+                ok = (d1 >= 0) & (d1 <= lim) & (tail <= lim)
+                calm = torch.where(ok, calm + 1, torch.zeros_like(calm))
+                settled = calm >= energy_checks
+        if solver.phase == 'lo':
+            if (worst <= solver.switch_tol or active == 0 or it >= maxiter // 2 or
+                    (energy_tol is not None and bool((settled | ~act).all()))):
+                solver.to_high_precision()
+                solver.active.fill_(True)
+                switch_at = it
+                thetas.clear()
+                calm.zero_()
             continue
-        if solver.phase == 'hi' and active == 0:
+        if energy_tol is not None and bool((settled & act).any()):
+            solver.active.copy_((act & ~settled).to(plan.device))
+            active = int((act & ~settled).sum())
+        if active == 0:
             break
     torch.cuda.synchronize(plan.device)
     info = {'rel_residual': solver.rel.clone(), 'switch_iteration': switch_at, 'iterations': it,

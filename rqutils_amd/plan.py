@@ -72,6 +72,7 @@ class Plan:
         _capi.check(lib.rqx_plan_get_info(self._handle, ctypes.byref(info)))
         self.info = info.as_dict()
         self.is_real = bool(self.info['is_real'])
+        self.n_problems = int(self.info['n_problems'])
         self.problems = []
         for k in range(self.info['n_problems']):
             pi = _capi.ProblemInfo()
@@ -288,6 +289,32 @@ class Plan:
         s_d = 16 if precision == 64 else 8
         return info['entries'] * (4 + s_val) + 12 * info['slices'] + self.n_states * (s_d + 2 * s_v)
 
+    # ------------------------------------------------------------------ per-problem vector ops
+    def dots(self, pairs):
+        """Per-problem ``sum conj(a) * b`` for each ``(a, b)`` pair -> (K, len(pairs)) complex128.
+
+        One fused pass over all pairs; a single-problem plan gives ``K = 1``.
+        """
+        n = len(pairs)
+        a_ptrs = (ctypes.c_void_p * n)(*[a.data_ptr() for a, _ in pairs])
+        b_ptrs = (ctypes.c_void_p * n)(*[b.data_ptr() for _, b in pairs])
+        out = torch.empty((self.n_problems, n, 2), dtype=torch.float64, device=self.device)
+        dt = pairs[0][0].dtype
+        stream = _capi.current_stream(self.device)
+        _capi.check(_capi.lib().rqx_batch_dots(self._handle, n, a_ptrs, b_ptrs, out.data_ptr(),
+                                               _DTYPE_CODES[dt], stream))
+        return torch.view_as_complex(out)
+
+    def combine(self, vectors, coef, out):
+        """``out = sum_q coef[:, q] * vectors[q]`` per problem (coef: (K, n) complex128, device)."""
+        n = len(vectors)
+        v_ptrs = (ctypes.c_void_p * n)(*[v.data_ptr() for v in vectors])
+        c = torch.view_as_real(coef.to(torch.complex128).contiguous()).contiguous()
+        stream = _capi.current_stream(self.device)
+        _capi.check(_capi.lib().rqx_batch_combine(self._handle, n, v_ptrs, c.data_ptr(), out.data_ptr(),
+                                                  _DTYPE_CODES[out.dtype], stream))
+        return out
+
 
 class BatchPlan(Plan):
     """Many (Hamiltonian, subspace) problems resident together; every kernel covers all of them.
@@ -341,7 +368,6 @@ class BatchPlan(Plan):
         self._load_info()
         self.n_states = int(self.info['n_states'])
         self.n_total = int(self.info['n_total'])
-        self.n_problems = K
         offs = [p['row_offset'] for p in self.problems] + [self.n_total]
         self.row_offsets = torch.tensor(offs, dtype=torch.int64, device=self.device)
         self.row_problem = torch.repeat_interleave(
@@ -366,30 +392,6 @@ class BatchPlan(Plan):
             jp = J - (1 if p['has_diagonal'] else 0)
             total += 8 * n + 8 * n * jp + s_v * n + T * (8 + s_c) + 12 * J
         return total
-
-    # ---------------------------------------------------------------- batched vector ops
-    def dots(self, pairs):
-        """Per-problem ``sum conj(a) * b`` for each ``(a, b)`` pair -> (K, len(pairs)) complex128."""
-        n = len(pairs)
-        a_ptrs = (ctypes.c_void_p * n)(*[a.data_ptr() for a, _ in pairs])
-        b_ptrs = (ctypes.c_void_p * n)(*[b.data_ptr() for _, b in pairs])
-        out = torch.empty((self.n_problems, n, 2), dtype=torch.float64, device=self.device)
-        dt = pairs[0][0].dtype
-        stream = _capi.current_stream(self.device)
-        _capi.check(_capi.lib().rqx_batch_dots(self._handle, n, a_ptrs, b_ptrs, out.data_ptr(),
-                                               _DTYPE_CODES[dt], stream))
-        return torch.view_as_complex(out)
-
-    def combine(self, vectors, coef, out):
-        """``out = sum_q coef[:, q] * vectors[q]`` per problem (coef: (K, n) complex128, device)."""
-        n = len(vectors)
-        v_ptrs = (ctypes.c_void_p * n)(*[v.data_ptr() for v in vectors])
-        c = torch.view_as_real(coef.to(torch.complex128).contiguous()).contiguous()
-        stream = _capi.current_stream(self.device)
-        _capi.check(_capi.lib().rqx_batch_combine(self._handle, n, v_ptrs, c.data_ptr(), out.data_ptr(),
-                                                  _DTYPE_CODES[out.dtype], stream))
-        return out
-
 
 def eigenpair_batch(mats: torch.Tensor):
     """Lowest eigenpairs of ``(K, n, n)`` Hermitian device matrices -> (theta (K,), kappa (K, n))."""

@@ -146,6 +146,27 @@ def test_batch_locg_matches_eigsh(problems, use_graph):
     assert info['switch_iteration'] is not None
 
 
+@pytest.mark.parametrize('cached', [False, True])
+def test_single_plan_graph_locg_energy_stop(problems, cached):
+    """A single Plan drives the graph-captured step; the energy-tail stop reaches 1e-8 of eigsh."""
+    _say(f'single-plan graph locg cached={cached}')
+    t, s = problems[4]                       # JW model, real
+    plan = Plan(t, s, n_total=len(s) + 11)   # filler rows stay 0
+    if cached:
+        plan.build_cache(32)
+        plan.build_cache(64)
+    diag = plan.diagonal().real[:plan.n_states]
+    x0 = torch.zeros(plan.n_total, dtype=torch.complex128, device=DEV)
+    x0[int(torch.argmin(diag))] = 1
+    theta, x, it, info = batch_ground_locg(plan, x0, tol=1e-12, maxiter=3000, check_every=5,
+                                           energy_tol=3e-10)
+    h = _csr(Plan(t, s))
+    e = scipy.sparse.linalg.eigsh(h, k=1, which='SA', tol=1e-12)[0][0]
+    assert abs(float(theta[0]) - e) < 1e-8 * abs(e), (float(theta[0]), e)
+    assert it < 3000 and info['switch_iteration'] is not None
+    assert torch.count_nonzero(x[plan.n_states:]) == 0
+
+
 def test_batch_graph_replays_match_eager(problems):
     """The captured step replayed N times gives the eager trajectory."""
     _say('graph vs eager')
