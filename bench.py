@@ -1,13 +1,16 @@
 """Benchmark: SQD subspace matvec throughput on one MI355X (+ CPU baseline).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload jw1m|jw25m|rand8k]
-                    [--dtype c64|f32|f64|c128] [--no-cpu] [--locg]
+                    [--dtype c64|f32|f64|c128] [--no-cpu] [--no-locg] [--no-extras]
 
-A step is one matvec ``y = H_S x`` over the whole subspace with the term
-tables, the index and ``x`` resident in HBM.  ``value`` is the algorithmic
-byte rate of the matvec (bytes per matvec from the HBM model divided by the
-kernel time measured with HIP events on the launch stream).  One JSON line is
-printed on stdout.
+A step is one matvec ``y = H_S x`` over the whole subspace with the plan and
+``x`` resident in HBM.  The headline path is the one ``run_sqd`` takes by
+default (``cache_strategy`` (1, *)): the hits, built once by the matrix-free
+tile kernel, streamed from the device hit cache.  ``value`` is that matvec's
+byte rate (cache + diagonal + x once + y, divided by the kernel time measured
+with HIP events on the launch stream).  The matrix-free kernel is timed over
+the same steps and reported under ``matrix_free``.  One JSON line is printed
+on stdout.
 """
 import argparse
 import json
@@ -49,7 +52,11 @@ def traffic_from_profiles(workload, dtype):
 
 
 def cpu_baseline(w, alg_bytes, budget_s):
-    """Oracle C port of the reference's strategy (1, 0), timed on a sample of groups."""
+    """Oracle C port of the reference's strategy (1, 0), timed on a sample of groups.
+
+    ``value`` puts the CPU matvec on the headline's scale: the headline's bytes
+    per matvec divided by the CPU seconds per matvec.
+    """
     from oracle import build as obuild, cport
     if not os.path.exists(cport.LIB):
         obuild.build()
@@ -349,7 +356,6 @@ def main():
     ap.add_argument('--locg-maxiter', type=int, default=1000)
     ap.add_argument('--mixed-tol', type=float, default=1e-6)
     ap.add_argument('--energy-tol', type=float, default=3e-10)
-    ap.add_argument('--no-cache', action='store_true', help='skip the hit-cache matvec and LOCG legs')
     ap.add_argument('--no-extras', action='store_true',
                     help='skip the 25M matvec, the 64-problem batch and the gradient timings')
     args = ap.parse_args()
@@ -388,27 +394,55 @@ def main():
     else:
         x = torch.from_numpy(xr).to(dev, dt)
     y = torch.empty_like(x)
-
-    for _ in range(args.warmup):
-        plan.matvec(x, out=y)
-    torch.cuda.synchronize()
-    log(f'warmup done ({args.warmup} matvecs)')
     stream = torch.cuda.current_stream(dev)
-    ev0 = torch.cuda.Event(enable_timing=True)
-    ev1 = torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    wall0 = time.perf_counter()
-    ev0.record(stream)
-    for _ in range(args.steps):
-        plan.matvec(x, out=y)
-    ev1.record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - wall0
-    ms = ev0.elapsed_time(ev1) / args.steps
-    log(f'timed {args.steps} matvecs: {ms:.3f} ms/matvec (events), wall {wall * 1e3 / args.steps:.3f} ms')
 
-    achieved = alg / (ms * 1e-3) / 1e9
-    traffic = traffic_from_profiles(w.name, args.dtype)
+    def timed(label):
+        for _ in range(args.warmup):
+            plan.matvec(x, out=y)
+        torch.cuda.synchronize()
+        ev0 = torch.cuda.Event(enable_timing=True)
+        ev1 = torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        wall0 = time.perf_counter()
+        ev0.record(stream)
+        for _ in range(args.steps):
+            plan.matvec(x, out=y)
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - wall0
+        ms = ev0.elapsed_time(ev1) / args.steps
+        log(f'{label}: {args.steps} matvecs after {args.warmup} warmup, {ms:.4f} ms/matvec (events), '
+            f'wall {wall * 1e3 / args.steps:.4f} ms')
+        return ms
+
+    # matrix-free kernel (the tile kernel searching every (row, group) pair)
+    ms_mf = timed('matrix-free')
+    ach_mf = alg / (ms_mf * 1e-3) / 1e9
+    matrix_free = {
+        'ms_per_matvec': round(ms_mf, 4),
+        'model_bytes_per_matvec': alg,
+        'roofline': {'bound': 'lds', 'achieved': round(ach_mf, 2), 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
+                     'frac': round(ach_mf / HBM_PEAK_GBS, 4), 'traffic': traffic_from_profiles(w.name, args.dtype),
+                     'note': 'model bytes count index probes that LDS/L2 serve; measured traffic is HBM bytes'},
+    }
+    # hit cache (run_sqd's default cache_strategy (1, *)): built once, then every matvec streams it
+    precision = 64 if dt in (torch.float64, torch.complex128) else 32
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    cinfo = plan.build_cache(precision)
+    torch.cuda.synchronize()
+    t_cache = time.perf_counter() - t0
+    log(f'hit cache: {cinfo} built in {t_cache:.3f} s')
+    ms = timed('cached')
+    y_cached = y.clone()
+    plan.use_cache(False)
+    plan.matvec(x, out=y)
+    plan.use_cache(True)
+    identical = bool(torch.equal(y, y_cached))
+    del y_cached
+    nbytes = plan.cached_bytes(dt)
+    achieved = nbytes / (ms * 1e-3) / 1e9
+    traffic = traffic_from_profiles(w.name, args.dtype + ':cached')
     result = {
         'metric': METRIC,
         'value': round(achieved, 2),
@@ -424,13 +458,18 @@ def main():
         'data': 'synthetic (seeded generators, rqutils_amd/workloads.py)',
         'config': {
             'workload': w.name,
+            'path': 'hit cache (cache_strategy (1, *), the run_sqd default); matrix-free kernel in matrix_free',
             'n_states': plan.n_states,
             'n_groups': plan.info['n_groups'],
             'n_terms': plan.info['n_terms'],
             'hits_per_matvec': counts['hits'],
             'segment_probes_per_matvec': counts['probes'],
             'split_bits': plan.info['split_bits'],
-            'algorithmic_bytes_per_matvec': alg,
+            'bytes_per_matvec': nbytes,
+            'cache_bytes': cinfo['bytes'],
+            'cache_entries_incl_padding': cinfo['entries'],
+            'cache_build_s': round(t_cache, 4),
+            'identical_to_matrix_free': identical,
             'setup_s': round(t_setup, 3),
         },
         'roofline': {
@@ -441,18 +480,18 @@ def main():
             'frac': round(achieved / HBM_PEAK_GBS, 4),
             'traffic': traffic,
         },
+        'matrix_free': matrix_free,
     }
+    plan.drop_cache()
     if not args.no_locg:
         result['locg'] = locg_time_to_solution(plan, not args.locg_real, args.locg_maxiter, args.mixed_tol,
                                                args.energy_tol)
-    if not args.no_cache:
-        result['cache'] = {'matvec': cache_leg(plan, dt)}
-        if not args.no_locg:
-            loc = locg_cached(plan, not args.locg_real, args.locg_maxiter, args.mixed_tol, args.energy_tol,
-                              result['locg']['e_star'])
-            loc['speedup_vs_matrix_free_plain'] = round(result['locg']['plain']['time_to_target_s'] /
-                                                        loc['mixed']['time_s'], 3)
-            result['cache']['locg'] = loc
+    if not args.no_locg:
+        loc = locg_cached(plan, not args.locg_real, args.locg_maxiter, args.mixed_tol, args.energy_tol,
+                          result['locg']['e_star'])
+        loc['speedup_vs_matrix_free_plain'] = round(result['locg']['plain']['time_to_target_s'] /
+                                                    loc['mixed_graph']['time_s'], 3)
+        result['locg_cached'] = loc
         plan.drop_cache()
     if not args.no_extras:
         del y, x
@@ -461,7 +500,7 @@ def main():
                             'grad36q': extra_grad()}
     if not args.no_cpu:
         log(f'cpu baseline (budget {args.cpu_budget} s) ...')
-        result['cpu_baseline'] = cpu_baseline(w, alg, args.cpu_budget)
+        result['cpu_baseline'] = cpu_baseline(w, nbytes, args.cpu_budget)
         log(f"cpu baseline {result['cpu_baseline']['value']:.3f} GB/s")
     print(json.dumps(result), flush=True)
 

@@ -155,6 +155,9 @@ int rqx_diagonal(const rqx_plan *plan, double *diag, rqx_stream_t stream);
  * Replaces apply_h_x_cached + make_x_cached_matvec on one device
  * (rqutils/sqd.py:694-709,746-751) and every other cache strategy
  * (sqd.py:643-743): all give the same numbers.  x and y must not alias.
+ * Reads the hit cache of the dtype's precision when one is built (see
+ * rqx_plan_build_cache), else searches every (row, group) pair on the fly;
+ * both give the same values.
  */
 int rqx_matvec(const rqx_plan *plan, const void *x, void *y, int32_t dtype, rqx_stream_t stream);
 
