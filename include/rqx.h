@@ -200,10 +200,12 @@ int rqx_plan_update_coef(rqx_plan *plan, const double *coef, int32_t is_real, rq
  * Hit cache: the reference's cache_strategy (1, 2) (rqutils/sqd.py:343-407: source
  * indices per X group from get_xsource, coefficient sums per group from
  * get_diagonal, applied by apply_h_xz_cached, sqd.py:731-743), compacted to the
- * hits only.  Stored as sliced ELL: 64 rows per slice, entry k of row r at
- * off[r / 64] + 64 k + r % 64, each slice padded to its widest row.  Entries are
- * written in the order the matrix-free kernel accumulates them and with the same
- * values, so rqx_matvec gives identical results with or without the cache.
+ * hits only.  Stored as sliced ELL: 64 rows per slice, each slice in two
+ * sections (hits of groups that flip no bit below the split, whose gathers
+ * coalesce across a slice, and the rest), each padded to its widest row.
+ * Entries are written in the order the matrix-free kernel accumulates them and
+ * with the same values, so rqx_matvec gives identical results with or without
+ * the cache.
  *
  * rqx_plan_build_cache: precision 32 (used by RQX_F32 / RQX_C64 matvecs) or 64
  * (RQX_F64 / RQX_C128); values are real when the Hamiltonian is.  Allocates

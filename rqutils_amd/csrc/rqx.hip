@@ -53,8 +53,14 @@ int fail(int code, const std::string &msg) {
 
 enum { MODE_MATVEC = 0, MODE_LOOKUP = 1, MODE_COUNT = 2, MODE_GRAD = 3, MODE_ROWCOUNT = 4, MODE_FILL = 5 };
 
-// hit cache: sliced ELL, 64 rows per slice (one wave), entry k of row r at
-// off[r / 64] + k * 64 + r % 64, padded to the slice's widest row with (r, 0)
+// hit cache: sliced ELL, 64 rows per slice (one wave), two sections per slice.
+// Section A holds the hits of groups with no flips in the low split bits (the
+// source has the row's own low key in another segment, so the rows of a slice
+// usually all hit the same such groups and their gathers coalesce); section B
+// the other hits.  Entry k of row r: off[s] + 64 k + r % 64 (A, k < wa[s]) and
+// off[s] + 64 (wa[s] + k) + r % 64 (B, k < wb[s]), s = r / 64; rows are padded to
+// the slice's widths with (r, 0).  A row's y is (diagonal + B hits) + A hits,
+// each part accumulated in group order, in the matrix-free kernel too.
 constexpr int SLICE = 64;
 
 // One problem of a plan (a batched plan holds several).  Row, segment, class,
@@ -92,11 +98,13 @@ struct KParams {
     const uint64_t *g_x;           // full X mask per group
     const void *gy;                // gradient mode: upstream gradient (same dtype as x)
     double *grad;                  // gradient mode: per-term output (accumulated)
-    uint32_t *rowcnt;              // row-count mode: non-diagonal hits per row
+    uint32_t *rowcnt;              // row-count mode: section-B hits per row
+    uint32_t *rowcnt_a;            // row-count mode: section-A hits per row
     int32_t *c_col;                // fill mode: cache column indices
     void *c_val;                   // fill mode: cache values (R, or complex R)
     const int64_t *c_off;          // fill mode: first entry of each slice
-    const uint32_t *c_width;       // fill mode: entries per row of each slice
+    const uint32_t *c_wa;          // fill mode: section-A entries per row of each slice
+    const uint32_t *c_wb;          // fill mode: section-B entries per row of each slice
     int64_t n_total;
     uint32_t n_tiles;
 };
@@ -127,6 +135,11 @@ __device__ inline void fma_acc(float2 &acc, float2 c, float2 v) {
     acc.y = fmaf(c.x, v.y, acc.y);
     acc.y = fmaf(c.y, v.x, acc.y);
 }
+
+__device__ inline double vadd(double a, double b) { return a + b; }
+__device__ inline float vadd(float a, float b) { return a + b; }
+__device__ inline double2 vadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ inline float2 vadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 
 // Map the hardware dispatch order (round robin over XCDs) to contiguous tile
 // ranges per XCD, so neighbouring tiles share one XCD's L2.
@@ -225,8 +238,9 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
     const uint64_t shi = p.seg_hi[seg];
     const V *x = static_cast<const V *>(xin);
 
-    V acc = vzero(static_cast<V *>(nullptr));
-    unsigned long long n_hit = 0, n_probe = 0, n_look = 0;
+    V acc = vzero(static_cast<V *>(nullptr));       // diagonal + section-B groups
+    V acc_a = vzero(static_cast<V *>(nullptr));     // section-A groups
+    unsigned long long n_hit = 0, n_probe = 0, n_look = 0, n_hit_a = 0;
     V gv = vzero(static_cast<V *>(nullptr));
     if constexpr (MODE == MODE_GRAD) {
         if (active) gv = static_cast<const V *>(p.gy)[i];
@@ -272,10 +286,10 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
             uint32_t g0 = p.cls_goff[cls];
             const uint32_t g1 = p.cls_goff[cls + 1];
             if (pd.has_diag && g0 == pd.grp_base) g0 += 1;
-            auto handle = [&](uint32_t g, uint32_t base, uint32_t tl) {
+            auto handle = [&](uint32_t g, uint32_t base, uint32_t tl, bool sec_a) {
                 const bool hit = active && (s_keys[base] == tl);
                 if constexpr (MODE == MODE_MATVEC) {
-                    if (hit) fma_acc(acc, group_coef<R>(p, g, s), x[ta + base]);
+                    if (hit) fma_acc(sec_a ? acc_a : acc, group_coef<R>(p, g, s), x[ta + base]);
                 } else if constexpr (MODE == MODE_LOOKUP) {
                     if (hit) {
                         const size_t o = static_cast<size_t>(g) * p.n_total + i;
@@ -287,15 +301,19 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
                     if (hit) conj_mul(gv, x[ta + base], wre, wim);
                     grad_group<R>(p, g, s, hit, wre, wim);
                 } else if constexpr (MODE == MODE_ROWCOUNT) {
-                    n_hit += hit;
+                    if (sec_a) n_hit_a += hit;
+                    else n_hit += hit;
                 } else if constexpr (MODE == MODE_FILL) {
                     if (hit) {
-                        const int64_t slot = p.c_off[i / SLICE] + static_cast<int64_t>(n_hit) * SLICE + (i % SLICE);
+                        const int64_t k = sec_a ? static_cast<int64_t>(n_hit_a)
+                                                : static_cast<int64_t>(p.c_wa[i / SLICE]) + static_cast<int64_t>(n_hit);
+                        const int64_t slot = p.c_off[i / SLICE] + k * SLICE + (i % SLICE);
                         p.c_col[slot] = static_cast<int32_t>(ta + base);
                         const CC cg = group_coef<R>(p, g, s);
                         if constexpr (CPLX) static_cast<CC *>(p.c_val)[slot] = cg;
                         else static_cast<R *>(p.c_val)[slot] = cg.x;
-                        n_hit += 1;
+                        if (sec_a) n_hit_a += 1;
+                        else n_hit += 1;
                     }
                 } else {
                     n_hit += hit;
@@ -303,6 +321,18 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
                 }
             };
             uint32_t g = g0;
+            // groups are sorted by X mask, so a class's group without low flips
+            // (section A) is its first one
+            if (g < g1 && p.g_xlo[g] == 0) {
+                uint32_t base = 0, len = tn;
+                while (len > 1) {
+                    const uint32_t half = len >> 1;
+                    base = (s_keys[base + half] <= slo) ? base + half : base;
+                    len -= half;
+                }
+                handle(g, base, slo, true);
+                ++g;
+            }
             // four independent searches in lock step: the dependent LDS loads of
             // one search overlap with those of the other three
             for (; g + 4 <= g1; g += 4) {
@@ -317,10 +347,10 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
                     b3 = (s_keys[b3 + half] <= t3) ? b3 + half : b3;
                     len -= half;
                 }
-                handle(g, b0, t0);
-                handle(g + 1, b1, t1);
-                handle(g + 2, b2, t2);
-                handle(g + 3, b3, t3);
+                handle(g, b0, t0, false);
+                handle(g + 1, b1, t1, false);
+                handle(g + 2, b2, t2, false);
+                handle(g + 3, b3, t3, false);
             }
             for (; g < g1; ++g) {
                 const uint32_t tl = slo ^ p.g_xlo[g];
@@ -330,25 +360,31 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
                     base = (s_keys[base + half] <= tl) ? base + half : base;
                     len -= half;
                 }
-                handle(g, base, tl);
+                handle(g, base, tl, false);
             }
             __syncthreads();          // before the next target overwrites s_keys
         }
     }
 
     if constexpr (MODE == MODE_MATVEC) {
-        if (active) static_cast<V *>(yout)[i] = acc;
+        if (active) static_cast<V *>(yout)[i] = vadd(acc, acc_a);
     } else if constexpr (MODE == MODE_ROWCOUNT) {
-        if (active) p.rowcnt[i] = static_cast<uint32_t>(n_hit);
+        if (active) {
+            p.rowcnt[i] = static_cast<uint32_t>(n_hit);
+            p.rowcnt_a[i] = static_cast<uint32_t>(n_hit_a);
+        }
     } else if constexpr (MODE == MODE_FILL) {
         if (active) {
-            const uint32_t w = p.c_width[i / SLICE];
-            for (uint32_t k = static_cast<uint32_t>(n_hit); k < w; ++k) {
-                const int64_t slot = p.c_off[i / SLICE] + static_cast<int64_t>(k) * SLICE + (i % SLICE);
+            const int64_t o = p.c_off[i / SLICE] + (i % SLICE);
+            const uint32_t wa = p.c_wa[i / SLICE], wb = p.c_wb[i / SLICE];
+            auto pad = [&](uint32_t k) {
+                const int64_t slot = o + static_cast<int64_t>(k) * SLICE;
                 p.c_col[slot] = static_cast<int32_t>(i);
                 if constexpr (CPLX) static_cast<CC *>(p.c_val)[slot] = CC{R(0), R(0)};
                 else static_cast<R *>(p.c_val)[slot] = R(0);
-            }
+            };
+            for (uint32_t k = static_cast<uint32_t>(n_hit_a); k < wa; ++k) pad(k);
+            for (uint32_t k = wa + static_cast<uint32_t>(n_hit); k < wa + wb; ++k) pad(k);
         }
     } else if constexpr (MODE == MODE_COUNT) {
         // wave reduction, one atomic per wave and counter (the class lookups are
@@ -400,10 +436,20 @@ __global__ __launch_bounds__(TILE) void k_tiles_queue(const KParams p, const voi
 }
 
 
+// y[begin, end) = value; grid-stride, so any length fits a capped grid
 template <typename V>
 __global__ void k_fill(V *__restrict__ y, int64_t begin, int64_t end, V value) {
-    const int64_t i = begin + static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i < end) y[i] = value;
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+    for (int64_t i = begin + static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < end; i += stride)
+        y[i] = value;
+}
+
+template <typename V>
+hipError_t fill_async(V *y, int64_t begin, int64_t end, V value, hipStream_t st) {
+    if (end <= begin) return hipSuccess;
+    const int64_t blocks = std::min<int64_t>((end - begin + 255) / 256, int64_t(1) << 16);
+    hipLaunchKernelGGL(k_fill<V>, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, st, y, begin, end, value);
+    return hipGetLastError();
 }
 
 __global__ void k_narrow(const double2 *__restrict__ in, float2 *__restrict__ out, int64_t n) {
@@ -729,33 +775,10 @@ __device__ inline double2 stream_load(const double2 *p) {
     return make_double2(__builtin_nontemporal_load(&p->x), __builtin_nontemporal_load(&p->y));
 }
 
-// y = H x from the hit cache: one wave per 64-row slice, one row per lane,
-// entries read lane-contiguously (and marked non-temporal: they are read once
-// per matvec, x is the data worth keeping in L2); the same accumulation
-// sequence as k_tiles (diagonal first, then hits in group order), so results
-// are identical.
-template <typename R, bool CPLXV, bool CPLXC>
-__global__ __launch_bounds__(256) void k_spmv_sell(const int32_t *__restrict__ col, const void *__restrict__ val,
-                                                   const int64_t *__restrict__ off, const uint32_t *__restrict__ width,
-                                                   const void *__restrict__ diag, int64_t n_states,
-                                                   const void *__restrict__ xin, void *__restrict__ yout) {
-    using V = typename VT<R, CPLXV>::T;
-    using CC = typename C2<R>::T;
-    using CV = typename std::conditional<CPLXC, CC, R>::type;
-    constexpr int U = spmv_unroll<V, CV>();
-    const int64_t row = static_cast<int64_t>(spmv_block()) * blockDim.x + threadIdx.x;
-    if (row >= n_states) return;
-    // a wave's 64 rows are one slice (256-thread blocks start on slice bounds)
-    const int64_t slice = row / SLICE;
-    const int lane = static_cast<int>(row % SLICE);
-    const V *x = static_cast<const V *>(xin);
-    const CV *v = static_cast<const CV *>(val);
-    V acc = vzero(static_cast<V *>(nullptr));
-    if (diag) fma_acc(acc, static_cast<const CC *>(diag)[row], x[row]);   // NULL: no problem has an x = 0 group
-    const uint32_t w = __builtin_amdgcn_readfirstlane(width[slice]);
-    const int64_t e0 = off[slice];
-    const int32_t *c = col + e0 + lane;
-    const CV *vv = v + e0 + lane;
+// acc += the w entries of one section of a row, in order: column and value
+// loads of U entries, then their gathers of x, then the FMAs
+template <int U, typename V, typename CV>
+__device__ inline void spmv_section(V &acc, const int32_t *c, const CV *vv, uint32_t w, const V *__restrict__ x) {
     uint32_t k = 0;
     for (; k + U <= w; k += U) {
         int32_t ci[U];
@@ -778,7 +801,40 @@ __global__ __launch_bounds__(256) void k_spmv_sell(const int32_t *__restrict__ c
         c += SLICE;
         vv += SLICE;
     }
-    static_cast<V *>(yout)[row] = acc;
+}
+
+// y = H x from the hit cache: one wave per 64-row slice, one row per lane,
+// entries read lane-contiguously (and marked non-temporal: they are read once
+// per matvec, x is the data worth keeping in L2); the same accumulation
+// sequence as k_tiles (diagonal and section B, section A, their sum), so
+// results are identical.
+template <typename R, bool CPLXV, bool CPLXC>
+__global__ __launch_bounds__(256) void k_spmv_sell(const int32_t *__restrict__ col, const void *__restrict__ val,
+                                                   const int64_t *__restrict__ off, const uint32_t *__restrict__ wa,
+                                                   const uint32_t *__restrict__ wb, const void *__restrict__ diag,
+                                                   int64_t n_states, const void *__restrict__ xin,
+                                                   void *__restrict__ yout) {
+    using V = typename VT<R, CPLXV>::T;
+    using CC = typename C2<R>::T;
+    using CV = typename std::conditional<CPLXC, CC, R>::type;
+    constexpr int U = spmv_unroll<V, CV>();
+    const int64_t row = static_cast<int64_t>(spmv_block()) * blockDim.x + threadIdx.x;
+    if (row >= n_states) return;
+    // a wave's 64 rows are one slice (256-thread blocks start on slice bounds)
+    const int64_t slice = row / SLICE;
+    const int lane = static_cast<int>(row % SLICE);
+    const V *x = static_cast<const V *>(xin);
+    const CV *v = static_cast<const CV *>(val);
+    V acc = vzero(static_cast<V *>(nullptr));
+    if (diag) fma_acc(acc, static_cast<const CC *>(diag)[row], x[row]);   // NULL: no problem has an x = 0 group
+    const uint32_t w_a = __builtin_amdgcn_readfirstlane(wa[slice]);
+    const uint32_t w_b = __builtin_amdgcn_readfirstlane(wb[slice]);
+    const int64_t e0 = off[slice] + lane;
+    V acc_a = vzero(static_cast<V *>(nullptr));
+    spmv_section<U>(acc_a, col + e0, v + e0, w_a, x);
+    const int64_t e1 = e0 + static_cast<int64_t>(w_a) * SLICE;
+    spmv_section<U>(acc, col + e1, v + e1, w_b, x);
+    static_cast<V *>(yout)[row] = vadd(acc, acc_a);
 }
 
 __global__ void k_eigen_batch(const double *__restrict__ mats, int count, int n, double *__restrict__ out) {
@@ -897,7 +953,8 @@ struct HitCache {            // sliced-ELL cache of the hits (column, C_j(s_i)) 
     int32_t *col = nullptr;
     void *val = nullptr;
     int64_t *off = nullptr;
-    uint32_t *width = nullptr;
+    uint32_t *wa = nullptr;    // section-A width of each slice
+    uint32_t *wb = nullptr;    // section-B width of each slice
     int64_t entries = 0, nnz = 0, n_slices = 0, bytes = 0;
     int cplx = 0;
     bool built = false;
@@ -1320,7 +1377,8 @@ hipError_t fill_cache(const rqx_plan *p, int slot, hipStream_t st) {
     k.c_col = c.col;
     k.c_val = c.val;
     k.c_off = c.off;
-    k.c_width = c.width;
+    k.c_wa = c.wa;
+    k.c_wb = c.wb;
     if (slot == 1)
         return c.cplx ? launch_tiles<MODE_FILL, double, true>(p, k, nullptr, nullptr, st)
                       : launch_tiles<MODE_FILL, double, false>(p, k, nullptr, nullptr, st);
@@ -1332,11 +1390,11 @@ template <typename R, bool CPLXV>
 hipError_t spmv_launch(const rqx_plan *p, const HitCache &c, const void *diag, const void *x, void *y, hipStream_t st) {
     const unsigned blocks = static_cast<unsigned>((p->n_states + 255) / 256);
     if (c.cplx)
-        hipLaunchKernelGGL((k_spmv_sell<R, CPLXV, true>), dim3(blocks), dim3(256), 0, st, c.col, c.val, c.off, c.width,
-                           diag, p->n_states, x, y);
+        hipLaunchKernelGGL((k_spmv_sell<R, CPLXV, true>), dim3(blocks), dim3(256), 0, st, c.col, c.val, c.off, c.wa,
+                           c.wb, diag, p->n_states, x, y);
     else
-        hipLaunchKernelGGL((k_spmv_sell<R, CPLXV, false>), dim3(blocks), dim3(256), 0, st, c.col, c.val, c.off,
-                           c.width, diag, p->n_states, x, y);
+        hipLaunchKernelGGL((k_spmv_sell<R, CPLXV, false>), dim3(blocks), dim3(256), 0, st, c.col, c.val, c.off, c.wa,
+                           c.wb, diag, p->n_states, x, y);
     return hipGetLastError();
 }
 
@@ -1460,14 +1518,8 @@ int rqx_lookup(const rqx_plan *p, int32_t *idx, double *vals, rqx_stream_t strea
     hipStream_t st = static_cast<hipStream_t>(stream);
     const int64_t total = static_cast<int64_t>(p->n_groups) * p->n_total;
     if (total > 0) {
-        const unsigned blocks = static_cast<unsigned>((total + 255) / 256);
-        hipLaunchKernelGGL(k_fill<int32_t>, dim3(blocks), dim3(256), 0, st, idx, int64_t(0), total, int32_t(-1));
-        HIP_TRY(hipGetLastError());
-        if (vals) {
-            hipLaunchKernelGGL(k_fill<double2>, dim3(blocks), dim3(256), 0, st, reinterpret_cast<double2 *>(vals),
-                               int64_t(0), total, make_double2(0.0, 0.0));
-            HIP_TRY(hipGetLastError());
-        }
+        HIP_TRY(fill_async(idx, int64_t(0), total, int32_t(-1), st));
+        if (vals) HIP_TRY(fill_async(reinterpret_cast<double2 *>(vals), int64_t(0), total, make_double2(0.0, 0.0), st));
     }
     if (p->n_groups > 0) {
         KParams k = make_params(p, true);
@@ -1484,12 +1536,7 @@ int rqx_diagonal(const rqx_plan *p, double *diag, rqx_stream_t stream) {
     if (!diag && p->n_total > 0) return fail(RQX_ERR_ARG, "diag is NULL");
     hipStream_t st = static_cast<hipStream_t>(stream);
     double2 *d = reinterpret_cast<double2 *>(diag);
-    if (p->n_total > 0) {
-        const unsigned blocks = static_cast<unsigned>((p->n_total + 255) / 256);
-        hipLaunchKernelGGL(k_fill<double2>, dim3(blocks), dim3(256), 0, st, d, int64_t(0), p->n_total,
-                           make_double2(0.0, 0.0));
-        HIP_TRY(hipGetLastError());
-    }
+    if (p->n_total > 0) HIP_TRY(fill_async(d, int64_t(0), p->n_total, make_double2(0.0, 0.0), st));
     if (p->n_states > 0)
         HIP_TRY(hipMemcpyAsync(d, p->diag64, static_cast<size_t>(p->n_states) * 16, hipMemcpyDeviceToDevice, st));
     return RQX_OK;
@@ -1522,15 +1569,13 @@ int rqx_matvec(const rqx_plan *p, const void *x, void *y, int32_t dtype, rqx_str
         HIP_TRY(launch_dtype<MODE_MATVEC>(p, k, x, y, dtype, st));
     }
     if (p->n_total > p->n_states) {
-        const int64_t cnt = p->n_total - p->n_states;
-        const unsigned blocks = static_cast<unsigned>((cnt + 255) / 256);
+        const int64_t a = p->n_states, b = p->n_total;
         switch (dtype) {
-            case RQX_F64: hipLaunchKernelGGL(k_fill<double>, dim3(blocks), dim3(256), 0, st, static_cast<double *>(y), p->n_states, p->n_total, 0.0); break;
-            case RQX_C128: hipLaunchKernelGGL(k_fill<double2>, dim3(blocks), dim3(256), 0, st, static_cast<double2 *>(y), p->n_states, p->n_total, make_double2(0.0, 0.0)); break;
-            case RQX_F32: hipLaunchKernelGGL(k_fill<float>, dim3(blocks), dim3(256), 0, st, static_cast<float *>(y), p->n_states, p->n_total, 0.f); break;
-            default: hipLaunchKernelGGL(k_fill<float2>, dim3(blocks), dim3(256), 0, st, static_cast<float2 *>(y), p->n_states, p->n_total, make_float2(0.f, 0.f)); break;
+            case RQX_F64: HIP_TRY(fill_async(static_cast<double *>(y), a, b, 0.0, st)); break;
+            case RQX_C128: HIP_TRY(fill_async(static_cast<double2 *>(y), a, b, make_double2(0.0, 0.0), st)); break;
+            case RQX_F32: HIP_TRY(fill_async(static_cast<float *>(y), a, b, 0.f, st)); break;
+            default: HIP_TRY(fill_async(static_cast<float2 *>(y), a, b, make_float2(0.f, 0.f), st)); break;
         }
-        HIP_TRY(hipGetLastError());
     }
     return RQX_OK;
 }
@@ -1557,9 +1602,7 @@ int rqx_grad_coef(const rqx_plan *p, const void *x, const void *gy, double *grad
     if (p->n_states > 0 && (!x || !gy)) return fail(RQX_ERR_ARG, "x or gy is NULL");
     if (int rc = check_dtype(p, dtype)) return rc;
     hipStream_t st = static_cast<hipStream_t>(stream);
-    const unsigned fb = static_cast<unsigned>((p->n_terms + 255) / 256);
-    hipLaunchKernelGGL(k_fill<double>, dim3(fb), dim3(256), 0, st, grad, int64_t(0), p->n_terms, 0.0);
-    HIP_TRY(hipGetLastError());
+    HIP_TRY(fill_async(grad, int64_t(0), p->n_terms, 0.0, st));
     KParams k = make_params(p, is_fp64(dtype));
     k.gy = gy;
     k.grad = grad;
@@ -1618,39 +1661,44 @@ int rqx_plan_build_cache(rqx_plan *p, int32_t precision, int64_t max_bytes, rqx_
     if (n > INT32_MAX) return fail(RQX_ERR_LIMIT, "hit cache columns are int32: at most 2^31 - 1 rows");
     const int64_t n_slices = (n + SLICE - 1) / SLICE;
 
-    // pass 1: non-diagonal hits per row
+    // pass 1: non-diagonal hits per row, section B at [0, n), section A at [n, 2n)
     uint32_t *rowcnt = nullptr;
-    HIP_TRY(hipMalloc(&rowcnt, static_cast<size_t>(n) * 4));
+    HIP_TRY(hipMalloc(&rowcnt, static_cast<size_t>(n) * 8));
     KParams k = make_params(p, true);
     k.rowcnt = rowcnt;
+    k.rowcnt_a = rowcnt + n;
     hipError_t e = launch_tiles<MODE_ROWCOUNT, double, true>(p, k, nullptr, nullptr, st);
-    std::vector<uint32_t> cnt(static_cast<size_t>(n));
-    if (e == hipSuccess) e = hipMemcpyAsync(cnt.data(), rowcnt, static_cast<size_t>(n) * 4, hipMemcpyDeviceToHost, st);
+    std::vector<uint32_t> cnt(static_cast<size_t>(2 * n));
+    if (e == hipSuccess) e = hipMemcpyAsync(cnt.data(), rowcnt, static_cast<size_t>(n) * 8, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     const hipError_t ef = hipFree(rowcnt);
     if (e == hipSuccess) e = ef;
     if (e != hipSuccess) return fail(RQX_ERR_HIP, std::string("cache row count: ") + hipGetErrorString(e));
 
-    // slice widths and offsets
-    std::vector<uint32_t> width(static_cast<size_t>(n_slices), 0u);
+    // per-slice section widths and offsets
+    std::vector<uint32_t> wa(static_cast<size_t>(n_slices), 0u), wb(static_cast<size_t>(n_slices), 0u);
     std::vector<int64_t> off(static_cast<size_t>(n_slices), 0);
     int64_t entries = 0, nnz = 0;
     for (int64_t s = 0; s < n_slices; ++s) {
-        uint32_t w = 0;
+        uint32_t w_a = 0, w_b = 0;
         for (int64_t r = s * SLICE; r < std::min(n, (s + 1) * SLICE); ++r) {
-            w = std::max(w, cnt[static_cast<size_t>(r)]);
-            nnz += cnt[static_cast<size_t>(r)];
+            const uint32_t cb = cnt[static_cast<size_t>(r)], ca = cnt[static_cast<size_t>(n + r)];
+            w_b = std::max(w_b, cb);
+            w_a = std::max(w_a, ca);
+            nnz += static_cast<int64_t>(ca) + cb;
         }
-        width[static_cast<size_t>(s)] = w;
+        wa[static_cast<size_t>(s)] = w_a;
+        wb[static_cast<size_t>(s)] = w_b;
         off[static_cast<size_t>(s)] = entries;
-        entries += static_cast<int64_t>(w) * SLICE;
+        entries += static_cast<int64_t>(w_a + w_b) * SLICE;
     }
     const int cplx = p->is_real ? 0 : 1;
     const size_t vb = cache_val_bytes(slot, cplx);
     const size_t o_col = 0, o_val = align_up(static_cast<size_t>(entries) * 4),
                  o_off = o_val + align_up(static_cast<size_t>(entries) * vb),
-                 o_w = o_off + align_up(static_cast<size_t>(n_slices) * 8),
-                 total = o_w + align_up(static_cast<size_t>(n_slices) * 4);
+                 o_wa = o_off + align_up(static_cast<size_t>(n_slices) * 8),
+                 o_wb = o_wa + align_up(static_cast<size_t>(n_slices) * 4),
+                 total = o_wb + align_up(static_cast<size_t>(n_slices) * 4);
     size_t free_b = 0, total_b = 0;
     HIP_TRY(hipMemGetInfo(&free_b, &total_b));
     const int64_t limit = max_bytes > 0 ? max_bytes : static_cast<int64_t>(free_b / 10 * 9);
@@ -1664,7 +1712,8 @@ int rqx_plan_build_cache(rqx_plan *p, int32_t precision, int64_t max_bytes, rqx_
     c.col = reinterpret_cast<int32_t *>(b + o_col);
     c.val = b + o_val;
     c.off = reinterpret_cast<int64_t *>(b + o_off);
-    c.width = reinterpret_cast<uint32_t *>(b + o_w);
+    c.wa = reinterpret_cast<uint32_t *>(b + o_wa);
+    c.wb = reinterpret_cast<uint32_t *>(b + o_wb);
     c.entries = entries;
     c.nnz = nnz;
     c.n_slices = n_slices;
@@ -1672,7 +1721,8 @@ int rqx_plan_build_cache(rqx_plan *p, int32_t precision, int64_t max_bytes, rqx_
     c.cplx = cplx;
     c.built = true;
     e = hipMemcpyAsync(c.off, off.data(), off.size() * 8, hipMemcpyHostToDevice, st);
-    if (e == hipSuccess) e = hipMemcpyAsync(c.width, width.data(), width.size() * 4, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(c.wa, wa.data(), wa.size() * 4, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(c.wb, wb.data(), wb.size() * 4, hipMemcpyHostToDevice, st);
     p->cache[slot] = c;
     // pass 2: columns and values in the order the tile kernel accumulates them
     if (e == hipSuccess) e = fill_cache(p, slot, st);
@@ -1785,9 +1835,7 @@ int rqx_batch_dots(const rqx_plan *p, int32_t n_pairs, const void *const *a, con
     }
     hipStream_t st = static_cast<hipStream_t>(stream);
     const int64_t n_out = 2 * static_cast<int64_t>(p->probs.size()) * n_pairs;
-    hipLaunchKernelGGL(k_fill<double>, dim3(static_cast<unsigned>((n_out + 255) / 256)), dim3(256), 0, st, out,
-                       int64_t(0), n_out, 0.0);
-    HIP_TRY(hipGetLastError());
+    HIP_TRY(fill_async(out, int64_t(0), n_out, 0.0, st));
     switch (dtype) {
         case RQX_F64: return dots_dispatch<double>(p, n_pairs, pk, out, st);
         case RQX_C128: return dots_dispatch<double2>(p, n_pairs, pk, out, st);

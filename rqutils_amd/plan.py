@@ -244,7 +244,8 @@ class Plan:
 
         The reference caches source indices and coefficient sums for all ``J x N``
         (group, row) pairs (``sqd.py:343-407``); this cache keeps only the hits,
-        as sliced ELL (64 rows per slice).  ``precision`` 64 serves float64 and
+        as sliced ELL (64 rows per slice, hits of groups without low-bit flips in
+        a section of their own so that their gathers coalesce).  ``precision`` 64 serves float64 and
         complex128 matvecs, 32 float32 and complex64.  Later ``matvec`` calls of
         that precision read the cache and return the same values as the
         matrix-free kernel.  Raises ``RqxError`` when the cache needs more than
@@ -275,7 +276,7 @@ class Plan:
     def cached_bytes(self, dtype):
         """Bytes one cached matvec must move: the cache, the diagonal, ``x`` once and ``y``.
 
-        ``entries (4 + s_val) + 12 slices + N (s_d + 2 s_v)`` with ``s_val`` the
+        ``entries (4 + s_val) + 16 slices + N (s_d + 2 s_v)`` with ``s_val`` the
         stored value bytes, ``s_d`` the diagonal entry bytes (complex) and
         ``s_v`` the vector element bytes; gathers of ``x`` past its first read
         are counted as cache hits.
@@ -287,7 +288,7 @@ class Plan:
         s_v = torch.empty(0, dtype=dtype).element_size()
         s_val = (8 if precision == 64 else 4) * (2 if info['complex'] else 1)
         s_d = 16 if precision == 64 else 8
-        return info['entries'] * (4 + s_val) + 12 * info['slices'] + self.n_states * (s_d + 2 * s_v)
+        return info['entries'] * (4 + s_val) + 16 * info['slices'] + self.n_states * (s_d + 2 * s_v)
 
     # ------------------------------------------------------------------ per-problem vector ops
     def dots(self, pairs):

@@ -65,13 +65,15 @@ def test_matvec_c128_cfg0_all_splits(cfg0, split):
     assert _rel(y, want) < RTOL64
 
 
-def test_matvec_bitwise_identical_across_splits(cfg0):
+def test_matvec_split_invariance(cfg0):
+    """The split only moves work between the two accumulation sections: same values to rounding."""
     _say('matvec split invariance')
     rng = np.random.default_rng(3)
     v = torch.from_numpy(rng.standard_normal(8192) + 1j * rng.standard_normal(8192)).to(DEV)
     ys = [Plan(cfg0.tables, cfg0.states, split_bits=s).matvec(v) for s in (0, 5, 11, 13)]
+    scale = float(ys[0].abs().max())
     for y in ys[1:]:
-        assert torch.equal(y, ys[0])
+        assert float((y - ys[0]).abs().max()) <= 1e-14 * scale
 
 
 def test_matvec_c64_cfg0(cfg0):
@@ -262,3 +264,18 @@ def test_hermitian_and_linear_cfg1_full():
             val = np.sum(cs * (1. - 2. * (np.bitwise_count(keys[pos] & zs) & 1)))
             want[pos] += val.real
     assert _rel(col, want) < RTOL64
+
+
+def test_lookup_table_beyond_2_32_entries():
+    """J x N above 2^32 entries (a 26 GB table): every miss, up to the last entry, reads -1."""
+    _say('lookup J*N > 2^32')
+    w = workloads.config1(n_alpha=600, n_beta=600)
+    plan = Plan(w.tables, w.states)
+    total = plan.info['n_groups'] * plan.n_total
+    assert total > 2 ** 32
+    idx = plan.lookup()
+    hits = plan.count()['hits']
+    assert int((idx >= 0).sum()) == hits
+    assert int(idx[-1].min()) >= -1 and int(idx.max()) < plan.n_states
+    del idx
+    torch.cuda.empty_cache()
