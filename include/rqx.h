@@ -229,6 +229,30 @@ int rqx_plan_cache_info(const rqx_plan *plan, int32_t precision, int64_t *out);
 /* Whether rqx_matvec uses a built cache (default 1); 0 forces the matrix-free kernel. */
 int rqx_plan_use_cache(rqx_plan *plan, int32_t enable);
 
+/* doubles of per-problem work space of rqx_locg_step */
+enum { RQX_LOCG_WORK = 168 };
+
+/*
+ * One LOCG iteration for every problem of the plan (a batched plan, or a single
+ * plan as one problem), with no host synchronisation, so a caller can capture
+ * it in a HIP graph.  The step is the reference's 3 x 3 Rayleigh-Ritz body
+ * (rqutils/ground_locg.py:197-423: residual, direction orthogonal to x and the
+ * previous direction, Rayleigh-Ritz on [x, y, p], x and y updated and
+ * re-orthonormalised) with the images A x, A y carried instead of recomputed,
+ * so it costs one matvec.  No counterpart in the reference (one problem per
+ * call, four matvecs per iteration).
+ *   dtype: RQX_F64 or RQX_C128 (the vectors); low != 0: the matvec runs in
+ *     RQX_F32 / RQX_C64 on low copies of p and A p;
+ *   vec: device vectors {x, Ax, y, Ay, p, Ap, p_low, Ap_low} over all rows
+ *     (p_low, Ap_low only with low); x, Ax, y, Ay are updated in place;
+ *   theta, rel: device, one double per problem: <x, A x> and |A x - theta x| /
+ *     |theta| at the start of the step; active: device, one byte per problem,
+ *     cleared when rel <= tol; an inactive problem's x and A x do not change;
+ *   work: device, RQX_LOCG_WORK doubles per problem.
+ */
+int rqx_locg_step(const rqx_plan *plan, int32_t dtype, int32_t low, void *const *vec, double *theta, double *rel,
+                  uint8_t *active, double *work, double tol, rqx_stream_t stream);
+
 /*
  * Lowest eigenpair of a small Hermitian matrix (n = 2 or 3), row-major,
  * complex interleaved, device pointers.  out = [theta, 0, k0re, k0im, ...].

@@ -26,7 +26,9 @@
 #include <cstring>
 #include <memory>
 #include <string>
+#include <initializer_list>
 #include <type_traits>
+#include <utility>
 #include <vector>
 
 #include "rqx.h"
@@ -737,6 +739,202 @@ __global__ __launch_bounds__(256) void k_batch_combine(PtrPack pk, const int64_t
             si += cr[q] * vi + ci[q] * vr;
         }
         store_c(static_cast<V *>(out)[i], sr, si);
+    }
+}
+
+// ---------------------------------------------------------------- fused LOCG step
+// One pass over the rows of every problem: outputs o = sum_q coef[o][q] in_q
+// (per-problem complex coefficients, written by the previous scalar stage),
+// and complex dots conj(u) v over inputs and outputs accumulated per problem.
+// Outputs may alias inputs (each element is read before it is written).  With
+// LO_IN the last input is stored in the low precision type LV; with LO_OUT the
+// last output is.
+constexpr int LP_IN = 6, LP_OUT = 4, LP_DOT = 9;
+
+struct LPass {
+    const void *in[LP_IN];
+    void *out[LP_OUT];
+    const double *coef;      // problem k: coef + k * stride, [o][q] complex
+    double *dots;            // problem k: dots + k * stride, one complex per dot
+    int32_t stride;          // doubles of per-problem work space
+    int8_t da[LP_DOT], db[LP_DOT];   // dot operands: 0..NIN-1 inputs, NIN + o outputs
+};
+
+template <typename V, typename LV, int NIN, int NOUT, int NDOT, bool LO_IN, bool LO_OUT>
+__global__ __launch_bounds__(256) void k_locg_pass(LPass a, const int64_t *__restrict__ rows) {
+    const int k = blockIdx.y;
+    const int64_t r0 = rows[k], r1 = rows[k + 1];
+    const double *cf = a.coef + static_cast<int64_t>(k) * a.stride;
+    double cr[NOUT > 0 ? NOUT : 1][NIN], ci[NOUT > 0 ? NOUT : 1][NIN];
+#pragma unroll
+    for (int o = 0; o < NOUT; ++o)
+#pragma unroll
+        for (int q = 0; q < NIN; ++q) {
+            cr[o][q] = cf[2 * (o * NIN + q)];
+            ci[o][q] = cf[2 * (o * NIN + q) + 1];
+        }
+    double dre[NDOT > 0 ? NDOT : 1], dim_[NDOT > 0 ? NDOT : 1];
+#pragma unroll
+    for (int d = 0; d < NDOT; ++d) dre[d] = dim_[d] = 0.0;
+    for (int64_t i = r0 + static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x; i < r1;
+         i += static_cast<int64_t>(gridDim.x) * 256) {
+        double vr[NIN + NOUT], vi[NIN + NOUT];
+#pragma unroll
+        for (int q = 0; q < NIN; ++q) {
+            if (LO_IN && q == NIN - 1) load_c(static_cast<const LV *>(a.in[q])[i], vr[q], vi[q]);
+            else load_c(static_cast<const V *>(a.in[q])[i], vr[q], vi[q]);
+        }
+#pragma unroll
+        for (int o = 0; o < NOUT; ++o) {
+            double sr = 0.0, si = 0.0;
+#pragma unroll
+            for (int q = 0; q < NIN; ++q) {
+                sr += cr[o][q] * vr[q] - ci[o][q] * vi[q];
+                si += cr[o][q] * vi[q] + ci[o][q] * vr[q];
+            }
+            if (LO_OUT && o == NOUT - 1) {
+                store_c(static_cast<LV *>(a.out[o])[i], sr, si);
+            } else {
+                store_c(static_cast<V *>(a.out[o])[i], sr, si);
+                // what was stored (real vectors drop the imaginary part)
+                load_c(static_cast<V *>(a.out[o])[i], sr, si);
+            }
+            vr[NIN + o] = sr;
+            vi[NIN + o] = si;
+        }
+#pragma unroll
+        for (int d = 0; d < NDOT; ++d) {
+            // operands picked by uniform selects (no dynamically indexed register arrays)
+            double ar = 0.0, ai = 0.0, br = 0.0, bi = 0.0;
+#pragma unroll
+            for (int j = 0; j < NIN + NOUT; ++j) {
+                if (a.da[d] == j) {
+                    ar = vr[j];
+                    ai = vi[j];
+                }
+                if (a.db[d] == j) {
+                    br = vr[j];
+                    bi = vi[j];
+                }
+            }
+            dre[d] += ar * br + ai * bi;
+            dim_[d] += ar * bi - ai * br;
+        }
+    }
+    if constexpr (NDOT > 0) {
+        __shared__ double s_red[4][2 * LP_DOT];
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+        for (int d = 0; d < NDOT; ++d) {
+            const double sr = wave_sum(dre[d]), si = wave_sum(dim_[d]);
+            if (lane == 0) {
+                s_red[wave][2 * d] = sr;
+                s_red[wave][2 * d + 1] = si;
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x < 2 * NDOT) {
+            const double v = s_red[0][threadIdx.x] + s_red[1][threadIdx.x] + s_red[2][threadIdx.x] +
+                             s_red[3][threadIdx.x];
+            if (v != 0.0) atomicAdd(&a.dots[static_cast<int64_t>(k) * a.stride + threadIdx.x], v);
+        }
+    }
+}
+
+// Per-problem work space of the fused step (doubles, complex slots at 2 * slot):
+// dot slots 0..19, coefficient blocks after them.
+enum {
+    LW_D_XAX = 0, LW_D_RR = 1, LW_D_XR = 2, LW_D_YR = 3, LW_D_XP = 4, LW_D_YP = 5, LW_D_PP = 6,
+    LW_D_RR9 = 7, LW_D_XX = 16, LW_D_XY = 17, LW_D_YY = 18, LW_D_YY2 = 19,
+    LW_C2 = 20, LW_C3 = 23, LW_C4 = 26, LW_C5 = 32, LW_C6 = 38, LW_C7 = 62, LW_C8 = 78,
+    LW_SLOTS = 82
+};
+constexpr int LOCG_WORK = 2 * LW_SLOTS + 4;   // doubles per problem
+static_assert(LOCG_WORK == RQX_LOCG_WORK, "work size in rqx.h");
+
+__device__ inline void put_c(double *w, int slot, double re, double im) {
+    w[2 * slot] = re;
+    w[2 * slot + 1] = im;
+}
+
+// Scalar stages between the passes: one thread per problem.
+__global__ void k_locg_scalar(int stage, int K, double *__restrict__ work, double *__restrict__ theta,
+                              double *__restrict__ rel, uint8_t *__restrict__ active, double tol, int real) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= K) return;
+    double *w = work + static_cast<int64_t>(k) * LOCG_WORK;
+    auto re = [&](int slot) { return w[2 * slot]; };
+    auto im = [&](int slot) { return w[2 * slot + 1]; };
+    if (stage == 1) {                  // theta = <x, Ax>;  r = Ax - theta x
+        const double th = re(LW_D_XAX);
+        theta[k] = th;
+        put_c(w, LW_C2 + 0, 1.0, 0.0);
+        put_c(w, LW_C2 + 1, -th, 0.0);
+        put_c(w, LW_C2 + 2, 0.0, 0.0);
+    } else if (stage == 2) {           // residual norm, freeze; first Gram-Schmidt pass
+        const double rn = sqrt(fmax(re(LW_D_RR), 0.0));
+        const double rl = rn / fmax(fabs(theta[k]), 1e-300);
+        rel[k] = rl;
+        if (!(rl > tol)) active[k] = 0;
+        put_c(w, LW_C3 + 0, 1.0, 0.0);
+        put_c(w, LW_C3 + 1, -re(LW_D_XR), -im(LW_D_XR));
+        put_c(w, LW_C3 + 2, -re(LW_D_YR), -im(LW_D_YR));
+    } else if (stage == 3) {           // second pass, normalised (both the vector and its low copy)
+        const double ar = re(LW_D_XP), ai = im(LW_D_XP), br = re(LW_D_YP), bi = im(LW_D_YP);
+        const double n2 = re(LW_D_PP) - (ar * ar + ai * ai) - (br * br + bi * bi);
+        const double inv = n2 > 0.0 ? 1.0 / sqrt(n2) : 0.0;
+        for (int o = 0; o < 2; ++o) {
+            put_c(w, LW_C4 + 3 * o + 0, inv, 0.0);
+            put_c(w, LW_C4 + 3 * o + 1, -ar * inv, -ai * inv);
+            put_c(w, LW_C4 + 3 * o + 2, -br * inv, -bi * inv);
+        }
+        for (int q = 0; q < 6; ++q) put_c(w, LW_C5 + q, q == 5 ? 1.0 : 0.0, 0.0);
+    } else if (stage == 5) {           // Rayleigh-Ritz on [x, y, p] -> coefficients
+        double m[18], out[8];
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) {
+                const int a = LW_D_RR9 + 3 * r + c, b = LW_D_RR9 + 3 * c + r;
+                m[2 * (3 * r + c)] = 0.5 * (re(a) + re(b));
+                m[2 * (3 * r + c) + 1] = 0.5 * (im(a) - im(b));
+            }
+        eigen_small(m, 3, out);
+        double c0r = out[2], c0i = out[3], c1r = out[4], c1i = out[5], c2r = out[6], c2i = out[7];
+        if (real) c0i = c1i = c2i = 0.0;
+        if (!active[k]) {
+            c0r = 1.0;
+            c0i = c1r = c1i = c2r = c2i = 0.0;
+        }
+        for (int q = 0; q < 24; ++q) put_c(w, LW_C6 + q, 0.0, 0.0);
+        // inputs (x, ax, y, ay, p, ap) -> outputs (y, ay, x, ax)
+        put_c(w, LW_C6 + 0 * 6 + 2, c1r, c1i);
+        put_c(w, LW_C6 + 0 * 6 + 4, c2r, c2i);
+        put_c(w, LW_C6 + 1 * 6 + 3, c1r, c1i);
+        put_c(w, LW_C6 + 1 * 6 + 5, c2r, c2i);
+        put_c(w, LW_C6 + 2 * 6 + 0, c0r, c0i);
+        put_c(w, LW_C6 + 2 * 6 + 2, c1r, c1i);
+        put_c(w, LW_C6 + 2 * 6 + 4, c2r, c2i);
+        put_c(w, LW_C6 + 3 * 6 + 1, c0r, c0i);
+        put_c(w, LW_C6 + 3 * 6 + 3, c1r, c1i);
+        put_c(w, LW_C6 + 3 * 6 + 5, c2r, c2i);
+    } else if (stage == 6) {           // normalise x; y minus its projection on x
+        const double nx2 = re(LW_D_XX);
+        const double inv = nx2 > 0.0 ? 1.0 / sqrt(nx2) : 0.0;
+        const double pr = nx2 > 0.0 ? re(LW_D_XY) / nx2 : 0.0, pi = nx2 > 0.0 ? im(LW_D_XY) / nx2 : 0.0;
+        for (int q = 0; q < 16; ++q) put_c(w, LW_C7 + q, 0.0, 0.0);
+        // inputs (x, ax, y, ay) -> outputs (x, ax, y, ay)
+        put_c(w, LW_C7 + 0 * 4 + 0, inv, 0.0);
+        put_c(w, LW_C7 + 1 * 4 + 1, inv, 0.0);
+        put_c(w, LW_C7 + 2 * 4 + 0, -pr, -pi);
+        put_c(w, LW_C7 + 2 * 4 + 2, 1.0, 0.0);
+        put_c(w, LW_C7 + 3 * 4 + 1, -pr, -pi);
+        put_c(w, LW_C7 + 3 * 4 + 3, 1.0, 0.0);
+    } else if (stage == 7) {           // normalise y and its image
+        const double ny2 = re(LW_D_YY2);
+        const double inv = ny2 > 0.0 ? 1.0 / sqrt(ny2) : 0.0;
+        put_c(w, LW_C8 + 0, inv, 0.0);
+        put_c(w, LW_C8 + 1, 0.0, 0.0);
+        put_c(w, LW_C8 + 2, 0.0, 0.0);
+        put_c(w, LW_C8 + 3, inv, 0.0);
     }
 }
 
@@ -1863,6 +2061,110 @@ int rqx_batch_combine(const rqx_plan *p, int32_t n_in, const void *const *v, con
         case RQX_F32: return combine_dispatch<float>(p, n_in, pk, coef, out, st);
         default: return combine_dispatch<float2>(p, n_in, pk, coef, out, st);
     }
+}
+
+}  // extern "C"
+
+namespace {
+template <typename V, typename LV, int NIN, int NOUT, int NDOT, bool LO_IN, bool LO_OUT>
+hipError_t lpass(const rqx_plan *p, const LPass &a, unsigned chunks, hipStream_t st) {
+    hipLaunchKernelGGL((k_locg_pass<V, LV, NIN, NOUT, NDOT, LO_IN, LO_OUT>), dim3(chunks, p->probs.size()), dim3(256),
+                       0, st, a, p->prob_rows);
+    return hipGetLastError();
+}
+
+LPass make_pass(double *work, int coef_slot, std::initializer_list<const void *> in,
+                std::initializer_list<void *> out, std::initializer_list<std::pair<int, int>> dots, int dot_slot) {
+    LPass a{};
+    int q = 0;
+    for (const void *v : in) a.in[q++] = v;
+    q = 0;
+    for (void *v : out) a.out[q++] = v;
+    a.coef = work + 2 * coef_slot;
+    a.dots = work + 2 * dot_slot;
+    a.stride = LOCG_WORK;
+    q = 0;
+    for (const auto &d : dots) {
+        a.da[q] = static_cast<int8_t>(d.first);
+        a.db[q] = static_cast<int8_t>(d.second);
+        ++q;
+    }
+    return a;
+}
+
+// One fused LOCG step (see rqx_locg_step) for vector type V (low copy type LV).
+template <typename V, typename LV>
+int locg_step(const rqx_plan *p, bool low, bool real, void *const *vec, double *theta, double *rel, uint8_t *active,
+              double *work, double tol, hipStream_t st) {
+    void *x = vec[0], *ax = vec[1], *y = vec[2], *ay = vec[3], *pv = vec[4], *ap = vec[5], *plo = vec[6], *aplo = vec[7];
+    const int K = static_cast<int>(p->probs.size());
+    const unsigned chunks = batch_chunks(p);
+    const unsigned sb = static_cast<unsigned>((K + 63) / 64);
+    const bool rv = std::is_same<V, double>::value;
+    auto scalar = [&](int stage) {
+        hipLaunchKernelGGL(k_locg_scalar, dim3(sb), dim3(64), 0, st, stage, K, work, theta, rel, active, tol,
+                           real ? 1 : 0);
+        return hipGetLastError();
+    };
+    // the step's dot slots start at zero; coefficient slots are written by each stage before use
+    HIP_TRY(fill_async(work, int64_t(0), static_cast<int64_t>(K) * LOCG_WORK, 0.0, st));
+    // 1: theta = <x, Ax>
+    HIP_TRY((lpass<V, LV, 2, 0, 1, false, false>(p, make_pass(work, 0, {x, ax}, {}, {{0, 1}}, LW_D_XAX), chunks, st)));
+    HIP_TRY(scalar(1));
+    // 2: r = Ax - theta x (into p); <r, r>, <x, r>, <y, r>
+    HIP_TRY((lpass<V, LV, 3, 1, 3, false, false>(
+        p, make_pass(work, LW_C2, {ax, x, y}, {pv}, {{3, 3}, {1, 3}, {2, 3}}, LW_D_RR), chunks, st)));
+    HIP_TRY(scalar(2));
+    // 3: p = r - <x,r> x - <y,r> y; <x, p>, <y, p>, <p, p>
+    HIP_TRY((lpass<V, LV, 3, 1, 3, false, false>(
+        p, make_pass(work, LW_C3, {pv, x, y}, {pv}, {{1, 3}, {2, 3}, {3, 3}}, LW_D_XP), chunks, st)));
+    HIP_TRY(scalar(3));
+    // 4: second Gram-Schmidt pass, normalised (plus the low copy for the bulk matvec); Ap; 5: Rayleigh-Ritz dots
+    if (low) {
+        HIP_TRY((lpass<V, LV, 3, 2, 0, false, true>(p, make_pass(work, LW_C4, {pv, x, y}, {pv, plo}, {}, 0), chunks, st)));
+        if (int rc = rqx_matvec(p, plo, aplo, rv ? RQX_F32 : RQX_C64, st)) return rc;
+        HIP_TRY((lpass<V, LV, 6, 1, 9, true, false>(
+            p, make_pass(work, LW_C5, {x, y, pv, ax, ay, aplo}, {ap},
+                         {{0, 3}, {0, 4}, {0, 6}, {1, 3}, {1, 4}, {1, 6}, {2, 3}, {2, 4}, {2, 6}}, LW_D_RR9),
+            chunks, st)));
+    } else {
+        HIP_TRY((lpass<V, LV, 3, 1, 0, false, false>(p, make_pass(work, LW_C4, {pv, x, y}, {pv}, {}, 0), chunks, st)));
+        if (int rc = rqx_matvec(p, pv, ap, rv ? RQX_F64 : RQX_C128, st)) return rc;
+        HIP_TRY((lpass<V, LV, 6, 0, 9, false, false>(
+            p, make_pass(work, 0, {x, y, pv, ax, ay, ap}, {},
+                         {{0, 3}, {0, 4}, {0, 5}, {1, 3}, {1, 4}, {1, 5}, {2, 3}, {2, 4}, {2, 5}}, LW_D_RR9),
+            chunks, st)));
+    }
+    HIP_TRY(scalar(5));
+    // 6: y' = c1 y + c2 p, Ay' likewise, x' = c0 x + y', Ax' likewise; <x', x'>, <x', y'>
+    HIP_TRY((lpass<V, LV, 6, 4, 2, false, false>(
+        p, make_pass(work, LW_C6, {x, ax, y, ay, pv, ap}, {y, ay, x, ax}, {{8, 8}, {8, 6}}, LW_D_XX), chunks, st)));
+    HIP_TRY(scalar(6));
+    // 7: x = x' / |x'|, Ax likewise; y = y' - <x, y'> x, Ay likewise; <y, y>
+    HIP_TRY((lpass<V, LV, 4, 4, 1, false, false>(
+        p, make_pass(work, LW_C7, {x, ax, y, ay}, {x, ax, y, ay}, {{6, 6}}, LW_D_YY2), chunks, st)));
+    HIP_TRY(scalar(7));
+    // 8: y and Ay normalised
+    HIP_TRY((lpass<V, LV, 2, 2, 0, false, false>(p, make_pass(work, LW_C8, {y, ay}, {y, ay}, {}, 0), chunks, st)));
+    return RQX_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int rqx_locg_step(const rqx_plan *p, int32_t dtype, int32_t low, void *const *vec, double *theta, double *rel,
+                  uint8_t *active, double *work, double tol, rqx_stream_t stream) {
+    g_err.clear();
+    if (int rc = check_plan(p)) return rc;
+    if (dtype != RQX_F64 && dtype != RQX_C128) return fail(RQX_ERR_ARG, "dtype must be RQX_F64 or RQX_C128");
+    if (int rc = check_dtype(p, dtype)) return rc;
+    if (!vec || !theta || !rel || !active || !work) return fail(RQX_ERR_ARG, "NULL argument");
+    for (int q = 0; q < (low ? 8 : 6); ++q)
+        if (!vec[q]) return fail(RQX_ERR_ARG, "NULL vector pointer");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    if (dtype == RQX_F64)
+        return locg_step<double, float>(p, low != 0, true, vec, theta, rel, active, work, tol, st);
+    return locg_step<double2, float2>(p, low != 0, false, vec, theta, rel, active, work, tol, st);
 }
 
 int rqx_eigenpair_batch(const double *mats, int32_t count, int32_t n, double *out, rqx_stream_t stream) {

@@ -7,19 +7,22 @@ No counterpart in the reference (``ground_locg`` solves one problem per call,
 * one batched matvec per iteration (one launch for all problems, persistent
   tile queue), complex64 / float32 in the bulk phase, the vector dtype in the
   refinement phase;
-* per-problem inner products in one fused pass (``rqx_batch_dots``), vector
-  updates with per-problem coefficients (``rqx_batch_combine``), and the 3x3
-  Rayleigh-Ritz eigenpairs of all problems in one launch;
+* the step is ``rqx_locg_step``: eight fused passes over the vectors (each a
+  linear update with per-problem coefficients plus the inner products the
+  next stage needs), small per-problem scalar kernels between them (the 3x3
+  Rayleigh-Ritz eigenpairs among them) and the matvec;
 * converged problems (relative residual <= ``tol``) are frozen: their
   coefficients become (1, 0, 0) so their vectors stop changing;
 * no host synchronisation inside a step, so the step is captured once in a
   HIP graph (``torch.cuda.CUDAGraph``) and replayed; convergence is polled
   every ``check_every`` steps.
 """
+import ctypes
 import time
 
 import torch
 
+from rqutils_amd import _capi
 from rqutils_amd.plan import Plan, eigenpair_batch
 
 _LO = {torch.float64: torch.float32, torch.complex128: torch.complex64}
@@ -49,6 +52,8 @@ class BatchLOCG:
         self.rel = torch.full((self.K,), float('inf'), dtype=torch.float64, device=dev)
         self.one = torch.ones(self.K, dtype=torch.complex128, device=dev)
         self.zero = torch.zeros(self.K, dtype=torch.complex128, device=dev)
+        self.work = torch.zeros(self.K * _capi.RQX_LOCG_WORK, dtype=torch.float64, device=dev)
+        self._vec_ptrs = None
         self.phase = 'lo' if mixed else 'hi'
         self.x.copy_(x0)
         self._normalize(self.x, None)
@@ -122,7 +127,22 @@ class BatchLOCG:
         self._update(c)
 
     def step(self):
-        """One LOCG iteration for every problem (no host synchronisation)."""
+        """One LOCG iteration for every problem (no host synchronisation): ``rqx_locg_step``.
+
+        Eight fused passes over the vectors, scalar stages on the device and one
+        matvec; the same sequence as ``step_reference``.
+        """
+        if self._vec_ptrs is None:
+            ptrs = [t.data_ptr() for t in (self.x, self.ax, self.y, self.ay, self.p, self.ap, self.p_lo, self.ap_lo)]
+            self._vec_ptrs = (ctypes.c_void_p * 8)(*ptrs)
+        dtype = _capi.RQX_C128 if self.hi == torch.complex128 else _capi.RQX_F64
+        stream = _capi.current_stream(self.plan.device)
+        _capi.check(_capi.lib().rqx_locg_step(self.plan._handle, dtype, int(self.phase == 'lo'), self._vec_ptrs,
+                                              self.theta.data_ptr(), self.rel.data_ptr(), self.active.data_ptr(),
+                                              self.work.data_ptr(), float(self.tol), stream))
+
+    def step_reference(self):
+        """The same iteration from the per-problem vector kernels and torch (one launch per operation)."""
         plan = self.plan
         self._residual()
         self._apply(self.p, self.ap)

@@ -167,6 +167,43 @@ def test_single_plan_graph_locg_energy_stop(problems, cached):
     assert torch.count_nonzero(x[plan.n_states:]) == 0
 
 
+@pytest.mark.parametrize('mixed', [False, True])
+def test_fused_step_follows_reference_step(problems, mixed):
+    """rqx_locg_step against the same iteration built from per-operation kernels."""
+    _say(f'fused step vs reference mixed={mixed}')
+    from rqutils_amd.locg_batch import BatchLOCG
+    bp = BatchPlan(problems[:3] + problems[4:])
+    rng = np.random.default_rng(12)
+    x0 = torch.from_numpy(rng.standard_normal(bp.n_total) + 1j * rng.standard_normal(bp.n_total)).to(DEV)
+    a = BatchLOCG(bp, x0, tol=0.0, mixed=mixed)
+    b = BatchLOCG(bp, x0, tol=0.0, mixed=mixed)
+    for it in range(25):
+        a.step()
+        b.step_reference()
+        torch.cuda.synchronize()
+        assert torch.allclose(a.theta, b.theta, rtol=1e-9, atol=0), (it, a.theta, b.theta)
+    for u, v in ((a.x, b.x), (a.ax, b.ax), (a.y, b.y)):
+        assert float((u - v).abs().max()) < 1e-6 * float(v.abs().max())
+
+
+def test_fused_step_real_vectors(problems):
+    """float64 vectors (real Hamiltonian): the fused step keeps real coefficients."""
+    _say('fused step real')
+    from rqutils_amd.locg_batch import BatchLOCG
+    t, s = problems[4]
+    plan = Plan(t, s)
+    rng = np.random.default_rng(13)
+    x0 = torch.from_numpy(rng.standard_normal(plan.n_total)).to(DEV)
+    a = BatchLOCG(plan, x0, tol=0.0, mixed=True)
+    b = BatchLOCG(plan, x0, tol=0.0, mixed=True)
+    for _ in range(20):
+        a.step()
+        b.step_reference()
+    torch.cuda.synchronize()
+    assert torch.allclose(a.theta, b.theta, rtol=1e-9, atol=0)
+    assert a.x.dtype == torch.float64 and bool(torch.isfinite(a.x).all())
+
+
 def test_batch_graph_replays_match_eager(problems):
     """The captured step replayed N times gives the eager trajectory."""
     _say('graph vs eager')
