@@ -207,22 +207,27 @@ int rqx_plan_update_coef(rqx_plan *plan, const double *coef, int32_t is_real, rq
  * with the same values, so rqx_matvec gives identical results with or without
  * the cache.
  *
- * rqx_plan_build_cache: precision 32 (used by RQX_F32 / RQX_C64 matvecs) or 64
- * (RQX_F64 / RQX_C128); values are real when the Hamiltonian is.  Allocates
- * (device memory; synchronous); RQX_ERR_LIMIT if it would need more than
- * max_bytes (max_bytes <= 0: 90% of the free device memory).  Rebuilding
- * replaces a cache of the same precision.  rqx_plan_update_coef rewrites the
- * values on its stream, or releases the cache if the coefficients change
- * between real and complex.
+ * rqx_plan_build_cache: precision 32 (used by RQX_F32 / RQX_C64 matvecs), 64
+ * (RQX_F64 / RQX_C128), or 0 for both in one pass over the hits with one shared
+ * column array; values are real when the Hamiltonian is.  Allocates (device
+ * memory; synchronous); RQX_ERR_LIMIT if it would need more than max_bytes
+ * (max_bytes <= 0: 90% of the free device memory).  Rebuilding replaces a cache
+ * of the same precision (both, for caches built together).
+ * rqx_plan_update_coef rewrites the values on its stream, or releases the cache
+ * if the coefficients change between real and complex.
  */
 int rqx_plan_build_cache(rqx_plan *plan, int32_t precision, int64_t max_bytes, rqx_stream_t stream);
 
-/* Release the cache of one precision (32 or 64), or both (0).  Synchronises the device. */
+/*
+ * Release the cache of one precision (32 or 64), or both (0); caches built
+ * together are released together.  Synchronises the device.
+ */
 int rqx_plan_drop_cache(rqx_plan *plan, int32_t precision);
 
 /*
  * out[0..6) = {built, stored hits (nnz), entries incl. padding, device bytes,
- * slices, complex values} for the cache of one precision.
+ * slices, complex values} for the cache of one precision (for caches built
+ * together: the 64-bit cache counts the shared block, the 32-bit its values).
  */
 int rqx_plan_cache_info(const rqx_plan *plan, int32_t precision, int64_t *out);
 

@@ -104,6 +104,8 @@ struct KParams {
     uint32_t *rowcnt_a;            // row-count mode: section-A hits per row
     int32_t *c_col;                // fill mode: cache column indices
     void *c_val;                   // fill mode: cache values (R, or complex R)
+    void *c_val_lo;                // fill mode (R = double, optional): FP32 values of the same hits
+    const void *coef_lo;           // fill mode with c_val_lo: FP32 coefficients
     const int64_t *c_off;          // fill mode: first entry of each slice
     const uint32_t *c_wa;          // fill mode: section-A entries per row of each slice
     const uint32_t *c_wb;          // fill mode: section-B entries per row of each slice
@@ -163,11 +165,11 @@ __device__ inline int find_u64(const uint64_t *__restrict__ a, int n, uint64_t t
     return (a[base] == t) ? base : -1;
 }
 
-// C_g(s) = sum_k coef_k (-1)^popcount(s & z_k)
+// C_g(s) = sum_k coef_k (-1)^popcount(s & z_k), coefficients in precision R
 template <typename R>
-__device__ inline typename C2<R>::T group_coef(const KParams &p, uint32_t g, uint64_t s) {
+__device__ inline typename C2<R>::T group_coef_from(const void *coef_v, const KParams &p, uint32_t g, uint64_t s) {
     using CC = typename C2<R>::T;
-    const CC *coef = static_cast<const CC *>(p.coef);
+    const CC *coef = static_cast<const CC *>(coef_v);
     CC acc;
     acc.x = R(0);
     acc.y = R(0);
@@ -179,6 +181,11 @@ __device__ inline typename C2<R>::T group_coef(const KParams &p, uint32_t g, uin
         acc.y += odd ? -c.y : c.y;
     }
     return acc;
+}
+
+template <typename R>
+__device__ inline typename C2<R>::T group_coef(const KParams &p, uint32_t g, uint64_t s) {
+    return group_coef_from<R>(p.coef, p, g, s);
 }
 
 // sum over the 64 lanes of a wave (every lane receives the total)
@@ -314,6 +321,11 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
                         const CC cg = group_coef<R>(p, g, s);
                         if constexpr (CPLX) static_cast<CC *>(p.c_val)[slot] = cg;
                         else static_cast<R *>(p.c_val)[slot] = cg.x;
+                        if (p.c_val_lo) {        // uniform: both precisions in one pass
+                            const float2 cl = group_coef_from<float>(p.coef_lo, p, g, s);
+                            if constexpr (CPLX) static_cast<float2 *>(p.c_val_lo)[slot] = cl;
+                            else static_cast<float *>(p.c_val_lo)[slot] = cl.x;
+                        }
                         if (sec_a) n_hit_a += 1;
                         else n_hit += 1;
                     }
@@ -384,6 +396,10 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
                 p.c_col[slot] = static_cast<int32_t>(i);
                 if constexpr (CPLX) static_cast<CC *>(p.c_val)[slot] = CC{R(0), R(0)};
                 else static_cast<R *>(p.c_val)[slot] = R(0);
+                if (p.c_val_lo) {
+                    if constexpr (CPLX) static_cast<float2 *>(p.c_val_lo)[slot] = make_float2(0.f, 0.f);
+                    else static_cast<float *>(p.c_val_lo)[slot] = 0.f;
+                }
             };
             for (uint32_t k = static_cast<uint32_t>(n_hit_a); k < wa; ++k) pad(k);
             for (uint32_t k = wa + static_cast<uint32_t>(n_hit); k < wa + wb; ++k) pad(k);
@@ -1156,6 +1172,7 @@ struct HitCache {            // sliced-ELL cache of the hits (column, C_j(s_i)) 
     int64_t entries = 0, nnz = 0, n_slices = 0, bytes = 0;
     int cplx = 0;
     bool built = false;
+    bool shared = false;       // both precisions built together: one block (owned by the FP64 cache), one column array
 };
 
 struct rqx_plan {
@@ -1568,8 +1585,18 @@ hipError_t free_cache(HitCache &c) {
     return e;
 }
 
+// release the cache of one slot; caches built together are released together
+hipError_t release_cache(rqx_plan *p, int slot) {
+    if (!p->cache[slot].shared) return free_cache(p->cache[slot]);
+    const hipError_t e = free_cache(p->cache[1]);
+    p->cache[0] = HitCache{};
+    return e;
+}
+
 // (re)write columns and values of a built cache from the current coefficients
+// (both precisions in one pass when they were built together)
 hipError_t fill_cache(const rqx_plan *p, int slot, hipStream_t st) {
+    if (p->cache[slot].shared) slot = 1;
     const HitCache &c = p->cache[slot];
     KParams k = make_params(p, slot == 1);
     k.c_col = c.col;
@@ -1577,6 +1604,10 @@ hipError_t fill_cache(const rqx_plan *p, int slot, hipStream_t st) {
     k.c_off = c.off;
     k.c_wa = c.wa;
     k.c_wb = c.wb;
+    if (c.shared) {
+        k.c_val_lo = p->cache[0].val;
+        k.coef_lo = p->coef32;
+    }
     if (slot == 1)
         return c.cplx ? launch_tiles<MODE_FILL, double, true>(p, k, nullptr, nullptr, st)
                       : launch_tiles<MODE_FILL, double, false>(p, k, nullptr, nullptr, st);
@@ -1833,13 +1864,13 @@ int rqx_plan_update_coef(rqx_plan *p, const double *coef, int32_t is_real, rqx_s
     p->is_real = is_real ? 1 : 0;
     // cached hits keep their columns; values are rewritten on the stream, or the
     // cache is released when the value type changes (real <-> complex)
-    for (int slot = 0; slot < 2; ++slot) {
+    for (int slot = 1; slot >= 0; --slot) {
         HitCache &c = p->cache[slot];
         if (!c.built) continue;
         if (c.cplx != (p->is_real ? 0 : 1)) {
             HIP_TRY(hipStreamSynchronize(st));
-            HIP_TRY(free_cache(c));
-        } else {
+            HIP_TRY(release_cache(p, slot));
+        } else if (!(c.shared && slot == 0)) {      // a shared pair is refilled in one pass
             HIP_TRY(fill_cache(p, slot, st));
         }
     }
@@ -1849,11 +1880,13 @@ int rqx_plan_update_coef(rqx_plan *p, const double *coef, int32_t is_real, rqx_s
 int rqx_plan_build_cache(rqx_plan *p, int32_t precision, int64_t max_bytes, rqx_stream_t stream) {
     g_err.clear();
     if (int rc = check_plan(p)) return rc;
-    const int slot = cache_slot(precision);
-    if (slot < 0) return fail(RQX_ERR_ARG, "precision must be 32 or 64");
+    const bool both = precision == 0;
+    const int slot = both ? 1 : cache_slot(precision);
+    if (slot < 0) return fail(RQX_ERR_ARG, "precision must be 0 (both), 32 or 64");
     hipStream_t st = static_cast<hipStream_t>(stream);
     HIP_TRY(hipStreamSynchronize(st));
-    HIP_TRY(free_cache(p->cache[slot]));
+    HIP_TRY(release_cache(p, slot));
+    if (both) HIP_TRY(release_cache(p, 0));
     if (p->n_states == 0) return RQX_OK;
     const int64_t n = p->n_states;
     if (n > INT32_MAX) return fail(RQX_ERR_LIMIT, "hit cache columns are int32: at most 2^31 - 1 rows");
@@ -1892,8 +1925,10 @@ int rqx_plan_build_cache(rqx_plan *p, int32_t precision, int64_t max_bytes, rqx_
     }
     const int cplx = p->is_real ? 0 : 1;
     const size_t vb = cache_val_bytes(slot, cplx);
+    const size_t vb_lo = both ? cache_val_bytes(0, cplx) : 0;
     const size_t o_col = 0, o_val = align_up(static_cast<size_t>(entries) * 4),
-                 o_off = o_val + align_up(static_cast<size_t>(entries) * vb),
+                 o_val_lo = o_val + align_up(static_cast<size_t>(entries) * vb),
+                 o_off = o_val_lo + (both ? align_up(static_cast<size_t>(entries) * vb_lo) : 0),
                  o_wa = o_off + align_up(static_cast<size_t>(n_slices) * 8),
                  o_wb = o_wa + align_up(static_cast<size_t>(n_slices) * 4),
                  total = o_wb + align_up(static_cast<size_t>(n_slices) * 4);
@@ -1918,15 +1953,23 @@ int rqx_plan_build_cache(rqx_plan *p, int32_t precision, int64_t max_bytes, rqx_
     c.bytes = static_cast<int64_t>(total);
     c.cplx = cplx;
     c.built = true;
+    c.shared = both;
     e = hipMemcpyAsync(c.off, off.data(), off.size() * 8, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipMemcpyAsync(c.wa, wa.data(), wa.size() * 4, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipMemcpyAsync(c.wb, wb.data(), wb.size() * 4, hipMemcpyHostToDevice, st);
     p->cache[slot] = c;
+    if (both) {                 // the FP32 cache: its own values, everything else shared
+        HitCache lo = c;
+        lo.block = nullptr;
+        lo.val = b + o_val_lo;
+        lo.bytes = static_cast<int64_t>(entries) * static_cast<int64_t>(vb_lo);
+        p->cache[0] = lo;
+    }
     // pass 2: columns and values in the order the tile kernel accumulates them
     if (e == hipSuccess) e = fill_cache(p, slot, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);   // host vectors above are released on return
     if (e != hipSuccess) {
-        (void)free_cache(p->cache[slot]);
+        (void)release_cache(p, slot);
         return fail(RQX_ERR_HIP, std::string("cache fill: ") + hipGetErrorString(e));
     }
     return RQX_OK;
@@ -1937,8 +1980,8 @@ int rqx_plan_drop_cache(rqx_plan *p, int32_t precision) {
     if (int rc = check_plan(p)) return rc;
     if (precision != 0 && cache_slot(precision) < 0) return fail(RQX_ERR_ARG, "precision must be 0, 32 or 64");
     HIP_TRY(hipDeviceSynchronize());
-    for (int slot = 0; slot < 2; ++slot)
-        if (precision == 0 || cache_slot(precision) == slot) HIP_TRY(free_cache(p->cache[slot]));
+    for (int slot = 1; slot >= 0; --slot)
+        if (p->cache[slot].built && (precision == 0 || cache_slot(precision) == slot)) HIP_TRY(release_cache(p, slot));
     return RQX_OK;
 }
 

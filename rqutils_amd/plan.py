@@ -246,14 +246,14 @@ class Plan:
         (group, row) pairs (``sqd.py:343-407``); this cache keeps only the hits,
         as sliced ELL (64 rows per slice, hits of groups without low-bit flips in
         a section of their own so that their gathers coalesce).  ``precision`` 64 serves float64 and
-        complex128 matvecs, 32 float32 and complex64.  Later ``matvec`` calls of
-        that precision read the cache and return the same values as the
-        matrix-free kernel.  Raises ``RqxError`` when the cache needs more than
+        complex128 matvecs, 32 float32 and complex64, 0 builds both in one pass
+        (one column array).  Later ``matvec`` calls of that precision read the
+        cache and return the same values as the matrix-free kernel.  Raises ``RqxError`` when the cache needs more than
         ``max_bytes`` (0: 90% of the free device memory).
         """
         stream = _capi.current_stream(self.device)
         _capi.check(_capi.lib().rqx_plan_build_cache(self._handle, int(precision), int(max_bytes), stream))
-        return self.cache_info(precision)
+        return self.cache_info(precision if precision else 64)
 
     def drop_cache(self, precision=0):
         """Release the cache of one precision (32 or 64) or both (0)."""

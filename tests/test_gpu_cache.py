@@ -164,3 +164,24 @@ def test_run_sqd_cache_strategies_identical(cases):
         assert float(out[cs][0]) == float(out[(0, 0)][0])
         assert torch.equal(out[cs][1], out[(0, 0)][1])
     assert out[(1, 0)][3] == 700
+
+
+@pytest.mark.parametrize('name', ['complex12', 'real_jw', 'no_diagonal'])
+def test_both_precisions_in_one_pass(cases, name):
+    """build_cache(0): one pass fills FP64 and FP32 values over shared columns; both bit-identical."""
+    _say(f'cache both {name}')
+    tables, states = cases[name]
+    plan = Plan(tables, states, n_total=len(states) + 5)
+    plan.build_cache(0)
+    i64, i32 = plan.cache_info(64), plan.cache_info(32)
+    assert i64['built'] and i32['built'] and i64['entries'] == i32['entries'] and i64['nnz'] == i32['nnz']
+    _check_identical(plan, seed=15)
+    rng = np.random.default_rng(16)
+    plan.set_alphas(torch.from_numpy(rng.standard_normal(plan.info['n_terms'])).to(DEV))
+    _check_identical(plan, seed=17)
+    plan.drop_cache(32)                      # built together, released together
+    assert not plan.cache_info(64)['built'] and not plan.cache_info(32)['built']
+    plan.build_cache(0)
+    plan.build_cache(64)                     # rebuilding one precision replaces the pair
+    assert plan.cache_info(64)['built'] and not plan.cache_info(32)['built']
+    _check_identical(plan, seed=18)
