@@ -65,6 +65,10 @@ enum { MODE_MATVEC = 0, MODE_LOOKUP = 1, MODE_COUNT = 2, MODE_GRAD = 3, MODE_ROW
 // each part accumulated in group order, in the matrix-free kernel too.
 constexpr int SLICE = 64;
 
+// terms fetched per scalar-memory round trip when a coefficient sum is evaluated;
+// the z and coefficient arrays are padded by as many entries
+constexpr int TERM_PAD = 4;
+
 // One problem of a plan (a batched plan holds several).  Row, segment, class,
 // group and term indices in the plan arrays are absolute; the descriptor gives
 // the problem's own ranges.
@@ -174,11 +178,27 @@ __device__ inline typename C2<R>::T group_coef_from(const void *coef_v, const KP
     acc.x = R(0);
     acc.y = R(0);
     const uint32_t k1 = p.g_toff[g + 1];
-    for (uint32_t k = p.g_toff[g]; k < k1; ++k) {
-        const CC c = coef[k];
-        const bool odd = __popcll(s & p.z[k]) & 1;
-        acc.x += odd ? -c.x : c.x;
-        acc.y += odd ? -c.y : c.y;
+    // the terms' (z, coef) are wave-uniform scalar loads, which return out of
+    // order: fetch TERM_PAD terms before using any, so one scalar-memory round
+    // trip serves them all (the term arrays carry TERM_PAD entries of padding;
+    // the sum keeps its term order)
+    for (uint32_t k = p.g_toff[g]; k < k1; k += TERM_PAD) {
+        uint64_t z[TERM_PAD];
+        CC c[TERM_PAD];
+#pragma unroll
+        for (int u = 0; u < TERM_PAD; ++u) {
+            z[u] = p.z[k + u];
+            c[u] = coef[k + u];
+        }
+        __builtin_amdgcn_sched_barrier(0);   // keep the loads ahead of their first use
+#pragma unroll
+        for (int u = 0; u < TERM_PAD; ++u) {
+            if (k + u < k1) {
+                const bool odd = __popcll(s & z[u]) & 1;
+                acc.x += odd ? -c[u].x : c[u].x;
+                acc.y += odd ? -c[u].y : c[u].y;
+            }
+        }
     }
     return acc;
 }
@@ -1481,6 +1501,11 @@ int build_plan(rqx_plan **out, const std::vector<ProblemIn> &in, int64_t n_total
         tile_prob.swap(pr);
     }
 
+    // term arrays padded with zero terms (coefficient sums fetch TERM_PAD terms at a time)
+    zall.resize(zall.size() + TERM_PAD, 0ull);
+    c64.resize(c64.size() + 2 * TERM_PAD, 0.0);
+    c32.resize(c32.size() + TERM_PAD, make_float2(0.f, 0.f));
+
     // ---- one device block, carved
     size_t off = 0;
     auto carve = [&](size_t bytes) {
@@ -1492,8 +1517,9 @@ int build_plan(rqx_plan **out, const std::vector<ProblemIn> &in, int64_t n_total
     const size_t o_keys = carve(n * 8), o_lo = carve(n * 4), o_seghi = carve(seg_hi_all.size() * 8),
                  o_segoff = carve(seg_off_all.size() * 4), o_clshi = carve(cls_hi_all.size() * 8),
                  o_clsgoff = carve(cls_goff_all.size() * 4), o_gxlo = carve(g_xlo.size() * 4),
-                 o_gtoff = carve(g_toff.size() * 4), o_gx = carve(g_x.size() * 8), o_z = carve(zall.size() * 8),
-                 o_c64 = carve(c64.size() * 8), o_c32 = carve(c32.size() * 8), o_d64 = carve(n * 16),
+                 o_gtoff = carve(g_toff.size() * 4), o_gx = carve(g_x.size() * 8),
+                 o_z = carve(zall.size() * 8), o_c64 = carve(c64.size() * 8),
+                 o_c32 = carve(c32.size() * 8), o_d64 = carve(n * 16),
                  o_d32 = carve(n * 8), o_trow = carve(tile_row0.size() * 4), o_tseg = carve(tile_seg.size() * 4),
                  o_tprob = carve(tile_prob.size() * 4), o_desc = carve(desc.size() * sizeof(ProbDesc)),
                  o_prows = carve(prob_rows.size() * 8), o_queue = carve(sizeof(unsigned int));
