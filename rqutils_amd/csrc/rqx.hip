@@ -115,6 +115,7 @@ struct KParams {
     const uint32_t *c_wb;          // fill mode: section-B entries per row of each slice
     int64_t n_total;
     uint32_t n_tiles;
+    int32_t coef_real;             // every phased coefficient is real: sums need only their real parts
 };
 
 template <typename R> struct C2;
@@ -142,6 +143,18 @@ __device__ inline void fma_acc(float2 &acc, float2 c, float2 v) {
     acc.x = fmaf(-c.y, v.y, acc.x);
     acc.y = fmaf(c.x, v.y, acc.y);
     acc.y = fmaf(c.y, v.x, acc.y);
+}
+
+// real coefficient (a Hamiltonian whose phased coefficients are all real): acc += c * v
+__device__ inline void fma_acc(double &acc, double c, double v) { acc = fma(c, v, acc); }
+__device__ inline void fma_acc(float &acc, float c, float v) { acc = fmaf(c, v, acc); }
+__device__ inline void fma_acc(double2 &acc, double c, double2 v) {
+    acc.x = fma(c, v.x, acc.x);
+    acc.y = fma(c, v.y, acc.y);
+}
+__device__ inline void fma_acc(float2 &acc, float c, float2 v) {
+    acc.x = fmaf(c, v.x, acc.x);
+    acc.y = fmaf(c, v.y, acc.y);
 }
 
 __device__ inline double vadd(double a, double b) { return a + b; }
@@ -206,6 +219,33 @@ __device__ inline typename C2<R>::T group_coef_from(const void *coef_v, const KP
 template <typename R>
 __device__ inline typename C2<R>::T group_coef(const KParams &p, uint32_t g, uint64_t s) {
     return group_coef_from<R>(p.coef, p, g, s);
+}
+
+// Re C_g(s), the whole sum when the coefficients are real (same order as group_coef_from)
+template <typename R>
+__device__ inline R group_coef_re_from(const void *coef_v, const KParams &p, uint32_t g, uint64_t s) {
+    using CC = typename C2<R>::T;
+    const CC *coef = static_cast<const CC *>(coef_v);
+    R acc = R(0);
+    const uint32_t k1 = p.g_toff[g + 1];
+    for (uint32_t k = p.g_toff[g]; k < k1; k += TERM_PAD) {
+        uint64_t z[TERM_PAD];
+        R c[TERM_PAD];
+#pragma unroll
+        for (int u = 0; u < TERM_PAD; ++u) {
+            z[u] = p.z[k + u];
+            c[u] = coef[k + u].x;
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int u = 0; u < TERM_PAD; ++u) {
+            if (k + u < k1) {
+                const bool odd = __popcll(s & z[u]) & 1;
+                acc += odd ? -c[u] : c[u];
+            }
+        }
+    }
+    return acc;
 }
 
 // sum over the 64 lanes of a wave (every lane receives the total)
@@ -318,7 +358,10 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
             auto handle = [&](uint32_t g, uint32_t base, uint32_t tl, bool sec_a) {
                 const bool hit = active && (s_keys[base] == tl);
                 if constexpr (MODE == MODE_MATVEC) {
-                    if (hit) fma_acc(sec_a ? acc_a : acc, group_coef<R>(p, g, s), x[ta + base]);
+                    if (hit) {
+                        if (p.coef_real) fma_acc(sec_a ? acc_a : acc, group_coef_re_from<R>(p.coef, p, g, s), x[ta + base]);
+                        else fma_acc(sec_a ? acc_a : acc, group_coef<R>(p, g, s), x[ta + base]);
+                    }
                 } else if constexpr (MODE == MODE_LOOKUP) {
                     if (hit) {
                         const size_t o = static_cast<size_t>(g) * p.n_total + i;
@@ -338,13 +381,13 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
                                                 : static_cast<int64_t>(p.c_wa[i / SLICE]) + static_cast<int64_t>(n_hit);
                         const int64_t slot = p.c_off[i / SLICE] + k * SLICE + (i % SLICE);
                         p.c_col[slot] = static_cast<int32_t>(ta + base);
-                        const CC cg = group_coef<R>(p, g, s);
-                        if constexpr (CPLX) static_cast<CC *>(p.c_val)[slot] = cg;
-                        else static_cast<R *>(p.c_val)[slot] = cg.x;
+                        if constexpr (CPLX) static_cast<CC *>(p.c_val)[slot] = group_coef<R>(p, g, s);
+                        else static_cast<R *>(p.c_val)[slot] = group_coef_re_from<R>(p.coef, p, g, s);
                         if (p.c_val_lo) {        // uniform: both precisions in one pass
-                            const float2 cl = group_coef_from<float>(p.coef_lo, p, g, s);
-                            if constexpr (CPLX) static_cast<float2 *>(p.c_val_lo)[slot] = cl;
-                            else static_cast<float *>(p.c_val_lo)[slot] = cl.x;
+                            if constexpr (CPLX)
+                                static_cast<float2 *>(p.c_val_lo)[slot] = group_coef_from<float>(p.coef_lo, p, g, s);
+                            else
+                                static_cast<float *>(p.c_val_lo)[slot] = group_coef_re_from<float>(p.coef_lo, p, g, s);
                         }
                         if (sec_a) n_hit_a += 1;
                         else n_hit += 1;
@@ -974,18 +1017,6 @@ __global__ void k_locg_scalar(int stage, int K, double *__restrict__ work, doubl
     }
 }
 
-// real coefficient (hit cache of a real Hamiltonian): acc += c * v
-__device__ inline void fma_acc(double &acc, double c, double v) { acc = fma(c, v, acc); }
-__device__ inline void fma_acc(float &acc, float c, float v) { acc = fmaf(c, v, acc); }
-__device__ inline void fma_acc(double2 &acc, double c, double2 v) {
-    acc.x = fma(c, v.x, acc.x);
-    acc.y = fma(c, v.y, acc.y);
-}
-__device__ inline void fma_acc(float2 &acc, float c, float2 v) {
-    acc.x = fmaf(c, v.x, acc.x);
-    acc.y = fmaf(c, v.y, acc.y);
-}
-
 // entries per batch of the cached matvec: the column and value loads of a batch,
 // then its gathers of x, are all in flight together (16 while a batch's
 // registers stay small, 8 for 16-byte vector entries or complex values)
@@ -1272,6 +1303,7 @@ KParams make_params(const rqx_plan *p, bool fp64) {
     k.queue = p->queue;
     k.n_total = p->n_total;
     k.n_tiles = static_cast<uint32_t>(p->n_tiles);
+    k.coef_real = p->is_real;
     return k;
 }
 
