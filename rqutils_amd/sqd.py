@@ -33,6 +33,7 @@ from rqutils_amd.paulis.symplectic import PauliSumXZ, masks_to_packed, packed_to
 from rqutils_amd._capi import RqxError
 from rqutils_amd.plan import Plan, sort_unique_u64
 from rqutils_amd.ground_locg import ground_locg
+from rqutils_amd.locg_batch import batch_ground_locg
 
 LOG = logging.getLogger(__name__)
 
@@ -53,11 +54,18 @@ class SQDOptions:
     """LOCG maximum iterations."""
     locg_tol: float | None = None
     """LOCG convergence tolerance."""
+    locg_precision: str = 'fp64'
+    """'fp64': the reference LOCG (``ground_locg``, FP64 throughout).  'mixed' (an
+    extension): the graph-captured LOCG step with an FP32 bulk matvec and FP64
+    vectors / Rayleigh-Ritz, stopped at relative residual ``locg_tol`` (default
+    1e-8); with ``cache_strategy`` (1, *) both cache precisions are built."""
 
     def __post_init__(self):
         cs = tuple(self.cache_strategy)
         if len(cs) != 2 or cs[0] not in (0, 1) or cs[1] not in (0, 1, 2):
             raise ValueError('cache_strategy must be (0|1, 0|1|2)')
+        if self.locg_precision not in ('fp64', 'mixed'):
+            raise ValueError("locg_precision must be 'fp64' or 'mixed'")
         if self.shard_bundle < 0:
             raise ValueError('bundle must be positive and divide the mesh size evenly')
 
@@ -193,11 +201,12 @@ def make_plan(hamiltonian: PauliSumXZ, keys, states_size=None, device=None, spli
 def _run(hamiltonian, keys, states_size, options, log_level=logging.INFO):
     plan = make_plan(hamiltonian, keys, states_size)
     tables = hamiltonian.tables
+    mixed = options.locg_precision == 'mixed'
     if options.cache_strategy[0] == 1 and plan.n_states > 0:
         # sqd.py:343-351 caches the source index of every (group, row) pair; the
         # hit cache holds the hits with their coefficient sums (strategy (1, 2))
         try:
-            info = plan.build_cache(64)
+            info = plan.build_cache(0 if mixed else 64)
             LOG.debug('Hit cache: %d hits, %d bytes', info['nnz'], info['bytes'])
         except RqxError as exc:
             LOG.info('Hit cache does not fit (%s); using the matrix-free kernel', exc)
@@ -208,8 +217,17 @@ def _run(hamiltonian, keys, states_size, options, log_level=logging.INFO):
         vinit = torch.argmin(diag)
     else:
         vinit = 0
-    eigval, eigvec, _ = ground_locg(plan.matvec, vinit, maxiter=options.locg_maxiter, tol=options.locg_tol,
-                                    vspace=(states_size, dtype, plan.device), log_level=log_level)
+    if mixed and plan.n_states > 0:
+        x0 = torch.zeros(states_size, dtype=dtype, device=plan.device)
+        x0[vinit] = 1
+        tol = 1e-8 if options.locg_tol is None else options.locg_tol
+        theta, eigvec, niter, info = batch_ground_locg(plan, x0, tol=tol, maxiter=options.locg_maxiter,
+                                                       check_every=5)
+        LOG.log(log_level, 'Mixed LOCG: %d iterations, relative residual %g', niter, float(info['rel_residual'][0]))
+        eigval = theta[0]
+    else:
+        eigval, eigvec, _ = ground_locg(plan.matvec, vinit, maxiter=options.locg_maxiter, tol=options.locg_tol,
+                                        vspace=(states_size, dtype, plan.device), log_level=log_level)
     if options.return_eigvec:
         return eigval, eigvec, plan.states, plan.n_states
     return (eigval,)

@@ -185,3 +185,22 @@ def test_both_precisions_in_one_pass(cases, name):
     plan.build_cache(64)                     # rebuilding one precision replaces the pair
     assert plan.cache_info(64)['built'] and not plan.cache_info(32)['built']
     _check_identical(plan, seed=18)
+
+
+@pytest.mark.parametrize('cs', [(0, 0), (1, 0)])
+def test_run_sqd_mixed_precision_matches_fp64(cs):
+    """SQDOptions(locg_precision='mixed'): same ground state as the reference LOCG."""
+    _say(f'run_sqd mixed cs={cs}')
+    from rqutils_amd.paulis.symplectic import PauliSumXZ, masks_to_packed
+    from rqutils_amd.sqd import SQDOptions, run_sqd
+    w = workloads.config1(n_alpha=60, n_beta=50)
+    h = PauliSumXZ.from_tables(w.tables, add_padding=True)
+    states_p = masks_to_packed(w.states, w.tables.num_qubits, padded=True)
+    size = len(w.states) + 9
+    e64, v64, _, dim = run_sqd(h, states_p, size, options=SQDOptions(cache_strategy=cs, locg_maxiter=3000))
+    emx, vmx, _, dim2 = run_sqd(h, states_p, size,
+                                options=SQDOptions(cache_strategy=cs, locg_maxiter=3000, locg_precision='mixed'))
+    assert dim == dim2 == len(w.states)
+    assert abs(float(emx) - float(e64)) <= 1e-9 * abs(float(e64))
+    assert abs(abs(torch.vdot(v64, vmx.to(v64.dtype)).item()) - 1.0) < 1e-6
+    assert torch.count_nonzero(vmx[dim:]) == 0

@@ -245,3 +245,20 @@ def test_small_eigenpair_routine_on_host(n):
         lam, vec = _eig_host(a)
         assert abs(lam - lam_r) < 1e-12 * max(1., abs(lam_r))
         assert np.allclose(vec, vec_r, atol=1e-9)
+
+
+def test_sqd_options_validation():
+    """SQDOptions: the reference's fields and checks (sqd.py:183-196) plus locg_precision."""
+    from rqutils_amd.sqd import SQDOptions, _as_options
+    o = SQDOptions()
+    assert o.cache_strategy == (1, 0) and o.shard_bundle == 1 and o.locg_maxiter == 1000
+    assert o.locg_tol is None and o.return_eigvec and o.locg_precision == 'fp64'
+    for bad in ((2, 0), (0, 3), (1,)):
+        with pytest.raises(ValueError):
+            SQDOptions(cache_strategy=bad)
+    with pytest.raises(ValueError):
+        SQDOptions(shard_bundle=-1)
+    with pytest.raises(ValueError):
+        SQDOptions(locg_precision='fp32')
+    assert _as_options({'locg_precision': 'mixed'}).locg_precision == 'mixed'
+    assert _as_options(None) == SQDOptions()
