@@ -246,8 +246,11 @@ enum { RQX_LOCG_WORK = 168 };
  * re-orthonormalised) with the images A x, A y carried instead of recomputed,
  * so it costs one matvec.  No counterpart in the reference (one problem per
  * call, four matvecs per iteration).
- *   dtype: RQX_F64 or RQX_C128 (the vectors); low != 0: the matvec runs in
- *     RQX_F32 / RQX_C64 on low copies of p and A p;
+ *   dtype: the vectors' type, any of the four (FP32 vectors: the bulk phase
+ *     of a mixed solve stored in low precision; the inner products, Rayleigh-
+ *     Ritz and normalisations are FP64 either way); low != 0 (RQX_F64 /
+ *     RQX_C128 vectors only): the matvec runs in RQX_F32 / RQX_C64 on low
+ *     copies of p and A p;
  *   vec: device vectors {x, Ax, y, Ay, p, Ap, p_low, Ap_low} over all rows
  *     (p_low, Ap_low only with low); x, Ax, y, Ay are updated in place;
  *   theta, rel: device, one double per problem: <x, A x> and |A x - theta x| /

@@ -2193,6 +2193,12 @@ LPass make_pass(double *work, int coef_slot, std::initializer_list<const void *>
     return a;
 }
 
+template <typename V> constexpr int32_t dtype_of();
+template <> constexpr int32_t dtype_of<double>() { return RQX_F64; }
+template <> constexpr int32_t dtype_of<double2>() { return RQX_C128; }
+template <> constexpr int32_t dtype_of<float>() { return RQX_F32; }
+template <> constexpr int32_t dtype_of<float2>() { return RQX_C64; }
+
 // One fused LOCG step (see rqx_locg_step) for vector type V (low copy type LV).
 template <typename V, typename LV>
 int locg_step(const rqx_plan *p, bool low, bool real, void *const *vec, double *theta, double *rel, uint8_t *active,
@@ -2201,7 +2207,6 @@ int locg_step(const rqx_plan *p, bool low, bool real, void *const *vec, double *
     const int K = static_cast<int>(p->probs.size());
     const unsigned chunks = batch_chunks(p);
     const unsigned sb = static_cast<unsigned>((K + 63) / 64);
-    const bool rv = std::is_same<V, double>::value;
     auto scalar = [&](int stage) {
         hipLaunchKernelGGL(k_locg_scalar, dim3(sb), dim3(64), 0, st, stage, K, work, theta, rel, active, tol,
                            real ? 1 : 0);
@@ -2223,14 +2228,14 @@ int locg_step(const rqx_plan *p, bool low, bool real, void *const *vec, double *
     // 4: second Gram-Schmidt pass, normalised (plus the low copy for the bulk matvec); Ap; 5: Rayleigh-Ritz dots
     if (low) {
         HIP_TRY((lpass<V, LV, 3, 2, 0, false, true>(p, make_pass(work, LW_C4, {pv, x, y}, {pv, plo}, {}, 0), chunks, st)));
-        if (int rc = rqx_matvec(p, plo, aplo, rv ? RQX_F32 : RQX_C64, st)) return rc;
+        if (int rc = rqx_matvec(p, plo, aplo, dtype_of<LV>(), st)) return rc;
         HIP_TRY((lpass<V, LV, 6, 1, 9, true, false>(
             p, make_pass(work, LW_C5, {x, y, pv, ax, ay, aplo}, {ap},
                          {{0, 3}, {0, 4}, {0, 6}, {1, 3}, {1, 4}, {1, 6}, {2, 3}, {2, 4}, {2, 6}}, LW_D_RR9),
             chunks, st)));
     } else {
         HIP_TRY((lpass<V, LV, 3, 1, 0, false, false>(p, make_pass(work, LW_C4, {pv, x, y}, {pv}, {}, 0), chunks, st)));
-        if (int rc = rqx_matvec(p, pv, ap, rv ? RQX_F64 : RQX_C128, st)) return rc;
+        if (int rc = rqx_matvec(p, pv, ap, dtype_of<V>(), st)) return rc;
         HIP_TRY((lpass<V, LV, 6, 0, 9, false, false>(
             p, make_pass(work, 0, {x, y, pv, ax, ay, ap}, {},
                          {{0, 3}, {0, 4}, {0, 5}, {1, 3}, {1, 4}, {1, 5}, {2, 3}, {2, 4}, {2, 5}}, LW_D_RR9),
@@ -2257,15 +2262,19 @@ int rqx_locg_step(const rqx_plan *p, int32_t dtype, int32_t low, void *const *ve
                   uint8_t *active, double *work, double tol, rqx_stream_t stream) {
     g_err.clear();
     if (int rc = check_plan(p)) return rc;
-    if (dtype != RQX_F64 && dtype != RQX_C128) return fail(RQX_ERR_ARG, "dtype must be RQX_F64 or RQX_C128");
     if (int rc = check_dtype(p, dtype)) return rc;
+    if (low && dtype != RQX_F64 && dtype != RQX_C128)
+        return fail(RQX_ERR_ARG, "low != 0 needs RQX_F64 or RQX_C128 vectors");
     if (!vec || !theta || !rel || !active || !work) return fail(RQX_ERR_ARG, "NULL argument");
     for (int q = 0; q < (low ? 8 : 6); ++q)
         if (!vec[q]) return fail(RQX_ERR_ARG, "NULL vector pointer");
     hipStream_t st = static_cast<hipStream_t>(stream);
-    if (dtype == RQX_F64)
-        return locg_step<double, float>(p, low != 0, true, vec, theta, rel, active, work, tol, st);
-    return locg_step<double2, float2>(p, low != 0, false, vec, theta, rel, active, work, tol, st);
+    switch (dtype) {
+        case RQX_F64: return locg_step<double, float>(p, low != 0, true, vec, theta, rel, active, work, tol, st);
+        case RQX_C128: return locg_step<double2, float2>(p, low != 0, false, vec, theta, rel, active, work, tol, st);
+        case RQX_F32: return locg_step<float, float>(p, false, true, vec, theta, rel, active, work, tol, st);
+        default: return locg_step<float2, float2>(p, false, false, vec, theta, rel, active, work, tol, st);
+    }
 }
 
 int rqx_eigenpair_batch(const double *mats, int32_t count, int32_t n, double *out, rqx_stream_t stream) {

@@ -31,7 +31,7 @@ _LO = {torch.float64: torch.float32, torch.complex128: torch.complex64}
 class BatchLOCG:
     """Persistent buffers + one LOCG step for all problems of a plan (``BatchPlan`` or single ``Plan``)."""
 
-    def __init__(self, plan: Plan, x0: torch.Tensor, tol=1e-7, switch_tol=1e-5, mixed=True):
+    def __init__(self, plan: Plan, x0: torch.Tensor, tol=1e-7, switch_tol=1e-5, mixed=True, lo_storage=False):
         if x0.dtype not in _LO:
             raise ValueError('x0 must be float64 or complex128')
         if x0.shape != (plan.n_total,) or x0.device != plan.device:
@@ -53,12 +53,24 @@ class BatchLOCG:
         self.one = torch.ones(self.K, dtype=torch.complex128, device=dev)
         self.zero = torch.zeros(self.K, dtype=torch.complex128, device=dev)
         self.work = torch.zeros(self.K * _capi.RQX_LOCG_WORK, dtype=torch.float64, device=dev)
-        self._vec_ptrs = None
+        self._vec_ptrs = {}
         self.phase = 'lo' if mixed else 'hi'
+        # lo_storage: the bulk phase keeps its vectors in low precision (the step's
+        # inner products and coefficients stay FP64) and x, y are widened and
+        # re-orthonormalised at the switch; by default the vectors stay FP64 with
+        # low copies of p and A p around the low-precision matvec (measured at 1M
+        # rows: the same time to solution, fewer and less noisy bulk iterations)
+        self.lo_storage = bool(mixed and lo_storage)
+        if self.lo_storage:
+            zl = lambda: torch.zeros(plan.n_total, dtype=self.lo, device=dev)  # noqa: E731
+            self.x_lo, self.ax_lo, self.y_lo, self.ay_lo = (zl() for _ in range(4))
         self.x.copy_(x0)
         self._normalize(self.x, None)
         self._apply(self.x, self.ax)
         self._first_step()
+        if self.lo_storage:
+            for lo_v, hi_v in ((self.x_lo, self.x), (self.ax_lo, self.ax), (self.y_lo, self.y), (self.ay_lo, self.ay)):
+                lo_v.copy_(hi_v)
 
     # ------------------------------------------------------------------ helpers
     def _apply(self, v, out):
@@ -132,17 +144,25 @@ class BatchLOCG:
         Eight fused passes over the vectors, scalar stages on the device and one
         matvec; the same sequence as ``step_reference``.
         """
-        if self._vec_ptrs is None:
-            ptrs = [t.data_ptr() for t in (self.x, self.ax, self.y, self.ay, self.p, self.ap, self.p_lo, self.ap_lo)]
-            self._vec_ptrs = (ctypes.c_void_p * 8)(*ptrs)
-        dtype = _capi.RQX_C128 if self.hi == torch.complex128 else _capi.RQX_F64
+        lo_store = self.phase == 'lo' and self.lo_storage
+        key = (self.phase, lo_store)
+        if key not in self._vec_ptrs:
+            vecs = ((self.x_lo, self.ax_lo, self.y_lo, self.ay_lo, self.p_lo, self.ap_lo) if lo_store else
+                    (self.x, self.ax, self.y, self.ay, self.p, self.ap, self.p_lo, self.ap_lo))
+            self._vec_ptrs[key] = (ctypes.c_void_p * 8)(*[t.data_ptr() for t in vecs])
+        dt = self.lo if lo_store else self.hi
+        dtype = {torch.complex128: _capi.RQX_C128, torch.float64: _capi.RQX_F64,
+                 torch.complex64: _capi.RQX_C64, torch.float32: _capi.RQX_F32}[dt]
+        low = int(self.phase == 'lo' and not lo_store)
         stream = _capi.current_stream(self.plan.device)
-        _capi.check(_capi.lib().rqx_locg_step(self.plan._handle, dtype, int(self.phase == 'lo'), self._vec_ptrs,
+        _capi.check(_capi.lib().rqx_locg_step(self.plan._handle, dtype, low, self._vec_ptrs[key],
                                               self.theta.data_ptr(), self.rel.data_ptr(), self.active.data_ptr(),
                                               self.work.data_ptr(), float(self.tol), stream))
 
     def step_reference(self):
         """The same iteration from the per-problem vector kernels and torch (one launch per operation)."""
+        if self.lo_storage and self.phase == 'lo':
+            raise ValueError('step_reference runs on FP64 vectors: build the solver with lo_storage=False')
         plan = self.plan
         self._residual()
         self._apply(self.p, self.ap)
@@ -156,7 +176,14 @@ class BatchLOCG:
         self._update(c)
 
     def to_high_precision(self):
-        """Switch to the vector dtype: recompute the carried images exactly."""
+        """Switch to the vector dtype: x and y widened and re-orthonormalised in FP64, images recomputed."""
+        if self.lo_storage and self.phase == 'lo':
+            self.x.copy_(self.x_lo)
+            self.y.copy_(self.y_lo)
+            self._normalize(self.x, None)
+            pr = self.plan.dots([(self.x, self.y)])[:, 0]
+            self.plan.combine([self.y, self.x], torch.stack([self.one, -pr], 1), self.y)
+            self._normalize(self.y, None)
         self.phase = 'hi'
         self._apply(self.x, self.ax)
         self._apply(self.y, self.ay)
@@ -164,7 +191,7 @@ class BatchLOCG:
 
 def batch_ground_locg(plan: Plan, x0, tol=1e-7, switch_tol=1e-5, maxiter=1000, check_every=10,
                       use_graph=True, mixed=True, record=False, energy_tol=None, energy_checks=2,
-                      switch_energy=1e-9):
+                      switch_energy=1e-9, lo_storage=False):
     """Ground eigenpairs of every problem of ``plan`` (a ``BatchPlan``, or a ``Plan`` as one problem).
 
     A problem stops (its vectors freeze) at relative residual ``tol``, or, with
@@ -179,7 +206,7 @@ def batch_ground_locg(plan: Plan, x0, tol=1e-7, switch_tol=1e-5, maxiter=1000, c
     relative residuals, the iteration of the precision switch and timings.
     """
     t0 = time.perf_counter()
-    solver = BatchLOCG(plan, x0, tol=tol, switch_tol=switch_tol, mixed=mixed)
+    solver = BatchLOCG(plan, x0, tol=tol, switch_tol=switch_tol, mixed=mixed, lo_storage=lo_storage)
     graphs = {}
 
     def run_step():
@@ -210,14 +237,17 @@ def batch_ground_locg(plan: Plan, x0, tol=1e-7, switch_tol=1e-5, maxiter=1000, c
         for _ in range(check_every):
             run_step()
             it += 1
-        act = solver.active.cpu()
+        # one device -> host copy per poll: active flags, theta, relative residuals
+        status = torch.cat([solver.active.to(torch.float64), solver.theta, solver.rel]).cpu()
+        K = solver.K
+        act = status[:K] > 0.5
+        th = status[K:2 * K]
         active = int(act.sum())
-        worst = float(solver.rel[solver.active].max().item()) if active else 0.0
+        worst = float(status[2 * K:][act].max()) if active else 0.0
         if record:
             history.append((time.perf_counter() - t0, it, active, worst, solver.phase))
         settled = torch.zeros(solver.K, dtype=torch.bool)
         if energy_tol is not None:
-            th = solver.theta.cpu()
             thetas.append(th)
             if len(thetas) >= 3:
                 d0, d1 = thetas[-3] - thetas[-2], thetas[-2] - thetas[-1]
@@ -241,6 +271,8 @@ def batch_ground_locg(plan: Plan, x0, tol=1e-7, switch_tol=1e-5, maxiter=1000, c
             active = int((act & ~settled).sum())
         if active == 0:
             break
+    if solver.phase == 'lo' and solver.lo_storage:
+        solver.x.copy_(solver.x_lo)
     torch.cuda.synchronize(plan.device)
     info = {'rel_residual': solver.rel.clone(), 'switch_iteration': switch_at, 'iterations': it,
             'seconds': time.perf_counter() - t0, 'history': history, 'graph': use_graph}

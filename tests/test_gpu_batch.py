@@ -175,8 +175,8 @@ def test_fused_step_follows_reference_step(problems, mixed):
     bp = BatchPlan(problems[:3] + problems[4:])
     rng = np.random.default_rng(12)
     x0 = torch.from_numpy(rng.standard_normal(bp.n_total) + 1j * rng.standard_normal(bp.n_total)).to(DEV)
-    a = BatchLOCG(bp, x0, tol=0.0, mixed=mixed)
-    b = BatchLOCG(bp, x0, tol=0.0, mixed=mixed)
+    a = BatchLOCG(bp, x0, tol=0.0, mixed=mixed, lo_storage=False)
+    b = BatchLOCG(bp, x0, tol=0.0, mixed=mixed, lo_storage=False)
     for it in range(25):
         a.step()
         b.step_reference()
@@ -194,14 +194,29 @@ def test_fused_step_real_vectors(problems):
     plan = Plan(t, s)
     rng = np.random.default_rng(13)
     x0 = torch.from_numpy(rng.standard_normal(plan.n_total)).to(DEV)
-    a = BatchLOCG(plan, x0, tol=0.0, mixed=True)
-    b = BatchLOCG(plan, x0, tol=0.0, mixed=True)
+    a = BatchLOCG(plan, x0, tol=0.0, mixed=True, lo_storage=False)
+    b = BatchLOCG(plan, x0, tol=0.0, mixed=True, lo_storage=False)
     for _ in range(20):
         a.step()
         b.step_reference()
     torch.cuda.synchronize()
     assert torch.allclose(a.theta, b.theta, rtol=1e-9, atol=0)
     assert a.x.dtype == torch.float64 and bool(torch.isfinite(a.x).all())
+
+
+@pytest.mark.parametrize('lo_storage', [False, True])
+def test_bulk_phase_storage_reaches_the_same_ground_state(problems, lo_storage):
+    """Bulk phase with FP32-stored vectors or FP64 vectors + low copies: both converge to eigsh."""
+    _say(f'bulk storage lo={lo_storage}')
+    bp = BatchPlan(problems[:2] + problems[4:])
+    rng = np.random.default_rng(14)
+    x0 = torch.from_numpy(rng.standard_normal(bp.n_total) + 1j * rng.standard_normal(bp.n_total)).to(DEV)
+    theta, x, it, info = batch_ground_locg(bp, x0, tol=1e-9, maxiter=3000, check_every=10, lo_storage=lo_storage)
+    assert x.dtype == torch.complex128 and info['switch_iteration'] is not None
+    for k, (t, s) in enumerate(problems[:2] + problems[4:]):
+        h = _csr(Plan(t, s))
+        e = scipy.sparse.linalg.eigsh(h, k=1, which='SA', tol=1e-12)[0][0]
+        assert abs(float(theta[k]) - e) < 1e-9 * max(1., abs(e)), (k, float(theta[k]), e)
 
 
 def test_batch_graph_replays_match_eager(problems):
