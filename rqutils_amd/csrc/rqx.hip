@@ -291,6 +291,42 @@ __device__ inline void grad_group(const KParams &p, uint32_t g, uint64_t s, bool
     }
 }
 
+// C_g(s) in FP64 and in FP32 (each summed in its own precision, term by term in
+// the order of group_coef_from) from one pass over the terms: the fill of a
+// cache built in both precisions
+template <bool RE_ONLY>
+__device__ inline void group_coef_dual(const KParams &p, uint32_t g, uint64_t s, double2 &hi, float2 &lo) {
+    const double2 *c64 = static_cast<const double2 *>(p.coef);
+    const float2 *c32 = static_cast<const float2 *>(p.coef_lo);
+    hi = make_double2(0.0, 0.0);
+    lo = make_float2(0.f, 0.f);
+    const uint32_t k1 = p.g_toff[g + 1];
+    for (uint32_t k = p.g_toff[g]; k < k1; k += TERM_PAD) {
+        uint64_t z[TERM_PAD];
+        double2 ch[TERM_PAD];
+        float2 cl[TERM_PAD];
+#pragma unroll
+        for (int u = 0; u < TERM_PAD; ++u) {
+            z[u] = p.z[k + u];
+            ch[u] = c64[k + u];
+            cl[u] = c32[k + u];
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int u = 0; u < TERM_PAD; ++u) {
+            if (k + u < k1) {
+                const bool odd = __popcll(s & z[u]) & 1;
+                hi.x += odd ? -ch[u].x : ch[u].x;
+                lo.x += odd ? -cl[u].x : cl[u].x;
+                if constexpr (!RE_ONLY) {
+                    hi.y += odd ? -ch[u].y : ch[u].y;
+                    lo.y += odd ? -cl[u].y : cl[u].y;
+                }
+            }
+        }
+    }
+}
+
 // One tile: <= 256 consecutive rows of one segment, one row per thread.
 template <int MODE, typename R, bool CPLX>
 __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s_keys, int32_t *s_tseg,
@@ -381,13 +417,21 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
                                                 : static_cast<int64_t>(p.c_wa[i / SLICE]) + static_cast<int64_t>(n_hit);
                         const int64_t slot = p.c_off[i / SLICE] + k * SLICE + (i % SLICE);
                         p.c_col[slot] = static_cast<int32_t>(ta + base);
-                        if constexpr (CPLX) static_cast<CC *>(p.c_val)[slot] = group_coef<R>(p, g, s);
-                        else static_cast<R *>(p.c_val)[slot] = group_coef_re_from<R>(p.coef, p, g, s);
-                        if (p.c_val_lo) {        // uniform: both precisions in one pass
-                            if constexpr (CPLX)
-                                static_cast<float2 *>(p.c_val_lo)[slot] = group_coef_from<float>(p.coef_lo, p, g, s);
-                            else
-                                static_cast<float *>(p.c_val_lo)[slot] = group_coef_re_from<float>(p.coef_lo, p, g, s);
+                        if (p.c_val_lo) {        // uniform: both precisions (R = double) in one pass
+                            double2 vh;
+                            float2 vl;
+                            group_coef_dual<!CPLX>(p, g, s, vh, vl);
+                            if constexpr (CPLX) {
+                                static_cast<CC *>(p.c_val)[slot] = CC{static_cast<R>(vh.x), static_cast<R>(vh.y)};
+                                static_cast<float2 *>(p.c_val_lo)[slot] = vl;
+                            } else {
+                                static_cast<R *>(p.c_val)[slot] = static_cast<R>(vh.x);
+                                static_cast<float *>(p.c_val_lo)[slot] = vl.x;
+                            }
+                        } else if constexpr (CPLX) {
+                            static_cast<CC *>(p.c_val)[slot] = group_coef<R>(p, g, s);
+                        } else {
+                            static_cast<R *>(p.c_val)[slot] = group_coef_re_from<R>(p.coef, p, g, s);
                         }
                         if (sec_a) n_hit_a += 1;
                         else n_hit += 1;
