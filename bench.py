@@ -243,8 +243,54 @@ def locg_cached(plan, complex_, maxiter, mixed_tol, energy_tol, e_star, target=1
             'speedup': round(t_plain / t, 3), 'mixed_graph': g}
 
 
+def locg_large(plan, maxiter=1000, energy_tol=1e-9, target=1e-8):
+    """Ground energy of the 25M-determinant model to 1e-8: plain FP64 LOCG vs mixed, both on hit caches.
+
+    E* is the plain run's own converged energy (the cached matvec equals the
+    matrix-free one bit for bit, so this is the matrix-free plain run too).
+    """
+    import torch
+    from rqutils_amd.ground_locg import ground_locg
+    from rqutils_amd.locg_batch import batch_ground_locg
+    diag = plan.diagonal().real[:plan.n_states]
+    x0 = torch.zeros(plan.n_total, dtype=torch.complex128, device=plan.device)
+    x0[int(torch.argmin(diag))] = 1
+    del diag
+    plan.drop_cache()
+    torch.cuda.synchronize()
+    trace = []
+    t0 = time.perf_counter()
+    plan.build_cache(64)
+    theta, _, niter = ground_locg(plan.matvec, x0.clone(), maxiter=maxiter,
+                                  callback=lambda k, th: trace.append((time.perf_counter() - t0, th)))
+    torch.cuda.synchronize()
+    t_total = time.perf_counter() - t0
+    e_star = float(theta)
+    t_plain = next((t for t, th in trace if abs(th - e_star) <= target * abs(e_star)), t_total)
+    log(f'25M plain LOCG (cached): E*={e_star:.10f}, {niter} iterations, within {target:g} after {t_plain:.2f} s')
+    plan.drop_cache()
+    torch.cuda.empty_cache()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    plan.build_cache(0)
+    torch.cuda.synchronize()
+    t_build = time.perf_counter() - t0
+    th, _, it, info = batch_ground_locg(plan, x0, tol=1e-12, maxiter=maxiter, check_every=5, energy_tol=energy_tol)
+    torch.cuda.synchronize()
+    t_mixed = time.perf_counter() - t0
+    err = abs(float(th[0]) - e_star) / abs(e_star)
+    log(f'25M mixed LOCG (cached, graph): rel err {err:.2e} in {t_mixed:.2f} s incl. {t_build:.2f} s build '
+        f'({it} iterations)')
+    plan.drop_cache()
+    return {'e_star': e_star, 'plain_c128_cached': {'time_to_target_s': round(t_plain, 3), 'total_s': round(t_total, 3),
+                                                     'iterations': niter},
+            'mixed_graph_cached': {'time_s': round(t_mixed, 3), 'cache_build_s': round(t_build, 3), 'iterations': it,
+                                   'switch_iteration': info['switch_iteration'], 'rel_err_vs_plain': err},
+            'speedup': round(t_plain / t_mixed, 3)}
+
+
 def extra_large_matvec(reps=3):
-    """25M-determinant JW matvec (40 spin-orbitals, 5000 x 5000), complex64."""
+    """25M-determinant JW model (40 spin-orbitals, 5000 x 5000): complex64 matvec and LOCG to 1e-8."""
     import numpy as np
     import torch
     from rqutils_amd import workloads
@@ -267,11 +313,15 @@ def extra_large_matvec(reps=3):
         cached = cache_leg(plan, torch.complex64, reps=5)
     except Exception as exc:  # the cache may not fit next to other allocations
         cached = {'error': str(exc)}
+    try:
+        locg = locg_large(plan)
+    except Exception as exc:
+        locg = {'error': str(exc)}
     out = {'workload': w.name, 'n_states': plan.n_states, 'n_groups': plan.info['n_groups'],
            'n_terms': plan.info['n_terms'], 'hits_per_matvec': hits, 'split_bits': plan.info['split_bits'],
            'dtype': 'c64', 'ms_per_matvec': round(ms, 3), 'algorithmic_bytes_per_matvec': alg,
            'achieved_GBs': round(alg / ms / 1e6, 1), 'frac_of_8TBs': round(alg / ms / 1e6 / HBM_PEAK_GBS, 4),
-           'setup_s': round(setup, 2), 'cached': cached}
+           'setup_s': round(setup, 2), 'cached': cached, 'locg': locg}
     del plan
     torch.cuda.empty_cache()
     return out
