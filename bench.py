@@ -32,11 +32,11 @@ def log(msg):
 def load_workload(name):
     from rqutils_amd import workloads
     if name == 'jw1m':
-        return workloads.config1()
+        return workloads.jw32()
     if name == 'jw25m':
-        return workloads.config4()
+        return workloads.jw40()
     if name == 'rand8k':
-        return workloads.config0()
+        return workloads.random20_300t()
     raise ValueError(f'unknown workload {name}')
 
 
@@ -295,7 +295,7 @@ def extra_large_matvec(reps=3):
     import torch
     from rqutils_amd import workloads
     from rqutils_amd.plan import Plan
-    w = workloads.config4()
+    w = workloads.jw40()
     t0 = time.perf_counter()
     plan = Plan(w.tables, w.states)
     torch.cuda.synchronize()
@@ -334,7 +334,7 @@ def extra_batch(maxiter=1000):
     from rqutils_amd import workloads
     from rqutils_amd.plan import BatchPlan
     from rqutils_amd.locg_batch import batch_ground_locg
-    probs = [workloads.config2_problem(i) for i in range(64)]
+    probs = [workloads.batch40_problem(i) for i in range(64)]
     t0 = time.perf_counter()
     bp = BatchPlan([(p.tables, p.states) for p in probs])
     torch.cuda.synchronize()
@@ -383,7 +383,7 @@ def extra_grad(reps=5):
     import torch
     from rqutils_amd import workloads
     from rqutils_amd.plan import Plan
-    w = workloads.config3()
+    w = workloads.random36_10kt()
     plan = Plan(w.tables, w.states)
     rng = np.random.default_rng(3003)
     n = plan.n_total

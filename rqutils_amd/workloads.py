@@ -2,7 +2,7 @@
 
 No counterpart in the reference (its example draws unseeded random Pauli
 strings, ``examples/sqd.py:131-142``).  Every draw uses
-``numpy.random.default_rng(1000 * config + role)`` with role 0 = Hamiltonian,
+``numpy.random.default_rng(1000 * stream + role)`` (one stream per workload) with role 0 = Hamiltonian,
 1 = states, 2 = start vector, 3 = gradient probes.
 
 Jordan-Wigner convention: spin-orbital ``(p, alpha)`` is qubit ``p`` and
@@ -20,12 +20,12 @@ import numpy as np
 
 from rqutils_amd.paulis.symplectic import TermTables
 
-P_CFG0 = (0.7, 0.1, 0.1, 0.1)
-P_CFG3 = (0.6, 0.133, 0.133, 0.134)
+P_SPARSE = (0.7, 0.1, 0.1, 0.1)
+P_DENSE36 = (0.6, 0.133, 0.133, 0.134)
 
 
-def rng_for(config, role):
-    return np.random.default_rng(1000 * config + role)
+def rng_for(stream, role):
+    return np.random.default_rng(1000 * stream + role)
 
 
 def random_pauli_codes(num_qubits, num_terms, probs, rng):
@@ -196,7 +196,7 @@ def product_subspace(num_spatial, n_alpha, n_beta, n_elec_alpha, n_elec_beta, rn
     return states.reshape(-1)
 
 
-# ---------------------------------------------------------------- named configurations
+# ---------------------------------------------------------------- named workloads
 
 @dataclass
 class Workload:
@@ -208,54 +208,54 @@ class Workload:
     alphas: np.ndarray | None = None
 
 
-def config0():
+def random20_300t():
     """20-qubit, 300-term random Pauli sum (I/X/Y/Z 0.7/0.1/0.1/0.1, N(0,1)); 8192 states."""
     rng = rng_for(0, 0)
-    codes = random_pauli_codes(20, 300, P_CFG0, rng)
+    codes = random_pauli_codes(20, 300, P_SPARSE, rng)
     alpha = rng.standard_normal(300)
     xm, zm = codes_to_masks(codes)
     tables = TermTables.from_masks(xm, zm, alpha, 20)
     states = random_subspace(20, 8192, rng_for(0, 1))
-    return Workload('cfg0-random-20q-300t-8192', tables, states, 20, codes_to_strings(codes), alpha)
+    return Workload('random-20q-300t-8192', tables, states, 20, codes_to_strings(codes), alpha)
 
 
-def jw_config(num_spatial, n_alpha, n_beta, n_elec, config):
-    rng = rng_for(config, 0)
+def jw_model(num_spatial, n_alpha, n_beta, n_elec, stream):
+    rng = rng_for(stream, 0)
     h, g = random_integrals(num_spatial, rng)
     tables = jw_hamiltonian(h, g)
-    states = product_subspace(num_spatial, n_alpha, n_beta, n_elec, n_elec, rng_for(config, 1))
-    return Workload(f'cfg{config}-jw-{2 * num_spatial}so-{n_alpha}x{n_beta}', tables, states, 2 * num_spatial)
+    states = product_subspace(num_spatial, n_alpha, n_beta, n_elec, n_elec, rng_for(stream, 1))
+    return Workload(f'jw-{2 * num_spatial}so-{n_alpha}x{n_beta}', tables, states, 2 * num_spatial)
 
 
-def config1(n_alpha=1000, n_beta=1000):
+def jw32(n_alpha=1000, n_beta=1000):
     """JW random model, 16 spatial orbitals, 8a+8b electrons, 1000 x 1000 determinants."""
-    return jw_config(16, n_alpha, n_beta, 8, 1)
+    return jw_model(16, n_alpha, n_beta, 8, 1)
 
 
-def config4(n_alpha=5000, n_beta=5000):
+def jw40(n_alpha=5000, n_beta=5000):
     """JW random model, 20 spatial orbitals, 10a+10b electrons, 5000 x 5000 determinants."""
-    return jw_config(20, n_alpha, n_beta, 10, 4)
+    return jw_model(20, n_alpha, n_beta, 10, 4)
 
 
-def config3(num_states=1_000_000, num_terms=10_000):
+def random36_10kt(num_states=1_000_000, num_terms=10_000):
     """36-qubit, 10k-term random Pauli sum (0.6/0.133/0.133/0.134) on 1M random states."""
     rng = rng_for(3, 0)
-    codes = random_pauli_codes(36, num_terms, P_CFG3, rng)
+    codes = random_pauli_codes(36, num_terms, P_DENSE36, rng)
     alpha = rng.standard_normal(num_terms)
     xm, zm = codes_to_masks(codes)
     tables = TermTables.from_masks(xm, zm, alpha, 36)
     states = random_subspace(36, num_states, rng_for(3, 1))
-    return Workload(f'cfg3-random-36q-{num_terms}t-{num_states}', tables, states, 36, None, alpha)
+    return Workload(f'random-36q-{num_terms}t-{num_states}', tables, states, 36, None, alpha)
 
 
-def config2_problem(index):
+def batch40_problem(index):
     """Problem ``index`` of the 64-problem batch: 40 qubits, T ~ U[500, 5000], N log-U[1e3, 1e6]."""
     rng = np.random.default_rng(2000 + 17 * index)
     num_terms = int(rng.integers(500, 5001))
     num_states = int(round(math.exp(rng.uniform(math.log(1e3), math.log(1e6)))))
-    codes = random_pauli_codes(40, num_terms, P_CFG0, rng)
+    codes = random_pauli_codes(40, num_terms, P_SPARSE, rng)
     alpha = rng.standard_normal(num_terms)
     xm, zm = codes_to_masks(codes)
     tables = TermTables.from_masks(xm, zm, alpha, 40)
     states = random_subspace(40, num_states, np.random.default_rng(2001 + 17 * index))
-    return Workload(f'cfg2-p{index}-40q-{num_terms}t-{num_states}', tables, states, 40, None, alpha)
+    return Workload(f'batch-p{index}-40q-{num_terms}t-{num_states}', tables, states, 40, None, alpha)

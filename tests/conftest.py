@@ -9,7 +9,7 @@ if ROOT not in sys.path:
 
 
 def pytest_configure(config):
-    config.addinivalue_line('markers', 'gpu: needs a ROCm GPU (runs on the MI355X box)')
+    config.addinivalue_line('markers', 'gpu: needs a ROCm GPU (MI355X)')
 
 
 def pytest_runtest_setup(item):

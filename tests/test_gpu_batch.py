@@ -35,7 +35,7 @@ def _random_problem(seed, n, t, ns, probs=(0.55, 0.15, 0.15, 0.15)):
 def problems():
     out = [_random_problem(1, 10, 60, 500), _random_problem(2, 12, 120, 2000),
            _random_problem(3, 16, 300, 3000, probs=(0.7, 0.1, 0.1, 0.1)), _random_problem(4, 9, 20, 50)]
-    w = workloads.config1(n_alpha=30, n_beta=25)
+    w = workloads.jw32(n_alpha=30, n_beta=25)
     out.append((w.tables, w.states))
     return out
 
@@ -69,7 +69,7 @@ def test_batch_matvec_equals_single_plans(problems):
 def test_persistent_workgroups_take_several_tiles(problems):
     """More tiles than persistent workgroups: every mode equals the one-tile-per-workgroup launch."""
     _say('persistent multi-tile')
-    w = workloads.config1(n_alpha=300, n_beta=300)
+    w = workloads.jw32(n_alpha=300, n_beta=300)
     bp = BatchPlan(list(problems) + [(w.tables, w.states)])
     bp.set_launch(True, 1)                       # grid = number of CUs
     assert bp.info['n_tiles'] > 2 * bp.info['n_cu']
@@ -232,7 +232,7 @@ def test_batch_graph_replays_match_eager(problems):
 
 def test_autotune_picks_a_valid_plan():
     _say('autotune')
-    w = workloads.config1(n_alpha=200, n_beta=200)
+    w = workloads.jw32(n_alpha=200, n_beta=200)
     plan, results = autotune(w.tables, w.states, reps=2)
     assert results and results[0]['ms'] <= results[-1]['ms']
     assert plan.info['split_bits'] == results[0]['split_bits']

@@ -34,7 +34,7 @@ def _random_problem(seed, n, t, ns, probs=(0.55, 0.15, 0.15, 0.15), drop_diagona
 
 @pytest.fixture(scope='module')
 def cases():
-    w = workloads.config1(n_alpha=40, n_beta=30)
+    w = workloads.jw32(n_alpha=40, n_beta=30)
     return {
         'complex12': _random_problem(2, 12, 120, 2000),
         'real_jw': (w.tables, w.states),
@@ -193,7 +193,7 @@ def test_run_sqd_mixed_precision_matches_fp64(cs):
     _say(f'run_sqd mixed cs={cs}')
     from rqutils_amd.paulis.symplectic import PauliSumXZ, masks_to_packed
     from rqutils_amd.sqd import SQDOptions, run_sqd
-    w = workloads.config1(n_alpha=60, n_beta=50)
+    w = workloads.jw32(n_alpha=60, n_beta=50)
     h = PauliSumXZ.from_tables(w.tables, add_padding=True)
     states_p = masks_to_packed(w.states, w.tables.num_qubits, padded=True)
     size = len(w.states) + 9

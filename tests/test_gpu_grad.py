@@ -120,7 +120,7 @@ def test_hellmann_feynman_matches_energy_fd():
 
 def test_grad_real_dtype_jw():
     _say('grad real dtype jw')
-    w = workloads.config1(n_alpha=30, n_beta=30)
+    w = workloads.jw32(n_alpha=30, n_beta=30)
     plan = Plan(w.tables, w.states)
     rng = np.random.default_rng(4)
     x = torch.from_numpy(rng.standard_normal(plan.n_total)).to(DEV)
@@ -132,10 +132,10 @@ def test_grad_real_dtype_jw():
     assert torch.allclose(g32, g_real, rtol=0, atol=2e-4 * g_real.abs().max().item())
 
 
-def test_grad_identities_at_cfg3_shape():
+def test_grad_identities_at_36q_10kt_1m_shape():
     """36 qubits, 10k terms, 1M states: sum_k alpha_k g_k = Re <gy, H x> (size-independent)."""
-    _say('grad identities cfg3')
-    w = workloads.config3()
+    _say('grad identities rand36')
+    w = workloads.random36_10kt()
     plan = Plan(w.tables, w.states)
     rng = np.random.default_rng(3003)
     n = plan.n_total
@@ -152,7 +152,7 @@ def test_grad_identities_at_cfg3_shape():
 
 def test_hellmann_feynman_jw_ground_state():
     _say('hf jw ground state')
-    w = workloads.config1(n_alpha=40, n_beta=40)
+    w = workloads.jw32(n_alpha=40, n_beta=40)
     plan = Plan(w.tables, w.states)
     diag = plan.diagonal().real
     theta, psi, _ = ground_locg(plan.matvec, torch.argmin(diag), maxiter=1500,

@@ -160,7 +160,7 @@ def test_matmul_full_space_matches_dense():
 def test_locg_jw_matches_eigsh():
     _say('locg jw small vs eigsh')
     import scipy.sparse.linalg
-    w = workloads.config1(n_alpha=40, n_beta=40)
+    w = workloads.jw32(n_alpha=40, n_beta=40)
     plan = Plan(w.tables, w.states)
     cols, vals = plan.lookup(values=True)
     n = plan.n_states

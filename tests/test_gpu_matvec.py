@@ -42,59 +42,59 @@ def _rel(a, b):
 
 
 @pytest.fixture(scope='module')
-def cfg0():
-    return workloads.config0()
+def rand20():
+    return workloads.random20_300t()
 
 
-def test_lookup_bit_exact_cfg0(cfg0):
-    _say('lookup cfg0')
-    plan = Plan(cfg0.tables, cfg0.states)
+def test_lookup_bit_exact_rand20(rand20):
+    _say('lookup rand20')
+    plan = Plan(rand20.tables, rand20.states)
     got = plan.lookup().cpu().numpy()
-    want = _oracle_lookup(cfg0.tables, cfg0.states, plan.n_total)
+    want = _oracle_lookup(rand20.tables, rand20.states, plan.n_total)
     assert np.array_equal(got, want)
 
 
 @pytest.mark.parametrize('split', [-1, 0, 7, 13])
-def test_matvec_c128_cfg0_all_splits(cfg0, split):
-    _say(f'matvec c128 cfg0 split={split}')
-    plan = Plan(cfg0.tables, cfg0.states, split_bits=split)
+def test_matvec_c128_rand20_all_splits(rand20, split):
+    _say(f'matvec c128 rand20 split={split}')
+    plan = Plan(rand20.tables, rand20.states, split_bits=split)
     rng = np.random.default_rng(2)
     v = rng.standard_normal(plan.n_total) + 1j * rng.standard_normal(plan.n_total)
     y = plan.matvec(torch.from_numpy(v).to(DEV)).cpu().numpy()
-    want = sqdref.matvec_u64(cfg0.tables.xmask, _groups(cfg0.tables), cfg0.states, v)
+    want = sqdref.matvec_u64(rand20.tables.xmask, _groups(rand20.tables), rand20.states, v)
     assert _rel(y, want) < RTOL64
 
 
-def test_matvec_split_invariance(cfg0):
+def test_matvec_split_invariance(rand20):
     """The split only moves work between the two accumulation sections: same values to rounding."""
     _say('matvec split invariance')
     rng = np.random.default_rng(3)
     v = torch.from_numpy(rng.standard_normal(8192) + 1j * rng.standard_normal(8192)).to(DEV)
-    ys = [Plan(cfg0.tables, cfg0.states, split_bits=s).matvec(v) for s in (0, 5, 11, 13)]
+    ys = [Plan(rand20.tables, rand20.states, split_bits=s).matvec(v) for s in (0, 5, 11, 13)]
     scale = float(ys[0].abs().max())
     for y in ys[1:]:
         assert float((y - ys[0]).abs().max()) <= 1e-14 * scale
 
 
-def test_matvec_c64_cfg0(cfg0):
-    _say('matvec c64 cfg0')
-    plan = Plan(cfg0.tables, cfg0.states)
+def test_matvec_c64_rand20(rand20):
+    _say('matvec c64 rand20')
+    plan = Plan(rand20.tables, rand20.states)
     rng = np.random.default_rng(4)
     v = rng.standard_normal(plan.n_total) + 1j * rng.standard_normal(plan.n_total)
     y = plan.matvec(torch.from_numpy(v.astype(np.complex64)).to(DEV)).cpu().numpy()
-    want = sqdref.matvec_u64(cfg0.tables.xmask, _groups(cfg0.tables), cfg0.states, v.astype(np.complex64))
+    want = sqdref.matvec_u64(rand20.tables.xmask, _groups(rand20.tables), rand20.states, v.astype(np.complex64))
     assert _rel(y, want) < RTOL32
 
 
-def test_real_dtype_rejected_for_complex_hamiltonian(cfg0):
-    plan = Plan(cfg0.tables, cfg0.states)
+def test_real_dtype_rejected_for_complex_hamiltonian(rand20):
+    plan = Plan(rand20.tables, rand20.states)
     with pytest.raises(ValueError, match='complex'):
         plan.matvec(torch.zeros(plan.n_total, dtype=torch.float64, device=DEV))
 
 
 @pytest.fixture(scope='module')
 def jw_small():
-    return workloads.config1(n_alpha=60, n_beta=50)
+    return workloads.jw32(n_alpha=60, n_beta=50)
 
 
 def test_lookup_and_matvec_jw_small(jw_small):
@@ -227,10 +227,10 @@ def test_eigenpair_small_matches_eigh(n):
         assert np.linalg.norm(a @ kappa - float(theta) * kappa) < 1e-10
 
 
-def test_hermitian_and_linear_cfg1_full():
+def test_hermitian_and_linear_jw32_full():
     """Size-independent properties at the full 1M-determinant shape."""
-    _say('hermiticity cfg1 full')
-    w = workloads.config1()
+    _say('hermiticity jw32 full')
+    w = workloads.jw32()
     plan = Plan(w.tables, w.states)
     rng = np.random.default_rng(6)
     u = torch.from_numpy(rng.standard_normal(plan.n_total)).to(DEV)
@@ -269,7 +269,7 @@ def test_hermitian_and_linear_cfg1_full():
 def test_lookup_table_beyond_2_32_entries():
     """J x N above 2^32 entries (a 26 GB table): every miss, up to the last entry, reads -1."""
     _say('lookup J*N > 2^32')
-    w = workloads.config1(n_alpha=600, n_beta=600)
+    w = workloads.jw32(n_alpha=600, n_beta=600)
     plan = Plan(w.tables, w.states)
     total = plan.info['n_groups'] * plan.n_total
     assert total > 2 ** 32

@@ -164,11 +164,11 @@ def test_jordan_wigner_matches_fock_space(M, seed):
     assert np.all(na[rows] == na[cols]) and np.all(nb[rows] == nb[cols])
 
 
-def test_config_shapes():
-    w0 = workloads.config0()
+def test_workload_shapes():
+    w0 = workloads.random20_300t()
     assert w0.states.shape == (8192,) and np.all(np.diff(w0.states.astype(np.int64)) > 0)
     assert w0.tables.num_terms == 300 and not w0.tables.is_real
-    w1 = workloads.config1(n_alpha=40, n_beta=30)
+    w1 = workloads.jw32(n_alpha=40, n_beta=30)
     assert w1.tables.num_groups == 18281 and w1.tables.max_group == 529
     assert w1.states.shape == (1200,)
     pc = np.bitwise_count(w1.states & np.uint64(0xffff))
