@@ -169,6 +169,19 @@ int rqx_matvec(const rqx_plan *plan, const void *x, void *y, int32_t dtype, rqx_
 int rqx_count(const rqx_plan *plan, int64_t *counts, rqx_stream_t stream);
 
 /*
+ * Every nonzero of the projected Hamiltonian as (row, column, element)
+ * triplets: the diagonal (x == 0 group) and every hit, elements complex
+ * (interleaved doubles).  Replaces the J x N columns/elements scan of hproj
+ * (rqutils/sqd.py:300-314) with output proportional to the nonzeros.  rows,
+ * cols, vals: device arrays of `capacity` entries (rqx_count's counts[0] is
+ * the exact number); *count (device) receives the number of nonzeros, also
+ * when it exceeds capacity (then only `capacity` are written).  The order of
+ * the triplets is unspecified (one atomic claim per wave).
+ */
+int rqx_plan_hits(const rqx_plan *plan, int64_t *rows, int32_t *cols, double *vals, int64_t capacity,
+                  int64_t *count, rqx_stream_t stream);
+
+/*
  * grad[k] = sum_i Re( conj(gy_i) * (P_k x)_i ) for every term k (plan term
  * order), i.e. the gradient of Re<gy, H x> with respect to the unphased real
  * coefficient alpha_k of term k, where (P_k x)_i = (-i)^{x.z_k}

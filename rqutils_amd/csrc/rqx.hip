@@ -53,7 +53,7 @@ int fail(int code, const std::string &msg) {
             return fail(RQX_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
-enum { MODE_MATVEC = 0, MODE_LOOKUP = 1, MODE_COUNT = 2, MODE_GRAD = 3, MODE_ROWCOUNT = 4, MODE_FILL = 5 };
+enum { MODE_MATVEC = 0, MODE_LOOKUP = 1, MODE_COUNT = 2, MODE_GRAD = 3, MODE_ROWCOUNT = 4, MODE_FILL = 5, MODE_COO = 6 };
 
 // hit cache: sliced ELL, 64 rows per slice (one wave), two sections per slice.
 // Section A holds the hits of groups with no flips in the low split bits (the
@@ -113,6 +113,11 @@ struct KParams {
     const int64_t *c_off;          // fill mode: first entry of each slice
     const uint32_t *c_wa;          // fill mode: section-A entries per row of each slice
     const uint32_t *c_wb;          // fill mode: section-B entries per row of each slice
+    unsigned long long *coo_count; // coo mode: entries claimed so far
+    int64_t *coo_row;              // coo mode: (row, column, element) of every hit, in claim order
+    int32_t *coo_col;
+    double2 *coo_val;
+    int64_t coo_capacity;
     int64_t n_total;
     uint32_t n_tiles;
     int32_t coef_real;             // every phased coefficient is real: sums need only their real parts
@@ -327,6 +332,27 @@ __device__ inline void group_coef_dual(const KParams &p, uint32_t g, uint64_t s,
     }
 }
 
+// Append (row, col, value) for the lanes with hit set: one atomic per wave for
+// the wave's entries, positions in lane order; entries past the capacity are
+// counted but not written.
+__device__ inline void coo_emit(const KParams &p, bool hit, int64_t row, int64_t col, double2 v) {
+    const uint64_t m = __ballot(hit);
+    if (m == 0) return;
+    const int lane = threadIdx.x & 63;
+    const int leader = __ffsll(static_cast<long long>(m)) - 1;
+    unsigned long long base = 0;
+    if (lane == leader) base = atomicAdd(p.coo_count, static_cast<unsigned long long>(__popcll(m)));
+    base = __shfl(base, leader, 64);
+    if (hit) {
+        const int64_t pos = static_cast<int64_t>(base) + __popcll(m & ((1ull << lane) - 1ull));
+        if (pos < p.coo_capacity) {
+            p.coo_row[pos] = row;
+            p.coo_col[pos] = static_cast<int32_t>(col);
+            p.coo_val[pos] = v;
+        }
+    }
+}
+
 // One tile: <= 256 consecutive rows of one segment, one row per thread.
 template <int MODE, typename R, bool CPLX>
 __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s_keys, int32_t *s_tseg,
@@ -365,6 +391,10 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
         } else if constexpr (MODE == MODE_COUNT) {
             n_hit += 1;
         }
+    }
+    if constexpr (MODE == MODE_COO) {
+        if (pd.has_diag)
+            coo_emit(p, active, i, i, active ? static_cast<const double2 *>(p.diag)[i] : make_double2(0.0, 0.0));
     }
 
     const int n_classes = static_cast<int>(pd.n_classes);
@@ -411,6 +441,8 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
                 } else if constexpr (MODE == MODE_ROWCOUNT) {
                     if (sec_a) n_hit_a += hit;
                     else n_hit += hit;
+                } else if constexpr (MODE == MODE_COO) {
+                    coo_emit(p, hit, i, ta + base, hit ? group_coef<double>(p, g, s) : make_double2(0.0, 0.0));
                 } else if constexpr (MODE == MODE_FILL) {
                     if (hit) {
                         const int64_t k = sec_a ? static_cast<int64_t>(n_hit_a)
@@ -1921,6 +1953,24 @@ int rqx_count(const rqx_plan *p, int64_t *counts, rqx_stream_t stream) {
     KParams k = make_params(p, true);
     k.counts = reinterpret_cast<unsigned long long *>(counts);
     HIP_TRY((launch_tiles<MODE_COUNT, double, true>(p, k, nullptr, nullptr, st)));
+    return RQX_OK;
+}
+
+int rqx_plan_hits(const rqx_plan *p, int64_t *rows, int32_t *cols, double *vals, int64_t capacity, int64_t *count,
+                  rqx_stream_t stream) {
+    g_err.clear();
+    if (int rc = check_plan(p)) return rc;
+    if (!count) return fail(RQX_ERR_ARG, "count is NULL");
+    if (capacity < 0 || (capacity > 0 && (!rows || !cols || !vals))) return fail(RQX_ERR_ARG, "bad output arrays");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    HIP_TRY(fill_async(count, int64_t(0), int64_t(1), int64_t(0), st));
+    KParams k = make_params(p, true);
+    k.coo_count = reinterpret_cast<unsigned long long *>(count);
+    k.coo_row = rows;
+    k.coo_col = cols;
+    k.coo_val = reinterpret_cast<double2 *>(vals);
+    k.coo_capacity = capacity;
+    HIP_TRY((launch_tiles<MODE_COO, double, true>(p, k, nullptr, nullptr, st)));
     return RQX_OK;
 }
 

@@ -220,6 +220,23 @@ class Plan:
         hits, probes, lookups = (int(v) for v in out.cpu())
         return {'hits': hits, 'probes': probes, 'class_lookups': lookups}
 
+    def hits(self):
+        """Every nonzero as device tensors ``(rows int64, cols int64, vals complex128)``, sorted by (row, col)."""
+        nnz = self.count()['hits']
+        rows = torch.empty(max(nnz, 1), dtype=torch.int64, device=self.device)
+        cols = torch.empty(max(nnz, 1), dtype=torch.int32, device=self.device)
+        vals = torch.empty((max(nnz, 1), 2), dtype=torch.float64, device=self.device)
+        count = torch.zeros(1, dtype=torch.int64, device=self.device)
+        stream = _capi.current_stream(self.device)
+        _capi.check(_capi.lib().rqx_plan_hits(self._handle, rows.data_ptr(), cols.data_ptr(), vals.data_ptr(),
+                                              nnz, count.data_ptr(), stream))
+        got = int(count.item())
+        if got != nnz:
+            raise _capi.RqxError(_capi.RQX_ERR_HIP, f'hit pass wrote {got} entries, count pass {nnz}')
+        rows, cols, vals = rows[:nnz], cols[:nnz].to(torch.int64), torch.view_as_complex(vals[:nnz])
+        order = torch.argsort(rows * max(self.n_total, 1) + cols)
+        return rows[order], cols[order], vals[order]
+
     def algorithmic_bytes(self, dtype, hits=None):
         """Per-matvec algorithmic bytes of the HBM model.
 

@@ -27,7 +27,7 @@ from typing import Any, Optional
 
 import numpy as np
 import torch
-from scipy.sparse import coo_array, csr_array
+from scipy.sparse import csr_array
 
 from rqutils_amd.paulis.symplectic import PauliSumXZ, masks_to_packed, packed_to_masks
 from rqutils_amd._capi import RqxError
@@ -245,11 +245,10 @@ def hproj(hamiltonian: HamiltonianInput, states, unique_states: bool = False) ->
         keys = np.unique(keys)
     plan = Plan(hamiltonian.tables, keys)
     n = plan.n_states
-    cols, vals = plan.lookup(values=True)
-    hit = cols[:, :n] >= 0
-    rows = torch.nonzero(hit, as_tuple=True)[1]
-    colv = cols[:, :n][hit].to(torch.int64)
-    data = vals[:, :n][hit]
+    # one triplet per nonzero (the reference scans J x N columns and elements, sqd.py:300-314)
+    rows, cols, data = plan.hits()
     if hamiltonian.tables.is_real:
         data = data.real
-    return csr_array(coo_array((data.cpu().numpy(), (rows.cpu().numpy(), colv.cpu().numpy())), shape=(n, n)))
+    indptr = torch.zeros(n + 1, dtype=torch.int64, device=rows.device)
+    indptr[1:] = torch.cumsum(torch.bincount(rows, minlength=n), 0)
+    return csr_array((data.cpu().numpy(), cols.cpu().numpy(), indptr.cpu().numpy()), shape=(n, n))

@@ -176,3 +176,35 @@ def test_locg_jw_matches_eigsh():
     theta, vec, niter = ground_locg(plan.matvec, torch.argmin(diag), maxiter=2000,
                                    vspace=(n, torch.float64, plan.device))
     assert abs(float(theta) - e_ref) < 1e-9 * abs(e_ref)
+
+
+def test_hproj_nonzero_triplets_match_lookup_and_scale():
+    """hproj from per-hit triplets: equal to the J x N lookup construction; beyond J x N = 2^32 it matches the matvec."""
+    _say('hproj triplets')
+    import scipy.sparse
+    from rqutils_amd import workloads
+    from rqutils_amd.paulis.symplectic import PauliSumXZ
+    from rqutils_amd.plan import Plan
+    from rqutils_amd.sqd import u64_to_states
+    w = workloads.jw32(n_alpha=40, n_beta=30)
+    h = PauliSumXZ.from_tables(w.tables)
+    m = hproj(h, u64_to_states(w.states, 32), unique_states=True)
+    plan = Plan(w.tables, w.states)
+    cols, vals = plan.lookup(values=True)
+    hit = cols >= 0
+    rows = torch.nonzero(hit, as_tuple=True)[1].cpu().numpy()
+    ref = scipy.sparse.csr_array((vals[hit].real.cpu().numpy(), (rows, cols[hit].long().cpu().numpy())),
+                                 shape=m.shape)
+    assert m.nnz == ref.nnz == plan.count()['hits']
+    assert (abs(m - ref)).max() == 0
+    # a table J x N beyond 2^32 entries would not fit the lookup construction at scale;
+    # the triplets give H v = plan.matvec(v)
+    w = workloads.jw32(n_alpha=600, n_beta=600)
+    plan = Plan(w.tables, w.states)
+    assert plan.info['n_groups'] * plan.n_total > 2 ** 32
+    m = hproj(PauliSumXZ.from_tables(w.tables), u64_to_states(w.states, 32), unique_states=True)
+    rng = np.random.default_rng(31)
+    v = rng.standard_normal(plan.n_total)
+    got = m @ v
+    want = plan.matvec(torch.from_numpy(v).to(DEV)).cpu().numpy()
+    assert np.max(np.abs(got - want)) <= 1e-12 * np.max(np.abs(want))
