@@ -202,6 +202,21 @@ def test_capi_argument_errors_without_gpu():
     assert 'n_qubits' in _capi.last_error()
     rc = lib.rqx_eigenpair_small(None, 4, None, None)
     assert rc == _capi.RQX_ERR_ARG
+    # every entry point that takes a plan rejects NULL before touching a device
+    vec = (ctypes.c_void_p * 8)()
+    assert lib.rqx_plan_get_info(None, None) == _capi.RQX_ERR_ARG
+    assert lib.rqx_plan_destroy(None) == _capi.RQX_OK
+    assert lib.rqx_matvec(None, None, None, _capi.RQX_C128, None) == _capi.RQX_ERR_ARG
+    assert lib.rqx_lookup(None, None, None, None) == _capi.RQX_ERR_ARG
+    assert lib.rqx_plan_build_cache(None, 64, 0, None) == _capi.RQX_ERR_ARG
+    assert lib.rqx_plan_drop_cache(None, 0) == _capi.RQX_ERR_ARG
+    assert lib.rqx_plan_cache_info(None, 64, None) == _capi.RQX_ERR_ARG
+    assert lib.rqx_plan_use_cache(None, 1) == _capi.RQX_ERR_ARG
+    assert lib.rqx_plan_hits(None, None, None, None, 0, None, None) == _capi.RQX_ERR_ARG
+    assert lib.rqx_locg_step(None, _capi.RQX_C128, 1, vec, None, None, None, None, 1e-8, None) == _capi.RQX_ERR_ARG
+    assert lib.rqx_batch_dots(None, 1, None, None, None, _capi.RQX_C128, None) == _capi.RQX_ERR_ARG
+    assert lib.rqx_plan_update_coef(None, None, 1, None) == _capi.RQX_ERR_ARG
+    assert 'NULL' in _capi.last_error()
 
 
 def _eig_host(a):
