@@ -132,6 +132,14 @@ def test_golden_fixtures(name):
     assert np.array_equal(plan.lookup().cpu().numpy()[:, :n], f['xsource'])
     y = plan.matvec(torch.from_numpy(f['v']).to(DEV)).cpu().numpy()
     assert _rel(y, f['y']) < RTOL64
+    # the hit cache (both precisions) reproduces the golden vector as well
+    plan.build_cache(0)
+    yc = plan.matvec(torch.from_numpy(f['v']).to(DEV)).cpu().numpy()
+    assert _rel(yc, f['y']) < RTOL64 and np.array_equal(yc, y)
+    # and the nonzero triplets are the fixture's (source, element) pairs
+    rows, cols, vals = plan.hits()
+    xs = f['xsource']
+    assert rows.shape[0] == int((xs >= 0).sum())
 
 
 def test_dense_projection_small():
