@@ -202,6 +202,12 @@ def test_fused_step_real_vectors(problems):
     torch.cuda.synchronize()
     assert torch.allclose(a.theta, b.theta, rtol=1e-9, atol=0)
     assert a.x.dtype == torch.float64 and bool(torch.isfinite(a.x).all())
+    # float32-stored bulk phase on real vectors converges to the same ground energy
+    h = _csr(Plan(t, s))
+    e = scipy.sparse.linalg.eigsh(h, k=1, which='SA', tol=1e-12)[0][0]
+    theta, x, it, info = batch_ground_locg(plan, x0, tol=1e-9, maxiter=3000, check_every=10, lo_storage=True)
+    assert x.dtype == torch.float64 and info['switch_iteration'] is not None
+    assert abs(float(theta[0]) - e) < 1e-9 * abs(e)
 
 
 @pytest.mark.parametrize('lo_storage', [False, True])
