@@ -233,7 +233,9 @@ int rqx_plan_build_cache(rqx_plan *plan, int32_t precision, int64_t max_bytes, r
 
 /*
  * Release the cache of one precision (32 or 64), or both (0); caches built
- * together are released together.  Synchronises the device.
+ * together are released together.  Synchronises the device.  A matvec captured
+ * in a HIP graph keeps the path chosen at capture (the cache's arrays when one
+ * was built): release or rebuild a cache only when no such graph will replay.
  */
 int rqx_plan_drop_cache(rqx_plan *plan, int32_t precision);
 
