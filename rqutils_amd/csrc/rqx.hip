@@ -260,6 +260,64 @@ __device__ inline R wave_sum(R v) {
     return v;
 }
 
+// Coefficient sums C_g(s) of one group for the lanes with `hit`, evaluated by
+// the whole wave: row-parallel (each hitting lane sums the group's terms in
+// order) or, when few lanes hit a long group, term-parallel (the 64 lanes split
+// the terms of one hitting row -- lane j takes terms j, j + 64, ... -- and a
+// butterfly sum joins them; one hitting row after the other).  The choice
+// depends only on the group's length and the number of hitting lanes, so every
+// pass over the same tiles (matvec, cache fill) evaluates a hit the same way.
+__device__ inline bool term_parallel(uint32_t n_terms, uint32_t n_hits) {
+    return n_hits * (((n_terms + 63u) >> 6) * 10u + 20u) < n_terms * 8u;
+}
+
+template <typename R, bool RE>
+__device__ inline void group_value_wave(const void *coef_v, const KParams &p, uint32_t g, uint64_t s, bool hit,
+                                        R &vx, R &vy) {
+    using CC = typename C2<R>::T;
+    vx = R(0);
+    vy = R(0);
+    const uint64_t m = __ballot(hit);
+    if (m == 0) return;
+    const uint32_t k0 = p.g_toff[g], k1 = p.g_toff[g + 1];
+    if (!term_parallel(k1 - k0, static_cast<uint32_t>(__popcll(m)))) {
+        if (hit) {
+            if constexpr (RE) {
+                vx = group_coef_re_from<R>(coef_v, p, g, s);
+            } else {
+                const CC c = group_coef_from<R>(coef_v, p, g, s);
+                vx = c.x;
+                vy = c.y;
+            }
+        }
+        return;
+    }
+    const CC *coef = static_cast<const CC *>(coef_v);
+    const int lane = threadIdx.x & 63;
+    for (uint64_t mm = m; mm != 0; mm &= mm - 1) {
+        const int L = __ffsll(static_cast<long long>(mm)) - 1;
+        const uint64_t sl = __shfl(s, L, 64);
+        R px = R(0), py = R(0);
+        for (uint32_t k = k0 + lane; k < k1; k += 64) {
+            const CC c = coef[k];
+            const bool odd = __popcll(sl & p.z[k]) & 1;
+            px += odd ? -c.x : c.x;
+            if constexpr (!RE) py += odd ? -c.y : c.y;
+        }
+        px = wave_sum(px);
+        if constexpr (!RE) py = wave_sum(py);
+        if (lane == L) {
+            vx = px;
+            vy = py;
+        }
+    }
+}
+
+// the same in FP64 and FP32 at once (the cache fill of both precisions): each
+// precision summed exactly as group_value_wave sums it
+template <bool RE>
+__device__ inline void group_value_wave_dual(const KParams &p, uint32_t g, uint64_t s, bool hit, double2 &hi, float2 &lo);
+
 // real and imaginary parts of conj(g) * v for real or complex entries
 __device__ inline void conj_mul(double g, double v, double &re, double &im) { re = g * v; im = 0.0; }
 __device__ inline void conj_mul(float g, float v, float &re, float &im) { re = g * v; im = 0.f; }
@@ -353,6 +411,49 @@ __device__ inline void coo_emit(const KParams &p, bool hit, int64_t row, int64_t
     }
 }
 
+template <bool RE>
+__device__ inline void group_value_wave_dual(const KParams &p, uint32_t g, uint64_t s, bool hit, double2 &hi, float2 &lo) {
+    hi = make_double2(0.0, 0.0);
+    lo = make_float2(0.f, 0.f);
+    const uint64_t m = __ballot(hit);
+    if (m == 0) return;
+    const uint32_t k0 = p.g_toff[g], k1 = p.g_toff[g + 1];
+    if (!term_parallel(k1 - k0, static_cast<uint32_t>(__popcll(m)))) {
+        if (hit) group_coef_dual<RE>(p, g, s, hi, lo);
+        return;
+    }
+    const double2 *c64 = static_cast<const double2 *>(p.coef);
+    const float2 *c32 = static_cast<const float2 *>(p.coef_lo);
+    const int lane = threadIdx.x & 63;
+    for (uint64_t mm = m; mm != 0; mm &= mm - 1) {
+        const int L = __ffsll(static_cast<long long>(mm)) - 1;
+        const uint64_t sl = __shfl(s, L, 64);
+        double hx = 0.0, hy = 0.0;
+        float lx = 0.f, ly = 0.f;
+        for (uint32_t k = k0 + lane; k < k1; k += 64) {
+            const bool odd = __popcll(sl & p.z[k]) & 1;
+            const double2 ch = c64[k];
+            const float2 cl = c32[k];
+            hx += odd ? -ch.x : ch.x;
+            lx += odd ? -cl.x : cl.x;
+            if constexpr (!RE) {
+                hy += odd ? -ch.y : ch.y;
+                ly += odd ? -cl.y : cl.y;
+            }
+        }
+        hx = wave_sum(hx);
+        lx = wave_sum(lx);
+        if constexpr (!RE) {
+            hy = wave_sum(hy);
+            ly = wave_sum(ly);
+        }
+        if (lane == L) {
+            hi = make_double2(hx, hy);
+            lo = make_float2(lx, ly);
+        }
+    }
+}
+
 // One tile: <= 256 consecutive rows of one segment, one row per thread.
 template <int MODE, typename R, bool CPLX>
 __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s_keys, int32_t *s_tseg,
@@ -424,9 +525,16 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
             auto handle = [&](uint32_t g, uint32_t base, uint32_t tl, bool sec_a) {
                 const bool hit = active && (s_keys[base] == tl);
                 if constexpr (MODE == MODE_MATVEC) {
-                    if (hit) {
-                        if (p.coef_real) fma_acc(sec_a ? acc_a : acc, group_coef_re_from<R>(p.coef, p, g, s), x[ta + base]);
-                        else fma_acc(sec_a ? acc_a : acc, group_coef<R>(p, g, s), x[ta + base]);
+                    R vx, vy;
+                    if (p.coef_real) {
+                        group_value_wave<R, true>(p.coef, p, g, s, hit, vx, vy);
+                        if (hit) fma_acc(sec_a ? acc_a : acc, vx, x[ta + base]);
+                    } else {
+                        group_value_wave<R, false>(p.coef, p, g, s, hit, vx, vy);
+                        CC c;
+                        c.x = vx;
+                        c.y = vy;
+                        if (hit) fma_acc(sec_a ? acc_a : acc, c, x[ta + base]);
                     }
                 } else if constexpr (MODE == MODE_LOOKUP) {
                     if (hit) {
@@ -444,15 +552,18 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
                 } else if constexpr (MODE == MODE_COO) {
                     coo_emit(p, hit, i, ta + base, hit ? group_coef<double>(p, g, s) : make_double2(0.0, 0.0));
                 } else if constexpr (MODE == MODE_FILL) {
+                    // values first (a wave-collective evaluation), then this lane's slot
+                    double2 vh = make_double2(0.0, 0.0);
+                    float2 vl = make_float2(0.f, 0.f);
+                    R vx = R(0), vy = R(0);
+                    if (p.c_val_lo) group_value_wave_dual<!CPLX>(p, g, s, hit, vh, vl);   // both precisions, R = double
+                    else group_value_wave<R, !CPLX>(p.coef, p, g, s, hit, vx, vy);
                     if (hit) {
                         const int64_t k = sec_a ? static_cast<int64_t>(n_hit_a)
                                                 : static_cast<int64_t>(p.c_wa[i / SLICE]) + static_cast<int64_t>(n_hit);
                         const int64_t slot = p.c_off[i / SLICE] + k * SLICE + (i % SLICE);
                         p.c_col[slot] = static_cast<int32_t>(ta + base);
-                        if (p.c_val_lo) {        // uniform: both precisions (R = double) in one pass
-                            double2 vh;
-                            float2 vl;
-                            group_coef_dual<!CPLX>(p, g, s, vh, vl);
+                        if (p.c_val_lo) {
                             if constexpr (CPLX) {
                                 static_cast<CC *>(p.c_val)[slot] = CC{static_cast<R>(vh.x), static_cast<R>(vh.y)};
                                 static_cast<float2 *>(p.c_val_lo)[slot] = vl;
@@ -461,9 +572,9 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
                                 static_cast<float *>(p.c_val_lo)[slot] = vl.x;
                             }
                         } else if constexpr (CPLX) {
-                            static_cast<CC *>(p.c_val)[slot] = group_coef<R>(p, g, s);
+                            static_cast<CC *>(p.c_val)[slot] = CC{vx, vy};
                         } else {
-                            static_cast<R *>(p.c_val)[slot] = group_coef_re_from<R>(p.coef, p, g, s);
+                            static_cast<R *>(p.c_val)[slot] = vx;
                         }
                         if (sec_a) n_hit_a += 1;
                         else n_hit += 1;
