@@ -550,7 +550,9 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
                     if (sec_a) n_hit_a += hit;
                     else n_hit += hit;
                 } else if constexpr (MODE == MODE_COO) {
-                    coo_emit(p, hit, i, ta + base, hit ? group_coef<double>(p, g, s) : make_double2(0.0, 0.0));
+                    double vx, vy;
+                    group_value_wave<double, false>(p.coef, p, g, s, hit, vx, vy);
+                    coo_emit(p, hit, i, ta + base, make_double2(vx, vy));
                 } else if constexpr (MODE == MODE_FILL) {
                     // values first (a wave-collective evaluation), then this lane's slot
                     double2 vh = make_double2(0.0, 0.0);

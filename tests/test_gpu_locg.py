@@ -196,7 +196,9 @@ def test_hproj_nonzero_triplets_match_lookup_and_scale():
     ref = scipy.sparse.csr_array((vals[hit].real.cpu().numpy(), (rows, cols[hit].long().cpu().numpy())),
                                  shape=m.shape)
     assert m.nnz == ref.nnz == plan.count()['hits']
-    assert (abs(m - ref)).max() == 0
+    # same sparsity; elements agree to rounding (a long group's sum may be split across lanes)
+    assert (abs(m - ref)).max() <= 1e-13 * abs(ref).max()
+    assert np.array_equal(m.indices, ref.indices) and np.array_equal(m.indptr, ref.indptr)
     # a table J x N beyond 2^32 entries would not fit the lookup construction at scale;
     # the triplets give H v = plan.matvec(v)
     w = workloads.jw32(n_alpha=600, n_beta=600)
