@@ -120,7 +120,6 @@ struct KParams {
     int64_t coo_capacity;
     int64_t n_total;
     uint32_t n_tiles;
-    int32_t coef_real;             // every phased coefficient is real: sums need only their real parts
 };
 
 template <typename R> struct C2;
@@ -455,7 +454,8 @@ __device__ inline void group_value_wave_dual(const KParams &p, uint32_t g, uint6
 }
 
 // One tile: <= 256 consecutive rows of one segment, one row per thread.
-template <int MODE, typename R, bool CPLX>
+// REALC: every phased coefficient is real (the matvec then sums real parts only).
+template <int MODE, typename R, bool CPLX, bool REALC = false>
 __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s_keys, int32_t *s_tseg,
                                     const void *__restrict__ xin, void *__restrict__ yout) {
     using V = typename VT<R, CPLX>::T;
@@ -526,11 +526,10 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
                 const bool hit = active && (s_keys[base] == tl);
                 if constexpr (MODE == MODE_MATVEC) {
                     R vx, vy;
-                    if (p.coef_real) {
-                        group_value_wave<R, true>(p.coef, p, g, s, hit, vx, vy);
+                    group_value_wave<R, REALC>(p.coef, p, g, s, hit, vx, vy);
+                    if constexpr (REALC) {
                         if (hit) fma_acc(sec_a ? acc_a : acc, vx, x[ta + base]);
                     } else {
-                        group_value_wave<R, false>(p.coef, p, g, s, hit, vx, vy);
                         CC c;
                         c.x = vx;
                         c.y = vy;
@@ -673,17 +672,17 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
 }
 
 // One workgroup per tile; tiles of one XCD are contiguous.
-template <int MODE, typename R, bool CPLX>
+template <int MODE, typename R, bool CPLX, bool REALC = false>
 __global__ __launch_bounds__(TILE) void k_tiles(const KParams p, const void *__restrict__ xin,
                                                 void *__restrict__ yout) {
     extern __shared__ uint32_t s_keys[];      // staged target segment (max_segment keys)
     __shared__ int32_t s_tseg[TILE];
-    process_tile<MODE, R, CPLX>(p, xcd_remap(blockIdx.x, gridDim.x), s_keys, s_tseg, xin, yout);
+    process_tile<MODE, R, CPLX, REALC>(p, xcd_remap(blockIdx.x, gridDim.x), s_keys, s_tseg, xin, yout);
 }
 
 // Persistent workgroups pull tiles from a queue (tiles are stored largest
 // first); every workgroup leaves when the queue is exhausted.
-template <int MODE, typename R, bool CPLX>
+template <int MODE, typename R, bool CPLX, bool REALC = false>
 __global__ __launch_bounds__(TILE) void k_tiles_queue(const KParams p, const void *__restrict__ xin,
                                                       void *__restrict__ yout) {
     extern __shared__ uint32_t s_keys[];
@@ -700,7 +699,7 @@ __global__ __launch_bounds__(TILE) void k_tiles_queue(const KParams p, const voi
         const uint32_t tile = __builtin_amdgcn_readfirstlane(s_tile);
         __syncthreads();              // every thread has read s_tile before the next claim
         if (tile >= p.n_tiles) break;
-        process_tile<MODE, R, CPLX>(p, tile, s_keys, s_tseg, xin, yout);
+        process_tile<MODE, R, CPLX, REALC>(p, tile, s_keys, s_tseg, xin, yout);
         __syncthreads();              // the whole workgroup leaves the tile before the next claim
     }
 }
@@ -1492,7 +1491,6 @@ KParams make_params(const rqx_plan *p, bool fp64) {
     k.queue = p->queue;
     k.n_total = p->n_total;
     k.n_tiles = static_cast<uint32_t>(p->n_tiles);
-    k.coef_real = p->is_real;
     return k;
 }
 
@@ -1506,7 +1504,7 @@ int check_plan(const rqx_plan *p) {
     return RQX_OK;
 }
 
-template <int MODE, typename R, bool CPLX>
+template <int MODE, typename R, bool CPLX, bool REALC = false>
 hipError_t launch_tiles(const rqx_plan *p, const KParams &k, const void *x, void *y, hipStream_t st) {
     if (p->n_tiles == 0) return hipSuccess;
     const size_t lds = seg_lds_bytes(p);
@@ -1514,9 +1512,9 @@ hipError_t launch_tiles(const rqx_plan *p, const KParams &k, const void *x, void
         hipLaunchKernelGGL(k_fill<unsigned int>, dim3(1), dim3(64), 0, st, p->queue, int64_t(0), int64_t(1), 0u);
         const int bpc = p->blocks_per_cu > 0 ? p->blocks_per_cu : p->occupancy;
         const int grid = std::max(1, std::min(p->n_tiles, p->n_cu * bpc));
-        hipLaunchKernelGGL((k_tiles_queue<MODE, R, CPLX>), dim3(grid), dim3(TILE), lds, st, k, x, y);
+        hipLaunchKernelGGL((k_tiles_queue<MODE, R, CPLX, REALC>), dim3(grid), dim3(TILE), lds, st, k, x, y);
     } else {
-        hipLaunchKernelGGL((k_tiles<MODE, R, CPLX>), dim3(p->n_tiles), dim3(TILE), lds, st, k, x, y);
+        hipLaunchKernelGGL((k_tiles<MODE, R, CPLX, REALC>), dim3(p->n_tiles), dim3(TILE), lds, st, k, x, y);
     }
     return hipGetLastError();
 }
@@ -1524,10 +1522,15 @@ hipError_t launch_tiles(const rqx_plan *p, const KParams &k, const void *x, void
 template <int MODE>
 hipError_t launch_dtype(const rqx_plan *p, const KParams &k, const void *x, void *y, int32_t dtype, hipStream_t st) {
     switch (dtype) {
-        case RQX_F64: return launch_tiles<MODE, double, false>(p, k, x, y, st);
-        case RQX_C128: return launch_tiles<MODE, double, true>(p, k, x, y, st);
-        case RQX_F32: return launch_tiles<MODE, float, false>(p, k, x, y, st);
-        default: return launch_tiles<MODE, float, true>(p, k, x, y, st);
+        // real vectors imply real coefficients; complex vectors pick the kernel by the coefficients
+        case RQX_F64: return launch_tiles<MODE, double, false, MODE == MODE_MATVEC>(p, k, x, y, st);
+        case RQX_F32: return launch_tiles<MODE, float, false, MODE == MODE_MATVEC>(p, k, x, y, st);
+        case RQX_C128:
+            if (MODE == MODE_MATVEC && p->is_real) return launch_tiles<MODE, double, true, MODE == MODE_MATVEC>(p, k, x, y, st);
+            return launch_tiles<MODE, double, true>(p, k, x, y, st);
+        default:
+            if (MODE == MODE_MATVEC && p->is_real) return launch_tiles<MODE, float, true, MODE == MODE_MATVEC>(p, k, x, y, st);
+            return launch_tiles<MODE, float, true>(p, k, x, y, st);
     }
 }
 
