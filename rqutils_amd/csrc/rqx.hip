@@ -269,13 +269,29 @@ __device__ inline R wave_sum(R v) {
 __device__ inline bool term_parallel(uint32_t n_terms, uint32_t n_hits) {
     return n_hits * (((n_terms + 63u) >> 6) * 10u + 20u) < n_terms * 8u;
 }
+// the rule never picks term-parallel for groups shorter than this (8 K > 30
+// at one hit); plans without such groups launch matvec kernels compiled with
+// the row-parallel path only (TP = false), which give the same sums
+constexpr int TP_MIN_TERMS = 4;
 
-template <typename R, bool RE>
+template <typename R, bool RE, bool TP = true>
 __device__ inline void group_value_wave(const void *coef_v, const KParams &p, uint32_t g, uint64_t s, bool hit,
                                         R &vx, R &vy) {
     using CC = typename C2<R>::T;
     vx = R(0);
     vy = R(0);
+    if constexpr (!TP) {
+        if (hit) {
+            if constexpr (RE) {
+                vx = group_coef_re_from<R>(coef_v, p, g, s);
+            } else {
+                const CC c = group_coef_from<R>(coef_v, p, g, s);
+                vx = c.x;
+                vy = c.y;
+            }
+        }
+        return;
+    }
     const uint64_t m = __ballot(hit);
     if (m == 0) return;
     const uint32_t k0 = p.g_toff[g], k1 = p.g_toff[g + 1];
@@ -455,7 +471,9 @@ __device__ inline void group_value_wave_dual(const KParams &p, uint32_t g, uint6
 
 // One tile: <= 256 consecutive rows of one segment, one row per thread.
 // REALC: every phased coefficient is real (the matvec then sums real parts only).
-template <int MODE, typename R, bool CPLX, bool REALC = false>
+// TP: some group may be evaluated term-parallel (the plan has groups of
+// TP_MIN_TERMS or more terms).
+template <int MODE, typename R, bool CPLX, bool REALC = false, bool TP = true>
 __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s_keys, int32_t *s_tseg,
                                     const void *__restrict__ xin, void *__restrict__ yout) {
     using V = typename VT<R, CPLX>::T;
@@ -526,7 +544,7 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
                 const bool hit = active && (s_keys[base] == tl);
                 if constexpr (MODE == MODE_MATVEC) {
                     R vx, vy;
-                    group_value_wave<R, REALC>(p.coef, p, g, s, hit, vx, vy);
+                    group_value_wave<R, REALC, TP>(p.coef, p, g, s, hit, vx, vy);
                     if constexpr (REALC) {
                         if (hit) fma_acc(sec_a ? acc_a : acc, vx, x[ta + base]);
                     } else {
@@ -672,17 +690,17 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
 }
 
 // One workgroup per tile; tiles of one XCD are contiguous.
-template <int MODE, typename R, bool CPLX, bool REALC = false>
+template <int MODE, typename R, bool CPLX, bool REALC = false, bool TP = true>
 __global__ __launch_bounds__(TILE) void k_tiles(const KParams p, const void *__restrict__ xin,
                                                 void *__restrict__ yout) {
     extern __shared__ uint32_t s_keys[];      // staged target segment (max_segment keys)
     __shared__ int32_t s_tseg[TILE];
-    process_tile<MODE, R, CPLX, REALC>(p, xcd_remap(blockIdx.x, gridDim.x), s_keys, s_tseg, xin, yout);
+    process_tile<MODE, R, CPLX, REALC, TP>(p, xcd_remap(blockIdx.x, gridDim.x), s_keys, s_tseg, xin, yout);
 }
 
 // Persistent workgroups pull tiles from a queue (tiles are stored largest
 // first); every workgroup leaves when the queue is exhausted.
-template <int MODE, typename R, bool CPLX, bool REALC = false>
+template <int MODE, typename R, bool CPLX, bool REALC = false, bool TP = true>
 __global__ __launch_bounds__(TILE) void k_tiles_queue(const KParams p, const void *__restrict__ xin,
                                                       void *__restrict__ yout) {
     extern __shared__ uint32_t s_keys[];
@@ -699,7 +717,7 @@ __global__ __launch_bounds__(TILE) void k_tiles_queue(const KParams p, const voi
         const uint32_t tile = __builtin_amdgcn_readfirstlane(s_tile);
         __syncthreads();              // every thread has read s_tile before the next claim
         if (tile >= p.n_tiles) break;
-        process_tile<MODE, R, CPLX, REALC>(p, tile, s_keys, s_tseg, xin, yout);
+        process_tile<MODE, R, CPLX, REALC, TP>(p, tile, s_keys, s_tseg, xin, yout);
         __syncthreads();              // the whole workgroup leaves the tile before the next claim
     }
 }
@@ -1396,6 +1414,7 @@ struct ProbHost {            // host record of one problem of a plan
     int32_t cls_base = 0, n_classes = 0;
     int32_t n_tiles = 0, max_seg = 0, L = 0, n_qubits = 0;
     int32_t has_diag = 0, max_group_terms = 0;
+    int32_t max_offdiag_terms = 0;   // longest group with x != 0
     uint32_t diag_terms = 0;   // terms of the x == 0 group
     int32_t is_real = 1;
     double cost = 0.0;         // cost-model estimate of one matvec
@@ -1419,6 +1438,7 @@ struct rqx_plan {
     int use_cache = 1;
     int device = 0;
     int is_real = 1;
+    int long_groups = 0;       // some group with x != 0 has TP_MIN_TERMS or more terms
     int max_seg = 0;
     int persistent = 0;        // launch the work-queue kernel
     int blocks_per_cu = 0;     // persistent grid: blocks per CU (0 = occupancy limit)
@@ -1504,7 +1524,7 @@ int check_plan(const rqx_plan *p) {
     return RQX_OK;
 }
 
-template <int MODE, typename R, bool CPLX, bool REALC = false>
+template <int MODE, typename R, bool CPLX, bool REALC = false, bool TP = true>
 hipError_t launch_tiles(const rqx_plan *p, const KParams &k, const void *x, void *y, hipStream_t st) {
     if (p->n_tiles == 0) return hipSuccess;
     const size_t lds = seg_lds_bytes(p);
@@ -1512,25 +1532,34 @@ hipError_t launch_tiles(const rqx_plan *p, const KParams &k, const void *x, void
         hipLaunchKernelGGL(k_fill<unsigned int>, dim3(1), dim3(64), 0, st, p->queue, int64_t(0), int64_t(1), 0u);
         const int bpc = p->blocks_per_cu > 0 ? p->blocks_per_cu : p->occupancy;
         const int grid = std::max(1, std::min(p->n_tiles, p->n_cu * bpc));
-        hipLaunchKernelGGL((k_tiles_queue<MODE, R, CPLX, REALC>), dim3(grid), dim3(TILE), lds, st, k, x, y);
+        hipLaunchKernelGGL((k_tiles_queue<MODE, R, CPLX, REALC, TP>), dim3(grid), dim3(TILE), lds, st, k, x, y);
     } else {
-        hipLaunchKernelGGL((k_tiles<MODE, R, CPLX, REALC>), dim3(p->n_tiles), dim3(TILE), lds, st, k, x, y);
+        hipLaunchKernelGGL((k_tiles<MODE, R, CPLX, REALC, TP>), dim3(p->n_tiles), dim3(TILE), lds, st, k, x, y);
     }
     return hipGetLastError();
 }
 
+// matvec kernels without the term-parallel evaluator where no off-diagonal
+// group is long enough to use it (other modes always carry it)
+template <int MODE, typename R, bool CPLX, bool REALC>
+hipError_t launch_tp(const rqx_plan *p, const KParams &k, const void *x, void *y, hipStream_t st) {
+    if (MODE == MODE_MATVEC && !p->long_groups) return launch_tiles<MODE, R, CPLX, REALC, MODE != MODE_MATVEC>(p, k, x, y, st);
+    return launch_tiles<MODE, R, CPLX, REALC>(p, k, x, y, st);
+}
+
 template <int MODE>
 hipError_t launch_dtype(const rqx_plan *p, const KParams &k, const void *x, void *y, int32_t dtype, hipStream_t st) {
+    constexpr bool MV = MODE == MODE_MATVEC;
     switch (dtype) {
         // real vectors imply real coefficients; complex vectors pick the kernel by the coefficients
-        case RQX_F64: return launch_tiles<MODE, double, false, MODE == MODE_MATVEC>(p, k, x, y, st);
-        case RQX_F32: return launch_tiles<MODE, float, false, MODE == MODE_MATVEC>(p, k, x, y, st);
+        case RQX_F64: return launch_tp<MODE, double, false, MV>(p, k, x, y, st);
+        case RQX_F32: return launch_tp<MODE, float, false, MV>(p, k, x, y, st);
         case RQX_C128:
-            if (MODE == MODE_MATVEC && p->is_real) return launch_tiles<MODE, double, true, MODE == MODE_MATVEC>(p, k, x, y, st);
-            return launch_tiles<MODE, double, true>(p, k, x, y, st);
+            if (MV && p->is_real) return launch_tp<MODE, double, true, MV>(p, k, x, y, st);
+            return launch_tp<MODE, double, true, false>(p, k, x, y, st);
         default:
-            if (MODE == MODE_MATVEC && p->is_real) return launch_tiles<MODE, float, true, MODE == MODE_MATVEC>(p, k, x, y, st);
-            return launch_tiles<MODE, float, true>(p, k, x, y, st);
+            if (MV && p->is_real) return launch_tp<MODE, float, true, MV>(p, k, x, y, st);
+            return launch_tp<MODE, float, true, false>(p, k, x, y, st);
     }
 }
 
@@ -1649,8 +1678,11 @@ int build_plan(rqx_plan **out, const std::vector<ProblemIn> &in, int64_t n_total
         ph.diag_terms = ph.has_diag ? static_cast<uint32_t>(q.grp_off[1]) : 0u;
         for (int64_t t = 0; t < q.n_terms; ++t)
             if (q.coef[2 * t + 1] != 0.0) ph.is_real = 0;
-        for (int32_t j = 0; j < q.n_groups; ++j)
-            ph.max_group_terms = std::max<int>(ph.max_group_terms, static_cast<int>(q.grp_off[j + 1] - q.grp_off[j]));
+        for (int32_t j = 0; j < q.n_groups; ++j) {
+            const int len = static_cast<int>(q.grp_off[j + 1] - q.grp_off[j]);
+            ph.max_group_terms = std::max(ph.max_group_terms, len);
+            if (q.xmask[j] != 0) ph.max_offdiag_terms = std::max(ph.max_offdiag_terms, len);
+        }
         Segmentation sg;
         if (int rc = choose_split(q, split_bits, ph.has_diag, sg, k)) return rc;
         ph.L = sg.L;
@@ -1696,6 +1728,7 @@ int build_plan(rqx_plan **out, const std::vector<ProblemIn> &in, int64_t n_total
                            static_cast<uint32_t>(ph.grp_base), lomask, ph.has_diag, 0};
         p->max_seg = std::max(p->max_seg, ph.max_seg);
         p->is_real = p->is_real && ph.is_real;
+        if (ph.max_offdiag_terms >= TP_MIN_TERMS) p->long_groups = 1;
         row_base += q.n_states;
         term_base += q.n_terms;
         grp_base += q.n_groups;
