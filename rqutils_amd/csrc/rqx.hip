@@ -581,7 +581,7 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
                         const int64_t k = sec_a ? static_cast<int64_t>(n_hit_a)
                                                 : static_cast<int64_t>(p.c_wa[i / SLICE]) + static_cast<int64_t>(n_hit);
                         const int64_t slot = p.c_off[i / SLICE] + k * SLICE + (i % SLICE);
-                        p.c_col[slot] = static_cast<int32_t>(ta + base);
+                        if (p.c_col) p.c_col[slot] = static_cast<int32_t>(ta + base);   // NULL: values only
                         if (p.c_val_lo) {
                             if constexpr (CPLX) {
                                 static_cast<CC *>(p.c_val)[slot] = CC{static_cast<R>(vh.x), static_cast<R>(vh.y)};
@@ -662,7 +662,7 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
             const uint32_t wa = p.c_wa[i / SLICE], wb = p.c_wb[i / SLICE];
             auto pad = [&](uint32_t k) {
                 const int64_t slot = o + static_cast<int64_t>(k) * SLICE;
-                p.c_col[slot] = static_cast<int32_t>(i);
+                if (p.c_col) p.c_col[slot] = static_cast<int32_t>(i);
                 if constexpr (CPLX) static_cast<CC *>(p.c_val)[slot] = CC{R(0), R(0)};
                 else static_cast<R *>(p.c_val)[slot] = R(0);
                 if (p.c_val_lo) {
@@ -1274,17 +1274,74 @@ __device__ inline void spmv_section(V &acc, const int32_t *c, const CV *vv, uint
     }
 }
 
+// a wave-uniform 64-bit value, held in scalar registers
+__device__ inline int64_t uniform64(int64_t v) {
+    const uint32_t lo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(v));
+    const uint32_t hi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(static_cast<uint64_t>(v) >> 32));
+    return static_cast<int64_t>((static_cast<uint64_t>(hi) << 32) | lo);
+}
+
+// section-A level whose columns are not row + (one offset for the whole slice)
+constexpr int32_t LEVEL_GATHER = INT32_MIN;
+
+// acc += the w section-A entries of a row, in order.  Level k of the slice is
+// either affine (del[k] = d: every lane's source row is its own row + d, so the
+// column is computed and the gather is contiguous) or stored (del[k] =
+// LEVEL_GATHER: columns read from c, 64 per such level).  Returns the number of
+// stored levels.
+template <int U, typename V, typename CV>
+__device__ inline uint32_t spmv_section_a(V &acc, const int32_t *c, const CV *vv, const int32_t *__restrict__ del,
+                                          uint32_t w, int32_t row, const V *__restrict__ x) {
+    uint32_t k = 0, j = 0;
+    for (; k + U <= w; k += U) {
+        int32_t d[U], ci[U];
+        CV cv[U];
+        V xv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            d[u] = del[k + u];
+            cv[u] = stream_load(vv + (k + u) * SLICE);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (d[u] == LEVEL_GATHER) {
+                ci[u] = stream_load(c + j * SLICE);
+                ++j;
+            } else {
+                ci[u] = row + d[u];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) xv[u] = x[ci[u]];
+#pragma unroll
+        for (int u = 0; u < U; ++u) fma_acc(acc, cv[u], xv[u]);
+    }
+    for (; k < w; ++k) {
+        const int32_t d = del[k];
+        int32_t ci = row + d;
+        if (d == LEVEL_GATHER) {
+            ci = stream_load(c + j * SLICE);
+            ++j;
+        }
+        fma_acc(acc, stream_load(vv + k * SLICE), x[ci]);
+    }
+    return j;
+}
+
 // y = H x from the hit cache: one wave per 64-row slice, one row per lane,
 // entries read lane-contiguously (and marked non-temporal: they are read once
 // per matvec, x is the data worth keeping in L2); the same accumulation
 // sequence as k_tiles (diagonal and section B, section A, their sum), so
-// results are identical.
+// results are identical.  Values of slice s start at off[s] (section A, then
+// B), its stored columns at coff[s] (stored section-A levels, then B), its
+// section-A level codes at aoff[s].
 template <typename R, bool CPLXV, bool CPLXC>
 __global__ __launch_bounds__(256) void k_spmv_sell(const int32_t *__restrict__ col, const void *__restrict__ val,
-                                                   const int64_t *__restrict__ off, const uint32_t *__restrict__ wa,
-                                                   const uint32_t *__restrict__ wb, const void *__restrict__ diag,
-                                                   int64_t n_states, const void *__restrict__ xin,
-                                                   void *__restrict__ yout) {
+                                                   const int64_t *__restrict__ off, const int64_t *__restrict__ coff,
+                                                   const int64_t *__restrict__ aoff, const int32_t *__restrict__ adel,
+                                                   const uint32_t *__restrict__ wa, const uint32_t *__restrict__ wb,
+                                                   const void *__restrict__ diag, int64_t n_states,
+                                                   const void *__restrict__ xin, void *__restrict__ yout) {
     using V = typename VT<R, CPLXV>::T;
     using CC = typename C2<R>::T;
     using CV = typename std::conditional<CPLXC, CC, R>::type;
@@ -1300,12 +1357,62 @@ __global__ __launch_bounds__(256) void k_spmv_sell(const int32_t *__restrict__ c
     if (diag) fma_acc(acc, static_cast<const CC *>(diag)[row], x[row]);   // NULL: no problem has an x = 0 group
     const uint32_t w_a = __builtin_amdgcn_readfirstlane(wa[slice]);
     const uint32_t w_b = __builtin_amdgcn_readfirstlane(wb[slice]);
-    const int64_t e0 = off[slice] + lane;
+    const int64_t o = uniform64(off[slice]), co = uniform64(coff[slice]), ao = uniform64(aoff[slice]);
     V acc_a = vzero(static_cast<V *>(nullptr));
-    spmv_section<U>(acc_a, col + e0, v + e0, w_a, x);
-    const int64_t e1 = e0 + static_cast<int64_t>(w_a) * SLICE;
-    spmv_section<U>(acc, col + e1, v + e1, w_b, x);
+    const uint32_t n_stored = spmv_section_a<U>(acc_a, col + co + lane, v + o + lane, adel + ao, w_a,
+                                                static_cast<int32_t>(row), x);
+    spmv_section<U>(acc, col + co + static_cast<int64_t>(n_stored) * SLICE + lane,
+                    v + o + static_cast<int64_t>(w_a) * SLICE + lane, w_b, x);
     static_cast<V *>(yout)[row] = vadd(acc, acc_a);
+}
+
+// Section-A level codes (one wave per slice) from the columns of the full
+// layout (entry k of row r at off[s] + 64 k + r % 64): del = d when every row r
+// of the slice has column r + d at that level (d != 0: padding has d = 0), else
+// LEVEL_GATHER; n_stored[s] = the slice's LEVEL_GATHER levels.
+__global__ __launch_bounds__(256) void k_cache_levels(const int32_t *__restrict__ col_full,
+                                                      const int64_t *__restrict__ off, const uint32_t *__restrict__ wa,
+                                                      const int64_t *__restrict__ aoff, int64_t n_states,
+                                                      int64_t n_slices, int32_t *__restrict__ adel,
+                                                      uint32_t *__restrict__ n_stored) {
+    const int64_t slice = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / SLICE;
+    if (slice >= n_slices) return;
+    const int lane = static_cast<int>(threadIdx.x % SLICE);
+    const int64_t row = slice * SLICE + lane;
+    const bool valid = row < n_states;
+    const uint32_t w = wa[slice];
+    const int64_t o = off[slice], ao = aoff[slice];
+    uint32_t stored = 0;
+    for (uint32_t k = 0; k < w; ++k) {
+        const int32_t d = valid ? col_full[o + static_cast<int64_t>(k) * SLICE + lane] - static_cast<int32_t>(row) : 0;
+        const int32_t d0 = __shfl(d, 0, 64);                    // lane 0 always holds a row
+        const bool affine = d0 != 0 && __ballot(valid && d != d0) == 0;
+        stored += affine ? 0u : 1u;
+        if (lane == 0) adel[ao + k] = affine ? d0 : LEVEL_GATHER;
+    }
+    if (lane == 0) n_stored[slice] = stored;
+}
+
+// Stored columns (one wave per slice): the LEVEL_GATHER section-A levels, then
+// every section-B level, copied from the full layout to coff[s] on.
+__global__ __launch_bounds__(256) void k_cache_columns(const int32_t *__restrict__ col_full,
+                                                       const int64_t *__restrict__ off, const uint32_t *__restrict__ wa,
+                                                       const uint32_t *__restrict__ wb, const int64_t *__restrict__ aoff,
+                                                       const int32_t *__restrict__ adel,
+                                                       const int64_t *__restrict__ coff, int64_t n_states,
+                                                       int64_t n_slices, int32_t *__restrict__ col) {
+    const int64_t slice = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / SLICE;
+    if (slice >= n_slices) return;
+    const int lane = static_cast<int>(threadIdx.x % SLICE);
+    const bool valid = slice * SLICE + lane < n_states;
+    const uint32_t w_a = wa[slice], w_b = wb[slice];
+    const int64_t o = off[slice], ao = aoff[slice];
+    int64_t dst = coff[slice] + lane;
+    for (uint32_t k = 0; k < w_a + w_b; ++k) {
+        if (k < w_a && adel[ao + k] != LEVEL_GATHER) continue;
+        col[dst] = valid ? col_full[o + static_cast<int64_t>(k) * SLICE + lane] : 0;
+        dst += SLICE;
+    }
 }
 
 __global__ void k_eigen_batch(const double *__restrict__ mats, int count, int n, double *__restrict__ out) {
@@ -1421,13 +1528,18 @@ struct ProbHost {            // host record of one problem of a plan
 };
 
 struct HitCache {            // sliced-ELL cache of the hits (column, C_j(s_i)) in one precision
-    void *block = nullptr;
-    int32_t *col = nullptr;
+    void *block = nullptr;     // values and per-slice arrays
+    void *col_block = nullptr; // stored columns
+    int32_t *col = nullptr;    // stored columns: gathered section-A levels and section B of each slice
     void *val = nullptr;
-    int64_t *off = nullptr;
+    int64_t *off = nullptr;    // first value of each slice
+    int64_t *coff = nullptr;   // first stored column of each slice
+    int64_t *aoff = nullptr;   // first section-A level code of each slice
+    int32_t *adel = nullptr;   // section-A level codes: row offset d, or LEVEL_GATHER
     uint32_t *wa = nullptr;    // section-A width of each slice
     uint32_t *wb = nullptr;    // section-B width of each slice
     int64_t entries = 0, nnz = 0, n_slices = 0, bytes = 0;
+    int64_t col_entries = 0, a_levels = 0;
     int cplx = 0;
     bool built = false;
     bool shared = false;       // both precisions built together: one block (owned by the FP64 cache), one column array
@@ -1864,6 +1976,10 @@ size_t cache_val_bytes(int slot, int cplx) { return (slot == 1 ? 8u : 4u) * (cpl
 hipError_t free_cache(HitCache &c) {
     hipError_t e = hipSuccess;
     if (c.block) e = hipFree(c.block);
+    if (c.col_block) {
+        const hipError_t e2 = hipFree(c.col_block);
+        if (e == hipSuccess) e = e2;
+    }
     c = HitCache{};
     return e;
 }
@@ -1876,13 +1992,14 @@ hipError_t release_cache(rqx_plan *p, int slot) {
     return e;
 }
 
-// (re)write columns and values of a built cache from the current coefficients
-// (both precisions in one pass when they were built together)
-hipError_t fill_cache(const rqx_plan *p, int slot, hipStream_t st) {
+// (re)write the values of a built cache from the current coefficients (both
+// precisions in one pass when they were built together); col_full: also write
+// every column into the full layout (values' indexing), at build time
+hipError_t fill_cache(const rqx_plan *p, int slot, hipStream_t st, int32_t *col_full = nullptr) {
     if (p->cache[slot].shared) slot = 1;
     const HitCache &c = p->cache[slot];
     KParams k = make_params(p, slot == 1);
-    k.c_col = c.col;
+    k.c_col = col_full;
     k.c_val = c.val;
     k.c_off = c.off;
     k.c_wa = c.wa;
@@ -1902,11 +2019,11 @@ template <typename R, bool CPLXV>
 hipError_t spmv_launch(const rqx_plan *p, const HitCache &c, const void *diag, const void *x, void *y, hipStream_t st) {
     const unsigned blocks = static_cast<unsigned>((p->n_states + 255) / 256);
     if (c.cplx)
-        hipLaunchKernelGGL((k_spmv_sell<R, CPLXV, true>), dim3(blocks), dim3(256), 0, st, c.col, c.val, c.off, c.wa,
-                           c.wb, diag, p->n_states, x, y);
+        hipLaunchKernelGGL((k_spmv_sell<R, CPLXV, true>), dim3(blocks), dim3(256), 0, st, c.col, c.val, c.off, c.coff,
+                           c.aoff, c.adel, c.wa, c.wb, diag, p->n_states, x, y);
     else
-        hipLaunchKernelGGL((k_spmv_sell<R, CPLXV, false>), dim3(blocks), dim3(256), 0, st, c.col, c.val, c.off, c.wa,
-                           c.wb, diag, p->n_states, x, y);
+        hipLaunchKernelGGL((k_spmv_sell<R, CPLXV, false>), dim3(blocks), dim3(256), 0, st, c.col, c.val, c.off, c.coff,
+                           c.aoff, c.adel, c.wa, c.wb, diag, p->n_states, x, y);
     return hipGetLastError();
 }
 
@@ -2209,8 +2326,8 @@ int rqx_plan_build_cache(rqx_plan *p, int32_t precision, int64_t max_bytes, rqx_
 
     // per-slice section widths and offsets
     std::vector<uint32_t> wa(static_cast<size_t>(n_slices), 0u), wb(static_cast<size_t>(n_slices), 0u);
-    std::vector<int64_t> off(static_cast<size_t>(n_slices), 0);
-    int64_t entries = 0, nnz = 0;
+    std::vector<int64_t> off(static_cast<size_t>(n_slices), 0), aoff(static_cast<size_t>(n_slices), 0);
+    int64_t entries = 0, nnz = 0, a_levels = 0;
     for (int64_t s = 0; s < n_slices; ++s) {
         uint32_t w_a = 0, w_b = 0;
         for (int64_t r = s * SLICE; r < std::min(n, (s + 1) * SLICE); ++r) {
@@ -2222,56 +2339,113 @@ int rqx_plan_build_cache(rqx_plan *p, int32_t precision, int64_t max_bytes, rqx_
         wa[static_cast<size_t>(s)] = w_a;
         wb[static_cast<size_t>(s)] = w_b;
         off[static_cast<size_t>(s)] = entries;
+        aoff[static_cast<size_t>(s)] = a_levels;
         entries += static_cast<int64_t>(w_a + w_b) * SLICE;
+        a_levels += w_a;
     }
     const int cplx = p->is_real ? 0 : 1;
     const size_t vb = cache_val_bytes(slot, cplx);
     const size_t vb_lo = both ? cache_val_bytes(0, cplx) : 0;
-    const size_t o_col = 0, o_val = align_up(static_cast<size_t>(entries) * 4),
-                 o_val_lo = o_val + align_up(static_cast<size_t>(entries) * vb),
+    const size_t o_val = 0, o_val_lo = align_up(static_cast<size_t>(entries) * vb),
                  o_off = o_val_lo + (both ? align_up(static_cast<size_t>(entries) * vb_lo) : 0),
-                 o_wa = o_off + align_up(static_cast<size_t>(n_slices) * 8),
+                 o_coff = o_off + align_up(static_cast<size_t>(n_slices) * 8),
+                 o_aoff = o_coff + align_up(static_cast<size_t>(n_slices) * 8),
+                 o_wa = o_aoff + align_up(static_cast<size_t>(n_slices) * 8),
                  o_wb = o_wa + align_up(static_cast<size_t>(n_slices) * 4),
-                 total = o_wb + align_up(static_cast<size_t>(n_slices) * 4);
+                 o_adel = o_wb + align_up(static_cast<size_t>(n_slices) * 4),
+                 total = o_adel + align_up(static_cast<size_t>(a_levels) * 4);
+    // the columns of the full layout are a build-time scratch array; the stored
+    // columns are at most as many
+    const size_t col_full_bytes = align_up(static_cast<size_t>(entries) * 4);
     size_t free_b = 0, total_b = 0;
     HIP_TRY(hipMemGetInfo(&free_b, &total_b));
     const int64_t limit = max_bytes > 0 ? max_bytes : static_cast<int64_t>(free_b / 10 * 9);
-    if (static_cast<int64_t>(total) > limit)
-        return fail(RQX_ERR_LIMIT, "hit cache needs " + std::to_string(total) + " bytes, limit " +
+    const int64_t need = static_cast<int64_t>(total + 2 * col_full_bytes);
+    if (need > limit)
+        return fail(RQX_ERR_LIMIT, "hit cache needs " + std::to_string(need) + " bytes (while building), limit " +
                                        std::to_string(limit));
 
     HitCache c;
     HIP_TRY(hipMalloc(&c.block, total));
     char *b = static_cast<char *>(c.block);
-    c.col = reinterpret_cast<int32_t *>(b + o_col);
     c.val = b + o_val;
     c.off = reinterpret_cast<int64_t *>(b + o_off);
+    c.coff = reinterpret_cast<int64_t *>(b + o_coff);
+    c.aoff = reinterpret_cast<int64_t *>(b + o_aoff);
     c.wa = reinterpret_cast<uint32_t *>(b + o_wa);
     c.wb = reinterpret_cast<uint32_t *>(b + o_wb);
+    c.adel = reinterpret_cast<int32_t *>(b + o_adel);
     c.entries = entries;
     c.nnz = nnz;
     c.n_slices = n_slices;
-    c.bytes = static_cast<int64_t>(total);
+    c.a_levels = a_levels;
     c.cplx = cplx;
-    c.built = true;
     c.shared = both;
-    e = hipMemcpyAsync(c.off, off.data(), off.size() * 8, hipMemcpyHostToDevice, st);
+    int32_t *col_full = nullptr;
+    uint32_t *n_stored = nullptr;
+    auto fail_build = [&](hipError_t err, const char *what) {
+        (void)hipFree(col_full);
+        (void)hipFree(n_stored);
+        (void)free_cache(c);
+        return fail(RQX_ERR_HIP, std::string(what) + hipGetErrorString(err));
+    };
+    e = hipMalloc(&col_full, col_full_bytes);
+    if (e == hipSuccess) e = hipMalloc(&n_stored, static_cast<size_t>(n_slices) * 4);
+    if (e == hipSuccess) e = hipMemcpyAsync(c.off, off.data(), off.size() * 8, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(c.aoff, aoff.data(), aoff.size() * 8, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipMemcpyAsync(c.wa, wa.data(), wa.size() * 4, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipMemcpyAsync(c.wb, wb.data(), wb.size() * 4, hipMemcpyHostToDevice, st);
+    if (e != hipSuccess) return fail_build(e, "cache allocation: ");
     p->cache[slot] = c;
     if (both) {                 // the FP32 cache: its own values, everything else shared
         HitCache lo = c;
         lo.block = nullptr;
         lo.val = b + o_val_lo;
-        lo.bytes = static_cast<int64_t>(entries) * static_cast<int64_t>(vb_lo);
         p->cache[0] = lo;
     }
-    // pass 2: columns and values in the order the tile kernel accumulates them
-    if (e == hipSuccess) e = fill_cache(p, slot, st);
-    if (e == hipSuccess) e = hipStreamSynchronize(st);   // host vectors above are released on return
-    if (e != hipSuccess) {
-        (void)release_cache(p, slot);
-        return fail(RQX_ERR_HIP, std::string("cache fill: ") + hipGetErrorString(e));
+    // pass 2: values (and every column, full layout) in the order the tile kernel accumulates them
+    e = fill_cache(p, slot, st, col_full);
+    p->cache[slot] = HitCache{};
+    if (both) p->cache[0] = HitCache{};
+    if (e != hipSuccess) return fail_build(e, "cache fill: ");
+    // section-A level codes, then the stored columns
+    const unsigned wave_blocks = static_cast<unsigned>((n_slices * SLICE + 255) / 256);
+    hipLaunchKernelGGL(k_cache_levels, dim3(wave_blocks), dim3(256), 0, st, col_full, c.off, c.wa, c.aoff, n, n_slices,
+                       c.adel, n_stored);
+    e = hipGetLastError();
+    std::vector<uint32_t> stored(static_cast<size_t>(n_slices));
+    if (e == hipSuccess) e = hipMemcpyAsync(stored.data(), n_stored, stored.size() * 4, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return fail_build(e, "cache levels: ");
+    std::vector<int64_t> coff(static_cast<size_t>(n_slices), 0);
+    int64_t col_entries = 0;
+    for (int64_t s = 0; s < n_slices; ++s) {
+        coff[static_cast<size_t>(s)] = col_entries;
+        col_entries += static_cast<int64_t>(stored[static_cast<size_t>(s)] + wb[static_cast<size_t>(s)]) * SLICE;
+    }
+    e = hipMalloc(&c.col_block, align_up(static_cast<size_t>(col_entries) * 4));
+    c.col = static_cast<int32_t *>(c.col_block);
+    if (e == hipSuccess) e = hipMemcpyAsync(c.coff, coff.data(), coff.size() * 8, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_cache_columns, dim3(wave_blocks), dim3(256), 0, st, col_full, c.off, c.wa, c.wb, c.aoff,
+                           c.adel, c.coff, n, n_slices, c.col);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(st);   // host vectors and the scratch arrays are released below
+    if (e != hipSuccess) return fail_build(e, "cache columns: ");
+    (void)hipFree(col_full);
+    (void)hipFree(n_stored);
+    c.col_entries = col_entries;
+    c.bytes = static_cast<int64_t>(total + align_up(static_cast<size_t>(col_entries) * 4));
+    c.built = true;
+    p->cache[slot] = c;
+    if (both) {
+        HitCache lo = c;
+        lo.block = nullptr;
+        lo.col_block = nullptr;
+        lo.val = b + o_val_lo;
+        lo.bytes = static_cast<int64_t>(entries) * static_cast<int64_t>(vb_lo);
+        p->cache[0] = lo;
     }
     return RQX_OK;
 }
@@ -2298,6 +2472,8 @@ int rqx_plan_cache_info(const rqx_plan *p, int32_t precision, int64_t *out) {
     out[3] = c.bytes;
     out[4] = c.n_slices;
     out[5] = c.cplx;
+    out[6] = c.col_entries;
+    out[7] = c.a_levels;
     return RQX_OK;
 }
 

@@ -277,10 +277,11 @@ class Plan:
         _capi.check(_capi.lib().rqx_plan_drop_cache(self._handle, int(precision)))
 
     def cache_info(self, precision=64):
-        """``built``, stored hits, entries incl. padding, device bytes, slices, complex values."""
-        out = np.zeros(6, dtype=np.int64)
+        """``built``, stored hits, value entries incl. padding, device bytes, slices,
+        complex values, stored columns, first-section levels."""
+        out = np.zeros(8, dtype=np.int64)
         _capi.check(_capi.lib().rqx_plan_cache_info(self._handle, int(precision), _capi.ptr(out)))
-        keys = ('built', 'nnz', 'entries', 'bytes', 'slices', 'complex')
+        keys = ('built', 'nnz', 'entries', 'bytes', 'slices', 'complex', 'col_entries', 'a_levels')
         info = {k: int(v) for k, v in zip(keys, out)}
         info['built'] = bool(info['built'])
         info['complex'] = bool(info['complex'])
@@ -293,8 +294,10 @@ class Plan:
     def cached_bytes(self, dtype):
         """Bytes one cached matvec must move: the cache, the diagonal, ``x`` once and ``y``.
 
-        ``entries (4 + s_val) + 16 slices + N (s_d + 2 s_v)`` with ``s_val`` the
-        stored value bytes, ``s_d`` the diagonal entry bytes (complex) and
+        ``entries s_val + 4 col_entries + 4 a_levels + 40 slices + N (s_d + 2 s_v)``
+        with ``s_val`` the stored value bytes, ``col_entries`` the stored
+        columns, ``a_levels`` the first-section level codes, 40 bytes of offsets
+        and widths per slice, ``s_d`` the diagonal entry bytes (complex) and
         ``s_v`` the vector element bytes; gathers of ``x`` past its first read
         are counted as cache hits.
         """
@@ -305,7 +308,8 @@ class Plan:
         s_v = torch.empty(0, dtype=dtype).element_size()
         s_val = (8 if precision == 64 else 4) * (2 if info['complex'] else 1)
         s_d = 16 if precision == 64 else 8
-        return info['entries'] * (4 + s_val) + 16 * info['slices'] + self.n_states * (s_d + 2 * s_v)
+        return (info['entries'] * s_val + 4 * info['col_entries'] + 4 * info['a_levels'] + 40 * info['slices'] +
+                self.n_states * (s_d + 2 * s_v))
 
     # ------------------------------------------------------------------ per-problem vector ops
     def dots(self, pairs):

@@ -77,6 +77,7 @@ def test_cached_matvec_identical(cases, name):
         assert info['built'] and info['complex'] == (not plan.is_real)
         assert info['nnz'] == hits - (plan.n_states if plan.info['has_diagonal'] else 0)
         assert info['entries'] >= info['nnz'] and info['entries'] % 64 == 0
+        assert info['col_entries'] % 64 == 0 and info['col_entries'] <= info['entries']
     _check_identical(plan)
     # the cached values are the oracle's matrix elements
     x = _vec(plan.n_total, torch.complex128, 3)
@@ -90,6 +91,18 @@ def test_cached_matvec_identical(cases, name):
     assert not plan.cache_info(64)['built'] and plan.cache_info(32)['built']
     plan.drop_cache()
     assert not plan.cache_info(32)['built']
+
+
+def test_product_subspace_levels_store_offsets():
+    """Product subspaces: most first-section levels are a row offset (no stored columns), same matvec."""
+    _say('cache product-subspace levels')
+    w = workloads.jw32(n_alpha=300, n_beta=200)
+    plan = Plan(w.tables, w.states)
+    info = plan.build_cache(32)
+    stored_a = info['col_entries'] // 64 - (info['entries'] // 64 - info['a_levels'])
+    print(f"first-section levels {info['a_levels']}, stored {stored_a}", flush=True)
+    assert info['a_levels'] > 0 and stored_a < info['a_levels']
+    _check_identical(plan, seed=19)
 
 
 def test_cached_batch_matvec_identical(cases):
