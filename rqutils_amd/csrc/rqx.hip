@@ -59,11 +59,30 @@ enum { MODE_MATVEC = 0, MODE_LOOKUP = 1, MODE_COUNT = 2, MODE_GRAD = 3, MODE_ROW
 // Section A holds the hits of groups with no flips in the low split bits (the
 // source has the row's own low key in another segment, so the rows of a slice
 // usually all hit the same such groups and their gathers coalesce); section B
-// the other hits.  Entry k of row r: off[s] + 64 k + r % 64 (A, k < wa[s]) and
-// off[s] + 64 (wa[s] + k) + r % 64 (B, k < wb[s]), s = r / 64; rows are padded to
+// the other hits.  Entries are stored in packets of PACK consecutive entries of
+// one row, read with 16-byte loads: entry k of row r (k counted over section
+// A, then B; both widths multiples of PACK) sits at
+//   off[s] + (k / PACK) 64 PACK + (q / e) 64 e + (r % 64) e + q % e,
+// q = k % PACK, s = r / 64, e = the entries of the array's type in 16 bytes (4
+// for columns and FP32 values, 2 for FP64 or complex FP32, 1 for complex FP64),
+// so each 16-byte load of a wave reads 1 KB contiguously.  Rows are padded to
 // the slice's widths with (r, 0).  A row's y is (diagonal + B hits) + A hits,
 // each part accumulated in group order, in the matrix-free kernel too.
 constexpr int SLICE = 64;
+constexpr int PACK = 4;
+
+template <typename T> constexpr int pack_sub() { return (16 / sizeof(T)) < PACK ? static_cast<int>(16 / sizeof(T)) : PACK; }
+
+template <typename T> __host__ __device__ inline int64_t packed_slot(int64_t off, uint32_t k, int lane) {
+    constexpr int E = pack_sub<T>();
+    const uint32_t q = k % PACK;
+    return off + static_cast<int64_t>(k / PACK) * (SLICE * PACK) + static_cast<int64_t>(q / E) * (SLICE * E) +
+           lane * E + (q % E);
+}
+
+template <typename T> __device__ inline void put_packed(void *a, int64_t off, uint32_t k, int lane, T v) {
+    static_cast<T *>(a)[packed_slot<T>(off, k, lane)] = v;
+}
 
 // terms fetched per scalar-memory round trip when a coefficient sum is evaluated;
 // the z and coefficient arrays are padded by as many entries
@@ -578,22 +597,23 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
                     if (p.c_val_lo) group_value_wave_dual<!CPLX>(p, g, s, hit, vh, vl);   // both precisions, R = double
                     else group_value_wave<R, !CPLX>(p.coef, p, g, s, hit, vx, vy);
                     if (hit) {
-                        const int64_t k = sec_a ? static_cast<int64_t>(n_hit_a)
-                                                : static_cast<int64_t>(p.c_wa[i / SLICE]) + static_cast<int64_t>(n_hit);
-                        const int64_t slot = p.c_off[i / SLICE] + k * SLICE + (i % SLICE);
-                        if (p.c_col) p.c_col[slot] = static_cast<int32_t>(ta + base);   // NULL: values only
+                        const uint32_t k = sec_a ? static_cast<uint32_t>(n_hit_a)
+                                                 : p.c_wa[i / SLICE] + static_cast<uint32_t>(n_hit);
+                        const int64_t o = p.c_off[i / SLICE];
+                        const int ln = static_cast<int>(i % SLICE);
+                        if (p.c_col) put_packed<int32_t>(p.c_col, o, k, ln, static_cast<int32_t>(ta + base));   // NULL: values only
                         if (p.c_val_lo) {
                             if constexpr (CPLX) {
-                                static_cast<CC *>(p.c_val)[slot] = CC{static_cast<R>(vh.x), static_cast<R>(vh.y)};
-                                static_cast<float2 *>(p.c_val_lo)[slot] = vl;
+                                put_packed<CC>(p.c_val, o, k, ln, CC{static_cast<R>(vh.x), static_cast<R>(vh.y)});
+                                put_packed<float2>(p.c_val_lo, o, k, ln, vl);
                             } else {
-                                static_cast<R *>(p.c_val)[slot] = static_cast<R>(vh.x);
-                                static_cast<float *>(p.c_val_lo)[slot] = vl.x;
+                                put_packed<R>(p.c_val, o, k, ln, static_cast<R>(vh.x));
+                                put_packed<float>(p.c_val_lo, o, k, ln, vl.x);
                             }
                         } else if constexpr (CPLX) {
-                            static_cast<CC *>(p.c_val)[slot] = CC{vx, vy};
+                            put_packed<CC>(p.c_val, o, k, ln, CC{vx, vy});
                         } else {
-                            static_cast<R *>(p.c_val)[slot] = vx;
+                            put_packed<R>(p.c_val, o, k, ln, vx);
                         }
                         if (sec_a) n_hit_a += 1;
                         else n_hit += 1;
@@ -658,16 +678,16 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
         }
     } else if constexpr (MODE == MODE_FILL) {
         if (active) {
-            const int64_t o = p.c_off[i / SLICE] + (i % SLICE);
+            const int64_t o = p.c_off[i / SLICE];
+            const int ln = static_cast<int>(i % SLICE);
             const uint32_t wa = p.c_wa[i / SLICE], wb = p.c_wb[i / SLICE];
             auto pad = [&](uint32_t k) {
-                const int64_t slot = o + static_cast<int64_t>(k) * SLICE;
-                if (p.c_col) p.c_col[slot] = static_cast<int32_t>(i);
-                if constexpr (CPLX) static_cast<CC *>(p.c_val)[slot] = CC{R(0), R(0)};
-                else static_cast<R *>(p.c_val)[slot] = R(0);
+                if (p.c_col) put_packed<int32_t>(p.c_col, o, k, ln, static_cast<int32_t>(i));
+                if constexpr (CPLX) put_packed<CC>(p.c_val, o, k, ln, CC{R(0), R(0)});
+                else put_packed<R>(p.c_val, o, k, ln, R(0));
                 if (p.c_val_lo) {
-                    if constexpr (CPLX) static_cast<float2 *>(p.c_val_lo)[slot] = make_float2(0.f, 0.f);
-                    else static_cast<float *>(p.c_val_lo)[slot] = 0.f;
+                    if constexpr (CPLX) put_packed<float2>(p.c_val_lo, o, k, ln, make_float2(0.f, 0.f));
+                    else put_packed<float>(p.c_val_lo, o, k, ln, 0.f);
                 }
             };
             for (uint32_t k = static_cast<uint32_t>(n_hit_a); k < wa; ++k) pad(k);
@@ -1246,31 +1266,55 @@ __device__ inline double2 stream_load(const double2 *p) {
     return make_double2(__builtin_nontemporal_load(&p->x), __builtin_nontemporal_load(&p->y));
 }
 
-// acc += the w entries of one section of a row, in order: column and value
-// loads of U entries, then their gathers of x, then the FMAs
+typedef int32_t i32x4 __attribute__((ext_vector_type(4)));
+
+// one packet (PACK consecutive entries of a row) with 16-byte non-temporal
+// loads; p: the row's first entry of the packet (its 16-byte pieces are 64 e
+// entries apart)
+template <typename T>
+__device__ inline void load_packet(const T *p, T *out) {
+    constexpr int E = pack_sub<T>();
+    static_assert(sizeof(T) * E == 16, "16-byte pieces");
+    constexpr int N = PACK / E;
+    i32x4 t[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) t[i] = __builtin_nontemporal_load(reinterpret_cast<const i32x4 *>(p + i * SLICE * E));
+    __builtin_memcpy(out, t, sizeof(t));
+}
+
+// acc += the w entries (a multiple of PACK) of one section of a row, in order:
+// column and value packets of U entries, then their gathers of x, then the
+// FMAs.  c, vv: the row's first packet.
 template <int U, typename V, typename CV>
 __device__ inline void spmv_section(V &acc, const int32_t *c, const CV *vv, uint32_t w, const V *__restrict__ x) {
+    static_assert(U % PACK == 0, "whole packets per batch");
+    constexpr int64_t STEP = SLICE * PACK;        // entries between a row's packets
     uint32_t k = 0;
     for (; k + U <= w; k += U) {
         int32_t ci[U];
         CV cv[U];
         V xv[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            ci[u] = stream_load(c + u * SLICE);
-            cv[u] = stream_load(vv + u * SLICE);
+        for (int pk = 0; pk < U / PACK; ++pk) {
+            load_packet(c + pk * STEP, ci + pk * PACK);
+            load_packet(vv + pk * STEP, cv + pk * PACK);
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) xv[u] = x[ci[u]];
 #pragma unroll
         for (int u = 0; u < U; ++u) fma_acc(acc, cv[u], xv[u]);
-        c += U * SLICE;
-        vv += U * SLICE;
+        c += (U / PACK) * STEP;
+        vv += (U / PACK) * STEP;
     }
-    for (; k < w; ++k) {
-        fma_acc(acc, stream_load(vv), x[stream_load(c)]);
-        c += SLICE;
-        vv += SLICE;
+    for (; k < w; k += PACK) {
+        int32_t ci[PACK];
+        CV cv[PACK];
+        load_packet(c, ci);
+        load_packet(vv, cv);
+#pragma unroll
+        for (int u = 0; u < PACK; ++u) fma_acc(acc, cv[u], x[ci[u]]);
+        c += STEP;
+        vv += STEP;
     }
 }
 
@@ -1284,24 +1328,26 @@ __device__ inline int64_t uniform64(int64_t v) {
 // section-A level whose columns are not row + (one offset for the whole slice)
 constexpr int32_t LEVEL_GATHER = INT32_MIN;
 
-// acc += the w section-A entries of a row, in order.  Level k of the slice is
-// either affine (del[k] = d: every lane's source row is its own row + d, so the
-// column is computed and the gather is contiguous) or stored (del[k] =
-// LEVEL_GATHER: columns read from c, 64 per such level).  Returns the number of
-// stored levels.
+// acc += the w (a multiple of PACK) section-A entries of a row, in order.
+// Level k of the slice is either affine (del[k] = d: every lane's source row is
+// its own row + d, so the column is computed and the gather is contiguous) or
+// stored (del[k] = LEVEL_GATHER: columns read from c, 64 lane-contiguous per
+// such level).  Values come in packets (vv: the row's first).  Returns the
+// number of stored levels.
 template <int U, typename V, typename CV>
 __device__ inline uint32_t spmv_section_a(V &acc, const int32_t *c, const CV *vv, const int32_t *__restrict__ del,
                                           uint32_t w, int32_t row, const V *__restrict__ x) {
+    static_assert(U % PACK == 0, "whole packets per batch");
+    constexpr int64_t STEP = SLICE * PACK;
     uint32_t k = 0, j = 0;
     for (; k + U <= w; k += U) {
         int32_t d[U], ci[U];
         CV cv[U];
         V xv[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            d[u] = del[k + u];
-            cv[u] = stream_load(vv + (k + u) * SLICE);
-        }
+        for (int u = 0; u < U; ++u) d[u] = del[k + u];
+#pragma unroll
+        for (int pk = 0; pk < U / PACK; ++pk) load_packet(vv + pk * STEP, cv + pk * PACK);
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             if (d[u] == LEVEL_GATHER) {
@@ -1315,15 +1361,22 @@ __device__ inline uint32_t spmv_section_a(V &acc, const int32_t *c, const CV *vv
         for (int u = 0; u < U; ++u) xv[u] = x[ci[u]];
 #pragma unroll
         for (int u = 0; u < U; ++u) fma_acc(acc, cv[u], xv[u]);
+        vv += (U / PACK) * STEP;
     }
-    for (; k < w; ++k) {
-        const int32_t d = del[k];
-        int32_t ci = row + d;
-        if (d == LEVEL_GATHER) {
-            ci = stream_load(c + j * SLICE);
-            ++j;
+    for (; k < w; k += PACK) {
+        CV cv[PACK];
+        load_packet(vv, cv);
+#pragma unroll
+        for (int u = 0; u < PACK; ++u) {
+            const int32_t d = del[k + u];
+            int32_t ci = row + d;
+            if (d == LEVEL_GATHER) {
+                ci = stream_load(c + j * SLICE);
+                ++j;
+            }
+            fma_acc(acc, cv[u], x[ci]);
         }
-        fma_acc(acc, stream_load(vv + k * SLICE), x[ci]);
+        vv += STEP;
     }
     return j;
 }
@@ -1359,16 +1412,17 @@ __global__ __launch_bounds__(256) void k_spmv_sell(const int32_t *__restrict__ c
     const uint32_t w_b = __builtin_amdgcn_readfirstlane(wb[slice]);
     const int64_t o = uniform64(off[slice]), co = uniform64(coff[slice]), ao = uniform64(aoff[slice]);
     V acc_a = vzero(static_cast<V *>(nullptr));
-    const uint32_t n_stored = spmv_section_a<U>(acc_a, col + co + lane, v + o + lane, adel + ao, w_a,
+    constexpr int EV = pack_sub<CV>();
+    const uint32_t n_stored = spmv_section_a<U>(acc_a, col + co + lane, v + o + lane * EV, adel + ao, w_a,
                                                 static_cast<int32_t>(row), x);
-    spmv_section<U>(acc, col + co + static_cast<int64_t>(n_stored) * SLICE + lane,
-                    v + o + static_cast<int64_t>(w_a) * SLICE + lane, w_b, x);
+    spmv_section<U>(acc, col + co + static_cast<int64_t>(n_stored) * SLICE + lane * PACK,
+                    v + o + static_cast<int64_t>(w_a) * SLICE + lane * EV, w_b, x);
     static_cast<V *>(yout)[row] = vadd(acc, acc_a);
 }
 
 // Section-A level codes (one wave per slice) from the columns of the full
-// layout (entry k of row r at off[s] + 64 k + r % 64): del = d when every row r
-// of the slice has column r + d at that level (d != 0: padding has d = 0), else
+// packed layout: del = d when every row r of the slice has column r + d at that
+// level (d = 0: a level of padding only, whose values are 0), else
 // LEVEL_GATHER; n_stored[s] = the slice's LEVEL_GATHER levels.
 __global__ __launch_bounds__(256) void k_cache_levels(const int32_t *__restrict__ col_full,
                                                       const int64_t *__restrict__ off, const uint32_t *__restrict__ wa,
@@ -1384,17 +1438,18 @@ __global__ __launch_bounds__(256) void k_cache_levels(const int32_t *__restrict_
     const int64_t o = off[slice], ao = aoff[slice];
     uint32_t stored = 0;
     for (uint32_t k = 0; k < w; ++k) {
-        const int32_t d = valid ? col_full[o + static_cast<int64_t>(k) * SLICE + lane] - static_cast<int32_t>(row) : 0;
+        const int32_t d = valid ? col_full[packed_slot<int32_t>(o, k, lane)] - static_cast<int32_t>(row) : 0;
         const int32_t d0 = __shfl(d, 0, 64);                    // lane 0 always holds a row
-        const bool affine = d0 != 0 && __ballot(valid && d != d0) == 0;
+        const bool affine = __ballot(valid && d != d0) == 0;
         stored += affine ? 0u : 1u;
         if (lane == 0) adel[ao + k] = affine ? d0 : LEVEL_GATHER;
     }
     if (lane == 0) n_stored[slice] = stored;
 }
 
-// Stored columns (one wave per slice): the LEVEL_GATHER section-A levels, then
-// every section-B level, copied from the full layout to coff[s] on.
+// Stored columns (one wave per slice) from coff[s] on: the LEVEL_GATHER
+// section-A levels (64 lane-contiguous columns each), then section B as packed
+// in the full layout.
 __global__ __launch_bounds__(256) void k_cache_columns(const int32_t *__restrict__ col_full,
                                                        const int64_t *__restrict__ off, const uint32_t *__restrict__ wa,
                                                        const uint32_t *__restrict__ wb, const int64_t *__restrict__ aoff,
@@ -1407,11 +1462,16 @@ __global__ __launch_bounds__(256) void k_cache_columns(const int32_t *__restrict
     const bool valid = slice * SLICE + lane < n_states;
     const uint32_t w_a = wa[slice], w_b = wb[slice];
     const int64_t o = off[slice], ao = aoff[slice];
-    int64_t dst = coff[slice] + lane;
-    for (uint32_t k = 0; k < w_a + w_b; ++k) {
-        if (k < w_a && adel[ao + k] != LEVEL_GATHER) continue;
-        col[dst] = valid ? col_full[o + static_cast<int64_t>(k) * SLICE + lane] : 0;
+    int64_t dst = coff[slice];
+    for (uint32_t k = 0; k < w_a; ++k) {
+        if (adel[ao + k] != LEVEL_GATHER) continue;
+        col[dst + lane] = valid ? col_full[packed_slot<int32_t>(o, k, lane)] : 0;
         dst += SLICE;
+    }
+    const int64_t src = o + static_cast<int64_t>(w_a) * SLICE;
+    for (int64_t e = lane; e < static_cast<int64_t>(w_b) * SLICE; e += SLICE) {
+        const int owner = static_cast<int>((e % (SLICE * PACK)) / PACK);   // the lane whose packet holds e
+        col[dst + e] = (slice * SLICE + owner < n_states) ? col_full[src + e] : 0;
     }
 }
 
@@ -2336,6 +2396,8 @@ int rqx_plan_build_cache(rqx_plan *p, int32_t precision, int64_t max_bytes, rqx_
             w_a = std::max(w_a, ca);
             nnz += static_cast<int64_t>(ca) + cb;
         }
+        w_a = (w_a + PACK - 1) / PACK * PACK;      // whole packets
+        w_b = (w_b + PACK - 1) / PACK * PACK;
         wa[static_cast<size_t>(s)] = w_a;
         wb[static_cast<size_t>(s)] = w_b;
         off[static_cast<size_t>(s)] = entries;
