@@ -80,9 +80,27 @@ template <typename T> __host__ __device__ inline int64_t packed_slot(int64_t off
            lane * E + (q % E);
 }
 
-template <typename T> __device__ inline void put_packed(void *a, int64_t off, uint32_t k, int lane, T v) {
-    static_cast<T *>(a)[packed_slot<T>(off, k, lane)] = v;
+typedef int32_t i32x4 __attribute__((ext_vector_type(4)));
+
+// the packet of entries kb .. kb + PACK - 1 (kb a multiple of PACK) of a row, as
+// 16-byte stores
+template <typename T> __device__ inline void store_packet(void *a, int64_t off, uint32_t kb, int lane, const T *v) {
+    constexpr int E = pack_sub<T>();
+#pragma unroll
+    for (int i = 0; i < PACK / E; ++i) {
+        i32x4 t;
+        __builtin_memcpy(&t, v + i * E, 16);
+        *reinterpret_cast<i32x4 *>(static_cast<T *>(a) + packed_slot<T>(off, kb + i * E, lane)) = t;
+    }
 }
+
+// fill pass: the last PACK entries of one section of a row (column, value and,
+// when both precisions are filled, FP32 value), stored once the packet is whole
+template <typename FV, typename FL> struct FillBuf {
+    int32_t c[PACK];
+    FV v[PACK];
+    FL l[PACK];
+};
 
 // terms fetched per scalar-memory round trip when a coefficient sum is evaluated;
 // the z and coefficient arrays are padded by as many entries
@@ -510,6 +528,30 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
     V acc = vzero(static_cast<V *>(nullptr));       // diagonal + section-B groups
     V acc_a = vzero(static_cast<V *>(nullptr));     // section-A groups
     unsigned long long n_hit = 0, n_probe = 0, n_look = 0, n_hit_a = 0;
+    using FV = typename std::conditional<CPLX, CC, R>::type;          // fill: stored value
+    using FL = typename std::conditional<CPLX, float2, float>::type;  // fill: FP32 value of a dual fill
+    FillBuf<FV, FL> fill_a, fill_b;
+    // append entry k of a section of this row; a whole packet goes to memory
+    auto fill_push = [&](FillBuf<FV, FL> &b, uint32_t k, int32_t c, FV v, FL l) {
+#pragma unroll
+        for (int u = 0; u + 1 < PACK; ++u) {
+            b.c[u] = b.c[u + 1];
+            b.v[u] = b.v[u + 1];
+            b.l[u] = b.l[u + 1];
+        }
+        b.c[PACK - 1] = c;
+        b.v[PACK - 1] = v;
+        b.l[PACK - 1] = l;
+        if (k % PACK == PACK - 1) {
+            const int32_t q = static_cast<int32_t>(i);
+            const int64_t o = p.c_off[q / SLICE];
+            const int ln = q % SLICE;
+            const uint32_t kb = k - (PACK - 1);
+            if (p.c_col) store_packet<int32_t>(p.c_col, o, kb, ln, b.c);   // NULL: values only
+            store_packet<FV>(p.c_val, o, kb, ln, b.v);
+            if (p.c_val_lo) store_packet<FL>(p.c_val_lo, o, kb, ln, b.l);
+        }
+    };
     V gv = vzero(static_cast<V *>(nullptr));
     if constexpr (MODE == MODE_GRAD) {
         if (active) gv = static_cast<const V *>(p.gy)[i];
@@ -597,26 +639,23 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
                     if (p.c_val_lo) group_value_wave_dual<!CPLX>(p, g, s, hit, vh, vl);   // both precisions, R = double
                     else group_value_wave<R, !CPLX>(p.coef, p, g, s, hit, vx, vy);
                     if (hit) {
-                        const uint32_t k = sec_a ? static_cast<uint32_t>(n_hit_a)
-                                                 : p.c_wa[i / SLICE] + static_cast<uint32_t>(n_hit);
-                        const int64_t o = p.c_off[i / SLICE];
-                        const int ln = static_cast<int>(i % SLICE);
-                        if (p.c_col) put_packed<int32_t>(p.c_col, o, k, ln, static_cast<int32_t>(ta + base));   // NULL: values only
-                        if (p.c_val_lo) {
-                            if constexpr (CPLX) {
-                                put_packed<CC>(p.c_val, o, k, ln, CC{static_cast<R>(vh.x), static_cast<R>(vh.y)});
-                                put_packed<float2>(p.c_val_lo, o, k, ln, vl);
-                            } else {
-                                put_packed<R>(p.c_val, o, k, ln, static_cast<R>(vh.x));
-                                put_packed<float>(p.c_val_lo, o, k, ln, vl.x);
-                            }
-                        } else if constexpr (CPLX) {
-                            put_packed<CC>(p.c_val, o, k, ln, CC{vx, vy});
+                        FV v;
+                        FL l;
+                        if constexpr (CPLX) {
+                            v = p.c_val_lo ? CC{static_cast<R>(vh.x), static_cast<R>(vh.y)} : CC{vx, vy};
+                            l = vl;
                         } else {
-                            put_packed<R>(p.c_val, o, k, ln, vx);
+                            v = p.c_val_lo ? static_cast<R>(vh.x) : vx;
+                            l = vl.x;
                         }
-                        if (sec_a) n_hit_a += 1;
-                        else n_hit += 1;
+                        const int32_t c = static_cast<int32_t>(ta + base);
+                        if (sec_a) {
+                            fill_push(fill_a, static_cast<uint32_t>(n_hit_a), c, v, l);
+                            n_hit_a += 1;
+                        } else {
+                            fill_push(fill_b, p.c_wa[i / SLICE] + static_cast<uint32_t>(n_hit), c, v, l);
+                            n_hit += 1;
+                        }
                     }
                 } else {
                     n_hit += hit;
@@ -678,20 +717,14 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
         }
     } else if constexpr (MODE == MODE_FILL) {
         if (active) {
-            const int64_t o = p.c_off[i / SLICE];
-            const int ln = static_cast<int>(i % SLICE);
             const uint32_t wa = p.c_wa[i / SLICE], wb = p.c_wb[i / SLICE];
-            auto pad = [&](uint32_t k) {
-                if (p.c_col) put_packed<int32_t>(p.c_col, o, k, ln, static_cast<int32_t>(i));
-                if constexpr (CPLX) put_packed<CC>(p.c_val, o, k, ln, CC{R(0), R(0)});
-                else put_packed<R>(p.c_val, o, k, ln, R(0));
-                if (p.c_val_lo) {
-                    if constexpr (CPLX) put_packed<float2>(p.c_val_lo, o, k, ln, make_float2(0.f, 0.f));
-                    else put_packed<float>(p.c_val_lo, o, k, ln, 0.f);
-                }
-            };
-            for (uint32_t k = static_cast<uint32_t>(n_hit_a); k < wa; ++k) pad(k);
-            for (uint32_t k = wa + static_cast<uint32_t>(n_hit); k < wa + wb; ++k) pad(k);
+            // pad to the slice's widths (multiples of PACK): the last packets go out
+            const FV v0 = vzero(static_cast<FV *>(nullptr));
+            const FL l0 = vzero(static_cast<FL *>(nullptr));
+            for (uint32_t k = static_cast<uint32_t>(n_hit_a); k < wa; ++k)
+                fill_push(fill_a, k, static_cast<int32_t>(i), v0, l0);
+            for (uint32_t k = wa + static_cast<uint32_t>(n_hit); k < wa + wb; ++k)
+                fill_push(fill_b, k, static_cast<int32_t>(i), v0, l0);
         }
     } else if constexpr (MODE == MODE_COUNT) {
         // wave reduction, one atomic per wave and counter (the class lookups are
@@ -1265,8 +1298,6 @@ __device__ inline float2 stream_load(const float2 *p) {
 __device__ inline double2 stream_load(const double2 *p) {
     return make_double2(__builtin_nontemporal_load(&p->x), __builtin_nontemporal_load(&p->y));
 }
-
-typedef int32_t i32x4 __attribute__((ext_vector_type(4)));
 
 // one packet (PACK consecutive entries of a row) with 16-byte non-temporal
 // loads; p: the row's first entry of the packet (its 16-byte pieces are 64 e
