@@ -531,6 +531,14 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
     using FV = typename std::conditional<CPLX, CC, R>::type;          // fill: stored value
     using FL = typename std::conditional<CPLX, float2, float>::type;  // fill: FP32 value of a dual fill
     FillBuf<FV, FL> fill_a, fill_b;
+    int64_t fill_off = 0;         // the row's slice: first entry and section-A width
+    uint32_t fill_wa = 0;
+    if constexpr (MODE == MODE_FILL) {
+        if (active) {
+            fill_off = p.c_off[i / SLICE];
+            fill_wa = p.c_wa[i / SLICE];
+        }
+    }
     // append entry k of a section of this row; a whole packet goes to memory
     auto fill_push = [&](FillBuf<FV, FL> &b, uint32_t k, int32_t c, FV v, FL l) {
 #pragma unroll
@@ -543,9 +551,8 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
         b.v[PACK - 1] = v;
         b.l[PACK - 1] = l;
         if (k % PACK == PACK - 1) {
-            const int32_t q = static_cast<int32_t>(i);
-            const int64_t o = p.c_off[q / SLICE];
-            const int ln = q % SLICE;
+            const int64_t o = fill_off;
+            const int ln = static_cast<int>(i % SLICE);
             const uint32_t kb = k - (PACK - 1);
             if (p.c_col) store_packet<int32_t>(p.c_col, o, kb, ln, b.c);   // NULL: values only
             store_packet<FV>(p.c_val, o, kb, ln, b.v);
@@ -653,7 +660,7 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
                             fill_push(fill_a, static_cast<uint32_t>(n_hit_a), c, v, l);
                             n_hit_a += 1;
                         } else {
-                            fill_push(fill_b, p.c_wa[i / SLICE] + static_cast<uint32_t>(n_hit), c, v, l);
+                            fill_push(fill_b, fill_wa + static_cast<uint32_t>(n_hit), c, v, l);
                             n_hit += 1;
                         }
                     }
@@ -717,7 +724,7 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
         }
     } else if constexpr (MODE == MODE_FILL) {
         if (active) {
-            const uint32_t wa = p.c_wa[i / SLICE], wb = p.c_wb[i / SLICE];
+            const uint32_t wa = fill_wa, wb = p.c_wb[i / SLICE];
             // pad to the slice's widths (multiples of PACK): the last packets go out
             const FV v0 = vzero(static_cast<FV *>(nullptr));
             const FL l0 = vzero(static_cast<FL *>(nullptr));
