@@ -191,7 +191,7 @@ class BatchLOCG:
 
 def batch_ground_locg(plan: Plan, x0, tol=1e-7, switch_tol=1e-5, maxiter=1000, check_every=10,
                       use_graph=True, mixed=True, record=False, energy_tol=None, energy_checks=2,
-                      switch_energy=1e-9, lo_storage=True):
+                      switch_energy=1e-9, lo_storage=True, check_every_hi=1):
     """Ground eigenpairs of every problem of ``plan`` (a ``BatchPlan``, or a ``Plan`` as one problem).
 
     A problem stops (its vectors freeze) at relative residual ``tol``, or, with
@@ -200,7 +200,10 @@ def batch_ground_locg(plan: Plan, x0, tol=1e-7, switch_tol=1e-5, maxiter=1000, c
     remaining decrease are both below ``energy_tol * |theta|`` at
     ``energy_checks`` consecutive polls.  In the low-precision phase the same
     test with the limit ``max(energy_tol, 30 switch_energy)`` (theta carries
-    FP32 image errors there) hands every problem over to the FP64 phase.
+    FP32 image errors there) hands every problem over to the FP64 phase, which
+    then polls every ``check_every_hi`` steps: it starts from a vector the FP32
+    phase has converged, so its energy settles within a few steps and a
+    sparse poll would only add steps.
 
     Returns ``(theta (K,), x, iterations, info)`` where ``info`` holds per-problem
     relative residuals, the iteration of the precision switch and timings.
@@ -234,7 +237,7 @@ def batch_ground_locg(plan: Plan, x0, tol=1e-7, switch_tol=1e-5, maxiter=1000, c
     thetas = []
     calm = torch.zeros(solver.K, dtype=torch.int64)
     while it < maxiter:
-        for _ in range(check_every):
+        for _ in range(check_every if switch_at is None else check_every_hi):
             run_step()
             it += 1
         # one device -> host copy per poll: active flags, theta, relative residuals
