@@ -129,6 +129,11 @@ struct KParams {
     const uint32_t *g_toff;
     const uint64_t *z;
     const void *coef;      // double2[T] or float2[T]
+    const int32_t *g_tab;  // per group: first entry of its sign table, or -1
+    const void *tab_re;    // sign tables in the precision of coef: real parts
+    const void *tab_im;    //   and imaginary parts
+    const float *tab_lo_re;  // FP32 sign tables (dual fill)
+    const float *tab_lo_im;
     const void *diag;      // double2[N] or float2[N]
     const uint32_t *tile_row0;
     const uint32_t *tile_seg;
@@ -306,6 +311,24 @@ __device__ inline R wave_sum(R v) {
 __device__ inline bool term_parallel(uint32_t n_terms, uint32_t n_hits) {
     return n_hits * (((n_terms + 63u) >> 6) * 10u + 20u) < n_terms * 8u;
 }
+// Sign tables.  When every term of a group has z_k = z_0 ^ t_k with t_k inside
+// the group's X mask x (JW double excitations: z differs only on the four
+// flipped orbitals), C(s) = (-1)^popc(s & z_0) tab[s restricted to x], where
+// tab[u] = sum_k c_k (-1)^popc(u & t_k) over the 2^popc(x) patterns u, summed
+// in term order.  Negation commutes with round-to-nearest, so this equals the
+// term-by-term sum bit for bit.
+constexpr int TAB_MAX_BITS = 4;      // tables of at most 16 entries
+constexpr int TAB_MIN_TERMS = 3;
+
+// the bits of s under the (wave-uniform) mask x, packed low
+__device__ inline uint32_t tab_index(uint64_t x, uint64_t s) {
+    uint32_t idx = 0;
+    int j = 0;
+    for (uint64_t xm = x; xm != 0; xm &= xm - 1, ++j)
+        idx |= static_cast<uint32_t>((s >> (__ffsll(static_cast<long long>(xm)) - 1)) & 1ull) << j;
+    return idx;
+}
+
 // the rule never picks term-parallel for groups shorter than this (8 K > 30
 // at one hit); plans without such groups launch matvec kernels compiled with
 // the row-parallel path only (TP = false), which give the same sums
@@ -317,7 +340,22 @@ __device__ inline void group_value_wave(const void *coef_v, const KParams &p, ui
     using CC = typename C2<R>::T;
     vx = R(0);
     vy = R(0);
-    if constexpr (!TP) {
+    if constexpr (TP) {
+        const int32_t toff = p.g_tab[g];
+        if (toff >= 0) {
+            if (hit) {
+                const uint32_t u = static_cast<uint32_t>(toff) + tab_index(p.g_x[g], s);
+                const bool neg = __popcll(s & p.z[p.g_toff[g]]) & 1;
+                const R re = static_cast<const R *>(p.tab_re)[u];
+                vx = neg ? -re : re;
+                if constexpr (!RE) {
+                    const R im = static_cast<const R *>(p.tab_im)[u];
+                    vy = neg ? -im : im;
+                }
+            }
+            return;
+        }
+    } else {
         if (hit) {
             if constexpr (RE) {
                 vx = group_coef_re_from<R>(coef_v, p, g, s);
@@ -467,6 +505,24 @@ template <bool RE>
 __device__ inline void group_value_wave_dual(const KParams &p, uint32_t g, uint64_t s, bool hit, double2 &hi, float2 &lo) {
     hi = make_double2(0.0, 0.0);
     lo = make_float2(0.f, 0.f);
+    const int32_t toff = p.g_tab[g];
+    if (toff >= 0) {
+        if (hit) {
+            const uint32_t u = static_cast<uint32_t>(toff) + tab_index(p.g_x[g], s);
+            const bool neg = __popcll(s & p.z[p.g_toff[g]]) & 1;
+            const double hre = static_cast<const double *>(p.tab_re)[u];
+            const float lre = p.tab_lo_re[u];
+            hi.x = neg ? -hre : hre;
+            lo.x = neg ? -lre : lre;
+            if constexpr (!RE) {
+                const double him = static_cast<const double *>(p.tab_im)[u];
+                const float lim = p.tab_lo_im[u];
+                hi.y = neg ? -him : him;
+                lo.y = neg ? -lim : lim;
+            }
+        }
+        return;
+    }
     const uint64_t m = __ballot(hit);
     if (m == 0) return;
     const uint32_t k0 = p.g_toff[g], k1 = p.g_toff[g + 1];
@@ -508,8 +564,9 @@ __device__ inline void group_value_wave_dual(const KParams &p, uint32_t g, uint6
 
 // One tile: <= 256 consecutive rows of one segment, one row per thread.
 // REALC: every phased coefficient is real (the matvec then sums real parts only).
-// TP: some group may be evaluated term-parallel (the plan has groups of
-// TP_MIN_TERMS or more terms).
+// TP: the full evaluator (sign tables, term-parallel sums); a plan with neither
+// sign tables nor groups of TP_MIN_TERMS or more terms (random Pauli sums: one
+// term per X mask) launches the plain one, which gives the same sums.
 template <int MODE, typename R, bool CPLX, bool REALC = false, bool TP = true>
 __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s_keys, int32_t *s_tseg,
                                     const void *__restrict__ xin, void *__restrict__ yout) {
@@ -797,6 +854,44 @@ hipError_t fill_async(V *y, int64_t begin, int64_t end, V value, hipStream_t st)
     const int64_t blocks = std::min<int64_t>((end - begin + 255) / 256, int64_t(1) << 16);
     hipLaunchKernelGGL(k_fill<V>, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, st, y, begin, end, value);
     return hipGetLastError();
+}
+
+// sign tables of the groups that have one (one thread per group), in FP64 and
+// FP32, each summed in term order as the term-by-term evaluation sums
+__global__ void k_group_tables(const int32_t *__restrict__ g_tab, const uint64_t *__restrict__ g_x,
+                               const uint32_t *__restrict__ g_toff, const uint64_t *__restrict__ z,
+                               const double2 *__restrict__ c64, const float2 *__restrict__ c32, int32_t n_groups,
+                               double *__restrict__ t64re, double *__restrict__ t64im, float *__restrict__ t32re,
+                               float *__restrict__ t32im) {
+    const int32_t g = static_cast<int32_t>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (g >= n_groups) return;
+    const int32_t off = g_tab[g];
+    if (off < 0) return;
+    const uint64_t x = g_x[g];
+    const uint32_t k0 = g_toff[g], k1 = g_toff[g + 1];
+    const uint64_t z0 = z[k0];
+    const uint32_t n_pat = 1u << __popcll(x);
+    for (uint32_t u = 0; u < n_pat; ++u) {
+        uint64_t sb = 0;              // pattern u deposited on the bits of x
+        int j = 0;
+        for (uint64_t xm = x; xm != 0; xm &= xm - 1, ++j)
+            if ((u >> j) & 1u) sb |= xm & (~xm + 1ull);
+        double2 a = make_double2(0.0, 0.0);
+        float2 f = make_float2(0.f, 0.f);
+        for (uint32_t k = k0; k < k1; ++k) {
+            const bool odd = __popcll(sb & (z[k] ^ z0)) & 1;
+            const double2 c = c64[k];
+            const float2 d = c32[k];
+            a.x += odd ? -c.x : c.x;
+            a.y += odd ? -c.y : c.y;
+            f.x += odd ? -d.x : d.x;
+            f.y += odd ? -d.y : d.y;
+        }
+        t64re[off + u] = a.x;
+        t64im[off + u] = a.y;
+        t32re[off + u] = f.x;
+        t32im[off + u] = f.y;
+    }
 }
 
 __global__ void k_narrow(const double2 *__restrict__ in, float2 *__restrict__ out, int64_t n) {
@@ -1648,7 +1743,7 @@ struct rqx_plan {
     int use_cache = 1;
     int device = 0;
     int is_real = 1;
-    int long_groups = 0;       // some group with x != 0 has TP_MIN_TERMS or more terms
+    int long_groups = 0;       // some group has a sign table, or x != 0 and TP_MIN_TERMS or more terms
     int max_seg = 0;
     int persistent = 0;        // launch the work-queue kernel
     int blocks_per_cu = 0;     // persistent grid: blocks per CU (0 = occupancy limit)
@@ -1679,11 +1774,19 @@ struct rqx_plan {
     ProbDesc *prob = nullptr;
     int64_t *prob_rows = nullptr;   // K + 1 row offsets
     unsigned int *queue = nullptr;
+    int32_t *g_tab = nullptr;       // per group: first sign-table entry, or -1
+    double *tab64re = nullptr;      // sign tables, FP64 and FP32, real and imaginary parts
+    double *tab64im = nullptr;
+    float *tab32re = nullptr;
+    float *tab32im = nullptr;
 };
 
 namespace {
 
 size_t align_up(size_t v) { return (v + 255) & ~size_t(255); }
+
+struct ProblemIn;
+bool has_sign_table(const ProblemIn &q, int32_t j);
 
 struct ProblemIn {
     int32_t n_qubits;
@@ -1697,8 +1800,28 @@ struct ProblemIn {
     int64_t n_states;
 };
 
+// group j of q gets a sign table: off-diagonal, at least TAB_MIN_TERMS terms,
+// at most TAB_MAX_BITS X bits, and every term's Z mask equal to the first
+// one's outside the X bits
+bool has_sign_table(const ProblemIn &q, int32_t j) {
+    const uint64_t x = q.xmask[j];
+    const int64_t k0 = q.grp_off[j], k1 = q.grp_off[j + 1];
+    if (x == 0 || k1 - k0 < TAB_MIN_TERMS || __builtin_popcountll(x) > TAB_MAX_BITS) return false;
+    for (int64_t t = k0 + 1; t < k1; ++t)
+        if ((q.zmask[t] ^ q.zmask[k0]) & ~x) return false;
+    return true;
+}
+
 // dynamic LDS of the tile kernel: one staged segment of uint32 low keys
 size_t seg_lds_bytes(const rqx_plan *p) { return static_cast<size_t>(std::max(p->max_seg, 1)) * 4; }
+
+// (re)compute the sign tables from the plan's coefficients
+void launch_group_tables(const rqx_plan *p, hipStream_t st) {
+    if (p->n_groups == 0) return;
+    const unsigned blocks = static_cast<unsigned>((p->n_groups + 255) / 256);
+    hipLaunchKernelGGL(k_group_tables, dim3(blocks), dim3(256), 0, st, p->g_tab, p->g_x, p->g_toff, p->z, p->coef64,
+                       p->coef32, p->n_groups, p->tab64re, p->tab64im, p->tab32re, p->tab32im);
+}
 
 KParams make_params(const rqx_plan *p, bool fp64) {
     KParams k{};
@@ -1713,6 +1836,11 @@ KParams make_params(const rqx_plan *p, bool fp64) {
     k.g_x = p->g_x;
     k.z = p->z;
     k.coef = fp64 ? static_cast<const void *>(p->coef64) : static_cast<const void *>(p->coef32);
+    k.g_tab = p->g_tab;
+    k.tab_re = fp64 ? static_cast<const void *>(p->tab64re) : static_cast<const void *>(p->tab32re);
+    k.tab_im = fp64 ? static_cast<const void *>(p->tab64im) : static_cast<const void *>(p->tab32im);
+    k.tab_lo_re = p->tab32re;
+    k.tab_lo_im = p->tab32im;
     k.diag = fp64 ? static_cast<const void *>(p->diag64) : static_cast<const void *>(p->diag32);
     k.tile_row0 = p->tile_row0;
     k.tile_seg = p->tile_seg;
@@ -1749,8 +1877,8 @@ hipError_t launch_tiles(const rqx_plan *p, const KParams &k, const void *x, void
     return hipGetLastError();
 }
 
-// matvec kernels without the term-parallel evaluator where no off-diagonal
-// group is long enough to use it (other modes always carry it)
+// matvec kernels with the plain evaluator where no group has a sign table or
+// is long enough for term-parallel sums (other modes always carry the full one)
 template <int MODE, typename R, bool CPLX, bool REALC>
 hipError_t launch_tp(const rqx_plan *p, const KParams &k, const void *x, void *y, hipStream_t st) {
     if (MODE == MODE_MATVEC && !p->long_groups) return launch_tiles<MODE, R, CPLX, REALC, MODE != MODE_MATVEC>(p, k, x, y, st);
@@ -1872,6 +2000,8 @@ int build_plan(rqx_plan **out, const std::vector<ProblemIn> &in, int64_t n_total
     std::vector<ProbDesc> desc(static_cast<size_t>(std::max(K, 1)));
     std::vector<int64_t> prob_rows(static_cast<size_t>(K) + 1, 0);
     std::vector<double> tile_cost;
+    std::vector<int32_t> g_tab;    // per group: first sign-table entry, or -1
+    int64_t n_tab = 0;
     int64_t row_base = 0, term_base = 0;
     int32_t grp_base = 0;
     for (int k = 0; k < K; ++k) {
@@ -1891,7 +2021,8 @@ int build_plan(rqx_plan **out, const std::vector<ProblemIn> &in, int64_t n_total
         for (int32_t j = 0; j < q.n_groups; ++j) {
             const int len = static_cast<int>(q.grp_off[j + 1] - q.grp_off[j]);
             ph.max_group_terms = std::max(ph.max_group_terms, len);
-            if (q.xmask[j] != 0) ph.max_offdiag_terms = std::max(ph.max_offdiag_terms, len);
+            // groups with a sign table are never evaluated term by term
+            if (q.xmask[j] != 0 && !has_sign_table(q, j)) ph.max_offdiag_terms = std::max(ph.max_offdiag_terms, len);
         }
         Segmentation sg;
         if (int rc = choose_split(q, split_bits, ph.has_diag, sg, k)) return rc;
@@ -1925,6 +2056,13 @@ int build_plan(rqx_plan **out, const std::vector<ProblemIn> &in, int64_t n_total
             g_xlo.push_back(static_cast<uint32_t>(q.xmask[j]) & lomask);
             g_x.push_back(q.xmask[j]);
             g_toff.push_back(static_cast<uint32_t>(term_base + q.grp_off[j]));
+            if (has_sign_table(q, j)) {
+                p->long_groups = 1;         // the full evaluator reads the tables
+                g_tab.push_back(static_cast<int32_t>(n_tab));
+                n_tab += int64_t(1) << __builtin_popcountll(q.xmask[j]);
+            } else {
+                g_tab.push_back(-1);
+            }
         }
         for (int64_t t = 0; t < q.n_terms; ++t) {
             zall.push_back(q.zmask[t]);
@@ -1989,7 +2127,10 @@ int build_plan(rqx_plan **out, const std::vector<ProblemIn> &in, int64_t n_total
                  o_c32 = carve(c32.size() * 8), o_d64 = carve(n * 16),
                  o_d32 = carve(n * 8), o_trow = carve(tile_row0.size() * 4), o_tseg = carve(tile_seg.size() * 4),
                  o_tprob = carve(tile_prob.size() * 4), o_desc = carve(desc.size() * sizeof(ProbDesc)),
-                 o_prows = carve(prob_rows.size() * 8), o_queue = carve(sizeof(unsigned int));
+                 o_prows = carve(prob_rows.size() * 8), o_queue = carve(sizeof(unsigned int)),
+                 o_gtab = carve(g_tab.size() * 4), o_t64re = carve(static_cast<size_t>(n_tab) * 8),
+                 o_t64im = carve(static_cast<size_t>(n_tab) * 8), o_t32re = carve(static_cast<size_t>(n_tab) * 4),
+                 o_t32im = carve(static_cast<size_t>(n_tab) * 4);
     HIP_TRY(hipMalloc(&p->block, off));
     char *b = static_cast<char *>(p->block);
     p->keys = reinterpret_cast<uint64_t *>(b + o_keys);
@@ -2012,6 +2153,11 @@ int build_plan(rqx_plan **out, const std::vector<ProblemIn> &in, int64_t n_total
     p->prob = reinterpret_cast<ProbDesc *>(b + o_desc);
     p->prob_rows = reinterpret_cast<int64_t *>(b + o_prows);
     p->queue = reinterpret_cast<unsigned int *>(b + o_queue);
+    p->g_tab = reinterpret_cast<int32_t *>(b + o_gtab);
+    p->tab64re = reinterpret_cast<double *>(b + o_t64re);
+    p->tab64im = reinterpret_cast<double *>(b + o_t64im);
+    p->tab32re = reinterpret_cast<float *>(b + o_t32re);
+    p->tab32im = reinterpret_cast<float *>(b + o_t32im);
     rqx_plan *raw = p.get();
     auto release = [&]() { (void)hipFree(raw->block); };
 
@@ -2028,7 +2174,7 @@ int build_plan(rqx_plan **out, const std::vector<ProblemIn> &in, int64_t n_total
         up(p->coef64, c64.data(), c64.size() * 8), up(p->coef32, c32.data(), c32.size() * 8),
         up(p->tile_row0, tile_row0.data(), tile_row0.size() * 4), up(p->tile_seg, tile_seg.data(), tile_seg.size() * 4),
         up(p->tile_prob, tile_prob.data(), tile_prob.size() * 4), up(p->prob, desc.data(), desc.size() * sizeof(ProbDesc)),
-        up(p->prob_rows, prob_rows.data(), prob_rows.size() * 8)};
+        up(p->prob_rows, prob_rows.data(), prob_rows.size() * 8), up(p->g_tab, g_tab.data(), g_tab.size() * 4)};
     for (hipError_t ei : errs)
         if (ei != hipSuccess) {
             release();
@@ -2041,6 +2187,7 @@ int build_plan(rqx_plan **out, const std::vector<ProblemIn> &in, int64_t n_total
         hipLaunchKernelGGL(k_diag, dim3(blocks), dim3(256), 0, st, p->keys + ph.row_base, ph.n_states, p->z, p->coef64,
                            k0, k0 + ph.diag_terms, p->diag64 + ph.row_base, p->diag32 + ph.row_base);
     }
+    launch_group_tables(raw, st);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e == hipSuccess) {
@@ -2368,6 +2515,8 @@ int rqx_plan_update_coef(rqx_plan *p, const double *coef, int32_t is_real, rqx_s
     HIP_TRY(hipMemcpyAsync(p->coef64, coef, static_cast<size_t>(p->n_terms) * 16, hipMemcpyDeviceToDevice, st));
     const unsigned tb = static_cast<unsigned>((p->n_terms + 255) / 256);
     hipLaunchKernelGGL(k_narrow, dim3(tb), dim3(256), 0, st, p->coef64, p->coef32, p->n_terms);
+    HIP_TRY(hipGetLastError());
+    launch_group_tables(p, st);
     HIP_TRY(hipGetLastError());
     for (const ProbHost &ph : p->probs) {
         if (ph.n_states == 0) continue;
