@@ -287,3 +287,55 @@ def test_lookup_table_beyond_2_32_entries():
     assert int(idx[-1].min()) >= -1 and int(idx.max()) < plan.n_states
     del idx
     torch.cuda.empty_cache()
+
+
+def _table_problem(seed, n=14, n_groups=60, n_states=3000):
+    """Groups whose terms differ only on their X bits (sign tables), mixed with ordinary groups."""
+    rng = np.random.default_rng(seed)
+    xs, zs = [], []
+    for g in range(n_groups):
+        bits = rng.choice(n, size=int(rng.integers(1, 6)), replace=False)
+        x = int(sum(1 << int(b) for b in bits))
+        z0 = int(rng.integers(0, 1 << n))
+        k = int(rng.integers(1, 9))
+        for _ in range(k):
+            if g % 3 == 2:                        # ordinary: z anywhere
+                z = int(rng.integers(0, 1 << n))
+            else:                                 # z_0 ^ (a subset of x)
+                z = z0 ^ (x & int(rng.integers(0, 1 << n)))
+            xs.append(x)
+            zs.append(z)
+    xs.append(0)                                  # a diagonal term
+    zs.append(5)
+    xm = np.array(xs, dtype=np.uint64)
+    zm = np.array(zs, dtype=np.uint64)
+    tables = TermTables.from_masks(xm, zm, rng.standard_normal(xm.shape[0]), n)
+    states = workloads.random_subspace(n, n_states, rng)
+    return tables, states, xm, zm
+
+
+def test_sign_table_groups_match_oracle_and_follow_updates():
+    """Groups evaluated by sign table (incl. complex phases) equal the oracle; set_alphas rebuilds the tables."""
+    _say('sign tables')
+    tables, states, xm, zm = _table_problem(31)
+    sizes = np.diff(tables.grp_off)
+    bits = np.array([bin(int(x)).count('1') for x in tables.xmask])
+    assert np.any((sizes >= 3) & (bits <= 4) & (tables.xmask != 0))   # some groups take the table path
+    assert np.any(tables.coef.imag != 0)                              # Y phases: complex tables
+    plan = Plan(tables, states)
+    rng = np.random.default_rng(32)
+    v = rng.standard_normal(plan.n_total) + 1j * rng.standard_normal(plan.n_total)
+    want = sqdref.matvec_u64(tables.xmask, _groups(tables), states, v)
+    y = plan.matvec(torch.from_numpy(v).to(DEV)).cpu().numpy()
+    assert _rel(y, want) < RTOL64
+    y32 = plan.matvec(torch.from_numpy(v.astype(np.complex64)).to(DEV)).cpu().numpy()
+    assert _rel(y32, want) < RTOL32
+    # new coefficients: the tables follow (oracle rebuilt from the same masks)
+    alpha = rng.standard_normal(plan.info['n_terms'])
+    plan.set_alphas(torch.from_numpy(alpha).to(DEV))
+    phases = plan.term_phases().cpu().numpy()
+    coef = alpha * phases
+    t2 = TermTables(tables.xmask, tables.grp_off, tables.zmask, coef, tables.num_qubits)
+    want2 = sqdref.matvec_u64(t2.xmask, _groups(t2), states, v)
+    y2 = plan.matvec(torch.from_numpy(v).to(DEV)).cpu().numpy()
+    assert _rel(y2, want2) < RTOL64
