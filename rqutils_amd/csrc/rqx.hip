@@ -125,6 +125,8 @@ struct KParams {
     const uint32_t *seg_off;
     const uint64_t *cls_hi;
     const uint32_t *cls_goff;
+    const uint32_t *st_off;        // per segment: its first (class, target segment) pair
+    const uint2 *st_list;          // pairs of the classes whose target segment exists, class order
     const uint32_t *g_xlo;
     const uint32_t *g_toff;
     const uint64_t *z;
@@ -568,7 +570,7 @@ __device__ inline void group_value_wave_dual(const KParams &p, uint32_t g, uint6
 // sign tables nor groups of TP_MIN_TERMS or more terms (random Pauli sums: one
 // term per X mask) launches the plain one, which gives the same sums.
 template <int MODE, typename R, bool CPLX, bool REALC = false, bool TP = true>
-__device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s_keys, int32_t *s_tseg,
+__device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s_keys,
                                     const void *__restrict__ xin, void *__restrict__ yout) {
     using V = typename VT<R, CPLX>::T;
     using CC = typename C2<R>::T;
@@ -579,7 +581,6 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
     const bool active = i < p.seg_off[seg + 1];
     const uint64_t s = active ? p.keys[i] : 0ull;
     const uint32_t slo = static_cast<uint32_t>(s) & pd.lomask;
-    const uint64_t shi = p.seg_hi[seg];
     const V *x = static_cast<const V *>(xin);
 
     V acc = vzero(static_cast<V *>(nullptr));       // diagonal + section-B groups
@@ -641,27 +642,19 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
             coo_emit(p, active, i, i, active ? static_cast<const double2 *>(p.diag)[i] : make_double2(0.0, 0.0));
     }
 
-    const int n_classes = static_cast<int>(pd.n_classes);
-    for (int cb = 0; cb < n_classes; cb += TILE) {
-        const int c = cb + tid;
-        int t = -1;
-        if (c < n_classes) {
-            t = find_u64(p.seg_hi + pd.seg_base, static_cast<int>(pd.n_segs), shi ^ p.cls_hi[pd.cls_base + c]);
-            if (t >= 0) t += static_cast<int>(pd.seg_base);
-            if constexpr (MODE == MODE_COUNT) n_look += 1;
-        }
-        __syncthreads();              // readers of the previous chunk are done
-        s_tseg[tid] = t;
-        __syncthreads();
-        const int cn = min(TILE, n_classes - cb);
-        for (int cc = 0; cc < cn; ++cc) {
-            const int ts = __builtin_amdgcn_readfirstlane(s_tseg[cc]);
-            if (ts < 0) continue;
+    // the classes whose target segment exists for this tile's segment, in class
+    // order, with that segment (listed once per plan by k_seg_targets)
+    const uint32_t e1 = p.st_off[seg + 1];
+    for (uint32_t e = p.st_off[seg]; e < e1; ++e) {
+        {
+            const uint2 ct = p.st_list[e];
+            const uint32_t ts = ct.y;
+            if constexpr (MODE == MODE_COUNT) n_look += (tid == 0);
             const uint32_t ta = p.seg_off[ts];
             const uint32_t tn = p.seg_off[ts + 1] - ta;
             for (uint32_t k = tid; k < tn; k += TILE) s_keys[k] = p.lo[ta + k];
             __syncthreads();
-            const uint32_t cls = pd.cls_base + cb + cc;
+            const uint32_t cls = ct.x;
             uint32_t g0 = p.cls_goff[cls];
             const uint32_t g1 = p.cls_goff[cls + 1];
             if (pd.has_diag && g0 == pd.grp_base) g0 += 1;
@@ -811,8 +804,7 @@ template <int MODE, typename R, bool CPLX, bool REALC = false, bool TP = true>
 __global__ __launch_bounds__(TILE) void k_tiles(const KParams p, const void *__restrict__ xin,
                                                 void *__restrict__ yout) {
     extern __shared__ uint32_t s_keys[];      // staged target segment (max_segment keys)
-    __shared__ int32_t s_tseg[TILE];
-    process_tile<MODE, R, CPLX, REALC, TP>(p, xcd_remap(blockIdx.x, gridDim.x), s_keys, s_tseg, xin, yout);
+    process_tile<MODE, R, CPLX, REALC, TP>(p, xcd_remap(blockIdx.x, gridDim.x), s_keys, xin, yout);
 }
 
 // Persistent workgroups pull tiles from a queue (tiles are stored largest
@@ -821,7 +813,6 @@ template <int MODE, typename R, bool CPLX, bool REALC = false, bool TP = true>
 __global__ __launch_bounds__(TILE) void k_tiles_queue(const KParams p, const void *__restrict__ xin,
                                                       void *__restrict__ yout) {
     extern __shared__ uint32_t s_keys[];
-    __shared__ int32_t s_tseg[TILE];
     __shared__ uint32_t s_tile;
     // a workgroup can claim at most n_tiles + 1 times (the last claim is past the end)
     for (uint32_t claim = 0; claim <= p.n_tiles; ++claim) {
@@ -834,7 +825,7 @@ __global__ __launch_bounds__(TILE) void k_tiles_queue(const KParams p, const voi
         const uint32_t tile = __builtin_amdgcn_readfirstlane(s_tile);
         __syncthreads();              // every thread has read s_tile before the next claim
         if (tile >= p.n_tiles) break;
-        process_tile<MODE, R, CPLX, REALC, TP>(p, tile, s_keys, s_tseg, xin, yout);
+        process_tile<MODE, R, CPLX, REALC, TP>(p, tile, s_keys, xin, yout);
         __syncthreads();              // the whole workgroup leaves the tile before the next claim
     }
 }
@@ -854,6 +845,43 @@ hipError_t fill_async(V *y, int64_t begin, int64_t end, V value, hipStream_t st)
     const int64_t blocks = std::min<int64_t>((end - begin + 255) / 256, int64_t(1) << 16);
     hipLaunchKernelGGL(k_fill<V>, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, st, y, begin, end, value);
     return hipGetLastError();
+}
+
+// (class, target segment) pairs of one problem's segments: one workgroup per
+// segment m, the problem's classes c in chunks of 256, kept when the segment
+// seg_hi[m] ^ cls_hi[c] exists, in class order.  list == NULL: count only
+// (count[m - seg0]); else write from st_off[m] on.
+__global__ __launch_bounds__(256) void k_seg_targets(const uint64_t *__restrict__ seg_hi,
+                                                     const uint64_t *__restrict__ cls_hi, uint32_t seg0,
+                                                     uint32_t n_segs, uint32_t cls0, uint32_t n_classes,
+                                                     uint32_t *__restrict__ count, const uint32_t *__restrict__ st_off,
+                                                     uint2 *__restrict__ list) {
+    __shared__ uint32_t s_wave[TILE / 64];
+    const uint32_t m = seg0 + blockIdx.x;
+    const uint64_t shi = seg_hi[m];
+    const int tid = static_cast<int>(threadIdx.x), lane = tid & 63, wave = tid >> 6;
+    uint32_t written = 0;
+    for (uint32_t cb = 0; cb < n_classes; cb += TILE) {
+        const uint32_t c = cb + static_cast<uint32_t>(tid);
+        int t = -1;
+        if (c < n_classes) t = find_u64(seg_hi + seg0, static_cast<int>(n_segs), shi ^ cls_hi[cls0 + c]);
+        const uint64_t bal = __ballot(t >= 0);
+        if (lane == 0) s_wave[wave] = static_cast<uint32_t>(__popcll(bal));
+        __syncthreads();
+        uint32_t before = 0, total = 0;
+        for (int w = 0; w < TILE / 64; ++w) {
+            before += (w < wave) ? s_wave[w] : 0u;
+            total += s_wave[w];
+        }
+        if (t >= 0 && list) {
+            const uint32_t below = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(bal >> 32),
+                                                             __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(bal), 0u));
+            list[st_off[m] + written + before + below] = make_uint2(cls0 + c, seg0 + static_cast<uint32_t>(t));
+        }
+        written += total;
+        __syncthreads();              // s_wave is rewritten by the next chunk
+    }
+    if (tid == 0 && count) count[blockIdx.x] = written;
 }
 
 // sign tables of the groups that have one (one thread per group), in FP64 and
@@ -1774,6 +1802,9 @@ struct rqx_plan {
     ProbDesc *prob = nullptr;
     int64_t *prob_rows = nullptr;   // K + 1 row offsets
     unsigned int *queue = nullptr;
+    void *st_block = nullptr;       // per-segment target lists (own allocation)
+    uint32_t *st_off = nullptr;     // M + 1 offsets into st_list
+    uint2 *st_list = nullptr;       // (class, target segment) pairs
     int32_t *g_tab = nullptr;       // per group: first sign-table entry, or -1
     double *tab64re = nullptr;      // sign tables, FP64 and FP32, real and imaginary parts
     double *tab64im = nullptr;
@@ -1823,6 +1854,56 @@ void launch_group_tables(const rqx_plan *p, hipStream_t st) {
                        p->coef32, p->n_groups, p->tab64re, p->tab64im, p->tab32re, p->tab32im);
 }
 
+// per segment, the classes whose target segment exists and that segment
+// (two passes of k_seg_targets: counts, then the pairs at their offsets)
+hipError_t build_targets(rqx_plan *p, hipStream_t st) {
+    const size_t M = static_cast<size_t>(std::max(p->n_segs, 0));
+    uint32_t *cnt = nullptr;
+    hipError_t e = hipMalloc(&cnt, std::max<size_t>(M, 1) * 4);
+    if (e != hipSuccess) return e;
+    e = fill_async(cnt, int64_t(0), static_cast<int64_t>(M), 0u, st);
+    if (e != hipSuccess) {
+        (void)hipFree(cnt);
+        return e;
+    }
+    for (const ProbHost &ph : p->probs)
+        if (ph.n_segs > 0 && ph.n_classes > 0)
+            hipLaunchKernelGGL(k_seg_targets, dim3(ph.n_segs), dim3(TILE), 0, st, p->seg_hi, p->cls_hi,
+                               static_cast<uint32_t>(ph.seg_base), static_cast<uint32_t>(ph.n_segs),
+                               static_cast<uint32_t>(ph.cls_base), static_cast<uint32_t>(ph.n_classes),
+                               cnt + ph.seg_base, nullptr, nullptr);
+    std::vector<uint32_t> hc(M);
+    e = hipGetLastError();
+    if (e == hipSuccess && M) e = hipMemcpyAsync(hc.data(), cnt, M * 4, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    (void)hipFree(cnt);
+    if (e != hipSuccess) return e;
+    std::vector<uint32_t> off(M + 1, 0);
+    uint64_t total = 0;
+    for (size_t m = 0; m < M; ++m) {
+        off[m] = static_cast<uint32_t>(total);
+        total += hc[m];
+        if (total > UINT32_MAX) return hipErrorInvalidValue;
+    }
+    off[M] = static_cast<uint32_t>(total);
+    const size_t o_list = align_up((M + 1) * 4);
+    e = hipMalloc(&p->st_block, o_list + std::max<size_t>(total, 1) * sizeof(uint2));
+    if (e != hipSuccess) return e;
+    p->st_off = static_cast<uint32_t *>(p->st_block);
+    p->st_list = reinterpret_cast<uint2 *>(static_cast<char *>(p->st_block) + o_list);
+    e = hipMemcpyAsync(p->st_off, off.data(), off.size() * 4, hipMemcpyHostToDevice, st);
+    if (e != hipSuccess) return e;
+    for (const ProbHost &ph : p->probs)
+        if (ph.n_segs > 0 && ph.n_classes > 0)
+            hipLaunchKernelGGL(k_seg_targets, dim3(ph.n_segs), dim3(TILE), 0, st, p->seg_hi, p->cls_hi,
+                               static_cast<uint32_t>(ph.seg_base), static_cast<uint32_t>(ph.n_segs),
+                               static_cast<uint32_t>(ph.cls_base), static_cast<uint32_t>(ph.n_classes), nullptr,
+                               p->st_off, p->st_list);
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(st);   // off is released on return
+    return e;
+}
+
 KParams make_params(const rqx_plan *p, bool fp64) {
     KParams k{};
     k.keys = p->keys;
@@ -1831,6 +1912,8 @@ KParams make_params(const rqx_plan *p, bool fp64) {
     k.seg_off = p->seg_off;
     k.cls_hi = p->cls_hi;
     k.cls_goff = p->cls_goff;
+    k.st_off = p->st_off;
+    k.st_list = p->st_list;
     k.g_xlo = p->g_xlo;
     k.g_toff = p->g_toff;
     k.g_x = p->g_x;
@@ -2159,7 +2242,10 @@ int build_plan(rqx_plan **out, const std::vector<ProblemIn> &in, int64_t n_total
     p->tab32re = reinterpret_cast<float *>(b + o_t32re);
     p->tab32im = reinterpret_cast<float *>(b + o_t32im);
     rqx_plan *raw = p.get();
-    auto release = [&]() { (void)hipFree(raw->block); };
+    auto release = [&]() {
+        (void)hipFree(raw->block);
+        (void)hipFree(raw->st_block);
+    };
 
     auto up = [&](void *dst, const void *src, size_t bytes) -> hipError_t {
         if (bytes == 0) return hipSuccess;
@@ -2189,6 +2275,7 @@ int build_plan(rqx_plan **out, const std::vector<ProblemIn> &in, int64_t n_total
     }
     launch_group_tables(raw, st);
     hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = build_targets(raw, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e == hipSuccess) {
         int occ = 0;
@@ -2317,6 +2404,8 @@ int rqx_plan_destroy(rqx_plan *plan) {
     HIP_TRY(hipGetDevice(&prev));
     HIP_TRY(hipSetDevice(plan->device));
     hipError_t e = hipFree(plan->block);
+    const hipError_t es = hipFree(plan->st_block);
+    if (e == hipSuccess) e = es;
     for (HitCache &c : plan->cache) {
         const hipError_t ec = free_cache(c);
         if (e == hipSuccess) e = ec;
