@@ -26,6 +26,7 @@
 #include <cstring>
 #include <memory>
 #include <string>
+#include <thread>
 #include <initializer_list>
 #include <type_traits>
 #include <utility>
@@ -126,6 +127,9 @@ struct KParams {
     const uint64_t *cls_hi;
     const uint32_t *cls_goff;
     const uint32_t *st_off;        // per segment: its first (class, target segment) pair
+    const uint32_t *h_off;         // hashed segments (NULL: binary search): first table entry,
+    const uint32_t *h_log;         //   log2 of the bucket count,
+    const uint32_t *h_tab;         //   tables of (low key << 13 | rank), two entries per bucket
     const uint2 *st_list;          // pairs of the classes whose target segment exists, class order
     const uint32_t *g_xlo;
     const uint32_t *g_toff;
@@ -219,6 +223,33 @@ __device__ inline uint32_t xcd_remap(uint32_t b, uint32_t nb) {
 }
 
 // index of t in sorted a[0..n) or -1
+// Hashed segments: the low keys of a segment sit in a two-choice cuckoo table
+// of 2^lg buckets of two u32 entries (key << HASH_RANK_BITS | rank within the
+// segment; empty = all ones, never a key with L <= 18), so a lookup reads two
+// 8-byte buckets that do not depend on each other.
+constexpr int HASH_RANK_BITS = 13;     // segments hold at most 8192 rows
+constexpr uint32_t HASH_EMPTY = 0xFFFFFFFFu;
+constexpr int HASH_MAX_KEY_BITS = 18;
+
+__host__ __device__ inline uint32_t hash_b1(uint32_t k, uint32_t lg) { return (k * 0x9E3779B1u) >> (32u - lg); }
+__host__ __device__ inline uint32_t hash_b2(uint32_t k, uint32_t lg) {
+    return ((k ^ 0x5BD1E995u) * 0x85EBCA6Bu) >> (32u - lg);
+}
+
+// look t up in the staged table; on a hit, rank = its position in the segment
+__device__ inline bool hash_find(const uint32_t *tab, uint32_t lg, uint32_t t, uint32_t &rank) {
+    const uint2 a = reinterpret_cast<const uint2 *>(tab)[hash_b1(t, lg)];
+    const uint2 b = reinterpret_cast<const uint2 *>(tab)[hash_b2(t, lg)];
+    constexpr uint32_t RM = (1u << HASH_RANK_BITS) - 1u;
+    bool f = false;
+    rank = 0;
+    if ((a.x >> HASH_RANK_BITS) == t) { f = true; rank = a.x & RM; }
+    if ((a.y >> HASH_RANK_BITS) == t) { f = true; rank = a.y & RM; }
+    if ((b.x >> HASH_RANK_BITS) == t) { f = true; rank = b.x & RM; }
+    if ((b.y >> HASH_RANK_BITS) == t) { f = true; rank = b.y & RM; }
+    return f;
+}
+
 __device__ inline int find_u64(const uint64_t *__restrict__ a, int n, uint64_t t) {
     if (n <= 0) return -1;
     int base = 0, len = n;
@@ -652,14 +683,33 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
             if constexpr (MODE == MODE_COUNT) n_look += (tid == 0);
             const uint32_t ta = p.seg_off[ts];
             const uint32_t tn = p.seg_off[ts + 1] - ta;
-            for (uint32_t k = tid; k < tn; k += TILE) s_keys[k] = p.lo[ta + k];
+            const bool hashed = p.h_off != nullptr;          // the same for the whole plan
+            uint32_t lg = 0;
+            if (hashed) {
+                lg = p.h_log[ts];
+                const uint32_t ho = p.h_off[ts], ns = 2u << lg;
+                for (uint32_t k = tid; k < ns; k += TILE) s_keys[k] = p.h_tab[ho + k];
+            } else {
+                for (uint32_t k = tid; k < tn; k += TILE) s_keys[k] = p.lo[ta + k];
+            }
             __syncthreads();
+            // position of t in the target segment and whether it is there
+            auto search = [&](uint32_t t, uint32_t &base) -> bool {
+                if (hashed) return hash_find(s_keys, lg, t, base);
+                base = 0;
+                for (uint32_t len = tn; len > 1;) {
+                    const uint32_t half = len >> 1;
+                    base = (s_keys[base + half] <= t) ? base + half : base;
+                    len -= half;
+                }
+                return s_keys[base] == t;
+            };
             const uint32_t cls = ct.x;
             uint32_t g0 = p.cls_goff[cls];
             const uint32_t g1 = p.cls_goff[cls + 1];
             if (pd.has_diag && g0 == pd.grp_base) g0 += 1;
-            auto handle = [&](uint32_t g, uint32_t base, uint32_t tl, bool sec_a) {
-                const bool hit = active && (s_keys[base] == tl);
+            auto handle = [&](uint32_t g, uint32_t base, bool found, bool sec_a) {
+                const bool hit = active && found;
                 if constexpr (MODE == MODE_MATVEC) {
                     R vx, vy;
                     group_value_wave<R, REALC, TP>(p.coef, p, g, s, hit, vx, vy);
@@ -723,13 +773,9 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
             // groups are sorted by X mask, so a class's group without low flips
             // (section A) is its first one
             if (g < g1 && p.g_xlo[g] == 0) {
-                uint32_t base = 0, len = tn;
-                while (len > 1) {
-                    const uint32_t half = len >> 1;
-                    base = (s_keys[base + half] <= slo) ? base + half : base;
-                    len -= half;
-                }
-                handle(g, base, slo, true);
+                uint32_t base;
+                const bool f = search(slo, base);
+                handle(g, base, f, true);
                 ++g;
             }
             // four independent searches in lock step: the dependent LDS loads of
@@ -737,29 +783,36 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
             for (; g + 4 <= g1; g += 4) {
                 const uint32_t t0 = slo ^ p.g_xlo[g], t1 = slo ^ p.g_xlo[g + 1];
                 const uint32_t t2 = slo ^ p.g_xlo[g + 2], t3 = slo ^ p.g_xlo[g + 3];
-                uint32_t b0 = 0, b1 = 0, b2 = 0, b3 = 0, len = tn;
-                while (len > 1) {
-                    const uint32_t half = len >> 1;
-                    b0 = (s_keys[b0 + half] <= t0) ? b0 + half : b0;
-                    b1 = (s_keys[b1 + half] <= t1) ? b1 + half : b1;
-                    b2 = (s_keys[b2 + half] <= t2) ? b2 + half : b2;
-                    b3 = (s_keys[b3 + half] <= t3) ? b3 + half : b3;
-                    len -= half;
+                uint32_t b0 = 0, b1 = 0, b2 = 0, b3 = 0;
+                bool f0, f1, f2, f3;
+                if (hashed) {
+                    f0 = hash_find(s_keys, lg, t0, b0);
+                    f1 = hash_find(s_keys, lg, t1, b1);
+                    f2 = hash_find(s_keys, lg, t2, b2);
+                    f3 = hash_find(s_keys, lg, t3, b3);
+                } else {
+                    for (uint32_t len = tn; len > 1;) {
+                        const uint32_t half = len >> 1;
+                        b0 = (s_keys[b0 + half] <= t0) ? b0 + half : b0;
+                        b1 = (s_keys[b1 + half] <= t1) ? b1 + half : b1;
+                        b2 = (s_keys[b2 + half] <= t2) ? b2 + half : b2;
+                        b3 = (s_keys[b3 + half] <= t3) ? b3 + half : b3;
+                        len -= half;
+                    }
+                    f0 = s_keys[b0] == t0;
+                    f1 = s_keys[b1] == t1;
+                    f2 = s_keys[b2] == t2;
+                    f3 = s_keys[b3] == t3;
                 }
-                handle(g, b0, t0, false);
-                handle(g + 1, b1, t1, false);
-                handle(g + 2, b2, t2, false);
-                handle(g + 3, b3, t3, false);
+                handle(g, b0, f0, false);
+                handle(g + 1, b1, f1, false);
+                handle(g + 2, b2, f2, false);
+                handle(g + 3, b3, f3, false);
             }
             for (; g < g1; ++g) {
-                const uint32_t tl = slo ^ p.g_xlo[g];
-                uint32_t base = 0, len = tn;
-                while (len > 1) {
-                    const uint32_t half = len >> 1;
-                    base = (s_keys[base + half] <= tl) ? base + half : base;
-                    len -= half;
-                }
-                handle(g, base, tl, false);
+                uint32_t base;
+                const bool f = search(slo ^ p.g_xlo[g], base);
+                handle(g, base, f, false);
             }
             __syncthreads();          // before the next target overwrites s_keys
         }
@@ -1802,6 +1855,11 @@ struct rqx_plan {
     ProbDesc *prob = nullptr;
     int64_t *prob_rows = nullptr;   // K + 1 row offsets
     unsigned int *queue = nullptr;
+    int hashed = 0;                 // targets found by hash tables (else binary search)
+    int hash_words = 0;             // largest segment hash table, u32 entries
+    uint32_t *h_off = nullptr;      // per segment: first hash-table entry
+    uint32_t *h_log = nullptr;      // per segment: log2 of its bucket count
+    uint32_t *h_tab = nullptr;      // hash tables
     void *st_block = nullptr;       // per-segment target lists (own allocation)
     uint32_t *st_off = nullptr;     // M + 1 offsets into st_list
     uint2 *st_list = nullptr;       // (class, target segment) pairs
@@ -1843,8 +1901,79 @@ bool has_sign_table(const ProblemIn &q, int32_t j) {
     return true;
 }
 
-// dynamic LDS of the tile kernel: one staged segment of uint32 low keys
-size_t seg_lds_bytes(const rqx_plan *p) { return static_cast<size_t>(std::max(p->max_seg, 1)) * 4; }
+// two-choice cuckoo table of one segment's sorted low keys (2^lg buckets of two
+// entries, key << HASH_RANK_BITS | rank); false if the insertions do not settle
+bool cuckoo_build(const uint32_t *keys, uint32_t n, uint32_t lg, uint32_t *tab) {
+    const uint32_t nb = 1u << lg;
+    std::fill(tab, tab + 2 * nb, HASH_EMPTY);
+    uint64_t rng = 0x9E3779B97F4A7C15ull ^ n;
+    for (uint32_t r = 0; r < n; ++r) {
+        uint32_t e = (keys[r] << HASH_RANK_BITS) | r;
+        bool placed = false;
+        for (int kick = 0; kick < 512 && !placed; ++kick) {
+            const uint32_t k = e >> HASH_RANK_BITS;
+            const uint32_t c[2] = {hash_b1(k, lg), hash_b2(k, lg)};
+            for (int i = 0; i < 2 && !placed; ++i)
+                for (int j = 0; j < 2 && !placed; ++j)
+                    if (tab[2 * c[i] + j] == HASH_EMPTY) {
+                        tab[2 * c[i] + j] = e;
+                        placed = true;
+                    }
+            if (placed) break;
+            rng = rng * 6364136223846793005ull + 1442695040888963407ull;
+            const uint32_t slot = 2 * c[(rng >> 33) & 1] + static_cast<uint32_t>((rng >> 34) & 1);
+            std::swap(e, tab[slot]);
+        }
+        if (!placed) return false;
+    }
+    return true;
+}
+
+// Hash tables of every segment (lo: low keys, seg_off: M + 1 row offsets), built
+// on the host threads; h_log[m] = log2 buckets, h_off[m] = first entry.  False if
+// some segment's table does not settle (the plan then uses binary search).
+bool build_hash_tables(const std::vector<uint32_t> &lo, const std::vector<uint32_t> &seg_off,
+                       std::vector<uint32_t> &h_off, std::vector<uint32_t> &h_log, std::vector<uint32_t> &h_tab) {
+    const size_t M = seg_off.size() - 1;
+    std::vector<std::vector<uint32_t>> tabs(M);
+    h_log.assign(M, 0);
+    std::vector<char> ok(M, 1);
+    const unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    auto work = [&](unsigned w) {
+        for (size_t m = w; m < M; m += nt) {
+            const uint32_t n = seg_off[m + 1] - seg_off[m];
+            uint32_t lg = 1;
+            while ((2u << lg) * 7u < n * 10u) ++lg;     // load <= 0.7
+            bool done = false;
+            for (int tries = 0; tries < 3 && !done; ++tries, ++lg) {
+                tabs[m].resize(size_t(2) << lg);
+                done = cuckoo_build(lo.data() + seg_off[m], n, lg, tabs[m].data());
+                if (done) h_log[m] = lg;
+            }
+            ok[m] = done;
+        }
+    };
+    std::vector<std::thread> pool;
+    for (unsigned w = 1; w < nt; ++w) pool.emplace_back(work, w);
+    work(0);
+    for (std::thread &t : pool) t.join();
+    h_off.assign(M, 0);
+    uint64_t total = 0;
+    for (size_t m = 0; m < M; ++m) {
+        if (!ok[m]) return false;
+        h_off[m] = static_cast<uint32_t>(total);
+        total += tabs[m].size();
+        if (total > UINT32_MAX) return false;
+    }
+    h_tab.resize(static_cast<size_t>(total));
+    for (size_t m = 0; m < M; ++m) std::copy(tabs[m].begin(), tabs[m].end(), h_tab.begin() + h_off[m]);
+    return true;
+}
+
+// dynamic LDS of the tile kernel: one staged segment of uint32 low keys, or its hash table
+size_t seg_lds_bytes(const rqx_plan *p) {
+    return static_cast<size_t>(std::max(p->hashed ? p->hash_words : p->max_seg, 1)) * 4;
+}
 
 // (re)compute the sign tables from the plan's coefficients
 void launch_group_tables(const rqx_plan *p, hipStream_t st) {
@@ -1913,6 +2042,9 @@ KParams make_params(const rqx_plan *p, bool fp64) {
     k.cls_hi = p->cls_hi;
     k.cls_goff = p->cls_goff;
     k.st_off = p->st_off;
+    k.h_off = p->hashed ? p->h_off : nullptr;
+    k.h_log = p->h_log;
+    k.h_tab = p->h_tab;
     k.st_list = p->st_list;
     k.g_xlo = p->g_xlo;
     k.g_toff = p->g_toff;
@@ -2189,6 +2321,19 @@ int build_plan(rqx_plan **out, const std::vector<ProblemIn> &in, int64_t n_total
         tile_prob.swap(pr);
     }
 
+    // hashed target lookups when every problem's low keys fit the entries
+    std::vector<uint32_t> h_off, h_log, h_tab;
+    bool hash_fits = true;
+    for (const ProbHost &ph : p->probs) hash_fits = hash_fits && ph.L <= HASH_MAX_KEY_BITS;
+    if (hash_fits && !seg_off_all.empty() && build_hash_tables(lo, seg_off_all, h_off, h_log, h_tab)) {
+        p->hashed = 1;
+        for (uint32_t lg : h_log) p->hash_words = std::max(p->hash_words, 2 << lg);
+    } else {
+        h_off.clear();
+        h_log.clear();
+        h_tab.clear();
+    }
+
     // term arrays padded with zero terms (coefficient sums fetch TERM_PAD terms at a time)
     zall.resize(zall.size() + TERM_PAD, 0ull);
     c64.resize(c64.size() + 2 * TERM_PAD, 0.0);
@@ -2213,7 +2358,8 @@ int build_plan(rqx_plan **out, const std::vector<ProblemIn> &in, int64_t n_total
                  o_prows = carve(prob_rows.size() * 8), o_queue = carve(sizeof(unsigned int)),
                  o_gtab = carve(g_tab.size() * 4), o_t64re = carve(static_cast<size_t>(n_tab) * 8),
                  o_t64im = carve(static_cast<size_t>(n_tab) * 8), o_t32re = carve(static_cast<size_t>(n_tab) * 4),
-                 o_t32im = carve(static_cast<size_t>(n_tab) * 4);
+                 o_t32im = carve(static_cast<size_t>(n_tab) * 4), o_hoff = carve(h_off.size() * 4),
+                 o_hlog = carve(h_log.size() * 4), o_htab = carve(h_tab.size() * 4);
     HIP_TRY(hipMalloc(&p->block, off));
     char *b = static_cast<char *>(p->block);
     p->keys = reinterpret_cast<uint64_t *>(b + o_keys);
@@ -2241,6 +2387,9 @@ int build_plan(rqx_plan **out, const std::vector<ProblemIn> &in, int64_t n_total
     p->tab64im = reinterpret_cast<double *>(b + o_t64im);
     p->tab32re = reinterpret_cast<float *>(b + o_t32re);
     p->tab32im = reinterpret_cast<float *>(b + o_t32im);
+    p->h_off = reinterpret_cast<uint32_t *>(b + o_hoff);
+    p->h_log = reinterpret_cast<uint32_t *>(b + o_hlog);
+    p->h_tab = reinterpret_cast<uint32_t *>(b + o_htab);
     rqx_plan *raw = p.get();
     auto release = [&]() {
         (void)hipFree(raw->block);
@@ -2260,7 +2409,9 @@ int build_plan(rqx_plan **out, const std::vector<ProblemIn> &in, int64_t n_total
         up(p->coef64, c64.data(), c64.size() * 8), up(p->coef32, c32.data(), c32.size() * 8),
         up(p->tile_row0, tile_row0.data(), tile_row0.size() * 4), up(p->tile_seg, tile_seg.data(), tile_seg.size() * 4),
         up(p->tile_prob, tile_prob.data(), tile_prob.size() * 4), up(p->prob, desc.data(), desc.size() * sizeof(ProbDesc)),
-        up(p->prob_rows, prob_rows.data(), prob_rows.size() * 8), up(p->g_tab, g_tab.data(), g_tab.size() * 4)};
+        up(p->prob_rows, prob_rows.data(), prob_rows.size() * 8), up(p->g_tab, g_tab.data(), g_tab.size() * 4),
+        up(p->h_off, h_off.data(), h_off.size() * 4), up(p->h_log, h_log.data(), h_log.size() * 4),
+        up(p->h_tab, h_tab.data(), h_tab.size() * 4)};
     for (hipError_t ei : errs)
         if (ei != hipSuccess) {
             release();
