@@ -339,3 +339,24 @@ def test_sign_table_groups_match_oracle_and_follow_updates():
     want2 = sqdref.matvec_u64(t2.xmask, _groups(t2), states, v)
     y2 = plan.matvec(torch.from_numpy(v).to(DEV)).cpu().numpy()
     assert _rel(y2, want2) < RTOL64
+
+
+@pytest.mark.parametrize('split', [12, 18, 22, 26])
+def test_hashed_and_binary_search_targets(split):
+    """Splits up to 18 bits find targets by hash table, wider ones by binary search: both exact."""
+    _say(f'target search split={split}')
+    rng = np.random.default_rng(41)
+    codes = workloads.random_pauli_codes(30, 150, (0.6, 0.14, 0.13, 0.13), rng)
+    xm, zm = workloads.codes_to_masks(codes)
+    tables = TermTables.from_masks(xm, zm, rng.standard_normal(xm.shape[0]), 30)
+    states = workloads.random_subspace(30, 4000, rng)
+    # states sharing their high bits, so that targets exist for many groups
+    states = np.unique(np.concatenate([states, states ^ np.uint64(1 << 3), states ^ np.uint64(1 << 20)]))
+    plan = Plan(tables, states, split_bits=split)
+    assert plan.info['split_bits'] == split
+    got = plan.lookup().cpu().numpy()
+    assert np.array_equal(got, _oracle_lookup(tables, states, plan.n_total))
+    v = rng.standard_normal(plan.n_total) + 1j * rng.standard_normal(plan.n_total)
+    y = plan.matvec(torch.from_numpy(v).to(DEV)).cpu().numpy()
+    want = sqdref.matvec_u64(tables.xmask, _groups(tables), states, v)
+    assert _rel(y, want) < RTOL64
