@@ -332,6 +332,7 @@ def extra_batch(maxiter=1000):
     import numpy as np
     import torch
     from rqutils_amd import workloads
+    from rqutils_amd.autotune import autotune_batch
     from rqutils_amd.plan import BatchPlan
     from rqutils_amd.locg_batch import batch_ground_locg
     probs = [workloads.batch40_problem(i) for i in range(64)]
@@ -343,10 +344,11 @@ def extra_batch(maxiter=1000):
     x = torch.from_numpy(rng.standard_normal(bp.n_total) + 1j * rng.standard_normal(bp.n_total)).to(bp.device)
     x64 = x.to(torch.complex64)
     y = torch.empty_like(x64)
-    ms_q = _event_ms(lambda: bp.matvec(x64, out=y), 5)
     bp.set_launch(False)
     ms_t = _event_ms(lambda: bp.matvec(x64, out=y), 5)
-    bp.set_launch(True)
+    # the persistent grid picked by measurement (the autotuner's batch mode)
+    bp, tuned = autotune_batch(bp, reps=3)
+    ms_q = _event_ms(lambda: bp.matvec(x64, out=y), 5)
     theta, _, it, info = batch_ground_locg(bp, x, tol=1e-7, maxiter=maxiter, check_every=10, use_graph=True)
     rel = info['rel_residual']
     log(f'batch: matvec {ms_q:.1f} ms (queue) / {ms_t:.1f} ms (per tile); LOCG {info["seconds"]:.2f} s, '
@@ -364,6 +366,7 @@ def extra_batch(maxiter=1000):
         f'LOCG {info_c["seconds"]:.2f} s, {int((rel_c <= 1e-7).sum())}/64 converged')
     out = {'problems': 64, 'sum_states': bp.n_states, 'sum_terms': bp.info['n_terms'], 'tiles': bp.info['n_tiles'],
            'matvec_ms_work_queue': round(ms_q, 3), 'matvec_ms_one_block_per_tile': round(ms_t, 3),
+           'autotuned_launch': tuned[0], 'autotune_results': tuned,
            'locg_seconds': round(info['seconds'], 3), 'locg_iterations': it,
            'converged_rel_residual_1e-7': int((rel <= 1e-7).sum()), 'worst_rel_residual': float(rel.max()),
            'precision_switch_iteration': info['switch_iteration'], 'hip_graph': True,

@@ -4,12 +4,14 @@ The plan's host cost model picks a split L (rows are segmented by ``s >> L``);
 this module times the matvec with HIP events for splits around that choice
 and for both launch shapes (one workgroup per tile, or persistent workgroups
 on a tile queue at the occupancy limit or twice fewer), and returns the
-fastest plan.  Timings are kept so callers can report them.
+fastest plan.  Timings are kept so callers can report them.  ``autotune_batch``
+does the same for a batched plan over the persistent grid size (the batch's
+splits are the cost model's, one per problem).
 """
 import torch
 
 from rqutils_amd import _capi
-from rqutils_amd.plan import Plan
+from rqutils_amd.plan import BatchPlan, Plan
 
 
 def time_matvec(plan, dtype=torch.complex64, reps=3, warmup=1):
@@ -63,4 +65,22 @@ def autotune(tables, states, n_total=None, dtype=torch.complex64, deltas=(-3, -2
     results.sort(key=lambda r: r['ms'])
     _, plan, persistent, bpc = best
     plan.set_launch(persistent, bpc)
+    return plan, results
+
+
+def autotune_batch(problems, dtype=torch.complex64, reps=3, device=None,
+                   launch_shapes=((True, 0), (True, 2), (True, 4), (True, 6), (True, 8), (False, 0))):
+    """Best launch shape of a ``BatchPlan`` over ``problems`` by measured matvec time.
+
+    ``launch_shapes`` are ``(persistent, blocks_per_cu)`` pairs (0 = the
+    occupancy limit).  Returns ``(plan, results)`` with ``results`` sorted
+    fastest first, each ``{'persistent', 'blocks_per_cu', 'ms'}``.
+    """
+    plan = problems if isinstance(problems, BatchPlan) else BatchPlan(problems, device=device)
+    results = []
+    for persistent, bpc in launch_shapes:
+        plan.set_launch(persistent, bpc)
+        results.append({'persistent': persistent, 'blocks_per_cu': bpc, 'ms': time_matvec(plan, dtype, reps)})
+    results.sort(key=lambda r: r['ms'])
+    plan.set_launch(results[0]['persistent'], results[0]['blocks_per_cu'])
     return plan, results

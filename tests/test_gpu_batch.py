@@ -245,3 +245,17 @@ def test_autotune_picks_a_valid_plan():
     ref = Plan(w.tables, w.states)
     x = torch.randn(plan.n_total, dtype=torch.float64, device=DEV)
     assert torch.allclose(plan.matvec(x), ref.matvec(x), rtol=0, atol=1e-12)
+
+
+def test_autotune_batch_picks_a_launch_shape():
+    """The batched autotuner leaves the fastest measured launch shape set; the matvec is unchanged."""
+    _say('autotune batch')
+    from rqutils_amd.autotune import autotune_batch
+    probs = [_random_problem(70 + k, 12, 60, 400 + 300 * k) for k in range(4)]
+    ref = BatchPlan(probs)
+    x = torch.randn(ref.n_total, dtype=torch.complex128, device=DEV)
+    want = ref.matvec(x)
+    plan, results = autotune_batch(probs, reps=2)
+    assert len(results) == 6 and results[0]['ms'] <= results[-1]['ms']
+    assert plan.info['persistent'] == int(results[0]['persistent'])
+    assert torch.equal(plan.matvec(x), want)
