@@ -89,8 +89,11 @@ const char *rqx_last_error(void);
 
 /*
  * Build a plan: upload the grouped term tables and the sorted subspace index
- * to the current device, choose the segment split and precompute the x == 0
- * group's diagonal.  All input pointers are HOST pointers.
+ * to the current device, choose the segment split, precompute the x == 0
+ * group's diagonal, the per-segment lists of classes whose target segment
+ * exists, the sign tables of groups whose terms differ only on their X bits,
+ * and (splits of at most 18 bits) per-segment hash tables of the low keys,
+ * built on host threads.  All input pointers are HOST pointers.
  * split_bits < 0 selects the split with the host cost model.
  * Replaces the setup half of run_sqd (rqutils/sqd.py:341-351: uniquified
  * states + the J x N xsource cache) -- no J x N cache is built.
@@ -164,7 +167,8 @@ int rqx_matvec(const rqx_plan *plan, const void *x, void *y, int32_t dtype, rqx_
 /*
  * counts[0] = structural nonzeros (row, group) hits incl. the diagonal,
  * counts[1] = (row, group) pairs resolved at the segment level (the fine probes),
- * counts[2] = segment-level class lookups.  counts: device, 3 x int64 (overwritten).
+ * counts[2] = (tile, target segment) pairs visited (each stages one target).
+ * counts: device, 3 x int64 (overwritten).
  */
 int rqx_count(const rqx_plan *plan, int64_t *counts, rqx_stream_t stream);
 

@@ -213,7 +213,7 @@ class Plan:
         self.is_real = is_real
 
     def count(self):
-        """Structural nonzeros, segment-level probes and class lookups of one matvec."""
+        """Structural nonzeros, segment-level probes and (tile, target segment) pairs of one matvec."""
         out = torch.zeros(3, dtype=torch.int64, device=self.device)
         stream = _capi.current_stream(self.device)
         _capi.check(_capi.lib().rqx_count(self._handle, out.data_ptr(), stream))
