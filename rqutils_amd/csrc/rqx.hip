@@ -135,7 +135,8 @@ struct KParams {
     const uint32_t *g_toff;
     const uint64_t *z;
     const void *coef;      // double2[T] or float2[T]
-    const int32_t *g_tab;  // per group: first entry of its sign table, or -1
+    const int32_t *g_tab;  // per group: index of its sign table among the tables, or -1
+    uint32_t n_tab;        // sign tables; entry u of table t at u * n_tab + t
     const void *tab_re;    // sign tables in the precision of coef: real parts
     const void *tab_im;    //   and imaginary parts
     const float *tab_lo_re;  // FP32 sign tables (dual fill)
@@ -351,6 +352,7 @@ __device__ inline bool term_parallel(uint32_t n_terms, uint32_t n_hits) {
 // in term order.  Negation commutes with round-to-nearest, so this equals the
 // term-by-term sum bit for bit.
 constexpr int TAB_MAX_BITS = 4;      // tables of at most 16 entries
+constexpr int TAB_ENTRIES = 1 << TAB_MAX_BITS;
 constexpr int TAB_MIN_TERMS = 3;
 
 // the bits of s under the (wave-uniform) mask x, packed low
@@ -377,7 +379,7 @@ __device__ inline void group_value_wave(const void *coef_v, const KParams &p, ui
         const int32_t toff = p.g_tab[g];
         if (toff >= 0) {
             if (hit) {
-                const uint32_t u = static_cast<uint32_t>(toff) + tab_index(p.g_x[g], s);
+                const uint32_t u = tab_index(p.g_x[g], s) * p.n_tab + static_cast<uint32_t>(toff);
                 const bool neg = __popcll(s & p.z[p.g_toff[g]]) & 1;
                 const R re = static_cast<const R *>(p.tab_re)[u];
                 vx = neg ? -re : re;
@@ -541,7 +543,7 @@ __device__ inline void group_value_wave_dual(const KParams &p, uint32_t g, uint6
     const int32_t toff = p.g_tab[g];
     if (toff >= 0) {
         if (hit) {
-            const uint32_t u = static_cast<uint32_t>(toff) + tab_index(p.g_x[g], s);
+            const uint32_t u = tab_index(p.g_x[g], s) * p.n_tab + static_cast<uint32_t>(toff);
             const bool neg = __popcll(s & p.z[p.g_toff[g]]) & 1;
             const double hre = static_cast<const double *>(p.tab_re)[u];
             const float lre = p.tab_lo_re[u];
@@ -942,12 +944,12 @@ __global__ __launch_bounds__(256) void k_seg_targets(const uint64_t *__restrict_
 __global__ void k_group_tables(const int32_t *__restrict__ g_tab, const uint64_t *__restrict__ g_x,
                                const uint32_t *__restrict__ g_toff, const uint64_t *__restrict__ z,
                                const double2 *__restrict__ c64, const float2 *__restrict__ c32, int32_t n_groups,
-                               double *__restrict__ t64re, double *__restrict__ t64im, float *__restrict__ t32re,
-                               float *__restrict__ t32im) {
+                               uint32_t n_tab, double *__restrict__ t64re, double *__restrict__ t64im,
+                               float *__restrict__ t32re, float *__restrict__ t32im) {
     const int32_t g = static_cast<int32_t>(blockIdx.x * blockDim.x + threadIdx.x);
     if (g >= n_groups) return;
-    const int32_t off = g_tab[g];
-    if (off < 0) return;
+    const int32_t t = g_tab[g];
+    if (t < 0) return;
     const uint64_t x = g_x[g];
     const uint32_t k0 = g_toff[g], k1 = g_toff[g + 1];
     const uint64_t z0 = z[k0];
@@ -968,10 +970,13 @@ __global__ void k_group_tables(const int32_t *__restrict__ g_tab, const uint64_t
             f.x += odd ? -d.x : d.x;
             f.y += odd ? -d.y : d.y;
         }
-        t64re[off + u] = a.x;
-        t64im[off + u] = a.y;
-        t32re[off + u] = f.x;
-        t32im[off + u] = f.y;
+        // pattern-major: the few patterns a subspace uses are shared by all
+        // tables, so their entries of neighbouring groups share cache lines
+        const size_t o = static_cast<size_t>(u) * n_tab + static_cast<uint32_t>(t);
+        t64re[o] = a.x;
+        t64im[o] = a.y;
+        t32re[o] = f.x;
+        t32im[o] = f.y;
     }
 }
 
@@ -1863,7 +1868,8 @@ struct rqx_plan {
     void *st_block = nullptr;       // per-segment target lists (own allocation)
     uint32_t *st_off = nullptr;     // M + 1 offsets into st_list
     uint2 *st_list = nullptr;       // (class, target segment) pairs
-    int32_t *g_tab = nullptr;       // per group: first sign-table entry, or -1
+    int32_t *g_tab = nullptr;       // per group: its sign table, or -1
+    uint32_t n_tab = 0;             // sign tables (entry u of table t at u * n_tab + t)
     double *tab64re = nullptr;      // sign tables, FP64 and FP32, real and imaginary parts
     double *tab64im = nullptr;
     float *tab32re = nullptr;
@@ -1980,7 +1986,7 @@ void launch_group_tables(const rqx_plan *p, hipStream_t st) {
     if (p->n_groups == 0) return;
     const unsigned blocks = static_cast<unsigned>((p->n_groups + 255) / 256);
     hipLaunchKernelGGL(k_group_tables, dim3(blocks), dim3(256), 0, st, p->g_tab, p->g_x, p->g_toff, p->z, p->coef64,
-                       p->coef32, p->n_groups, p->tab64re, p->tab64im, p->tab32re, p->tab32im);
+                       p->coef32, p->n_groups, p->n_tab, p->tab64re, p->tab64im, p->tab32re, p->tab32im);
 }
 
 // per segment, the classes whose target segment exists and that segment
@@ -2052,6 +2058,7 @@ KParams make_params(const rqx_plan *p, bool fp64) {
     k.z = p->z;
     k.coef = fp64 ? static_cast<const void *>(p->coef64) : static_cast<const void *>(p->coef32);
     k.g_tab = p->g_tab;
+    k.n_tab = p->n_tab;
     k.tab_re = fp64 ? static_cast<const void *>(p->tab64re) : static_cast<const void *>(p->tab32re);
     k.tab_im = fp64 ? static_cast<const void *>(p->tab64im) : static_cast<const void *>(p->tab32im);
     k.tab_lo_re = p->tab32re;
@@ -2271,10 +2278,10 @@ int build_plan(rqx_plan **out, const std::vector<ProblemIn> &in, int64_t n_total
             g_xlo.push_back(static_cast<uint32_t>(q.xmask[j]) & lomask);
             g_x.push_back(q.xmask[j]);
             g_toff.push_back(static_cast<uint32_t>(term_base + q.grp_off[j]));
-            if (has_sign_table(q, j)) {
+            if (has_sign_table(q, j) && n_tab < (int64_t(1) << 27)) {   // entries indexed in 32 bits
                 p->long_groups = 1;         // the full evaluator reads the tables
                 g_tab.push_back(static_cast<int32_t>(n_tab));
-                n_tab += int64_t(1) << __builtin_popcountll(q.xmask[j]);
+                n_tab += 1;
             } else {
                 g_tab.push_back(-1);
             }
@@ -2321,6 +2328,8 @@ int build_plan(rqx_plan **out, const std::vector<ProblemIn> &in, int64_t n_total
         tile_prob.swap(pr);
     }
 
+    p->n_tab = static_cast<uint32_t>(n_tab);
+
     // hashed target lookups when every problem's low keys fit the entries
     std::vector<uint32_t> h_off, h_log, h_tab;
     bool hash_fits = true;
@@ -2356,9 +2365,10 @@ int build_plan(rqx_plan **out, const std::vector<ProblemIn> &in, int64_t n_total
                  o_d32 = carve(n * 8), o_trow = carve(tile_row0.size() * 4), o_tseg = carve(tile_seg.size() * 4),
                  o_tprob = carve(tile_prob.size() * 4), o_desc = carve(desc.size() * sizeof(ProbDesc)),
                  o_prows = carve(prob_rows.size() * 8), o_queue = carve(sizeof(unsigned int)),
-                 o_gtab = carve(g_tab.size() * 4), o_t64re = carve(static_cast<size_t>(n_tab) * 8),
-                 o_t64im = carve(static_cast<size_t>(n_tab) * 8), o_t32re = carve(static_cast<size_t>(n_tab) * 4),
-                 o_t32im = carve(static_cast<size_t>(n_tab) * 4), o_hoff = carve(h_off.size() * 4),
+                 o_gtab = carve(g_tab.size() * 4), o_t64re = carve(static_cast<size_t>(n_tab) * TAB_ENTRIES * 8),
+                 o_t64im = carve(static_cast<size_t>(n_tab) * TAB_ENTRIES * 8),
+                 o_t32re = carve(static_cast<size_t>(n_tab) * TAB_ENTRIES * 4),
+                 o_t32im = carve(static_cast<size_t>(n_tab) * TAB_ENTRIES * 4), o_hoff = carve(h_off.size() * 4),
                  o_hlog = carve(h_log.size() * 4), o_htab = carve(h_tab.size() * 4);
     HIP_TRY(hipMalloc(&p->block, off));
     char *b = static_cast<char *>(p->block);
