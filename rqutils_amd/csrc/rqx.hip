@@ -710,6 +710,59 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
             uint32_t g0 = p.cls_goff[cls];
             const uint32_t g1 = p.cls_goff[cls + 1];
             if (pd.has_diag && g0 == pd.grp_base) g0 += 1;
+            // a hit's value (and, for the matvec, its x entry) first, then its
+            // use: the full-evaluator matvec takes four groups' values before
+            // using them, so their loads are in flight together; values are
+            // used in group order
+            struct HitVal {
+                R vx, vy;
+                double2 vh;
+                float2 vl;
+                V xv;
+            };
+            auto value = [&](uint32_t g, uint32_t base, bool hit) {
+                HitVal h{};
+                if constexpr (MODE == MODE_MATVEC) {
+                    group_value_wave<R, REALC, TP>(p.coef, p, g, s, hit, h.vx, h.vy);
+                    if (hit) h.xv = x[ta + base];
+                } else if constexpr (MODE == MODE_FILL) {
+                    if (p.c_val_lo) group_value_wave_dual<!CPLX>(p, g, s, hit, h.vh, h.vl);   // both precisions, R = double
+                    else group_value_wave<R, !CPLX>(p.coef, p, g, s, hit, h.vx, h.vy);
+                }
+                return h;
+            };
+            auto consume = [&](uint32_t base, bool hit, bool sec_a, const HitVal &h) {
+                if constexpr (MODE == MODE_MATVEC) {
+                    if constexpr (REALC) {
+                        if (hit) fma_acc(sec_a ? acc_a : acc, h.vx, h.xv);
+                    } else {
+                        CC c;
+                        c.x = h.vx;
+                        c.y = h.vy;
+                        if (hit) fma_acc(sec_a ? acc_a : acc, c, h.xv);
+                    }
+                } else if constexpr (MODE == MODE_FILL) {
+                    if (hit) {
+                        FV v;
+                        FL l;
+                        if constexpr (CPLX) {
+                            v = p.c_val_lo ? CC{static_cast<R>(h.vh.x), static_cast<R>(h.vh.y)} : CC{h.vx, h.vy};
+                            l = h.vl;
+                        } else {
+                            v = p.c_val_lo ? static_cast<R>(h.vh.x) : h.vx;
+                            l = h.vl.x;
+                        }
+                        const int32_t c = static_cast<int32_t>(ta + base);
+                        if (sec_a) {
+                            fill_push(fill_a, static_cast<uint32_t>(n_hit_a), c, v, l);
+                            n_hit_a += 1;
+                        } else {
+                            fill_push(fill_b, fill_wa + static_cast<uint32_t>(n_hit), c, v, l);
+                            n_hit += 1;
+                        }
+                    }
+                }
+            };
             auto handle = [&](uint32_t g, uint32_t base, bool found, bool sec_a) {
                 const bool hit = active && found;
                 if constexpr (MODE == MODE_MATVEC) {
@@ -723,6 +776,8 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
                         c.y = vy;
                         if (hit) fma_acc(sec_a ? acc_a : acc, c, x[ta + base]);
                     }
+                } else if constexpr (MODE == MODE_FILL) {
+                    consume(base, hit, sec_a, value(g, base, hit));
                 } else if constexpr (MODE == MODE_LOOKUP) {
                     if (hit) {
                         const size_t o = static_cast<size_t>(g) * p.n_total + i;
@@ -740,32 +795,6 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
                     double vx, vy;
                     group_value_wave<double, false>(p.coef, p, g, s, hit, vx, vy);
                     coo_emit(p, hit, i, ta + base, make_double2(vx, vy));
-                } else if constexpr (MODE == MODE_FILL) {
-                    // values first (a wave-collective evaluation), then this lane's slot
-                    double2 vh = make_double2(0.0, 0.0);
-                    float2 vl = make_float2(0.f, 0.f);
-                    R vx = R(0), vy = R(0);
-                    if (p.c_val_lo) group_value_wave_dual<!CPLX>(p, g, s, hit, vh, vl);   // both precisions, R = double
-                    else group_value_wave<R, !CPLX>(p.coef, p, g, s, hit, vx, vy);
-                    if (hit) {
-                        FV v;
-                        FL l;
-                        if constexpr (CPLX) {
-                            v = p.c_val_lo ? CC{static_cast<R>(vh.x), static_cast<R>(vh.y)} : CC{vx, vy};
-                            l = vl;
-                        } else {
-                            v = p.c_val_lo ? static_cast<R>(vh.x) : vx;
-                            l = vl.x;
-                        }
-                        const int32_t c = static_cast<int32_t>(ta + base);
-                        if (sec_a) {
-                            fill_push(fill_a, static_cast<uint32_t>(n_hit_a), c, v, l);
-                            n_hit_a += 1;
-                        } else {
-                            fill_push(fill_b, fill_wa + static_cast<uint32_t>(n_hit), c, v, l);
-                            n_hit += 1;
-                        }
-                    }
                 } else {
                     n_hit += hit;
                     n_probe += active;
@@ -806,10 +835,20 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
                     f2 = s_keys[b2] == t2;
                     f3 = s_keys[b3] == t3;
                 }
-                handle(g, b0, f0, false);
-                handle(g + 1, b1, f1, false);
-                handle(g + 2, b2, f2, false);
-                handle(g + 3, b3, f3, false);
+                if constexpr (MODE == MODE_MATVEC && TP) {     // (measured: slower for single-term groups and the fill)
+                    const bool h0 = active && f0, h1 = active && f1, h2 = active && f2, h3 = active && f3;
+                    const HitVal v0 = value(g, b0, h0), v1 = value(g + 1, b1, h1);
+                    const HitVal v2 = value(g + 2, b2, h2), v3 = value(g + 3, b3, h3);
+                    consume(b0, h0, false, v0);
+                    consume(b1, h1, false, v1);
+                    consume(b2, h2, false, v2);
+                    consume(b3, h3, false, v3);
+                } else {
+                    handle(g, b0, f0, false);
+                    handle(g + 1, b1, f1, false);
+                    handle(g + 2, b2, f2, false);
+                    handle(g + 3, b3, f3, false);
+                }
             }
             for (; g < g1; ++g) {
                 uint32_t base;
@@ -1833,7 +1872,8 @@ struct rqx_plan {
     int max_seg = 0;
     int persistent = 0;        // launch the work-queue kernel
     int blocks_per_cu = 0;     // persistent grid: blocks per CU (0 = occupancy limit)
-    int occupancy = 1;         // resident tile workgroups per CU
+    int occupancy = 1;         // resident tile workgroups per CU (FP64 complex matvec kernel)
+    mutable std::vector<std::pair<const void *, int>> occ_cache;   // per tile-kernel instance
     int n_cu = 1;
     int64_t n_states = 0, n_total = 0, n_terms = 0;
     int32_t n_groups = 0, n_classes = 0, n_segs = 0, n_tiles = 0;
@@ -2090,7 +2130,20 @@ hipError_t launch_tiles(const rqx_plan *p, const KParams &k, const void *x, void
     const size_t lds = seg_lds_bytes(p);
     if (p->persistent) {
         hipLaunchKernelGGL(k_fill<unsigned int>, dim3(1), dim3(64), 0, st, p->queue, int64_t(0), int64_t(1), 0u);
-        const int bpc = p->blocks_per_cu > 0 ? p->blocks_per_cu : p->occupancy;
+        int bpc = p->blocks_per_cu;
+        if (bpc <= 0) {   // the occupancy of this very kernel (their register counts differ)
+            const void *fn = reinterpret_cast<const void *>(&k_tiles_queue<MODE, R, CPLX, REALC, TP>);
+            bpc = 0;
+            for (const auto &e : p->occ_cache)
+                if (e.first == fn) bpc = e.second;
+            if (bpc <= 0) {
+                int occ = 0;
+                if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, TILE, lds) != hipSuccess || occ < 1)
+                    occ = p->occupancy;
+                p->occ_cache.emplace_back(fn, occ);
+                bpc = occ;
+            }
+        }
         const int grid = std::max(1, std::min(p->n_tiles, p->n_cu * bpc));
         hipLaunchKernelGGL((k_tiles_queue<MODE, R, CPLX, REALC, TP>), dim3(grid), dim3(TILE), lds, st, k, x, y);
     } else {
