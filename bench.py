@@ -425,7 +425,19 @@ def main():
     ap.add_argument('--energy-tol', type=float, default=1e-9)
     ap.add_argument('--no-extras', action='store_true',
                     help='skip the 25M matvec, the 64-problem batch and the gradient timings')
+    ap.add_argument('--calibrate-copy', action='store_true',
+                    help='only copy 256 MiB three times (a kernel of known traffic for the PMC byte counters)')
     args = ap.parse_args()
+
+    if args.calibrate_copy:
+        import torch
+        src = torch.randn(64 * 1024 * 1024, device='cuda')
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({'copy_bytes_per_dispatch': src.numel() * 4, 'dispatches': 3}))
+        return
 
     if args.gpus != 1:
         print(json.dumps({'error': 'this build runs on one GPU; --gpus must be 1 '
