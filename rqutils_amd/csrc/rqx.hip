@@ -95,8 +95,8 @@ template <typename T> __device__ inline void store_packet(void *a, int64_t off, 
     }
 }
 
-// fill pass: the last PACK entries of one section of a row (column, value and,
-// when both precisions are filled, FP32 value), stored once the packet is whole
+// fill pass: the last PACK section-B entries of a row (column, value and, when
+// both precisions are filled, FP32 value), stored once the packet is whole
 template <typename FV, typename FL> struct FillBuf {
     int32_t c[PACK];
     FV v[PACK];
@@ -621,7 +621,7 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
     unsigned long long n_hit = 0, n_probe = 0, n_look = 0, n_hit_a = 0;
     using FV = typename std::conditional<CPLX, CC, R>::type;          // fill: stored value
     using FL = typename std::conditional<CPLX, float2, float>::type;  // fill: FP32 value of a dual fill
-    FillBuf<FV, FL> fill_a, fill_b;
+    FillBuf<FV, FL> fill_b;
     int64_t fill_off = 0;         // the row's slice: first entry and section-A width
     uint32_t fill_wa = 0;
     if constexpr (MODE == MODE_FILL) {
@@ -630,7 +630,15 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
             fill_wa = p.c_wa[i / SLICE];
         }
     }
-    // append entry k of a section of this row; a whole packet goes to memory
+    // section A: entry k of this row stored at once (in product subspaces the
+    // lanes of a wave write the same entry index, so the stores coalesce)
+    auto fill_put = [&](uint32_t k, int32_t c, FV v, FL l) {
+        const int ln = static_cast<int>(i % SLICE);
+        if (p.c_col) p.c_col[packed_slot<int32_t>(fill_off, k, ln)] = c;   // NULL: values only
+        static_cast<FV *>(p.c_val)[packed_slot<FV>(fill_off, k, ln)] = v;
+        if (p.c_val_lo) static_cast<FL *>(p.c_val_lo)[packed_slot<FL>(fill_off, k, ln)] = l;
+    };
+    // section B: append entry k of this row; a whole packet goes to memory
     auto fill_push = [&](FillBuf<FV, FL> &b, uint32_t k, int32_t c, FV v, FL l) {
 #pragma unroll
         for (int u = 0; u + 1 < PACK; ++u) {
@@ -754,7 +762,7 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
                         }
                         const int32_t c = static_cast<int32_t>(ta + base);
                         if (sec_a) {
-                            fill_push(fill_a, static_cast<uint32_t>(n_hit_a), c, v, l);
+                            fill_put(static_cast<uint32_t>(n_hit_a), c, v, l);
                             n_hit_a += 1;
                         } else {
                             fill_push(fill_b, fill_wa + static_cast<uint32_t>(n_hit), c, v, l);
@@ -873,7 +881,7 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
             const FV v0 = vzero(static_cast<FV *>(nullptr));
             const FL l0 = vzero(static_cast<FL *>(nullptr));
             for (uint32_t k = static_cast<uint32_t>(n_hit_a); k < wa; ++k)
-                fill_push(fill_a, k, static_cast<int32_t>(i), v0, l0);
+                fill_put(k, static_cast<int32_t>(i), v0, l0);
             for (uint32_t k = wa + static_cast<uint32_t>(n_hit); k < wa + wb; ++k)
                 fill_push(fill_b, k, static_cast<int32_t>(i), v0, l0);
         }
