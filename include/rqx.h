@@ -222,19 +222,22 @@ int rqx_plan_update_coef(rqx_plan *plan, const double *coef, int32_t is_real, rq
  * coalesce across a slice, and the rest), each padded to its widest row.  A
  * level of the first section whose 64 source rows are the slice's rows shifted
  * by one offset (product subspaces: almost all) stores that offset instead of
- * 64 columns.  Entries are written in the order the matrix-free kernel
- * accumulates them and with the same values, so rqx_matvec gives identical
- * results with or without the cache.
+ * 64 columns, and one whose 64 values are equal by construction (one group
+ * evaluated the same way for every row, rows agreeing on its Z bits) stores one
+ * value.  Entries are written in the order the matrix-free kernel accumulates
+ * them and with the same values, so rqx_matvec gives identical results with or
+ * without the cache.
  *
  * rqx_plan_build_cache: precision 32 (used by RQX_F32 / RQX_C64 matvecs), 64
- * (RQX_F64 / RQX_C128), or 0 for both in one pass over the hits with one shared
- * column array; values are real when the Hamiltonian is.  Allocates (device
- * memory; synchronous); RQX_ERR_LIMIT if it would need more than max_bytes
- * (max_bytes <= 0: 90% of the free device memory), counting two column arrays
- * of the padded layout that the build uses for scratch.  Rebuilding replaces a cache
- * of the same precision (both, for caches built together).
- * rqx_plan_update_coef rewrites the values on its stream, or releases the cache
- * if the coefficients change between real and complex.
+ * (RQX_F64 / RQX_C128), or 0 for both in one value pass over the hits with
+ * shared columns and level codes; values are real when the Hamiltonian is.
+ * Allocates (device memory; synchronous); RQX_ERR_LIMIT if it would need more
+ * than max_bytes (max_bytes <= 0: 90% of the free device memory), counting the
+ * scratch of the build (the columns of the padded layout, the groups of the
+ * first section's entries).  Rebuilding replaces a cache of the same precision
+ * (both, for caches built together).  rqx_plan_update_coef rewrites the values
+ * on its stream in place (no allocation), or releases the cache if the
+ * coefficients change between real and complex.
  */
 int rqx_plan_build_cache(rqx_plan *plan, int32_t precision, int64_t max_bytes, rqx_stream_t stream);
 
@@ -247,10 +250,11 @@ int rqx_plan_build_cache(rqx_plan *plan, int32_t precision, int64_t max_bytes, r
 int rqx_plan_drop_cache(rqx_plan *plan, int32_t precision);
 
 /*
- * out[0..8) = {built, stored hits (nnz), value entries incl. padding, device
- * bytes, slices, complex values, stored columns, first-section levels} for the
- * cache of one precision (for caches built together: the 64-bit cache counts
- * the shared arrays, the 32-bit its values).
+ * out[0..10) = {built, stored hits (nnz), stored values, device bytes, slices,
+ * complex values, stored columns, first-section levels, second-section values
+ * (incl. padding), first-section values of levels without one value for the
+ * whole slice} for the cache of one precision (for caches built together: the
+ * 64-bit cache counts the shared arrays, the 32-bit its values).
  */
 int rqx_plan_cache_info(const rqx_plan *plan, int32_t precision, int64_t *out);
 

@@ -54,7 +54,8 @@ int fail(int code, const std::string &msg) {
             return fail(RQX_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
-enum { MODE_MATVEC = 0, MODE_LOOKUP = 1, MODE_COUNT = 2, MODE_GRAD = 3, MODE_ROWCOUNT = 4, MODE_FILL = 5, MODE_COO = 6 };
+enum { MODE_MATVEC = 0, MODE_LOOKUP = 1, MODE_COUNT = 2, MODE_GRAD = 3, MODE_ROWCOUNT = 4, MODE_FILL = 5, MODE_COO = 6,
+       MODE_FILLCOL = 7 };
 
 // hit cache: sliced ELL, 64 rows per slice (one wave), two sections per slice.
 // Section A holds the hits of groups with no flips in the low split bits (the
@@ -95,8 +96,9 @@ template <typename T> __device__ inline void store_packet(void *a, int64_t off, 
     }
 }
 
-// fill pass: the last PACK section-B entries of a row (column, value and, when
-// both precisions are filled, FP32 value), stored once the packet is whole
+// column / value pass: the last PACK section-B entries of a row (column, or
+// value and, when both precisions are filled, FP32 value), stored once the
+// packet is whole
 template <typename FV, typename FL> struct FillBuf {
     int32_t c[PACK];
     FV v[PACK];
@@ -155,13 +157,29 @@ struct KParams {
     double *grad;                  // gradient mode: per-term output (accumulated)
     uint32_t *rowcnt;              // row-count mode: section-B hits per row
     uint32_t *rowcnt_a;            // row-count mode: section-A hits per row
-    int32_t *c_col;                // fill mode: cache column indices
-    void *c_val;                   // fill mode: cache values (R, or complex R)
-    void *c_val_lo;                // fill mode (R = double, optional): FP32 values of the same hits
-    const void *coef_lo;           // fill mode with c_val_lo: FP32 coefficients
-    const int64_t *c_off;          // fill mode: first entry of each slice
-    const uint32_t *c_wa;          // fill mode: section-A entries per row of each slice
-    const uint32_t *c_wb;          // fill mode: section-B entries per row of each slice
+    // hit cache passes.  Column pass: columns of the full packed layout (slice s
+    // from c_off[s], section A then B) and the group of each section-A entry
+    // (-1 on padding; level k of slice s at (c_aoff[s] + k) * 64 + lane).  Value
+    // pass: values straight into the stored layout (section B packed from
+    // c_boff[s]; section-A level k by its code c_acode[c_aoff[s] + k]: -1 for one
+    // value at c_aval[c_aoff[s] + k], else its index j among the slice's levels
+    // with 64 values, at c_vala[c_avoff[s] + 64 j + lane]).
+    int32_t *c_col;
+    int32_t *c_gid;
+    const int64_t *c_off;
+    const uint32_t *c_wa;          // section-A entries per row of each slice
+    const uint32_t *c_wb;          // section-B entries per row of each slice
+    const int64_t *c_aoff;
+    const int64_t *c_boff;
+    const int64_t *c_avoff;
+    const int32_t *c_acode;
+    void *c_val;                   // section-B values (R, or complex R)
+    void *c_aval;
+    void *c_vala;
+    void *c_val_lo;                // (R = double, optional) FP32 values of the same hits
+    void *c_aval_lo;
+    void *c_vala_lo;
+    const void *coef_lo;           // with c_val_lo: FP32 coefficients
     unsigned long long *coo_count; // coo mode: entries claimed so far
     int64_t *coo_row;              // coo mode: (row, column, element) of every hit, in claim order
     int32_t *coo_col;
@@ -622,23 +640,41 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
     using FV = typename std::conditional<CPLX, CC, R>::type;          // fill: stored value
     using FL = typename std::conditional<CPLX, float2, float>::type;  // fill: FP32 value of a dual fill
     FillBuf<FV, FL> fill_b;
-    int64_t fill_off = 0;         // the row's slice: first entry and section-A width
+    const int ln = static_cast<int>(i % SLICE);
+    // the row's slice: first full-layout entry (column pass), first section-B
+    // value and first stored section-A value (value pass), first section-A level
+    int64_t fill_off = 0, fill_bo = 0, fill_avo = 0, fill_ao = 0;
     uint32_t fill_wa = 0;
-    if constexpr (MODE == MODE_FILL) {
+    if constexpr (MODE == MODE_FILLCOL) {
         if (active) {
             fill_off = p.c_off[i / SLICE];
             fill_wa = p.c_wa[i / SLICE];
+            fill_ao = p.c_aoff[i / SLICE];
+        }
+    } else if constexpr (MODE == MODE_FILL) {
+        if (active) {
+            fill_bo = p.c_boff[i / SLICE];
+            fill_avo = p.c_avoff[i / SLICE];
+            fill_ao = p.c_aoff[i / SLICE];
         }
     }
-    // section A: entry k of this row stored at once (in product subspaces the
-    // lanes of a wave write the same entry index, so the stores coalesce)
-    auto fill_put = [&](uint32_t k, int32_t c, FV v, FL l) {
-        const int ln = static_cast<int>(i % SLICE);
-        if (p.c_col) p.c_col[packed_slot<int32_t>(fill_off, k, ln)] = c;   // NULL: values only
-        static_cast<FV *>(p.c_val)[packed_slot<FV>(fill_off, k, ln)] = v;
-        if (p.c_val_lo) static_cast<FL *>(p.c_val_lo)[packed_slot<FL>(fill_off, k, ln)] = l;
+    // section A, column pass: entry k of this row stored at once (in product
+    // subspaces the lanes of a wave write the same entry index, so the stores
+    // coalesce)
+    auto fill_put_col = [&](uint32_t k, int32_t c, int32_t g) {
+        p.c_col[packed_slot<int32_t>(fill_off, k, ln)] = c;
+        p.c_gid[(fill_ao + k) * SLICE + ln] = g;
+    };
+    // section A, value pass: the level's one value (every lane of a level with
+    // one value holds the same value) or the row's value of a stored level
+    auto fill_put = [&](uint32_t k, FV v, FL l) {
+        const int32_t code = p.c_acode[fill_ao + k];
+        const int64_t o = code < 0 ? fill_ao + k : fill_avo + static_cast<int64_t>(code) * SLICE + ln;
+        static_cast<FV *>(code < 0 ? p.c_aval : p.c_vala)[o] = v;
+        if (p.c_val_lo) static_cast<FL *>(code < 0 ? p.c_aval_lo : p.c_vala_lo)[o] = l;
     };
     // section B: append entry k of this row; a whole packet goes to memory
+    // (columns in the column pass, values in the value pass)
     auto fill_push = [&](FillBuf<FV, FL> &b, uint32_t k, int32_t c, FV v, FL l) {
 #pragma unroll
         for (int u = 0; u + 1 < PACK; ++u) {
@@ -650,12 +686,13 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
         b.v[PACK - 1] = v;
         b.l[PACK - 1] = l;
         if (k % PACK == PACK - 1) {
-            const int64_t o = fill_off;
-            const int ln = static_cast<int>(i % SLICE);
             const uint32_t kb = k - (PACK - 1);
-            if (p.c_col) store_packet<int32_t>(p.c_col, o, kb, ln, b.c);   // NULL: values only
-            store_packet<FV>(p.c_val, o, kb, ln, b.v);
-            if (p.c_val_lo) store_packet<FL>(p.c_val_lo, o, kb, ln, b.l);
+            if constexpr (MODE == MODE_FILLCOL) {
+                store_packet<int32_t>(p.c_col, fill_off, fill_wa + kb, ln, b.c);
+            } else {
+                store_packet<FV>(p.c_val, fill_bo, kb, ln, b.v);
+                if (p.c_val_lo) store_packet<FL>(p.c_val_lo, fill_bo, kb, ln, b.l);
+            }
         }
     };
     V gv = vzero(static_cast<V *>(nullptr));
@@ -739,7 +776,7 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
                 }
                 return h;
             };
-            auto consume = [&](uint32_t base, bool hit, bool sec_a, const HitVal &h) {
+            auto consume = [&](uint32_t g, uint32_t base, bool hit, bool sec_a, const HitVal &h) {
                 if constexpr (MODE == MODE_MATVEC) {
                     if constexpr (REALC) {
                         if (hit) fma_acc(sec_a ? acc_a : acc, h.vx, h.xv);
@@ -760,12 +797,11 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
                             v = p.c_val_lo ? static_cast<R>(h.vh.x) : h.vx;
                             l = h.vl.x;
                         }
-                        const int32_t c = static_cast<int32_t>(ta + base);
                         if (sec_a) {
-                            fill_put(static_cast<uint32_t>(n_hit_a), c, v, l);
+                            fill_put(static_cast<uint32_t>(n_hit_a), v, l);
                             n_hit_a += 1;
                         } else {
-                            fill_push(fill_b, fill_wa + static_cast<uint32_t>(n_hit), c, v, l);
+                            fill_push(fill_b, static_cast<uint32_t>(n_hit), 0, v, l);
                             n_hit += 1;
                         }
                     }
@@ -785,7 +821,18 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
                         if (hit) fma_acc(sec_a ? acc_a : acc, c, x[ta + base]);
                     }
                 } else if constexpr (MODE == MODE_FILL) {
-                    consume(base, hit, sec_a, value(g, base, hit));
+                    consume(g, base, hit, sec_a, value(g, base, hit));
+                } else if constexpr (MODE == MODE_FILLCOL) {
+                    if (hit) {
+                        const int32_t c = static_cast<int32_t>(ta + base);
+                        if (sec_a) {
+                            fill_put_col(static_cast<uint32_t>(n_hit_a), c, static_cast<int32_t>(g));
+                            n_hit_a += 1;
+                        } else {
+                            fill_push(fill_b, static_cast<uint32_t>(n_hit), c, FV{}, FL{});
+                            n_hit += 1;
+                        }
+                    }
                 } else if constexpr (MODE == MODE_LOOKUP) {
                     if (hit) {
                         const size_t o = static_cast<size_t>(g) * p.n_total + i;
@@ -847,10 +894,10 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
                     const bool h0 = active && f0, h1 = active && f1, h2 = active && f2, h3 = active && f3;
                     const HitVal v0 = value(g, b0, h0), v1 = value(g + 1, b1, h1);
                     const HitVal v2 = value(g + 2, b2, h2), v3 = value(g + 3, b3, h3);
-                    consume(b0, h0, false, v0);
-                    consume(b1, h1, false, v1);
-                    consume(b2, h2, false, v2);
-                    consume(b3, h3, false, v3);
+                    consume(g, b0, h0, false, v0);
+                    consume(g + 1, b1, h1, false, v1);
+                    consume(g + 2, b2, h2, false, v2);
+                    consume(g + 3, b3, h3, false, v3);
                 } else {
                     handle(g, b0, f0, false);
                     handle(g + 1, b1, f1, false);
@@ -874,15 +921,18 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
             p.rowcnt[i] = static_cast<uint32_t>(n_hit);
             p.rowcnt_a[i] = static_cast<uint32_t>(n_hit_a);
         }
-    } else if constexpr (MODE == MODE_FILL) {
+    } else if constexpr (MODE == MODE_FILL || MODE == MODE_FILLCOL) {
         if (active) {
-            const uint32_t wa = fill_wa, wb = p.c_wb[i / SLICE];
-            // pad to the slice's widths (multiples of PACK): the last packets go out
+            // pad to the slice's widths (multiples of PACK): the last packets go
+            // out; padding has the row's own column, group -1 and value 0
+            const uint32_t wa = p.c_wa[i / SLICE], wb = p.c_wb[i / SLICE];
             const FV v0 = vzero(static_cast<FV *>(nullptr));
             const FL l0 = vzero(static_cast<FL *>(nullptr));
-            for (uint32_t k = static_cast<uint32_t>(n_hit_a); k < wa; ++k)
-                fill_put(k, static_cast<int32_t>(i), v0, l0);
-            for (uint32_t k = wa + static_cast<uint32_t>(n_hit); k < wa + wb; ++k)
+            for (uint32_t k = static_cast<uint32_t>(n_hit_a); k < wa; ++k) {
+                if constexpr (MODE == MODE_FILLCOL) fill_put_col(k, static_cast<int32_t>(i), -1);
+                else fill_put(k, v0, l0);
+            }
+            for (uint32_t k = static_cast<uint32_t>(n_hit); k < wb; ++k)
                 fill_push(fill_b, k, static_cast<int32_t>(i), v0, l0);
         }
     } else if constexpr (MODE == MODE_COUNT) {
@@ -1599,26 +1649,29 @@ __device__ inline int64_t uniform64(int64_t v) {
 // section-A level whose columns are not row + (one offset for the whole slice)
 constexpr int32_t LEVEL_GATHER = INT32_MIN;
 
-// acc += the w (a multiple of PACK) section-A entries of a row, in order.
-// Level k of the slice is either affine (del[k] = d: every lane's source row is
-// its own row + d, so the column is computed and the gather is contiguous) or
-// stored (del[k] = LEVEL_GATHER: columns read from c, 64 lane-contiguous per
-// such level).  Values come in packets (vv: the row's first).  Returns the
-// number of stored levels.
+// acc += the w section-A entries of a row, in order.  Level k of the slice is
+// affine (del[k] = d: every lane's source row is its own row + d, so the
+// column is computed and the gather is contiguous) or stored (del[k] =
+// LEVEL_GATHER: columns read from c, 64 lane-contiguous per such level); and
+// has one value for all lanes (code[k] = -1: aval[k], a scalar load) or 64
+// (code[k] = j, the count of such levels before k: read from va + 64 j,
+// lane-contiguous).  Returns the number of stored-column levels.
 template <int U, typename V, typename CV>
-__device__ inline uint32_t spmv_section_a(V &acc, const int32_t *c, const CV *vv, const int32_t *__restrict__ del,
+__device__ inline uint32_t spmv_section_a(V &acc, const int32_t *c, const CV *va, const CV *__restrict__ aval,
+                                          const int32_t *__restrict__ del, const int32_t *__restrict__ code,
                                           uint32_t w, int32_t row, const V *__restrict__ x) {
-    static_assert(U % PACK == 0, "whole packets per batch");
-    constexpr int64_t STEP = SLICE * PACK;
     uint32_t k = 0, j = 0;
     for (; k + U <= w; k += U) {
-        int32_t d[U], ci[U];
+        int32_t d[U], cd[U], ci[U];
         CV cv[U];
         V xv[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) d[u] = del[k + u];
+        for (int u = 0; u < U; ++u) {
+            d[u] = del[k + u];
+            cd[u] = code[k + u];
+        }
 #pragma unroll
-        for (int pk = 0; pk < U / PACK; ++pk) load_packet(vv + pk * STEP, cv + pk * PACK);
+        for (int u = 0; u < U; ++u) cv[u] = cd[u] < 0 ? aval[k + u] : stream_load(va + static_cast<int64_t>(cd[u]) * SLICE);
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             if (d[u] == LEVEL_GATHER) {
@@ -1632,22 +1685,16 @@ __device__ inline uint32_t spmv_section_a(V &acc, const int32_t *c, const CV *vv
         for (int u = 0; u < U; ++u) xv[u] = x[ci[u]];
 #pragma unroll
         for (int u = 0; u < U; ++u) fma_acc(acc, cv[u], xv[u]);
-        vv += (U / PACK) * STEP;
     }
-    for (; k < w; k += PACK) {
-        CV cv[PACK];
-        load_packet(vv, cv);
-#pragma unroll
-        for (int u = 0; u < PACK; ++u) {
-            const int32_t d = del[k + u];
-            int32_t ci = row + d;
-            if (d == LEVEL_GATHER) {
-                ci = stream_load(c + j * SLICE);
-                ++j;
-            }
-            fma_acc(acc, cv[u], x[ci]);
+    for (; k < w; ++k) {
+        const int32_t d = del[k], cd = code[k];
+        const CV cv = cd < 0 ? aval[k] : stream_load(va + static_cast<int64_t>(cd) * SLICE);
+        int32_t ci = row + d;
+        if (d == LEVEL_GATHER) {
+            ci = stream_load(c + j * SLICE);
+            ++j;
         }
-        vv += STEP;
+        fma_acc(acc, cv, x[ci]);
     }
     return j;
 }
@@ -1656,13 +1703,16 @@ __device__ inline uint32_t spmv_section_a(V &acc, const int32_t *c, const CV *vv
 // entries read lane-contiguously (and marked non-temporal: they are read once
 // per matvec, x is the data worth keeping in L2); the same accumulation
 // sequence as k_tiles (diagonal and section B, section A, their sum), so
-// results are identical.  Values of slice s start at off[s] (section A, then
-// B), its stored columns at coff[s] (stored section-A levels, then B), its
-// section-A level codes at aoff[s].
+// results are identical.  Per slice s: section-A level codes (offsets, value
+// codes) and single values at aoff[s], the other section-A values at avoff[s],
+// stored columns at coff[s] (stored section-A levels, then B), section-B
+// values at boff[s].
 template <typename R, bool CPLXV, bool CPLXC>
-__global__ __launch_bounds__(256) void k_spmv_sell(const int32_t *__restrict__ col, const void *__restrict__ val,
-                                                   const int64_t *__restrict__ off, const int64_t *__restrict__ coff,
-                                                   const int64_t *__restrict__ aoff, const int32_t *__restrict__ adel,
+__global__ __launch_bounds__(256) void k_spmv_sell(const int32_t *__restrict__ col, const void *__restrict__ valb,
+                                                   const void *__restrict__ vala, const void *__restrict__ aval,
+                                                   const int64_t *__restrict__ boff, const int64_t *__restrict__ avoff,
+                                                   const int64_t *__restrict__ coff, const int64_t *__restrict__ aoff,
+                                                   const int32_t *__restrict__ adel, const int32_t *__restrict__ acode,
                                                    const uint32_t *__restrict__ wa, const uint32_t *__restrict__ wb,
                                                    const void *__restrict__ diag, int64_t n_states,
                                                    const void *__restrict__ xin, void *__restrict__ yout) {
@@ -1676,46 +1726,76 @@ __global__ __launch_bounds__(256) void k_spmv_sell(const int32_t *__restrict__ c
     const int64_t slice = row / SLICE;
     const int lane = static_cast<int>(row % SLICE);
     const V *x = static_cast<const V *>(xin);
-    const CV *v = static_cast<const CV *>(val);
     V acc = vzero(static_cast<V *>(nullptr));
     if (diag) fma_acc(acc, static_cast<const CC *>(diag)[row], x[row]);   // NULL: no problem has an x = 0 group
     const uint32_t w_a = __builtin_amdgcn_readfirstlane(wa[slice]);
     const uint32_t w_b = __builtin_amdgcn_readfirstlane(wb[slice]);
-    const int64_t o = uniform64(off[slice]), co = uniform64(coff[slice]), ao = uniform64(aoff[slice]);
+    const int64_t bo = uniform64(boff[slice]), avo = uniform64(avoff[slice]);
+    const int64_t co = uniform64(coff[slice]), ao = uniform64(aoff[slice]);
     V acc_a = vzero(static_cast<V *>(nullptr));
     constexpr int EV = pack_sub<CV>();
-    const uint32_t n_stored = spmv_section_a<U>(acc_a, col + co + lane, v + o + lane * EV, adel + ao, w_a,
+    const uint32_t n_stored = spmv_section_a<U>(acc_a, col + co + lane, static_cast<const CV *>(vala) + avo + lane,
+                                                static_cast<const CV *>(aval) + ao, adel + ao, acode + ao, w_a,
                                                 static_cast<int32_t>(row), x);
     spmv_section<U>(acc, col + co + static_cast<int64_t>(n_stored) * SLICE + lane * PACK,
-                    v + o + static_cast<int64_t>(w_a) * SLICE + lane * EV, w_b, x);
+                    static_cast<const CV *>(valb) + bo + lane * EV, w_b, x);
     static_cast<V *>(yout)[row] = vadd(acc, acc_a);
 }
 
-// Section-A level codes (one wave per slice) from the columns of the full
-// packed layout: del = d when every row r of the slice has column r + d at that
+// Section-A level codes (one wave per slice) from the column pass (columns in
+// the full packed layout, the group of each section-A entry level-major, -1 on
+// padding): del = d when every row r of the slice has column r + d at that
 // level (d = 0: a level of padding only, whose values are 0), else
-// LEVEL_GATHER; n_stored[s] = the slice's LEVEL_GATHER levels.
+// LEVEL_GATHER; code = -1 when every lane's value is the same by construction
+// (the same group, rows that agree on every bit of the group's Z masks, and a
+// group summed the same way for every row: a sign table, or too short for
+// term-parallel sums; or padding only), else the level's index among the
+// slice's levels that store 64 values.  n_stored / n_vals: the slice's
+// LEVEL_GATHER levels / levels with 64 values.
 __global__ __launch_bounds__(256) void k_cache_levels(const int32_t *__restrict__ col_full,
+                                                      const int32_t *__restrict__ gid_a,
                                                       const int64_t *__restrict__ off, const uint32_t *__restrict__ wa,
-                                                      const int64_t *__restrict__ aoff, int64_t n_states,
+                                                      const int64_t *__restrict__ aoff, const uint64_t *__restrict__ keys,
+                                                      const uint64_t *__restrict__ g_zor,
+                                                      const int32_t *__restrict__ g_tab,
+                                                      const uint32_t *__restrict__ g_toff, int64_t n_states,
                                                       int64_t n_slices, int32_t *__restrict__ adel,
-                                                      uint32_t *__restrict__ n_stored) {
+                                                      int32_t *__restrict__ acode, uint32_t *__restrict__ n_stored,
+                                                      uint32_t *__restrict__ n_vals) {
     const int64_t slice = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / SLICE;
     if (slice >= n_slices) return;
     const int lane = static_cast<int>(threadIdx.x % SLICE);
     const int64_t row = slice * SLICE + lane;
     const bool valid = row < n_states;
+    const uint64_t s_me = valid ? keys[row] : 0ull;
+    const uint64_t s_0 = __shfl(s_me, 0, 64);                 // lane 0 always holds a row
     const uint32_t w = wa[slice];
     const int64_t o = off[slice], ao = aoff[slice];
-    uint32_t stored = 0;
+    uint32_t stored = 0, vals = 0;
     for (uint32_t k = 0; k < w; ++k) {
-        const int32_t d = valid ? col_full[packed_slot<int32_t>(o, k, lane)] - static_cast<int32_t>(row) : 0;
-        const int32_t d0 = __shfl(d, 0, 64);                    // lane 0 always holds a row
+        const int64_t slot = packed_slot<int32_t>(o, k, lane);
+        const int32_t d = valid ? col_full[slot] - static_cast<int32_t>(row) : 0;
+        const int32_t d0 = __shfl(d, 0, 64);
         const bool affine = __ballot(valid && d != d0) == 0;
+        const int32_t g = valid ? gid_a[(ao + k) * SLICE + lane] : -1;
+        const int32_t g0 = __shfl(g, 0, 64);
+        bool uniform = __ballot(valid && g != g0) == 0;
+        if (uniform && g0 >= 0) {
+            const uint32_t len = g_toff[g0 + 1] - g_toff[g0];
+            uniform = (g_tab[g0] >= 0 || len < static_cast<uint32_t>(TP_MIN_TERMS)) &&
+                      __ballot(valid && ((s_me ^ s_0) & g_zor[g0]) != 0) == 0;
+        }
+        if (lane == 0) {
+            adel[ao + k] = affine ? d0 : LEVEL_GATHER;
+            acode[ao + k] = uniform ? -1 : static_cast<int32_t>(vals);
+        }
         stored += affine ? 0u : 1u;
-        if (lane == 0) adel[ao + k] = affine ? d0 : LEVEL_GATHER;
+        vals += uniform ? 0u : 1u;
     }
-    if (lane == 0) n_stored[slice] = stored;
+    if (lane == 0) {
+        n_stored[slice] = stored;
+        n_vals[slice] = vals;
+    }
 }
 
 // Stored columns (one wave per slice) from coff[s] on: the LEVEL_GATHER
@@ -1859,21 +1939,26 @@ struct ProbHost {            // host record of one problem of a plan
 };
 
 struct HitCache {            // sliced-ELL cache of the hits (column, C_j(s_i)) in one precision
-    void *block = nullptr;     // values and per-slice arrays
+    void *block = nullptr;     // per-slice arrays, level codes, section-B and single values
     void *col_block = nullptr; // stored columns
+    void *vala_block = nullptr;// section-A values of the levels with 64
     int32_t *col = nullptr;    // stored columns: gathered section-A levels and section B of each slice
-    void *val = nullptr;
-    int64_t *off = nullptr;    // first value of each slice
+    void *valb = nullptr;      // section-B values (packets)
+    void *vala = nullptr;      // section-A values of the levels with 64 (lane-contiguous per level)
+    void *aval = nullptr;      // the value of each section-A level with one value
+    int64_t *boff = nullptr;   // first section-B value of each slice
+    int64_t *avoff = nullptr;  // first vala value of each slice
     int64_t *coff = nullptr;   // first stored column of each slice
-    int64_t *aoff = nullptr;   // first section-A level code of each slice
+    int64_t *aoff = nullptr;   // first section-A level of each slice
     int32_t *adel = nullptr;   // section-A level codes: row offset d, or LEVEL_GATHER
+    int32_t *acode = nullptr;  // section-A level codes: -1 (one value, aval) or index among the vala levels
     uint32_t *wa = nullptr;    // section-A width of each slice
     uint32_t *wb = nullptr;    // section-B width of each slice
-    int64_t entries = 0, nnz = 0, n_slices = 0, bytes = 0;
-    int64_t col_entries = 0, a_levels = 0;
+    int64_t entries = 0, nnz = 0, n_slices = 0, bytes = 0;   // entries: stored values
+    int64_t full_entries = 0, col_entries = 0, a_levels = 0, valb_entries = 0, vala_entries = 0;
     int cplx = 0;
     bool built = false;
-    bool shared = false;       // both precisions built together: one block (owned by the FP64 cache), one column array
+    bool shared = false;       // both precisions built together: the FP64 cache owns the allocations
 };
 
 struct rqx_plan {
@@ -1921,6 +2006,7 @@ struct rqx_plan {
     void *st_block = nullptr;       // per-segment target lists (own allocation)
     uint32_t *st_off = nullptr;     // M + 1 offsets into st_list
     uint2 *st_list = nullptr;       // (class, target segment) pairs
+    uint64_t *g_zor = nullptr;      // per group: union of its terms' Z masks
     int32_t *g_tab = nullptr;       // per group: its sign table, or -1
     uint32_t n_tab = 0;             // sign tables (entry u of table t at u * n_tab + t)
     double *tab64re = nullptr;      // sign tables, FP64 and FP32, real and imaginary parts
@@ -2289,6 +2375,7 @@ int build_plan(rqx_plan **out, const std::vector<ProblemIn> &in, int64_t n_total
     std::vector<int64_t> prob_rows(static_cast<size_t>(K) + 1, 0);
     std::vector<double> tile_cost;
     std::vector<int32_t> g_tab;    // per group: first sign-table entry, or -1
+    std::vector<uint64_t> g_zor;   // per group: union of its terms' Z masks
     int64_t n_tab = 0;
     int64_t row_base = 0, term_base = 0;
     int32_t grp_base = 0;
@@ -2344,6 +2431,9 @@ int build_plan(rqx_plan **out, const std::vector<ProblemIn> &in, int64_t n_total
             g_xlo.push_back(static_cast<uint32_t>(q.xmask[j]) & lomask);
             g_x.push_back(q.xmask[j]);
             g_toff.push_back(static_cast<uint32_t>(term_base + q.grp_off[j]));
+            uint64_t zor = 0;
+            for (int64_t t = q.grp_off[j]; t < q.grp_off[j + 1]; ++t) zor |= q.zmask[t];
+            g_zor.push_back(zor);
             if (has_sign_table(q, j) && n_tab < (int64_t(1) << 27)) {   // entries indexed in 32 bits
                 p->long_groups = 1;         // the full evaluator reads the tables
                 g_tab.push_back(static_cast<int32_t>(n_tab));
@@ -2431,7 +2521,7 @@ int build_plan(rqx_plan **out, const std::vector<ProblemIn> &in, int64_t n_total
                  o_d32 = carve(n * 8), o_trow = carve(tile_row0.size() * 4), o_tseg = carve(tile_seg.size() * 4),
                  o_tprob = carve(tile_prob.size() * 4), o_desc = carve(desc.size() * sizeof(ProbDesc)),
                  o_prows = carve(prob_rows.size() * 8), o_queue = carve(sizeof(unsigned int)),
-                 o_gtab = carve(g_tab.size() * 4), o_t64re = carve(static_cast<size_t>(n_tab) * TAB_ENTRIES * 8),
+                 o_gtab = carve(g_tab.size() * 4), o_gzor = carve(g_zor.size() * 8), o_t64re = carve(static_cast<size_t>(n_tab) * TAB_ENTRIES * 8),
                  o_t64im = carve(static_cast<size_t>(n_tab) * TAB_ENTRIES * 8),
                  o_t32re = carve(static_cast<size_t>(n_tab) * TAB_ENTRIES * 4),
                  o_t32im = carve(static_cast<size_t>(n_tab) * TAB_ENTRIES * 4), o_hoff = carve(h_off.size() * 4),
@@ -2459,6 +2549,7 @@ int build_plan(rqx_plan **out, const std::vector<ProblemIn> &in, int64_t n_total
     p->prob_rows = reinterpret_cast<int64_t *>(b + o_prows);
     p->queue = reinterpret_cast<unsigned int *>(b + o_queue);
     p->g_tab = reinterpret_cast<int32_t *>(b + o_gtab);
+    p->g_zor = reinterpret_cast<uint64_t *>(b + o_gzor);
     p->tab64re = reinterpret_cast<double *>(b + o_t64re);
     p->tab64im = reinterpret_cast<double *>(b + o_t64im);
     p->tab32re = reinterpret_cast<float *>(b + o_t32re);
@@ -2486,6 +2577,7 @@ int build_plan(rqx_plan **out, const std::vector<ProblemIn> &in, int64_t n_total
         up(p->tile_row0, tile_row0.data(), tile_row0.size() * 4), up(p->tile_seg, tile_seg.data(), tile_seg.size() * 4),
         up(p->tile_prob, tile_prob.data(), tile_prob.size() * 4), up(p->prob, desc.data(), desc.size() * sizeof(ProbDesc)),
         up(p->prob_rows, prob_rows.data(), prob_rows.size() * 8), up(p->g_tab, g_tab.data(), g_tab.size() * 4),
+        up(p->g_zor, g_zor.data(), g_zor.size() * 8),
         up(p->h_off, h_off.data(), h_off.size() * 4), up(p->h_log, h_log.data(), h_log.size() * 4),
         up(p->h_tab, h_tab.data(), h_tab.size() * 4)};
     for (hipError_t ei : errs)
@@ -2534,9 +2626,9 @@ size_t cache_val_bytes(int slot, int cplx) { return (slot == 1 ? 8u : 4u) * (cpl
 
 hipError_t free_cache(HitCache &c) {
     hipError_t e = hipSuccess;
-    if (c.block) e = hipFree(c.block);
-    if (c.col_block) {
-        const hipError_t e2 = hipFree(c.col_block);
+    for (void *b : {c.block, c.col_block, c.vala_block}) {
+        if (!b) continue;
+        const hipError_t e2 = hipFree(b);
         if (e == hipSuccess) e = e2;
     }
     c = HitCache{};
@@ -2551,20 +2643,42 @@ hipError_t release_cache(rqx_plan *p, int slot) {
     return e;
 }
 
-// (re)write the values of a built cache from the current coefficients (both
-// precisions in one pass when they were built together); col_full: also write
-// every column into the full layout (values' indexing), at build time
-hipError_t fill_cache(const rqx_plan *p, int slot, hipStream_t st, int32_t *col_full = nullptr) {
+// the column pass: columns in the full packed layout (slice s from off[s]) and
+// the group of each section-A entry (level-major), in the order the tile kernel
+// accumulates the hits
+hipError_t fill_columns(const rqx_plan *p, const HitCache &c, const int64_t *off, int32_t *col_full, int32_t *gid_a,
+                        hipStream_t st) {
+    KParams k = make_params(p, false);
+    k.c_col = col_full;
+    k.c_gid = gid_a;
+    k.c_off = off;
+    k.c_wa = c.wa;
+    k.c_wb = c.wb;
+    k.c_aoff = c.aoff;
+    return launch_tiles<MODE_FILLCOL, float, false>(p, k, nullptr, nullptr, st);
+}
+
+// the value pass: the values of the cache in slot (both precisions in one pass
+// when they were built together) from the current coefficients, straight into
+// the stored layout; it needs no memory beyond the cache
+hipError_t fill_values(const rqx_plan *p, int slot, hipStream_t st) {
     if (p->cache[slot].shared) slot = 1;
     const HitCache &c = p->cache[slot];
     KParams k = make_params(p, slot == 1);
-    k.c_col = col_full;
-    k.c_val = c.val;
-    k.c_off = c.off;
     k.c_wa = c.wa;
     k.c_wb = c.wb;
+    k.c_aoff = c.aoff;
+    k.c_boff = c.boff;
+    k.c_avoff = c.avoff;
+    k.c_acode = c.acode;
+    k.c_val = c.valb;
+    k.c_aval = c.aval;
+    k.c_vala = c.vala;
     if (c.shared) {
-        k.c_val_lo = p->cache[0].val;
+        const HitCache &lo = p->cache[0];
+        k.c_val_lo = lo.valb;
+        k.c_aval_lo = lo.aval;
+        k.c_vala_lo = lo.vala;
         k.coef_lo = p->coef32;
     }
     if (slot == 1)
@@ -2578,11 +2692,12 @@ template <typename R, bool CPLXV>
 hipError_t spmv_launch(const rqx_plan *p, const HitCache &c, const void *diag, const void *x, void *y, hipStream_t st) {
     const unsigned blocks = static_cast<unsigned>((p->n_states + 255) / 256);
     if (c.cplx)
-        hipLaunchKernelGGL((k_spmv_sell<R, CPLXV, true>), dim3(blocks), dim3(256), 0, st, c.col, c.val, c.off, c.coff,
-                           c.aoff, c.adel, c.wa, c.wb, diag, p->n_states, x, y);
+        hipLaunchKernelGGL((k_spmv_sell<R, CPLXV, true>), dim3(blocks), dim3(256), 0, st, c.col, c.valb, c.vala, c.aval,
+                           c.boff, c.avoff, c.coff, c.aoff, c.adel, c.acode, c.wa, c.wb, diag, p->n_states, x, y);
     else
-        hipLaunchKernelGGL((k_spmv_sell<R, CPLXV, false>), dim3(blocks), dim3(256), 0, st, c.col, c.val, c.off, c.coff,
-                           c.aoff, c.adel, c.wa, c.wb, diag, p->n_states, x, y);
+        hipLaunchKernelGGL((k_spmv_sell<R, CPLXV, false>), dim3(blocks), dim3(256), 0, st, c.col, c.valb, c.vala,
+                           c.aval, c.boff, c.avoff, c.coff, c.aoff, c.adel, c.acode, c.wa, c.wb, diag, p->n_states, x,
+                           y);
     return hipGetLastError();
 }
 
@@ -2852,7 +2967,7 @@ int rqx_plan_update_coef(rqx_plan *p, const double *coef, int32_t is_real, rqx_s
             HIP_TRY(hipStreamSynchronize(st));
             HIP_TRY(release_cache(p, slot));
         } else if (!(c.shared && slot == 0)) {      // a shared pair is refilled in one pass
-            HIP_TRY(fill_cache(p, slot, st));
+            HIP_TRY(fill_values(p, slot, st));
         }
     }
     return RQX_OK;
@@ -2887,10 +3002,11 @@ int rqx_plan_build_cache(rqx_plan *p, int32_t precision, int64_t max_bytes, rqx_
     if (e == hipSuccess) e = ef;
     if (e != hipSuccess) return fail(RQX_ERR_HIP, std::string("cache row count: ") + hipGetErrorString(e));
 
-    // per-slice section widths and offsets
+    // per-slice section widths and offsets of the fill's full layout
     std::vector<uint32_t> wa(static_cast<size_t>(n_slices), 0u), wb(static_cast<size_t>(n_slices), 0u);
-    std::vector<int64_t> off(static_cast<size_t>(n_slices), 0), aoff(static_cast<size_t>(n_slices), 0);
-    int64_t entries = 0, nnz = 0, a_levels = 0;
+    std::vector<int64_t> off(static_cast<size_t>(n_slices), 0), aoff(static_cast<size_t>(n_slices), 0),
+        boff(static_cast<size_t>(n_slices), 0);
+    int64_t full = 0, nnz = 0, a_levels = 0, b_entries = 0;
     for (int64_t s = 0; s < n_slices; ++s) {
         uint32_t w_a = 0, w_b = 0;
         for (int64_t r = s * SLICE; r < std::min(n, (s + 1) * SLICE); ++r) {
@@ -2903,29 +3019,36 @@ int rqx_plan_build_cache(rqx_plan *p, int32_t precision, int64_t max_bytes, rqx_
         w_b = (w_b + PACK - 1) / PACK * PACK;
         wa[static_cast<size_t>(s)] = w_a;
         wb[static_cast<size_t>(s)] = w_b;
-        off[static_cast<size_t>(s)] = entries;
+        off[static_cast<size_t>(s)] = full;
         aoff[static_cast<size_t>(s)] = a_levels;
-        entries += static_cast<int64_t>(w_a + w_b) * SLICE;
+        boff[static_cast<size_t>(s)] = b_entries;
+        full += static_cast<int64_t>(w_a + w_b) * SLICE;
         a_levels += w_a;
+        b_entries += static_cast<int64_t>(w_b) * SLICE;
     }
     const int cplx = p->is_real ? 0 : 1;
     const size_t vb = cache_val_bytes(slot, cplx);
     const size_t vb_lo = both ? cache_val_bytes(0, cplx) : 0;
-    const size_t o_val = 0, o_val_lo = align_up(static_cast<size_t>(entries) * vb),
-                 o_off = o_val_lo + (both ? align_up(static_cast<size_t>(entries) * vb_lo) : 0),
-                 o_coff = o_off + align_up(static_cast<size_t>(n_slices) * 8),
-                 o_aoff = o_coff + align_up(static_cast<size_t>(n_slices) * 8),
-                 o_wa = o_aoff + align_up(static_cast<size_t>(n_slices) * 8),
-                 o_wb = o_wa + align_up(static_cast<size_t>(n_slices) * 4),
-                 o_adel = o_wb + align_up(static_cast<size_t>(n_slices) * 4),
-                 total = o_adel + align_up(static_cast<size_t>(a_levels) * 4);
-    // the columns of the full layout are a build-time scratch array; the stored
-    // columns are at most as many
-    const size_t col_full_bytes = align_up(static_cast<size_t>(entries) * 4);
+    const size_t ns8 = align_up(static_cast<size_t>(n_slices) * 8), ns4 = align_up(static_cast<size_t>(n_slices) * 4);
+    const size_t o_valb = 0, o_valb_lo = align_up(static_cast<size_t>(b_entries) * vb),
+                 o_aval = o_valb_lo + (both ? align_up(static_cast<size_t>(b_entries) * vb_lo) : 0),
+                 o_aval_lo = o_aval + align_up(static_cast<size_t>(a_levels) * vb),
+                 o_boff = o_aval_lo + (both ? align_up(static_cast<size_t>(a_levels) * vb_lo) : 0),
+                 o_avoff = o_boff + ns8, o_coff = o_avoff + ns8, o_aoff = o_coff + ns8,
+                 o_wa = o_aoff + ns8, o_wb = o_wa + ns4, o_adel = o_wb + ns4,
+                 o_acode = o_adel + align_up(static_cast<size_t>(a_levels) * 4),
+                 total = o_acode + align_up(static_cast<size_t>(a_levels) * 4);
+    // scratch of the column pass (columns of the full layout, section-A groups,
+    // slice offsets, per-slice counts); the stored columns and section-A values
+    // allocated later are at most as many as the full layout holds
+    const size_t nf = static_cast<size_t>(std::max<int64_t>(full, 1));
+    const size_t na = static_cast<size_t>(std::max<int64_t>(a_levels, 1)) * SLICE;
+    const size_t scratch = align_up(nf * 4) + align_up(na * 4) + ns8 * 2;
+    const size_t stored_max = align_up(nf * 4) + align_up(na * (vb + vb_lo));
     size_t free_b = 0, total_b = 0;
     HIP_TRY(hipMemGetInfo(&free_b, &total_b));
     const int64_t limit = max_bytes > 0 ? max_bytes : static_cast<int64_t>(free_b / 10 * 9);
-    const int64_t need = static_cast<int64_t>(total + 2 * col_full_bytes);
+    const int64_t need = static_cast<int64_t>(total + scratch + stored_max);
     if (need > limit)
         return fail(RQX_ERR_LIMIT, "hit cache needs " + std::to_string(need) + " bytes (while building), limit " +
                                        std::to_string(limit));
@@ -2933,85 +3056,108 @@ int rqx_plan_build_cache(rqx_plan *p, int32_t precision, int64_t max_bytes, rqx_
     HitCache c;
     HIP_TRY(hipMalloc(&c.block, total));
     char *b = static_cast<char *>(c.block);
-    c.val = b + o_val;
-    c.off = reinterpret_cast<int64_t *>(b + o_off);
+    c.valb = b + o_valb;
+    c.aval = b + o_aval;
+    c.boff = reinterpret_cast<int64_t *>(b + o_boff);
+    c.avoff = reinterpret_cast<int64_t *>(b + o_avoff);
     c.coff = reinterpret_cast<int64_t *>(b + o_coff);
     c.aoff = reinterpret_cast<int64_t *>(b + o_aoff);
     c.wa = reinterpret_cast<uint32_t *>(b + o_wa);
     c.wb = reinterpret_cast<uint32_t *>(b + o_wb);
     c.adel = reinterpret_cast<int32_t *>(b + o_adel);
-    c.entries = entries;
+    c.acode = reinterpret_cast<int32_t *>(b + o_acode);
+    c.full_entries = full;
     c.nnz = nnz;
     c.n_slices = n_slices;
     c.a_levels = a_levels;
+    c.valb_entries = b_entries;
     c.cplx = cplx;
     c.shared = both;
-    int32_t *col_full = nullptr;
-    uint32_t *n_stored = nullptr;
+    int32_t *col_full = nullptr, *gid_a = nullptr;
+    int64_t *off_d = nullptr;
+    uint32_t *counts = nullptr;
+    auto release_scratch = [&]() {
+        for (void *q : {static_cast<void *>(col_full), static_cast<void *>(gid_a), static_cast<void *>(off_d),
+                        static_cast<void *>(counts)})
+            (void)hipFree(q);
+    };
     auto fail_build = [&](hipError_t err, const char *what) {
-        (void)hipFree(col_full);
-        (void)hipFree(n_stored);
+        (void)hipStreamSynchronize(st);
+        release_scratch();
+        p->cache[slot] = HitCache{};
+        if (both) p->cache[0] = HitCache{};
         (void)free_cache(c);
         return fail(RQX_ERR_HIP, std::string(what) + hipGetErrorString(err));
     };
-    e = hipMalloc(&col_full, col_full_bytes);
-    if (e == hipSuccess) e = hipMalloc(&n_stored, static_cast<size_t>(n_slices) * 4);
-    if (e == hipSuccess) e = hipMemcpyAsync(c.off, off.data(), off.size() * 8, hipMemcpyHostToDevice, st);
+    e = hipMalloc(&col_full, nf * 4);
+    if (e == hipSuccess) e = hipMalloc(&gid_a, na * 4);
+    if (e == hipSuccess) e = hipMalloc(&off_d, ns8);
+    if (e == hipSuccess) e = hipMalloc(&counts, ns8);
+    if (e == hipSuccess) e = hipMemcpyAsync(off_d, off.data(), off.size() * 8, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(c.boff, boff.data(), boff.size() * 8, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipMemcpyAsync(c.aoff, aoff.data(), aoff.size() * 8, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipMemcpyAsync(c.wa, wa.data(), wa.size() * 4, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipMemcpyAsync(c.wb, wb.data(), wb.size() * 4, hipMemcpyHostToDevice, st);
     if (e != hipSuccess) return fail_build(e, "cache allocation: ");
-    p->cache[slot] = c;
-    if (both) {                 // the FP32 cache: its own values, everything else shared
-        HitCache lo = c;
-        lo.block = nullptr;
-        lo.val = b + o_val_lo;
-        p->cache[0] = lo;
-    }
-    // pass 2: values (and every column, full layout) in the order the tile kernel accumulates them
-    e = fill_cache(p, slot, st, col_full);
-    p->cache[slot] = HitCache{};
-    if (both) p->cache[0] = HitCache{};
-    if (e != hipSuccess) return fail_build(e, "cache fill: ");
-    // section-A level codes, then the stored columns
+    // pass 2: columns of every hit and groups of the section-A hits
+    e = fill_columns(p, c, off_d, col_full, gid_a, st);
+    if (e != hipSuccess) return fail_build(e, "cache columns: ");
+    // section-A level codes: offsets and single values
     const unsigned wave_blocks = static_cast<unsigned>((n_slices * SLICE + 255) / 256);
-    hipLaunchKernelGGL(k_cache_levels, dim3(wave_blocks), dim3(256), 0, st, col_full, c.off, c.wa, c.aoff, n, n_slices,
-                       c.adel, n_stored);
+    hipLaunchKernelGGL(k_cache_levels, dim3(wave_blocks), dim3(256), 0, st, col_full, gid_a, off_d, c.wa, c.aoff,
+                       p->keys, p->g_zor, p->g_tab, p->g_toff, n, n_slices, c.adel, c.acode, counts, counts + n_slices);
     e = hipGetLastError();
-    std::vector<uint32_t> stored(static_cast<size_t>(n_slices));
-    if (e == hipSuccess) e = hipMemcpyAsync(stored.data(), n_stored, stored.size() * 4, hipMemcpyDeviceToHost, st);
+    std::vector<uint32_t> hc(static_cast<size_t>(2 * n_slices));
+    if (e == hipSuccess) e = hipMemcpyAsync(hc.data(), counts, hc.size() * 4, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) return fail_build(e, "cache levels: ");
-    std::vector<int64_t> coff(static_cast<size_t>(n_slices), 0);
-    int64_t col_entries = 0;
+    std::vector<int64_t> coff(static_cast<size_t>(n_slices), 0), avoff(static_cast<size_t>(n_slices), 0);
+    int64_t col_entries = 0, a_entries = 0;
     for (int64_t s = 0; s < n_slices; ++s) {
         coff[static_cast<size_t>(s)] = col_entries;
-        col_entries += static_cast<int64_t>(stored[static_cast<size_t>(s)] + wb[static_cast<size_t>(s)]) * SLICE;
+        avoff[static_cast<size_t>(s)] = a_entries;
+        col_entries += static_cast<int64_t>(hc[static_cast<size_t>(s)] + wb[static_cast<size_t>(s)]) * SLICE;
+        a_entries += static_cast<int64_t>(hc[static_cast<size_t>(n_slices + s)]) * SLICE;
     }
-    e = hipMalloc(&c.col_block, align_up(static_cast<size_t>(col_entries) * 4));
+    const size_t o_vala_lo = align_up(static_cast<size_t>(std::max<int64_t>(a_entries, 1)) * vb);
+    e = hipMalloc(&c.col_block, align_up(static_cast<size_t>(std::max<int64_t>(col_entries, 1)) * 4));
+    if (e == hipSuccess)
+        e = hipMalloc(&c.vala_block, o_vala_lo + (both ? static_cast<size_t>(std::max<int64_t>(a_entries, 1)) * vb_lo : 0));
+    if (e != hipSuccess) return fail_build(e, "cache allocation: ");
     c.col = static_cast<int32_t *>(c.col_block);
-    if (e == hipSuccess) e = hipMemcpyAsync(c.coff, coff.data(), coff.size() * 8, hipMemcpyHostToDevice, st);
+    c.vala = c.vala_block;
+    c.col_entries = col_entries;
+    c.vala_entries = a_entries;
+    c.entries = b_entries + a_entries + a_levels;
+    c.bytes = static_cast<int64_t>(total + align_up(static_cast<size_t>(col_entries) * 4) + o_vala_lo +
+                                   (both ? static_cast<size_t>(a_entries) * vb_lo : 0));
+    HitCache lo;                // the FP32 cache of a pair: its own values, everything else shared
+    if (both) {
+        lo = c;
+        lo.block = nullptr;
+        lo.col_block = nullptr;
+        lo.vala_block = nullptr;
+        lo.valb = b + o_valb_lo;
+        lo.aval = b + o_aval_lo;
+        lo.vala = static_cast<char *>(c.vala_block) + o_vala_lo;
+        lo.bytes = (b_entries + a_entries + a_levels) * static_cast<int64_t>(vb_lo);
+    }
+    p->cache[slot] = c;
+    if (both) p->cache[0] = lo;
+    e = hipMemcpyAsync(c.coff, coff.data(), coff.size() * 8, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(c.avoff, avoff.data(), avoff.size() * 8, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) {
-        hipLaunchKernelGGL(k_cache_columns, dim3(wave_blocks), dim3(256), 0, st, col_full, c.off, c.wa, c.wb, c.aoff,
+        hipLaunchKernelGGL(k_cache_columns, dim3(wave_blocks), dim3(256), 0, st, col_full, off_d, c.wa, c.wb, c.aoff,
                            c.adel, c.coff, n, n_slices, c.col);
         e = hipGetLastError();
     }
+    // pass 3: the values, straight into the stored layout
+    if (e == hipSuccess) e = fill_values(p, slot, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);   // host vectors and the scratch arrays are released below
-    if (e != hipSuccess) return fail_build(e, "cache columns: ");
-    (void)hipFree(col_full);
-    (void)hipFree(n_stored);
-    c.col_entries = col_entries;
-    c.bytes = static_cast<int64_t>(total + align_up(static_cast<size_t>(col_entries) * 4));
-    c.built = true;
-    p->cache[slot] = c;
-    if (both) {
-        HitCache lo = c;
-        lo.block = nullptr;
-        lo.col_block = nullptr;
-        lo.val = b + o_val_lo;
-        lo.bytes = static_cast<int64_t>(entries) * static_cast<int64_t>(vb_lo);
-        p->cache[0] = lo;
-    }
+    if (e != hipSuccess) return fail_build(e, "cache columns and values: ");
+    release_scratch();
+    p->cache[slot].built = true;
+    if (both) p->cache[0].built = true;
     return RQX_OK;
 }
 
@@ -3039,6 +3185,8 @@ int rqx_plan_cache_info(const rqx_plan *p, int32_t precision, int64_t *out) {
     out[5] = c.cplx;
     out[6] = c.col_entries;
     out[7] = c.a_levels;
+    out[8] = c.valb_entries;
+    out[9] = c.vala_entries;
     return RQX_OK;
 }
 

@@ -277,11 +277,13 @@ class Plan:
         _capi.check(_capi.lib().rqx_plan_drop_cache(self._handle, int(precision)))
 
     def cache_info(self, precision=64):
-        """``built``, stored hits, value entries incl. padding, device bytes, slices,
-        complex values, stored columns, first-section levels."""
-        out = np.zeros(8, dtype=np.int64)
+        """``built``, stored hits, stored values, device bytes, slices, complex values,
+        stored columns, first-section levels, second-section values (incl. padding)
+        and first-section values of the levels without a single value."""
+        out = np.zeros(10, dtype=np.int64)
         _capi.check(_capi.lib().rqx_plan_cache_info(self._handle, int(precision), _capi.ptr(out)))
-        keys = ('built', 'nnz', 'entries', 'bytes', 'slices', 'complex', 'col_entries', 'a_levels')
+        keys = ('built', 'nnz', 'entries', 'bytes', 'slices', 'complex', 'col_entries', 'a_levels',
+                'b_entries', 'a_entries')
         info = {k: int(v) for k, v in zip(keys, out)}
         info['built'] = bool(info['built'])
         info['complex'] = bool(info['complex'])
@@ -294,12 +296,12 @@ class Plan:
     def cached_bytes(self, dtype):
         """Bytes one cached matvec must move: the cache, the diagonal, ``x`` once and ``y``.
 
-        ``entries s_val + 4 col_entries + 4 a_levels + 40 slices + N (s_d + 2 s_v)``
+        ``entries s_val + 4 col_entries + 8 a_levels + 40 slices + N (s_d + 2 s_v)``
         with ``s_val`` the stored value bytes, ``col_entries`` the stored
-        columns, ``a_levels`` the first-section level codes, 40 bytes of offsets
-        and widths per slice, ``s_d`` the diagonal entry bytes (complex) and
-        ``s_v`` the vector element bytes; gathers of ``x`` past its first read
-        are counted as cache hits.
+        columns, ``a_levels`` the first-section level codes (offset and
+        single-value flag), 40 bytes of offsets and widths per slice, ``s_d`` the
+        diagonal entry bytes (complex) and ``s_v`` the vector element bytes;
+        gathers of ``x`` past its first read are counted as cache hits.
         """
         precision = 64 if dtype in (torch.float64, torch.complex128) else 32
         info = self.cache_info(precision)
@@ -308,7 +310,7 @@ class Plan:
         s_v = torch.empty(0, dtype=dtype).element_size()
         s_val = (8 if precision == 64 else 4) * (2 if info['complex'] else 1)
         s_d = 16 if precision == 64 else 8
-        return (info['entries'] * s_val + 4 * info['col_entries'] + 4 * info['a_levels'] + 40 * info['slices'] +
+        return (info['entries'] * s_val + 4 * info['col_entries'] + 8 * info['a_levels'] + 40 * info['slices'] +
                 self.n_states * (s_d + 2 * s_v))
 
     # ------------------------------------------------------------------ per-problem vector ops

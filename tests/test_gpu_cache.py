@@ -76,8 +76,8 @@ def test_cached_matvec_identical(cases, name):
         info = plan.build_cache(precision)
         assert info['built'] and info['complex'] == (not plan.is_real)
         assert info['nnz'] == hits - (plan.n_states if plan.info['has_diagonal'] else 0)
-        assert info['entries'] >= info['nnz'] and info['entries'] % 64 == 0
-        assert info['col_entries'] % 64 == 0 and info['col_entries'] <= info['entries']
+        assert info['b_entries'] % 64 == 0 and info['a_entries'] % 64 == 0 and info['col_entries'] % 64 == 0
+        assert info['entries'] == info['b_entries'] + info['a_entries'] + info['a_levels']
     _check_identical(plan)
     # the cached values are the oracle's matrix elements
     x = _vec(plan.n_total, torch.complex128, 3)
@@ -99,9 +99,10 @@ def test_product_subspace_levels_store_offsets():
     w = workloads.jw32(n_alpha=300, n_beta=200)
     plan = Plan(w.tables, w.states)
     info = plan.build_cache(32)
-    stored_a = info['col_entries'] // 64 - (info['entries'] // 64 - info['a_levels'])
-    print(f"first-section levels {info['a_levels']}, stored {stored_a}", flush=True)
-    assert info['a_levels'] > 0 and stored_a < info['a_levels']
+    stored_a = (info['col_entries'] - info['b_entries']) // 64       # levels that store their columns
+    valued_a = info['a_entries'] // 64                                # levels that store 64 values
+    print(f"first-section levels {info['a_levels']}, stored columns {stored_a}, stored values {valued_a}", flush=True)
+    assert info['a_levels'] > 0 and stored_a < info['a_levels'] and valued_a < info['a_levels']
     _check_identical(plan, seed=19)
 
 
