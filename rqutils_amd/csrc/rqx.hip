@@ -756,9 +756,9 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
             const uint32_t g1 = p.cls_goff[cls + 1];
             if (pd.has_diag && g0 == pd.grp_base) g0 += 1;
             // a hit's value (and, for the matvec, its x entry) first, then its
-            // use: the full-evaluator matvec takes four groups' values before
-            // using them, so their loads are in flight together; values are
-            // used in group order
+            // use: the full-evaluator matvec and value pass take four groups'
+            // values before using them, so their loads are in flight together;
+            // values are used in group order
             struct HitVal {
                 R vx, vy;
                 double2 vh;
@@ -890,7 +890,7 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
                     f2 = s_keys[b2] == t2;
                     f3 = s_keys[b3] == t3;
                 }
-                if constexpr (MODE == MODE_MATVEC && TP) {     // (measured: slower for single-term groups and the fill)
+                if constexpr ((MODE == MODE_MATVEC || MODE == MODE_FILL) && TP) {   // (measured: slower for single-term groups)
                     const bool h0 = active && f0, h1 = active && f1, h2 = active && f2, h3 = active && f3;
                     const HitVal v0 = value(g, b0, h0), v1 = value(g + 1, b1, h1);
                     const HitVal v2 = value(g + 2, b2, h2), v3 = value(g + 3, b3, h3);
@@ -1740,6 +1740,30 @@ __global__ __launch_bounds__(256) void k_spmv_sell(const int32_t *__restrict__ c
     spmv_section<U>(acc, col + co + static_cast<int64_t>(n_stored) * SLICE + lane * PACK,
                     static_cast<const CV *>(valb) + bo + lane * EV, w_b, x);
     static_cast<V *>(yout)[row] = vadd(acc, acc_a);
+}
+
+// Section widths of each slice (one wave per slice): the largest row count of
+// each section over the slice's rows, rounded up to whole packets; the hits of
+// all rows are summed into *nnz (one atomic per wave).
+__global__ __launch_bounds__(256) void k_slice_widths(const uint32_t *__restrict__ cnt_b,
+                                                      const uint32_t *__restrict__ cnt_a, int64_t n_states,
+                                                      int64_t n_slices, uint32_t *__restrict__ wa,
+                                                      uint32_t *__restrict__ wb, unsigned long long *nnz) {
+    const int64_t slice = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / SLICE;
+    if (slice >= n_slices) return;
+    const int64_t row = slice * SLICE + threadIdx.x % SLICE;
+    uint32_t b = row < n_states ? cnt_b[row] : 0u, a = row < n_states ? cnt_a[row] : 0u;
+    unsigned long long sum = static_cast<unsigned long long>(a) + b;
+    for (int off = 32; off > 0; off >>= 1) {
+        b = max(b, static_cast<uint32_t>(__shfl_xor(static_cast<int>(b), off, 64)));
+        a = max(a, static_cast<uint32_t>(__shfl_xor(static_cast<int>(a), off, 64)));
+        sum += __shfl_xor(sum, off, 64);
+    }
+    if (threadIdx.x % SLICE == 0) {
+        wa[slice] = (a + PACK - 1) / PACK * PACK;
+        wb[slice] = (b + PACK - 1) / PACK * PACK;
+        atomicAdd(nnz, sum);
+    }
 }
 
 // Section-A level codes (one wave per slice) from the column pass (columns in
@@ -2988,37 +3012,39 @@ int rqx_plan_build_cache(rqx_plan *p, int32_t precision, int64_t max_bytes, rqx_
     if (n > INT32_MAX) return fail(RQX_ERR_LIMIT, "hit cache columns are int32: at most 2^31 - 1 rows");
     const int64_t n_slices = (n + SLICE - 1) / SLICE;
 
-    // pass 1: non-diagonal hits per row, section B at [0, n), section A at [n, 2n)
+    // pass 1: non-diagonal hits per row (section B at [0, n), section A at
+    // [n, 2n)), then each slice's section widths (after 2n: A widths, B widths,
+    // the hit total)
     uint32_t *rowcnt = nullptr;
-    HIP_TRY(hipMalloc(&rowcnt, static_cast<size_t>(n) * 8));
+    HIP_TRY(hipMalloc(&rowcnt, static_cast<size_t>(2 * n + 2 * n_slices + 2) * 4));
     KParams k = make_params(p, true);
     k.rowcnt = rowcnt;
     k.rowcnt_a = rowcnt + n;
-    hipError_t e = launch_tiles<MODE_ROWCOUNT, double, true>(p, k, nullptr, nullptr, st);
-    std::vector<uint32_t> cnt(static_cast<size_t>(2 * n));
-    if (e == hipSuccess) e = hipMemcpyAsync(cnt.data(), rowcnt, static_cast<size_t>(n) * 8, hipMemcpyDeviceToHost, st);
+    uint32_t *w_dev = rowcnt + 2 * n;
+    unsigned long long *nnz_dev = reinterpret_cast<unsigned long long *>(w_dev + 2 * n_slices);
+    hipError_t e = fill_async<unsigned long long>(nnz_dev, 0, 1, 0ull, st);
+    if (e == hipSuccess) e = launch_tiles<MODE_ROWCOUNT, double, true>(p, k, nullptr, nullptr, st);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_slice_widths, dim3(static_cast<unsigned>((n_slices * SLICE + 255) / 256)), dim3(256), 0, st,
+                           rowcnt, rowcnt + n, n, n_slices, w_dev, w_dev + n_slices, nnz_dev);
+        e = hipGetLastError();
+    }
+    std::vector<uint32_t> wab(static_cast<size_t>(2 * n_slices + 2));
+    if (e == hipSuccess) e = hipMemcpyAsync(wab.data(), w_dev, wab.size() * 4, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     const hipError_t ef = hipFree(rowcnt);
     if (e == hipSuccess) e = ef;
     if (e != hipSuccess) return fail(RQX_ERR_HIP, std::string("cache row count: ") + hipGetErrorString(e));
 
-    // per-slice section widths and offsets of the fill's full layout
-    std::vector<uint32_t> wa(static_cast<size_t>(n_slices), 0u), wb(static_cast<size_t>(n_slices), 0u);
+    // offsets of the column pass's full layout and of the stored section B
+    const std::vector<uint32_t> wa(wab.begin(), wab.begin() + n_slices), wb(wab.begin() + n_slices, wab.begin() + 2 * n_slices);
+    int64_t nnz = 0;
+    std::memcpy(&nnz, wab.data() + 2 * n_slices, 8);
     std::vector<int64_t> off(static_cast<size_t>(n_slices), 0), aoff(static_cast<size_t>(n_slices), 0),
         boff(static_cast<size_t>(n_slices), 0);
-    int64_t full = 0, nnz = 0, a_levels = 0, b_entries = 0;
+    int64_t full = 0, a_levels = 0, b_entries = 0;
     for (int64_t s = 0; s < n_slices; ++s) {
-        uint32_t w_a = 0, w_b = 0;
-        for (int64_t r = s * SLICE; r < std::min(n, (s + 1) * SLICE); ++r) {
-            const uint32_t cb = cnt[static_cast<size_t>(r)], ca = cnt[static_cast<size_t>(n + r)];
-            w_b = std::max(w_b, cb);
-            w_a = std::max(w_a, ca);
-            nnz += static_cast<int64_t>(ca) + cb;
-        }
-        w_a = (w_a + PACK - 1) / PACK * PACK;      // whole packets
-        w_b = (w_b + PACK - 1) / PACK * PACK;
-        wa[static_cast<size_t>(s)] = w_a;
-        wb[static_cast<size_t>(s)] = w_b;
+        const uint32_t w_a = wa[static_cast<size_t>(s)], w_b = wb[static_cast<size_t>(s)];
         off[static_cast<size_t>(s)] = full;
         aoff[static_cast<size_t>(s)] = a_levels;
         boff[static_cast<size_t>(s)] = b_entries;
