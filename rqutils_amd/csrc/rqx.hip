@@ -2331,14 +2331,65 @@ int validate_problem(const ProblemIn &q, int k) {
 
 // Split choice: largest L whose segments fit SEG_CAP, then the cheapest of the
 // candidates below it by the sampled cost model (or the caller's L).
+// host threads for the O(N) passes of a large subspace's index build (1 below 2^18 rows)
+unsigned index_threads(int64_t n) {
+    return n >= (int64_t(1) << 18) ? std::max(1u, std::min(16u, std::thread::hardware_concurrency())) : 1u;
+}
+
+// run f(w, begin, end) over nt contiguous chunks w of [0, n), one host thread each
+template <typename F> void parallel_chunks(int64_t n, unsigned nt, F f) {
+    if (nt <= 1) {
+        f(0u, int64_t(0), n);
+        return;
+    }
+    std::vector<std::thread> pool;
+    for (unsigned w = 1; w < nt; ++w) pool.emplace_back(f, w, n * w / nt, n * (w + 1) / nt);
+    f(0u, int64_t(0), n / nt);
+    for (std::thread &t : pool) t.join();
+}
+
+// max_segment over chunks: each chunk's first and last segment start and its
+// longest segment between them, joined in order
+int max_segment_par(const std::vector<int8_t> &hd, int L, unsigned nt) {
+    const int64_t n = static_cast<int64_t>(hd.size());
+    if (nt <= 1 || n < 2) return max_segment(hd, L);
+    std::vector<int64_t> first(nt, -1), last(nt, -1), inner(nt, 0);
+    parallel_chunks(n, nt, [&](unsigned w, int64_t b, int64_t e) {
+        int64_t f = -1, l = -1, m = 0;
+        for (int64_t i = b; i < e; ++i) {
+            if (i == 0 || hd[static_cast<size_t>(i)] >= L) {   // row i starts a segment
+                if (l >= 0) m = std::max(m, i - l);
+                if (f < 0) f = i;
+                l = i;
+            }
+        }
+        first[w] = f;
+        last[w] = l;
+        inner[w] = m;
+    });
+    int64_t best = 0, prev = -1;
+    for (unsigned w = 0; w < nt; ++w) {
+        best = std::max(best, inner[w]);
+        if (first[w] < 0) continue;
+        if (prev >= 0) best = std::max(best, first[w] - prev);
+        prev = last[w];
+    }
+    best = std::max(best, n - prev);
+    return static_cast<int>(std::min<int64_t>(best, INT32_MAX));
+}
+
 int choose_split(const ProblemIn &q, int32_t split_bits, bool has_diag, Segmentation &best, int k) {
+    const unsigned nt = index_threads(q.n_states);
     std::vector<int8_t> hd(static_cast<size_t>(q.n_states), 64);
-    for (int64_t i = 1; i < q.n_states; ++i) hd[i] = static_cast<int8_t>(highest_diff_bit(q.states[i - 1], q.states[i]));
+    parallel_chunks(q.n_states, nt, [&](unsigned, int64_t b, int64_t e) {
+        for (int64_t i = std::max<int64_t>(b, 1); i < e; ++i)
+            hd[static_cast<size_t>(i)] = static_cast<int8_t>(highest_diff_bit(q.states[i - 1], q.states[i]));
+    });
     const int Lmax_bits = std::min(32, q.n_qubits);
     const std::string tag = "problem " + std::to_string(k) + ": ";
     if (split_bits >= 0) {
         if (split_bits > Lmax_bits) return fail(RQX_ERR_ARG, tag + "split_bits must be <= min(32, n_qubits)");
-        if (max_segment(hd, split_bits) > SEG_CAP)
+        if (max_segment_par(hd, split_bits, nt) > SEG_CAP)
             return fail(RQX_ERR_LIMIT, tag + "split_bits gives a segment larger than " + std::to_string(SEG_CAP) + " rows");
         best.L = split_bits;
         build_segments(best, q.states, q.n_states, q.xmask, q.n_groups);
@@ -2348,19 +2399,29 @@ int choose_split(const ProblemIn &q, int32_t split_bits, bool has_diag, Segmenta
     int lo_l = 0, hi_l = Lmax_bits;
     while (lo_l < hi_l) {
         const int mid = (lo_l + hi_l + 1) / 2;
-        if (max_segment(hd, mid) <= SEG_CAP) lo_l = mid; else hi_l = mid - 1;
+        if (max_segment_par(hd, mid, nt) <= SEG_CAP) lo_l = mid; else hi_l = mid - 1;
     }
-    bool first = true;
-    for (int L = lo_l; L >= std::max(0, lo_l - 8); --L) {
-        Segmentation cand;
-        cand.L = L;
-        build_segments(cand, q.states, q.n_states, q.xmask, q.n_groups);
-        cand.cost = estimate_cost(cand, has_diag);
-        if (first || cand.cost < best.cost) {
-            best = std::move(cand);
-            first = false;
-        }
+    // up to nine candidate splits, each segmented and costed on its own host
+    // thread for large subspaces; the first cheapest (largest L) wins
+    const int n_cand = lo_l - std::max(0, lo_l - 8) + 1;
+    std::vector<Segmentation> cand(static_cast<size_t>(n_cand));
+    auto eval = [&](int c) {
+        cand[c].L = lo_l - c;
+        build_segments(cand[c], q.states, q.n_states, q.xmask, q.n_groups);
+        cand[c].cost = estimate_cost(cand[c], has_diag);
+    };
+    if (nt > 1) {
+        std::vector<std::thread> pool;
+        for (int c = 1; c < n_cand; ++c) pool.emplace_back(eval, c);
+        eval(0);
+        for (std::thread &t : pool) t.join();
+    } else {
+        for (int c = 0; c < n_cand; ++c) eval(c);
     }
+    int bi = 0;
+    for (int c = 1; c < n_cand; ++c)
+        if (cand[c].cost < cand[bi].cost) bi = c;
+    best = std::move(cand[bi]);
     return RQX_OK;
 }
 
