@@ -121,6 +121,17 @@ struct ProbDesc {
     int32_t pad;
 };
 
+// What the coefficient evaluator needs of a group, fetched with one scalar load
+// (separate arrays cost one dependent round trip each: scalar loads return out
+// of order, so every use waits for all loads in flight)
+struct __attribute__((aligned(32))) GroupDesc {
+    uint64_t x;          // X mask
+    uint64_t z0;         // Z mask of the group's first term
+    uint32_t k0, k1;     // its terms [k0, k1)
+    int32_t tab;         // its sign table, or -1
+    uint32_t pad;
+};
+
 struct KParams {
     const uint64_t *keys;
     const uint32_t *lo;
@@ -138,6 +149,7 @@ struct KParams {
     const uint64_t *z;
     const void *coef;      // double2[T] or float2[T]
     const int32_t *g_tab;  // per group: index of its sign table among the tables, or -1
+    const GroupDesc *gdesc;  // per group: X mask, first Z mask, term range, sign table
     uint32_t n_tab;        // sign tables; entry u of table t at u * n_tab + t
     const void *tab_re;    // sign tables in the precision of coef: real parts
     const void *tab_im;    //   and imaginary parts
@@ -282,18 +294,18 @@ __device__ inline int find_u64(const uint64_t *__restrict__ a, int n, uint64_t t
 
 // C_g(s) = sum_k coef_k (-1)^popcount(s & z_k), coefficients in precision R
 template <typename R>
-__device__ inline typename C2<R>::T group_coef_from(const void *coef_v, const KParams &p, uint32_t g, uint64_t s) {
+__device__ inline typename C2<R>::T group_coef_range(const void *coef_v, const KParams &p, uint32_t k0, uint32_t k1,
+                                                     uint64_t s) {
     using CC = typename C2<R>::T;
     const CC *coef = static_cast<const CC *>(coef_v);
     CC acc;
     acc.x = R(0);
     acc.y = R(0);
-    const uint32_t k1 = p.g_toff[g + 1];
     // the terms' (z, coef) are wave-uniform scalar loads, which return out of
     // order: fetch TERM_PAD terms before using any, so one scalar-memory round
     // trip serves them all (the term arrays carry TERM_PAD entries of padding;
     // the sum keeps its term order)
-    for (uint32_t k = p.g_toff[g]; k < k1; k += TERM_PAD) {
+    for (uint32_t k = k0; k < k1; k += TERM_PAD) {
         uint64_t z[TERM_PAD];
         CC c[TERM_PAD];
 #pragma unroll
@@ -315,18 +327,22 @@ __device__ inline typename C2<R>::T group_coef_from(const void *coef_v, const KP
 }
 
 template <typename R>
+__device__ inline typename C2<R>::T group_coef_from(const void *coef_v, const KParams &p, uint32_t g, uint64_t s) {
+    return group_coef_range<R>(coef_v, p, p.g_toff[g], p.g_toff[g + 1], s);
+}
+
+template <typename R>
 __device__ inline typename C2<R>::T group_coef(const KParams &p, uint32_t g, uint64_t s) {
     return group_coef_from<R>(p.coef, p, g, s);
 }
 
-// Re C_g(s), the whole sum when the coefficients are real (same order as group_coef_from)
+// Re C_g(s), the whole sum when the coefficients are real (same order as group_coef_range)
 template <typename R>
-__device__ inline R group_coef_re_from(const void *coef_v, const KParams &p, uint32_t g, uint64_t s) {
+__device__ inline R group_coef_re_range(const void *coef_v, const KParams &p, uint32_t k0, uint32_t k1, uint64_t s) {
     using CC = typename C2<R>::T;
     const CC *coef = static_cast<const CC *>(coef_v);
     R acc = R(0);
-    const uint32_t k1 = p.g_toff[g + 1];
-    for (uint32_t k = p.g_toff[g]; k < k1; k += TERM_PAD) {
+    for (uint32_t k = k0; k < k1; k += TERM_PAD) {
         uint64_t z[TERM_PAD];
         R c[TERM_PAD];
 #pragma unroll
@@ -344,6 +360,11 @@ __device__ inline R group_coef_re_from(const void *coef_v, const KParams &p, uin
         }
     }
     return acc;
+}
+
+template <typename R>
+__device__ inline R group_coef_re_from(const void *coef_v, const KParams &p, uint32_t g, uint64_t s) {
+    return group_coef_re_range<R>(coef_v, p, p.g_toff[g], p.g_toff[g + 1], s);
 }
 
 // sum over the 64 lanes of a wave (every lane receives the total)
@@ -393,22 +414,7 @@ __device__ inline void group_value_wave(const void *coef_v, const KParams &p, ui
     using CC = typename C2<R>::T;
     vx = R(0);
     vy = R(0);
-    if constexpr (TP) {
-        const int32_t toff = p.g_tab[g];
-        if (toff >= 0) {
-            if (hit) {
-                const uint32_t u = tab_index(p.g_x[g], s) * p.n_tab + static_cast<uint32_t>(toff);
-                const bool neg = __popcll(s & p.z[p.g_toff[g]]) & 1;
-                const R re = static_cast<const R *>(p.tab_re)[u];
-                vx = neg ? -re : re;
-                if constexpr (!RE) {
-                    const R im = static_cast<const R *>(p.tab_im)[u];
-                    vy = neg ? -im : im;
-                }
-            }
-            return;
-        }
-    } else {
+    if constexpr (!TP) {
         if (hit) {
             if constexpr (RE) {
                 vx = group_coef_re_from<R>(coef_v, p, g, s);
@@ -420,15 +426,29 @@ __device__ inline void group_value_wave(const void *coef_v, const KParams &p, ui
         }
         return;
     }
+    const GroupDesc d = p.gdesc[g];
+    if (d.tab >= 0) {
+        if (hit) {
+            const uint32_t u = tab_index(d.x, s) * p.n_tab + static_cast<uint32_t>(d.tab);
+            const bool neg = __popcll(s & d.z0) & 1;
+            const R re = static_cast<const R *>(p.tab_re)[u];
+            vx = neg ? -re : re;
+            if constexpr (!RE) {
+                const R im = static_cast<const R *>(p.tab_im)[u];
+                vy = neg ? -im : im;
+            }
+        }
+        return;
+    }
     const uint64_t m = __ballot(hit);
     if (m == 0) return;
-    const uint32_t k0 = p.g_toff[g], k1 = p.g_toff[g + 1];
+    const uint32_t k0 = d.k0, k1 = d.k1;
     if (!term_parallel(k1 - k0, static_cast<uint32_t>(__popcll(m)))) {
         if (hit) {
             if constexpr (RE) {
-                vx = group_coef_re_from<R>(coef_v, p, g, s);
+                vx = group_coef_re_range<R>(coef_v, p, k0, k1, s);
             } else {
-                const CC c = group_coef_from<R>(coef_v, p, g, s);
+                const CC c = group_coef_range<R>(coef_v, p, k0, k1, s);
                 vx = c.x;
                 vy = c.y;
             }
@@ -501,13 +521,12 @@ __device__ inline void grad_group(const KParams &p, uint32_t g, uint64_t s, bool
 // the order of group_coef_from) from one pass over the terms: the fill of a
 // cache built in both precisions
 template <bool RE_ONLY>
-__device__ inline void group_coef_dual(const KParams &p, uint32_t g, uint64_t s, double2 &hi, float2 &lo) {
+__device__ inline void group_coef_dual(const KParams &p, uint32_t k0, uint32_t k1, uint64_t s, double2 &hi, float2 &lo) {
     const double2 *c64 = static_cast<const double2 *>(p.coef);
     const float2 *c32 = static_cast<const float2 *>(p.coef_lo);
     hi = make_double2(0.0, 0.0);
     lo = make_float2(0.f, 0.f);
-    const uint32_t k1 = p.g_toff[g + 1];
-    for (uint32_t k = p.g_toff[g]; k < k1; k += TERM_PAD) {
+    for (uint32_t k = k0; k < k1; k += TERM_PAD) {
         uint64_t z[TERM_PAD];
         double2 ch[TERM_PAD];
         float2 cl[TERM_PAD];
@@ -558,11 +577,11 @@ template <bool RE>
 __device__ inline void group_value_wave_dual(const KParams &p, uint32_t g, uint64_t s, bool hit, double2 &hi, float2 &lo) {
     hi = make_double2(0.0, 0.0);
     lo = make_float2(0.f, 0.f);
-    const int32_t toff = p.g_tab[g];
-    if (toff >= 0) {
+    const GroupDesc d = p.gdesc[g];
+    if (d.tab >= 0) {
         if (hit) {
-            const uint32_t u = tab_index(p.g_x[g], s) * p.n_tab + static_cast<uint32_t>(toff);
-            const bool neg = __popcll(s & p.z[p.g_toff[g]]) & 1;
+            const uint32_t u = tab_index(d.x, s) * p.n_tab + static_cast<uint32_t>(d.tab);
+            const bool neg = __popcll(s & d.z0) & 1;
             const double hre = static_cast<const double *>(p.tab_re)[u];
             const float lre = p.tab_lo_re[u];
             hi.x = neg ? -hre : hre;
@@ -578,9 +597,9 @@ __device__ inline void group_value_wave_dual(const KParams &p, uint32_t g, uint6
     }
     const uint64_t m = __ballot(hit);
     if (m == 0) return;
-    const uint32_t k0 = p.g_toff[g], k1 = p.g_toff[g + 1];
+    const uint32_t k0 = d.k0, k1 = d.k1;
     if (!term_parallel(k1 - k0, static_cast<uint32_t>(__popcll(m)))) {
-        if (hit) group_coef_dual<RE>(p, g, s, hi, lo);
+        if (hit) group_coef_dual<RE>(p, k0, k1, s, hi, lo);
         return;
     }
     const double2 *c64 = static_cast<const double2 *>(p.coef);
@@ -2032,6 +2051,7 @@ struct rqx_plan {
     uint2 *st_list = nullptr;       // (class, target segment) pairs
     uint64_t *g_zor = nullptr;      // per group: union of its terms' Z masks
     int32_t *g_tab = nullptr;       // per group: its sign table, or -1
+    GroupDesc *gdesc = nullptr;     // per group: X mask, first Z mask, term range, sign table
     uint32_t n_tab = 0;             // sign tables (entry u of table t at u * n_tab + t)
     double *tab64re = nullptr;      // sign tables, FP64 and FP32, real and imaginary parts
     double *tab64im = nullptr;
@@ -2221,6 +2241,7 @@ KParams make_params(const rqx_plan *p, bool fp64) {
     k.z = p->z;
     k.coef = fp64 ? static_cast<const void *>(p->coef64) : static_cast<const void *>(p->coef32);
     k.g_tab = p->g_tab;
+    k.gdesc = p->gdesc;
     k.n_tab = p->n_tab;
     k.tab_re = fp64 ? static_cast<const void *>(p->tab64re) : static_cast<const void *>(p->tab32re);
     k.tab_im = fp64 ? static_cast<const void *>(p->tab64im) : static_cast<const void *>(p->tab32im);
@@ -2461,6 +2482,7 @@ int build_plan(rqx_plan **out, const std::vector<ProblemIn> &in, int64_t n_total
     std::vector<double> tile_cost;
     std::vector<int32_t> g_tab;    // per group: first sign-table entry, or -1
     std::vector<uint64_t> g_zor;   // per group: union of its terms' Z masks
+    std::vector<GroupDesc> gdesc;
     int64_t n_tab = 0;
     int64_t row_base = 0, term_base = 0;
     int32_t grp_base = 0;
@@ -2526,6 +2548,13 @@ int build_plan(rqx_plan **out, const std::vector<ProblemIn> &in, int64_t n_total
             } else {
                 g_tab.push_back(-1);
             }
+            GroupDesc gd{};
+            gd.x = q.xmask[j];
+            gd.z0 = q.grp_off[j + 1] > q.grp_off[j] ? q.zmask[q.grp_off[j]] : 0ull;
+            gd.k0 = static_cast<uint32_t>(term_base + q.grp_off[j]);
+            gd.k1 = static_cast<uint32_t>(term_base + q.grp_off[j + 1]);
+            gd.tab = g_tab.back();
+            gdesc.push_back(gd);
         }
         for (int64_t t = 0; t < q.n_terms; ++t) {
             zall.push_back(q.zmask[t]);
@@ -2606,7 +2635,8 @@ int build_plan(rqx_plan **out, const std::vector<ProblemIn> &in, int64_t n_total
                  o_d32 = carve(n * 8), o_trow = carve(tile_row0.size() * 4), o_tseg = carve(tile_seg.size() * 4),
                  o_tprob = carve(tile_prob.size() * 4), o_desc = carve(desc.size() * sizeof(ProbDesc)),
                  o_prows = carve(prob_rows.size() * 8), o_queue = carve(sizeof(unsigned int)),
-                 o_gtab = carve(g_tab.size() * 4), o_gzor = carve(g_zor.size() * 8), o_t64re = carve(static_cast<size_t>(n_tab) * TAB_ENTRIES * 8),
+                 o_gtab = carve(g_tab.size() * 4), o_gzor = carve(g_zor.size() * 8),
+                 o_gdesc = carve(gdesc.size() * sizeof(GroupDesc)), o_t64re = carve(static_cast<size_t>(n_tab) * TAB_ENTRIES * 8),
                  o_t64im = carve(static_cast<size_t>(n_tab) * TAB_ENTRIES * 8),
                  o_t32re = carve(static_cast<size_t>(n_tab) * TAB_ENTRIES * 4),
                  o_t32im = carve(static_cast<size_t>(n_tab) * TAB_ENTRIES * 4), o_hoff = carve(h_off.size() * 4),
@@ -2635,6 +2665,7 @@ int build_plan(rqx_plan **out, const std::vector<ProblemIn> &in, int64_t n_total
     p->queue = reinterpret_cast<unsigned int *>(b + o_queue);
     p->g_tab = reinterpret_cast<int32_t *>(b + o_gtab);
     p->g_zor = reinterpret_cast<uint64_t *>(b + o_gzor);
+    p->gdesc = reinterpret_cast<GroupDesc *>(b + o_gdesc);
     p->tab64re = reinterpret_cast<double *>(b + o_t64re);
     p->tab64im = reinterpret_cast<double *>(b + o_t64im);
     p->tab32re = reinterpret_cast<float *>(b + o_t32re);
@@ -2662,7 +2693,7 @@ int build_plan(rqx_plan **out, const std::vector<ProblemIn> &in, int64_t n_total
         up(p->tile_row0, tile_row0.data(), tile_row0.size() * 4), up(p->tile_seg, tile_seg.data(), tile_seg.size() * 4),
         up(p->tile_prob, tile_prob.data(), tile_prob.size() * 4), up(p->prob, desc.data(), desc.size() * sizeof(ProbDesc)),
         up(p->prob_rows, prob_rows.data(), prob_rows.size() * 8), up(p->g_tab, g_tab.data(), g_tab.size() * 4),
-        up(p->g_zor, g_zor.data(), g_zor.size() * 8),
+        up(p->g_zor, g_zor.data(), g_zor.size() * 8), up(p->gdesc, gdesc.data(), gdesc.size() * sizeof(GroupDesc)),
         up(p->h_off, h_off.data(), h_off.size() * 4), up(p->h_log, h_log.data(), h_log.size() * 4),
         up(p->h_tab, h_tab.data(), h_tab.size() * 4)};
     for (hipError_t ei : errs)
