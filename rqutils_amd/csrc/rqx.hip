@@ -129,7 +129,8 @@ struct __attribute__((aligned(32))) GroupDesc {
     uint64_t z0;         // Z mask of the group's first term
     uint32_t k0, k1;     // its terms [k0, k1)
     int32_t tab;         // its sign table, or -1
-    uint32_t pad;
+    uint32_t xpos;       // with a sign table: the positions of x's bits, 6 bits each from
+                         // the lowest, and their count in bits 24..31
 };
 
 struct KParams {
@@ -394,12 +395,14 @@ constexpr int TAB_MAX_BITS = 4;      // tables of at most 16 entries
 constexpr int TAB_ENTRIES = 1 << TAB_MAX_BITS;
 constexpr int TAB_MIN_TERMS = 3;
 
-// the bits of s under the (wave-uniform) mask x, packed low
-__device__ inline uint32_t tab_index(uint64_t x, uint64_t s) {
+// the bits of s at a group's X positions (packed in its descriptor), packed low:
+// the sign-table pattern of s
+__device__ inline uint32_t tab_index_packed(uint32_t xpos, uint64_t s) {
+    const uint32_t nb = xpos >> 24;
     uint32_t idx = 0;
-    int j = 0;
-    for (uint64_t xm = x; xm != 0; xm &= xm - 1, ++j)
-        idx |= static_cast<uint32_t>((s >> (__ffsll(static_cast<long long>(xm)) - 1)) & 1ull) << j;
+#pragma unroll
+    for (int j = 0; j < TAB_MAX_BITS; ++j)
+        if (static_cast<uint32_t>(j) < nb) idx |= static_cast<uint32_t>((s >> ((xpos >> (6 * j)) & 63u)) & 1ull) << j;
     return idx;
 }
 
@@ -409,27 +412,26 @@ __device__ inline uint32_t tab_index(uint64_t x, uint64_t s) {
 constexpr int TP_MIN_TERMS = 4;
 
 template <typename R, bool RE, bool TP = true>
-__device__ inline void group_value_wave(const void *coef_v, const KParams &p, uint32_t g, uint64_t s, bool hit,
-                                        R &vx, R &vy) {
+__device__ inline void group_value_wave(const void *coef_v, const KParams &p, const GroupDesc &d, uint64_t s,
+                                        bool hit, R &vx, R &vy) {
     using CC = typename C2<R>::T;
     vx = R(0);
     vy = R(0);
     if constexpr (!TP) {
         if (hit) {
             if constexpr (RE) {
-                vx = group_coef_re_from<R>(coef_v, p, g, s);
+                vx = group_coef_re_range<R>(coef_v, p, d.k0, d.k1, s);
             } else {
-                const CC c = group_coef_from<R>(coef_v, p, g, s);
+                const CC c = group_coef_range<R>(coef_v, p, d.k0, d.k1, s);
                 vx = c.x;
                 vy = c.y;
             }
         }
         return;
     }
-    const GroupDesc d = p.gdesc[g];
     if (d.tab >= 0) {
         if (hit) {
-            const uint32_t u = tab_index(d.x, s) * p.n_tab + static_cast<uint32_t>(d.tab);
+            const uint32_t u = tab_index_packed(d.xpos, s) * p.n_tab + static_cast<uint32_t>(d.tab);
             const bool neg = __popcll(s & d.z0) & 1;
             const R re = static_cast<const R *>(p.tab_re)[u];
             vx = neg ? -re : re;
@@ -476,10 +478,31 @@ __device__ inline void group_value_wave(const void *coef_v, const KParams &p, ui
     }
 }
 
+template <typename R, bool RE, bool TP = true>
+__device__ inline void group_value_wave(const void *coef_v, const KParams &p, uint32_t g, uint64_t s, bool hit,
+                                        R &vx, R &vy) {
+    if constexpr (TP) {
+        group_value_wave<R, RE, TP>(coef_v, p, p.gdesc[g], s, hit, vx, vy);
+    } else {            // the plain evaluator needs only the term range
+        vx = R(0);
+        vy = R(0);
+        if (hit) {
+            if constexpr (RE) {
+                vx = group_coef_re_from<R>(coef_v, p, g, s);
+            } else {
+                const typename C2<R>::T c = group_coef_from<R>(coef_v, p, g, s);
+                vx = c.x;
+                vy = c.y;
+            }
+        }
+    }
+}
+
 // the same in FP64 and FP32 at once (the cache fill of both precisions): each
 // precision summed exactly as group_value_wave sums it
 template <bool RE>
-__device__ inline void group_value_wave_dual(const KParams &p, uint32_t g, uint64_t s, bool hit, double2 &hi, float2 &lo);
+__device__ inline void group_value_wave_dual(const KParams &p, const GroupDesc &d, uint64_t s, bool hit, double2 &hi,
+                                             float2 &lo);
 
 // real and imaginary parts of conj(g) * v for real or complex entries
 __device__ inline void conj_mul(double g, double v, double &re, double &im) { re = g * v; im = 0.0; }
@@ -574,13 +597,13 @@ __device__ inline void coo_emit(const KParams &p, bool hit, int64_t row, int64_t
 }
 
 template <bool RE>
-__device__ inline void group_value_wave_dual(const KParams &p, uint32_t g, uint64_t s, bool hit, double2 &hi, float2 &lo) {
+__device__ inline void group_value_wave_dual(const KParams &p, const GroupDesc &d, uint64_t s, bool hit, double2 &hi,
+                                             float2 &lo) {
     hi = make_double2(0.0, 0.0);
     lo = make_float2(0.f, 0.f);
-    const GroupDesc d = p.gdesc[g];
     if (d.tab >= 0) {
         if (hit) {
-            const uint32_t u = tab_index(d.x, s) * p.n_tab + static_cast<uint32_t>(d.tab);
+            const uint32_t u = tab_index_packed(d.xpos, s) * p.n_tab + static_cast<uint32_t>(d.tab);
             const bool neg = __popcll(s & d.z0) & 1;
             const double hre = static_cast<const double *>(p.tab_re)[u];
             const float lre = p.tab_lo_re[u];
@@ -784,14 +807,14 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
                 float2 vl;
                 V xv;
             };
-            auto value = [&](uint32_t g, uint32_t base, bool hit) {
+            auto value = [&](const GroupDesc &d, uint32_t base, bool hit) {
                 HitVal h{};
                 if constexpr (MODE == MODE_MATVEC) {
-                    group_value_wave<R, REALC, TP>(p.coef, p, g, s, hit, h.vx, h.vy);
+                    group_value_wave<R, REALC, TP>(p.coef, p, d, s, hit, h.vx, h.vy);
                     if (hit) h.xv = x[ta + base];
                 } else if constexpr (MODE == MODE_FILL) {
-                    if (p.c_val_lo) group_value_wave_dual<!CPLX>(p, g, s, hit, h.vh, h.vl);   // both precisions, R = double
-                    else group_value_wave<R, !CPLX>(p.coef, p, g, s, hit, h.vx, h.vy);
+                    if (p.c_val_lo) group_value_wave_dual<!CPLX>(p, d, s, hit, h.vh, h.vl);   // both precisions, R = double
+                    else group_value_wave<R, !CPLX>(p.coef, p, d, s, hit, h.vx, h.vy);
                 }
                 return h;
             };
@@ -840,7 +863,7 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
                         if (hit) fma_acc(sec_a ? acc_a : acc, c, x[ta + base]);
                     }
                 } else if constexpr (MODE == MODE_FILL) {
-                    consume(g, base, hit, sec_a, value(g, base, hit));
+                    consume(g, base, hit, sec_a, value(p.gdesc[g], base, hit));
                 } else if constexpr (MODE == MODE_FILLCOL) {
                     if (hit) {
                         const int32_t c = static_cast<int32_t>(ta + base);
@@ -886,8 +909,26 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
             // four independent searches in lock step: the dependent LDS loads of
             // one search overlap with those of the other three
             for (; g + 4 <= g1; g += 4) {
-                const uint32_t t0 = slo ^ p.g_xlo[g], t1 = slo ^ p.g_xlo[g + 1];
-                const uint32_t t2 = slo ^ p.g_xlo[g + 2], t3 = slo ^ p.g_xlo[g + 3];
+                // the full evaluator fetches the four groups' descriptors at once,
+                // ahead of the searches (their X masks give the targets)
+                constexpr bool ILP = (MODE == MODE_MATVEC || MODE == MODE_FILL) && TP;
+                GroupDesc d0{}, d1{}, d2{}, d3{};
+                uint32_t t0, t1, t2, t3;
+                if constexpr (ILP) {
+                    d0 = p.gdesc[g];
+                    d1 = p.gdesc[g + 1];
+                    d2 = p.gdesc[g + 2];
+                    d3 = p.gdesc[g + 3];
+                    t0 = slo ^ (static_cast<uint32_t>(d0.x) & pd.lomask);
+                    t1 = slo ^ (static_cast<uint32_t>(d1.x) & pd.lomask);
+                    t2 = slo ^ (static_cast<uint32_t>(d2.x) & pd.lomask);
+                    t3 = slo ^ (static_cast<uint32_t>(d3.x) & pd.lomask);
+                } else {
+                    t0 = slo ^ p.g_xlo[g];
+                    t1 = slo ^ p.g_xlo[g + 1];
+                    t2 = slo ^ p.g_xlo[g + 2];
+                    t3 = slo ^ p.g_xlo[g + 3];
+                }
                 uint32_t b0 = 0, b1 = 0, b2 = 0, b3 = 0;
                 bool f0, f1, f2, f3;
                 if (hashed) {
@@ -909,10 +950,10 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
                     f2 = s_keys[b2] == t2;
                     f3 = s_keys[b3] == t3;
                 }
-                if constexpr ((MODE == MODE_MATVEC || MODE == MODE_FILL) && TP) {   // (measured: slower for single-term groups)
+                if constexpr (ILP) {   // (measured: slower for single-term groups)
                     const bool h0 = active && f0, h1 = active && f1, h2 = active && f2, h3 = active && f3;
-                    const HitVal v0 = value(g, b0, h0), v1 = value(g + 1, b1, h1);
-                    const HitVal v2 = value(g + 2, b2, h2), v3 = value(g + 3, b3, h3);
+                    const HitVal v0 = value(d0, b0, h0), v1 = value(d1, b1, h1);
+                    const HitVal v2 = value(d2, b2, h2), v3 = value(d3, b3, h3);
                     consume(g, b0, h0, false, v0);
                     consume(g + 1, b1, h1, false, v1);
                     consume(g + 2, b2, h2, false, v2);
@@ -2554,6 +2595,12 @@ int build_plan(rqx_plan **out, const std::vector<ProblemIn> &in, int64_t n_total
             gd.k0 = static_cast<uint32_t>(term_base + q.grp_off[j]);
             gd.k1 = static_cast<uint32_t>(term_base + q.grp_off[j + 1]);
             gd.tab = g_tab.back();
+            if (gd.tab >= 0) {
+                uint32_t nb = 0;
+                for (uint64_t xm = gd.x; xm != 0; xm &= xm - 1, ++nb)
+                    gd.xpos |= static_cast<uint32_t>(__builtin_ctzll(xm)) << (6 * nb);
+                gd.xpos |= nb << 24;
+            }
             gdesc.push_back(gd);
         }
         for (int64_t t = 0; t < q.n_terms; ++t) {
