@@ -133,6 +133,15 @@ struct __attribute__((aligned(32))) GroupDesc {
                          // the lowest, and their count in bits 24..31
 };
 
+// One (class, target segment) pair of a segment's target list, with what the
+// tile kernel needs of both (one scalar load)
+struct __attribute__((aligned(32))) TargetDesc {
+    uint32_t ta, tn;     // the target segment's rows [ta, ta + tn)
+    uint32_t ho, lg;     // its hash table (hashed plans): first entry, log2 buckets
+    uint32_t g0, g1;     // the class's groups [g0, g1), without the x == 0 group
+    uint32_t cls, ts;    // the class and the target segment
+};
+
 struct KParams {
     const uint64_t *keys;
     const uint32_t *lo;
@@ -144,7 +153,7 @@ struct KParams {
     const uint32_t *h_off;         // hashed segments (NULL: binary search): first table entry,
     const uint32_t *h_log;         //   log2 of the bucket count,
     const uint32_t *h_tab;         //   tables of (low key << 13 | rank), two entries per bucket
-    const uint2 *st_list;          // pairs of the classes whose target segment exists, class order
+    const TargetDesc *st_list;     // pairs of the classes whose target segment exists, class order
     const uint32_t *g_xlo;
     const uint32_t *g_toff;
     const uint64_t *z;
@@ -767,16 +776,13 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
     const uint32_t e1 = p.st_off[seg + 1];
     for (uint32_t e = p.st_off[seg]; e < e1; ++e) {
         {
-            const uint2 ct = p.st_list[e];
-            const uint32_t ts = ct.y;
+            const TargetDesc td = p.st_list[e];
             if constexpr (MODE == MODE_COUNT) n_look += (tid == 0);
-            const uint32_t ta = p.seg_off[ts];
-            const uint32_t tn = p.seg_off[ts + 1] - ta;
+            const uint32_t ta = td.ta, tn = td.tn;
             const bool hashed = p.h_off != nullptr;          // the same for the whole plan
-            uint32_t lg = 0;
+            const uint32_t lg = td.lg;
             if (hashed) {
-                lg = p.h_log[ts];
-                const uint32_t ho = p.h_off[ts], ns = 2u << lg;
+                const uint32_t ho = td.ho, ns = 2u << lg;
                 for (uint32_t k = tid; k < ns; k += TILE) s_keys[k] = p.h_tab[ho + k];
             } else {
                 for (uint32_t k = tid; k < tn; k += TILE) s_keys[k] = p.lo[ta + k];
@@ -793,10 +799,7 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
                 }
                 return s_keys[base] == t;
             };
-            const uint32_t cls = ct.x;
-            uint32_t g0 = p.cls_goff[cls];
-            const uint32_t g1 = p.cls_goff[cls + 1];
-            if (pd.has_diag && g0 == pd.grp_base) g0 += 1;
+            const uint32_t g0 = td.g0, g1 = td.g1;
             // a hit's value (and, for the matvec, its x entry) first, then its
             // use: the full-evaluator matvec and value pass take four groups'
             // values before using them, so their loads are in flight together;
@@ -1067,12 +1070,18 @@ hipError_t fill_async(V *y, int64_t begin, int64_t end, V value, hipStream_t st)
 // (class, target segment) pairs of one problem's segments: one workgroup per
 // segment m, the problem's classes c in chunks of 256, kept when the segment
 // seg_hi[m] ^ cls_hi[c] exists, in class order.  list == NULL: count only
-// (count[m - seg0]); else write from st_off[m] on.
+// (count[m - seg0]); else write from st_off[m] on, with the target's rows and
+// hash table (h_off == NULL: none) and the class's groups (skipping the group
+// skip_grp, the problem's x == 0 group, if it is the class's first).
 __global__ __launch_bounds__(256) void k_seg_targets(const uint64_t *__restrict__ seg_hi,
                                                      const uint64_t *__restrict__ cls_hi, uint32_t seg0,
                                                      uint32_t n_segs, uint32_t cls0, uint32_t n_classes,
                                                      uint32_t *__restrict__ count, const uint32_t *__restrict__ st_off,
-                                                     uint2 *__restrict__ list) {
+                                                     const uint32_t *__restrict__ seg_off,
+                                                     const uint32_t *__restrict__ h_off,
+                                                     const uint32_t *__restrict__ h_log,
+                                                     const uint32_t *__restrict__ cls_goff, uint32_t skip_grp,
+                                                     TargetDesc *__restrict__ list) {
     __shared__ uint32_t s_wave[TILE / 64];
     const uint32_t m = seg0 + blockIdx.x;
     const uint64_t shi = seg_hi[m];
@@ -1093,7 +1102,18 @@ __global__ __launch_bounds__(256) void k_seg_targets(const uint64_t *__restrict_
         if (t >= 0 && list) {
             const uint32_t below = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(bal >> 32),
                                                              __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(bal), 0u));
-            list[st_off[m] + written + before + below] = make_uint2(cls0 + c, seg0 + static_cast<uint32_t>(t));
+            const uint32_t ts = seg0 + static_cast<uint32_t>(t), cls = cls0 + c;
+            TargetDesc d;
+            d.ta = seg_off[ts];
+            d.tn = seg_off[ts + 1] - d.ta;
+            d.ho = h_off ? h_off[ts] : 0u;
+            d.lg = h_off ? h_log[ts] : 0u;
+            d.g0 = cls_goff[cls];
+            d.g1 = cls_goff[cls + 1];
+            if (d.g0 == skip_grp) d.g0 += 1;
+            d.cls = cls;
+            d.ts = ts;
+            list[st_off[m] + written + before + below] = d;
         }
         written += total;
         __syncthreads();              // s_wave is rewritten by the next chunk
@@ -2089,7 +2109,7 @@ struct rqx_plan {
     uint32_t *h_tab = nullptr;      // hash tables
     void *st_block = nullptr;       // per-segment target lists (own allocation)
     uint32_t *st_off = nullptr;     // M + 1 offsets into st_list
-    uint2 *st_list = nullptr;       // (class, target segment) pairs
+    TargetDesc *st_list = nullptr;  // (class, target segment) pairs with their ranges
     uint64_t *g_zor = nullptr;      // per group: union of its terms' Z masks
     int32_t *g_tab = nullptr;       // per group: its sign table, or -1
     GroupDesc *gdesc = nullptr;     // per group: X mask, first Z mask, term range, sign table
@@ -2230,7 +2250,7 @@ hipError_t build_targets(rqx_plan *p, hipStream_t st) {
             hipLaunchKernelGGL(k_seg_targets, dim3(ph.n_segs), dim3(TILE), 0, st, p->seg_hi, p->cls_hi,
                                static_cast<uint32_t>(ph.seg_base), static_cast<uint32_t>(ph.n_segs),
                                static_cast<uint32_t>(ph.cls_base), static_cast<uint32_t>(ph.n_classes),
-                               cnt + ph.seg_base, nullptr, nullptr);
+                               cnt + ph.seg_base, nullptr, nullptr, nullptr, nullptr, nullptr, 0u, nullptr);
     std::vector<uint32_t> hc(M);
     e = hipGetLastError();
     if (e == hipSuccess && M) e = hipMemcpyAsync(hc.data(), cnt, M * 4, hipMemcpyDeviceToHost, st);
@@ -2246,10 +2266,10 @@ hipError_t build_targets(rqx_plan *p, hipStream_t st) {
     }
     off[M] = static_cast<uint32_t>(total);
     const size_t o_list = align_up((M + 1) * 4);
-    e = hipMalloc(&p->st_block, o_list + std::max<size_t>(total, 1) * sizeof(uint2));
+    e = hipMalloc(&p->st_block, o_list + std::max<size_t>(total, 1) * sizeof(TargetDesc));
     if (e != hipSuccess) return e;
     p->st_off = static_cast<uint32_t *>(p->st_block);
-    p->st_list = reinterpret_cast<uint2 *>(static_cast<char *>(p->st_block) + o_list);
+    p->st_list = reinterpret_cast<TargetDesc *>(static_cast<char *>(p->st_block) + o_list);
     e = hipMemcpyAsync(p->st_off, off.data(), off.size() * 4, hipMemcpyHostToDevice, st);
     if (e != hipSuccess) return e;
     for (const ProbHost &ph : p->probs)
@@ -2257,7 +2277,8 @@ hipError_t build_targets(rqx_plan *p, hipStream_t st) {
             hipLaunchKernelGGL(k_seg_targets, dim3(ph.n_segs), dim3(TILE), 0, st, p->seg_hi, p->cls_hi,
                                static_cast<uint32_t>(ph.seg_base), static_cast<uint32_t>(ph.n_segs),
                                static_cast<uint32_t>(ph.cls_base), static_cast<uint32_t>(ph.n_classes), nullptr,
-                               p->st_off, p->st_list);
+                               p->st_off, p->seg_off, p->hashed ? p->h_off : nullptr, p->h_log, p->cls_goff,
+                               ph.has_diag ? static_cast<uint32_t>(ph.grp_base) : UINT32_MAX, p->st_list);
     e = hipGetLastError();
     if (e == hipSuccess) e = hipStreamSynchronize(st);   // off is released on return
     return e;
