@@ -109,6 +109,27 @@ template <typename FV, typename FL> struct FillBuf {
 // the z and coefficient arrays are padded by as many entries
 constexpr int TERM_PAD = 4;
 
+// A load of plan data (term and group tables, target lists, tile lists) through
+// the constant address space: the kernels never write these arrays, and saying
+// so lets the compiler serve wave-uniform loads from the scalar cache even in
+// kernels that store inside their loops (the cache passes, the gradient), where
+// it must otherwise assume a store may have changed them and load with vector
+// instructions.
+template <typename T> __device__ inline T ldc(const T *a) {
+    if constexpr (std::is_arithmetic<T>::value) {
+        return *(const __attribute__((address_space(4))) T *)(a);
+    } else {                      // structs and vector types: word by word
+        static_assert(sizeof(T) % 4 == 0, "whole words");
+        const __attribute__((address_space(4))) uint32_t *w = (const __attribute__((address_space(4))) uint32_t *)(a);
+        uint32_t t[sizeof(T) / 4];
+#pragma unroll
+        for (size_t k = 0; k < sizeof(T) / 4; ++k) t[k] = w[k];
+        T v;
+        __builtin_memcpy(&v, t, sizeof(T));
+        return v;
+    }
+}
+
 // One problem of a plan (a batched plan holds several).  Row, segment, class,
 // group and term indices in the plan arrays are absolute; the descriptor gives
 // the problem's own ranges.
@@ -320,8 +341,8 @@ __device__ inline typename C2<R>::T group_coef_range(const void *coef_v, const K
         CC c[TERM_PAD];
 #pragma unroll
         for (int u = 0; u < TERM_PAD; ++u) {
-            z[u] = p.z[k + u];
-            c[u] = coef[k + u];
+            z[u] = ldc(p.z + k + u);
+            c[u] = ldc(coef + k + u);
         }
         __builtin_amdgcn_sched_barrier(0);   // keep the loads ahead of their first use
 #pragma unroll
@@ -338,7 +359,7 @@ __device__ inline typename C2<R>::T group_coef_range(const void *coef_v, const K
 
 template <typename R>
 __device__ inline typename C2<R>::T group_coef_from(const void *coef_v, const KParams &p, uint32_t g, uint64_t s) {
-    return group_coef_range<R>(coef_v, p, p.g_toff[g], p.g_toff[g + 1], s);
+    return group_coef_range<R>(coef_v, p, ldc(p.g_toff + g), ldc(p.g_toff + g + 1), s);
 }
 
 template <typename R>
@@ -357,8 +378,8 @@ __device__ inline R group_coef_re_range(const void *coef_v, const KParams &p, ui
         R c[TERM_PAD];
 #pragma unroll
         for (int u = 0; u < TERM_PAD; ++u) {
-            z[u] = p.z[k + u];
-            c[u] = coef[k + u].x;
+            z[u] = ldc(p.z + k + u);
+            c[u] = ldc(coef + k + u).x;
         }
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
@@ -374,7 +395,7 @@ __device__ inline R group_coef_re_range(const void *coef_v, const KParams &p, ui
 
 template <typename R>
 __device__ inline R group_coef_re_from(const void *coef_v, const KParams &p, uint32_t g, uint64_t s) {
-    return group_coef_re_range<R>(coef_v, p, p.g_toff[g], p.g_toff[g + 1], s);
+    return group_coef_re_range<R>(coef_v, p, ldc(p.g_toff + g), ldc(p.g_toff + g + 1), s);
 }
 
 // sum over the 64 lanes of a wave (every lane receives the total)
@@ -491,7 +512,7 @@ template <typename R, bool RE, bool TP = true>
 __device__ inline void group_value_wave(const void *coef_v, const KParams &p, uint32_t g, uint64_t s, bool hit,
                                         R &vx, R &vy) {
     if constexpr (TP) {
-        group_value_wave<R, RE, TP>(coef_v, p, p.gdesc[g], s, hit, vx, vy);
+        group_value_wave<R, RE, TP>(coef_v, p, ldc(p.gdesc + g), s, hit, vx, vy);
     } else {            // the plain evaluator needs only the term range
         vx = R(0);
         vy = R(0);
@@ -537,10 +558,10 @@ __device__ inline R re_phase(int ph, R re, R im) {
 template <typename R>
 __device__ inline void grad_group(const KParams &p, uint32_t g, uint64_t s, bool hit, R wre, R wim) {
     if (__ballot(hit) == 0) return;
-    const uint64_t xg = p.g_x[g];
-    const uint32_t k1 = p.g_toff[g + 1];
-    for (uint32_t k = p.g_toff[g]; k < k1; ++k) {
-        const uint64_t zk = p.z[k];
+    const uint64_t xg = ldc(p.g_x + g);
+    const uint32_t k1 = ldc(p.g_toff + g + 1);
+    for (uint32_t k = ldc(p.g_toff + g); k < k1; ++k) {
+        const uint64_t zk = ldc(p.z + k);
         const int ph = __popcll(xg & zk) & 3;
         R val = hit ? re_phase(ph, wre, wim) : R(0);
         if (__popcll(s & zk) & 1) val = -val;
@@ -564,9 +585,9 @@ __device__ inline void group_coef_dual(const KParams &p, uint32_t k0, uint32_t k
         float2 cl[TERM_PAD];
 #pragma unroll
         for (int u = 0; u < TERM_PAD; ++u) {
-            z[u] = p.z[k + u];
-            ch[u] = c64[k + u];
-            cl[u] = c32[k + u];
+            z[u] = ldc(p.z + k + u);
+            ch[u] = ldc(c64 + k + u);
+            cl[u] = ldc(c32 + k + u);
         }
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
@@ -677,10 +698,10 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
     using V = typename VT<R, CPLX>::T;
     using CC = typename C2<R>::T;
     const int tid = threadIdx.x;
-    const ProbDesc pd = p.prob[p.tile_prob[tile]];
-    const uint32_t seg = p.tile_seg[tile];
-    const uint32_t i = p.tile_row0[tile] + tid;
-    const bool active = i < p.seg_off[seg + 1];
+    const ProbDesc pd = ldc(p.prob + ldc(p.tile_prob + tile));
+    const uint32_t seg = ldc(p.tile_seg + tile);
+    const uint32_t i = ldc(p.tile_row0 + tile) + tid;
+    const bool active = i < ldc(p.seg_off + seg + 1);
     const uint64_t s = active ? p.keys[i] : 0ull;
     const uint32_t slo = static_cast<uint32_t>(s) & pd.lomask;
     const V *x = static_cast<const V *>(xin);
@@ -773,10 +794,10 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
 
     // the classes whose target segment exists for this tile's segment, in class
     // order, with that segment (listed once per plan by k_seg_targets)
-    const uint32_t e1 = p.st_off[seg + 1];
-    for (uint32_t e = p.st_off[seg]; e < e1; ++e) {
+    const uint32_t e1 = ldc(p.st_off + seg + 1);
+    for (uint32_t e = ldc(p.st_off + seg); e < e1; ++e) {
         {
-            const TargetDesc td = p.st_list[e];
+            const TargetDesc td = ldc(p.st_list + e);
             if constexpr (MODE == MODE_COUNT) n_look += (tid == 0);
             const uint32_t ta = td.ta, tn = td.tn;
             const bool hashed = p.h_off != nullptr;          // the same for the whole plan
@@ -866,7 +887,7 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
                         if (hit) fma_acc(sec_a ? acc_a : acc, c, x[ta + base]);
                     }
                 } else if constexpr (MODE == MODE_FILL) {
-                    consume(g, base, hit, sec_a, value(p.gdesc[g], base, hit));
+                    consume(g, base, hit, sec_a, value(ldc(p.gdesc + g), base, hit));
                 } else if constexpr (MODE == MODE_FILLCOL) {
                     if (hit) {
                         const int32_t c = static_cast<int32_t>(ta + base);
@@ -903,7 +924,7 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
             uint32_t g = g0;
             // groups are sorted by X mask, so a class's group without low flips
             // (section A) is its first one
-            if (g < g1 && p.g_xlo[g] == 0) {
+            if (g < g1 && ldc(p.g_xlo + g) == 0) {
                 uint32_t base;
                 const bool f = search(slo, base);
                 handle(g, base, f, true);
@@ -918,19 +939,19 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
                 GroupDesc d0{}, d1{}, d2{}, d3{};
                 uint32_t t0, t1, t2, t3;
                 if constexpr (ILP) {
-                    d0 = p.gdesc[g];
-                    d1 = p.gdesc[g + 1];
-                    d2 = p.gdesc[g + 2];
-                    d3 = p.gdesc[g + 3];
+                    d0 = ldc(p.gdesc + g);
+                    d1 = ldc(p.gdesc + g + 1);
+                    d2 = ldc(p.gdesc + g + 2);
+                    d3 = ldc(p.gdesc + g + 3);
                     t0 = slo ^ (static_cast<uint32_t>(d0.x) & pd.lomask);
                     t1 = slo ^ (static_cast<uint32_t>(d1.x) & pd.lomask);
                     t2 = slo ^ (static_cast<uint32_t>(d2.x) & pd.lomask);
                     t3 = slo ^ (static_cast<uint32_t>(d3.x) & pd.lomask);
                 } else {
-                    t0 = slo ^ p.g_xlo[g];
-                    t1 = slo ^ p.g_xlo[g + 1];
-                    t2 = slo ^ p.g_xlo[g + 2];
-                    t3 = slo ^ p.g_xlo[g + 3];
+                    t0 = slo ^ ldc(p.g_xlo + g);
+                    t1 = slo ^ ldc(p.g_xlo + g + 1);
+                    t2 = slo ^ ldc(p.g_xlo + g + 2);
+                    t3 = slo ^ ldc(p.g_xlo + g + 3);
                 }
                 uint32_t b0 = 0, b1 = 0, b2 = 0, b3 = 0;
                 bool f0, f1, f2, f3;
@@ -970,7 +991,7 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
             }
             for (; g < g1; ++g) {
                 uint32_t base;
-                const bool f = search(slo ^ p.g_xlo[g], base);
+                const bool f = search(slo ^ ldc(p.g_xlo + g), base);
                 handle(g, base, f, false);
             }
             __syncthreads();          // before the next target overwrites s_keys
