@@ -1824,15 +1824,14 @@ __global__ __launch_bounds__(256) void k_spmv_sell(const int32_t *__restrict__ c
     const int64_t row = static_cast<int64_t>(spmv_block()) * blockDim.x + threadIdx.x;
     if (row >= n_states) return;
     // a wave's 64 rows are one slice (256-thread blocks start on slice bounds)
-    const int64_t slice = row / SLICE;
+    const int64_t slice = uniform64(row / SLICE);   // per-slice tables: scalar loads
     const int lane = static_cast<int>(row % SLICE);
     const V *x = static_cast<const V *>(xin);
     V acc = vzero(static_cast<V *>(nullptr));
     if (diag) fma_acc(acc, static_cast<const CC *>(diag)[row], x[row]);   // NULL: no problem has an x = 0 group
-    const uint32_t w_a = __builtin_amdgcn_readfirstlane(wa[slice]);
-    const uint32_t w_b = __builtin_amdgcn_readfirstlane(wb[slice]);
-    const int64_t bo = uniform64(boff[slice]), avo = uniform64(avoff[slice]);
-    const int64_t co = uniform64(coff[slice]), ao = uniform64(aoff[slice]);
+    const uint32_t w_a = ldc(wa + slice), w_b = ldc(wb + slice);
+    const int64_t bo = ldc(boff + slice), avo = ldc(avoff + slice);
+    const int64_t co = ldc(coff + slice), ao = ldc(aoff + slice);
     V acc_a = vzero(static_cast<V *>(nullptr));
     constexpr int EV = pack_sub<CV>();
     const uint32_t n_stored = spmv_section_a<U>(acc_a, col + co + lane, static_cast<const CV *>(vala) + avo + lane,
