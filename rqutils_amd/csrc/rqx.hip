@@ -2402,6 +2402,23 @@ hipError_t launch_dtype(const rqx_plan *p, const KParams &k, const void *x, void
     }
 }
 
+// host threads for the O(N) passes of a large subspace's index build (1 below 2^18 rows)
+unsigned index_threads(int64_t n) {
+    return n >= (int64_t(1) << 18) ? std::max(1u, std::min(16u, std::thread::hardware_concurrency())) : 1u;
+}
+
+// run f(w, begin, end) over nt contiguous chunks w of [0, n), one host thread each
+template <typename F> void parallel_chunks(int64_t n, unsigned nt, F f) {
+    if (nt <= 1) {
+        f(0u, int64_t(0), n);
+        return;
+    }
+    std::vector<std::thread> pool;
+    for (unsigned w = 1; w < nt; ++w) pool.emplace_back(f, w, n * w / nt, n * (w + 1) / nt);
+    f(0u, int64_t(0), n / nt);
+    for (std::thread &t : pool) t.join();
+}
+
 int validate_problem(const ProblemIn &q, int k) {
     const std::string tag = "problem " + std::to_string(k) + ": ";
     if (q.n_qubits < 1 || q.n_qubits > 64) return fail(RQX_ERR_ARG, tag + "n_qubits must be in [1, 64]");
@@ -2424,33 +2441,29 @@ int validate_problem(const ProblemIn &q, int k) {
     }
     for (int64_t t = 0; t < q.n_terms; ++t)
         if (q.zmask[t] & ~qmask) return fail(RQX_ERR_ARG, tag + "zmask has bits beyond n_qubits");
-    for (int64_t i = 0; i < q.n_states; ++i) {
-        if (q.states[i] & ~qmask) return fail(RQX_ERR_ARG, tag + "state has bits beyond n_qubits");
-        if (i > 0 && q.states[i] <= q.states[i - 1])
-            return fail(RQX_ERR_ARG, tag + "states must be sorted ascending and unique");
+    // the first bad state in row order (chunks on host threads for large subspaces)
+    const unsigned nt = index_threads(q.n_states);
+    std::vector<int64_t> bad_at(nt, INT64_MAX);
+    std::vector<int> bad_kind(nt, 0);
+    parallel_chunks(q.n_states, nt, [&](unsigned w, int64_t b, int64_t e) {
+        for (int64_t i = b; i < e; ++i) {
+            const int kind = (q.states[i] & ~qmask) ? 1 : (i > 0 && q.states[i] <= q.states[i - 1]) ? 2 : 0;
+            if (kind) {
+                bad_at[w] = i;
+                bad_kind[w] = kind;
+                return;
+            }
+        }
+    });
+    for (unsigned w = 0; w < nt; ++w) {
+        if (bad_kind[w] == 1) return fail(RQX_ERR_ARG, tag + "state has bits beyond n_qubits");
+        if (bad_kind[w] == 2) return fail(RQX_ERR_ARG, tag + "states must be sorted ascending and unique");
     }
     return RQX_OK;
 }
 
 // Split choice: largest L whose segments fit SEG_CAP, then the cheapest of the
 // candidates below it by the sampled cost model (or the caller's L).
-// host threads for the O(N) passes of a large subspace's index build (1 below 2^18 rows)
-unsigned index_threads(int64_t n) {
-    return n >= (int64_t(1) << 18) ? std::max(1u, std::min(16u, std::thread::hardware_concurrency())) : 1u;
-}
-
-// run f(w, begin, end) over nt contiguous chunks w of [0, n), one host thread each
-template <typename F> void parallel_chunks(int64_t n, unsigned nt, F f) {
-    if (nt <= 1) {
-        f(0u, int64_t(0), n);
-        return;
-    }
-    std::vector<std::thread> pool;
-    for (unsigned w = 1; w < nt; ++w) pool.emplace_back(f, w, n * w / nt, n * (w + 1) / nt);
-    f(0u, int64_t(0), n / nt);
-    for (std::thread &t : pool) t.join();
-}
-
 // max_segment over chunks: each chunk's first and last segment start and its
 // longest segment between them, joined in order
 int max_segment_par(const std::vector<int8_t> &hd, int L, unsigned nt) {
@@ -2556,7 +2569,7 @@ int build_plan(rqx_plan **out, const std::vector<ProblemIn> &in, int64_t n_total
     // host arrays of the whole plan
     std::vector<uint32_t> lo(static_cast<size_t>(N)), seg_off_all, cls_goff_all, g_xlo, g_toff, tile_row0, tile_seg,
         tile_prob;
-    std::vector<uint64_t> keys(static_cast<size_t>(N)), seg_hi_all, cls_hi_all, g_x, zall;
+    std::vector<uint64_t> seg_hi_all, cls_hi_all, g_x, zall;   // (keys are uploaded from the caller's arrays)
     std::vector<double> c64;
     std::vector<float2> c32;
     std::vector<ProbDesc> desc(static_cast<size_t>(std::max(K, 1)));
@@ -2598,10 +2611,9 @@ int build_plan(rqx_plan **out, const std::vector<ProblemIn> &in, int64_t n_total
         ph.cls_base = static_cast<int32_t>(cls_hi_all.size());
         ph.n_classes = static_cast<int32_t>(sg.cls_hi.size());
         const uint32_t lomask = (sg.L >= 32) ? 0xffffffffu : ((1u << sg.L) - 1u);
-        for (int64_t i = 0; i < q.n_states; ++i) {
-            keys[row_base + i] = q.states[i];
-            lo[row_base + i] = static_cast<uint32_t>(q.states[i]) & lomask;
-        }
+        parallel_chunks(q.n_states, index_threads(q.n_states), [&](unsigned, int64_t b, int64_t e) {
+            for (int64_t i = b; i < e; ++i) lo[row_base + i] = static_cast<uint32_t>(q.states[i]) & lomask;
+        });
         for (int32_t m = 0; m < ph.n_segs; ++m) {
             seg_hi_all.push_back(sg.seg_hi[m]);
             seg_off_all.push_back(static_cast<uint32_t>(row_base + sg.seg_off[m]));
@@ -2763,6 +2775,7 @@ int build_plan(rqx_plan **out, const std::vector<ProblemIn> &in, int64_t n_total
     p->h_tab = reinterpret_cast<uint32_t *>(b + o_htab);
     rqx_plan *raw = p.get();
     auto release = [&]() {
+        (void)hipStreamSynchronize(st);   // no copy or kernel of the plan still in flight
         (void)hipFree(raw->block);
         (void)hipFree(raw->st_block);
     };
@@ -2772,7 +2785,7 @@ int build_plan(rqx_plan **out, const std::vector<ProblemIn> &in, int64_t n_total
         return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st);
     };
     const hipError_t errs[] = {
-        up(p->keys, keys.data(), n * 8), up(p->lo, lo.data(), n * 4),
+        up(p->lo, lo.data(), n * 4),
         up(p->seg_hi, seg_hi_all.data(), seg_hi_all.size() * 8), up(p->seg_off, seg_off_all.data(), seg_off_all.size() * 4),
         up(p->cls_hi, cls_hi_all.data(), cls_hi_all.size() * 8), up(p->cls_goff, cls_goff_all.data(), cls_goff_all.size() * 4),
         up(p->g_xlo, g_xlo.data(), g_xlo.size() * 4), up(p->g_toff, g_toff.data(), g_toff.size() * 4),
@@ -2784,6 +2797,13 @@ int build_plan(rqx_plan **out, const std::vector<ProblemIn> &in, int64_t n_total
         up(p->g_zor, g_zor.data(), g_zor.size() * 8), up(p->gdesc, gdesc.data(), gdesc.size() * sizeof(GroupDesc)),
         up(p->h_off, h_off.data(), h_off.size() * 4), up(p->h_log, h_log.data(), h_log.size() * 4),
         up(p->h_tab, h_tab.data(), h_tab.size() * 4)};
+    std::vector<hipError_t> errs_keys;   // the keys straight from the callers' arrays (read before return)
+    for (int k = 0; k < K; ++k) errs_keys.push_back(up(p->keys + prob_rows[k], in[k].states, static_cast<size_t>(in[k].n_states) * 8));
+    for (hipError_t ei : errs_keys)
+        if (ei != hipSuccess) {
+            release();
+            return fail(RQX_ERR_HIP, std::string("hipMemcpyAsync(plan): ") + hipGetErrorString(ei));
+        }
     for (hipError_t ei : errs)
         if (ei != hipSuccess) {
             release();
