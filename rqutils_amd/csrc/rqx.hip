@@ -201,8 +201,9 @@ struct KParams {
     uint32_t *rowcnt;              // row-count mode: section-B hits per row
     uint32_t *rowcnt_a;            // row-count mode: section-A hits per row
     // hit cache passes.  Column pass: columns of the full packed layout (slice s
-    // from c_off[s], section A then B) and the group of each section-A entry
-    // (-1 on padding; level k of slice s at (c_aoff[s] + k) * 64 + lane).  Value
+    // from c_off[s], section A then B), the group of each section-A entry (-1 on
+    // padding; level k of slice s at (c_aoff[s] + k) * 64 + lane) and the values
+    // of section B (as in the value pass).  Value
     // pass: values straight into the stored layout (section B packed from
     // c_boff[s]; section-A level k by its code c_acode[c_aoff[s] + k]: -1 for one
     // value at c_aval[c_aoff[s] + k], else its index j among the slice's levels
@@ -223,6 +224,7 @@ struct KParams {
     void *c_aval_lo;
     void *c_vala_lo;
     const void *coef_lo;           // with c_val_lo: FP32 coefficients
+    int32_t c_a_only;              // value pass: section A only (the column pass wrote section B's values)
     unsigned long long *coo_count; // coo mode: entries claimed so far
     int64_t *coo_row;              // coo mode: (row, column, element) of every hit, in claim order
     int32_t *coo_col;
@@ -722,6 +724,7 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
             fill_off = p.c_off[i / SLICE];
             fill_wa = p.c_wa[i / SLICE];
             fill_ao = p.c_aoff[i / SLICE];
+            fill_bo = p.c_boff[i / SLICE];
         }
     } else if constexpr (MODE == MODE_FILL) {
         if (active) {
@@ -746,7 +749,7 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
         if (p.c_val_lo) static_cast<FL *>(code < 0 ? p.c_aval_lo : p.c_vala_lo)[o] = l;
     };
     // section B: append entry k of this row; a whole packet goes to memory
-    // (columns in the column pass, values in the value pass)
+    // (columns and values in the column pass, values in a full value pass)
     auto fill_push = [&](FillBuf<FV, FL> &b, uint32_t k, int32_t c, FV v, FL l) {
 #pragma unroll
         for (int u = 0; u + 1 < PACK; ++u) {
@@ -759,12 +762,9 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
         b.l[PACK - 1] = l;
         if (k % PACK == PACK - 1) {
             const uint32_t kb = k - (PACK - 1);
-            if constexpr (MODE == MODE_FILLCOL) {
-                store_packet<int32_t>(p.c_col, fill_off, fill_wa + kb, ln, b.c);
-            } else {
-                store_packet<FV>(p.c_val, fill_bo, kb, ln, b.v);
-                if (p.c_val_lo) store_packet<FL>(p.c_val_lo, fill_bo, kb, ln, b.l);
-            }
+            if constexpr (MODE == MODE_FILLCOL) store_packet<int32_t>(p.c_col, fill_off, fill_wa + kb, ln, b.c);
+            store_packet<FV>(p.c_val, fill_bo, kb, ln, b.v);
+            if (p.c_val_lo) store_packet<FL>(p.c_val_lo, fill_bo, kb, ln, b.l);
         }
     };
     V gv = vzero(static_cast<V *>(nullptr));
@@ -836,7 +836,7 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
                 if constexpr (MODE == MODE_MATVEC) {
                     group_value_wave<R, REALC, TP>(p.coef, p, d, s, hit, h.vx, h.vy);
                     if (hit) h.xv = x[ta + base];
-                } else if constexpr (MODE == MODE_FILL) {
+                } else if constexpr (MODE == MODE_FILL || MODE == MODE_FILLCOL) {
                     if (p.c_val_lo) group_value_wave_dual<!CPLX>(p, d, s, hit, h.vh, h.vl);   // both precisions, R = double
                     else group_value_wave<R, !CPLX>(p.coef, p, d, s, hit, h.vx, h.vy);
                 }
@@ -852,7 +852,7 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
                         c.y = h.vy;
                         if (hit) fma_acc(sec_a ? acc_a : acc, c, h.xv);
                     }
-                } else if constexpr (MODE == MODE_FILL) {
+                } else if constexpr (MODE == MODE_FILL || MODE == MODE_FILLCOL) {
                     if (hit) {
                         FV v;
                         FL l;
@@ -863,11 +863,11 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
                             v = p.c_val_lo ? static_cast<R>(h.vh.x) : h.vx;
                             l = h.vl.x;
                         }
-                        if (sec_a) {
+                        if (sec_a) {                   // value pass only
                             fill_put(static_cast<uint32_t>(n_hit_a), v, l);
                             n_hit_a += 1;
                         } else {
-                            fill_push(fill_b, static_cast<uint32_t>(n_hit), 0, v, l);
+                            fill_push(fill_b, static_cast<uint32_t>(n_hit), static_cast<int32_t>(ta + base), v, l);
                             n_hit += 1;
                         }
                     }
@@ -889,15 +889,14 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
                 } else if constexpr (MODE == MODE_FILL) {
                     consume(g, base, hit, sec_a, value(ldc(p.gdesc + g), base, hit));
                 } else if constexpr (MODE == MODE_FILLCOL) {
-                    if (hit) {
-                        const int32_t c = static_cast<int32_t>(ta + base);
-                        if (sec_a) {
-                            fill_put_col(static_cast<uint32_t>(n_hit_a), c, static_cast<int32_t>(g));
+                    if (sec_a) {                       // columns and groups; values in the value pass
+                        if (hit) {
+                            fill_put_col(static_cast<uint32_t>(n_hit_a), static_cast<int32_t>(ta + base),
+                                         static_cast<int32_t>(g));
                             n_hit_a += 1;
-                        } else {
-                            fill_push(fill_b, static_cast<uint32_t>(n_hit), c, FV{}, FL{});
-                            n_hit += 1;
                         }
+                    } else {
+                        consume(g, base, hit, false, value(ldc(p.gdesc + g), base, hit));
                     }
                 } else if constexpr (MODE == MODE_LOOKUP) {
                     if (hit) {
@@ -930,12 +929,15 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
                 handle(g, base, f, true);
                 ++g;
             }
+            if constexpr (MODE == MODE_FILL) {
+                if (p.c_a_only) g = g1;       // section B was filled by the column pass
+            }
             // four independent searches in lock step: the dependent LDS loads of
             // one search overlap with those of the other three
             for (; g + 4 <= g1; g += 4) {
                 // the full evaluator fetches the four groups' descriptors at once,
                 // ahead of the searches (their X masks give the targets)
-                constexpr bool ILP = (MODE == MODE_MATVEC || MODE == MODE_FILL) && TP;
+                constexpr bool ILP = (MODE == MODE_MATVEC || MODE == MODE_FILL || MODE == MODE_FILLCOL) && TP;
                 GroupDesc d0{}, d1{}, d2{}, d3{};
                 uint32_t t0, t1, t2, t3;
                 if constexpr (ILP) {
@@ -1016,8 +1018,9 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
                 if constexpr (MODE == MODE_FILLCOL) fill_put_col(k, static_cast<int32_t>(i), -1);
                 else fill_put(k, v0, l0);
             }
-            for (uint32_t k = static_cast<uint32_t>(n_hit); k < wb; ++k)
-                fill_push(fill_b, k, static_cast<int32_t>(i), v0, l0);
+            if (!(MODE == MODE_FILL && p.c_a_only))
+                for (uint32_t k = static_cast<uint32_t>(n_hit); k < wb; ++k)
+                    fill_push(fill_b, k, static_cast<int32_t>(i), v0, l0);
         }
     } else if constexpr (MODE == MODE_COUNT) {
         // wave reduction, one atomic per wave and counter (the class lookups are
@@ -1038,7 +1041,7 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
 // the fill pass with real values is register-bound (both precisions' values,
 // packet buffers): six waves per SIMD (a few registers spill) measured 6%
 // faster than five; complex values would spill too much
-template <int MODE, bool CPLX> constexpr int FILL_WAVES = (MODE == MODE_FILL && !CPLX) ? 6 : 1;
+template <int MODE, bool CPLX> constexpr int FILL_WAVES = ((MODE == MODE_FILL || MODE == MODE_FILLCOL) && !CPLX) ? 6 : 1;
 
 // One workgroup per tile; tiles of one XCD are contiguous.
 template <int MODE, typename R, bool CPLX, bool REALC = false, bool TP = true>
@@ -2870,22 +2873,33 @@ hipError_t release_cache(rqx_plan *p, int slot) {
 // the column pass: columns in the full packed layout (slice s from off[s]) and
 // the group of each section-A entry (level-major), in the order the tile kernel
 // accumulates the hits
-hipError_t fill_columns(const rqx_plan *p, const HitCache &c, const int64_t *off, int32_t *col_full, int32_t *gid_a,
-                        hipStream_t st) {
-    KParams k = make_params(p, false);
+hipError_t fill_columns(const rqx_plan *p, int slot, const HitCache &c, const int64_t *off, int32_t *col_full,
+                        int32_t *gid_a, void *valb_lo, hipStream_t st) {
+    KParams k = make_params(p, slot == 1);
     k.c_col = col_full;
     k.c_gid = gid_a;
     k.c_off = off;
     k.c_wa = c.wa;
     k.c_wb = c.wb;
     k.c_aoff = c.aoff;
-    return launch_tiles<MODE_FILLCOL, float, false>(p, k, nullptr, nullptr, st);
+    k.c_boff = c.boff;
+    k.c_val = c.valb;
+    if (valb_lo) {
+        k.c_val_lo = valb_lo;
+        k.coef_lo = p->coef32;
+    }
+    if (slot == 1)
+        return c.cplx ? launch_tiles<MODE_FILLCOL, double, true>(p, k, nullptr, nullptr, st)
+                      : launch_tiles<MODE_FILLCOL, double, false>(p, k, nullptr, nullptr, st);
+    return c.cplx ? launch_tiles<MODE_FILLCOL, float, true>(p, k, nullptr, nullptr, st)
+                  : launch_tiles<MODE_FILLCOL, float, false>(p, k, nullptr, nullptr, st);
 }
 
 // the value pass: the values of the cache in slot (both precisions in one pass
 // when they were built together) from the current coefficients, straight into
-// the stored layout; it needs no memory beyond the cache
-hipError_t fill_values(const rqx_plan *p, int slot, hipStream_t st) {
+// the stored layout (a_only: section A only, after a column pass); it needs no
+// memory beyond the cache
+hipError_t fill_values(const rqx_plan *p, int slot, hipStream_t st, bool a_only = false) {
     if (p->cache[slot].shared) slot = 1;
     const HitCache &c = p->cache[slot];
     KParams k = make_params(p, slot == 1);
@@ -2898,6 +2912,7 @@ hipError_t fill_values(const rqx_plan *p, int slot, hipStream_t st) {
     k.c_val = c.valb;
     k.c_aval = c.aval;
     k.c_vala = c.vala;
+    k.c_a_only = a_only ? 1 : 0;
     if (c.shared) {
         const HitCache &lo = p->cache[0];
         k.c_val_lo = lo.valb;
@@ -3325,8 +3340,8 @@ int rqx_plan_build_cache(rqx_plan *p, int32_t precision, int64_t max_bytes, rqx_
     if (e == hipSuccess) e = hipMemcpyAsync(c.wa, wa.data(), wa.size() * 4, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipMemcpyAsync(c.wb, wb.data(), wb.size() * 4, hipMemcpyHostToDevice, st);
     if (e != hipSuccess) return fail_build(e, "cache allocation: ");
-    // pass 2: columns of every hit and groups of the section-A hits
-    e = fill_columns(p, c, off_d, col_full, gid_a, st);
+    // pass 2: columns of every hit, groups of the section-A hits, section-B values
+    e = fill_columns(p, slot, c, off_d, col_full, gid_a, both ? static_cast<void *>(b + o_valb_lo) : nullptr, st);
     if (e != hipSuccess) return fail_build(e, "cache columns: ");
     // section-A level codes: offsets and single values
     const unsigned wave_blocks = static_cast<unsigned>((n_slices * SLICE + 255) / 256);
@@ -3377,8 +3392,8 @@ int rqx_plan_build_cache(rqx_plan *p, int32_t precision, int64_t max_bytes, rqx_
                            c.adel, c.coff, n, n_slices, c.col);
         e = hipGetLastError();
     }
-    // pass 3: the values, straight into the stored layout
-    if (e == hipSuccess) e = fill_values(p, slot, st);
+    // pass 3: the section-A values, straight into the stored layout
+    if (e == hipSuccess) e = fill_values(p, slot, st, true);
     if (e == hipSuccess) e = hipStreamSynchronize(st);   // host vectors and the scratch arrays are released below
     if (e != hipSuccess) return fail_build(e, "cache columns and values: ");
     release_scratch();
