@@ -104,6 +104,14 @@ def test_product_subspace_levels_store_offsets():
     print(f"first-section levels {info['a_levels']}, stored columns {stored_a}, stored values {valued_a}", flush=True)
     assert info['a_levels'] > 0 and stored_a < info['a_levels'] and valued_a < info['a_levels']
     _check_identical(plan, seed=19)
+    # new coefficients rewrite the single and stored values in place (a full value pass)
+    rng = np.random.default_rng(20)
+    plan.set_alphas(torch.from_numpy(rng.standard_normal(plan.info['n_terms'])).to(DEV))
+    _check_identical(plan, seed=21)
+    plan.build_cache(0)                      # both precisions: section B in the column pass, A after
+    _check_identical(plan, seed=22)
+    plan.set_alphas(torch.from_numpy(rng.standard_normal(plan.info['n_terms'])).to(DEV))
+    _check_identical(plan, seed=23)
 
 
 def test_cached_batch_matvec_identical(cases):
