@@ -2450,13 +2450,11 @@ int validate_problem(const ProblemIn &q, int k) {
         if (q.zmask[t] & ~qmask) return fail(RQX_ERR_ARG, tag + "zmask has bits beyond n_qubits");
     // the first bad state in row order (chunks on host threads for large subspaces)
     const unsigned nt = index_threads(q.n_states);
-    std::vector<int64_t> bad_at(nt, INT64_MAX);
-    std::vector<int> bad_kind(nt, 0);
+    std::vector<int> bad_kind(nt, 0);   // per chunk: its first bad state's fault
     parallel_chunks(q.n_states, nt, [&](unsigned w, int64_t b, int64_t e) {
         for (int64_t i = b; i < e; ++i) {
             const int kind = (q.states[i] & ~qmask) ? 1 : (i > 0 && q.states[i] <= q.states[i - 1]) ? 2 : 0;
             if (kind) {
-                bad_at[w] = i;
                 bad_kind[w] = kind;
                 return;
             }

@@ -187,6 +187,25 @@ def test_edge_cases():
     assert torch.count_nonzero(plan.matvec(torch.ones(8, dtype=torch.float64, device=DEV))) == 0
 
 
+def test_plan_state_errors_large_subspace():
+    """Subspaces of 2^18 states or more are checked on host threads: the first bad state in row
+    order still decides the error."""
+    t = TermTables.from_masks(np.array([0], np.uint64), np.array([1], np.uint64), np.array([1.]), 20)
+    big = np.arange(1 << 19, dtype=np.uint64)
+    bad = big.copy()
+    bad[400000] = bad[399999]                 # first: a repeated state
+    bad[450000] = np.uint64(1) << np.uint64(25)   # later: bits beyond 20 qubits
+    with pytest.raises(ValueError, match='sorted'):
+        Plan(t, bad)
+    bad = big.copy()
+    bad[100] = np.uint64(1) << np.uint64(21)  # first: bits beyond (row 101 is then out of order too)
+    bad[300000] = 5
+    with pytest.raises(ValueError, match='beyond'):
+        Plan(t, bad)
+    plan = Plan(t, big)
+    assert plan.n_states == big.shape[0]
+
+
 def test_plan_argument_errors():
     t = TermTables.from_masks(np.array([0], np.uint64), np.array([1], np.uint64), np.array([1.]), 4)
     with pytest.raises(ValueError, match='sorted'):
