@@ -1043,13 +1043,18 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
 // spill) measured 6% faster than five; complex values would spill too much.
 // The matvec is bound by the latency of its loads: asking for eight waves per
 // SIMD (the compiler then keeps within 64 VGPRs and 96 SGPRs) measured 3-6%
-// faster with the full evaluator (seven: the same) and 11% with the plain one.
-template <int MODE, bool CPLX>
-constexpr int TILE_WAVES = ((MODE == MODE_FILL || MODE == MODE_FILLCOL) && !CPLX) ? 6 : (MODE == MODE_MATVEC ? 8 : 1);
+// faster with the full evaluator (seven: the same) and 11% with the plain one;
+// the FP64 complex-coefficient evaluator would spill at eight and runs at six
+// (6% faster than at eight).
+template <int MODE, typename R, bool CPLX, bool REALC>
+constexpr int TILE_WAVES = ((MODE == MODE_FILL || MODE == MODE_FILLCOL) && !CPLX) ? 6
+                           : MODE != MODE_MATVEC                                  ? 1
+                           : (sizeof(R) == 8 && CPLX && !REALC)                   ? 6   // would spill at 8
+                                                                                  : 8;
 
 // One workgroup per tile; tiles of one XCD are contiguous.
 template <int MODE, typename R, bool CPLX, bool REALC = false, bool TP = true>
-__global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(TILE_WAVES<MODE, CPLX>))) void k_tiles(const KParams p, const void *__restrict__ xin,
+__global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(TILE_WAVES<MODE, R, CPLX, REALC>))) void k_tiles(const KParams p, const void *__restrict__ xin,
                                                 void *__restrict__ yout) {
     extern __shared__ uint32_t s_keys[];      // staged target segment (max_segment keys)
     process_tile<MODE, R, CPLX, REALC, TP>(p, xcd_remap(blockIdx.x, gridDim.x), s_keys, xin, yout);
@@ -1058,7 +1063,7 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(TILE_WAVES
 // Persistent workgroups pull tiles from a queue (tiles are stored largest
 // first); every workgroup leaves when the queue is exhausted.
 template <int MODE, typename R, bool CPLX, bool REALC = false, bool TP = true>
-__global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(TILE_WAVES<MODE, CPLX>))) void k_tiles_queue(const KParams p, const void *__restrict__ xin,
+__global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(TILE_WAVES<MODE, R, CPLX, REALC>))) void k_tiles_queue(const KParams p, const void *__restrict__ xin,
                                                       void *__restrict__ yout) {
     extern __shared__ uint32_t s_keys[];
     __shared__ uint32_t s_tile;
