@@ -261,8 +261,9 @@ int rqx_plan_cache_info(const rqx_plan *plan, int32_t precision, int64_t *out);
 /* Whether rqx_matvec uses a built cache (default 1); 0 forces the matrix-free kernel. */
 int rqx_plan_use_cache(rqx_plan *plan, int32_t enable);
 
-/* doubles of per-problem work space of rqx_locg_step */
-enum { RQX_LOCG_WORK = 168 };
+/* doubles of per-problem work space of rqx_locg_step, and the slot of it that the
+ * step reads but never writes: the scale of the relative residual (below) */
+enum { RQX_LOCG_WORK = 168, RQX_LOCG_SCALE = 164 };
 
 /*
  * One LOCG iteration for every problem of the plan (a batched plan, or a single
@@ -281,9 +282,12 @@ enum { RQX_LOCG_WORK = 168 };
  *   vec: device vectors {x, Ax, y, Ay, p, Ap, p_low, Ap_low} over all rows
  *     (p_low, Ap_low only with low); x, Ax, y, Ay are updated in place;
  *   theta, rel: device, one double per problem: <x, A x> and |A x - theta x| /
- *     |theta| at the start of the step; active: device, one byte per problem,
- *     cleared when rel <= tol; an inactive problem's x and A x do not change;
- *   work: device, RQX_LOCG_WORK doubles per problem.
+ *     max(|theta|, scale) at the start of the step; active: device, one byte per
+ *     problem, cleared when rel <= tol; an inactive problem's x and A x do not change;
+ *   work: device, RQX_LOCG_WORK doubles per problem; problem k's
+ *     work[k * RQX_LOCG_WORK + RQX_LOCG_SCALE] is the caller's scale (e.g. |A x0|
+ *     for the normalised start vector x0, so that a zero eigenvalue still
+ *     converges; 0: |theta| alone), the rest is scratch.
  */
 int rqx_locg_step(const rqx_plan *plan, int32_t dtype, int32_t low, void *const *vec, double *theta, double *rel,
                   uint8_t *active, double *work, double tol, rqx_stream_t stream);

@@ -139,3 +139,94 @@ def matvec_u64(xmasks, groups, keys, vec):
         diag = diagonal_u64(zs, cs, keys)
         out += apply_xgrp(src, diag, vec)
     return out
+
+
+def _group_values(zmask, coef, toff, group, states):
+    """``C_j(s) = sum_k c_k (1 - 2 (popc(s & z_k) & 1))`` (``sqd.py:603-628``) for
+    many (group, state) pairs at once: terms ``toff[j]..toff[j+1]`` of group ``j``."""
+    zmask = np.asarray(zmask, dtype=np.uint64)
+    coef = np.asarray(coef, dtype=np.complex128)
+    toff = np.asarray(toff, dtype=np.int64)
+    group = np.asarray(group, dtype=np.int64)
+    states = np.asarray(states, dtype=np.uint64)
+    out = np.zeros(group.shape[0], dtype=np.complex128)
+    cnt = toff[group + 1] - toff[group]
+    ends = np.cumsum(cnt)
+    p0 = 0
+    while p0 < group.shape[0]:
+        # pairs [p0, p1) with at most ~4M term evaluations between them
+        p1 = max(p0 + 1, int(np.searchsorted(ends, (ends[p0 - 1] if p0 else 0) + (1 << 22), side='right')))
+        g, c, st = group[p0:p1], cnt[p0:p1], states[p0:p1]
+        pair = np.repeat(np.arange(g.shape[0]), c)
+        first = np.repeat(toff[g] - np.concatenate([[0], np.cumsum(c)[:-1]]), c)
+        term = np.arange(pair.shape[0]) + first
+        par = np.bitwise_count(st[pair] & zmask[term]) & 1
+        np.add.at(out, pair + p0, coef[term] * (1. - 2. * par))
+        p0 = p1
+    return out
+
+
+def hits_u64(xmask, grp_off, zmask, coef, keys, g0=0, g1=None):
+    """Every nonzero of groups ``g0..g1`` of the projected Hamiltonian as triplets.
+
+    The matrix form of ``apply_h_x_cached`` (``sqd.py:694-709``): group ``j``
+    puts ``C_j(s_i)`` at ``(i, rank(s_i ^ x_j))`` for each row whose flipped
+    state is in the subspace (``get_xsource``, here ``np.searchsorted``).
+    Returns ``(rows int64, cols int64, vals complex128)``.
+    """
+    keys = np.asarray(keys, dtype=np.uint64)
+    xmask = np.asarray(xmask, dtype=np.uint64)
+    g1 = xmask.shape[0] if g1 is None else g1
+    zmask = np.asarray(zmask, dtype=np.uint64)
+    coef = np.asarray(coef, dtype=np.complex128)
+    rows, cols, vals = [], [], []
+    for j in range(g0, g1):
+        src = xsource_searchsorted(keys, xmask[j])
+        r = np.nonzero(src >= 0)[0]
+        s = keys[r]
+        v = np.zeros(r.shape[0], dtype=np.complex128)
+        for k in range(grp_off[j], grp_off[j + 1]):
+            v += coef[k] * (1. - 2. * (np.bitwise_count(s & zmask[k]) & 1))
+        rows.append(r)
+        cols.append(src[r])
+        vals.append(v)
+    rows = np.concatenate(rows) if rows else np.zeros(0, np.int64)
+    cols = np.concatenate(cols) if cols else np.zeros(0, np.int64)
+    vals = np.concatenate(vals) if vals else np.zeros(0, np.complex128)
+    return rows.astype(np.int64), cols.astype(np.int64), vals
+
+
+def csr_u64(xmask, grp_off, zmask, coef, keys, real=None):
+    """Projected Hamiltonian ``H_S`` as a SciPy CSR matrix (duplicates summed).
+
+    ``real`` (default: every coefficient real) stores float64 values.
+    """
+    import scipy.sparse
+    n = np.asarray(keys).shape[0]
+    rows, cols, vals = hits_u64(xmask, grp_off, zmask, coef, keys)
+    if real is None:
+        real = bool(np.all(np.asarray(coef).imag == 0)) if np.iscomplexobj(coef) else True
+    if real:
+        vals = vals.real
+    return scipy.sparse.csr_matrix((vals, (rows, cols)), shape=(n, n))
+
+
+def matvec_rows_u64(xmask, grp_off, zmask, coef, keys, vec, rows):
+    """``(H_S v)_i`` for the listed rows only: every (row, group) pair searched with
+    ``np.searchsorted``, ``C_j(s_i)`` summed term by term (``sqd.py:603-709``)."""
+    keys = np.asarray(keys, dtype=np.uint64)
+    xmask = np.asarray(xmask, dtype=np.uint64)
+    rows = np.asarray(rows, dtype=np.int64)
+    n = keys.shape[0]
+    out = np.zeros(rows.shape[0], dtype=np.complex128)
+    if n == 0 or rows.size == 0 or xmask.size == 0:
+        return out
+    s = keys[rows]
+    tgt = s[:, None] ^ xmask[None, :]
+    pos = np.searchsorted(keys, tgt.reshape(-1)).reshape(tgt.shape)
+    pos_c = np.minimum(pos, n - 1)
+    hit = keys[pos_c] == tgt
+    ri, gj = np.nonzero(hit)
+    vals = _group_values(zmask, coef, grp_off, gj, s[ri])
+    np.add.at(out, ri, vals * np.asarray(vec)[pos_c[ri, gj]])
+    return out

@@ -22,6 +22,7 @@ RQX_C128 = 1
 RQX_F32 = 2
 RQX_C64 = 3
 RQX_LOCG_WORK = 168
+RQX_LOCG_SCALE = 164
 
 # every symbol declared in include/rqx.h
 EXPORTS = (

@@ -1419,10 +1419,13 @@ __device__ inline void store_c(float &o, double re, double) { o = static_cast<fl
 __device__ inline void store_c(double2 &o, double re, double im) { o = make_double2(re, im); }
 __device__ inline void store_c(float2 &o, double re, double im) { o = make_float2(static_cast<float>(re), static_cast<float>(im)); }
 
-// out[k * NP + q] += sum_{i in problem k} conj(a_q[i]) * b_q[i]   (complex double)
+// Per-block partial sums of conj(a_q[i]) * b_q[i] over the rows i of problem k:
+// part[(k * gridDim.x + blockIdx.x) * 2 * NP + {2q, 2q + 1}].  Every block writes
+// its slot (zeros when it has no rows); k_dots_finish adds the slots in block
+// order, so the sums do not depend on the order in which blocks run.
 template <typename V, int NP>
 __global__ __launch_bounds__(256) void k_batch_dots(PtrPack pk, const int64_t *__restrict__ rows,
-                                                    double *__restrict__ out) {
+                                                    double *__restrict__ part) {
     __shared__ double s_red[4][2 * NP];
     const int k = blockIdx.y;
     const int64_t r0 = rows[k], r1 = rows[k + 1];
@@ -1452,7 +1455,27 @@ __global__ __launch_bounds__(256) void k_batch_dots(PtrPack pk, const int64_t *_
     __syncthreads();
     if (threadIdx.x < 2 * NP) {
         const double v = s_red[0][threadIdx.x] + s_red[1][threadIdx.x] + s_red[2][threadIdx.x] + s_red[3][threadIdx.x];
-        if (v != 0.0) atomicAdd(&out[2 * static_cast<int64_t>(k) * NP + threadIdx.x], v);
+        part[(static_cast<int64_t>(k) * gridDim.x + blockIdx.x) * (2 * NP) + threadIdx.x] = v;
+    }
+}
+
+// Sum of the `chunks` partial slots of problem k (slot stride `width` doubles, `n`
+// values each) in a fixed order: lane l adds slots l, l + 64, ... in turn, then the
+// butterfly sum over the wave.  Lane 0 receives value t in its return.
+__device__ inline double part_sum(const double *__restrict__ part, int k, int chunks, int width, int t) {
+    const int lane = threadIdx.x & 63;
+    double acc = 0.0;
+    for (int c = lane; c < chunks; c += 64) acc += part[(static_cast<int64_t>(k) * chunks + c) * width + t];
+    return wave_sum(acc);
+}
+
+// out[2 (k NP + q) + {0, 1}] = the sums of problem k's partial slots; one wave per problem.
+__global__ __launch_bounds__(64) void k_dots_finish(const double *__restrict__ part, int chunks, int np,
+                                                   double *__restrict__ out) {
+    const int k = blockIdx.x;
+    for (int t = 0; t < 2 * np; ++t) {
+        const double v = part_sum(part, k, chunks, 2 * np, t);
+        if (threadIdx.x == 0) out[2 * static_cast<int64_t>(k) * np + t] = v;
     }
 }
 
@@ -1496,7 +1519,7 @@ struct LPass {
     const void *in[LP_IN];
     void *out[LP_OUT];
     const double *coef;      // problem k: coef + k * stride, [o][q] complex
-    double *dots;            // problem k: dots + k * stride, one complex per dot
+    double *part;            // per-block partial dots: (k * gridDim.x + block) * 2 LP_DOT doubles
     int32_t stride;          // doubles of per-problem work space
     int8_t da[LP_DOT], db[LP_DOT];   // dot operands: 0..NIN-1 inputs, NIN + o outputs
 };
@@ -1577,7 +1600,7 @@ __global__ __launch_bounds__(256) void k_locg_pass(LPass a, const int64_t *__res
         if (threadIdx.x < 2 * NDOT) {
             const double v = s_red[0][threadIdx.x] + s_red[1][threadIdx.x] + s_red[2][threadIdx.x] +
                              s_red[3][threadIdx.x];
-            if (v != 0.0) atomicAdd(&a.dots[static_cast<int64_t>(k) * a.stride + threadIdx.x], v);
+            a.part[(static_cast<int64_t>(k) * gridDim.x + blockIdx.x) * (2 * LP_DOT) + threadIdx.x] = v;
         }
     }
 }
@@ -1592,18 +1615,29 @@ enum {
 };
 constexpr int LOCG_WORK = 2 * LW_SLOTS + 4;   // doubles per problem
 static_assert(LOCG_WORK == RQX_LOCG_WORK, "work size in rqx.h");
+static_assert(RQX_LOCG_SCALE >= 2 * LW_SLOTS && RQX_LOCG_SCALE < LOCG_WORK, "scale slot in rqx.h");
 
 __device__ inline void put_c(double *w, int slot, double re, double im) {
     w[2 * slot] = re;
     w[2 * slot + 1] = im;
 }
 
-// Scalar stages between the passes: one thread per problem.
-__global__ void k_locg_scalar(int stage, int K, double *__restrict__ work, double *__restrict__ theta,
-                              double *__restrict__ rel, uint8_t *__restrict__ active, double tol, int real) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+// Scalar stages between the passes: one wave per problem.  The wave first adds the
+// preceding pass's `ndot` partial dots (fixed order, part_sum) into the work slots
+// from `dot_slot` on; thread 0 then runs the stage.
+__global__ __launch_bounds__(64) void k_locg_scalar(int stage, int K, double *__restrict__ work,
+                                                    double *__restrict__ theta, double *__restrict__ rel,
+                                                    uint8_t *__restrict__ active, double tol, int real,
+                                                    const double *__restrict__ part, int chunks, int dot_slot,
+                                                    int ndot) {
+    const int k = blockIdx.x;
     if (k >= K) return;
     double *w = work + static_cast<int64_t>(k) * LOCG_WORK;
+    for (int t = 0; t < 2 * ndot; ++t) {
+        const double v = part_sum(part, k, chunks, 2 * LP_DOT, t);
+        if (threadIdx.x == 0) w[2 * dot_slot + t] = v;
+    }
+    if (threadIdx.x != 0) return;
     auto re = [&](int slot) { return w[2 * slot]; };
     auto im = [&](int slot) { return w[2 * slot + 1]; };
     if (stage == 1) {                  // theta = <x, Ax>;  r = Ax - theta x
@@ -1613,8 +1647,9 @@ __global__ void k_locg_scalar(int stage, int K, double *__restrict__ work, doubl
         put_c(w, LW_C2 + 1, -th, 0.0);
         put_c(w, LW_C2 + 2, 0.0, 0.0);
     } else if (stage == 2) {           // residual norm, freeze; first Gram-Schmidt pass
+        // relative to max(|theta|, the caller's scale): still defined where the eigenvalue is 0
         const double rn = sqrt(fmax(re(LW_D_RR), 0.0));
-        const double rl = rn / fmax(fabs(theta[k]), 1e-300);
+        const double rl = rn / fmax(fmax(fabs(theta[k]), w[RQX_LOCG_SCALE]), 1e-300);
         rel[k] = rl;
         if (!(rl > tol)) active[k] = 0;
         put_c(w, LW_C3 + 0, 1.0, 0.0);
@@ -2097,6 +2132,12 @@ struct HitCache {            // sliced-ELL cache of the hits (column, C_j(s_i)) 
     bool shared = false;       // both precisions built together: the FP64 cache owns the allocations
 };
 
+// workgroups per problem of the batched vector kernels (256 threads each): about
+// 2048 rows per workgroup, at most 1024
+inline int vector_chunks(int64_t max_rows) {
+    return static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(1024, (max_rows + 2047) / 2048)));
+}
+
 struct rqx_plan {
     HitCache cache[2];         // [0]: FP32 values, [1]: FP64 values
     int use_cache = 1;
@@ -2134,6 +2175,8 @@ struct rqx_plan {
     ProbDesc *prob = nullptr;
     int64_t *prob_rows = nullptr;   // K + 1 row offsets
     unsigned int *queue = nullptr;
+    int chunks = 1;                 // workgroups per problem of the vector kernels
+    double *dot_part = nullptr;     // their per-workgroup partial dots: K x chunks x 2 LP_DOT
     int hashed = 0;                 // targets found by hash tables (else binary search)
     int hash_words = 0;             // largest segment hash table, u32 entries
     uint32_t *h_off = nullptr;      // per segment: first hash-table entry
@@ -2214,7 +2257,9 @@ bool cuckoo_build(const uint32_t *keys, uint32_t n, uint32_t lg, uint32_t *tab) 
 // Hash tables of every segment (lo: low keys, seg_off: M + 1 row offsets), built
 // on the host threads; h_log[m] = log2 buckets, h_off[m] = first entry.  False if
 // some segment's table does not settle (the plan then uses binary search).
-bool build_hash_tables(const std::vector<uint32_t> &lo, const std::vector<uint32_t> &seg_off,
+// Returns false (the plan then uses binary search) when a segment's table cannot be
+// built within `max_words` u32 entries, the tile kernel's LDS limit per workgroup.
+bool build_hash_tables(const std::vector<uint32_t> &lo, const std::vector<uint32_t> &seg_off, size_t max_words,
                        std::vector<uint32_t> &h_off, std::vector<uint32_t> &h_log, std::vector<uint32_t> &h_tab) {
     const size_t M = seg_off.size() - 1;
     std::vector<std::vector<uint32_t>> tabs(M);
@@ -2227,7 +2272,7 @@ bool build_hash_tables(const std::vector<uint32_t> &lo, const std::vector<uint32
             uint32_t lg = 1;
             while ((2u << lg) * 7u < n * 10u) ++lg;     // load <= 0.7
             bool done = false;
-            for (int tries = 0; tries < 3 && !done; ++tries, ++lg) {
+            for (int tries = 0; tries < 3 && !done && (size_t(2) << lg) <= max_words; ++tries, ++lg) {
                 tabs[m].resize(size_t(2) << lg);
                 done = cuckoo_build(lo.data() + seg_off[m], n, lg, tabs[m].data());
                 if (done) h_log[m] = lg;
@@ -2714,7 +2759,10 @@ int build_plan(rqx_plan **out, const std::vector<ProblemIn> &in, int64_t n_total
     std::vector<uint32_t> h_off, h_log, h_tab;
     bool hash_fits = true;
     for (const ProbHost &ph : p->probs) hash_fits = hash_fits && ph.L <= HASH_MAX_KEY_BITS;
-    if (hash_fits && !seg_off_all.empty() && build_hash_tables(lo, seg_off_all, h_off, h_log, h_tab)) {
+    int lds_max = 65536;
+    HIP_TRY(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, p->device));
+    if (hash_fits && !seg_off_all.empty() &&
+        build_hash_tables(lo, seg_off_all, static_cast<size_t>(std::max(lds_max - 256, 0)) / 4, h_off, h_log, h_tab)) {
         p->hashed = 1;
         for (uint32_t lg : h_log) p->hash_words = std::max(p->hash_words, 2 << lg);
     } else {
@@ -2751,6 +2799,10 @@ int build_plan(rqx_plan **out, const std::vector<ProblemIn> &in, int64_t n_total
                  o_t32re = carve(static_cast<size_t>(n_tab) * TAB_ENTRIES * 4),
                  o_t32im = carve(static_cast<size_t>(n_tab) * TAB_ENTRIES * 4), o_hoff = carve(h_off.size() * 4),
                  o_hlog = carve(h_log.size() * 4), o_htab = carve(h_tab.size() * 4);
+    int64_t max_rows = 0;
+    for (const ProbHost &ph : p->probs) max_rows = std::max(max_rows, ph.n_states);
+    p->chunks = vector_chunks(max_rows);
+    const size_t o_part = carve(static_cast<size_t>(K) * p->chunks * 2 * LP_DOT * 8);
     HIP_TRY(hipMalloc(&p->block, off));
     char *b = static_cast<char *>(p->block);
     p->keys = reinterpret_cast<uint64_t *>(b + o_keys);
@@ -2773,6 +2825,7 @@ int build_plan(rqx_plan **out, const std::vector<ProblemIn> &in, int64_t n_total
     p->prob = reinterpret_cast<ProbDesc *>(b + o_desc);
     p->prob_rows = reinterpret_cast<int64_t *>(b + o_prows);
     p->queue = reinterpret_cast<unsigned int *>(b + o_queue);
+    p->dot_part = reinterpret_cast<double *>(b + o_part);
     p->g_tab = reinterpret_cast<int32_t *>(b + o_gtab);
     p->g_zor = reinterpret_cast<uint64_t *>(b + o_gzor);
     p->gdesc = reinterpret_cast<GroupDesc *>(b + o_gdesc);
@@ -3449,16 +3502,14 @@ int rqx_plan_use_cache(rqx_plan *p, int32_t enable) {
 }  // extern "C"
 
 namespace {
-unsigned batch_chunks(const rqx_plan *p) {
-    int64_t mx = 0;
-    for (const ProbHost &ph : p->probs) mx = std::max(mx, ph.n_states);
-    return static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>(1024, (mx + 2047) / 2048)));
-}
+unsigned batch_chunks(const rqx_plan *p) { return static_cast<unsigned>(p->chunks); }
 
 template <typename V, int NP>
 void dots_launch(const rqx_plan *p, const PtrPack &pk, double *out, hipStream_t st) {
     hipLaunchKernelGGL((k_batch_dots<V, NP>), dim3(batch_chunks(p), p->probs.size()), dim3(256), 0, st, pk,
-                       p->prob_rows, out);
+                       p->prob_rows, p->dot_part);
+    hipLaunchKernelGGL(k_dots_finish, dim3(p->probs.size()), dim3(64), 0, st, p->dot_part,
+                       static_cast<int>(batch_chunks(p)), NP, out);
 }
 
 template <typename V>
@@ -3513,8 +3564,6 @@ int rqx_batch_dots(const rqx_plan *p, int32_t n_pairs, const void *const *a, con
         pk.b[q] = b[q];
     }
     hipStream_t st = static_cast<hipStream_t>(stream);
-    const int64_t n_out = 2 * static_cast<int64_t>(p->probs.size()) * n_pairs;
-    HIP_TRY(fill_async(out, int64_t(0), n_out, 0.0, st));
     switch (dtype) {
         case RQX_F64: return dots_dispatch<double>(p, n_pairs, pk, out, st);
         case RQX_C128: return dots_dispatch<double2>(p, n_pairs, pk, out, st);
@@ -3554,15 +3603,15 @@ hipError_t lpass(const rqx_plan *p, const LPass &a, unsigned chunks, hipStream_t
     return hipGetLastError();
 }
 
-LPass make_pass(double *work, int coef_slot, std::initializer_list<const void *> in,
-                std::initializer_list<void *> out, std::initializer_list<std::pair<int, int>> dots, int dot_slot) {
+LPass make_pass(const rqx_plan *p, double *work, int coef_slot, std::initializer_list<const void *> in,
+                std::initializer_list<void *> out, std::initializer_list<std::pair<int, int>> dots) {
     LPass a{};
     int q = 0;
     for (const void *v : in) a.in[q++] = v;
     q = 0;
     for (void *v : out) a.out[q++] = v;
     a.coef = work + 2 * coef_slot;
-    a.dots = work + 2 * dot_slot;
+    a.part = p->dot_part;
     a.stride = LOCG_WORK;
     q = 0;
     for (const auto &d : dots) {
@@ -3586,52 +3635,52 @@ int locg_step(const rqx_plan *p, bool low, bool real, void *const *vec, double *
     void *x = vec[0], *ax = vec[1], *y = vec[2], *ay = vec[3], *pv = vec[4], *ap = vec[5], *plo = vec[6], *aplo = vec[7];
     const int K = static_cast<int>(p->probs.size());
     const unsigned chunks = batch_chunks(p);
-    const unsigned sb = static_cast<unsigned>((K + 63) / 64);
-    auto scalar = [&](int stage) {
-        hipLaunchKernelGGL(k_locg_scalar, dim3(sb), dim3(64), 0, st, stage, K, work, theta, rel, active, tol,
-                           real ? 1 : 0);
+    // stage s reduces the ndot dots of the pass before it into the slots from dot_slot on
+    auto scalar = [&](int stage, int dot_slot, int ndot) {
+        hipLaunchKernelGGL(k_locg_scalar, dim3(K), dim3(64), 0, st, stage, K, work, theta, rel, active, tol,
+                           real ? 1 : 0, p->dot_part, static_cast<int>(chunks), dot_slot, ndot);
         return hipGetLastError();
     };
-    // the step's dot slots start at zero; coefficient slots are written by each stage before use
-    HIP_TRY(fill_async(work, int64_t(0), static_cast<int64_t>(K) * LOCG_WORK, 0.0, st));
+    // every coefficient slot is written by a stage and every dot slot by a reduction before
+    // it is read, so the work space needs no clearing (its scale slot is the caller's)
     // 1: theta = <x, Ax>
-    HIP_TRY((lpass<V, LV, 2, 0, 1, false, false>(p, make_pass(work, 0, {x, ax}, {}, {{0, 1}}, LW_D_XAX), chunks, st)));
-    HIP_TRY(scalar(1));
+    HIP_TRY((lpass<V, LV, 2, 0, 1, false, false>(p, make_pass(p, work, 0, {x, ax}, {}, {{0, 1}}), chunks, st)));
+    HIP_TRY(scalar(1, LW_D_XAX, 1));
     // 2: r = Ax - theta x (into p); <r, r>, <x, r>, <y, r>
     HIP_TRY((lpass<V, LV, 3, 1, 3, false, false>(
-        p, make_pass(work, LW_C2, {ax, x, y}, {pv}, {{3, 3}, {1, 3}, {2, 3}}, LW_D_RR), chunks, st)));
-    HIP_TRY(scalar(2));
+        p, make_pass(p, work, LW_C2, {ax, x, y}, {pv}, {{3, 3}, {1, 3}, {2, 3}}), chunks, st)));
+    HIP_TRY(scalar(2, LW_D_RR, 3));
     // 3: p = r - <x,r> x - <y,r> y; <x, p>, <y, p>, <p, p>
     HIP_TRY((lpass<V, LV, 3, 1, 3, false, false>(
-        p, make_pass(work, LW_C3, {pv, x, y}, {pv}, {{1, 3}, {2, 3}, {3, 3}}, LW_D_XP), chunks, st)));
-    HIP_TRY(scalar(3));
+        p, make_pass(p, work, LW_C3, {pv, x, y}, {pv}, {{1, 3}, {2, 3}, {3, 3}}), chunks, st)));
+    HIP_TRY(scalar(3, LW_D_XP, 3));
     // 4: second Gram-Schmidt pass, normalised (plus the low copy for the bulk matvec); Ap; 5: Rayleigh-Ritz dots
     if (low) {
-        HIP_TRY((lpass<V, LV, 3, 2, 0, false, true>(p, make_pass(work, LW_C4, {pv, x, y}, {pv, plo}, {}, 0), chunks, st)));
+        HIP_TRY((lpass<V, LV, 3, 2, 0, false, true>(p, make_pass(p, work, LW_C4, {pv, x, y}, {pv, plo}, {}), chunks, st)));
         if (int rc = rqx_matvec(p, plo, aplo, dtype_of<LV>(), st)) return rc;
         HIP_TRY((lpass<V, LV, 6, 1, 9, true, false>(
-            p, make_pass(work, LW_C5, {x, y, pv, ax, ay, aplo}, {ap},
-                         {{0, 3}, {0, 4}, {0, 6}, {1, 3}, {1, 4}, {1, 6}, {2, 3}, {2, 4}, {2, 6}}, LW_D_RR9),
+            p, make_pass(p, work, LW_C5, {x, y, pv, ax, ay, aplo}, {ap},
+                         {{0, 3}, {0, 4}, {0, 6}, {1, 3}, {1, 4}, {1, 6}, {2, 3}, {2, 4}, {2, 6}}),
             chunks, st)));
     } else {
-        HIP_TRY((lpass<V, LV, 3, 1, 0, false, false>(p, make_pass(work, LW_C4, {pv, x, y}, {pv}, {}, 0), chunks, st)));
+        HIP_TRY((lpass<V, LV, 3, 1, 0, false, false>(p, make_pass(p, work, LW_C4, {pv, x, y}, {pv}, {}), chunks, st)));
         if (int rc = rqx_matvec(p, pv, ap, dtype_of<V>(), st)) return rc;
         HIP_TRY((lpass<V, LV, 6, 0, 9, false, false>(
-            p, make_pass(work, 0, {x, y, pv, ax, ay, ap}, {},
-                         {{0, 3}, {0, 4}, {0, 5}, {1, 3}, {1, 4}, {1, 5}, {2, 3}, {2, 4}, {2, 5}}, LW_D_RR9),
+            p, make_pass(p, work, 0, {x, y, pv, ax, ay, ap}, {},
+                         {{0, 3}, {0, 4}, {0, 5}, {1, 3}, {1, 4}, {1, 5}, {2, 3}, {2, 4}, {2, 5}}),
             chunks, st)));
     }
-    HIP_TRY(scalar(5));
+    HIP_TRY(scalar(5, LW_D_RR9, 9));
     // 6: y' = c1 y + c2 p, Ay' likewise, x' = c0 x + y', Ax' likewise; <x', x'>, <x', y'>
     HIP_TRY((lpass<V, LV, 6, 4, 2, false, false>(
-        p, make_pass(work, LW_C6, {x, ax, y, ay, pv, ap}, {y, ay, x, ax}, {{8, 8}, {8, 6}}, LW_D_XX), chunks, st)));
-    HIP_TRY(scalar(6));
+        p, make_pass(p, work, LW_C6, {x, ax, y, ay, pv, ap}, {y, ay, x, ax}, {{8, 8}, {8, 6}}), chunks, st)));
+    HIP_TRY(scalar(6, LW_D_XX, 2));
     // 7: x = x' / |x'|, Ax likewise; y = y' - <x, y'> x, Ay likewise; <y, y>
     HIP_TRY((lpass<V, LV, 4, 4, 1, false, false>(
-        p, make_pass(work, LW_C7, {x, ax, y, ay}, {x, ax, y, ay}, {{6, 6}}, LW_D_YY2), chunks, st)));
-    HIP_TRY(scalar(7));
+        p, make_pass(p, work, LW_C7, {x, ax, y, ay}, {x, ax, y, ay}, {{6, 6}}), chunks, st)));
+    HIP_TRY(scalar(7, LW_D_YY2, 1));
     // 8: y and Ay normalised
-    HIP_TRY((lpass<V, LV, 2, 2, 0, false, false>(p, make_pass(work, LW_C8, {y, ay}, {y, ay}, {}, 0), chunks, st)));
+    HIP_TRY((lpass<V, LV, 2, 2, 0, false, false>(p, make_pass(p, work, LW_C8, {y, ay}, {y, ay}, {}), chunks, st)));
     return RQX_OK;
 }
 }  // namespace

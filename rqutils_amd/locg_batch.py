@@ -11,14 +11,16 @@ No counterpart in the reference (``ground_locg`` solves one problem per call,
   linear update with per-problem coefficients plus the inner products the
   next stage needs), small per-problem scalar kernels between them (the 3x3
   Rayleigh-Ritz eigenpairs among them) and the matvec;
-* converged problems (relative residual <= ``tol``) are frozen: their
-  coefficients become (1, 0, 0) so their vectors stop changing;
+* converged problems (residual ``|A x - theta x| <= tol max(|theta|, |A x0|)``,
+  ``x0`` the normalised start vector) are frozen: their coefficients become
+  (1, 0, 0) so their vectors stop changing;
 * no host synchronisation inside a step, so the step is captured once in a
   HIP graph (``torch.cuda.CUDAGraph``) and replayed; convergence is polled
   every ``check_every`` steps.
 """
 import ctypes
 import time
+import warnings
 
 import torch
 
@@ -67,6 +69,10 @@ class BatchLOCG:
         self.x.copy_(x0)
         self._normalize(self.x, None)
         self._apply(self.x, self.ax)
+        # residual scale: max(|theta|, |A x0|) (|A x0| sizes the operator, so a problem
+        # whose ground energy is 0 still converges); read by the fused step from its work slot
+        self.scale = torch.sqrt(plan.dots([(self.ax, self.ax)])[:, 0].real)
+        self.work.view(self.K, _capi.RQX_LOCG_WORK)[:, _capi.RQX_LOCG_SCALE] = self.scale
         self._first_step()
         if self.lo_storage:
             for lo_v, hi_v in ((self.x_lo, self.x), (self.ax_lo, self.ax), (self.y_lo, self.y), (self.ay_lo, self.ay)):
@@ -95,7 +101,7 @@ class BatchLOCG:
         self.theta.copy_(th)
         plan.combine([self.ax, self.x], torch.stack([self.one, -th.to(torch.complex128)], 1), self.r)
         rn = torch.sqrt(plan.dots([(self.r, self.r)])[:, 0].real)
-        self.rel.copy_(rn / th.abs().clamp_min(1e-300))
+        self.rel.copy_(rn / torch.maximum(th.abs(), self.scale).clamp_min(1e-300))
         self.active &= self.rel > self.tol
         self.p.copy_(self.r)
         for _ in range(2):
@@ -191,6 +197,7 @@ class BatchLOCG:
 
 def batch_ground_locg(plan: Plan, x0, tol=1e-7, switch_tol=1e-5, maxiter=1000, check_every=10,
                       use_graph=True, mixed=True, record=False, energy_tol=None, energy_checks=2,
+                      energy_rel_residual=1e-7,
                       switch_energy=1e-9, lo_storage=True, check_every_hi=1):
     """Ground eigenpairs of every problem of ``plan`` (a ``BatchPlan``, or a ``Plan`` as one problem).
 
@@ -198,7 +205,8 @@ def batch_ground_locg(plan: Plan, x0, tol=1e-7, switch_tol=1e-5, maxiter=1000, c
     ``energy_tol``, once its theta, polled every ``check_every`` steps, has
     settled: the last decrease and the geometric (Aitken) estimate of the
     remaining decrease are both below ``energy_tol * |theta|`` at
-    ``energy_checks`` consecutive polls.  In the low-precision phase the same
+    ``energy_checks`` consecutive polls, with its relative residual at most
+    ``energy_rel_residual`` (FP64 phase).  In the low-precision phase the same
     test with the limit ``max(energy_tol, 30 switch_energy)`` (theta carries
     FP32 image errors there) hands every problem over to the FP64 phase, which
     then polls every ``check_every_hi`` steps: it starts from a vector the FP32
@@ -269,6 +277,11 @@ def batch_ground_locg(plan: Plan, x0, tol=1e-7, switch_tol=1e-5, maxiter=1000, c
                 thetas.clear()
                 calm.zero_()
             continue
+        if energy_tol is not None:
+            # a stalled theta is not yet converged: near an excited eigenvector theta can sit
+            # still for tens of iterations (relative residual ~1e-6) before it falls to the
+            # ground energy, so the FP64 phase also asks for a small residual
+            settled &= status[2 * K:] <= energy_rel_residual
         if energy_tol is not None and bool((settled & act).any()):
             solver.active.copy_((act & ~settled).to(plan.device))
             active = int((act & ~settled).sum())
@@ -277,6 +290,11 @@ def batch_ground_locg(plan: Plan, x0, tol=1e-7, switch_tol=1e-5, maxiter=1000, c
     if solver.phase == 'lo' and solver.lo_storage:
         solver.x.copy_(solver.x_lo)
     torch.cuda.synchronize(plan.device)
+    converged = ~solver.active.cpu()        # stopped by the residual or the energy test
+    if not bool(converged.all()):
+        warnings.warn(f'batch_ground_locg: {int((~converged).sum())} of {solver.K} problems not converged '
+                      f'after {it} iterations', RuntimeWarning, stacklevel=2)
     info = {'rel_residual': solver.rel.clone(), 'switch_iteration': switch_at, 'iterations': it,
-            'seconds': time.perf_counter() - t0, 'history': history, 'graph': use_graph}
+            'seconds': time.perf_counter() - t0, 'history': history, 'graph': use_graph,
+            'converged': converged}
     return solver.theta.clone(), solver.x, it, info

@@ -49,7 +49,7 @@ class MixedLOCGResult:
 
 def ground_locg_mixed(matvec_hi, matvec_lo, xinit, maxiter=1000, tol=1e-6, switch_tol=1e-6,
                       check_every=1, stagnation=100, record=False, energy_tol=None, energy_checks=3,
-                      switch_energy=1e-9):
+                      switch_energy=1e-9, energy_rel_residual=1e-7):
     """Ground eigenpair with a low-precision bulk matvec and FP64 refinement.
 
     Args:
@@ -58,7 +58,8 @@ def ground_locg_mixed(matvec_hi, matvec_lo, xinit, maxiter=1000, tol=1e-6, switc
             to run the FP64 phase only.
         xinit: start vector (float64 / complex128 tensor).
         maxiter: total iteration limit.
-        tol: stop when ``|A x - theta x| <= tol |theta|`` with FP64 images.
+        tol: stop when ``|A x - theta x| <= tol max(|theta|, |A x0|)`` with FP64
+            images (``x0`` the normalised start vector).
         switch_tol: leave the low-precision phase at this relative residual.
         check_every: iterations between host-side convergence checks.
         stagnation: leave the low phase after this many checks without a new
@@ -67,7 +68,8 @@ def ground_locg_mixed(matvec_hi, matvec_lo, xinit, maxiter=1000, tol=1e-6, switc
         energy_tol: if set, also stop in the FP64 phase once the remaining
             decrease of theta, estimated from the last three decreases as a
             geometric tail (Aitken), and the last decrease are both below
-            ``energy_tol * |theta|`` for ``energy_checks`` consecutive checks.
+            ``energy_tol * |theta|`` for ``energy_checks`` consecutive checks and
+            the relative residual is at most ``energy_rel_residual``.
 
     Returns:
         ``MixedLOCGResult`` (iterable as ``theta, x, niter``).
@@ -88,6 +90,7 @@ def ground_locg_mixed(matvec_hi, matvec_lo, xinit, maxiter=1000, tol=1e-6, switc
     phase = 'lo' if matvec_lo is not None else 'hi'
     x = xinit / _nrm(xinit)
     ax = mv(x, phase)
+    scale = float(_nrm(ax))
     y = ay = None
     history = []
     best = float('inf')
@@ -101,7 +104,8 @@ def ground_locg_mixed(matvec_hi, matvec_lo, xinit, maxiter=1000, tol=1e-6, switc
         theta = _dot(x, ax).real
         r = ax - theta.to(hi) * x
         if niter % check_every == 0 or niter >= maxiter:
-            rel = float(_nrm(r) / theta.abs())
+            # relative to max(|theta|, |A x0|): defined also for a zero eigenvalue
+            rel = float(_nrm(r)) / max(abs(float(theta)), scale, 1e-300)
             if record:
                 history.append((time.perf_counter() - t_start, float(theta), rel, phase))
             if energy_tol is not None:
@@ -116,11 +120,16 @@ def ground_locg_mixed(matvec_hi, matvec_lo, xinit, maxiter=1000, tol=1e-6, switc
                     tail = d1 * d1 / (d0 - d1) if d0 > d1 >= 0 else float('inf')
                     calm = calm + 1 if (0 <= d1 <= lim and tail <= lim) else 0
                     if calm >= energy_checks:
+                        # a stalled theta near an excited eigenvector is not converged: the
+                        # FP64 phase also asks for a small residual
                         if phase == 'hi':
-                            break
-                        force_switch = True
-                        hi_thetas.clear()
-                        calm = 0
+                            if rel <= energy_rel_residual:
+                                break
+                            calm -= 1
+                        else:
+                            force_switch = True
+                            hi_thetas.clear()
+                            calm = 0
             if phase == 'lo':
                 if rel < best * 0.98:
                     best, since_best = rel, 0

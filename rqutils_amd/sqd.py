@@ -224,6 +224,8 @@ def _run(hamiltonian, keys, states_size, options, log_level=logging.INFO):
         theta, eigvec, niter, info = batch_ground_locg(plan, x0, tol=tol, maxiter=options.locg_maxiter,
                                                        check_every=5)
         LOG.log(log_level, 'Mixed LOCG: %d iterations, relative residual %g', niter, float(info['rel_residual'][0]))
+        if not bool(info['converged'][0]):
+            LOG.warning('Mixed LOCG stopped at maxiter=%d before reaching tol=%g', options.locg_maxiter, tol)
         eigval = theta[0]
     else:
         eigval, eigvec, _ = ground_locg(plan.matvec, vinit, maxiter=options.locg_maxiter, tol=options.locg_tol,

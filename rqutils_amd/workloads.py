@@ -259,3 +259,24 @@ def batch40_problem(index):
     tables = TermTables.from_masks(xm, zm, alpha, 40)
     states = random_subspace(40, num_states, np.random.default_rng(2001 + 17 * index))
     return Workload(f'batch-p{index}-40q-{num_terms}t-{num_states}', tables, states, 40, None, alpha)
+
+
+def fingerprint(w):
+    """Sizes and SHA-256 prefixes of a generated workload's tables and states, so that a
+    stored result (a reference ground energy) can be matched to the exact instance."""
+    import hashlib
+
+    def sha(a):
+        return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()[:16]
+
+    t = w.tables
+    return {
+        'n_states': int(w.states.shape[0]),
+        'n_groups': int(t.num_groups),
+        'n_terms': int(t.num_terms),
+        'states_sha256': sha(w.states),
+        'xmask_sha256': sha(t.xmask),
+        'grp_off_sha256': sha(t.grp_off),
+        'zmask_sha256': sha(t.zmask),
+        'coef_abs_sum': float(np.sum(np.abs(t.coef))),
+    }

@@ -7,6 +7,7 @@ import scipy.sparse
 import scipy.sparse.linalg
 import torch
 
+from oracle import sqdref
 from rqutils_amd import workloads
 from rqutils_amd.autotune import autotune
 from rqutils_amd.locg_batch import batch_ground_locg
@@ -40,12 +41,21 @@ def problems():
     return out
 
 
-def _csr(plan):
-    cols, vals = plan.lookup(values=True)
-    n = plan.n_states
-    hit = cols >= 0
-    rows = torch.nonzero(hit, as_tuple=True)[1].cpu().numpy()
-    return scipy.sparse.csr_matrix((vals[hit].cpu().numpy(), (rows, cols[hit].cpu().numpy())), shape=(n, n))
+_CSR = {}
+
+
+def _csr(t, s):
+    """The oracle's projected Hamiltonian (CPU: searchsorted sources, term-by-term sums)."""
+    key = (id(t), id(s))
+    if key not in _CSR:
+        _CSR[key] = sqdref.csr_u64(t.xmask, t.grp_off, t.zmask, t.coef, s)
+    return _CSR[key]
+
+
+def _e0(h):
+    if h.shape[0] <= 400:
+        return np.linalg.eigvalsh(h.toarray())[0]
+    return scipy.sparse.linalg.eigsh(h, k=1, which='SA', tol=1e-13)[0][0]
 
 
 def test_batch_matvec_equals_single_plans(problems):
@@ -139,9 +149,7 @@ def test_batch_locg_matches_eigsh(problems, use_graph):
     x0 = torch.from_numpy(rng.standard_normal(bp.n_total) + 1j * rng.standard_normal(bp.n_total)).to(DEV)
     theta, x, it, info = batch_ground_locg(bp, x0, tol=1e-7, maxiter=3000, check_every=10, use_graph=use_graph)
     for k, (t, s) in enumerate(problems):
-        h = _csr(Plan(t, s))
-        e = scipy.sparse.linalg.eigsh(h, k=1, which='SA', tol=1e-12)[0][0] if h.shape[0] > 2 else \
-            np.linalg.eigvalsh(h.toarray())[0]
+        e = _e0(_csr(t, s))
         assert abs(float(theta[k]) - e) < 1e-8 * max(1., abs(e)), (k, float(theta[k]), e)
     assert info['switch_iteration'] is not None
 
@@ -160,8 +168,7 @@ def test_single_plan_graph_locg_energy_stop(problems, cached):
     x0[int(torch.argmin(diag))] = 1
     theta, x, it, info = batch_ground_locg(plan, x0, tol=1e-12, maxiter=3000, check_every=5,
                                            energy_tol=3e-10)
-    h = _csr(Plan(t, s))
-    e = scipy.sparse.linalg.eigsh(h, k=1, which='SA', tol=1e-12)[0][0]
+    e = _e0(_csr(t, s))
     assert abs(float(theta[0]) - e) < 1e-8 * abs(e), (float(theta[0]), e)
     assert it < 3000 and info['switch_iteration'] is not None
     assert torch.count_nonzero(x[plan.n_states:]) == 0
@@ -203,8 +210,7 @@ def test_fused_step_real_vectors(problems):
     assert torch.allclose(a.theta, b.theta, rtol=1e-9, atol=0)
     assert a.x.dtype == torch.float64 and bool(torch.isfinite(a.x).all())
     # float32-stored bulk phase on real vectors converges to the same ground energy
-    h = _csr(Plan(t, s))
-    e = scipy.sparse.linalg.eigsh(h, k=1, which='SA', tol=1e-12)[0][0]
+    e = _e0(_csr(t, s))
     theta, x, it, info = batch_ground_locg(plan, x0, tol=1e-9, maxiter=3000, check_every=10, lo_storage=True)
     assert x.dtype == torch.float64 and info['switch_iteration'] is not None
     assert abs(float(theta[0]) - e) < 1e-9 * abs(e)
@@ -220,8 +226,7 @@ def test_bulk_phase_storage_reaches_the_same_ground_state(problems, lo_storage):
     theta, x, it, info = batch_ground_locg(bp, x0, tol=1e-9, maxiter=3000, check_every=10, lo_storage=lo_storage)
     assert x.dtype == torch.complex128 and info['switch_iteration'] is not None
     for k, (t, s) in enumerate(problems[:2] + problems[4:]):
-        h = _csr(Plan(t, s))
-        e = scipy.sparse.linalg.eigsh(h, k=1, which='SA', tol=1e-12)[0][0]
+        e = _e0(_csr(t, s))
         assert abs(float(theta[k]) - e) < 1e-9 * max(1., abs(e)), (k, float(theta[k]), e)
 
 
@@ -259,3 +264,35 @@ def test_autotune_batch_picks_a_launch_shape():
     assert len(results) == 6 and results[0]['ms'] <= results[-1]['ms']
     assert plan.info['persistent'] == int(results[0]['persistent'])
     assert torch.equal(plan.matvec(x), want)
+
+
+def test_batch_locg_is_deterministic(problems):
+    """Per-problem dots are summed in a fixed order: two solves give the same bits and iteration count."""
+    _say('batch locg deterministic')
+    bp = BatchPlan(problems)
+    rng = np.random.default_rng(19)
+    x0 = torch.from_numpy(rng.standard_normal(bp.n_total) + 1j * rng.standard_normal(bp.n_total)).to(DEV)
+    runs = [batch_ground_locg(bp, x0, tol=1e-8, maxiter=3000, check_every=5, use_graph=True) for _ in range(2)]
+    (th_a, x_a, it_a, _), (th_b, x_b, it_b, _) = runs
+    assert it_a == it_b
+    assert torch.equal(th_a, th_b) and torch.equal(x_a, x_b)
+    d1 = bp.dots([(x_a, x0), (x0, x0)])
+    d2 = bp.dots([(x_a, x0), (x0, x0)])
+    assert torch.equal(d1, d2)
+
+
+def test_zero_ground_energy_converges():
+    """A Hamiltonian shifted so that its ground energy is 0 still meets the relative residual."""
+    _say('zero ground energy')
+    t, s = _random_problem(5, 10, 60, 300)
+    e0 = scipy.sparse.linalg.eigsh(_csr(t, s), k=1, which='SA', tol=1e-13)[0][0] if len(s) > 400 else \
+        np.linalg.eigvalsh(_csr(t, s).toarray())[0]
+    shifted = TermTables.from_masks(np.concatenate([t.term_xmask(), [0]]).astype(np.uint64),
+                                    np.concatenate([t.zmask, [0]]).astype(np.uint64),
+                                    np.concatenate([t.alphas().real, [-e0]]), t.num_qubits)
+    plan = Plan(shifted, s)
+    rng = np.random.default_rng(20)
+    x0 = torch.from_numpy(rng.standard_normal(plan.n_total) + 1j * rng.standard_normal(plan.n_total)).to(DEV)
+    theta, x, it, info = batch_ground_locg(plan, x0, tol=1e-8, maxiter=3000, check_every=5)
+    assert bool(info['converged'][0]) and it < 3000
+    assert abs(float(theta[0])) <= 1e-7 * abs(e0)

@@ -6,6 +6,7 @@ import pytest
 import torch
 
 from oracle import sqdref
+import estar
 from rqutils_amd import _capi, workloads
 from rqutils_amd.paulis.symplectic import TermTables
 from rqutils_amd.plan import BatchPlan, Plan
@@ -223,6 +224,9 @@ def test_run_sqd_mixed_precision_matches_fp64(cs):
     emx, vmx, _, dim2 = run_sqd(h, states_p, size,
                                 options=SQDOptions(cache_strategy=cs, locg_maxiter=3000, locg_precision='mixed'))
     assert dim == dim2 == len(w.states)
+    e_star = estar.load('jw-32so-60x50', w)      # eigsh of the oracle's CSR matrix
+    assert abs(float(e64) - e_star) <= 1e-9 * abs(e_star)
+    assert abs(float(emx) - e_star) <= 1e-9 * abs(e_star)
     assert abs(float(emx) - float(e64)) <= 1e-9 * abs(float(e64))
     assert abs(abs(torch.vdot(v64, vmx.to(v64.dtype)).item()) - 1.0) < 1e-6
     assert torch.count_nonzero(vmx[dim:]) == 0

@@ -158,20 +158,15 @@ def test_matmul_full_space_matches_dense():
 
 
 def test_locg_jw_matches_eigsh():
+    """Plain LOCG on a JW subspace reaches the eigsh energy of the oracle's CSR matrix."""
     _say('locg jw small vs eigsh')
     import scipy.sparse.linalg
     w = workloads.jw32(n_alpha=40, n_beta=40)
     plan = Plan(w.tables, w.states)
-    cols, vals = plan.lookup(values=True)
+    t = w.tables
+    hs = sqdref.csr_u64(t.xmask, t.grp_off, t.zmask, t.coef, w.states)
+    e_ref = scipy.sparse.linalg.eigsh(hs, k=1, which='SA', tol=1e-13)[0][0]
     n = plan.n_states
-    hit = cols >= 0
-    rows = torch.nonzero(hit, as_tuple=True)[1].cpu().numpy()
-    cc = cols[hit].cpu().numpy()
-    data = vals[hit].real.cpu().numpy()
-    del cols, vals, hit
-    import scipy.sparse
-    hs = scipy.sparse.csr_matrix((data, (rows, cc)), shape=(n, n))
-    e_ref = scipy.sparse.linalg.eigsh(hs, k=1, which='SA', tol=1e-12)[0][0]
     diag = plan.diagonal().real
     theta, vec, niter = ground_locg(plan.matvec, torch.argmin(diag), maxiter=2000,
                                    vspace=(n, torch.float64, plan.device))
@@ -190,11 +185,8 @@ def test_hproj_nonzero_triplets_match_lookup_and_scale():
     h = PauliSumXZ.from_tables(w.tables)
     m = hproj(h, u64_to_states(w.states, 32), unique_states=True)
     plan = Plan(w.tables, w.states)
-    cols, vals = plan.lookup(values=True)
-    hit = cols >= 0
-    rows = torch.nonzero(hit, as_tuple=True)[1].cpu().numpy()
-    ref = scipy.sparse.csr_array((vals[hit].real.cpu().numpy(), (rows, cols[hit].long().cpu().numpy())),
-                                 shape=m.shape)
+    t = w.tables
+    ref = sqdref.csr_u64(t.xmask, t.grp_off, t.zmask, t.coef, w.states)
     assert m.nnz == ref.nnz == plan.count()['hits']
     # same sparsity; elements agree to rounding (a long group's sum may be split across lanes)
     assert (abs(m - ref)).max() <= 1e-13 * abs(ref).max()
@@ -210,3 +202,7 @@ def test_hproj_nonzero_triplets_match_lookup_and_scale():
     got = m @ v
     want = plan.matvec(torch.from_numpy(v).to(DEV)).cpu().numpy()
     assert np.max(np.abs(got - want)) <= 1e-12 * np.max(np.abs(want))
+    rows = np.sort(rng.choice(plan.n_states, size=300, replace=False))
+    t = w.tables
+    ora = sqdref.matvec_rows_u64(t.xmask, t.grp_off, t.zmask, t.coef, w.states, v, rows)
+    assert np.max(np.abs(got[rows] - ora)) <= 1e-12 * np.max(np.abs(ora))

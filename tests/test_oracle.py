@@ -210,3 +210,107 @@ def test_locg_oracle_orth_deflation_gives_first_excited():
     theta, vec, _ = locgref.ground_locg(lambda x: h @ x, rng.standard_normal(1 << nq), orth=(v[:, 0],),
                                         maxiter=2000)
     assert abs(theta - w[1]) < 1e-8 * abs(w[1])
+
+
+# ---------------------------------------------------------------------------
+# the oracle's C port (timed as the CPU baseline) and the CSR / row-sampled forms
+# ---------------------------------------------------------------------------
+
+def _tables_case(kind, seed):
+    """Term tables for the C-port pins: complex coefficients (Y phases and complex
+    alphas) with a diagonal group, the same without any x = 0 term, or one term per
+    X mask (every group a single term)."""
+    from rqutils_amd.paulis.symplectic import TermTables
+    rng = np.random.default_rng(seed)
+    n = 12
+    if kind == 'single_term':
+        xm = np.unique(rng.integers(1, 1 << n, size=80).astype(np.uint64))
+        zm = rng.integers(0, 1 << n, size=xm.shape[0]).astype(np.uint64)
+        alpha = rng.standard_normal(xm.shape[0])
+    else:
+        codes = rng.choice(4, size=(150, n), p=(0.45, 0.2, 0.2, 0.15))
+        w = np.uint64(1) << np.arange(n - 1, -1, -1, dtype=np.uint64)
+        xm = np.bitwise_or.reduce(np.where((codes == 1) | (codes == 2), w, np.uint64(0)), axis=1)
+        zm = np.bitwise_or.reduce(np.where((codes == 2) | (codes == 3), w, np.uint64(0)), axis=1)
+        alpha = rng.standard_normal(150) + 1j * rng.standard_normal(150)
+        if kind == 'no_diagonal':
+            keep = xm != 0
+            xm, zm, alpha = xm[keep], zm[keep], alpha[keep]
+        else:                     # a diagonal group of several Z strings
+            zd = rng.integers(0, 1 << n, size=6).astype(np.uint64)
+            xm = np.concatenate([xm, np.zeros(6, np.uint64)])
+            zm = np.concatenate([zm, zd])
+            alpha = np.concatenate([alpha, rng.standard_normal(6)])
+    t = TermTables.from_masks(xm.astype(np.uint64), zm.astype(np.uint64), alpha, n)
+    keys = np.sort(rng.choice(1 << n, size=900, replace=False).astype(np.uint64))
+    return t, keys
+
+
+def _groups(t):
+    return [(t.zmask[a:b], t.coef[a:b]) for a, b in zip(t.grp_off[:-1], t.grp_off[1:])]
+
+
+def _require_cport():
+    from oracle import build as obuild, cport
+    import os
+    if not os.path.exists(cport.LIB):
+        obuild.build()
+    return cport
+
+
+@pytest.mark.parametrize('kind', ['complex', 'no_diagonal', 'single_term'])
+def test_cport_matches_numpy_oracle(kind):
+    cport = _require_cport()
+    t, keys = _tables_case(kind, {'complex': 31, 'no_diagonal': 32, 'single_term': 33}[kind])
+    assert t.has_diagonal == (kind == 'complex')
+    if kind == 'single_term':
+        assert np.all(np.diff(t.grp_off) == 1)
+    else:
+        assert not t.is_real
+    for xm in list(t.xmask[:20]) + [0, (1 << 12) - 1]:
+        assert np.array_equal(cport.xsource(keys, xm), sqdref.xsource_searchsorted(keys, xm))
+    rng = np.random.default_rng(5)
+    v = rng.standard_normal(keys.shape[0]) + 1j * rng.standard_normal(keys.shape[0])
+    want = sqdref.matvec_u64(t.xmask, _groups(t), keys, v)
+    got = cport.matvec(t, keys, v)
+    assert np.max(np.abs(got - want)) <= 1e-13 * np.max(np.abs(want))
+
+
+def test_cport_timed_sample_extrapolates_one_matvec():
+    cport = _require_cport()
+    t, keys = _tables_case('complex', 34)
+    secs, groups, threads = cport.timed_sample(t, keys, budget_s=0.5)
+    assert secs > 0 and 1 <= groups <= t.num_groups and threads >= 1
+
+
+@pytest.mark.parametrize('kind', ['complex', 'no_diagonal', 'single_term'])
+def test_oracle_csr_and_row_samples_match_dense(kind):
+    """The CSR form and the row-sampled matvec against the textbook projected matrix;
+    complex alphas make H non-Hermitian, so a transposed matrix would fail."""
+    t, keys = _tables_case(kind, {'complex': 41, 'no_diagonal': 42, 'single_term': 43}[kind])
+    keys = keys[:300]
+    strings = _tables_to_strings(t)
+    hs = dense.projected_matrix(strings, t.alphas(), keys)
+    csr = sqdref.csr_u64(t.xmask, t.grp_off, t.zmask, t.coef, keys)
+    assert np.allclose(csr.toarray(), hs, rtol=0, atol=1e-12 * max(1, np.abs(hs).max()))
+    if kind != 'single_term':
+        assert not np.allclose(hs, hs.conj().T)
+    rng = np.random.default_rng(6)
+    v = rng.standard_normal(keys.shape[0]) + 1j * rng.standard_normal(keys.shape[0])
+    rows = np.sort(rng.choice(keys.shape[0], size=40, replace=False))
+    got = sqdref.matvec_rows_u64(t.xmask, t.grp_off, t.zmask, t.coef, keys, v, rows)
+    assert np.allclose(got, (hs @ v)[rows], rtol=0, atol=1e-12 * np.abs(hs @ v).max())
+
+
+def _tables_to_strings(t):
+    """Pauli strings of the tables' terms (char c = qubit n-1-c): X on x & ~z, Y on x & z, Z on z & ~x."""
+    n = t.num_qubits
+    out = []
+    for xm, zm in zip(t.term_xmask(), t.zmask):
+        s = []
+        for c in range(n):
+            q = n - 1 - c
+            xb, zb = (int(xm) >> q) & 1, (int(zm) >> q) & 1
+            s.append('IXZY'[xb + 2 * zb])
+        out.append(''.join(s))
+    return out
