@@ -292,6 +292,26 @@ enum { RQX_LOCG_WORK = 168, RQX_LOCG_SCALE = 164 };
 int rqx_locg_step(const rqx_plan *plan, int32_t dtype, int32_t low, void *const *vec, double *theta, double *rel,
                   uint8_t *active, double *work, double tol, rqx_stream_t stream);
 
+/* problems of a plan that rqx_locg_step_static serves (its work space: 5.5 MB per GPU) */
+enum { RQX_LOCG_STATIC_PROBLEMS = 4096 };
+
+/*
+ * rqx_locg_step without a work-space argument, for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  No counterpart in the reference.  The work space is
+ * a file-scope device array of the library (RQX_LOCG_STATIC_PROBLEMS x
+ * RQX_LOCG_WORK doubles, one per GPU, held until the process ends); the call
+ * allocates nothing.  scale: device, one double per problem (the residual scale
+ * of rqx_locg_step's work slot, e.g. |A x0|), or NULL for 0; it is copied into
+ * the work space on every call.  Every call on one GPU shares that array and
+ * leaves its contents behind: calls that overlap on one GPU (other streams,
+ * other threads, other plans) race.  The plan's device must be the current
+ * device.  RQX_ERR_LIMIT for plans of more than RQX_LOCG_STATIC_PROBLEMS
+ * problems.  Otherwise as rqx_locg_step.
+ */
+int rqx_locg_step_static(const rqx_plan *plan, int32_t dtype, int32_t low, void *const *vec, double *theta,
+                         double *rel, uint8_t *active, const double *scale, double tol, rqx_stream_t stream);
+
 /*
  * Lowest eigenpair of a small Hermitian matrix (n = 2 or 3), row-major,
  * complex interleaved, device pointers.  out = [theta, 0, k0re, k0im, ...].

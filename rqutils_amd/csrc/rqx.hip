@@ -25,6 +25,7 @@
 #include <cstdint>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <initializer_list>
@@ -3704,6 +3705,61 @@ int rqx_locg_step(const rqx_plan *p, int32_t dtype, int32_t low, void *const *ve
         case RQX_F32: return locg_step<float, float>(p, false, true, vec, theta, rel, active, work, tol, st);
         default: return locg_step<float2, float2>(p, false, false, vec, theta, rel, active, work, tol, st);
     }
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- rqx_locg_step_static
+// The work space of rqx_locg_step for callers that cannot allocate: a file-scope
+// device array (each GPU holds its own copy until the process ends).  Host code
+// uses its address from hipGetSymbolAddress, looked up once per device with that
+// device current; the array's name is not a device address in host code.
+static __device__ __align__(256) double scratch[RQX_LOCG_STATIC_PROBLEMS * LOCG_WORK];
+
+namespace {
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double *g_scratch[MAX_DEVICES] = {};
+
+// scratch's address on the current device (looked up on its first use there)
+int scratch_on_current_device(double **out) {
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= MAX_DEVICES) return fail(RQX_ERR_LIMIT, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return fail(RQX_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch[dev] = static_cast<double *>(ptr);
+    }
+    *out = g_scratch[dev];
+    return RQX_OK;
+}
+
+// work[k * LOCG_WORK + RQX_LOCG_SCALE] = scale[k] (0 without scale)
+__global__ void k_locg_scale(double *__restrict__ work, const double *__restrict__ scale, int K) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < K) work[static_cast<int64_t>(k) * LOCG_WORK + RQX_LOCG_SCALE] = scale ? scale[k] : 0.0;
+}
+}  // namespace
+
+extern "C" {
+
+int rqx_locg_step_static(const rqx_plan *p, int32_t dtype, int32_t low, void *const *vec, double *theta,
+                         double *rel, uint8_t *active, const double *scale, double tol, rqx_stream_t stream) {
+    g_err.clear();
+    if (int rc = check_plan(p)) return rc;
+    const int K = static_cast<int>(p->probs.size());
+    if (K > RQX_LOCG_STATIC_PROBLEMS)
+        return fail(RQX_ERR_LIMIT, "rqx_locg_step_static: the plan has more than RQX_LOCG_STATIC_PROBLEMS problems");
+    double *work = nullptr;
+    if (int rc = scratch_on_current_device(&work)) return rc;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    hipLaunchKernelGGL(k_locg_scale, dim3(static_cast<unsigned>((K + 63) / 64)), dim3(64), 0, st, work, scale, K);
+    HIP_TRY(hipGetLastError());
+    return rqx_locg_step(p, dtype, low, vec, theta, rel, active, work, tol, stream);
 }
 
 int rqx_eigenpair_batch(const double *mats, int32_t count, int32_t n, double *out, rqx_stream_t stream) {
