@@ -250,11 +250,14 @@ int rqx_plan_build_cache(rqx_plan *plan, int32_t precision, int64_t max_bytes, r
 int rqx_plan_drop_cache(rqx_plan *plan, int32_t precision);
 
 /*
- * out[0..10) = {built, stored hits (nnz), stored values, device bytes, slices,
+ * out[0..14) = {built, stored hits (nnz), stored values, device bytes, slices,
  * complex values, stored columns, first-section levels, second-section values
  * (incl. padding), first-section values of levels without one value for the
- * whole slice} for the cache of one precision (for caches built together: the
- * 64-bit cache counts the shared arrays, the 32-bit its values).
+ * whole slice, second-section hits (the other second-section entries are
+ * padding), padding entries among those first-section values, padding entries
+ * among the stored first-section columns, first-section levels of padding only}
+ * for the cache of one precision (for caches built together: the 64-bit cache
+ * counts the shared arrays, the 32-bit its values).
  */
 int rqx_plan_cache_info(const rqx_plan *plan, int32_t precision, int64_t *out);
 

@@ -93,6 +93,48 @@ void oracle_apply_group(const uint64_t *keys, int64_t n, const int32_t *src, con
     }
 }
 
+/*
+ * The same source indices by binary search (OpenMP over rows): the untimed setup
+ * of the CPU baseline at sizes where the sort-merge of every group would take
+ * minutes (the reference builds this cache once, sqd.py:347-351).
+ */
+void oracle_xsource_bsearch(const uint64_t *keys, int64_t n, uint64_t x, int32_t *out) {
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; ++i) {
+        const uint64_t t = keys[i] ^ x;
+        int64_t lo = 0, hi = n;
+        while (lo < hi) {
+            const int64_t mid = lo + (hi - lo) / 2;
+            if (keys[mid] < t) lo = mid + 1;
+            else hi = mid;
+        }
+        out[i] = (lo < n && keys[lo] == t) ? (int32_t)lo : -1;
+    }
+}
+
+/*
+ * oracle_apply_group in single precision: complex64 vectors (interleaved floats),
+ * coefficients c as complex interleaved doubles rounded to float, sums in float.
+ */
+void oracle_apply_group_c64(const uint64_t *keys, int64_t n, const int32_t *src, const uint64_t *z,
+                            const double *c, int64_t k, const float *v, float *y) {
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; ++i) {
+        const int32_t s = src[i];
+        if (s < 0) continue;
+        float dr = 0.0f, di = 0.0f;
+        const uint64_t b = keys[i];
+        for (int64_t t = 0; t < k; ++t) {
+            const float sg = (__builtin_popcountll(b & z[t]) & 1) ? -1.0f : 1.0f;
+            dr += sg * (float)c[2 * t];
+            di += sg * (float)c[2 * t + 1];
+        }
+        const float vr = v[2 * s], vi = v[2 * s + 1];
+        y[2 * i] += dr * vr - di * vi;
+        y[2 * i + 1] += dr * vi + di * vr;
+    }
+}
+
 int oracle_num_threads(void) {
     int t = 1;
 #pragma omp parallel

@@ -1892,7 +1892,8 @@ __global__ __launch_bounds__(256) void k_spmv_sell(const int32_t *__restrict__ c
 
 // Section widths of each slice (one wave per slice): the largest row count of
 // each section over the slice's rows, rounded up to whole packets; the hits of
-// all rows are summed into *nnz (one atomic per wave).
+// all rows are summed into nnz[0] and the section-B hits into nnz[1] (one atomic
+// each per wave).
 __global__ __launch_bounds__(256) void k_slice_widths(const uint32_t *__restrict__ cnt_b,
                                                       const uint32_t *__restrict__ cnt_a, int64_t n_states,
                                                       int64_t n_slices, uint32_t *__restrict__ wa,
@@ -1901,16 +1902,18 @@ __global__ __launch_bounds__(256) void k_slice_widths(const uint32_t *__restrict
     if (slice >= n_slices) return;
     const int64_t row = slice * SLICE + threadIdx.x % SLICE;
     uint32_t b = row < n_states ? cnt_b[row] : 0u, a = row < n_states ? cnt_a[row] : 0u;
-    unsigned long long sum = static_cast<unsigned long long>(a) + b;
+    unsigned long long sum = static_cast<unsigned long long>(a) + b, sum_b = b;
     for (int off = 32; off > 0; off >>= 1) {
         b = max(b, static_cast<uint32_t>(__shfl_xor(static_cast<int>(b), off, 64)));
         a = max(a, static_cast<uint32_t>(__shfl_xor(static_cast<int>(a), off, 64)));
         sum += __shfl_xor(sum, off, 64);
+        sum_b += __shfl_xor(sum_b, off, 64);
     }
     if (threadIdx.x % SLICE == 0) {
         wa[slice] = (a + PACK - 1) / PACK * PACK;
         wb[slice] = (b + PACK - 1) / PACK * PACK;
         atomicAdd(nnz, sum);
+        atomicAdd(nnz + 1, sum_b);
     }
 }
 
@@ -1923,7 +1926,9 @@ __global__ __launch_bounds__(256) void k_slice_widths(const uint32_t *__restrict
 // group summed the same way for every row: a sign table, or too short for
 // term-parallel sums; or padding only), else the level's index among the
 // slice's levels that store 64 values.  n_stored / n_vals: the slice's
-// LEVEL_GATHER levels / levels with 64 values.
+// LEVEL_GATHER levels / levels with 64 values; pad[0..3) (one atomic each per
+// wave): padding lanes of the levels with 64 values, padding lanes of the
+// LEVEL_GATHER levels, levels of padding only.
 __global__ __launch_bounds__(256) void k_cache_levels(const int32_t *__restrict__ col_full,
                                                       const int32_t *__restrict__ gid_a,
                                                       const int64_t *__restrict__ off, const uint32_t *__restrict__ wa,
@@ -1933,7 +1938,7 @@ __global__ __launch_bounds__(256) void k_cache_levels(const int32_t *__restrict_
                                                       const uint32_t *__restrict__ g_toff, int64_t n_states,
                                                       int64_t n_slices, int32_t *__restrict__ adel,
                                                       int32_t *__restrict__ acode, uint32_t *__restrict__ n_stored,
-                                                      uint32_t *__restrict__ n_vals) {
+                                                      uint32_t *__restrict__ n_vals, unsigned long long *pad) {
     const int64_t slice = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / SLICE;
     if (slice >= n_slices) return;
     const int lane = static_cast<int>(threadIdx.x % SLICE);
@@ -1944,6 +1949,7 @@ __global__ __launch_bounds__(256) void k_cache_levels(const int32_t *__restrict_
     const uint32_t w = wa[slice];
     const int64_t o = off[slice], ao = aoff[slice];
     uint32_t stored = 0, vals = 0;
+    unsigned long long pad_vals = 0, pad_cols = 0, pad_levels = 0;
     for (uint32_t k = 0; k < w; ++k) {
         const int64_t slot = packed_slot<int32_t>(o, k, lane);
         const int32_t d = valid ? col_full[slot] - static_cast<int32_t>(row) : 0;
@@ -1963,10 +1969,17 @@ __global__ __launch_bounds__(256) void k_cache_levels(const int32_t *__restrict_
         }
         stored += affine ? 0u : 1u;
         vals += uniform ? 0u : 1u;
+        const unsigned long long n_pad = __popcll(__ballot(!(valid && g >= 0)));
+        pad_vals += uniform ? 0ull : n_pad;
+        pad_cols += affine ? 0ull : n_pad;
+        pad_levels += n_pad == SLICE ? 1ull : 0ull;
     }
     if (lane == 0) {
         n_stored[slice] = stored;
         n_vals[slice] = vals;
+        atomicAdd(pad, pad_vals);
+        atomicAdd(pad + 1, pad_cols);
+        atomicAdd(pad + 2, pad_levels);
     }
 }
 
@@ -2128,6 +2141,8 @@ struct HitCache {            // sliced-ELL cache of the hits (column, C_j(s_i)) 
     uint32_t *wb = nullptr;    // section-B width of each slice
     int64_t entries = 0, nnz = 0, n_slices = 0, bytes = 0;   // entries: stored values
     int64_t full_entries = 0, col_entries = 0, a_levels = 0, valb_entries = 0, vala_entries = 0;
+    int64_t b_hits = 0;        // section-B hits (the rest of valb is padding)
+    int64_t pad_vala = 0, pad_cola = 0, pad_levels = 0;   // padding of the section-A levels (k_cache_levels)
     int cplx = 0;
     bool built = false;
     bool shared = false;       // both precisions built together: the FP64 cache owns the allocations
@@ -3292,20 +3307,20 @@ int rqx_plan_build_cache(rqx_plan *p, int32_t precision, int64_t max_bytes, rqx_
     // [n, 2n)), then each slice's section widths (after 2n: A widths, B widths,
     // the hit total)
     uint32_t *rowcnt = nullptr;
-    HIP_TRY(hipMalloc(&rowcnt, static_cast<size_t>(2 * n + 2 * n_slices + 2) * 4));
+    HIP_TRY(hipMalloc(&rowcnt, static_cast<size_t>(2 * n + 2 * n_slices + 4) * 4));
     KParams k = make_params(p, true);
     k.rowcnt = rowcnt;
     k.rowcnt_a = rowcnt + n;
     uint32_t *w_dev = rowcnt + 2 * n;
     unsigned long long *nnz_dev = reinterpret_cast<unsigned long long *>(w_dev + 2 * n_slices);
-    hipError_t e = fill_async<unsigned long long>(nnz_dev, 0, 1, 0ull, st);
+    hipError_t e = fill_async<unsigned long long>(nnz_dev, 0, 2, 0ull, st);
     if (e == hipSuccess) e = launch_tiles<MODE_ROWCOUNT, double, true>(p, k, nullptr, nullptr, st);
     if (e == hipSuccess) {
         hipLaunchKernelGGL(k_slice_widths, dim3(static_cast<unsigned>((n_slices * SLICE + 255) / 256)), dim3(256), 0, st,
                            rowcnt, rowcnt + n, n, n_slices, w_dev, w_dev + n_slices, nnz_dev);
         e = hipGetLastError();
     }
-    std::vector<uint32_t> wab(static_cast<size_t>(2 * n_slices + 2));
+    std::vector<uint32_t> wab(static_cast<size_t>(2 * n_slices + 4));
     if (e == hipSuccess) e = hipMemcpyAsync(wab.data(), w_dev, wab.size() * 4, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     const hipError_t ef = hipFree(rowcnt);
@@ -3314,8 +3329,9 @@ int rqx_plan_build_cache(rqx_plan *p, int32_t precision, int64_t max_bytes, rqx_
 
     // offsets of the column pass's full layout and of the stored section B
     const std::vector<uint32_t> wa(wab.begin(), wab.begin() + n_slices), wb(wab.begin() + n_slices, wab.begin() + 2 * n_slices);
-    int64_t nnz = 0;
+    int64_t nnz = 0, nnz_b = 0;
     std::memcpy(&nnz, wab.data() + 2 * n_slices, 8);
+    std::memcpy(&nnz_b, wab.data() + 2 * n_slices + 2, 8);
     std::vector<int64_t> off(static_cast<size_t>(n_slices), 0), aoff(static_cast<size_t>(n_slices), 0),
         boff(static_cast<size_t>(n_slices), 0);
     int64_t full = 0, a_levels = 0, b_entries = 0;
@@ -3373,6 +3389,7 @@ int rqx_plan_build_cache(rqx_plan *p, int32_t precision, int64_t max_bytes, rqx_
     c.n_slices = n_slices;
     c.a_levels = a_levels;
     c.valb_entries = b_entries;
+    c.b_hits = nnz_b;
     c.cplx = cplx;
     c.shared = both;
     int32_t *col_full = nullptr, *gid_a = nullptr;
@@ -3394,7 +3411,10 @@ int rqx_plan_build_cache(rqx_plan *p, int32_t precision, int64_t max_bytes, rqx_
     e = hipMalloc(&col_full, nf * 4);
     if (e == hipSuccess) e = hipMalloc(&gid_a, na * 4);
     if (e == hipSuccess) e = hipMalloc(&off_d, ns8);
-    if (e == hipSuccess) e = hipMalloc(&counts, ns8);
+    if (e == hipSuccess) e = hipMalloc(&counts, ns8 + 3 * 8);
+    unsigned long long *pad_dev = counts ? reinterpret_cast<unsigned long long *>(
+                                               reinterpret_cast<char *>(counts) + ns8) : nullptr;
+    if (e == hipSuccess) e = fill_async<unsigned long long>(pad_dev, 0, 3, 0ull, st);
     if (e == hipSuccess) e = hipMemcpyAsync(off_d, off.data(), off.size() * 8, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipMemcpyAsync(c.boff, boff.data(), boff.size() * 8, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipMemcpyAsync(c.aoff, aoff.data(), aoff.size() * 8, hipMemcpyHostToDevice, st);
@@ -3407,10 +3427,13 @@ int rqx_plan_build_cache(rqx_plan *p, int32_t precision, int64_t max_bytes, rqx_
     // section-A level codes: offsets and single values
     const unsigned wave_blocks = static_cast<unsigned>((n_slices * SLICE + 255) / 256);
     hipLaunchKernelGGL(k_cache_levels, dim3(wave_blocks), dim3(256), 0, st, col_full, gid_a, off_d, c.wa, c.aoff,
-                       p->keys, p->g_zor, p->g_tab, p->g_toff, n, n_slices, c.adel, c.acode, counts, counts + n_slices);
+                       p->keys, p->g_zor, p->g_tab, p->g_toff, n, n_slices, c.adel, c.acode, counts, counts + n_slices,
+                       pad_dev);
     e = hipGetLastError();
     std::vector<uint32_t> hc(static_cast<size_t>(2 * n_slices));
+    unsigned long long pads[3] = {0, 0, 0};
     if (e == hipSuccess) e = hipMemcpyAsync(hc.data(), counts, hc.size() * 4, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(pads, pad_dev, sizeof(pads), hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) return fail_build(e, "cache levels: ");
     std::vector<int64_t> coff(static_cast<size_t>(n_slices), 0), avoff(static_cast<size_t>(n_slices), 0);
@@ -3430,6 +3453,9 @@ int rqx_plan_build_cache(rqx_plan *p, int32_t precision, int64_t max_bytes, rqx_
     c.vala = c.vala_block;
     c.col_entries = col_entries;
     c.vala_entries = a_entries;
+    c.pad_vala = static_cast<int64_t>(pads[0]);
+    c.pad_cola = static_cast<int64_t>(pads[1]);
+    c.pad_levels = static_cast<int64_t>(pads[2]);
     c.entries = b_entries + a_entries + a_levels;
     c.bytes = static_cast<int64_t>(total + align_up(static_cast<size_t>(col_entries) * 4) + o_vala_lo +
                                    (both ? static_cast<size_t>(a_entries) * vb_lo : 0));
@@ -3489,6 +3515,10 @@ int rqx_plan_cache_info(const rqx_plan *p, int32_t precision, int64_t *out) {
     out[7] = c.a_levels;
     out[8] = c.valb_entries;
     out[9] = c.vala_entries;
+    out[10] = c.b_hits;
+    out[11] = c.pad_vala;
+    out[12] = c.pad_cola;
+    out[13] = c.pad_levels;
     return RQX_OK;
 }
 

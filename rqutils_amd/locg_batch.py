@@ -33,7 +33,7 @@ _LO = {torch.float64: torch.float32, torch.complex128: torch.complex64}
 class BatchLOCG:
     """Persistent buffers + one LOCG step for all problems of a plan (``BatchPlan`` or single ``Plan``)."""
 
-    def __init__(self, plan: Plan, x0: torch.Tensor, tol=1e-7, switch_tol=1e-5, mixed=True, lo_storage=True):
+    def __init__(self, plan: Plan, x0: torch.Tensor, tol=1e-7, switch_tol=1e-6, mixed=True, lo_storage=True):
         if x0.dtype not in _LO:
             raise ValueError('x0 must be float64 or complex128')
         if x0.shape != (plan.n_total,) or x0.device != plan.device:
@@ -195,23 +195,26 @@ class BatchLOCG:
         self._apply(self.y, self.ay)
 
 
-def batch_ground_locg(plan: Plan, x0, tol=1e-7, switch_tol=1e-5, maxiter=1000, check_every=10,
+def batch_ground_locg(plan: Plan, x0, tol=1e-7, switch_tol=1e-6, maxiter=1000, check_every=10,
                       use_graph=True, mixed=True, record=False, energy_tol=None, energy_checks=2,
-                      energy_rel_residual=1e-7,
-                      switch_energy=1e-9, lo_storage=True, check_every_hi=1):
+                      energy_rel_residual=1e-7, floor_polls=3, lo_storage=True, check_every_hi=1):
     """Ground eigenpairs of every problem of ``plan`` (a ``BatchPlan``, or a ``Plan`` as one problem).
 
+    The bulk phase (low-precision matvec) runs until every active problem's
+    relative residual, polled every ``check_every`` steps, is at most
+    ``switch_tol`` or has reached its low-precision floor: its best value has not
+    improved by 1.5x at ``floor_polls`` consecutive polls (a residual 100x above
+    the best so far, a descent after a stall near an excited eigenvector, starts
+    a new best).  The FP64 phase then polls every ``check_every_hi`` steps (its
+    start vector is converged to the FP32 floor, so a few steps finish it).
+
     A problem stops (its vectors freeze) at relative residual ``tol``, or, with
-    ``energy_tol``, once its theta, polled every ``check_every`` steps, has
-    settled: the last decrease and the geometric (Aitken) estimate of the
-    remaining decrease are both below ``energy_tol * |theta|`` at
-    ``energy_checks`` consecutive polls, with its relative residual at most
-    ``energy_rel_residual`` (FP64 phase).  In the low-precision phase the same
-    test with the limit ``max(energy_tol, 30 switch_energy)`` (theta carries
-    FP32 image errors there) hands every problem over to the FP64 phase, which
-    then polls every ``check_every_hi`` steps: it starts from a vector the FP32
-    phase has converged, so its energy settles within a few steps and a
-    sparse poll would only add steps.
+    ``energy_tol``, in the FP64 phase once its theta has settled -- the last
+    decrease and the geometric (Aitken) estimate of the remaining decrease both
+    below ``energy_tol * |theta|`` at ``energy_checks`` consecutive polls -- and
+    its relative residual is at most ``energy_rel_residual`` (near an excited
+    eigenvector theta can sit still for tens of iterations with a residual of
+    ~1e-6 before it falls to the ground energy).
 
     Returns ``(theta (K,), x, iterations, info)`` where ``info`` holds per-problem
     relative residuals, the iteration of the precision switch and timings.
@@ -239,50 +242,52 @@ def batch_ground_locg(plan: Plan, x0, tol=1e-7, switch_tol=1e-5, maxiter=1000, c
             return
         g.replay()
 
+    K = solver.K
     it = 1
     switch_at = None
     history = []
     thetas = []
-    calm = torch.zeros(solver.K, dtype=torch.int64)
+    calm = torch.zeros(K, dtype=torch.int64)
+    best = torch.full((K,), float('inf'), dtype=torch.float64)
+    since = torch.zeros(K, dtype=torch.int64)
     while it < maxiter:
         for _ in range(check_every if switch_at is None else check_every_hi):
             run_step()
             it += 1
         # one device -> host copy per poll: active flags, theta, relative residuals
         status = torch.cat([solver.active.to(torch.float64), solver.theta, solver.rel]).cpu()
-        K = solver.K
         act = status[:K] > 0.5
         th = status[K:2 * K]
+        rel = status[2 * K:]
         active = int(act.sum())
-        worst = float(status[2 * K:][act].max()) if active else 0.0
+        worst = float(rel[act].max()) if active else 0.0
         if record:
             history.append((time.perf_counter() - t0, it, active, worst, solver.phase))
-        settled = torch.zeros(solver.K, dtype=torch.bool)
-        if energy_tol is not None:
-            thetas.append(th)
-            if len(thetas) >= 3:
-                d0, d1 = thetas[-3] - thetas[-2], thetas[-2] - thetas[-1]
-                scale = energy_tol if solver.phase == 'hi' else max(energy_tol, 30 * switch_energy)
-                lim = scale * th.abs()
-                tail = torch.where((d0 > d1) & (d1 >= 0), d1 * d1 / (d0 - d1), torch.full_like(d1, float('inf')))
-                ok = (d1 >= 0) & (d1 <= lim) & (tail <= lim)
-                calm = torch.where(ok, calm + 1, torch.zeros_like(calm))
-                settled = calm >= energy_checks
         if solver.phase == 'lo':
-            if (worst <= solver.switch_tol or active == 0 or it >= maxiter // 2 or
-                    (energy_tol is not None and bool((settled | ~act).all()))):
+            restart = rel > 100 * best
+            better = rel < best / 1.5
+            since = torch.where(restart | better, torch.zeros_like(since), since + 1)
+            best = torch.where(restart | better, rel, best)
+            ready = ~act | (rel <= solver.switch_tol) | (since >= floor_polls)
+            if bool(ready.all()) or it >= maxiter // 2:
                 solver.to_high_precision()
                 solver.active.fill_(True)
                 switch_at = it
-                thetas.clear()
-                calm.zero_()
             continue
-        if energy_tol is not None:
-            # a stalled theta is not yet converged: near an excited eigenvector theta can sit
-            # still for tens of iterations (relative residual ~1e-6) before it falls to the
-            # ground energy, so the FP64 phase also asks for a small residual
-            settled &= status[2 * K:] <= energy_rel_residual
-        if energy_tol is not None and bool((settled & act).any()):
+        if energy_tol is None:
+            if active == 0:
+                break
+            continue
+        thetas.append(th)
+        settled = torch.zeros(K, dtype=torch.bool)
+        if len(thetas) >= 3:
+            d0, d1 = thetas[-3] - thetas[-2], thetas[-2] - thetas[-1]
+            lim = energy_tol * th.abs()
+            tail = torch.where((d0 > d1) & (d1 >= 0), d1 * d1 / (d0 - d1), torch.full_like(d1, float('inf')))
+            ok = (d1 >= 0) & (d1 <= lim) & (tail <= lim)
+            calm = torch.where(ok, calm + 1, torch.zeros_like(calm))
+            settled = (calm >= energy_checks) & (rel <= energy_rel_residual)
+        if bool((settled & act).any()):
             solver.active.copy_((act & ~settled).to(plan.device))
             active = int((act & ~settled).sum())
         if active == 0:

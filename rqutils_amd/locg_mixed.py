@@ -11,10 +11,11 @@ the vector dtype and spends four matvecs per iteration, ``:256,346``).  Here:
 * ``[x, y, p]`` is orthonormalised in FP64 (classical Gram-Schmidt, two
   passes), the 3x3 Rayleigh-Ritz matrix ``V^H A V`` is formed from the carried
   images and solved by the same analytic eigen routine as the plain path;
-* once the FP64 residual of the low phase reaches ``switch_tol`` (relative to
-  ``|theta|``) or stops improving, ``Ax`` and ``Ay`` are recomputed with the
-  FP64 matvec (``matvec_hi``) and the iteration continues in FP64 until
-  ``tol``: the returned energy is a Rayleigh quotient of FP64 images.
+* once the FP64 residual of the low phase reaches ``switch_tol`` or stops
+  improving (the low-precision floor), ``Ax`` and ``Ay`` are recomputed with
+  the FP64 matvec (``matvec_hi``) and the iteration continues in FP64 until
+  ``tol`` (or the energy test): the returned energy is a Rayleigh quotient of
+  FP64 images.
 
 Per iteration: 1 matvec, about 25 FP64 vector passes, 1 host synchronisation
 every ``check_every`` iterations.
@@ -48,8 +49,8 @@ class MixedLOCGResult:
 
 
 def ground_locg_mixed(matvec_hi, matvec_lo, xinit, maxiter=1000, tol=1e-6, switch_tol=1e-6,
-                      check_every=1, stagnation=100, record=False, energy_tol=None, energy_checks=3,
-                      switch_energy=1e-9, energy_rel_residual=1e-7):
+                      check_every=1, stagnation=15, record=False, energy_tol=None, energy_checks=3,
+                      energy_rel_residual=1e-7):
     """Ground eigenpair with a low-precision bulk matvec and FP64 refinement.
 
     Args:
@@ -62,14 +63,17 @@ def ground_locg_mixed(matvec_hi, matvec_lo, xinit, maxiter=1000, tol=1e-6, switc
             images (``x0`` the normalised start vector).
         switch_tol: leave the low-precision phase at this relative residual.
         check_every: iterations between host-side convergence checks.
-        stagnation: leave the low phase after this many checks without a new
-            best residual.
+        stagnation: also leave the low phase after this many checks without
+            improving the best residual by 1.5x (its low-precision floor); a
+            residual 100x above the best, a descent after a stall near an
+            excited eigenvector, starts a new best.
         record: keep ``(seconds, theta, rel_residual, phase)`` per check.
         energy_tol: if set, also stop in the FP64 phase once the remaining
             decrease of theta, estimated from the last three decreases as a
             geometric tail (Aitken), and the last decrease are both below
             ``energy_tol * |theta|`` for ``energy_checks`` consecutive checks and
-            the relative residual is at most ``energy_rel_residual``.
+            the relative residual is at most ``energy_rel_residual`` (a stalled
+            theta near an excited eigenvector is not converged).
 
     Returns:
         ``MixedLOCGResult`` (iterable as ``theta, x, niter``).
@@ -99,7 +103,6 @@ def ground_locg_mixed(matvec_hi, matvec_lo, xinit, maxiter=1000, tol=1e-6, switc
     niter = 0
     hi_thetas = []
     calm = 0
-    force_switch = False
     while True:
         theta = _dot(x, ax).real
         r = ax - theta.to(hi) * x
@@ -108,41 +111,31 @@ def ground_locg_mixed(matvec_hi, matvec_lo, xinit, maxiter=1000, tol=1e-6, switc
             rel = float(_nrm(r)) / max(abs(float(theta)), scale, 1e-300)
             if record:
                 history.append((time.perf_counter() - t_start, float(theta), rel, phase))
-            if energy_tol is not None:
-                th = float(theta)
-                hi_thetas.append(th)
-                if len(hi_thetas) >= 3:
-                    d0 = hi_thetas[-3] - hi_thetas[-2]
-                    d1 = hi_thetas[-2] - hi_thetas[-1]
-                    # the low phase's theta carries FP32 image errors: compare with a
-                    # looser limit there and hand over to the FP64 phase when calm
-                    lim = (energy_tol if phase == 'hi' else max(energy_tol, 30 * switch_energy)) * abs(th)
-                    tail = d1 * d1 / (d0 - d1) if d0 > d1 >= 0 else float('inf')
-                    calm = calm + 1 if (0 <= d1 <= lim and tail <= lim) else 0
-                    if calm >= energy_checks:
-                        # a stalled theta near an excited eigenvector is not converged: the
-                        # FP64 phase also asks for a small residual
-                        if phase == 'hi':
-                            if rel <= energy_rel_residual:
-                                break
-                            calm -= 1
-                        else:
-                            force_switch = True
-                            hi_thetas.clear()
-                            calm = 0
             if phase == 'lo':
-                if rel < best * 0.98:
+                if rel > 100 * best or rel < best / 1.5:
                     best, since_best = rel, 0
                 else:
                     since_best += 1
-                if rel <= switch_tol or since_best >= stagnation or niter >= maxiter or force_switch:
+                if rel <= switch_tol or since_best >= stagnation or niter >= maxiter:
                     phase = 'hi'
                     ax = mv(x, 'hi')
                     if y is not None:
                         ay = mv(y, 'hi')
                     continue
-            elif rel <= tol or niter >= maxiter:
-                break
+            else:
+                if rel <= tol or niter >= maxiter:
+                    break
+                if energy_tol is not None:
+                    th = float(theta)
+                    hi_thetas.append(th)
+                    if len(hi_thetas) >= 3:
+                        d0 = hi_thetas[-3] - hi_thetas[-2]
+                        d1 = hi_thetas[-2] - hi_thetas[-1]
+                        lim = energy_tol * abs(th)
+                        tail = d1 * d1 / (d0 - d1) if d0 > d1 >= 0 else float('inf')
+                        calm = calm + 1 if (0 <= d1 <= lim and tail <= lim) else 0
+                        if calm >= energy_checks and rel <= energy_rel_residual:
+                            break
         # residual direction, orthonormal to x and y (two CGS passes)
         p = r
         for _ in range(2):

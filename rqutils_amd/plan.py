@@ -278,12 +278,14 @@ class Plan:
 
     def cache_info(self, precision=64):
         """``built``, stored hits, stored values, device bytes, slices, complex values,
-        stored columns, first-section levels, second-section values (incl. padding)
-        and first-section values of the levels without a single value."""
-        out = np.zeros(10, dtype=np.int64)
+        stored columns, first-section levels, second-section values (incl. padding),
+        first-section values of the levels without a single value, second-section
+        hits, and the padding entries among the stored first-section values and
+        columns and the first-section levels of padding only."""
+        out = np.zeros(14, dtype=np.int64)
         _capi.check(_capi.lib().rqx_plan_cache_info(self._handle, int(precision), _capi.ptr(out)))
         keys = ('built', 'nnz', 'entries', 'bytes', 'slices', 'complex', 'col_entries', 'a_levels',
-                'b_entries', 'a_entries')
+                'b_entries', 'a_entries', 'b_hits', 'a_pad_values', 'a_pad_columns', 'a_pad_levels')
         info = {k: int(v) for k, v in zip(keys, out)}
         info['built'] = bool(info['built'])
         info['complex'] = bool(info['complex'])
@@ -312,6 +314,19 @@ class Plan:
         s_d = 16 if precision == 64 else 8
         return (info['entries'] * s_val + 4 * info['col_entries'] + 8 * info['a_levels'] + 40 * info['slices'] +
                 self.n_states * (s_d + 2 * s_v))
+
+    def cached_padding_bytes(self, dtype):
+        """Bytes of ``cached_bytes`` that are padding: second-section entries beyond each
+        row's hits (value and column), padding lanes of the stored first-section values
+        and columns, and first-section levels of padding only (codes and value)."""
+        precision = 64 if dtype in (torch.float64, torch.complex128) else 32
+        info = self.cache_info(precision)
+        if not info['built']:
+            raise ValueError(f'no {precision}-bit cache is built')
+        s_val = (8 if precision == 64 else 4) * (2 if info['complex'] else 1)
+        b_pad = info['b_entries'] - info['b_hits']
+        return (b_pad * (s_val + 4) + info['a_pad_values'] * s_val + info['a_pad_columns'] * 4 +
+                info['a_pad_levels'] * (8 + s_val))
 
     # ------------------------------------------------------------------ per-problem vector ops
     def dots(self, pairs):

@@ -268,19 +268,27 @@ def test_cport_matches_numpy_oracle(kind):
     else:
         assert not t.is_real
     for xm in list(t.xmask[:20]) + [0, (1 << 12) - 1]:
-        assert np.array_equal(cport.xsource(keys, xm), sqdref.xsource_searchsorted(keys, xm))
+        want_src = sqdref.xsource_searchsorted(keys, xm)
+        assert np.array_equal(cport.xsource(keys, xm), want_src)
+        assert np.array_equal(cport.xsource_bsearch(keys, xm), want_src)
     rng = np.random.default_rng(5)
     v = rng.standard_normal(keys.shape[0]) + 1j * rng.standard_normal(keys.shape[0])
     want = sqdref.matvec_u64(t.xmask, _groups(t), keys, v)
     got = cport.matvec(t, keys, v)
     assert np.max(np.abs(got - want)) <= 1e-13 * np.max(np.abs(want))
+    # single precision (the CPU baseline of complex64 matvecs): FP32 rounding only
+    got32 = cport.matvec(t, keys, v.astype(np.complex64), bsearch=True)
+    assert got32.dtype == np.complex64
+    err = np.max(np.abs(got32 - want)) / np.max(np.abs(want))
+    assert err <= 2e-5 and err > 1e-12
 
 
 def test_cport_timed_sample_extrapolates_one_matvec():
     cport = _require_cport()
     t, keys = _tables_case('complex', 34)
-    secs, groups, threads = cport.timed_sample(t, keys, budget_s=0.5)
-    assert secs > 0 and 1 <= groups <= t.num_groups and threads >= 1
+    for dt in (np.complex128, np.complex64):
+        secs, groups, threads, frac = cport.timed_sample(t, keys, budget_s=0.5, dtype=dt)
+        assert secs > 0 and 1 <= groups <= t.num_groups and threads >= 1 and 0 < frac <= 1
 
 
 @pytest.mark.parametrize('kind', ['complex', 'no_diagonal', 'single_term'])
