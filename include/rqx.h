@@ -79,6 +79,10 @@ typedef struct {
     int32_t has_diagonal;
     int32_t is_real;
     double cost;            /* cost-model estimate of one matvec (arbitrary units) */
+    int32_t section_bits;   /* hit-cache section split: groups whose X mask has no bit below it
+                               form the first section (>= split_bits; larger when the states are a
+                               product of low and high strings at that bit) */
+    int32_t reserved;
 } rqx_problem_info;
 
 /* Library version (major*10000 + minor*100 + patch). */

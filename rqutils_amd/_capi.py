@@ -91,6 +91,8 @@ class ProblemInfo(ctypes.Structure):
         ('has_diagonal', ctypes.c_int32),
         ('is_real', ctypes.c_int32),
         ('cost', ctypes.c_double),
+        ('section_bits', ctypes.c_int32),
+        ('reserved', ctypes.c_int32),
     ]
 
     def as_dict(self):

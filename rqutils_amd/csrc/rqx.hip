@@ -97,6 +97,60 @@ template <typename T> __device__ inline void store_packet(void *a, int64_t off, 
     }
 }
 
+// Section B is stored as SELL-8-64: the rows of a slice in descending order of
+// their section-B entry count (position l = 0..63, ties in row order), chunk j =
+// positions [8 j, 8 j + 8) padded to its widest row, W[j] packets.  Packet level q
+// holds the packets of the chunks with W[j] > q -- a prefix of the positions,
+// since the widths descend -- lane-contiguous: entry k (q = k / PACK, u = k % PACK)
+// of position l sits at
+//   off[s] + 32 sum_j min(q, W[j]) + (u / e) n_q e + l e + u % e,   n_q = 8 #{j : W[j] > q}.
+constexpr int CH = 8;                  // rows per chunk
+constexpr int NCH = SLICE / CH;        // chunks per slice
+
+struct SellLevel {
+    int64_t base;   // first entry of the level, from the slice's first entry
+    int32_t n;      // positions holding a packet at this level
+};
+
+__host__ __device__ inline SellLevel sell_level(const uint32_t *W, uint32_t q) {
+    int64_t b = 0;
+    int32_t m = 0;
+#pragma unroll
+    for (int j = 0; j < NCH; ++j) {
+        b += W[j] < q ? W[j] : q;
+        m += W[j] > q ? 1 : 0;
+    }
+    return SellLevel{b * (CH * PACK), m * CH};
+}
+
+template <typename T> __host__ __device__ inline int64_t sell_slot(int64_t off, const SellLevel &lv, uint32_t u, int pos) {
+    constexpr int E = pack_sub<T>();
+    return off + lv.base + static_cast<int64_t>(u / E) * (lv.n * E) + pos * E + (u % E);
+}
+
+// the packet of level q of position pos, as 16-byte stores
+template <typename T>
+__device__ inline void store_packet_sell(void *a, int64_t off, const uint32_t *W, uint32_t q, int pos, const T *v) {
+    constexpr int E = pack_sub<T>();
+    const SellLevel lv = sell_level(W, q);
+#pragma unroll
+    for (int i = 0; i < PACK / E; ++i) {
+        i32x4 t;
+        __builtin_memcpy(&t, v + i * E, 16);
+        *reinterpret_cast<i32x4 *>(static_cast<T *>(a) + sell_slot<T>(off, lv, i * E, pos)) = t;
+    }
+}
+
+// the chunk widths of a slice (8 x uint16 in 16 bytes), widened
+__device__ inline void load_chunk_widths(const uint16_t *bw, int64_t slice, uint32_t *W) {
+    const i32x4 t = *reinterpret_cast<const i32x4 *>(bw + slice * NCH);
+#pragma unroll
+    for (int j = 0; j < NCH / 2; ++j) {
+        W[2 * j] = static_cast<uint32_t>(t[j]) & 0xffffu;
+        W[2 * j + 1] = static_cast<uint32_t>(t[j]) >> 16;
+    }
+}
+
 // column / value pass: the last PACK section-B entries of a row (column, or
 // value and, when both precisions are filled, FP32 value), stored once the
 // packet is whole
@@ -140,8 +194,10 @@ struct ProbDesc {
     uint32_t grp_base;              // first group (the x == 0 group when has_diag)
     uint32_t lomask;                // low-key mask 2^L - 1 of the problem's split
     int32_t has_diag;
-    int32_t pad;
+    int32_t a_bits;                 // section split: groups with x & (2^a_bits - 1) == 0 are section A (a_bits >= L)
 };
+
+__device__ inline uint64_t section_mask(int32_t a_bits) { return a_bits >= 64 ? ~0ull : ((1ull << a_bits) - 1ull); }
 
 // What the coefficient evaluator needs of a group, fetched with one scalar load
 // (separate arrays cost one dependent round trip each: scalar loads return out
@@ -213,7 +269,9 @@ struct KParams {
     int32_t *c_gid;
     const int64_t *c_off;
     const uint32_t *c_wa;          // section-A entries per row of each slice
-    const uint32_t *c_wb;          // section-B entries per row of each slice
+    const uint32_t *c_wb;          // section-B entries per row of each slice (the widest row)
+    const uint8_t *c_bpos;         // section B: each row's position in its slice (SELL order)
+    const uint16_t *c_bw;          // section B: the 8 chunk widths (packets) of each slice
     const int64_t *c_aoff;
     const int64_t *c_boff;
     const int64_t *c_avoff;
@@ -720,18 +778,25 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
     // value and first stored section-A value (value pass), first section-A level
     int64_t fill_off = 0, fill_bo = 0, fill_avo = 0, fill_ao = 0;
     uint32_t fill_wa = 0;
+    // section B (SELL): the row's position in its slice and the slice's chunk widths
+    int fill_pos = 0;
+    uint32_t fill_W[NCH] = {};
     if constexpr (MODE == MODE_FILLCOL) {
         if (active) {
             fill_off = p.c_off[i / SLICE];
             fill_wa = p.c_wa[i / SLICE];
             fill_ao = p.c_aoff[i / SLICE];
             fill_bo = p.c_boff[i / SLICE];
+            fill_pos = p.c_bpos[i];
+            load_chunk_widths(p.c_bw, i / SLICE, fill_W);
         }
     } else if constexpr (MODE == MODE_FILL) {
         if (active) {
             fill_bo = p.c_boff[i / SLICE];
             fill_avo = p.c_avoff[i / SLICE];
             fill_ao = p.c_aoff[i / SLICE];
+            fill_pos = p.c_bpos[i];
+            load_chunk_widths(p.c_bw, i / SLICE, fill_W);
         }
     }
     // section A, column pass: entry k of this row stored at once (in product
@@ -750,7 +815,8 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
         if (p.c_val_lo) static_cast<FL *>(code < 0 ? p.c_aval_lo : p.c_vala_lo)[o] = l;
     };
     // section B: append entry k of this row; a whole packet goes to memory
-    // (columns and values in the column pass, values in a full value pass)
+    // (columns of the full layout and SELL values in the column pass, SELL values
+    // in a full value pass)
     auto fill_push = [&](FillBuf<FV, FL> &b, uint32_t k, int32_t c, FV v, FL l) {
 #pragma unroll
         for (int u = 0; u + 1 < PACK; ++u) {
@@ -764,8 +830,8 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
         if (k % PACK == PACK - 1) {
             const uint32_t kb = k - (PACK - 1);
             if constexpr (MODE == MODE_FILLCOL) store_packet<int32_t>(p.c_col, fill_off, fill_wa + kb, ln, b.c);
-            store_packet<FV>(p.c_val, fill_bo, kb, ln, b.v);
-            if (p.c_val_lo) store_packet<FL>(p.c_val_lo, fill_bo, kb, ln, b.l);
+            store_packet_sell<FV>(p.c_val, fill_bo, fill_W, kb / PACK, fill_pos, b.v);
+            if (p.c_val_lo) store_packet_sell<FL>(p.c_val_lo, fill_bo, fill_W, kb / PACK, fill_pos, b.l);
         }
     };
     V gv = vzero(static_cast<V *>(nullptr));
@@ -922,12 +988,15 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
                 }
             };
             uint32_t g = g0;
-            // groups are sorted by X mask, so a class's group without low flips
-            // (section A) is its first one
+            // groups are sorted by X mask, so a class's group without low flips is
+            // its first one; it is section A when it also flips no bit below the
+            // problem's section split (its bits [L, a_bits) are zero), else section B
             if (g < g1 && ldc(p.g_xlo + g) == 0) {
+                const bool sec_a = (ldc(p.g_x + g) & section_mask(pd.a_bits)) == 0;
                 uint32_t base;
                 const bool f = search(slo, base);
-                handle(g, base, f, true);
+                if (!(MODE == MODE_FILL && p.c_a_only && !sec_a))   // section B was filled by the column pass
+                    handle(g, base, f, sec_a);
                 ++g;
             }
             if constexpr (MODE == MODE_FILL) {
@@ -1010,9 +1079,10 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
         }
     } else if constexpr (MODE == MODE_FILL || MODE == MODE_FILLCOL) {
         if (active) {
-            // pad to the slice's widths (multiples of PACK): the last packets go
-            // out; padding has the row's own column, group -1 and value 0
-            const uint32_t wa = p.c_wa[i / SLICE], wb = p.c_wb[i / SLICE];
+            // pad to the slice's section-A width and to the row's chunk width in
+            // section B (multiples of PACK): the last packets go out; padding has
+            // the row's own column, group -1 and value 0
+            const uint32_t wa = p.c_wa[i / SLICE], wb = fill_W[fill_pos / CH] * PACK;
             const FV v0 = vzero(static_cast<FV *>(nullptr));
             const FL l0 = vzero(static_cast<FL *>(nullptr));
             for (uint32_t k = static_cast<uint32_t>(n_hit_a); k < wa; ++k) {
@@ -1730,6 +1800,30 @@ template <typename V, typename CV> constexpr int spmv_unroll() {
 // one XCD's L2 sees the gathers of neighbouring rows
 __device__ inline uint32_t spmv_block() { return xcd_remap(blockIdx.x, gridDim.x); }
 
+// The cached matvec's workgroups take SPMV_WG rows (8 slices) each and first copy
+// x over their staging window -- the rows whose states share the high part (bits
+// >= the problem's section split) of the workgroup's middle row, clipped to the
+// LDS budget around that row -- into LDS.  Second-section hits whose source lies
+// in the window (in a product subspace: the flips of low strings only, the bulk
+// of them) then read x from LDS; the others read global memory.  The values read
+// are the same either way.
+constexpr int SPMV_WG = 512;
+constexpr int SPMV_LDS_BYTES = 64 * 1024;
+
+template <typename V> constexpr int stage_cap() { return SPMV_LDS_BYTES / static_cast<int>(sizeof(V)); }
+
+// x[c] from the staged window [lo, lo + len) when it holds c, else from memory;
+// branch free (both loads issue; a lane outside the window reads the window's
+// first entry from memory, the same address for every such lane)
+template <typename V>
+__device__ inline V gather_staged(const V *__restrict__ x, const V *xs, int32_t lo, int32_t len, int32_t c) {
+    const uint32_t o = static_cast<uint32_t>(c - lo);
+    const bool in = o < static_cast<uint32_t>(len);
+    const V a = xs[in ? o : 0u];
+    const V b = x[in ? lo : c];
+    return in ? a : b;
+}
+
 template <typename T> __device__ inline T stream_load(const T *p) { return __builtin_nontemporal_load(p); }
 __device__ inline float2 stream_load(const float2 *p) {
     return make_float2(__builtin_nontemporal_load(&p->x), __builtin_nontemporal_load(&p->y));
@@ -1785,6 +1879,71 @@ __device__ inline void spmv_section(V &acc, const int32_t *c, const CV *vv, uint
         for (int u = 0; u < PACK; ++u) fma_acc(acc, cv[u], x[ci[u]]);
         c += STEP;
         vv += STEP;
+    }
+}
+
+// one packet of a SELL level: its 16-byte pieces are n e entries apart
+template <typename T>
+__device__ inline void load_packet_sell(const T *p, int32_t n, T *out) {
+    constexpr int E = pack_sub<T>();
+    static_assert(sizeof(T) * E == 16, "16-byte pieces");
+    constexpr int N = PACK / E;
+    i32x4 t[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+        t[i] = __builtin_nontemporal_load(reinterpret_cast<const i32x4 *>(p + static_cast<int64_t>(i) * n * E));
+    __builtin_memcpy(out, t, sizeof(t));
+}
+
+// acc += section B of the row at SELL position pos, in order: level q holds a
+// packet for the positions below n_q (the chunk widths W are wave-uniform, so the
+// level bases are scalar), U / PACK levels' loads, then their gathers, then the
+// multiply-adds.  Branch free: a position past n_q reads position 0's packet
+// (the same cache lines) and adds value 0 times its own row's x, as the padding
+// of a fully padded layout would.
+template <int U, typename V, typename CV>
+__device__ inline void spmv_section_sell(V &acc, const int32_t *c, const CV *vv, const uint32_t *W, int pos,
+                                         int32_t row, const V *__restrict__ x, const V *xs, int32_t lo, int32_t len) {
+    static_assert(U % PACK == 0, "whole packets per batch");
+    constexpr int L = U / PACK;
+    constexpr int EC = pack_sub<int32_t>(), EV = pack_sub<CV>();
+    const CV zero = vzero(static_cast<CV *>(nullptr));
+    const uint32_t w0 = W[0];
+    uint32_t q = 0;
+    for (; q + L <= w0; q += L) {
+        int32_t ci[U];
+        CV cv[U];
+        V xv[U];
+        bool on[L];
+#pragma unroll
+        for (int l = 0; l < L; ++l) {
+            const SellLevel lv = sell_level(W, q + l);
+            on[l] = pos < lv.n;
+            const int at = on[l] ? pos : 0;
+            load_packet_sell(c + lv.base + at * EC, lv.n, ci + l * PACK);
+            load_packet_sell(vv + lv.base + at * EV, lv.n, cv + l * PACK);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            ci[u] = on[u / PACK] ? ci[u] : row;
+            cv[u] = on[u / PACK] ? cv[u] : zero;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) xv[u] = gather_staged(x, xs, lo, len, ci[u]);
+#pragma unroll
+        for (int u = 0; u < U; ++u) fma_acc(acc, cv[u], xv[u]);
+    }
+    for (; q < w0; ++q) {
+        const SellLevel lv = sell_level(W, q);
+        const bool on = pos < lv.n;
+        const int at = on ? pos : 0;
+        int32_t ci[PACK];
+        CV cv[PACK];
+        load_packet_sell(c + lv.base + at * EC, lv.n, ci);
+        load_packet_sell(vv + lv.base + at * EV, lv.n, cv);
+#pragma unroll
+        for (int u = 0; u < PACK; ++u)
+            fma_acc(acc, on ? cv[u] : zero, gather_staged(x, xs, lo, len, on ? ci[u] : row));
     }
 }
 
@@ -1857,51 +2016,121 @@ __device__ inline uint32_t spmv_section_a(V &acc, const int32_t *c, const CV *va
 // stored columns at coff[s] (stored section-A levels, then B), section-B
 // values at boff[s].
 template <typename R, bool CPLXV, bool CPLXC>
-__global__ __launch_bounds__(256) void k_spmv_sell(const int32_t *__restrict__ col, const void *__restrict__ valb,
-                                                   const void *__restrict__ vala, const void *__restrict__ aval,
-                                                   const int64_t *__restrict__ boff, const int64_t *__restrict__ avoff,
-                                                   const int64_t *__restrict__ coff, const int64_t *__restrict__ aoff,
-                                                   const int32_t *__restrict__ adel, const int32_t *__restrict__ acode,
-                                                   const uint32_t *__restrict__ wa, const uint32_t *__restrict__ wb,
-                                                   const void *__restrict__ diag, int64_t n_states,
-                                                   const void *__restrict__ xin, void *__restrict__ yout) {
+__global__ __launch_bounds__(SPMV_WG) void k_spmv_sell(const int32_t *__restrict__ col, const void *__restrict__ valb,
+                                                       const void *__restrict__ vala, const void *__restrict__ aval,
+                                                       const int64_t *__restrict__ boff, const int64_t *__restrict__ avoff,
+                                                       const int64_t *__restrict__ coff, const int64_t *__restrict__ aoff,
+                                                       const int32_t *__restrict__ adel, const int32_t *__restrict__ acode,
+                                                       const uint32_t *__restrict__ wa, const uint8_t *__restrict__ bperm,
+                                                       const uint16_t *__restrict__ bw, const int2 *__restrict__ win,
+                                                       const void *__restrict__ diag, int64_t n_states,
+                                                       const void *__restrict__ xin, void *__restrict__ yout) {
     using V = typename VT<R, CPLXV>::T;
     using CC = typename C2<R>::T;
     using CV = typename std::conditional<CPLXC, CC, R>::type;
     constexpr int U = spmv_unroll<V, CV>();
-    const int64_t row = static_cast<int64_t>(spmv_block()) * blockDim.x + threadIdx.x;
-    if (row >= n_states) return;
-    // a wave's 64 rows are one slice (256-thread blocks start on slice bounds)
-    const int64_t slice = uniform64(row / SLICE);   // per-slice tables: scalar loads
-    const int lane = static_cast<int>(row % SLICE);
+    extern __shared__ int4 stage_lds[];
+    V *xs = reinterpret_cast<V *>(stage_lds);
     const V *x = static_cast<const V *>(xin);
+    const uint32_t wg = spmv_block();
+    // the staging window: the workgroup's block of rows, clipped to the LDS budget
+    // around its middle row
+    const int2 bwin = ldc(win + wg);
+    int32_t lo = bwin.x, len = bwin.y;
+    if (len > stage_cap<V>()) {
+        const int64_t mid = std::min<int64_t>(static_cast<int64_t>(wg) * SPMV_WG + SPMV_WG / 2, n_states - 1);
+        const int64_t l = std::max<int64_t>(lo, std::min<int64_t>(mid - stage_cap<V>() / 2,
+                                                                  static_cast<int64_t>(lo) + len - stage_cap<V>()));
+        lo = static_cast<int32_t>(l);
+        len = stage_cap<V>();
+    }
+    for (int32_t i = threadIdx.x; i < len; i += SPMV_WG) xs[i] = x[lo + i];
+    __syncthreads();
+    // a wave's 64 lanes are one slice; lane l takes the slice's row at SELL position l
+    const int64_t first = static_cast<int64_t>(wg) * SPMV_WG + (threadIdx.x & ~(SLICE - 1));
+    const int64_t slice = uniform64(first / SLICE);   // per-slice tables: scalar loads
+    const int pos = static_cast<int>(threadIdx.x % SLICE);
+    if (first >= n_states) return;
+    const int lane = bperm[slice * SLICE + pos];      // the row within the slice
+    const int64_t row = slice * SLICE + lane;
+    if (row >= n_states) return;
     V acc = vzero(static_cast<V *>(nullptr));
     if (diag) fma_acc(acc, static_cast<const CC *>(diag)[row], x[row]);   // NULL: no problem has an x = 0 group
-    const uint32_t w_a = ldc(wa + slice), w_b = ldc(wb + slice);
+    const uint32_t w_a = ldc(wa + slice);
     const int64_t bo = ldc(boff + slice), avo = ldc(avoff + slice);
     const int64_t co = ldc(coff + slice), ao = ldc(aoff + slice);
+    uint32_t W[NCH];
+    const i32x4 wv = ldc(reinterpret_cast<const i32x4 *>(bw + slice * NCH));
+#pragma unroll
+    for (int j = 0; j < NCH / 2; ++j) {
+        W[2 * j] = static_cast<uint32_t>(wv[j]) & 0xffffu;
+        W[2 * j + 1] = static_cast<uint32_t>(wv[j]) >> 16;
+    }
     V acc_a = vzero(static_cast<V *>(nullptr));
-    constexpr int EV = pack_sub<CV>();
     const uint32_t n_stored = spmv_section_a<U>(acc_a, col + co + lane, static_cast<const CV *>(vala) + avo + lane,
                                                 static_cast<const CV *>(aval) + ao, adel + ao, acode + ao, w_a,
                                                 static_cast<int32_t>(row), x);
-    spmv_section<U>(acc, col + co + static_cast<int64_t>(n_stored) * SLICE + lane * PACK,
-                    static_cast<const CV *>(valb) + bo + lane * EV, w_b, x);
+    spmv_section_sell<U>(acc, col + co + static_cast<int64_t>(n_stored) * SLICE,
+                         static_cast<const CV *>(valb) + bo, W, pos, static_cast<int32_t>(row), x, xs, lo, len);
     static_cast<V *>(yout)[row] = vadd(acc, acc_a);
+}
+
+// Staging windows of the cached matvec (one thread per workgroup of SPMV_WG rows):
+// the rows around the workgroup's middle row whose states agree with it on every
+// bit at or above its problem's section split (a_bits >= 64: the whole problem).
+// win[w] = (first row, rows); the kernel clips it to its LDS budget.
+__global__ void k_stage_windows(const uint64_t *__restrict__ keys, const int64_t *__restrict__ prob_rows,
+                                const ProbDesc *__restrict__ prob, int K, int64_t n_states, int64_t n_wg,
+                                int2 *__restrict__ win, int *__restrict__ win_max) {
+    const int64_t w = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (w >= n_wg) return;
+    const int64_t r = std::min<int64_t>(w * SPMV_WG + SPMV_WG / 2, n_states - 1);
+    int lo_k = 0, hi_k = K;                  // the problem of row r: prob_rows[k] <= r < prob_rows[k + 1]
+    while (hi_k - lo_k > 1) {
+        const int m = (lo_k + hi_k) / 2;
+        if (prob_rows[m] <= r) lo_k = m;
+        else hi_k = m;
+    }
+    const int64_t p0 = prob_rows[lo_k], p1 = prob_rows[lo_k + 1];
+    const int a = prob[lo_k].a_bits;
+    const uint64_t hi = a >= 64 ? 0ull : (keys[r] >> a);
+    auto high = [&](int64_t i) { return a >= 64 ? 0ull : (keys[i] >> a); };
+    int64_t b = p0, e = r;                   // first row of the block in [p0, r]
+    while (b < e) {
+        const int64_t m = b + (e - b) / 2;
+        if (high(m) < hi) b = m + 1;
+        else e = m;
+    }
+    int64_t b2 = r + 1, e2 = p1;             // one past its last row in [r + 1, p1]
+    while (b2 < e2) {
+        const int64_t m = b2 + (e2 - b2) / 2;
+        if (high(m) <= hi) b2 = m + 1;
+        else e2 = m;
+    }
+    const int64_t len = std::min<int64_t>(b2 - b, INT32_MAX);
+    win[w] = make_int2(static_cast<int32_t>(b), static_cast<int32_t>(len));
+    atomicMax(win_max, static_cast<int>(len));
 }
 
 // Section widths of each slice (one wave per slice): the largest row count of
 // each section over the slice's rows, rounded up to whole packets; the hits of
 // all rows are summed into nnz[0] and the section-B hits into nnz[1] (one atomic
-// each per wave).
+// each per wave).  Section B's SELL order: each row's position (descending
+// section-B count, ties in row order; rows past the states count 0) in bpos,
+// the row of each position in bperm, the chunk widths (packets) in bw and their
+// sum in bsum.
 __global__ __launch_bounds__(256) void k_slice_widths(const uint32_t *__restrict__ cnt_b,
                                                       const uint32_t *__restrict__ cnt_a, int64_t n_states,
                                                       int64_t n_slices, uint32_t *__restrict__ wa,
-                                                      uint32_t *__restrict__ wb, unsigned long long *nnz) {
+                                                      uint32_t *__restrict__ wb, uint32_t *__restrict__ bsum,
+                                                      uint8_t *__restrict__ bpos, uint8_t *__restrict__ bperm,
+                                                      uint16_t *__restrict__ bw, unsigned long long *nnz) {
     const int64_t slice = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / SLICE;
     if (slice >= n_slices) return;
-    const int64_t row = slice * SLICE + threadIdx.x % SLICE;
-    uint32_t b = row < n_states ? cnt_b[row] : 0u, a = row < n_states ? cnt_a[row] : 0u;
+    const int lane = static_cast<int>(threadIdx.x % SLICE);
+    const int64_t row = slice * SLICE + lane;
+    const uint32_t my_b = row < n_states ? cnt_b[row] : 0u;
+    uint32_t b = my_b, a = row < n_states ? cnt_a[row] : 0u;
     unsigned long long sum = static_cast<unsigned long long>(a) + b, sum_b = b;
     for (int off = 32; off > 0; off >>= 1) {
         b = max(b, static_cast<uint32_t>(__shfl_xor(static_cast<int>(b), off, 64)));
@@ -1909,9 +2138,21 @@ __global__ __launch_bounds__(256) void k_slice_widths(const uint32_t *__restrict
         sum += __shfl_xor(sum, off, 64);
         sum_b += __shfl_xor(sum_b, off, 64);
     }
-    if (threadIdx.x % SLICE == 0) {
+    int rank = 0;
+    for (int m = 0; m < SLICE; ++m) {
+        const uint32_t c = static_cast<uint32_t>(__shfl(static_cast<int>(my_b), m, 64));
+        rank += (c > my_b || (c == my_b && m < lane)) ? 1 : 0;
+    }
+    bpos[row] = static_cast<uint8_t>(rank);
+    bperm[slice * SLICE + rank] = static_cast<uint8_t>(lane);
+    const uint32_t w_chunk = (my_b + PACK - 1) / PACK;       // the widest row of a chunk is its first position
+    if (rank % CH == 0) bw[slice * NCH + rank / CH] = static_cast<uint16_t>(w_chunk);
+    uint32_t ws = rank % CH == 0 ? w_chunk : 0u;
+    for (int off = 32; off > 0; off >>= 1) ws += static_cast<uint32_t>(__shfl_xor(static_cast<int>(ws), off, 64));
+    if (lane == 0) {
         wa[slice] = (a + PACK - 1) / PACK * PACK;
         wb[slice] = (b + PACK - 1) / PACK * PACK;
+        bsum[slice] = ws;
         atomicAdd(nnz, sum);
         atomicAdd(nnz + 1, sum_b);
     }
@@ -1984,19 +2225,21 @@ __global__ __launch_bounds__(256) void k_cache_levels(const int32_t *__restrict_
 }
 
 // Stored columns (one wave per slice) from coff[s] on: the LEVEL_GATHER
-// section-A levels (64 lane-contiguous columns each), then section B as packed
-// in the full layout.
+// section-A levels (64 lane-contiguous columns each), then section B in SELL
+// order (each row's packets up to its chunk width, from the full layout).
 __global__ __launch_bounds__(256) void k_cache_columns(const int32_t *__restrict__ col_full,
                                                        const int64_t *__restrict__ off, const uint32_t *__restrict__ wa,
-                                                       const uint32_t *__restrict__ wb, const int64_t *__restrict__ aoff,
-                                                       const int32_t *__restrict__ adel,
+                                                       const int64_t *__restrict__ aoff,
+                                                       const int32_t *__restrict__ adel, const uint8_t *__restrict__ bpos,
+                                                       const uint16_t *__restrict__ bw,
                                                        const int64_t *__restrict__ coff, int64_t n_states,
                                                        int64_t n_slices, int32_t *__restrict__ col) {
     const int64_t slice = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / SLICE;
     if (slice >= n_slices) return;
     const int lane = static_cast<int>(threadIdx.x % SLICE);
-    const bool valid = slice * SLICE + lane < n_states;
-    const uint32_t w_a = wa[slice], w_b = wb[slice];
+    const int64_t row = slice * SLICE + lane;
+    const bool valid = row < n_states;
+    const uint32_t w_a = wa[slice];
     const int64_t o = off[slice], ao = aoff[slice];
     int64_t dst = coff[slice];
     for (uint32_t k = 0; k < w_a; ++k) {
@@ -2004,10 +2247,14 @@ __global__ __launch_bounds__(256) void k_cache_columns(const int32_t *__restrict
         col[dst + lane] = valid ? col_full[packed_slot<int32_t>(o, k, lane)] : 0;
         dst += SLICE;
     }
-    const int64_t src = o + static_cast<int64_t>(w_a) * SLICE;
-    for (int64_t e = lane; e < static_cast<int64_t>(w_b) * SLICE; e += SLICE) {
-        const int owner = static_cast<int>((e % (SLICE * PACK)) / PACK);   // the lane whose packet holds e
-        col[dst + e] = (slice * SLICE + owner < n_states) ? col_full[src + e] : 0;
+    uint32_t W[NCH];
+    load_chunk_widths(bw, slice, W);
+    const int pos = bpos[row];
+    const uint32_t w_me = W[pos / CH];
+    for (uint32_t q = 0; q < w_me; ++q) {
+        i32x4 t = {0, 0, 0, 0};
+        if (valid) t = *reinterpret_cast<const i32x4 *>(col_full + packed_slot<int32_t>(o, w_a + q * PACK, lane));
+        *reinterpret_cast<i32x4 *>(col + sell_slot<int32_t>(dst, sell_level(W, q), 0, pos)) = t;
     }
 }
 
@@ -2115,7 +2362,7 @@ struct ProbHost {            // host record of one problem of a plan
     int32_t grp_base = 0, n_groups = 0;
     int32_t seg_base = 0, n_segs = 0;
     int32_t cls_base = 0, n_classes = 0;
-    int32_t n_tiles = 0, max_seg = 0, L = 0, n_qubits = 0;
+    int32_t n_tiles = 0, max_seg = 0, L = 0, n_qubits = 0, a_bits = 0;
     int32_t has_diag = 0, max_group_terms = 0;
     int32_t max_offdiag_terms = 0;   // longest group with x != 0
     uint32_t diag_terms = 0;   // terms of the x == 0 group
@@ -2127,6 +2374,12 @@ struct HitCache {            // sliced-ELL cache of the hits (column, C_j(s_i)) 
     void *block = nullptr;     // per-slice arrays, level codes, section-B and single values
     void *col_block = nullptr; // stored columns
     void *vala_block = nullptr;// section-A values of the levels with 64
+    void *sell_block = nullptr;// section B's SELL order: bpos, bperm, bw
+    uint8_t *bpos = nullptr;   // each row's position in its slice (descending section-B count)
+    uint8_t *bperm = nullptr;  // the row (within the slice) at each position
+    uint16_t *bw = nullptr;    // the 8 chunk widths (packets) of each slice
+    int2 *win = nullptr;       // the cached matvec's staging window of each workgroup (first row, rows)
+    int32_t win_max = 0;       // the widest window (rows)
     int32_t *col = nullptr;    // stored columns: gathered section-A levels and section B of each slice
     void *valb = nullptr;      // section-B values (packets)
     void *vala = nullptr;      // section-A values of the levels with 64 (lane-contiguous per level)
@@ -2565,6 +2818,36 @@ int max_segment_par(const std::vector<int8_t> &hd, int L, unsigned nt) {
     return static_cast<int>(std::min<int64_t>(best, INT32_MAX));
 }
 
+// Section split of the hit cache: the largest b in (L, n_qubits) at which the sorted
+// states are an exact product -- every block of equal s >> b holds the same number
+// of states with the same low parts s & (2^b - 1) (a Cartesian product of high and
+// low strings, e.g. beta x alpha determinants) -- else L.  Groups flipping no bit
+// below b then map every row of a block to the same row of another block, so their
+// cache levels are row offsets.
+int32_t section_split(const uint64_t *st, int64_t n, int32_t L, int32_t n_qubits) {
+    if (n < 2) return L;
+    for (int32_t b = n_qubits - 1; b > L; --b) {
+        const uint64_t lm = (b >= 64) ? ~0ull : ((1ull << b) - 1ull);
+        int64_t m = 1;
+        while (m < n && (st[m] >> b) == (st[0] >> b)) ++m;
+        if (m == n || n % m != 0) continue;
+        bool ok = true;
+        for (int64_t blk = m; blk < n && ok; blk += m) {
+            if ((st[blk] >> b) == (st[blk - 1] >> b) || (st[blk + m - 1] >> b) != (st[blk] >> b)) {
+                ok = false;
+                break;
+            }
+            for (int64_t j = 0; j < m; ++j)
+                if ((st[blk + j] & lm) != (st[j] & lm)) {
+                    ok = false;
+                    break;
+                }
+        }
+        if (ok) return b;
+    }
+    return L;
+}
+
 int choose_split(const ProblemIn &q, int32_t split_bits, bool has_diag, Segmentation &best, int k) {
     const unsigned nt = index_threads(q.n_states);
     std::vector<int8_t> hd(static_cast<size_t>(q.n_states), 64);
@@ -2675,6 +2958,7 @@ int build_plan(rqx_plan **out, const std::vector<ProblemIn> &in, int64_t n_total
         Segmentation sg;
         if (int rc = choose_split(q, split_bits, ph.has_diag, sg, k)) return rc;
         ph.L = sg.L;
+        ph.a_bits = section_split(q.states, q.n_states, sg.L, q.n_qubits);
         ph.max_seg = sg.max_seg;
         ph.cost = sg.cost;
         ph.seg_base = static_cast<int32_t>(seg_hi_all.size());
@@ -2736,7 +3020,7 @@ int build_plan(rqx_plan **out, const std::vector<ProblemIn> &in, int64_t n_total
         for (int32_t t = 0; t < ph.n_tiles; ++t) tile_cost.push_back(ph.n_tiles ? ph.cost / ph.n_tiles : 0.0);
         desc[k] = ProbDesc{static_cast<uint32_t>(ph.seg_base), static_cast<uint32_t>(ph.n_segs),
                            static_cast<uint32_t>(ph.cls_base), static_cast<uint32_t>(ph.n_classes),
-                           static_cast<uint32_t>(ph.grp_base), lomask, ph.has_diag, 0};
+                           static_cast<uint32_t>(ph.grp_base), lomask, ph.has_diag, ph.a_bits};
         p->max_seg = std::max(p->max_seg, ph.max_seg);
         p->is_real = p->is_real && ph.is_real;
         if (ph.max_offdiag_terms >= TP_MIN_TERMS) p->long_groups = 1;
@@ -2929,7 +3213,7 @@ size_t cache_val_bytes(int slot, int cplx) { return (slot == 1 ? 8u : 4u) * (cpl
 
 hipError_t free_cache(HitCache &c) {
     hipError_t e = hipSuccess;
-    for (void *b : {c.block, c.col_block, c.vala_block}) {
+    for (void *b : {c.block, c.col_block, c.vala_block, c.sell_block}) {
         if (!b) continue;
         const hipError_t e2 = hipFree(b);
         if (e == hipSuccess) e = e2;
@@ -2957,6 +3241,8 @@ hipError_t fill_columns(const rqx_plan *p, int slot, const HitCache &c, const in
     k.c_off = off;
     k.c_wa = c.wa;
     k.c_wb = c.wb;
+    k.c_bpos = c.bpos;
+    k.c_bw = c.bw;
     k.c_aoff = c.aoff;
     k.c_boff = c.boff;
     k.c_val = c.valb;
@@ -2981,6 +3267,8 @@ hipError_t fill_values(const rqx_plan *p, int slot, hipStream_t st, bool a_only 
     KParams k = make_params(p, slot == 1);
     k.c_wa = c.wa;
     k.c_wb = c.wb;
+    k.c_bpos = c.bpos;
+    k.c_bw = c.bw;
     k.c_aoff = c.aoff;
     k.c_boff = c.boff;
     k.c_avoff = c.avoff;
@@ -3005,14 +3293,17 @@ hipError_t fill_values(const rqx_plan *p, int slot, hipStream_t st, bool a_only 
 
 template <typename R, bool CPLXV>
 hipError_t spmv_launch(const rqx_plan *p, const HitCache &c, const void *diag, const void *x, void *y, hipStream_t st) {
-    const unsigned blocks = static_cast<unsigned>((p->n_states + 255) / 256);
+    using V = typename VT<R, CPLXV>::T;
+    const unsigned blocks = static_cast<unsigned>((p->n_states + SPMV_WG - 1) / SPMV_WG);
+    const size_t lds = static_cast<size_t>(std::max(1, std::min(c.win_max, stage_cap<V>()))) * sizeof(V);
     if (c.cplx)
-        hipLaunchKernelGGL((k_spmv_sell<R, CPLXV, true>), dim3(blocks), dim3(256), 0, st, c.col, c.valb, c.vala, c.aval,
-                           c.boff, c.avoff, c.coff, c.aoff, c.adel, c.acode, c.wa, c.wb, diag, p->n_states, x, y);
+        hipLaunchKernelGGL((k_spmv_sell<R, CPLXV, true>), dim3(blocks), dim3(SPMV_WG), lds, st, c.col, c.valb, c.vala,
+                           c.aval, c.boff, c.avoff, c.coff, c.aoff, c.adel, c.acode, c.wa, c.bperm, c.bw, c.win, diag,
+                           p->n_states, x, y);
     else
-        hipLaunchKernelGGL((k_spmv_sell<R, CPLXV, false>), dim3(blocks), dim3(256), 0, st, c.col, c.valb, c.vala,
-                           c.aval, c.boff, c.avoff, c.coff, c.aoff, c.adel, c.acode, c.wa, c.wb, diag, p->n_states, x,
-                           y);
+        hipLaunchKernelGGL((k_spmv_sell<R, CPLXV, false>), dim3(blocks), dim3(SPMV_WG), lds, st, c.col, c.valb, c.vala,
+                           c.aval, c.boff, c.avoff, c.coff, c.aoff, c.adel, c.acode, c.wa, c.bperm, c.bw, c.win, diag,
+                           p->n_states, x, y);
     return hipGetLastError();
 }
 
@@ -3119,6 +3410,7 @@ int rqx_plan_get_problem(const rqx_plan *p, int32_t k, rqx_problem_info *out) {
     out->has_diagonal = ph.has_diag;
     out->is_real = ph.is_real;
     out->cost = ph.cost;
+    out->section_bits = ph.a_bits;
     return RQX_OK;
 }
 
@@ -3303,35 +3595,63 @@ int rqx_plan_build_cache(rqx_plan *p, int32_t precision, int64_t max_bytes, rqx_
     if (n > INT32_MAX) return fail(RQX_ERR_LIMIT, "hit cache columns are int32: at most 2^31 - 1 rows");
     const int64_t n_slices = (n + SLICE - 1) / SLICE;
 
+    // section B's SELL order (kept with the cache): positions, permutation, chunk widths
+    const size_t ns64 = align_up(static_cast<size_t>(n_slices) * SLICE);
+    const int64_t n_wg = (n + SPMV_WG - 1) / SPMV_WG;
+    const size_t sell_bytes = 2 * ns64 + align_up(static_cast<size_t>(n_slices) * NCH * 2) +
+                              align_up(static_cast<size_t>(n_wg) * 8) + 256;
+    void *sell = nullptr;
+    HIP_TRY(hipMalloc(&sell, sell_bytes));
+    uint8_t *bpos = static_cast<uint8_t *>(sell), *bperm = bpos + ns64;
+    uint16_t *bw = reinterpret_cast<uint16_t *>(bperm + ns64);
+    int2 *win = reinterpret_cast<int2 *>(reinterpret_cast<char *>(bw) + align_up(static_cast<size_t>(n_slices) * NCH * 2));
+    int *win_max_dev = reinterpret_cast<int *>(reinterpret_cast<char *>(win) + align_up(static_cast<size_t>(n_wg) * 8));
+    int win_max = 0;
     // pass 1: non-diagonal hits per row (section B at [0, n), section A at
-    // [n, 2n)), then each slice's section widths (after 2n: A widths, B widths,
-    // the hit total)
+    // [n, 2n)), then each slice's section widths and SELL order (after 2n: A
+    // widths, B widths, B chunk-width sums, a spare word per slice, the hit totals)
     uint32_t *rowcnt = nullptr;
-    HIP_TRY(hipMalloc(&rowcnt, static_cast<size_t>(2 * n + 2 * n_slices + 4) * 4));
+    hipError_t e = hipMalloc(&rowcnt, static_cast<size_t>(2 * n + 4 * n_slices + 4) * 4);
+    if (e != hipSuccess) {
+        (void)hipFree(sell);
+        return fail(RQX_ERR_HIP, std::string("cache row count: ") + hipGetErrorString(e));
+    }
     KParams k = make_params(p, true);
     k.rowcnt = rowcnt;
     k.rowcnt_a = rowcnt + n;
     uint32_t *w_dev = rowcnt + 2 * n;
-    unsigned long long *nnz_dev = reinterpret_cast<unsigned long long *>(w_dev + 2 * n_slices);
-    hipError_t e = fill_async<unsigned long long>(nnz_dev, 0, 2, 0ull, st);
+    unsigned long long *nnz_dev = reinterpret_cast<unsigned long long *>(w_dev + 4 * n_slices);
+    e = fill_async<unsigned long long>(nnz_dev, 0, 2, 0ull, st);
+    if (e == hipSuccess) e = fill_async<int>(win_max_dev, 0, 1, 0, st);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_stage_windows, dim3(static_cast<unsigned>((n_wg + 255) / 256)), dim3(256), 0, st, p->keys,
+                           p->prob_rows, p->prob, static_cast<int>(p->probs.size()), n, n_wg, win, win_max_dev);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(&win_max, win_max_dev, sizeof(int), hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = launch_tiles<MODE_ROWCOUNT, double, true>(p, k, nullptr, nullptr, st);
     if (e == hipSuccess) {
         hipLaunchKernelGGL(k_slice_widths, dim3(static_cast<unsigned>((n_slices * SLICE + 255) / 256)), dim3(256), 0, st,
-                           rowcnt, rowcnt + n, n, n_slices, w_dev, w_dev + n_slices, nnz_dev);
+                           rowcnt, rowcnt + n, n, n_slices, w_dev, w_dev + n_slices, w_dev + 2 * n_slices, bpos, bperm,
+                           bw, nnz_dev);
         e = hipGetLastError();
     }
-    std::vector<uint32_t> wab(static_cast<size_t>(2 * n_slices + 4));
+    std::vector<uint32_t> wab(static_cast<size_t>(4 * n_slices + 4));
     if (e == hipSuccess) e = hipMemcpyAsync(wab.data(), w_dev, wab.size() * 4, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     const hipError_t ef = hipFree(rowcnt);
     if (e == hipSuccess) e = ef;
-    if (e != hipSuccess) return fail(RQX_ERR_HIP, std::string("cache row count: ") + hipGetErrorString(e));
+    if (e != hipSuccess) {
+        (void)hipFree(sell);
+        return fail(RQX_ERR_HIP, std::string("cache row count: ") + hipGetErrorString(e));
+    }
 
     // offsets of the column pass's full layout and of the stored section B
-    const std::vector<uint32_t> wa(wab.begin(), wab.begin() + n_slices), wb(wab.begin() + n_slices, wab.begin() + 2 * n_slices);
+    const std::vector<uint32_t> wa(wab.begin(), wab.begin() + n_slices), wb(wab.begin() + n_slices, wab.begin() + 2 * n_slices),
+        bsum(wab.begin() + 2 * n_slices, wab.begin() + 3 * n_slices);
     int64_t nnz = 0, nnz_b = 0;
-    std::memcpy(&nnz, wab.data() + 2 * n_slices, 8);
-    std::memcpy(&nnz_b, wab.data() + 2 * n_slices + 2, 8);
+    std::memcpy(&nnz, wab.data() + 4 * n_slices, 8);
+    std::memcpy(&nnz_b, wab.data() + 4 * n_slices + 2, 8);
     std::vector<int64_t> off(static_cast<size_t>(n_slices), 0), aoff(static_cast<size_t>(n_slices), 0),
         boff(static_cast<size_t>(n_slices), 0);
     int64_t full = 0, a_levels = 0, b_entries = 0;
@@ -3342,7 +3662,7 @@ int rqx_plan_build_cache(rqx_plan *p, int32_t precision, int64_t max_bytes, rqx_
         boff[static_cast<size_t>(s)] = b_entries;
         full += static_cast<int64_t>(w_a + w_b) * SLICE;
         a_levels += w_a;
-        b_entries += static_cast<int64_t>(w_b) * SLICE;
+        b_entries += static_cast<int64_t>(bsum[static_cast<size_t>(s)]) * (CH * PACK);
     }
     const int cplx = p->is_real ? 0 : 1;
     const size_t vb = cache_val_bytes(slot, cplx);
@@ -3367,12 +3687,24 @@ int rqx_plan_build_cache(rqx_plan *p, int32_t precision, int64_t max_bytes, rqx_
     HIP_TRY(hipMemGetInfo(&free_b, &total_b));
     const int64_t limit = max_bytes > 0 ? max_bytes : static_cast<int64_t>(free_b / 10 * 9);
     const int64_t need = static_cast<int64_t>(total + scratch + stored_max);
-    if (need > limit)
+    if (need > limit) {
+        (void)hipFree(sell);
         return fail(RQX_ERR_LIMIT, "hit cache needs " + std::to_string(need) + " bytes (while building), limit " +
                                        std::to_string(limit));
+    }
 
     HitCache c;
-    HIP_TRY(hipMalloc(&c.block, total));
+    c.sell_block = sell;
+    c.bpos = bpos;
+    c.bperm = bperm;
+    c.bw = bw;
+    c.win = win;
+    c.win_max = win_max;
+    e = hipMalloc(&c.block, total);
+    if (e != hipSuccess) {
+        (void)free_cache(c);
+        return fail(RQX_ERR_HIP, std::string("cache allocation: ") + hipGetErrorString(e));
+    }
     char *b = static_cast<char *>(c.block);
     c.valb = b + o_valb;
     c.aval = b + o_aval;
@@ -3441,7 +3773,8 @@ int rqx_plan_build_cache(rqx_plan *p, int32_t precision, int64_t max_bytes, rqx_
     for (int64_t s = 0; s < n_slices; ++s) {
         coff[static_cast<size_t>(s)] = col_entries;
         avoff[static_cast<size_t>(s)] = a_entries;
-        col_entries += static_cast<int64_t>(hc[static_cast<size_t>(s)] + wb[static_cast<size_t>(s)]) * SLICE;
+        col_entries += static_cast<int64_t>(hc[static_cast<size_t>(s)]) * SLICE +
+                       static_cast<int64_t>(bsum[static_cast<size_t>(s)]) * (CH * PACK);
         a_entries += static_cast<int64_t>(hc[static_cast<size_t>(n_slices + s)]) * SLICE;
     }
     const size_t o_vala_lo = align_up(static_cast<size_t>(std::max<int64_t>(a_entries, 1)) * vb);
@@ -3458,13 +3791,14 @@ int rqx_plan_build_cache(rqx_plan *p, int32_t precision, int64_t max_bytes, rqx_
     c.pad_levels = static_cast<int64_t>(pads[2]);
     c.entries = b_entries + a_entries + a_levels;
     c.bytes = static_cast<int64_t>(total + align_up(static_cast<size_t>(col_entries) * 4) + o_vala_lo +
-                                   (both ? static_cast<size_t>(a_entries) * vb_lo : 0));
+                                   (both ? static_cast<size_t>(a_entries) * vb_lo : 0) + sell_bytes);
     HitCache lo;                // the FP32 cache of a pair: its own values, everything else shared
     if (both) {
         lo = c;
         lo.block = nullptr;
         lo.col_block = nullptr;
         lo.vala_block = nullptr;
+        lo.sell_block = nullptr;
         lo.valb = b + o_valb_lo;
         lo.aval = b + o_aval_lo;
         lo.vala = static_cast<char *>(c.vala_block) + o_vala_lo;
@@ -3475,8 +3809,8 @@ int rqx_plan_build_cache(rqx_plan *p, int32_t precision, int64_t max_bytes, rqx_
     e = hipMemcpyAsync(c.coff, coff.data(), coff.size() * 8, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipMemcpyAsync(c.avoff, avoff.data(), avoff.size() * 8, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) {
-        hipLaunchKernelGGL(k_cache_columns, dim3(wave_blocks), dim3(256), 0, st, col_full, off_d, c.wa, c.wb, c.aoff,
-                           c.adel, c.coff, n, n_slices, c.col);
+        hipLaunchKernelGGL(k_cache_columns, dim3(wave_blocks), dim3(256), 0, st, col_full, off_d, c.wa, c.aoff,
+                           c.adel, c.bpos, c.bw, c.coff, n, n_slices, c.col);
         e = hipGetLastError();
     }
     // pass 3: the section-A values, straight into the stored layout

@@ -298,12 +298,13 @@ class Plan:
     def cached_bytes(self, dtype):
         """Bytes one cached matvec must move: the cache, the diagonal, ``x`` once and ``y``.
 
-        ``entries s_val + 4 col_entries + 8 a_levels + 40 slices + N (s_d + 2 s_v)``
+        ``entries s_val + 4 col_entries + 8 a_levels + 116 slices + N (s_d + 2 s_v)``
         with ``s_val`` the stored value bytes, ``col_entries`` the stored
         columns, ``a_levels`` the first-section level codes (offset and
-        single-value flag), 40 bytes of offsets and widths per slice, ``s_d`` the
-        diagonal entry bytes (complex) and ``s_v`` the vector element bytes;
-        gathers of ``x`` past its first read are counted as cache hits.
+        single-value flag), 116 bytes per slice (four offsets and the section-A
+        width: 36, the second section's row order: 64, its chunk widths: 16),
+        ``s_d`` the diagonal entry bytes (complex) and ``s_v`` the vector element
+        bytes; gathers of ``x`` past its first read are counted as cache hits.
         """
         precision = 64 if dtype in (torch.float64, torch.complex128) else 32
         info = self.cache_info(precision)
@@ -312,7 +313,7 @@ class Plan:
         s_v = torch.empty(0, dtype=dtype).element_size()
         s_val = (8 if precision == 64 else 4) * (2 if info['complex'] else 1)
         s_d = 16 if precision == 64 else 8
-        return (info['entries'] * s_val + 4 * info['col_entries'] + 8 * info['a_levels'] + 40 * info['slices'] +
+        return (info['entries'] * s_val + 4 * info['col_entries'] + 8 * info['a_levels'] + 116 * info['slices'] +
                 self.n_states * (s_d + 2 * s_v))
 
     def cached_padding_bytes(self, dtype):

@@ -230,3 +230,34 @@ def test_run_sqd_mixed_precision_matches_fp64(cs):
     assert abs(float(emx) - float(e64)) <= 1e-9 * abs(float(e64))
     assert abs(abs(torch.vdot(v64, vmx.to(v64.dtype)).item()) - 1.0) < 1e-6
     assert torch.count_nonzero(vmx[dim:]) == 0
+
+
+def test_section_split_follows_the_product_structure(cases):
+    """Product subspaces (alpha strings x beta strings) get the alpha/beta boundary as the cache's
+    section split, also where the search split is smaller; other subspaces keep the search split.
+    Cached and matrix-free matvecs stay identical and match the oracle."""
+    _say('section split')
+    w = workloads.jw32(n_alpha=128, n_beta=90)    # beta blocks of two whole 64-row slices
+    for split in (-1, 12, 16):
+        plan = Plan(w.tables, w.states, split_bits=split)
+        pi = plan.problems[0]
+        # never below the search split (the first section is a subset of its classes' first groups)
+        assert pi['section_bits'] == max(16, pi['split_bits']), pi
+        plan.build_cache(0)
+        _check_identical(plan, seed=21)
+        info = plan.cache_info(64)
+        if pi['split_bits'] <= 16:
+            # beta flips only, slices inside one beta block: every first-section level is a row
+            # offset (no stored first-section column)
+            assert info['col_entries'] == info['b_entries'] and info['a_pad_columns'] == 0, info
+        rng = np.random.default_rng(22)
+        x = rng.standard_normal(plan.n_total) + 1j * rng.standard_normal(plan.n_total)
+        y = plan.matvec(torch.from_numpy(x).to(DEV)).cpu().numpy()
+        rows = np.sort(rng.choice(plan.n_states, size=200, replace=False))
+        t = w.tables
+        want = sqdref.matvec_rows_u64(t.xmask, t.grp_off, t.zmask, t.coef, w.states, x, rows)
+        assert np.max(np.abs(y[rows] - want)) <= RTOL64 * np.max(np.abs(want))
+        plan.close()
+    tables, states = cases['complex12']
+    plan = Plan(tables, states)
+    assert plan.problems[0]['section_bits'] == plan.problems[0]['split_bits']
