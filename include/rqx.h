@@ -168,6 +168,22 @@ int rqx_diagonal(const rqx_plan *plan, double *diag, rqx_stream_t stream);
  */
 int rqx_matvec(const rqx_plan *plan, const void *x, void *y, int32_t dtype, rqx_stream_t stream);
 
+/* qubits of the largest full space rqx_full_apply serves */
+enum { RQX_FULL_MAX_QUBITS = 40 };
+
+/*
+ * Full-space action on n_cols right-hand sides: y[b * n_cols + c] =
+ * sum_j C_j(b) x[(b ^ xmask_j) * n_cols + c] for every basis state b of 2^n
+ * (n = the plan's qubits; x, y: device arrays of 2^n x n_cols entries of dtype,
+ * row major, must not alias).  Uses only the plan's term tables (the plan may
+ * hold any subspace, or none); no index is searched.  Replaces
+ * PauliSumXZ.matmul (rqutils/paulis/symplectic.py:130-149): the same gather
+ * rhs[indices ^ xsig] times the group's signed coefficient sum at the output
+ * index, for all columns in one launch.  Single-problem plans; at most
+ * RQX_FULL_MAX_QUBITS qubits (the caller allocates 2^n x n_cols entries).
+ */
+int rqx_full_apply(const rqx_plan *plan, const void *x, void *y, int64_t n_cols, int32_t dtype, rqx_stream_t stream);
+
 /*
  * counts[0] = structural nonzeros (row, group) hits incl. the diagonal,
  * counts[1] = (row, group) pairs resolved at the segment level (the fine probes),

@@ -24,6 +24,7 @@ RQX_C64 = 3
 RQX_LOCG_WORK = 168
 RQX_LOCG_SCALE = 164
 RQX_LOCG_STATIC_PROBLEMS = 4096
+RQX_FULL_MAX_QUBITS = 40
 
 # every symbol declared in include/rqx.h
 EXPORTS = (
@@ -33,7 +34,7 @@ EXPORTS = (
     'rqx_plan_update_coef', 'rqx_plan_create_batch', 'rqx_plan_get_problem', 'rqx_plan_set_launch',
     'rqx_eigenpair_batch', 'rqx_batch_dots', 'rqx_batch_combine', 'rqx_plan_build_cache',
     'rqx_plan_drop_cache', 'rqx_plan_cache_info', 'rqx_plan_use_cache', 'rqx_locg_step',
-    'rqx_plan_hits', 'rqx_locg_step_static',
+    'rqx_plan_hits', 'rqx_locg_step_static', 'rqx_full_apply',
 )
 
 
@@ -132,6 +133,7 @@ _SIGS = {
     'rqx_plan_use_cache': (_i32, [_vp, _i32]),
     'rqx_locg_step': (_i32, [_vp, _i32, _i32, _vp, _vp, _vp, _vp, _vp, ctypes.c_double, _vp]),
     'rqx_locg_step_static': (_i32, [_vp, _i32, _i32, _vp, _vp, _vp, _vp, _vp, ctypes.c_double, _vp]),
+    'rqx_full_apply': (_i32, [_vp, _vp, _vp, _i64, _i32, _vp]),
     'rqx_plan_hits': (_i32, [_vp, _vp, _vp, _vp, _i64, _vp, _vp]),
     'rqx_eigenpair_batch': (_i32, [_vp, _i32, _i32, _vp, _vp]),
     'rqx_batch_dots': (_i32, [_vp, _i32, _vp, _vp, _vp, _i32, _vp]),

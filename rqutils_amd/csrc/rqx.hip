@@ -2258,6 +2258,49 @@ __global__ __launch_bounds__(256) void k_cache_columns(const int32_t *__restrict
     }
 }
 
+// Full-space action (no subspace index): y[b, c] = sum_j C_j(b) x[b ^ x_j, c] over
+// every basis state b of 2^n, for n_cols right-hand-side columns stored row major,
+// C_j(b) = sum_k coef_k (-1)^popcount(b & z_k) summed term by term in group order.
+// One thread per (b, c); the groups' masks and coefficients are wave-uniform
+// scalar loads, and the 64 lanes of a wave gather from one aligned block of x
+// (b ^ x_j permutes the low bits of a run of consecutive b).
+template <typename V, typename R, bool REALC>
+__global__ __launch_bounds__(256) void k_full_apply(const uint64_t *__restrict__ g_x,
+                                                    const uint32_t *__restrict__ g_toff,
+                                                    const uint64_t *__restrict__ z, const void *__restrict__ coef_v,
+                                                    int32_t n_groups, int64_t n_items, int64_t n_cols,
+                                                    const V *__restrict__ x, V *__restrict__ y) {
+    using CC = typename C2<R>::T;
+    const CC *coef = static_cast<const CC *>(coef_v);
+    for (int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < n_items;
+         t += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const int64_t col = t % n_cols;
+        const uint64_t b = static_cast<uint64_t>(t / n_cols);
+        V acc = vzero(static_cast<V *>(nullptr));
+        for (int32_t j = 0; j < n_groups; ++j) {
+            const uint64_t xj = ldc(g_x + j);
+            const uint32_t k0 = ldc(g_toff + j), k1 = ldc(g_toff + j + 1);
+            R cr = R(0), ci = R(0);
+            for (uint32_t k = k0; k < k1; ++k) {
+                const CC c = ldc(coef + k);
+                const bool neg = __popcll(b & ldc(z + k)) & 1;
+                cr += neg ? -c.x : c.x;
+                if constexpr (!REALC) ci += neg ? -c.y : c.y;
+            }
+            const V xv = x[static_cast<int64_t>(b ^ xj) * n_cols + col];
+            if constexpr (REALC) {
+                fma_acc(acc, cr, xv);
+            } else {
+                CC cc;
+                cc.x = cr;
+                cc.y = ci;
+                fma_acc(acc, cc, xv);
+            }
+        }
+        y[t] = acc;
+    }
+}
+
 __global__ void k_eigen_batch(const double *__restrict__ mats, int count, int n, double *__restrict__ out) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k < count) eigen_small(mats + static_cast<int64_t>(k) * 2 * n * n, n, out + static_cast<int64_t>(k) * (2 + 2 * n));
@@ -3451,6 +3494,58 @@ int rqx_diagonal(const rqx_plan *p, double *diag, rqx_stream_t stream) {
     if (p->n_total > 0) HIP_TRY(fill_async(d, int64_t(0), p->n_total, make_double2(0.0, 0.0), st));
     if (p->n_states > 0)
         HIP_TRY(hipMemcpyAsync(d, p->diag64, static_cast<size_t>(p->n_states) * 16, hipMemcpyDeviceToDevice, st));
+    return RQX_OK;
+}
+
+}  // extern "C"
+
+namespace {
+template <typename V, typename R>
+hipError_t full_apply_launch(const rqx_plan *p, const void *coef, int64_t n_items, int64_t n_cols, const void *x,
+                             void *y, hipStream_t st) {
+    const int64_t want = (n_items + 255) / 256;
+    const unsigned blocks = static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>(want, int64_t(p->n_cu) * 64)));
+    if (p->is_real)
+        hipLaunchKernelGGL((k_full_apply<V, R, true>), dim3(blocks), dim3(256), 0, st, p->g_x, p->g_toff, p->z, coef,
+                           p->n_groups, n_items, n_cols, static_cast<const V *>(x), static_cast<V *>(y));
+    else
+        hipLaunchKernelGGL((k_full_apply<V, R, false>), dim3(blocks), dim3(256), 0, st, p->g_x, p->g_toff, p->z, coef,
+                           p->n_groups, n_items, n_cols, static_cast<const V *>(x), static_cast<V *>(y));
+    return hipGetLastError();
+}
+}  // namespace
+
+extern "C" {
+
+int rqx_full_apply(const rqx_plan *p, const void *x, void *y, int64_t n_cols, int32_t dtype, rqx_stream_t stream) {
+    g_err.clear();
+    if (int rc = check_plan(p)) return rc;
+    if (p->probs.size() != 1) return fail(RQX_ERR_ARG, "rqx_full_apply needs a single-problem plan");
+    const int nq = p->probs[0].n_qubits;
+    if (nq > RQX_FULL_MAX_QUBITS) return fail(RQX_ERR_LIMIT, "rqx_full_apply: more than RQX_FULL_MAX_QUBITS qubits");
+    if (n_cols < 1) return fail(RQX_ERR_ARG, "n_cols must be positive");
+    if (!x || !y) return fail(RQX_ERR_ARG, "x or y is NULL");
+    if (x == y) return fail(RQX_ERR_ARG, "x and y must not alias");
+    if (int rc = check_dtype(p, dtype)) return rc;
+    const int64_t dim = int64_t(1) << nq;
+    if (dim > INT64_MAX / n_cols) return fail(RQX_ERR_LIMIT, "rqx_full_apply: 2^n x n_cols overflows");
+    const int64_t n_items = dim * n_cols;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    if (p->n_groups == 0) {
+        switch (dtype) {
+            case RQX_F64: HIP_TRY(fill_async(static_cast<double *>(y), int64_t(0), n_items, 0.0, st)); break;
+            case RQX_C128: HIP_TRY(fill_async(static_cast<double2 *>(y), int64_t(0), n_items, make_double2(0.0, 0.0), st)); break;
+            case RQX_F32: HIP_TRY(fill_async(static_cast<float *>(y), int64_t(0), n_items, 0.0f, st)); break;
+            default: HIP_TRY(fill_async(static_cast<float2 *>(y), int64_t(0), n_items, make_float2(0.f, 0.f), st)); break;
+        }
+        return RQX_OK;
+    }
+    switch (dtype) {
+        case RQX_F64: HIP_TRY((full_apply_launch<double, double>(p, p->coef64, n_items, n_cols, x, y, st))); break;
+        case RQX_C128: HIP_TRY((full_apply_launch<double2, double>(p, p->coef64, n_items, n_cols, x, y, st))); break;
+        case RQX_F32: HIP_TRY((full_apply_launch<float, float>(p, p->coef32, n_items, n_cols, x, y, st))); break;
+        default: HIP_TRY((full_apply_launch<float2, float>(p, p->coef32, n_items, n_cols, x, y, st))); break;
+    }
     return RQX_OK;
 }
 

@@ -33,7 +33,7 @@ _LO = {torch.float64: torch.float32, torch.complex128: torch.complex64}
 class BatchLOCG:
     """Persistent buffers + one LOCG step for all problems of a plan (``BatchPlan`` or single ``Plan``)."""
 
-    def __init__(self, plan: Plan, x0: torch.Tensor, tol=1e-7, switch_tol=1e-6, mixed=True, lo_storage=True):
+    def __init__(self, plan: Plan, x0: torch.Tensor, tol=1e-7, switch_tol=1e-6, mixed=True, lo_storage=True, orth=()):
         if x0.dtype not in _LO:
             raise ValueError('x0 must be float64 or complex128')
         if x0.shape != (plan.n_total,) or x0.device != plan.device:
@@ -66,7 +66,22 @@ class BatchLOCG:
         if self.lo_storage:
             zl = lambda: torch.zeros(plan.n_total, dtype=self.lo, device=dev)  # noqa: E731
             self.x_lo, self.ax_lo, self.y_lo, self.ay_lo = (zl() for _ in range(4))
+        # deflation vectors (each over all rows: per problem, its rows are that problem's
+        # vector), normalised per problem, with their images in the vector dtype
+        self.orth, self.orth_img = [], []
+        for o in orth:
+            o = torch.as_tensor(o, device=dev).to(self.hi).contiguous()
+            if o.shape != (plan.n_total,):
+                raise ValueError('orth vectors must be device vectors over all rows')
+            o = o.clone()
+            self._normalize(o, None)
+            self.orth.append(o)
+            self.orth_img.append(plan.matvec(o))
+        if self.lo_storage and self.orth:
+            self.orth_lo = [o.to(self.lo) for o in self.orth]
+            self.orth_img_lo = [a.to(self.lo) for a in self.orth_img]
         self.x.copy_(x0)
+        self._deflate_vectors(self.x)
         self._normalize(self.x, None)
         self._apply(self.x, self.ax)
         # residual scale: max(|theta|, |A x0|) (|A x0| sizes the operator, so a problem
@@ -79,6 +94,32 @@ class BatchLOCG:
                 lo_v.copy_(hi_v)
 
     # ------------------------------------------------------------------ helpers
+    def _deflate_vectors(self, v, av=None, orth=None, imgs=None):
+        """Remove the orth components of v (per problem), and the same combination of the
+        orth images from its image av."""
+        orth = self.orth if orth is None else orth
+        imgs = self.orth_img if imgs is None else imgs
+        for o, ao in zip(orth, imgs):
+            c = self.plan.dots([(o, v)])[:, 0]
+            coef = torch.stack([self.one, -c], 1)
+            self.plan.combine([v, o], coef, v)
+            if av is not None:
+                self.plan.combine([av, ao], coef, av)
+
+    def _deflate(self, lo_copies=None):
+        """x and y (with their images) orthogonal to the orth vectors again, in the storage
+        of the current phase (no host synchronisation: part of the captured step)."""
+        if not self.orth:
+            return
+        if lo_copies is None:
+            lo_copies = self.phase == 'lo' and self.lo_storage
+        if lo_copies:
+            vecs, orth, imgs = ((self.x_lo, self.ax_lo), (self.y_lo, self.ay_lo)), self.orth_lo, self.orth_img_lo
+        else:
+            vecs, orth, imgs = ((self.x, self.ax), (self.y, self.ay)), self.orth, self.orth_img
+        for v, av in vecs:
+            self._deflate_vectors(v, av, orth, imgs)
+
     def _apply(self, v, out):
         if self.phase == 'lo':
             self.p_lo.copy_(v)
@@ -143,6 +184,7 @@ class BatchLOCG:
             c2 = c2.real.to(torch.complex128)
         c = torch.stack([c2[:, 0], self.zero, c2[:, 1]], 1)
         self._update(c)
+        self._deflate(lo_copies=False)        # the low copies are made after this step
 
     def step(self):
         """One LOCG iteration for every problem (no host synchronisation): ``rqx_locg_step``.
@@ -164,6 +206,7 @@ class BatchLOCG:
         _capi.check(_capi.lib().rqx_locg_step(self.plan._handle, dtype, low, self._vec_ptrs[key],
                                               self.theta.data_ptr(), self.rel.data_ptr(), self.active.data_ptr(),
                                               self.work.data_ptr(), float(self.tol), stream))
+        self._deflate()
 
     def step_reference(self):
         """The same iteration from the per-problem vector kernels and torch (one launch per operation)."""
@@ -180,6 +223,7 @@ class BatchLOCG:
         if not self.x.is_complex():
             c = c.real.to(torch.complex128)
         self._update(c)
+        self._deflate()
 
     def to_high_precision(self):
         """Switch to the vector dtype: x and y widened and re-orthonormalised in FP64, images recomputed."""
@@ -191,13 +235,14 @@ class BatchLOCG:
             self.plan.combine([self.y, self.x], torch.stack([self.one, -pr], 1), self.y)
             self._normalize(self.y, None)
         self.phase = 'hi'
+        self._deflate(lo_copies=False)
         self._apply(self.x, self.ax)
         self._apply(self.y, self.ay)
 
 
 def batch_ground_locg(plan: Plan, x0, tol=1e-7, switch_tol=1e-6, maxiter=1000, check_every=10,
                       use_graph=True, mixed=True, record=False, energy_tol=None, energy_checks=2,
-                      energy_rel_residual=1e-7, floor_polls=3, lo_storage=True, check_every_hi=1):
+                      energy_rel_residual=1e-7, floor_polls=3, lo_storage=True, check_every_hi=1, orth=()):
     """Ground eigenpairs of every problem of ``plan`` (a ``BatchPlan``, or a ``Plan`` as one problem).
 
     The bulk phase (low-precision matvec) runs until every active problem's
@@ -216,11 +261,16 @@ def batch_ground_locg(plan: Plan, x0, tol=1e-7, switch_tol=1e-6, maxiter=1000, c
     eigenvector theta can sit still for tens of iterations with a residual of
     ~1e-6 before it falls to the ground energy).
 
+    ``orth``: vectors over all rows that every problem's solution is kept
+    orthogonal to (per problem, its own rows of each vector; deflation as in
+    ``rqutils/ground_locg.py:221-222,283-284,304-306``): the start vector is
+    projected out of them, and ``x`` and ``y`` again after every step.
+
     Returns ``(theta (K,), x, iterations, info)`` where ``info`` holds per-problem
     relative residuals, the iteration of the precision switch and timings.
     """
     t0 = time.perf_counter()
-    solver = BatchLOCG(plan, x0, tol=tol, switch_tol=switch_tol, mixed=mixed, lo_storage=lo_storage)
+    solver = BatchLOCG(plan, x0, tol=tol, switch_tol=switch_tol, mixed=mixed, lo_storage=lo_storage, orth=orth)
     graphs = {}
 
     def run_step():

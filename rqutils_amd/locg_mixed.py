@@ -50,7 +50,7 @@ class MixedLOCGResult:
 
 def ground_locg_mixed(matvec_hi, matvec_lo, xinit, maxiter=1000, tol=1e-6, switch_tol=1e-6,
                       check_every=1, stagnation=15, record=False, energy_tol=None, energy_checks=3,
-                      energy_rel_residual=1e-7):
+                      energy_rel_residual=1e-7, orth=()):
     """Ground eigenpair with a low-precision bulk matvec and FP64 refinement.
 
     Args:
@@ -74,6 +74,11 @@ def ground_locg_mixed(matvec_hi, matvec_lo, xinit, maxiter=1000, tol=1e-6, switc
             ``energy_tol * |theta|`` for ``energy_checks`` consecutive checks and
             the relative residual is at most ``energy_rel_residual`` (a stalled
             theta near an excited eigenvector is not converged).
+        orth: vectors the solution is kept orthogonal to (deflation, as in
+            ``rqutils/ground_locg.py:221-222,283-284,304-306``): the start vector
+            and every residual direction are projected out of them, and ``x``
+            and ``y`` again after each update (their carried images corrected
+            with the images of the orth vectors, computed once in FP64).
 
     Returns:
         ``MixedLOCGResult`` (iterable as ``theta, x, niter``).
@@ -92,7 +97,23 @@ def ground_locg_mixed(matvec_hi, matvec_lo, xinit, maxiter=1000, tol=1e-6, switc
         return matvec_hi(v)
 
     phase = 'lo' if matvec_lo is not None else 'hi'
-    x = xinit / _nrm(xinit)
+    orth = [o.to(hi) / _nrm(o.to(hi)) for o in orth]
+    orth_img = [matvec_hi(o) for o in orth]
+
+    def deflate(v, av=None):
+        """v (and its image av) with the orth components removed (two passes)."""
+        for _ in range(2):
+            for o, ao in zip(orth, orth_img):
+                c = _dot(o, v)
+                v = v - c * o
+                if av is not None:
+                    av = av - c * ao
+        return v, av
+
+    x = xinit.to(hi)
+    if orth:
+        x, _ = deflate(x)
+    x = x / _nrm(x)
     ax = mv(x, phase)
     scale = float(_nrm(ax))
     y = ay = None
@@ -136,9 +157,11 @@ def ground_locg_mixed(matvec_hi, matvec_lo, xinit, maxiter=1000, tol=1e-6, switc
                         calm = calm + 1 if (0 <= d1 <= lim and tail <= lim) else 0
                         if calm >= energy_checks and rel <= energy_rel_residual:
                             break
-        # residual direction, orthonormal to x and y (two CGS passes)
+        # residual direction, orthonormal to x and y (and the orth vectors), two CGS passes
         p = r
         for _ in range(2):
+            for o in orth:
+                p = p - o * _dot(o, p)
             p = p - x * _dot(x, p)
             if y is not None:
                 p = p - y * _dot(y, p)
@@ -163,6 +186,9 @@ def ground_locg_mixed(matvec_hi, matvec_lo, xinit, maxiter=1000, tol=1e-6, switc
         axn = sum(ci * v for ci, v in zip(c, images))
         d = sum(ci * v for ci, v in zip(c[1:], basis[1:]))
         ad = sum(ci * v for ci, v in zip(c[1:], images[1:]))
+        if orth:
+            xn, axn = deflate(xn, axn)
+            d, ad = deflate(d, ad)
         s = _nrm(xn)
         x, ax = xn / s, axn / s
         proj = _dot(x, d)

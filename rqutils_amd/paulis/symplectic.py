@@ -317,10 +317,13 @@ class PauliSumXZ:
         return self.x, self.z, self.c
 
     def matmul(self, rhs):
-        """Full-space action ``H @ rhs`` on a ``2^n`` vector (``symplectic.py:130-149``).
+        """Full-space action ``H @ rhs`` on ``2^n`` rows (``symplectic.py:130-149``).
 
-        Runs the same X-grouped gfx950 kernel with the complete computational
-        basis as the subspace.
+        ``rhs`` has shape ``(2^n,)`` or ``(2^n, ...)``; every column is applied
+        in one launch of the direct gfx950 kernel (``rqx_full_apply``: the gather
+        ``rhs[b ^ x_j]`` times the group's signed coefficient sum at ``b``, no
+        subspace index).  NumPy input gives a NumPy result, a device tensor a
+        device tensor.
         """
         from rqutils_amd.plan import Plan
         import torch
@@ -330,12 +333,11 @@ class PauliSumXZ:
                              f' num_qubits={self.num_qubits}')
         as_numpy = not isinstance(rhs, torch.Tensor)
         t = torch.as_tensor(np.asarray(rhs) if as_numpy else rhs)
-        plan = Plan.full_space(self._tables, device=t.device if t.is_cuda else None)
+        device = t.device if t.is_cuda else torch.device('cuda', torch.cuda.current_device())
+        plan = getattr(self, '_terms_plan', None)
+        if plan is None or plan.device != device:
+            plan = Plan(self._tables, np.zeros(0, dtype=np.uint64), device=device)
+            self._terms_plan = plan
         dtype = torch.complex128 if not self._tables.is_real or t.is_complex() else torch.float64
-        t = t.to(device=plan.device, dtype=dtype)
-        cols = t.reshape(dim, -1)
-        out = torch.empty_like(cols)
-        for k in range(cols.shape[1]):
-            out[:, k] = plan.matvec(cols[:, k].contiguous())
-        out = out.reshape(t.shape)
+        out = plan.full_apply(t.to(device=device, dtype=dtype))
         return out.cpu().numpy() if as_numpy else out

@@ -86,11 +86,36 @@ class Plan:
 
     @classmethod
     def full_space(cls, tables: TermTables, device=None):
-        """Plan over the complete ``2^n`` computational basis (``symplectic.py:130-149``)."""
+        """Plan over the complete ``2^n`` computational basis as a subspace (n <= 30)."""
         n = tables.num_qubits
         if n > 30:
-            raise ValueError('full-space action needs num_qubits <= 30')
+            raise ValueError('a full-space subspace plan needs num_qubits <= 30 (use full_apply)')
         return cls(tables, np.arange(1 << n, dtype=np.uint64), device=device)
+
+    def full_apply(self, rhs, out=None):
+        """``H @ rhs`` over the whole ``2^n`` space with no subspace index (``symplectic.py:130-149``).
+
+        ``rhs``: device tensor of shape ``(2^n,)`` or ``(2^n, k)`` (float32/64,
+        complex64/128); every column is applied in one launch.  Uses only the
+        plan's term tables, so a plan built over an empty subspace serves.
+        """
+        if not isinstance(rhs, torch.Tensor) or rhs.device != self.device:
+            raise ValueError('rhs must be a tensor on the plan device')
+        n = self.info['n_qubits']
+        dim = 1 << n
+        if rhs.shape[0] != dim:
+            raise ValueError(f'RHS axis 0 size {rhs.shape[0]} is incompatible with num_qubits={n}')
+        if rhs.dtype not in _DTYPE_CODES:
+            raise ValueError(f'unsupported dtype {rhs.dtype}')
+        cols = rhs.reshape(dim, -1).contiguous()
+        if out is None:
+            out = torch.empty_like(cols)
+        elif out.shape != cols.shape or out.dtype != cols.dtype or not out.is_contiguous():
+            raise ValueError('out must be a contiguous tensor like rhs')
+        stream = _capi.current_stream(self.device)
+        _capi.check(_capi.lib().rqx_full_apply(self._handle, cols.data_ptr(), out.data_ptr(), cols.shape[1],
+                                               _DTYPE_CODES[cols.dtype], stream))
+        return out.reshape(rhs.shape)
 
     # ------------------------------------------------------------------ lifetime
     def close(self):

@@ -206,3 +206,99 @@ def test_hproj_nonzero_triplets_match_lookup_and_scale():
     t = w.tables
     ora = sqdref.matvec_rows_u64(t.xmask, t.grp_off, t.zmask, t.coef, w.states, v, rows)
     assert np.max(np.abs(got[rows] - ora)) <= 1e-12 * np.max(np.abs(ora))
+
+
+def test_mixed_and_batched_locg_deflate_to_the_first_excited_state():
+    """orth deflation in the mixed and graph-captured batched solvers: with the ground state
+    projected out, both converge to the first excited energy of the periodic TFIM (eigh)."""
+    _say('orth in mixed / batched locg')
+    from rqutils_amd.locg_batch import batch_ground_locg
+    from rqutils_amd.locg_mixed import ground_locg_mixed
+    nq, hx = 10, 1.3
+    paulis, coeffs = _tfim(nq, hx)
+    w, v = np.linalg.eigh(dense.full_matrix(paulis, coeffs, nq).real)
+    assert w[1] - w[0] > 1e-3
+    h = PauliSumXZ.from_paulisum((paulis, coeffs))
+    plan = Plan.full_space(h.tables)
+    ground = torch.from_numpy(v[:, 0].copy()).to(DEV)
+    rng = np.random.default_rng(41)
+    x0 = torch.from_numpy(rng.standard_normal(1 << nq)).to(DEV)
+    res = ground_locg_mixed(plan.matvec, plan.matvec, x0, maxiter=3000, tol=1e-10, orth=(ground,))
+    assert abs(float(res.theta) - w[1]) < 1e-8 * abs(w[1]), (float(res.theta), w[1])
+    assert abs(float(torch.dot(res.x, ground))) < 1e-6
+    for graph in (False, True):
+        theta, x, it, info = batch_ground_locg(plan, x0, tol=1e-10, maxiter=3000, check_every=5, use_graph=graph,
+                                               orth=(ground,))
+        assert abs(float(theta[0]) - w[1]) < 1e-8 * abs(w[1]), (graph, float(theta[0]), w[1])
+        assert abs(float(torch.dot(x, ground))) < 1e-6
+    # without deflation the same solver finds the ground state
+    theta, _, _, _ = batch_ground_locg(plan, x0, tol=1e-10, maxiter=3000, check_every=5)
+    assert abs(float(theta[0]) - w[0]) < 1e-8 * abs(w[0])
+
+
+@pytest.mark.parametrize('dtype', [torch.complex128, torch.float64, torch.complex64])
+def test_full_apply_matches_dense_and_subspace_kernel(dtype):
+    """The direct full-space kernel (no index): dense textbook matrix for several columns,
+    and the subspace kernel over the complete basis at 20 qubits."""
+    _say(f'full apply {dtype}')
+    rng = np.random.default_rng(9)
+    n = 9
+    codes = rng.choice(4, size=(50, n))
+    if not dtype.is_complex:                      # a real Hamiltonian: even numbers of Y
+        odd = np.nonzero(np.sum(codes == 2, axis=1) % 2 == 1)[0]
+        codes[odd, np.argmax(codes[odd] == 2, axis=1)] = 3
+    paulis = [''.join('IXYZ'[c] for c in row) for row in codes]
+    coeffs = rng.standard_normal(50)
+    h = PauliSumXZ.from_paulisum((paulis, coeffs))
+    assert h.tables.is_real == (not dtype.is_complex)
+    v = rng.standard_normal((1 << n, 4))
+    if dtype.is_complex:
+        v = v + 1j * rng.standard_normal((1 << n, 4))
+    plan = Plan(h.tables, np.zeros(0, dtype=np.uint64))
+    got = plan.full_apply(torch.from_numpy(v).to(DEV, dtype)).cpu().numpy()
+    want = dense.full_matrix(paulis, coeffs, n) @ v
+    tol = 1e-12 if dtype != torch.complex64 else 2e-5
+    assert np.max(np.abs(got - want)) <= tol * np.max(np.abs(want))
+    # 20 qubits: the direct kernel against the subspace kernel over all 2^20 states
+    w = workloads.random20_300t()
+    full = Plan.full_space(w.tables)
+    x = torch.from_numpy(rng.standard_normal(1 << 20) + 1j * rng.standard_normal(1 << 20)).to(DEV)
+    direct = Plan(w.tables, np.zeros(0, dtype=np.uint64)).full_apply(x)
+    ref = full.matvec(x)
+    assert float((direct - ref).abs().max()) <= 1e-12 * float(ref.abs().max())
+
+
+def test_full_apply_beyond_thirty_qubits():
+    """31 qubits (past the subspace index's 2^31-row limit): sampled rows against the
+    CPU evaluation of sum_j C_j(b) x[b ^ x_j] with x given by a closed form."""
+    _say('full apply 31 qubits')
+    n = 31
+    rng = np.random.default_rng(10)
+    codes = workloads.random_pauli_codes(n, 12, (0.7, 0.1, 0.1, 0.1), rng)
+    xm, zm = workloads.codes_to_masks(codes)
+    from rqutils_amd.paulis.symplectic import TermTables
+    t = TermTables.from_masks(xm, zm, rng.standard_normal(12), n)
+
+    def phase(b):                                   # x[b] = exp(i phi(b)), phi from b alone
+        return ((b % 1009) * 0.37 + (b % 997) * 0.11)
+
+    b = torch.arange(1 << n, dtype=torch.int64, device=DEV)
+    phi = (b % 1009).to(torch.float64) * 0.37 + (b % 997).to(torch.float64) * 0.11   # phase(b) in FP64
+    phi = torch.remainder(phi, 2 * np.pi).to(torch.float32)                            # reduced, then narrowed
+    del b
+    x = torch.polar(torch.ones_like(phi), phi)
+    del phi
+    plan = Plan(t, np.zeros(0, dtype=np.uint64))
+    y = plan.full_apply(x)
+    rows = rng.integers(0, 1 << n, size=64, dtype=np.uint64)
+    got = y[torch.from_numpy(rows.astype(np.int64)).to(DEV)].cpu().numpy()
+    want = np.zeros(64, dtype=np.complex128)
+    for xj, k0, k1 in zip(t.xmask, t.grp_off[:-1], t.grp_off[1:]):
+        c = np.zeros(64, dtype=np.complex128)
+        for k in range(k0, k1):
+            c += t.coef[k] * (1. - 2. * (np.bitwise_count(rows & t.zmask[k]) & 1))
+        src = (rows ^ np.uint64(xj)).astype(np.int64)
+        want += c * np.exp(1j * phase(src))
+    assert np.max(np.abs(got - want)) <= 1e-5 * np.max(np.abs(want))
+    del x, y
+    torch.cuda.empty_cache()

@@ -61,3 +61,21 @@ def test_mixed_locg_exact_start_vector():
     x0[0] = 1.
     res = ground_locg_mixed(lambda v: at @ v, None, x0, maxiter=10, tol=1e-12)
     assert float(res.theta) == 1.0 and res.niter == 0
+
+
+@pytest.mark.parametrize('complex_', [False, True])
+def test_mixed_locg_orth_deflation(complex_):
+    """orth deflation: with the two lowest eigenvectors projected out, the mixed solver
+    returns the third eigenvalue (as the reference's orth argument, ground_locg.py:221-222)."""
+    n = 300
+    a = _rand_herm(n, 5, complex_, gap_shift=1.0)
+    w, v = np.linalg.eigh(a)
+    at = torch.from_numpy(a)
+    a32 = at.to(torch.complex64 if complex_ else torch.float32)
+    rng = np.random.default_rng(6)
+    x0 = torch.from_numpy(rng.standard_normal(n) + (1j * rng.standard_normal(n) if complex_ else 0))
+    orth = tuple(torch.from_numpy(v[:, k].copy()) for k in (0, 1))
+    res = ground_locg_mixed(lambda u: at @ u, lambda u: a32 @ u, x0, maxiter=2000, tol=1e-10, orth=orth)
+    assert abs(float(res.theta) - w[2]) < 1e-9 * abs(w[2])
+    for o in orth:
+        assert abs(complex(torch.vdot(o.to(res.x.dtype), res.x))) < 1e-7
