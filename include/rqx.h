@@ -125,6 +125,36 @@ int rqx_plan_create_batch(rqx_plan **out, int32_t n_problems, const int32_t *n_q
                           const uint64_t *const *sorted_states, const int64_t *n_states, int32_t split_bits,
                           rqx_stream_t stream);
 
+/*
+ * rqx_plan_create with the sorted unique states already in DEVICE memory (e.g.
+ * the output of rqx_sort_unique_u64); the term tables are host pointers as in
+ * rqx_plan_create.  The passes over the states -- validation, the segment
+ * statistics of every split, the segment starts of the candidate splits, the
+ * section split, the low keys and the per-segment hash tables -- run on the
+ * device; only per-segment arrays are copied back for the cost model.  The
+ * plan equals the one rqx_plan_create builds from the same states.
+ * Replaces the setup half of run_sqd (rqutils/sqd.py:341-356) after
+ * uniquify_states (sqd.py:341, 445-482), with no host round trip of the states.
+ */
+int rqx_plan_create_device(rqx_plan **out, int32_t n_qubits,
+                           const uint64_t *xmask, const int64_t *grp_off, int32_t n_groups,
+                           const uint64_t *zmask, const double *coef, int64_t n_terms,
+                           const uint64_t *sorted_states, int64_t n_states, int64_t n_total,
+                           int32_t split_bits, rqx_stream_t stream);
+
+/*
+ * Checksums (64-bit FNV-1a over 64-bit words) of the plan's index arrays, in
+ * this order: keys, low keys, segment high parts, segment offsets, class high
+ * parts, class group offsets, tile first rows, tile segments, tile problems,
+ * problem descriptors, hash-table offsets, hash-table sizes, hash tables,
+ * target-list offsets, target lists, FP64 diagonal.  out: RQX_DIGEST_ARRAYS
+ * host words.  Two plans with equal digests hold the same index (e.g. one built
+ * by rqx_plan_create and one by rqx_plan_create_device from the same states).
+ * No counterpart in the reference; for reproducibility checks.
+ */
+enum { RQX_DIGEST_ARRAYS = 16 };
+int rqx_plan_digest(const rqx_plan *p, uint64_t *out, rqx_stream_t stream);
+
 int rqx_plan_destroy(rqx_plan *plan);
 
 int rqx_plan_get_info(const rqx_plan *plan, rqx_plan_info *out);
@@ -367,6 +397,19 @@ int rqx_batch_dots(const rqx_plan *plan, int32_t n_pairs, const void *const *a, 
  */
 int rqx_batch_combine(const rqx_plan *plan, int32_t n_in, const void *const *v, const double *coef, void *out,
                       int32_t dtype, rqx_stream_t stream);
+
+/*
+ * Packed state rows -> uint64 keys on the device.  rows: n_rows x row_bytes
+ * bytes in device memory, each row big endian with a pad bit first, then
+ * qubits n_qubits-1 .. 0, zero fill (row_bytes = ceil((n_qubits + 1) / 8));
+ * rows whose pad bit is set are filler rows (sqd.py:250-255, 473-478).  keys:
+ * device array of n_rows entries; keys[0..*n_real) are the real rows' states
+ * (bit q = qubit q) in input order; n_real: host pointer (the call
+ * synchronises `stream`).  Replaces the host unpacking of run_sqd's packed
+ * states (rqutils/sqd.py:255 builds them, sqd.py:341 uniquifies them).
+ */
+int rqx_pack_to_u64(const uint8_t *rows, int64_t n_rows, int32_t row_bytes, int32_t n_qubits,
+                    uint64_t *keys, int64_t *n_real, rqx_stream_t stream);
 
 /*
  * Sort and deduplicate n uint64 keys on the device: out[0..*n_unique) is the

@@ -25,6 +25,7 @@ RQX_LOCG_WORK = 168
 RQX_LOCG_SCALE = 164
 RQX_LOCG_STATIC_PROBLEMS = 4096
 RQX_FULL_MAX_QUBITS = 40
+RQX_DIGEST_ARRAYS = 16
 
 # every symbol declared in include/rqx.h
 EXPORTS = (
@@ -34,7 +35,8 @@ EXPORTS = (
     'rqx_plan_update_coef', 'rqx_plan_create_batch', 'rqx_plan_get_problem', 'rqx_plan_set_launch',
     'rqx_eigenpair_batch', 'rqx_batch_dots', 'rqx_batch_combine', 'rqx_plan_build_cache',
     'rqx_plan_drop_cache', 'rqx_plan_cache_info', 'rqx_plan_use_cache', 'rqx_locg_step',
-    'rqx_plan_hits', 'rqx_locg_step_static', 'rqx_full_apply',
+    'rqx_plan_hits', 'rqx_locg_step_static', 'rqx_full_apply', 'rqx_plan_create_device', 'rqx_pack_to_u64',
+    'rqx_plan_digest',
 )
 
 
@@ -112,6 +114,10 @@ _SIGS = {
     'rqx_last_error': (ctypes.c_char_p, []),
     'rqx_plan_create': (_i32, [ctypes.POINTER(_vp), _i32, _vp, _vp, _i32, _vp, _vp, _i64, _vp, _i64,
                                _i64, _i32, _vp]),
+    'rqx_plan_create_device': (_i32, [ctypes.POINTER(_vp), _i32, _vp, _vp, _i32, _vp, _vp, _i64, _vp, _i64,
+                                      _i64, _i32, _vp]),
+    'rqx_pack_to_u64': (_i32, [_vp, _i64, _i32, _i32, _vp, ctypes.POINTER(_i64), _vp]),
+    'rqx_plan_digest': (_i32, [_vp, _vp, _vp]),
     'rqx_plan_destroy': (_i32, [_vp]),
     'rqx_plan_get_info': (_i32, [_vp, ctypes.POINTER(PlanInfo)]),
     'rqx_lookup': (_i32, [_vp, _vp, _vp, _vp]),

@@ -2339,21 +2339,13 @@ int max_segment(const std::vector<int8_t> &hd, int L) {
     return best;
 }
 
-void build_segments(Segmentation &sg, const uint64_t *keys, int64_t n, const uint64_t *xmask, int32_t J) {
+// the longest segment and the classes (runs of equal xmask >> L) of a segmentation
+// whose seg_hi / seg_off are set
+void segment_classes(Segmentation &sg, const uint64_t *xmask, int32_t J) {
     const int L = sg.L;
-    sg.seg_hi.clear();
-    sg.seg_off.clear();
     sg.cls_hi.clear();
     sg.cls_goff.clear();
     sg.max_seg = 0;
-    for (int64_t i = 0; i < n; ++i) {
-        const uint64_t h = (L >= 64) ? 0 : keys[i] >> L;
-        if (i == 0 || h != sg.seg_hi.back()) {
-            sg.seg_hi.push_back(h);
-            sg.seg_off.push_back(static_cast<uint32_t>(i));
-        }
-    }
-    sg.seg_off.push_back(static_cast<uint32_t>(n));
     for (size_t m = 0; m + 1 < sg.seg_off.size(); ++m)
         sg.max_seg = std::max<int>(sg.max_seg, sg.seg_off[m + 1] - sg.seg_off[m]);
     for (int32_t j = 0; j < J; ++j) {
@@ -2364,6 +2356,21 @@ void build_segments(Segmentation &sg, const uint64_t *keys, int64_t n, const uin
         }
     }
     sg.cls_goff.push_back(static_cast<uint32_t>(J));
+}
+
+void build_segments(Segmentation &sg, const uint64_t *keys, int64_t n, const uint64_t *xmask, int32_t J) {
+    const int L = sg.L;
+    sg.seg_hi.clear();
+    sg.seg_off.clear();
+    for (int64_t i = 0; i < n; ++i) {
+        const uint64_t h = (L >= 64) ? 0 : keys[i] >> L;
+        if (i == 0 || h != sg.seg_hi.back()) {
+            sg.seg_hi.push_back(h);
+            sg.seg_off.push_back(static_cast<uint32_t>(i));
+        }
+    }
+    sg.seg_off.push_back(static_cast<uint32_t>(n));
+    segment_classes(sg, xmask, J);
 }
 
 // Estimated per-thread work of the tile kernel, from a deterministic sample of tiles.
@@ -2494,6 +2501,7 @@ struct rqx_plan {
     uint32_t *h_off = nullptr;      // per segment: first hash-table entry
     uint32_t *h_log = nullptr;      // per segment: log2 of its bucket count
     uint32_t *h_tab = nullptr;      // hash tables
+    void *hash_block = nullptr;     // h_off, h_log, h_tab of a device-built index (own allocation)
     void *st_block = nullptr;       // per-segment target lists (own allocation)
     uint32_t *st_off = nullptr;     // M + 1 offsets into st_list
     TargetDesc *st_list = nullptr;  // (class, target segment) pairs with their ranges
@@ -2539,10 +2547,12 @@ bool has_sign_table(const ProblemIn &q, int32_t j) {
 }
 
 // two-choice cuckoo table of one segment's sorted low keys (2^lg buckets of two
-// entries, key << HASH_RANK_BITS | rank); false if the insertions do not settle
-bool cuckoo_build(const uint32_t *keys, uint32_t n, uint32_t lg, uint32_t *tab) {
+// entries, key << HASH_RANK_BITS | rank); false if the insertions do not settle.
+// One routine for the host build and the device build (one thread per
+// segment), so both give the same table.
+__host__ __device__ bool cuckoo_build(const uint32_t *keys, uint32_t n, uint32_t lg, uint32_t *tab) {
     const uint32_t nb = 1u << lg;
-    std::fill(tab, tab + 2 * nb, HASH_EMPTY);
+    for (uint32_t i = 0; i < 2 * nb; ++i) tab[i] = HASH_EMPTY;
     uint64_t rng = 0x9E3779B97F4A7C15ull ^ n;
     for (uint32_t r = 0; r < n; ++r) {
         uint32_t e = (keys[r] << HASH_RANK_BITS) | r;
@@ -2559,7 +2569,9 @@ bool cuckoo_build(const uint32_t *keys, uint32_t n, uint32_t lg, uint32_t *tab) 
             if (placed) break;
             rng = rng * 6364136223846793005ull + 1442695040888963407ull;
             const uint32_t slot = 2 * c[(rng >> 33) & 1] + static_cast<uint32_t>((rng >> 34) & 1);
-            std::swap(e, tab[slot]);
+            const uint32_t evicted = tab[slot];
+            tab[slot] = e;
+            e = evicted;
         }
         if (!placed) return false;
     }
@@ -2788,7 +2800,10 @@ template <typename F> void parallel_chunks(int64_t n, unsigned nt, F f) {
     for (std::thread &t : pool) t.join();
 }
 
-int validate_problem(const ProblemIn &q, int k) {
+int dev_check_states(const uint64_t *st, int64_t n, uint64_t qmask, const std::string &tag, hipStream_t s);
+
+// dev: the states are in device memory (checked there, in row order as on the host)
+int validate_problem(const ProblemIn &q, int k, bool dev, hipStream_t st) {
     const std::string tag = "problem " + std::to_string(k) + ": ";
     if (q.n_qubits < 1 || q.n_qubits > 64) return fail(RQX_ERR_ARG, tag + "n_qubits must be in [1, 64]");
     if (q.n_groups < 0 || q.n_terms < 0 || q.n_states < 0) return fail(RQX_ERR_ARG, tag + "negative size");
@@ -2810,6 +2825,7 @@ int validate_problem(const ProblemIn &q, int k) {
     }
     for (int64_t t = 0; t < q.n_terms; ++t)
         if (q.zmask[t] & ~qmask) return fail(RQX_ERR_ARG, tag + "zmask has bits beyond n_qubits");
+    if (dev) return q.n_states > 0 ? dev_check_states(q.states, q.n_states, qmask, tag, st) : RQX_OK;
     // the first bad state in row order (chunks on host threads for large subspaces)
     const unsigned nt = index_threads(q.n_states);
     std::vector<int> bad_kind(nt, 0);   // per chunk: its first bad state's fault
@@ -2938,13 +2954,526 @@ int choose_split(const ProblemIn &q, int32_t split_bits, bool has_diag, Segmenta
     return RQX_OK;
 }
 
+// ---------------------------------------------------------------- device ingest
+// The O(N) passes of the index build over states already in device memory
+// (straight from the device sort/unique): validation, the segment statistics of
+// every split, the segment starts of the candidate splits, the section split,
+// the low keys and the cuckoo tables.  Only per-segment arrays (O(M) words) come
+// back to the host, where the cost model and the tile lists run as in the host
+// build, so both builds give the same plan.
+
+constexpr int ING_WG = 256;      // threads per workgroup of the ingest kernels
+constexpr int ING_ROWS = 16;     // rows per thread of k_level_stats
+constexpr int ING_LEVELS = 33;   // splits 0..32
+
+// the first bad state: the minimum over rows i of 2 i (bits beyond n_qubits) or
+// 2 i + 1 (not above its predecessor), the host check's first fault in row order
+__global__ void k_check_states(const uint64_t *__restrict__ st, int64_t n, uint64_t qmask,
+                               unsigned long long *first_bad) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint64_t s = st[i];
+    unsigned long long code = ~0ull;
+    if (s & ~qmask)
+        code = 2ull * static_cast<unsigned long long>(i);
+    else if (i > 0 && s <= st[i - 1])
+        code = 2ull * static_cast<unsigned long long>(i) + 1ull;
+    if (code != ~0ull) atomicMin(first_bad, code);
+}
+
+// the segment starts of one split over a run of rows: first and last start
+// (-1: none), the longest segment between two starts of the run, the starts
+struct SegRun {
+    int64_t f, l, m, c;
+};
+
+__device__ inline SegRun seg_join(const SegRun &a, const SegRun &b) {   // a's rows precede b's
+    if (a.c == 0) return b;
+    if (b.c == 0) return a;
+    const int64_t m = a.m > b.m ? a.m : b.m, gap = b.f - a.l;
+    return SegRun{a.f, b.l, m > gap ? m : gap, a.c + b.c};
+}
+
+// in-order join over the ING_WG threads of a workgroup (the result in thread 0)
+__device__ SegRun seg_reduce(SegRun r, SegRun *red) {
+    for (int off = 1; off < 64; off <<= 1) {
+        SegRun o;
+        o.f = __shfl_down(r.f, off);
+        o.l = __shfl_down(r.l, off);
+        o.m = __shfl_down(r.m, off);
+        o.c = __shfl_down(r.c, off);
+        if ((threadIdx.x & 63) + off < 64) r = seg_join(r, o);
+    }
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = r;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int w = 1; w < ING_WG / 64; ++w) r = seg_join(r, red[w]);
+    __syncthreads();   // red is reused by the next call
+    return r;
+}
+
+// per workgroup of ING_WG * ING_ROWS rows: its run of every split L < ING_LEVELS
+// (part[L * nwg + wg]; row i starts a segment of split L when i == 0 or its
+// highest bit differing from row i - 1 is >= L), and per highest differing bit
+// h the first row with that bit (first_pos[h], atomic minimum)
+__global__ __launch_bounds__(ING_WG) void k_level_stats(const uint64_t *__restrict__ st, int64_t n, int64_t nwg,
+                                                        SegRun *__restrict__ part, unsigned long long *first_pos) {
+    __shared__ SegRun red[ING_WG / 64];
+    __shared__ unsigned long long fpos[64];
+    if (threadIdx.x < 64) fpos[threadIdx.x] = ~0ull;
+    const int64_t i0 = (static_cast<int64_t>(blockIdx.x) * ING_WG + threadIdx.x) * ING_ROWS;
+    int hd[ING_ROWS];   // 64: row 0, -1: past the end
+    uint64_t prev = (i0 > 0 && i0 < n) ? st[i0 - 1] : 0ull;
+#pragma unroll
+    for (int r = 0; r < ING_ROWS; ++r) {
+        const int64_t i = i0 + r;
+        hd[r] = -1;
+        if (i < n) {
+            const uint64_t s = st[i];
+            hd[r] = (i == 0) ? 64 : 63 - __clzll(static_cast<long long>(prev ^ s));
+            prev = s;
+        }
+    }
+    __syncthreads();   // fpos initialised
+    uint64_t seen = 0;
+#pragma unroll
+    for (int r = 0; r < ING_ROWS; ++r)
+        if (hd[r] >= 0 && hd[r] < 64 && !((seen >> hd[r]) & 1ull)) {
+            seen |= 1ull << hd[r];
+            atomicMin(&fpos[hd[r]], static_cast<unsigned long long>(i0 + r));
+        }
+    for (int L = 0; L < ING_LEVELS; ++L) {
+        SegRun run{-1, -1, 0, 0};
+#pragma unroll
+        for (int r = 0; r < ING_ROWS; ++r)
+            if (hd[r] >= L) {
+                const int64_t i = i0 + r;
+                if (run.c)
+                    run.m = (i - run.l > run.m) ? i - run.l : run.m;
+                else
+                    run.f = i;
+                run.l = i;
+                ++run.c;
+            }
+        run = seg_reduce(run, red);   // synchronises the workgroup (fpos complete after the first)
+        if (threadIdx.x == 0) part[static_cast<int64_t>(L) * nwg + blockIdx.x] = run;
+    }
+    if (threadIdx.x < 64 && fpos[threadIdx.x] != ~0ull) atomicMin(&first_pos[threadIdx.x], fpos[threadIdx.x]);
+}
+
+// per split L (one workgroup each): the longest segment and the number of
+// segments, from the workgroups' runs joined in row order (out[2 L], out[2 L + 1])
+__global__ __launch_bounds__(ING_WG) void k_level_join(const SegRun *__restrict__ part, int64_t nwg, int64_t n,
+                                                       int64_t *__restrict__ out) {
+    __shared__ SegRun red[ING_WG / 64];
+    const int L = blockIdx.x;
+    const int64_t per = (nwg + ING_WG - 1) / ING_WG;
+    const int64_t w0 = static_cast<int64_t>(threadIdx.x) * per, w1 = (w0 + per < nwg) ? w0 + per : nwg;
+    SegRun run{-1, -1, 0, 0};
+    for (int64_t w = w0; w < w1; ++w) run = seg_join(run, part[static_cast<int64_t>(L) * nwg + w]);
+    run = seg_reduce(run, red);
+    if (threadIdx.x == 0) {
+        out[2 * L] = run.c ? (run.m > n - run.l ? run.m : n - run.l) : 0;
+        out[2 * L + 1] = run.c;
+    }
+}
+
+// the segment starts of split L, for the device select
+struct SegStartAt {
+    const uint64_t *st;
+    int L;
+    __host__ __device__ bool operator()(const uint32_t &i) const {
+        return i == 0 || (st[i] >> L) != (st[i - 1] >> L);
+    }
+};
+
+// seg_hi of the first min(M, *num) selected starts (num: the select's count)
+__global__ void k_seg_hi(const uint64_t *__restrict__ st, const uint32_t *__restrict__ seg_off, int64_t M,
+                         const int64_t *__restrict__ num, int L, uint64_t *__restrict__ seg_hi) {
+    const int64_t m = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (m < M && m < *num) seg_hi[m] = st[seg_off[m]] >> L;
+}
+
+struct SplitCands {   // candidate section splits: bit b[j], first-block length m[j]
+    int32_t n;
+    int32_t b[64];
+    uint32_t m[64];
+};
+
+// bit j of *fail: the states are no product at candidate j -- blocks of m[j]
+// rows with one value of s >> b[j] each, changing at every block start, whose
+// low parts s & (2^b[j] - 1) equal the first block's row for row
+__global__ void k_product_check(const uint64_t *__restrict__ st, int64_t n, SplitCands c,
+                                unsigned long long *fail) {
+    const int64_t i64 = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i64 >= n) return;
+    const uint32_t i = static_cast<uint32_t>(i64);
+    const uint64_t s = st[i], prev = i > 0 ? st[i - 1] : 0ull;
+    unsigned long long bad = 0;
+    for (int j = 0; j < c.n; ++j) {
+        const int b = c.b[j];
+        const uint32_t m = c.m[j], r = i % m;
+        const bool same_hi = i > 0 && (s >> b) == (prev >> b);
+        const uint64_t lm = (1ull << b) - 1ull;
+        if ((r == 0) == same_hi || (i >= m && (s & lm) != (st[r] & lm))) bad |= 1ull << j;
+    }
+    if (bad) atomicOr(fail, bad);
+}
+
+__global__ void k_low_keys(const uint64_t *__restrict__ st, int64_t n, uint32_t lomask, uint32_t *__restrict__ lo) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) lo[i] = static_cast<uint32_t>(st[i]) & lomask;
+}
+
+// one thread per segment m with lg[m] >= 0: its cuckoo table of 2 << lg[m]
+// entries at tab + off[m]; ok[m] = 1 when the insertions settled
+__global__ void k_cuckoo(const uint32_t *__restrict__ lo, const uint32_t *__restrict__ seg_off, int64_t M,
+                         const int8_t *__restrict__ lg, const uint64_t *__restrict__ off, uint32_t *tab,
+                         uint8_t *__restrict__ ok) {
+    const int64_t m = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (m >= M || lg[m] < 0) return;
+    const uint32_t r0 = seg_off[m];
+    ok[m] = cuckoo_build(lo + r0, seg_off[m + 1] - r0, static_cast<uint32_t>(lg[m]), tab + off[m]) ? 1 : 0;
+}
+
+// packed rows (big endian, pad bit first: sqd.py:255) -> uint64 keys and a
+// real-row flag (pad bit 0)
+__global__ void k_unpack_rows(const uint8_t *__restrict__ rows, int64_t n, int row_bytes, int n_qubits,
+                              uint64_t *__restrict__ keys, uint8_t *__restrict__ real) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint8_t *r = rows + i * row_bytes;
+    uint64_t v = 0;   // the first (up to) 8 bytes, left aligned
+    for (int k = 0; k < row_bytes && k < 8; ++k) v |= static_cast<uint64_t>(r[k]) << (56 - 8 * k);
+    uint64_t key;
+    if (row_bytes == 9)   // 64 qubits: the last key bit is the top bit of byte 8
+        key = (v << 1) | (static_cast<uint64_t>(r[8]) >> 7);
+    else
+        key = (v << 1) >> (64 - n_qubits);
+    keys[i] = key;
+    real[i] = (r[0] >> 7) == 0 ? 1 : 0;
+}
+
+inline unsigned grid_for(int64_t n, int wg = ING_WG) { return static_cast<unsigned>((n + wg - 1) / wg); }
+
+// device scratch of the plan build, released on scope exit (after a stream sync
+// by its user)
+struct DevBuf {
+    void *p = nullptr;
+    hipError_t alloc(size_t bytes) { return hipMalloc(&p, std::max<size_t>(bytes, 1)); }
+    template <typename T> T *at(size_t byte_off = 0) const {
+        return reinterpret_cast<T *>(static_cast<char *>(p) + byte_off);
+    }
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+};
+
+#define ING_TRY(expr)                                                                   \
+    do {                                                                                \
+        hipError_t e_ = (expr);                                                         \
+        if (e_ != hipSuccess)                                                           \
+            return fail(RQX_ERR_HIP, std::string("device ingest: ") + hipGetErrorString(e_)); \
+    } while (0)
+
+// the statistics of one problem's device states
+struct DevIngest {
+    const uint64_t *st = nullptr;
+    int64_t n = 0;
+    hipStream_t s = nullptr;
+    int64_t max_seg[ING_LEVELS] = {};   // longest segment of split L
+    int64_t n_seg[ING_LEVELS] = {};     // segments of split L
+    int64_t first_pos[64] = {};         // first row whose highest bit differing from its predecessor is h (n: none)
+};
+
+// rows beyond n_qubits or out of order, as the host check reports them
+int dev_check_states(const uint64_t *st, int64_t n, uint64_t qmask, const std::string &tag, hipStream_t s) {
+    hipPointerAttribute_t attr{};
+    if (hipPointerGetAttributes(&attr, st) != hipSuccess || attr.type != hipMemoryTypeDevice) {
+        (void)hipGetLastError();
+        return fail(RQX_ERR_ARG, tag + "sorted_states must be device memory");
+    }
+    DevBuf b;
+    ING_TRY(b.alloc(8));
+    unsigned long long *bad = b.at<unsigned long long>();
+    ING_TRY(fill_async(bad, int64_t(0), int64_t(1), ~0ull, s));
+    hipLaunchKernelGGL(k_check_states, dim3(grid_for(n)), dim3(ING_WG), 0, s, st, n, qmask, bad);
+    ING_TRY(hipGetLastError());
+    unsigned long long h = 0;
+    ING_TRY(hipMemcpyAsync(&h, bad, 8, hipMemcpyDeviceToHost, s));
+    ING_TRY(hipStreamSynchronize(s));
+    if (h == ~0ull) return RQX_OK;
+    return fail(RQX_ERR_ARG, tag + ((h & 1) ? "states must be sorted ascending and unique"
+                                            : "state has bits beyond n_qubits"));
+}
+
+int dev_level_stats(DevIngest &dv) {
+    const int64_t nwg = (dv.n + ING_WG * ING_ROWS - 1) / (ING_WG * ING_ROWS);
+    const size_t o_part = 0, o_out = align_up(static_cast<size_t>(nwg) * ING_LEVELS * sizeof(SegRun)),
+                 o_first = o_out + align_up(2 * ING_LEVELS * 8);
+    DevBuf b;
+    ING_TRY(b.alloc(o_first + 64 * 8));
+    ING_TRY(fill_async(b.at<unsigned long long>(o_first), int64_t(0), int64_t(64), ~0ull, dv.s));
+    hipLaunchKernelGGL(k_level_stats, dim3(static_cast<unsigned>(nwg)), dim3(ING_WG), 0, dv.s, dv.st, dv.n, nwg,
+                       b.at<SegRun>(o_part), b.at<unsigned long long>(o_first));
+    hipLaunchKernelGGL(k_level_join, dim3(ING_LEVELS), dim3(ING_WG), 0, dv.s, b.at<SegRun>(o_part), nwg, dv.n,
+                       b.at<int64_t>(o_out));
+    ING_TRY(hipGetLastError());
+    int64_t out[2 * ING_LEVELS];
+    unsigned long long first[64];
+    ING_TRY(hipMemcpyAsync(out, b.at<int64_t>(o_out), sizeof(out), hipMemcpyDeviceToHost, dv.s));
+    ING_TRY(hipMemcpyAsync(first, b.at<unsigned long long>(o_first), sizeof(first), hipMemcpyDeviceToHost, dv.s));
+    ING_TRY(hipStreamSynchronize(dv.s));
+    for (int L = 0; L < ING_LEVELS; ++L) {
+        dv.max_seg[L] = out[2 * L];
+        dv.n_seg[L] = out[2 * L + 1];
+    }
+    for (int h = 0; h < 64; ++h) dv.first_pos[h] = first[h] == ~0ull ? dv.n : static_cast<int64_t>(first[h]);
+    return RQX_OK;
+}
+
+// split L's segments from the device states (their starts by a device select),
+// then the classes and the cost model on the host, as build_segments does
+int dev_segmentation(const DevIngest &dv, int L, const ProblemIn &q, bool has_diag, Segmentation &sg) {
+    const int64_t M = dv.n_seg[L];
+    if (M < 1 || M > dv.n) return fail(RQX_ERR_HIP, "device ingest: bad segment count");
+    const int ni = static_cast<int>(dv.n);
+    size_t t_sel = 0;
+    hipcub::CountingInputIterator<uint32_t> rows(0);
+    ING_TRY(hipcub::DeviceSelect::If(nullptr, t_sel, rows, static_cast<uint32_t *>(nullptr),
+                                     static_cast<int64_t *>(nullptr), ni, SegStartAt{dv.st, L}, dv.s));
+    // the starts buffer holds n entries, whatever the select finds
+    const size_t o_off = align_up(t_sel), o_hi = o_off + align_up(static_cast<size_t>(dv.n) * 4),
+                 o_num = o_hi + align_up(static_cast<size_t>(M) * 8);
+    DevBuf b;
+    ING_TRY(b.alloc(o_num + 8));
+    ING_TRY(hipcub::DeviceSelect::If(b.p, t_sel, rows, b.at<uint32_t>(o_off), b.at<int64_t>(o_num), ni,
+                                     SegStartAt{dv.st, L}, dv.s));
+    hipLaunchKernelGGL(k_seg_hi, dim3(grid_for(M)), dim3(ING_WG), 0, dv.s, dv.st, b.at<uint32_t>(o_off), M,
+                       b.at<int64_t>(o_num), L, b.at<uint64_t>(o_hi));
+    ING_TRY(hipGetLastError());
+    sg.L = L;
+    sg.seg_off.assign(static_cast<size_t>(M) + 1, static_cast<uint32_t>(dv.n));
+    sg.seg_hi.resize(static_cast<size_t>(M));
+    int64_t num = 0;
+    ING_TRY(hipMemcpyAsync(sg.seg_off.data(), b.at<uint32_t>(o_off), static_cast<size_t>(M) * 4,
+                           hipMemcpyDeviceToHost, dv.s));
+    ING_TRY(hipMemcpyAsync(sg.seg_hi.data(), b.at<uint64_t>(o_hi), static_cast<size_t>(M) * 8,
+                           hipMemcpyDeviceToHost, dv.s));
+    ING_TRY(hipMemcpyAsync(&num, b.at<int64_t>(o_num), 8, hipMemcpyDeviceToHost, dv.s));
+    ING_TRY(hipStreamSynchronize(dv.s));
+    if (num != M) return fail(RQX_ERR_HIP, "device ingest: segment count mismatch");
+    segment_classes(sg, q.xmask, q.n_groups);
+    sg.cost = estimate_cost(sg, has_diag);
+    return RQX_OK;
+}
+
+// choose_split over device states: the same search and candidates, with the
+// longest segments of every split from one pass
+int dev_choose_split(const DevIngest &dv, const ProblemIn &q, int32_t split_bits, bool has_diag, Segmentation &best,
+                     int k) {
+    const int Lmax_bits = std::min(32, q.n_qubits);
+    const std::string tag = "problem " + std::to_string(k) + ": ";
+    if (split_bits >= 0) {
+        if (split_bits > Lmax_bits) return fail(RQX_ERR_ARG, tag + "split_bits must be <= min(32, n_qubits)");
+        if (dv.max_seg[split_bits] > SEG_CAP)
+            return fail(RQX_ERR_LIMIT, tag + "split_bits gives a segment larger than " + std::to_string(SEG_CAP) + " rows");
+        return dev_segmentation(dv, split_bits, q, has_diag, best);
+    }
+    int lo_l = 0, hi_l = Lmax_bits;
+    while (lo_l < hi_l) {
+        const int mid = (lo_l + hi_l + 1) / 2;
+        if (dv.max_seg[mid] <= SEG_CAP) lo_l = mid; else hi_l = mid - 1;
+    }
+    const int n_cand = lo_l - std::max(0, lo_l - 8) + 1;
+    for (int c = 0; c < n_cand; ++c) {   // the first cheapest (largest L) wins
+        Segmentation sg;
+        if (int rc = dev_segmentation(dv, lo_l - c, q, has_diag, sg)) return rc;
+        if (c == 0 || sg.cost < best.cost) best = std::move(sg);
+    }
+    return RQX_OK;
+}
+
+// section_split over device states: the candidate bits whose first block
+// divides the rows, checked in one pass (the largest that holds wins)
+int dev_section_split(const DevIngest &dv, int32_t L, int32_t n_qubits, int32_t &a_bits) {
+    a_bits = L;
+    if (dv.n < 2) return RQX_OK;
+    SplitCands c{};
+    for (int32_t b = n_qubits - 1; b > L; --b) {
+        int64_t m = dv.n;
+        for (int h = b; h < 64; ++h) m = std::min(m, dv.first_pos[h]);
+        if (m == dv.n || dv.n % m != 0) continue;
+        c.b[c.n] = b;
+        c.m[c.n] = static_cast<uint32_t>(m);
+        ++c.n;
+    }
+    if (c.n == 0) return RQX_OK;
+    DevBuf b;
+    ING_TRY(b.alloc(8));
+    ING_TRY(fill_async(b.at<unsigned long long>(), int64_t(0), int64_t(1), 0ull, dv.s));
+    hipLaunchKernelGGL(k_product_check, dim3(grid_for(dv.n)), dim3(ING_WG), 0, dv.s, dv.st, dv.n, c,
+                       b.at<unsigned long long>());
+    ING_TRY(hipGetLastError());
+    unsigned long long failed = 0;
+    ING_TRY(hipMemcpyAsync(&failed, b.at<unsigned long long>(), 8, hipMemcpyDeviceToHost, dv.s));
+    ING_TRY(hipStreamSynchronize(dv.s));
+    for (int j = 0; j < c.n; ++j)
+        if (!((failed >> j) & 1ull)) {
+            a_bits = c.b[j];
+            break;
+        }
+    return RQX_OK;
+}
+
+// build_hash_tables over the plan's device low keys: every segment tries its
+// table sizes in the host order, in rounds (one launch each); a round-0 success
+// everywhere leaves the tables in place, else all are rebuilt at their final
+// sizes.  hashed = false when some segment does not settle within max_words.
+int dev_hash_tables(rqx_plan *p, const std::vector<uint32_t> &seg_off, size_t max_words, hipStream_t s,
+                    bool &hashed) {
+    hashed = false;
+    const size_t M = seg_off.size() - 1;
+    if (M == 0) {   // no segment: empty tables, as on the host
+        hashed = true;
+        return RQX_OK;
+    }
+    std::vector<int8_t> lg0(M), fin(M, -1);
+    for (size_t m = 0; m < M; ++m) {
+        const uint32_t n = seg_off[m + 1] - seg_off[m];
+        uint32_t lg = 1;
+        while ((2u << lg) * 7u < n * 10u) ++lg;   // load <= 0.7
+        lg0[m] = static_cast<int8_t>(lg);
+    }
+    // the hash block: h_off, h_log (M words each), then the tables
+    const size_t o_log = align_up(M * 4), o_tab = o_log + align_up(M * 4);
+    auto layout = [&](const std::vector<int8_t> &lg, std::vector<uint64_t> &off) -> uint64_t {
+        uint64_t total = 0;
+        off.assign(M, 0);
+        for (size_t m = 0; m < M; ++m) {
+            off[m] = total;
+            if (lg[m] >= 0) total += uint64_t(2) << lg[m];
+        }
+        return total;
+    };
+    DevBuf meta;   // per segment: lg (1 byte), off (8 bytes), ok (1 byte)
+    const size_t o_moff = align_up(M), o_mok = o_moff + align_up(M * 8);
+    ING_TRY(meta.alloc(o_mok + M));
+    auto run = [&](const std::vector<int8_t> &lg, uint32_t *tab, std::vector<uint8_t> &ok) -> hipError_t {
+        std::vector<uint64_t> off;
+        layout(lg, off);
+        ok.assign(M, 0);
+        hipError_t e = hipMemcpyAsync(meta.at<int8_t>(), lg.data(), M, hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) e = hipMemcpyAsync(meta.at<uint64_t>(o_moff), off.data(), M * 8, hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) {
+            hipLaunchKernelGGL(k_cuckoo, dim3(grid_for(static_cast<int64_t>(M), 64)), dim3(64), 0, s, p->lo,
+                               p->seg_off, static_cast<int64_t>(M), meta.at<int8_t>(), meta.at<uint64_t>(o_moff), tab,
+                               meta.at<uint8_t>(o_mok));
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess) e = hipMemcpyAsync(ok.data(), meta.at<uint8_t>(o_mok), M, hipMemcpyDeviceToHost, s);
+        const hipError_t es = hipStreamSynchronize(s);   // lg and off are read before they go
+        return e == hipSuccess ? es : e;
+    };
+    std::vector<uint64_t> off0;
+    const uint64_t total0 = layout(lg0, off0);
+    for (size_t m = 0; m < M; ++m)
+        if ((size_t(2) << lg0[m]) > max_words) return RQX_OK;
+    if (total0 > UINT32_MAX) return RQX_OK;
+    void *blk = nullptr;
+    ING_TRY(hipMalloc(&blk, o_tab + std::max<uint64_t>(total0, 1) * 4));
+    auto drop = [&]() {
+        (void)hipStreamSynchronize(s);
+        (void)hipFree(blk);
+        blk = nullptr;
+    };
+    std::vector<uint8_t> ok;
+    hipError_t e = run(lg0, reinterpret_cast<uint32_t *>(static_cast<char *>(blk) + o_tab), ok);
+    if (e != hipSuccess) {
+        drop();
+        return fail(RQX_ERR_HIP, std::string("device ingest: ") + hipGetErrorString(e));
+    }
+    bool all = true;
+    for (size_t m = 0; m < M; ++m) {
+        if (ok[m]) fin[m] = lg0[m];
+        all = all && ok[m];
+    }
+    if (!all) {
+        // further tries of the failed segments (host: up to 3 sizes, lg0 .. lg0 + 2)
+        for (int tries = 1; tries < 3; ++tries) {
+            std::vector<int8_t> lg(M, -1);
+            bool pending = false;
+            for (size_t m = 0; m < M; ++m)
+                if (fin[m] < 0 && (size_t(2) << (lg0[m] + tries)) <= max_words) {
+                    lg[m] = static_cast<int8_t>(lg0[m] + tries);
+                    pending = true;
+                }
+            if (!pending) break;
+            std::vector<uint64_t> off;
+            const uint64_t total = layout(lg, off);
+            DevBuf scratch;
+            e = scratch.alloc(total * 4);
+            if (e == hipSuccess) e = run(lg, scratch.at<uint32_t>(), ok);
+            if (e != hipSuccess) {
+                drop();
+                return fail(RQX_ERR_HIP, std::string("device ingest: ") + hipGetErrorString(e));
+            }
+            for (size_t m = 0; m < M; ++m)
+                if (lg[m] >= 0 && ok[m]) fin[m] = lg[m];
+        }
+        for (size_t m = 0; m < M; ++m)
+            if (fin[m] < 0) {
+                drop();
+                return RQX_OK;   // some table does not settle: binary search
+            }
+        // every table at its final size, rebuilt in place (the same tables as the tries)
+        std::vector<uint64_t> off;
+        const uint64_t total = layout(fin, off);
+        drop();
+        if (total > UINT32_MAX) return RQX_OK;
+        ING_TRY(hipMalloc(&blk, o_tab + std::max<uint64_t>(total, 1) * 4));
+        e = run(fin, reinterpret_cast<uint32_t *>(static_cast<char *>(blk) + o_tab), ok);
+        for (size_t m = 0; e == hipSuccess && m < M; ++m)
+            if (!ok[m]) e = hipErrorUnknown;
+        if (e != hipSuccess) {
+            drop();
+            return fail(RQX_ERR_HIP, std::string("device ingest (hash rebuild): ") + hipGetErrorString(e));
+        }
+    }
+    std::vector<uint64_t> off;
+    layout(fin, off);
+    std::vector<uint32_t> h_off(M), h_log(M);
+    for (size_t m = 0; m < M; ++m) {
+        h_off[m] = static_cast<uint32_t>(off[m]);
+        h_log[m] = static_cast<uint32_t>(fin[m]);
+        p->hash_words = std::max(p->hash_words, 2 << fin[m]);
+    }
+    char *c = static_cast<char *>(blk);
+    e = hipMemcpyAsync(c, h_off.data(), M * 4, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(c + o_log, h_log.data(), M * 4, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+        drop();
+        p->hash_words = 0;
+        return fail(RQX_ERR_HIP, std::string("device ingest: ") + hipGetErrorString(e));
+    }
+    p->hash_block = blk;
+    p->h_off = reinterpret_cast<uint32_t *>(c);
+    p->h_log = reinterpret_cast<uint32_t *>(c + o_log);
+    p->h_tab = reinterpret_cast<uint32_t *>(c + o_tab);
+    hashed = true;
+    return RQX_OK;
+}
+
+// dev: every problem's states are in device memory (the device ingest passes
+// replace the host passes over them; the plan is the same)
 int build_plan(rqx_plan **out, const std::vector<ProblemIn> &in, int64_t n_total, int32_t split_bits,
-               int persistent, hipStream_t st) {
+               int persistent, hipStream_t st, bool dev = false) {
     *out = nullptr;
     const int K = static_cast<int>(in.size());
     int64_t N = 0, T = 0, J = 0;
     for (int k = 0; k < K; ++k) {
-        if (int rc = validate_problem(in[k], k)) return rc;
+        if (int rc = validate_problem(in[k], k, dev, st)) return rc;
         N += in[k].n_states;
         T += in[k].n_terms;
         J += in[k].n_groups;
@@ -2964,7 +3493,7 @@ int build_plan(rqx_plan **out, const std::vector<ProblemIn> &in, int64_t n_total
     p->n_groups = static_cast<int32_t>(J);
 
     // host arrays of the whole plan
-    std::vector<uint32_t> lo(static_cast<size_t>(N)), seg_off_all, cls_goff_all, g_xlo, g_toff, tile_row0, tile_seg,
+    std::vector<uint32_t> lo(dev ? 0 : static_cast<size_t>(N)), seg_off_all, cls_goff_all, g_xlo, g_toff, tile_row0, tile_seg,
         tile_prob;
     std::vector<uint64_t> seg_hi_all, cls_hi_all, g_x, zall;   // (keys are uploaded from the caller's arrays)
     std::vector<double> c64;
@@ -2999,9 +3528,19 @@ int build_plan(rqx_plan **out, const std::vector<ProblemIn> &in, int64_t n_total
             if (q.xmask[j] != 0 && !has_sign_table(q, j)) ph.max_offdiag_terms = std::max(ph.max_offdiag_terms, len);
         }
         Segmentation sg;
-        if (int rc = choose_split(q, split_bits, ph.has_diag, sg, k)) return rc;
+        if (dev && q.n_states > 0) {
+            DevIngest dv;
+            dv.st = q.states;
+            dv.n = q.n_states;
+            dv.s = st;
+            if (int rc = dev_level_stats(dv)) return rc;
+            if (int rc = dev_choose_split(dv, q, split_bits, ph.has_diag, sg, k)) return rc;
+            if (int rc = dev_section_split(dv, sg.L, q.n_qubits, ph.a_bits)) return rc;
+        } else {
+            if (int rc = choose_split(q, split_bits, ph.has_diag, sg, k)) return rc;
+            ph.a_bits = section_split(q.states, q.n_states, sg.L, q.n_qubits);
+        }
         ph.L = sg.L;
-        ph.a_bits = section_split(q.states, q.n_states, sg.L, q.n_qubits);
         ph.max_seg = sg.max_seg;
         ph.cost = sg.cost;
         ph.seg_base = static_cast<int32_t>(seg_hi_all.size());
@@ -3009,9 +3548,10 @@ int build_plan(rqx_plan **out, const std::vector<ProblemIn> &in, int64_t n_total
         ph.cls_base = static_cast<int32_t>(cls_hi_all.size());
         ph.n_classes = static_cast<int32_t>(sg.cls_hi.size());
         const uint32_t lomask = (sg.L >= 32) ? 0xffffffffu : ((1u << sg.L) - 1u);
-        parallel_chunks(q.n_states, index_threads(q.n_states), [&](unsigned, int64_t b, int64_t e) {
-            for (int64_t i = b; i < e; ++i) lo[row_base + i] = static_cast<uint32_t>(q.states[i]) & lomask;
-        });
+        if (!dev)
+            parallel_chunks(q.n_states, index_threads(q.n_states), [&](unsigned, int64_t b, int64_t e) {
+                for (int64_t i = b; i < e; ++i) lo[row_base + i] = static_cast<uint32_t>(q.states[i]) & lomask;
+            });
         for (int32_t m = 0; m < ph.n_segs; ++m) {
             seg_hi_all.push_back(sg.seg_hi[m]);
             seg_off_all.push_back(static_cast<uint32_t>(row_base + sg.seg_off[m]));
@@ -3104,8 +3644,11 @@ int build_plan(rqx_plan **out, const std::vector<ProblemIn> &in, int64_t n_total
     for (const ProbHost &ph : p->probs) hash_fits = hash_fits && ph.L <= HASH_MAX_KEY_BITS;
     int lds_max = 65536;
     HIP_TRY(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, p->device));
-    if (hash_fits && !seg_off_all.empty() &&
-        build_hash_tables(lo, seg_off_all, static_cast<size_t>(std::max(lds_max - 256, 0)) / 4, h_off, h_log, h_tab)) {
+    const size_t hash_max_words = static_cast<size_t>(std::max(lds_max - 256, 0)) / 4;
+    if (dev) {
+        // built after the upload, from the device low keys
+    } else if (hash_fits && !seg_off_all.empty() &&
+        build_hash_tables(lo, seg_off_all, hash_max_words, h_off, h_log, h_tab)) {
         p->hashed = 1;
         for (uint32_t lg : h_log) p->hash_words = std::max(p->hash_words, 2 << lg);
     } else {
@@ -3184,6 +3727,7 @@ int build_plan(rqx_plan **out, const std::vector<ProblemIn> &in, int64_t n_total
         (void)hipStreamSynchronize(st);   // no copy or kernel of the plan still in flight
         (void)hipFree(raw->block);
         (void)hipFree(raw->st_block);
+        (void)hipFree(raw->hash_block);
     };
 
     auto up = [&](void *dst, const void *src, size_t bytes) -> hipError_t {
@@ -3191,7 +3735,7 @@ int build_plan(rqx_plan **out, const std::vector<ProblemIn> &in, int64_t n_total
         return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st);
     };
     const hipError_t errs[] = {
-        up(p->lo, lo.data(), n * 4),
+        up(p->lo, lo.data(), lo.size() * 4),
         up(p->seg_hi, seg_hi_all.data(), seg_hi_all.size() * 8), up(p->seg_off, seg_off_all.data(), seg_off_all.size() * 4),
         up(p->cls_hi, cls_hi_all.data(), cls_hi_all.size() * 8), up(p->cls_goff, cls_goff_all.data(), cls_goff_all.size() * 4),
         up(p->g_xlo, g_xlo.data(), g_xlo.size() * 4), up(p->g_toff, g_toff.data(), g_toff.size() * 4),
@@ -3204,7 +3748,13 @@ int build_plan(rqx_plan **out, const std::vector<ProblemIn> &in, int64_t n_total
         up(p->h_off, h_off.data(), h_off.size() * 4), up(p->h_log, h_log.data(), h_log.size() * 4),
         up(p->h_tab, h_tab.data(), h_tab.size() * 4)};
     std::vector<hipError_t> errs_keys;   // the keys straight from the callers' arrays (read before return)
-    for (int k = 0; k < K; ++k) errs_keys.push_back(up(p->keys + prob_rows[k], in[k].states, static_cast<size_t>(in[k].n_states) * 8));
+    for (int k = 0; k < K; ++k) {
+        const size_t bytes = static_cast<size_t>(in[k].n_states) * 8;
+        errs_keys.push_back(!dev ? up(p->keys + prob_rows[k], in[k].states, bytes)
+                            : bytes ? hipMemcpyAsync(p->keys + prob_rows[k], in[k].states, bytes,
+                                                     hipMemcpyDeviceToDevice, st)
+                                    : hipSuccess);
+    }
     for (hipError_t ei : errs_keys)
         if (ei != hipSuccess) {
             release();
@@ -3215,12 +3765,24 @@ int build_plan(rqx_plan **out, const std::vector<ProblemIn> &in, int64_t n_total
             release();
             return fail(RQX_ERR_HIP, std::string("hipMemcpyAsync(plan): ") + hipGetErrorString(ei));
         }
-    for (const ProbHost &ph : p->probs) {
+    for (size_t k = 0; k < p->probs.size(); ++k) {
+        const ProbHost &ph = p->probs[k];
         if (ph.n_states == 0) continue;
         const unsigned blocks = static_cast<unsigned>((ph.n_states + 255) / 256);
         const uint32_t k0 = static_cast<uint32_t>(ph.term_base);
         hipLaunchKernelGGL(k_diag, dim3(blocks), dim3(256), 0, st, p->keys + ph.row_base, ph.n_states, p->z, p->coef64,
                            k0, k0 + ph.diag_terms, p->diag64 + ph.row_base, p->diag32 + ph.row_base);
+        if (dev)
+            hipLaunchKernelGGL(k_low_keys, dim3(grid_for(ph.n_states)), dim3(ING_WG), 0, st, p->keys + ph.row_base,
+                               ph.n_states, desc[k].lomask, p->lo + ph.row_base);
+    }
+    if (dev && hash_fits && !seg_off_all.empty()) {
+        bool hashed = false;
+        if (int rc = dev_hash_tables(raw, seg_off_all, hash_max_words, st, hashed)) {
+            release();
+            return rc;
+        }
+        p->hashed = hashed ? 1 : 0;
     }
     launch_group_tables(raw, st);
     hipError_t e = hipGetLastError();
@@ -3388,6 +3950,93 @@ int rqx_plan_create_batch(rqx_plan **out, int32_t n_problems, const int32_t *n_q
     return build_plan(out, in, -1, split_bits, 1, static_cast<hipStream_t>(stream));
 }
 
+int rqx_plan_create_device(rqx_plan **out, int32_t n_qubits, const uint64_t *xmask, const int64_t *grp_off,
+                           int32_t n_groups, const uint64_t *zmask, const double *coef, int64_t n_terms,
+                           const uint64_t *sorted_states, int64_t n_states, int64_t n_total, int32_t split_bits,
+                           rqx_stream_t stream) {
+    g_err.clear();
+    if (!out) return fail(RQX_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    if (n_total < n_states) return fail(RQX_ERR_ARG, "negative size or n_total < n_states");
+    std::vector<ProblemIn> in = {{n_qubits, xmask, grp_off, n_groups, zmask, coef, n_terms, sorted_states, n_states}};
+    return build_plan(out, in, n_total, split_bits, 0, static_cast<hipStream_t>(stream), true);
+}
+
+int rqx_plan_digest(const rqx_plan *p, uint64_t *out, rqx_stream_t stream) {
+    g_err.clear();
+    if (!p || !out) return fail(RQX_ERR_ARG, "NULL argument");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    HIP_TRY(hipStreamSynchronize(st));
+    const size_t N = static_cast<size_t>(p->n_states), M = static_cast<size_t>(p->n_segs),
+                 C = static_cast<size_t>(p->n_classes), T = static_cast<size_t>(p->n_tiles);
+    std::vector<uint32_t> hoff, hlog, soff(M + 1);
+    size_t htab = 0;
+    if (p->hashed && M) {
+        hoff.resize(M);
+        hlog.resize(M);
+        HIP_TRY(hipMemcpy(hoff.data(), p->h_off, M * 4, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(hlog.data(), p->h_log, M * 4, hipMemcpyDeviceToHost));
+        htab = static_cast<size_t>(hoff[M - 1]) + (size_t(2) << hlog[M - 1]);
+    }
+    HIP_TRY(hipMemcpy(soff.data(), p->st_off, (M + 1) * 4, hipMemcpyDeviceToHost));
+    const struct {
+        const void *ptr;
+        size_t bytes;
+    } arrays[RQX_DIGEST_ARRAYS] = {
+        {p->keys, N * 8}, {p->lo, N * 4}, {p->seg_hi, M * 8}, {p->seg_off, (M + 1) * 4},
+        {p->cls_hi, C * 8}, {p->cls_goff, (C + 1) * 4}, {p->tile_row0, T * 4}, {p->tile_seg, T * 4},
+        {p->tile_prob, T * 4}, {p->prob, p->probs.size() * sizeof(ProbDesc)},
+        {p->h_off, hoff.size() * 4}, {p->h_log, hlog.size() * 4}, {p->h_tab, htab * 4},
+        {p->st_off, (M + 1) * 4}, {p->st_list, static_cast<size_t>(soff[M]) * sizeof(TargetDesc)},
+        {p->diag64, N * 16}};
+    std::vector<uint64_t> buf;
+    for (int a = 0; a < RQX_DIGEST_ARRAYS; ++a) {
+        // FNV-1a over the array's 64-bit words (the tail zero-padded), seeded with its length
+        buf.assign((arrays[a].bytes + 7) / 8, 0ull);
+        if (arrays[a].bytes)
+            HIP_TRY(hipMemcpy(buf.data(), arrays[a].ptr, arrays[a].bytes, hipMemcpyDeviceToHost));
+        uint64_t h = 0xcbf29ce484222325ull ^ arrays[a].bytes;
+        for (uint64_t w : buf) h = (h ^ w) * 0x100000001b3ull;
+        out[a] = h;
+    }
+    return RQX_OK;
+}
+
+int rqx_pack_to_u64(const uint8_t *rows, int64_t n_rows, int32_t row_bytes, int32_t n_qubits, uint64_t *keys,
+                    int64_t *n_real, rqx_stream_t stream) {
+    g_err.clear();
+    if (!n_real) return fail(RQX_ERR_ARG, "n_real is NULL");
+    *n_real = 0;
+    if (n_qubits < 1 || n_qubits > 64) return fail(RQX_ERR_ARG, "n_qubits must be in [1, 64]");
+    if (row_bytes != (n_qubits + 8) / 8) return fail(RQX_ERR_ARG, "row_bytes must be ceil((n_qubits + 1) / 8)");
+    if (n_rows < 0) return fail(RQX_ERR_ARG, "n_rows < 0");
+    if (n_rows == 0) return RQX_OK;
+    if (n_rows > 0x7fffffffLL) return fail(RQX_ERR_LIMIT, "n_rows must be < 2^31");
+    if (!rows || !keys) return fail(RQX_ERR_ARG, "NULL argument");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const int ni = static_cast<int>(n_rows);
+    size_t t_sel = 0;
+    HIP_TRY(hipcub::DeviceSelect::Flagged(nullptr, t_sel, static_cast<const uint64_t *>(nullptr),
+                                          static_cast<const uint8_t *>(nullptr), keys, static_cast<int64_t *>(nullptr),
+                                          ni, st));
+    const size_t n = static_cast<size_t>(n_rows), o_all = align_up(t_sel), o_flag = o_all + align_up(n * 8),
+                 o_num = o_flag + align_up(n);
+    DevBuf b;
+    HIP_TRY(b.alloc(o_num + 8));
+    hipLaunchKernelGGL(k_unpack_rows, dim3(grid_for(n_rows)), dim3(ING_WG), 0, st, rows, n_rows, row_bytes, n_qubits,
+                       b.at<uint64_t>(o_all), b.at<uint8_t>(o_flag));
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess)
+        e = hipcub::DeviceSelect::Flagged(b.p, t_sel, b.at<const uint64_t>(o_all), b.at<const uint8_t>(o_flag), keys,
+                                          b.at<int64_t>(o_num), ni, st);
+    int64_t num = 0;
+    if (e == hipSuccess) e = hipMemcpyAsync(&num, b.at<int64_t>(o_num), 8, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return fail(RQX_ERR_HIP, std::string("pack_to_u64: ") + hipGetErrorString(e));
+    *n_real = num;
+    return RQX_OK;
+}
+
 int rqx_plan_destroy(rqx_plan *plan) {
     g_err.clear();
     if (!plan) return RQX_OK;
@@ -3397,6 +4046,8 @@ int rqx_plan_destroy(rqx_plan *plan) {
     hipError_t e = hipFree(plan->block);
     const hipError_t es = hipFree(plan->st_block);
     if (e == hipSuccess) e = es;
+    const hipError_t eh = hipFree(plan->hash_block);
+    if (e == hipSuccess) e = eh;
     for (HitCache &c : plan->cache) {
         const hipError_t ec = free_cache(c);
         if (e == hipSuccess) e = ec;

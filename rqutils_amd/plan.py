@@ -31,7 +31,10 @@ class Plan:
 
     Args:
         tables: X-grouped terms (``TermTables``).
-        sorted_states: sorted, unique uint64 states (numpy array or CPU tensor).
+        sorted_states: sorted, unique uint64 states: a numpy array or CPU tensor,
+            or an int64 / uint64 tensor on the plan device (e.g. from
+            ``sort_unique_u64``), whose index passes then run on the GPU
+            (``rqx_plan_create_device``; the plan is the same).
         n_total: vector length (>= number of states); rows past the states are
             filler slots that always receive 0 (``sqd.py:473-478``).
         device: CUDA device (default: current).
@@ -42,7 +45,17 @@ class Plan:
         _require_cuda()
         lib = _capi.lib()
         self.tables = tables
-        states = np.ascontiguousarray(np.asarray(sorted_states, dtype=np.uint64).reshape(-1))
+        on_device = isinstance(sorted_states, torch.Tensor) and sorted_states.is_cuda
+        if on_device:
+            if sorted_states.dtype not in (torch.int64, torch.uint64):
+                raise ValueError('device states must be a 64-bit integer tensor')
+            states = sorted_states.reshape(-1).contiguous()
+            if device is None:
+                device = states.device
+            elif torch.device(device).index not in (None, states.device.index):
+                raise ValueError('device states must be on the plan device')
+        else:
+            states = np.ascontiguousarray(np.asarray(sorted_states, dtype=np.uint64).reshape(-1))
         self.n_states = int(states.shape[0])
         self.n_total = int(self.n_states if n_total is None else n_total)
         if self.n_total < self.n_states:
@@ -54,13 +67,14 @@ class Plan:
         zmask = np.ascontiguousarray(tables.zmask, dtype=np.uint64)
         coef = np.ascontiguousarray(tables.coef, dtype=np.complex128).view(np.float64)
         handle = ctypes.c_void_p()
+        create = lib.rqx_plan_create_device if on_device else lib.rqx_plan_create
+        states_ptr = (states.data_ptr() if self.n_states else None) if on_device else _capi.ptr(states)
         with torch.cuda.device(self.device):
             stream = _capi.current_stream(self.device)
-            rc = lib.rqx_plan_create(ctypes.byref(handle), int(tables.num_qubits),
-                                     _capi.ptr(xmask), _capi.ptr(grp_off), int(tables.num_groups),
-                                     _capi.ptr(zmask), _capi.ptr(coef), int(tables.num_terms),
-                                     _capi.ptr(states), self.n_states, self.n_total, int(split_bits),
-                                     stream)
+            rc = create(ctypes.byref(handle), int(tables.num_qubits),
+                        _capi.ptr(xmask), _capi.ptr(grp_off), int(tables.num_groups),
+                        _capi.ptr(zmask), _capi.ptr(coef), int(tables.num_terms),
+                        states_ptr, self.n_states, self.n_total, int(split_bits), stream)
         _capi.check(rc)
         self._handle = handle
         self._states = states
@@ -117,6 +131,15 @@ class Plan:
                                                _DTYPE_CODES[cols.dtype], stream))
         return out.reshape(rhs.shape)
 
+    def digest(self):
+        """Checksums of the device index arrays (``rqx_plan_digest``): a dict of
+        array name -> 64-bit checksum; equal for equal indexes."""
+        names = ('keys', 'lo', 'seg_hi', 'seg_off', 'cls_hi', 'cls_goff', 'tile_row0', 'tile_seg',
+                 'tile_prob', 'prob', 'h_off', 'h_log', 'h_tab', 'st_off', 'st_list', 'diag64')
+        out = (ctypes.c_uint64 * _capi.RQX_DIGEST_ARRAYS)()
+        _capi.check(_capi.lib().rqx_plan_digest(self._handle, out, _capi.current_stream(self.device)))
+        return dict(zip(names, (int(v) for v in out)))
+
     # ------------------------------------------------------------------ lifetime
     def close(self):
         handle = getattr(self, '_handle', None)
@@ -136,6 +159,10 @@ class Plan:
 
     @property
     def states(self):
+        """The sorted unique states as a uint64 numpy array (copied from the device
+        once for a plan built from device states)."""
+        if isinstance(self._states, torch.Tensor):
+            self._states = self._states.cpu().numpy().view(np.uint64)
         return self._states
 
     # ------------------------------------------------------------------ kernels
@@ -482,6 +509,25 @@ def sort_unique_u64(keys: torch.Tensor):
                                                     out.data_ptr() if out.numel() else None,
                                                     ctypes.byref(n_unique), stream))
     return out[:n_unique.value]
+
+
+def pack_to_u64(rows: torch.Tensor, num_qubits: int):
+    """Packed state rows on the device (``uint8 (N, ceil((n + 1) / 8))``, pad bit
+    first, ``sqd.py:255``) -> int64 device tensor of the real rows' uint64 states
+    in input order (rows with the pad bit set are fillers and dropped)."""
+    _require_cuda()
+    if rows.dtype != torch.uint8 or rows.dim() != 2 or not rows.is_cuda:
+        raise ValueError('rows must be a 2-d uint8 device tensor')
+    rows = rows.contiguous()
+    out = torch.empty(rows.shape[0], dtype=torch.int64, device=rows.device)
+    n_real = ctypes.c_int64(0)
+    stream = _capi.current_stream(rows.device)
+    with torch.cuda.device(rows.device):
+        _capi.check(_capi.lib().rqx_pack_to_u64(rows.data_ptr() if rows.numel() else None, rows.shape[0],
+                                                rows.shape[1], int(num_qubits),
+                                                out.data_ptr() if out.numel() else None,
+                                                ctypes.byref(n_real), stream))
+    return out[:n_real.value]
 
 
 def eigenpair_small(mat: torch.Tensor):

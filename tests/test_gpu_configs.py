@@ -13,7 +13,7 @@ from oracle import sqdref
 from rqutils_amd import workloads
 from rqutils_amd.ground_locg import ground_locg
 from rqutils_amd.locg_batch import batch_ground_locg
-from rqutils_amd.plan import BatchPlan, Plan
+from rqutils_amd.plan import BatchPlan, Plan, sort_unique_u64
 
 pytestmark = pytest.mark.gpu
 DEV = 'cuda:0'
@@ -94,6 +94,22 @@ def test_jw25m_cache_identical_to_matrix_free(jw25m):
             assert torch.equal(plan.matvec(x), free[dt]), dt
     finally:
         plan.drop_cache()
+
+
+def test_jw25m_device_build_identical_to_host_build(jw25m):
+    """The plan built from device keys (the device sort/unique output, no host copy)
+    holds the same index as the host build: equal info and array checksums."""
+    w, plan = jw25m
+    _say('jw25m device build == host build')
+    keys = sort_unique_u64(torch.from_numpy(w.states.view(np.int64)).to(DEV))
+    dev = Plan(w.tables, keys)
+    try:
+        assert dev.info == plan.info and dev.problems == plan.problems
+        assert dev.digest() == plan.digest()
+    finally:
+        dev.close()
+        del keys
+        torch.cuda.empty_cache()
 
 
 # ---------------------------------------------------------------- 64 problems of 40 qubits
