@@ -404,6 +404,79 @@ def cpu_baseline(w, dt_name, useful_bytes, gpu_ms, budget_s):
 
 # ---------------------------------------------------------------------------- extras
 
+def setup_leg(w, dup_frac=0.1, seed=1003, host_path=True):
+    """run_sqd's setup from the reference's packed rows (``sqd.py:255`` layout: pad
+    bit first, big endian) to a ready plan, on the device: upload the rows,
+    unpack (``pack_to_u64``), sort/unique, build the index from the device keys.
+    The rows are the workload's states shuffled, with ``dup_frac`` of them
+    repeated (samples repeat); one untimed pass over 100k of them first loads the
+    kernels.  Returns the plan and the stage times; with
+    ``host_path`` also the time of the host route (unpack with numpy, sort/unique
+    on the device, index built from host keys) for comparison."""
+    import numpy as np
+    import torch
+    from rqutils_amd.paulis.symplectic import masks_to_packed, packed_to_masks
+    from rqutils_amd.plan import Plan, pack_to_u64, sort_unique_u64
+    from rqutils_amd.sqd import uniquify_states
+    rng = np.random.default_rng(seed)
+    n = w.states.shape[0]
+    rows_keys = np.concatenate([w.states, rng.choice(w.states, size=int(dup_frac * n))])
+    rng.shuffle(rows_keys)
+    t0 = time.perf_counter()
+    rows = masks_to_packed(rows_keys, w.num_qubits, padded=True)
+    log(f'setup: {rows.shape[0]} packed rows of {rows.shape[1]} bytes (packed in {time.perf_counter() - t0:.1f} s)')
+    del rows_keys
+    dev = torch.device('cuda', torch.cuda.current_device())
+    # the same route once on the first 100k rows: the process's first launch of each
+    # kernel (code object loading) and the allocator's first blocks are not timed
+    warm = Plan(w.tables, sort_unique_u64(pack_to_u64(torch.from_numpy(rows[:100_000]).to(dev), w.num_qubits)),
+                device=dev)
+    warm.close()
+    torch.cuda.synchronize()
+    times = {}
+    t = time.perf_counter()
+    t0 = t
+
+    def mark(name):
+        nonlocal t
+        torch.cuda.synchronize()
+        now = time.perf_counter()
+        times[name] = round(now - t, 4)
+        t = now
+
+    rows_d = torch.from_numpy(rows).to(dev)
+    mark('upload_s')
+    keys = pack_to_u64(rows_d, w.num_qubits)
+    mark('unpack_s')
+    keys_u = sort_unique_u64(keys)
+    mark('sort_unique_s')
+    plan = Plan(w.tables, keys_u, device=dev)
+    mark('index_s')
+    times['total_s'] = round(time.perf_counter() - t0, 4)
+    assert plan.n_states == n
+    del rows_d, keys, keys_u
+    out = {'rows': int(rows.shape[0]), 'row_bytes': int(rows.shape[1]), 'n_unique': n, 'device_path': times}
+    log(f'setup (device path): {times}')
+    if host_path:
+        t0 = time.perf_counter()
+        hk = packed_to_masks(rows, w.num_qubits, padded=True)
+        t_unpack = time.perf_counter() - t0
+        hu = uniquify_states(hk)
+        t_uniq = time.perf_counter() - t0 - t_unpack
+        hp = Plan(w.tables, hu, device=dev)
+        torch.cuda.synchronize()
+        total = time.perf_counter() - t0
+        same = hp.digest() == plan.digest()
+        hp.close()
+        del hk, hu
+        out['host_path'] = {'unpack_s': round(t_unpack, 4), 'sort_unique_s': round(t_uniq, 4),
+                            'index_s': round(total - t_unpack - t_uniq, 4), 'total_s': round(total, 4),
+                            'same_index': same}
+        log(f"setup (host path): {out['host_path']}")
+    torch.cuda.empty_cache()
+    return plan, out
+
+
 def extra_workload(name, dt, steps, warmup, locg=True, matrix_free_locg=True):
     """Another workload's matvec legs and LOCG times (ground energies against the oracle where it has one)."""
     import torch
@@ -513,6 +586,9 @@ def main():
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--cpu-budget', type=float, default=20.0)
     ap.add_argument('--split-bits', type=int, default=-1)
+    ap.add_argument('--no-setup', action='store_true',
+                    help='build the plan from host states instead of timing the setup from packed rows')
+    ap.add_argument('--no-host-setup', action='store_true', help='skip timing the host setup route')
     ap.add_argument('--no-locg', action='store_true')
     ap.add_argument('--locg-maxiter', type=int, default=1000)
     ap.add_argument('--energy-tol', type=float, default=1e-9)
@@ -552,10 +628,15 @@ def main():
     w = load_workload(args.workload)
     log(f'workload {w.name}: N={w.states.shape[0]} J={w.tables.num_groups} T={w.tables.num_terms} '
         f'(generated in {time.perf_counter() - t0:.1f} s)')
-    t0 = time.perf_counter()
-    plan = Plan(w.tables, w.states, split_bits=args.split_bits)
-    torch.cuda.synchronize()
-    t_setup = time.perf_counter() - t0
+    setup = None
+    if args.split_bits < 0 and not args.no_setup:
+        plan, setup = setup_leg(w, host_path=not args.no_host_setup)
+        t_setup = setup['device_path']['total_s']
+    else:
+        t0 = time.perf_counter()
+        plan = Plan(w.tables, w.states, split_bits=args.split_bits)
+        torch.cuda.synchronize()
+        t_setup = time.perf_counter() - t0
     log(f'plan: {plan.info} (setup {t_setup:.2f} s)')
     leg = matvec_leg(plan, dt, args.steps, args.warmup)
 
@@ -587,6 +668,8 @@ def main():
         'roofline': line['roofline'],
         'matrix_free': line['matrix_free'],
     }
+    if setup is not None:
+        result['setup'] = setup
     if isinstance(traffic, dict):
         result['traffic'] = traffic
     if not args.no_locg:

@@ -1819,9 +1819,9 @@ template <typename V>
 __device__ inline V gather_staged(const V *__restrict__ x, const V *xs, int32_t lo, int32_t len, int32_t c) {
     const uint32_t o = static_cast<uint32_t>(c - lo);
     const bool in = o < static_cast<uint32_t>(len);
-    const V a = xs[in ? o : 0u];
-    const V b = x[in ? lo : c];
-    return in ? a : b;
+    V a = xs[in ? o : 0u];
+    if (!in) a = x[c];   // a global load for the lanes outside the window only
+    return a;
 }
 
 template <typename T> __device__ inline T stream_load(const T *p) { return __builtin_nontemporal_load(p); }
@@ -3233,8 +3233,9 @@ int dev_level_stats(DevIngest &dv) {
 }
 
 // split L's segments from the device states (their starts by a device select),
-// then the classes and the cost model on the host, as build_segments does
-int dev_segmentation(const DevIngest &dv, int L, const ProblemIn &q, bool has_diag, Segmentation &sg) {
+// then (cost) the classes and the cost model on the host, as build_segments does
+int dev_segmentation(const DevIngest &dv, int L, const ProblemIn &q, bool has_diag, Segmentation &sg,
+                     bool cost = true) {
     const int64_t M = dv.n_seg[L];
     if (M < 1 || M > dv.n) return fail(RQX_ERR_HIP, "device ingest: bad segment count");
     const int ni = static_cast<int>(dv.n);
@@ -3263,8 +3264,10 @@ int dev_segmentation(const DevIngest &dv, int L, const ProblemIn &q, bool has_di
     ING_TRY(hipMemcpyAsync(&num, b.at<int64_t>(o_num), 8, hipMemcpyDeviceToHost, dv.s));
     ING_TRY(hipStreamSynchronize(dv.s));
     if (num != M) return fail(RQX_ERR_HIP, "device ingest: segment count mismatch");
-    segment_classes(sg, q.xmask, q.n_groups);
-    sg.cost = estimate_cost(sg, has_diag);
+    if (cost) {
+        segment_classes(sg, q.xmask, q.n_groups);
+        sg.cost = estimate_cost(sg, has_diag);
+    }
     return RQX_OK;
 }
 
@@ -3285,12 +3288,29 @@ int dev_choose_split(const DevIngest &dv, const ProblemIn &q, int32_t split_bits
         const int mid = (lo_l + hi_l + 1) / 2;
         if (dv.max_seg[mid] <= SEG_CAP) lo_l = mid; else hi_l = mid - 1;
     }
+    // every candidate's segments from the device, then the classes and the cost
+    // model of each on its own host thread (as choose_split); the first cheapest
+    // (largest L) wins
     const int n_cand = lo_l - std::max(0, lo_l - 8) + 1;
-    for (int c = 0; c < n_cand; ++c) {   // the first cheapest (largest L) wins
-        Segmentation sg;
-        if (int rc = dev_segmentation(dv, lo_l - c, q, has_diag, sg)) return rc;
-        if (c == 0 || sg.cost < best.cost) best = std::move(sg);
+    std::vector<Segmentation> cand(static_cast<size_t>(n_cand));
+    for (int c = 0; c < n_cand; ++c)
+        if (int rc = dev_segmentation(dv, lo_l - c, q, has_diag, cand[c], false)) return rc;
+    auto eval = [&](int c) {
+        segment_classes(cand[c], q.xmask, q.n_groups);
+        cand[c].cost = estimate_cost(cand[c], has_diag);
+    };
+    if (index_threads(q.n_states) > 1) {
+        std::vector<std::thread> pool;
+        for (int c = 1; c < n_cand; ++c) pool.emplace_back(eval, c);
+        eval(0);
+        for (std::thread &t : pool) t.join();
+    } else {
+        for (int c = 0; c < n_cand; ++c) eval(c);
     }
+    int bi = 0;
+    for (int c = 1; c < n_cand; ++c)
+        if (cand[c].cost < cand[bi].cost) bi = c;
+    best = std::move(cand[bi]);
     return RQX_OK;
 }
 

@@ -163,6 +163,12 @@ def masks_to_packed(masks, num_qubits, padded):
     nbytes = (width + 7) // 8
     shape = masks.shape
     flat = masks.reshape(-1)
+    if nbytes <= 8:
+        # one big-endian word per row: the row's bits start at the word's bit 8 nbytes - 1
+        qmask = np.uint64((1 << num_qubits) - 1)
+        words = ((flat & qmask) << np.uint64(8 * nbytes - width)).astype('>u8')
+        packed = words.view(np.uint8).reshape(-1, 8)[:, 8 - nbytes:]
+        return np.ascontiguousarray(packed).reshape(shape + (nbytes,))
     q = np.arange(num_qubits - 1, -1, -1, dtype=np.uint64)
     bits = ((flat[:, None] >> q[None, :]) & np.uint64(1)).astype(np.uint8)
     if padded:
@@ -177,8 +183,18 @@ def packed_to_masks(packed, num_qubits, padded):
     """Inverse of ``masks_to_packed`` (pad bits are discarded)."""
     packed = np.asarray(packed, dtype=np.uint8)
     shape = packed.shape[:-1]
-    bits = np.unpackbits(packed.reshape(-1, packed.shape[-1]), axis=1)
     start = 1 if padded else 0
+    nbytes = packed.shape[-1]
+    if nbytes <= 8 and 8 * nbytes >= start + num_qubits:
+        # one big-endian word per row; the key is the num_qubits bits after the pad bit
+        rows = packed.reshape(-1, nbytes)
+        buf = np.zeros((rows.shape[0], 8), dtype=np.uint8)
+        buf[:, 8 - nbytes:] = rows
+        words = buf.view('>u8').reshape(-1).astype(np.uint64)
+        shift = np.uint64(8 * nbytes - start - num_qubits)
+        qmask = np.uint64((1 << num_qubits) - 1)
+        return ((words >> shift) & qmask).reshape(shape)
+    bits = np.unpackbits(packed.reshape(-1, packed.shape[-1]), axis=1)
     if bits.shape[1] < start + num_qubits:
         raise ValueError('packed rows are too short for num_qubits')
     bits = bits[:, start:start + num_qubits].astype(np.uint64)

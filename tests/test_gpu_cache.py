@@ -261,3 +261,39 @@ def test_section_split_follows_the_product_structure(cases):
     tables, states = cases['complex12']
     plan = Plan(tables, states)
     assert plan.problems[0]['section_bits'] == plan.problems[0]['split_bits']
+
+
+def test_cached_matvec_graph_replays_match_eager():
+    """The cached matvec (both cache precisions built in one pass) captured in a HIP
+    graph: every replay, with eager work in between, equals the eager result and the
+    matrix-free kernel bit for bit."""
+    _say('cached matvec graph replays')
+    w = workloads.jw32(n_alpha=200, n_beta=150)
+    plan = Plan(w.tables, w.states)
+    plan.build_cache(0)
+    assert plan.cache_info(64)['a_levels'] > 0 and plan.cache_info(32)['a_levels'] > 0
+    rng = np.random.default_rng(41)
+    for dt in (torch.complex64, torch.complex128):
+        x = torch.from_numpy(rng.standard_normal(plan.n_total) + 1j * rng.standard_normal(plan.n_total)).to(DEV).to(dt)
+        want = plan.matvec(x)
+        plan.use_cache(False)
+        assert torch.equal(plan.matvec(x), want)      # the matrix-free kernel, bit for bit
+        plan.use_cache(True)
+        y = torch.empty_like(x)
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):
+            plan.matvec(x, out=y)
+        torch.cuda.current_stream().wait_stream(side)
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            plan.matvec(x, out=y)
+        for r in range(4):
+            y.zero_()
+            g.replay()
+            torch.cuda.synchronize()
+            assert torch.equal(y, want), (dt, r)
+            other = plan.matvec(2 * x)             # eager work between the replays
+            torch.cuda.synchronize()
+            assert torch.equal(other, plan.matvec(2 * x))
+    plan.close()
