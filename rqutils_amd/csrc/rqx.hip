@@ -353,23 +353,29 @@ __device__ inline uint32_t xcd_remap(uint32_t b, uint32_t nb) {
 constexpr int HASH_RANK_BITS = 13;     // segments hold at most 8192 rows
 constexpr uint32_t HASH_EMPTY = 0xFFFFFFFFu;
 constexpr int HASH_MAX_KEY_BITS = 18;
+static_assert(HASH_MAX_KEY_BITS + HASH_RANK_BITS < 32, "an empty entry (all ones) never matches a key");
+static_assert(HASH_MAX_KEY_BITS <= 24, "keys are 24-bit multiply operands");
 
-__host__ __device__ inline uint32_t hash_b1(uint32_t k, uint32_t lg) { return (k * 0x9E3779B1u) >> (32u - lg); }
+// Multiplicative hashes of a key (at most 18 bits) with 24-bit operands: full-rate
+// 24-bit multiplies on the GPU, where a 32-bit multiply issues at a quarter of the rate.
+__host__ __device__ inline uint32_t hash_b1(uint32_t k, uint32_t lg) {
+    return ((k & 0xFFFFFFu) * 0x9E3779u) >> (32u - lg);
+}
 __host__ __device__ inline uint32_t hash_b2(uint32_t k, uint32_t lg) {
-    return ((k ^ 0x5BD1E995u) * 0x85EBCA6Bu) >> (32u - lg);
+    return (((k ^ 0x5BD1E9u) & 0xFFFFFFu) * 0x85EBCBu) >> (32u - lg);
 }
 
-// look t up in the staged table; on a hit, rank = its position in the segment
+// look t up in the staged table; on a hit, rank = its position in the segment.
+// entry ^ (t << HASH_RANK_BITS) is the entry's rank when the entry holds t and at
+// least 2^HASH_RANK_BITS otherwise (empty entries too: keys have at most 18 bits),
+// so the smallest of the four is the answer.
 __device__ inline bool hash_find(const uint32_t *tab, uint32_t lg, uint32_t t, uint32_t &rank) {
     const uint2 a = reinterpret_cast<const uint2 *>(tab)[hash_b1(t, lg)];
     const uint2 b = reinterpret_cast<const uint2 *>(tab)[hash_b2(t, lg)];
-    constexpr uint32_t RM = (1u << HASH_RANK_BITS) - 1u;
-    bool f = false;
-    rank = 0;
-    if ((a.x >> HASH_RANK_BITS) == t) { f = true; rank = a.x & RM; }
-    if ((a.y >> HASH_RANK_BITS) == t) { f = true; rank = a.y & RM; }
-    if ((b.x >> HASH_RANK_BITS) == t) { f = true; rank = b.x & RM; }
-    if ((b.y >> HASH_RANK_BITS) == t) { f = true; rank = b.y & RM; }
+    const uint32_t tk = t << HASH_RANK_BITS;
+    const uint32_t m = min(min(a.x ^ tk, a.y ^ tk), min(b.x ^ tk, b.y ^ tk));
+    const bool f = m < (1u << HASH_RANK_BITS);
+    rank = f ? m : 0u;
     return f;
 }
 
@@ -497,6 +503,13 @@ __device__ inline uint32_t tab_index_packed(uint32_t xpos, uint64_t s) {
     return idx;
 }
 
+// the entry of s's pattern in group d's sign table: pattern u of table t sits at
+// u * n_tab + t (n_tab < 2^24, so a full-rate 24-bit multiply gives it)
+constexpr int64_t TAB_MAX_TABLES = int64_t(1) << 24;
+__device__ inline uint32_t tab_entry(const GroupDesc &d, uint64_t s, uint32_t n_tab) {
+    return (tab_index_packed(d.xpos, s) & 0xFFu) * (n_tab & 0xFFFFFFu) + static_cast<uint32_t>(d.tab);
+}
+
 // the rule never picks term-parallel for groups shorter than this (8 K > 30
 // at one hit); plans without such groups launch matvec kernels compiled with
 // the row-parallel path only (TP = false), which give the same sums
@@ -522,7 +535,7 @@ __device__ inline void group_value_wave(const void *coef_v, const KParams &p, co
     }
     if (d.tab >= 0) {
         if (hit) {
-            const uint32_t u = tab_index_packed(d.xpos, s) * p.n_tab + static_cast<uint32_t>(d.tab);
+            const uint32_t u = tab_entry(d, s, p.n_tab);
             const bool neg = __popcll(s & d.z0) & 1;
             const R re = static_cast<const R *>(p.tab_re)[u];
             vx = neg ? -re : re;
@@ -694,7 +707,7 @@ __device__ inline void group_value_wave_dual(const KParams &p, const GroupDesc &
     lo = make_float2(0.f, 0.f);
     if (d.tab >= 0) {
         if (hit) {
-            const uint32_t u = tab_index_packed(d.xpos, s) * p.n_tab + static_cast<uint32_t>(d.tab);
+            const uint32_t u = tab_entry(d, s, p.n_tab);
             const bool neg = __popcll(s & d.z0) & 1;
             const double hre = static_cast<const double *>(p.tab_re)[u];
             const float lre = p.tab_lo_re[u];
@@ -3593,7 +3606,7 @@ int build_plan(rqx_plan **out, const std::vector<ProblemIn> &in, int64_t n_total
             uint64_t zor = 0;
             for (int64_t t = q.grp_off[j]; t < q.grp_off[j + 1]; ++t) zor |= q.zmask[t];
             g_zor.push_back(zor);
-            if (has_sign_table(q, j) && n_tab < (int64_t(1) << 27)) {   // entries indexed in 32 bits
+            if (has_sign_table(q, j) && n_tab < TAB_MAX_TABLES) {
                 p->long_groups = 1;         // the full evaluator reads the tables
                 g_tab.push_back(static_cast<int32_t>(n_tab));
                 n_tab += 1;
