@@ -227,6 +227,7 @@ struct KParams {
     const uint32_t *seg_off;
     const uint64_t *cls_hi;
     const uint32_t *cls_goff;
+    const uint32_t *cls_lane;      // per class: 1 when every group lane_evaluable takes (the matvec's fast loop)
     const uint32_t *st_off;        // per segment: its first (class, target segment) pair
     const uint32_t *h_off;         // hashed segments (NULL: binary search): first table entry,
     const uint32_t *h_log;         //   log2 of the bucket count,
@@ -291,6 +292,7 @@ struct KParams {
     int64_t coo_capacity;
     int64_t n_total;
     uint32_t n_tiles;
+    uint32_t hq_off;               // matvec: byte offset of the per-wave hit queues in the dynamic LDS
 };
 
 template <typename R> struct C2;
@@ -602,6 +604,59 @@ __device__ inline void group_value_wave(const void *coef_v, const KParams &p, ui
     }
 }
 
+// Matvec hit queues (one per wave, in the dynamic LDS after the staged target):
+// the hits of the groups a lane can evaluate alone are queued -- group, row lane,
+// rank in the target -- and evaluated 64 at a time, one per lane (a hit rate of a
+// few percent leaves most lanes idle when a wave evaluates one group at a time);
+// each row then adds its own hits' terms in group order.  HQ_CAP entries per
+// wave (flushed above HQ_CAP - 64), HQ_LANE per row.
+constexpr int HQ_CAP = 128, HQ_LANE = 8;
+template <typename R, bool CPLX, bool REALC> struct HQLayout {
+    using V = typename VT<R, CPLX>::T;
+    static constexpr size_t meta = 0;                                  // uint2: group, lane << 16 | rank
+    static constexpr size_t vx = meta + HQ_CAP * 8;                    // R: the hit's coefficient sum (real part)
+    static constexpr size_t vy = vx + HQ_CAP * sizeof(R);              // R: imaginary part (complex coefficients)
+    static constexpr size_t xv = vy + (REALC ? 0 : HQ_CAP * sizeof(R));// V: x at the hit's column
+    static constexpr size_t srow = xv + HQ_CAP * sizeof(V);            // uint64: the wave's row states
+    static constexpr size_t lq = srow + 64 * 8;                        // uint8: each lane's queued entries
+    static constexpr size_t bytes = lq + 64 * HQ_LANE;                 // per wave
+};
+constexpr size_t HQ_LDS_MAX = (TILE / 64) * HQLayout<double, true, false>::bytes;   // the largest, per workgroup
+
+// C_g(s) of one (row, group) pair evaluated by one lane, for the groups whose
+// sum group_value_wave never splits over the wave: a sign-table group, or one of
+// fewer than TP_MIN_TERMS terms (summed in term order, as group_coef_range does,
+// with per-lane loads since the lanes of a wave evaluate different groups).
+// Equal, bit for bit, to what group_value_wave gives that row.
+__device__ inline bool lane_evaluable(const GroupDesc &d) {
+    return d.tab >= 0 || d.k1 - d.k0 < static_cast<uint32_t>(TP_MIN_TERMS);
+}
+
+template <typename R, bool RE>
+__device__ inline void lane_group_value(const KParams &p, const GroupDesc &d, uint64_t s, R &vx, R &vy) {
+    using CC = typename C2<R>::T;
+    vx = R(0);
+    vy = R(0);
+    if (d.tab >= 0) {
+        const uint32_t u = tab_entry(d, s, p.n_tab);
+        const bool neg = __popcll(s & d.z0) & 1;
+        const R re = static_cast<const R *>(p.tab_re)[u];
+        vx = neg ? -re : re;
+        if constexpr (!RE) {
+            const R im = static_cast<const R *>(p.tab_im)[u];
+            vy = neg ? -im : im;
+        }
+        return;
+    }
+    const CC *coef = static_cast<const CC *>(p.coef);
+    for (uint32_t k = d.k0; k < d.k1; ++k) {
+        const CC c = coef[k];
+        const bool odd = __popcll(s & p.z[k]) & 1;
+        vx += odd ? -c.x : c.x;
+        if constexpr (!RE) vy += odd ? -c.y : c.y;
+    }
+}
+
 // the same in FP64 and FP32 at once (the cache fill of both precisions): each
 // precision summed exactly as group_value_wave sums it
 template <bool RE>
@@ -779,6 +834,16 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
     const uint64_t s = active ? p.keys[i] : 0ull;
     const uint32_t slo = static_cast<uint32_t>(s) & pd.lomask;
     const V *x = static_cast<const V *>(xin);
+    // matvec with the full evaluator: the wave's hit queue
+    constexpr bool HQ = MODE == MODE_MATVEC && TP;
+    using HL = HQLayout<R, CPLX, REALC>;
+    const int lane = tid & 63;
+    char *hq = nullptr;
+    uint32_t hq_n = 0, my_n = 0;   // queued entries of the wave (uniform) and of this lane
+    if constexpr (HQ) {
+        hq = reinterpret_cast<char *>(s_keys) + p.hq_off + static_cast<size_t>(tid >> 6) * HL::bytes;
+        reinterpret_cast<uint64_t *>(hq + HL::srow)[lane] = s;
+    }
 
     V acc = vzero(static_cast<V *>(nullptr));       // diagonal + section-B groups
     V acc_a = vzero(static_cast<V *>(nullptr));     // section-A groups
@@ -953,6 +1018,58 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
                     }
                 }
             };
+            // evaluate the queued hits, one per lane, then let each row add its hits
+            // (entries of a row are queued in group order); empties the queue
+            auto hq_flush = [&]() {
+                if constexpr (HQ) {
+                    if (hq_n == 0) return;
+                    for (uint32_t e0 = 0; e0 < hq_n; e0 += 64) {
+                        const uint32_t e = e0 + static_cast<uint32_t>(lane);
+                        if (e < hq_n) {
+                            const uint2 mt = reinterpret_cast<const uint2 *>(hq + HL::meta)[e];
+                            const uint64_t sl = reinterpret_cast<const uint64_t *>(hq + HL::srow)[mt.y >> 16];
+                            R vx, vy;
+                            lane_group_value<R, REALC>(p, p.gdesc[mt.x], sl, vx, vy);
+                            reinterpret_cast<R *>(hq + HL::vx)[e] = vx;
+                            if constexpr (!REALC) reinterpret_cast<R *>(hq + HL::vy)[e] = vy;
+                            reinterpret_cast<V *>(hq + HL::xv)[e] = x[ta + (mt.y & 0xFFFFu)];
+                        }
+                    }
+                    for (uint32_t k = 0; __ballot(k < my_n) != 0; ++k) {
+                        if (k < my_n) {
+                            const uint32_t e = reinterpret_cast<const uint8_t *>(hq + HL::lq)[lane * HQ_LANE + k];
+                            const V xv = reinterpret_cast<const V *>(hq + HL::xv)[e];
+                            if constexpr (REALC) {
+                                fma_acc(acc, reinterpret_cast<const R *>(hq + HL::vx)[e], xv);
+                            } else {
+                                CC c;
+                                c.x = reinterpret_cast<const R *>(hq + HL::vx)[e];
+                                c.y = reinterpret_cast<const R *>(hq + HL::vy)[e];
+                                fma_acc(acc, c, xv);
+                            }
+                        }
+                    }
+                    hq_n = 0;
+                    my_n = 0;
+                }
+            };
+            // queue this group's hits (a group the lanes evaluate alone)
+            auto hq_push = [&](uint32_t g, uint32_t base, bool hit) {
+                if constexpr (HQ) {
+                    const uint64_t m = __ballot(hit);
+                    if (m == 0) return;
+                    if (hit) {
+                        const uint32_t pos = hq_n + __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(m >> 32),
+                                                                              __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0u));
+                        reinterpret_cast<uint2 *>(hq + HL::meta)[pos] =
+                            make_uint2(g, (static_cast<uint32_t>(lane) << 16) | base);
+                        reinterpret_cast<uint8_t *>(hq + HL::lq)[lane * HQ_LANE + my_n] = static_cast<uint8_t>(pos);
+                        ++my_n;
+                    }
+                    hq_n += static_cast<uint32_t>(__popcll(m));
+                    if (hq_n > static_cast<uint32_t>(HQ_CAP - 64) || __ballot(my_n == HQ_LANE) != 0) hq_flush();
+                }
+            };
             auto handle = [&](uint32_t g, uint32_t base, bool found, bool sec_a) {
                 const bool hit = active && found;
                 if constexpr (MODE == MODE_MATVEC) {
@@ -1017,6 +1134,45 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
             }
             // four independent searches in lock step: the dependent LDS loads of
             // one search overlap with those of the other three
+            auto search4 = [&](uint32_t t0, uint32_t t1, uint32_t t2, uint32_t t3, uint32_t &b0, uint32_t &b1,
+                               uint32_t &b2, uint32_t &b3, bool &f0, bool &f1, bool &f2, bool &f3) {
+                b0 = b1 = b2 = b3 = 0;
+                if (hashed) {
+                    f0 = hash_find(s_keys, lg, t0, b0);
+                    f1 = hash_find(s_keys, lg, t1, b1);
+                    f2 = hash_find(s_keys, lg, t2, b2);
+                    f3 = hash_find(s_keys, lg, t3, b3);
+                } else {
+                    for (uint32_t len = tn; len > 1;) {
+                        const uint32_t half = len >> 1;
+                        b0 = (s_keys[b0 + half] <= t0) ? b0 + half : b0;
+                        b1 = (s_keys[b1 + half] <= t1) ? b1 + half : b1;
+                        b2 = (s_keys[b2 + half] <= t2) ? b2 + half : b2;
+                        b3 = (s_keys[b3 + half] <= t3) ? b3 + half : b3;
+                        len -= half;
+                    }
+                    f0 = s_keys[b0] == t0;
+                    f1 = s_keys[b1] == t1;
+                    f2 = s_keys[b2] == t2;
+                    f3 = s_keys[b3] == t3;
+                }
+            };
+            // the matvec's loop for a class whose groups the lanes all evaluate
+            // alone: the searches need only the low X masks, every hit is queued
+            if constexpr (HQ) {
+                if (ldc(p.cls_lane + td.cls) != 0) {
+                    for (; g + 4 <= g1; g += 4) {
+                        uint32_t b0, b1, b2, b3;
+                        bool f0, f1, f2, f3;
+                        search4(slo ^ ldc(p.g_xlo + g), slo ^ ldc(p.g_xlo + g + 1), slo ^ ldc(p.g_xlo + g + 2),
+                                slo ^ ldc(p.g_xlo + g + 3), b0, b1, b2, b3, f0, f1, f2, f3);
+                        hq_push(g, b0, active && f0);
+                        hq_push(g + 1, b1, active && f1);
+                        hq_push(g + 2, b2, active && f2);
+                        hq_push(g + 3, b3, active && f3);
+                    }
+                }
+            }
             for (; g + 4 <= g1; g += 4) {
                 // the full evaluator fetches the four groups' descriptors at once,
                 // ahead of the searches (their X masks give the targets)
@@ -1038,28 +1194,27 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
                     t2 = slo ^ ldc(p.g_xlo + g + 2);
                     t3 = slo ^ ldc(p.g_xlo + g + 3);
                 }
-                uint32_t b0 = 0, b1 = 0, b2 = 0, b3 = 0;
+                uint32_t b0, b1, b2, b3;
                 bool f0, f1, f2, f3;
-                if (hashed) {
-                    f0 = hash_find(s_keys, lg, t0, b0);
-                    f1 = hash_find(s_keys, lg, t1, b1);
-                    f2 = hash_find(s_keys, lg, t2, b2);
-                    f3 = hash_find(s_keys, lg, t3, b3);
-                } else {
-                    for (uint32_t len = tn; len > 1;) {
-                        const uint32_t half = len >> 1;
-                        b0 = (s_keys[b0 + half] <= t0) ? b0 + half : b0;
-                        b1 = (s_keys[b1 + half] <= t1) ? b1 + half : b1;
-                        b2 = (s_keys[b2 + half] <= t2) ? b2 + half : b2;
-                        b3 = (s_keys[b3 + half] <= t3) ? b3 + half : b3;
-                        len -= half;
-                    }
-                    f0 = s_keys[b0] == t0;
-                    f1 = s_keys[b1] == t1;
-                    f2 = s_keys[b2] == t2;
-                    f3 = s_keys[b3] == t3;
-                }
-                if constexpr (ILP) {   // (measured: slower for single-term groups)
+                search4(t0, t1, t2, t3, b0, b1, b2, b3, f0, f1, f2, f3);
+                if constexpr (HQ) {
+                    // lane-evaluable groups into the queue; a group whose sum the wave
+                    // splits (long, no sign table) after the queue is flushed, in
+                    // group order
+                    const bool h0 = active && f0, h1 = active && f1, h2 = active && f2, h3 = active && f3;
+                    auto one = [&](const GroupDesc &d, uint32_t gg, uint32_t b, bool h) {
+                        if (lane_evaluable(d)) {
+                            hq_push(gg, b, h);
+                        } else {
+                            hq_flush();
+                            consume(gg, b, h, false, value(d, b, h));
+                        }
+                    };
+                    one(d0, g, b0, h0);
+                    one(d1, g + 1, b1, h1);
+                    one(d2, g + 2, b2, h2);
+                    one(d3, g + 3, b3, h3);
+                } else if constexpr (ILP) {   // (measured: slower for single-term groups)
                     const bool h0 = active && f0, h1 = active && f1, h2 = active && f2, h3 = active && f3;
                     const HitVal v0 = value(d0, b0, h0), v1 = value(d1, b1, h1);
                     const HitVal v2 = value(d2, b2, h2), v3 = value(d3, b3, h3);
@@ -1074,6 +1229,7 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
                     handle(g + 3, b3, f3, false);
                 }
             }
+            hq_flush();               // the queued hits precede the remaining groups
             for (; g < g1; ++g) {
                 uint32_t base;
                 const bool f = search(slo ^ ldc(p.g_xlo + g), base);
@@ -2493,6 +2649,7 @@ struct rqx_plan {
     uint32_t *seg_off = nullptr;
     uint64_t *cls_hi = nullptr;
     uint32_t *cls_goff = nullptr;
+    uint32_t *cls_lane = nullptr;   // per class: every group a lane evaluates alone (sign table or short)
     uint32_t *g_xlo = nullptr;
     uint32_t *g_toff = nullptr;
     uint64_t *g_x = nullptr;
@@ -2706,6 +2863,7 @@ KParams make_params(const rqx_plan *p, bool fp64) {
     k.seg_off = p->seg_off;
     k.cls_hi = p->cls_hi;
     k.cls_goff = p->cls_goff;
+    k.cls_lane = p->cls_lane;
     k.st_off = p->st_off;
     k.h_off = p->hashed ? p->h_off : nullptr;
     k.h_log = p->h_log;
@@ -2745,9 +2903,14 @@ int check_plan(const rqx_plan *p) {
 }
 
 template <int MODE, typename R, bool CPLX, bool REALC = false, bool TP = true>
-hipError_t launch_tiles(const rqx_plan *p, const KParams &k, const void *x, void *y, hipStream_t st) {
+hipError_t launch_tiles(const rqx_plan *p, const KParams &k_in, const void *x, void *y, hipStream_t st) {
     if (p->n_tiles == 0) return hipSuccess;
-    const size_t lds = seg_lds_bytes(p);
+    KParams k = k_in;
+    size_t lds = seg_lds_bytes(p);
+    if (MODE == MODE_MATVEC && TP) {   // the matvec's hit queues after the staged target
+        k.hq_off = static_cast<uint32_t>((lds + 15) & ~size_t(15));
+        lds = k.hq_off + (TILE / 64) * HQLayout<R, CPLX, REALC>::bytes;
+    }
     if (p->persistent) {
         hipLaunchKernelGGL(k_fill<unsigned int>, dim3(1), dim3(64), 0, st, p->queue, int64_t(0), int64_t(1), 0u);
         int bpc = p->blocks_per_cu;
@@ -3526,6 +3689,7 @@ int build_plan(rqx_plan **out, const std::vector<ProblemIn> &in, int64_t n_total
     p->n_groups = static_cast<int32_t>(J);
 
     // host arrays of the whole plan
+    std::vector<uint32_t> cls_lane_all;   // per class: every (off-diagonal) group lane-evaluable
     std::vector<uint32_t> lo(dev ? 0 : static_cast<size_t>(N)), seg_off_all, cls_goff_all, g_xlo, g_toff, tile_row0, tile_seg,
         tile_prob;
     std::vector<uint64_t> seg_hi_all, cls_hi_all, g_x, zall;   // (keys are uploaded from the caller's arrays)
@@ -3627,6 +3791,14 @@ int build_plan(rqx_plan **out, const std::vector<ProblemIn> &in, int64_t n_total
             }
             gdesc.push_back(gd);
         }
+        for (int32_t c = 0; c < ph.n_classes; ++c) {
+            uint32_t all = 1;
+            for (uint32_t j = sg.cls_goff[c]; j < sg.cls_goff[c + 1] && all; ++j)
+                if (q.xmask[j] != 0 && g_tab[static_cast<size_t>(grp_base) + j] < 0 &&
+                    q.grp_off[j + 1] - q.grp_off[j] >= TP_MIN_TERMS)
+                    all = 0;
+            cls_lane_all.push_back(all);
+        }
         for (int64_t t = 0; t < q.n_terms; ++t) {
             zall.push_back(q.zmask[t]);
             c64.push_back(q.coef[2 * t]);
@@ -3677,7 +3849,8 @@ int build_plan(rqx_plan **out, const std::vector<ProblemIn> &in, int64_t n_total
     for (const ProbHost &ph : p->probs) hash_fits = hash_fits && ph.L <= HASH_MAX_KEY_BITS;
     int lds_max = 65536;
     HIP_TRY(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, p->device));
-    const size_t hash_max_words = static_cast<size_t>(std::max(lds_max - 256, 0)) / 4;
+    // the matvec's hit queues share the workgroup's LDS with the staged table
+    const size_t hash_max_words = static_cast<size_t>(std::max(lds_max - 256 - static_cast<int>(HQ_LDS_MAX), 0)) / 4;
     if (dev) {
         // built after the upload, from the device low keys
     } else if (hash_fits && !seg_off_all.empty() &&
@@ -3705,7 +3878,8 @@ int build_plan(rqx_plan **out, const std::vector<ProblemIn> &in, int64_t n_total
     const size_t n = static_cast<size_t>(N);
     const size_t o_keys = carve(n * 8), o_lo = carve(n * 4), o_seghi = carve(seg_hi_all.size() * 8),
                  o_segoff = carve(seg_off_all.size() * 4), o_clshi = carve(cls_hi_all.size() * 8),
-                 o_clsgoff = carve(cls_goff_all.size() * 4), o_gxlo = carve(g_xlo.size() * 4),
+                 o_clsgoff = carve(cls_goff_all.size() * 4), o_clslane = carve(cls_lane_all.size() * 4),
+                 o_gxlo = carve(g_xlo.size() * 4),
                  o_gtoff = carve(g_toff.size() * 4), o_gx = carve(g_x.size() * 8),
                  o_z = carve(zall.size() * 8), o_c64 = carve(c64.size() * 8),
                  o_c32 = carve(c32.size() * 8), o_d64 = carve(n * 16),
@@ -3730,6 +3904,7 @@ int build_plan(rqx_plan **out, const std::vector<ProblemIn> &in, int64_t n_total
     p->seg_off = reinterpret_cast<uint32_t *>(b + o_segoff);
     p->cls_hi = reinterpret_cast<uint64_t *>(b + o_clshi);
     p->cls_goff = reinterpret_cast<uint32_t *>(b + o_clsgoff);
+    p->cls_lane = reinterpret_cast<uint32_t *>(b + o_clslane);
     p->g_xlo = reinterpret_cast<uint32_t *>(b + o_gxlo);
     p->g_toff = reinterpret_cast<uint32_t *>(b + o_gtoff);
     p->g_x = reinterpret_cast<uint64_t *>(b + o_gx);
@@ -3771,6 +3946,7 @@ int build_plan(rqx_plan **out, const std::vector<ProblemIn> &in, int64_t n_total
         up(p->lo, lo.data(), lo.size() * 4),
         up(p->seg_hi, seg_hi_all.data(), seg_hi_all.size() * 8), up(p->seg_off, seg_off_all.data(), seg_off_all.size() * 4),
         up(p->cls_hi, cls_hi_all.data(), cls_hi_all.size() * 8), up(p->cls_goff, cls_goff_all.data(), cls_goff_all.size() * 4),
+        up(p->cls_lane, cls_lane_all.data(), cls_lane_all.size() * 4),
         up(p->g_xlo, g_xlo.data(), g_xlo.size() * 4), up(p->g_toff, g_toff.data(), g_toff.size() * 4),
         up(p->g_x, g_x.data(), g_x.size() * 8), up(p->z, zall.data(), zall.size() * 8),
         up(p->coef64, c64.data(), c64.size() * 8), up(p->coef32, c32.data(), c32.size() * 8),
@@ -3824,7 +4000,8 @@ int build_plan(rqx_plan **out, const std::vector<ProblemIn> &in, int64_t n_total
     if (e == hipSuccess) {
         int occ = 0;
         e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &occ, reinterpret_cast<const void *>(&k_tiles_queue<MODE_MATVEC, double, true>), TILE, seg_lds_bytes(raw));
+            &occ, reinterpret_cast<const void *>(&k_tiles_queue<MODE_MATVEC, double, true>), TILE,
+            ((seg_lds_bytes(raw) + 15) & ~size_t(15)) + HQ_LDS_MAX);
         p->occupancy = std::max(occ, 1);
     }
     if (e != hipSuccess) {
