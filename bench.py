@@ -201,21 +201,62 @@ def x0_onehot(plan, dtype):
     return x0
 
 
-def graph_mixed(plan, x0, maxiter, energy_tol, e_ref, check_every=5):
-    """Mixed LOCG with the fused per-problem kernels, one HIP graph per step (single problem)."""
+def graph_mixed(plan, x0, maxiter, energy_tol, e_ref, check_every=5, probe_below=1e-4, target=1e-8):
+    """Mixed LOCG with the fused per-problem kernels, one HIP graph per step (single problem).
+
+    Below relative residual ``probe_below`` every poll also takes the FP64 Rayleigh quotient
+    of the current vector (one FP64 matvec, timed with the run): the first probe within
+    ``target`` of ``e_ref`` is the time to target, measured as for the plain run.
+    """
     import torch
     from rqutils_amd.locg_batch import batch_ground_locg
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     theta, _, it, info = batch_ground_locg(plan, x0.clone(), tol=1e-12, maxiter=maxiter, check_every=check_every,
-                                           energy_tol=energy_tol, use_graph=True)
+                                           energy_tol=energy_tol, use_graph=True, energy_probe=probe_below)
     torch.cuda.synchronize()
     t = time.perf_counter() - t0
     err = abs(float(theta[0]) - e_ref) / abs(e_ref)
-    log(f'graph mixed LOCG: rel err {err:.2e} in {t:.3f} s ({it} iterations, switch at {info["switch_iteration"]})')
-    return {'time_s': round(t, 4), 'iterations': it, 'switch_iteration': info['switch_iteration'],
+    hit = next(((ts, k, float(th[0])) for ts, k, th, _ in info['probes']
+                if abs(float(th[0]) - e_ref) <= target * abs(e_ref)), None)
+    t_target = hit[0] if hit is not None else t
+    log(f'graph mixed LOCG: within {target:g} after {t_target:.3f} s (iteration {hit[1] if hit else it}); '
+        f'own stop: rel err {err:.2e} in {t:.3f} s ({it} iterations, switch at {info["switch_iteration"]})')
+    return {'time_to_target_s': round(t_target, 4), 'target_iteration': hit[1] if hit else it,
+            'target_energy': hit[2] if hit else float(theta[0]), 'probes': len(info['probes']),
+            'time_s': round(t, 4), 'iterations': it, 'switch_iteration': info['switch_iteration'],
             'energy': float(theta[0]), 'rel_err': err, 'stop_energy_tail': energy_tol, 'check_every': check_every,
             'dtype': 'c64 matvec, c128 vectors/RR', 'hip_graph': True}
+
+
+_WARM = []
+
+
+def locg_warmup():
+    """Both solvers once on a small JW problem (seconds: 0.1), so that no timed run pays the
+    process's first use of a torch operator or the capture of its first graph (torch
+    loads its GPU code objects lazily; a first call of one operator costs up to 0.1 s)."""
+    if _WARM:
+        return
+    import torch
+    from rqutils_amd import workloads
+    from rqutils_amd.ground_locg import ground_locg
+    from rqutils_amd.locg_batch import batch_ground_locg
+    from rqutils_amd.locg_mixed import ground_locg_mixed
+    from rqutils_amd.plan import Plan
+    w = workloads.jw32(60, 50)
+    plan = Plan(w.tables, w.states)
+    x0 = x0_onehot(plan, torch.complex128)
+    for build in (64, 0):
+        plan.build_cache(build)
+        ground_locg(plan.matvec, x0.clone(), maxiter=20)
+        batch_ground_locg(plan, x0.clone(), tol=1e-12, maxiter=60, check_every=5, energy_tol=1e-9,
+                          use_graph=True, energy_probe=1.0)
+        plan.drop_cache()
+    ground_locg_mixed(plan.matvec, plan.matvec, x0.clone(), maxiter=40, tol=1e-9, energy_tol=1e-9)
+    torch.cuda.synchronize()
+    plan.close()
+    _WARM.append(True)
 
 
 def locg_legs(plan, e_oracle, maxiter=1000, energy_tol=1e-9, target=1e-8, matrix_free=True):
@@ -224,14 +265,19 @@ def locg_legs(plan, e_oracle, maxiter=1000, energy_tol=1e-9, target=1e-8, matrix
     ``matrix_free``, without them.
 
     The reference energy ``e_ref`` is the oracle's eigsh energy when a fixture exists for the
-    workload, else the plain run's own converged energy.  The plain run's time is the first
-    iteration within ``target`` of ``e_ref``; the mixed runs stop by their own rule (no
-    knowledge of ``e_ref``) and their whole time is counted.
+    workload, else the plain run's own converged energy.  Both times to target are measured
+    the same way: the first moment the run's energy estimate is within ``target`` of
+    ``e_ref`` -- for the plain run its theta (an FP64 Rayleigh quotient every iteration),
+    for the mixed run the FP64 Rayleigh quotient of its vector taken at its polls (the
+    bulk phase's own theta carries the FP32 image's error, so only these probes count;
+    their matvecs are inside the timed run).  The mixed run's own stop (its settle rule,
+    no knowledge of ``e_ref``) is reported beside it.
     """
     import torch
     from rqutils_amd.ground_locg import ground_locg
     from rqutils_amd.locg_mixed import ground_locg_mixed
     out = {}
+    locg_warmup()
     x0 = x0_onehot(plan, torch.complex128)
 
     def plain(label):
@@ -267,10 +313,12 @@ def locg_legs(plan, e_oracle, maxiter=1000, energy_tol=1e-9, target=1e-8, matrix
     plan.build_cache(0)                     # FP32 and FP64 values in one pass
     torch.cuda.synchronize()
     t_build = time.perf_counter() - t0
-    g = graph_mixed(plan, x0, maxiter, energy_tol, e_ref)
+    g = graph_mixed(plan, x0, maxiter, energy_tol, e_ref, target=target)
+    g['time_to_target_s'] = round(g['time_to_target_s'] + t_build, 4)
     g['time_s'] = round(g['time_s'] + t_build, 4)
     g['cache_build_s'] = round(t_build, 4)
-    g['speedup_vs_plain'] = round(t_plain / g['time_s'], 3)
+    g['speedup_vs_plain'] = round(t_plain / g['time_to_target_s'], 3)
+    g['speedup_own_stop_vs_plain_target'] = round(t_plain / g['time_s'], 3)
     out['mixed_graph_cached'] = g
     plan.drop_cache()
     if matrix_free:
@@ -290,8 +338,8 @@ def locg_legs(plan, e_oracle, maxiter=1000, energy_tol=1e-9, target=1e-8, matrix
         out['mixed_matrix_free'] = {'time_s': round(t, 4), 'iterations': res.niter, 'matvecs_low': res.matvecs_lo,
                                     'matvecs_fp64': res.matvecs_hi, 'energy': float(res.theta), 'rel_err': err,
                                     'speedup_vs_plain': round(t_mf / t, 3)}
-        g = graph_mixed(plan, x0, maxiter, energy_tol, e_ref)
-        g['speedup_vs_plain'] = round(t_mf / g['time_s'], 3)
+        g = graph_mixed(plan, x0, maxiter, energy_tol, e_ref, target=target)
+        g['speedup_vs_plain'] = round(t_mf / g['time_to_target_s'], 3)
         out['mixed_graph_matrix_free'] = g
     return out
 

@@ -225,6 +225,32 @@ class BatchLOCG:
         self._update(c)
         self._deflate()
 
+    def probe(self):
+        """FP64 Rayleigh quotient and relative residual of every problem's current x, from one
+        matvec in the vector dtype (the solver's state is unchanged).
+
+        In the bulk phase the step's own theta comes from a low-precision image (its error is
+        ~1e-8..1e-7 relative); this one is exact for the vector held, so it certifies the
+        energy.  With low-precision storage the FP64 ``x`` and ``Ax`` buffers are idle in the
+        bulk phase and hold the widened, normalised ``x`` and its image; otherwise two
+        scratch vectors are used.
+        """
+        if self.lo_storage and self.phase == 'lo':
+            q, aq = self.x, self.ax
+            q.copy_(self.x_lo)
+        else:
+            if not hasattr(self, '_probe_q'):
+                self._probe_q = torch.zeros_like(self.x)
+                self._probe_aq = torch.zeros_like(self.x)
+            q, aq = self._probe_q, self._probe_aq
+            q.copy_(self.x)
+        self._normalize(q, None)
+        self.plan.matvec(q, out=aq)
+        th = self.plan.dots([(q, aq)])[:, 0].real
+        self.plan.combine([aq, q], torch.stack([self.one, -th.to(torch.complex128)], 1), aq)
+        rn = torch.sqrt(self.plan.dots([(aq, aq)])[:, 0].real)
+        return th, rn / torch.maximum(th.abs(), self.scale).clamp_min(1e-300)
+
     def to_high_precision(self):
         """Switch to the vector dtype: x and y widened and re-orthonormalised in FP64, images recomputed."""
         if self.lo_storage and self.phase == 'lo':
@@ -242,15 +268,17 @@ class BatchLOCG:
 
 def batch_ground_locg(plan: Plan, x0, tol=1e-7, switch_tol=1e-6, maxiter=1000, check_every=10,
                       use_graph=True, mixed=True, record=False, energy_tol=None, energy_checks=2,
-                      energy_rel_residual=1e-7, floor_polls=3, lo_storage=True, check_every_hi=1, orth=()):
+                      energy_rel_residual=1e-7, floor_polls=3, lo_storage=True, check_every_hi=1, orth=(),
+                      energy_probe=None, floor_gate=1e-4):
     """Ground eigenpairs of every problem of ``plan`` (a ``BatchPlan``, or a ``Plan`` as one problem).
 
     The bulk phase (low-precision matvec) runs until every active problem's
     relative residual, polled every ``check_every`` steps, is at most
-    ``switch_tol`` or has reached its low-precision floor: its best value has not
-    improved by 1.5x at ``floor_polls`` consecutive polls (a residual 100x above
-    the best so far, a descent after a stall near an excited eigenvector, starts
-    a new best).  The FP64 phase then polls every ``check_every_hi`` steps (its
+    ``switch_tol`` or has reached its low-precision floor: its best value is at
+    most ``floor_gate`` and has not improved by 1.5x at ``floor_polls``
+    consecutive polls (a residual 100x above the best so far, a descent after a
+    stall near an excited eigenvector, starts a new best; a stall above
+    ``floor_gate`` is the iteration's, not the precision's).  The FP64 phase then polls every ``check_every_hi`` steps (its
     start vector is converged to the FP32 floor, so a few steps finish it).
 
     A problem stops (its vectors freeze) at relative residual ``tol``, or, with
@@ -259,7 +287,16 @@ def batch_ground_locg(plan: Plan, x0, tol=1e-7, switch_tol=1e-6, maxiter=1000, c
     below ``energy_tol * |theta|`` at ``energy_checks`` consecutive polls -- and
     its relative residual is at most ``energy_rel_residual`` (near an excited
     eigenvector theta can sit still for tens of iterations with a residual of
-    ~1e-6 before it falls to the ground energy).
+    ~1e-6 before it falls to the ground energy).  Differences of theta at the
+    rounding level count as settled (|decrease| <= ``energy_tol * |theta|``).
+
+    ``energy_probe``: a relative residual below which every bulk-phase poll also
+    computes each problem's FP64 Rayleigh quotient and residual of its current
+    ``x`` (``BatchLOCG.probe``, one FP64 matvec): the bulk phase converges the
+    vector as fast as an FP64 run does, but its own theta carries the low-precision
+    image's error, so these probes are what certify its energy along the way.
+    They are returned in ``info['probes']`` as ``(seconds, iteration, theta64,
+    rel64)`` and do not change the iteration.
 
     ``orth``: vectors over all rows that every problem's solution is kept
     orthogonal to (per problem, its own rows of each vector; deflation as in
@@ -297,6 +334,7 @@ def batch_ground_locg(plan: Plan, x0, tol=1e-7, switch_tol=1e-6, maxiter=1000, c
     switch_at = None
     history = []
     thetas = []
+    probes = []
     calm = torch.zeros(K, dtype=torch.int64)
     best = torch.full((K,), float('inf'), dtype=torch.float64)
     since = torch.zeros(K, dtype=torch.int64)
@@ -318,7 +356,10 @@ def batch_ground_locg(plan: Plan, x0, tol=1e-7, switch_tol=1e-6, maxiter=1000, c
             better = rel < best / 1.5
             since = torch.where(restart | better, torch.zeros_like(since), since + 1)
             best = torch.where(restart | better, rel, best)
-            ready = ~act | (rel <= solver.switch_tol) | (since >= floor_polls)
+            if energy_probe is not None and bool((act & (rel <= energy_probe)).any()):
+                th64, rel64 = solver.probe()
+                probes.append((time.perf_counter() - t0, it, th64.cpu(), rel64.cpu()))
+            ready = ~act | (rel <= solver.switch_tol) | ((since >= floor_polls) & (best <= floor_gate))
             if bool(ready.all()) or it >= maxiter // 2:
                 solver.to_high_precision()
                 solver.active.fill_(True)
@@ -333,8 +374,11 @@ def batch_ground_locg(plan: Plan, x0, tol=1e-7, switch_tol=1e-6, maxiter=1000, c
         if len(thetas) >= 3:
             d0, d1 = thetas[-3] - thetas[-2], thetas[-2] - thetas[-1]
             lim = energy_tol * th.abs()
-            tail = torch.where((d0 > d1) & (d1 >= 0), d1 * d1 / (d0 - d1), torch.full_like(d1, float('inf')))
-            ok = (d1 >= 0) & (d1 <= lim) & (tail <= lim)
+            # a rise at the rounding level is settled too (tail 0); a decrease that is not
+            # shrinking has no geometric tail (inf)
+            tail = torch.where((d0 > d1) & (d1 > 0), d1 * d1 / (d0 - d1).clamp_min(1e-300),
+                               torch.where(d1 <= 0, torch.zeros_like(d1), torch.full_like(d1, float('inf'))))
+            ok = (d1.abs() <= lim) & (tail <= lim)
             calm = torch.where(ok, calm + 1, torch.zeros_like(calm))
             settled = (calm >= energy_checks) & (rel <= energy_rel_residual)
         if bool((settled & act).any()):
@@ -351,5 +395,5 @@ def batch_ground_locg(plan: Plan, x0, tol=1e-7, switch_tol=1e-6, maxiter=1000, c
                       f'after {it} iterations', RuntimeWarning, stacklevel=2)
     info = {'rel_residual': solver.rel.clone(), 'switch_iteration': switch_at, 'iterations': it,
             'seconds': time.perf_counter() - t0, 'history': history, 'graph': use_graph,
-            'converged': converged}
+            'converged': converged, 'probes': probes}
     return solver.theta.clone(), solver.x, it, info
