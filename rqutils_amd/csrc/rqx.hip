@@ -2142,13 +2142,17 @@ __device__ inline uint32_t spmv_section_a(V &acc, const int32_t *c, const CV *va
         int32_t d[U], cd[U], ci[U];
         CV cv[U];
         V xv[U];
+        // codes and one-value levels' values are wave-uniform: scalar loads (every
+        // level has an aval slot); only stored-value levels issue a vector load
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            d[u] = del[k + u];
-            cd[u] = code[k + u];
+            d[u] = ldc(del + k + u);
+            cd[u] = ldc(code + k + u);
+            cv[u] = ldc(aval + k + u);
         }
 #pragma unroll
-        for (int u = 0; u < U; ++u) cv[u] = cd[u] < 0 ? aval[k + u] : stream_load(va + static_cast<int64_t>(cd[u]) * SLICE);
+        for (int u = 0; u < U; ++u)
+            if (cd[u] >= 0) cv[u] = stream_load(va + static_cast<int64_t>(cd[u]) * SLICE);
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             if (d[u] == LEVEL_GATHER) {
@@ -2164,8 +2168,9 @@ __device__ inline uint32_t spmv_section_a(V &acc, const int32_t *c, const CV *va
         for (int u = 0; u < U; ++u) fma_acc(acc, cv[u], xv[u]);
     }
     for (; k < w; ++k) {
-        const int32_t d = del[k], cd = code[k];
-        const CV cv = cd < 0 ? aval[k] : stream_load(va + static_cast<int64_t>(cd) * SLICE);
+        const int32_t d = ldc(del + k), cd = ldc(code + k);
+        CV cv = ldc(aval + k);
+        if (cd >= 0) cv = stream_load(va + static_cast<int64_t>(cd) * SLICE);
         int32_t ci = row + d;
         if (d == LEVEL_GATHER) {
             ci = stream_load(c + j * SLICE);
