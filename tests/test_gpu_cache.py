@@ -169,8 +169,11 @@ def test_cache_limit_is_an_error(cases):
 
 
 def test_run_sqd_cache_strategies_identical(cases):
-    """cache_strategy (1, *) runs on the hit cache, (0, *) matrix-free: the same eigenpair bit for bit."""
+    """Every cache_strategy of the reference (``rqutils/sqd.py:363-401``: source cache or not,
+    times no / sign-bit / diagonal cache) gives the same eigenpair bit for bit: (1, *) runs on the
+    hit cache, (0, *) matrix-free.  The energy is the oracle's dense eigenvalue."""
     _say('cache run_sqd strategies')
+    from oracle import dense
     from rqutils_amd.paulis.symplectic import PauliSumXZ, masks_to_packed
     from rqutils_amd.sqd import SQDOptions, run_sqd
     rng = np.random.default_rng(21)
@@ -180,13 +183,16 @@ def test_run_sqd_cache_strategies_identical(cases):
     keys = workloads.random_subspace(11, 700, rng)
     h = PauliSumXZ.from_paulisum((paulis, coeffs), force_real=True, add_padding=True)
     states_p = masks_to_packed(keys, 11, padded=True)
+    strategies = ((0, 0), (0, 1), (0, 2), (1, 0), (1, 1), (1, 2))
     out = {}
-    for cs in ((0, 0), (1, 0), (1, 2)):
+    for cs in strategies:
         out[cs] = run_sqd(h, states_p, 720, options=SQDOptions(cache_strategy=cs, locg_maxiter=2000))
-    for cs in ((1, 0), (1, 2)):
-        assert float(out[cs][0]) == float(out[(0, 0)][0])
-        assert torch.equal(out[cs][1], out[(0, 0)][1])
-    assert out[(1, 0)][3] == 700
+    for cs in strategies[1:]:
+        assert float(out[cs][0]) == float(out[(0, 0)][0]), cs
+        assert torch.equal(out[cs][1], out[(0, 0)][1]), cs
+        assert out[cs][3] == 700
+    e_ref = np.linalg.eigvalsh(dense.projected_matrix(paulis, coeffs, keys))[0]
+    assert abs(float(out[(0, 0)][0]) - e_ref) <= 1e-8 * abs(e_ref), (float(out[(0, 0)][0]), e_ref)
 
 
 @pytest.mark.parametrize('name', ['complex12', 'real_jw', 'no_diagonal'])

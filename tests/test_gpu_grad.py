@@ -148,6 +148,25 @@ def test_grad_identities_at_36q_10kt_1m_shape():
     assert abs(lhs - rhs) < 1e-9 * (abs(rhs) + torch.linalg.vector_norm(gy).item())
     # grad_x of Re<gy, H x> is H gy: <H gy, x> = <gy, H x>
     assert abs(torch.vdot(plan.matvec(gy), x).real.item() - rhs) < 1e-9 * (abs(rhs) + 1)
+    # central finite differences (step 1e-6, FP64) of L(alpha) = Re <gy, H(alpha) x> at full
+    # size, for the 6 largest gradient entries and 6 drawn at random
+    gh = g.cpu().numpy()
+    ks = list(np.argsort(-np.abs(gh))[:6]) + list(rng.choice(len(gh), size=6, replace=False))
+    step = 1e-6
+    worst = 0.0
+    for k in ks:
+        val = []
+        for sgn in (1.0, -1.0):
+            a = alpha.clone()
+            a[k] += sgn * step
+            plan.set_alphas(a)
+            val.append(torch.vdot(gy, plan.matvec(x)).real.item())
+        fd = (val[0] - val[1]) / (2 * step)
+        err = abs(fd - gh[k]) / max(abs(gh[k]), 1e-3 * np.abs(gh).max())
+        worst = max(worst, err)
+    plan.set_alphas(alpha)
+    print(f'[case] fd worst relative difference {worst:.2e} over {len(ks)} terms', flush=True)
+    assert worst < 1e-6, worst
 
 
 def test_hellmann_feynman_jw_ground_state():
