@@ -1761,11 +1761,32 @@ struct LPass {
     const double *coef;      // problem k: coef + k * stride, [o][q] complex
     double *part;            // per-block partial dots: (k * gridDim.x + block) * 2 LP_DOT doubles
     int32_t stride;          // doubles of per-problem work space
-    int8_t da[LP_DOT], db[LP_DOT];   // dot operands: 0..NIN-1 inputs, NIN + o outputs
 };
 
-template <typename V, typename LV, int NIN, int NOUT, int NDOT, bool LO_IN, bool LO_OUT>
+// The dot products of a pass, as a compile-time list: pair d is operands
+// P_d / 16 and P_d % 16 (0..NIN-1 the inputs, NIN + o output o), conj(first) * second.
+template <int... P> struct Dots {
+    static constexpr int n = sizeof...(P);
+    static constexpr int pairs[n > 0 ? n : 1] = {P...};
+};
+
+// re, im as the vector type V stores them: FP32 rounding, no imaginary part for
+// real vectors (the dots of a pass use the values its outputs hold)
+__device__ inline void as_stored(double &, double &im, const double *) { im = 0.0; }
+__device__ inline void as_stored(double &re, double &im, const float *) {
+    re = static_cast<double>(static_cast<float>(re));
+    im = 0.0;
+}
+__device__ inline void as_stored(double &, double &, const double2 *) {}
+__device__ inline void as_stored(double &re, double &im, const float2 *) {
+    re = static_cast<double>(static_cast<float>(re));
+    im = static_cast<double>(static_cast<float>(im));
+}
+
+template <typename V, typename LV, int NIN, int NOUT, typename DL, bool LO_IN, bool LO_OUT>
 __global__ __launch_bounds__(256) void k_locg_pass(LPass a, const int64_t *__restrict__ rows) {
+    constexpr int NDOT = DL::n;
+    static_assert(NDOT <= LP_DOT, "dots per pass");
     const int k = blockIdx.y;
     const int64_t r0 = rows[k], r1 = rows[k + 1];
     const double *cf = a.coef + static_cast<int64_t>(k) * a.stride;
@@ -1799,28 +1820,16 @@ __global__ __launch_bounds__(256) void k_locg_pass(LPass a, const int64_t *__res
             if (LO_OUT && o == NOUT - 1) {
                 store_c(static_cast<LV *>(a.out[o])[i], sr, si);
             } else {
+                as_stored(sr, si, static_cast<const V *>(nullptr));
                 store_c(static_cast<V *>(a.out[o])[i], sr, si);
-                // what was stored (real vectors drop the imaginary part)
-                load_c(static_cast<V *>(a.out[o])[i], sr, si);
             }
             vr[NIN + o] = sr;
             vi[NIN + o] = si;
         }
 #pragma unroll
         for (int d = 0; d < NDOT; ++d) {
-            // operands picked by uniform selects (no dynamically indexed register arrays)
-            double ar = 0.0, ai = 0.0, br = 0.0, bi = 0.0;
-#pragma unroll
-            for (int j = 0; j < NIN + NOUT; ++j) {
-                if (a.da[d] == j) {
-                    ar = vr[j];
-                    ai = vi[j];
-                }
-                if (a.db[d] == j) {
-                    br = vr[j];
-                    bi = vi[j];
-                }
-            }
+            const int pa = DL::pairs[d] / 16, pb = DL::pairs[d] % 16;
+            const double ar = vr[pa], ai = vi[pa], br = vr[pb], bi = vi[pb];
             dre[d] += ar * br + ai * bi;
             dim_[d] += ar * bi - ai * br;
         }
@@ -4922,15 +4931,15 @@ int rqx_batch_combine(const rqx_plan *p, int32_t n_in, const void *const *v, con
 }  // extern "C"
 
 namespace {
-template <typename V, typename LV, int NIN, int NOUT, int NDOT, bool LO_IN, bool LO_OUT>
+template <typename V, typename LV, int NIN, int NOUT, typename DL, bool LO_IN, bool LO_OUT>
 hipError_t lpass(const rqx_plan *p, const LPass &a, unsigned chunks, hipStream_t st) {
-    hipLaunchKernelGGL((k_locg_pass<V, LV, NIN, NOUT, NDOT, LO_IN, LO_OUT>), dim3(chunks, p->probs.size()), dim3(256),
+    hipLaunchKernelGGL((k_locg_pass<V, LV, NIN, NOUT, DL, LO_IN, LO_OUT>), dim3(chunks, p->probs.size()), dim3(256),
                        0, st, a, p->prob_rows);
     return hipGetLastError();
 }
 
 LPass make_pass(const rqx_plan *p, double *work, int coef_slot, std::initializer_list<const void *> in,
-                std::initializer_list<void *> out, std::initializer_list<std::pair<int, int>> dots) {
+                std::initializer_list<void *> out) {
     LPass a{};
     int q = 0;
     for (const void *v : in) a.in[q++] = v;
@@ -4939,12 +4948,6 @@ LPass make_pass(const rqx_plan *p, double *work, int coef_slot, std::initializer
     a.coef = work + 2 * coef_slot;
     a.part = p->dot_part;
     a.stride = LOCG_WORK;
-    q = 0;
-    for (const auto &d : dots) {
-        a.da[q] = static_cast<int8_t>(d.first);
-        a.db[q] = static_cast<int8_t>(d.second);
-        ++q;
-    }
     return a;
 }
 
@@ -4970,43 +4973,40 @@ int locg_step(const rqx_plan *p, bool low, bool real, void *const *vec, double *
     // every coefficient slot is written by a stage and every dot slot by a reduction before
     // it is read, so the work space needs no clearing (its scale slot is the caller's)
     // 1: theta = <x, Ax>
-    HIP_TRY((lpass<V, LV, 2, 0, 1, false, false>(p, make_pass(p, work, 0, {x, ax}, {}, {{0, 1}}), chunks, st)));
+    HIP_TRY((lpass<V, LV, 2, 0, Dots<0x01>, false, false>(p, make_pass(p, work, 0, {x, ax}, {}), chunks, st)));
     HIP_TRY(scalar(1, LW_D_XAX, 1));
     // 2: r = Ax - theta x (into p); <r, r>, <x, r>, <y, r>
-    HIP_TRY((lpass<V, LV, 3, 1, 3, false, false>(
-        p, make_pass(p, work, LW_C2, {ax, x, y}, {pv}, {{3, 3}, {1, 3}, {2, 3}}), chunks, st)));
+    HIP_TRY((lpass<V, LV, 3, 1, Dots<0x33, 0x13, 0x23>, false, false>(
+        p, make_pass(p, work, LW_C2, {ax, x, y}, {pv}), chunks, st)));
     HIP_TRY(scalar(2, LW_D_RR, 3));
     // 3: p = r - <x,r> x - <y,r> y; <x, p>, <y, p>, <p, p>
-    HIP_TRY((lpass<V, LV, 3, 1, 3, false, false>(
-        p, make_pass(p, work, LW_C3, {pv, x, y}, {pv}, {{1, 3}, {2, 3}, {3, 3}}), chunks, st)));
+    HIP_TRY((lpass<V, LV, 3, 1, Dots<0x13, 0x23, 0x33>, false, false>(
+        p, make_pass(p, work, LW_C3, {pv, x, y}, {pv}), chunks, st)));
     HIP_TRY(scalar(3, LW_D_XP, 3));
     // 4: second Gram-Schmidt pass, normalised (plus the low copy for the bulk matvec); Ap; 5: Rayleigh-Ritz dots
     if (low) {
-        HIP_TRY((lpass<V, LV, 3, 2, 0, false, true>(p, make_pass(p, work, LW_C4, {pv, x, y}, {pv, plo}, {}), chunks, st)));
+        HIP_TRY((lpass<V, LV, 3, 2, Dots<>, false, true>(p, make_pass(p, work, LW_C4, {pv, x, y}, {pv, plo}), chunks,
+                                                         st)));
         if (int rc = rqx_matvec(p, plo, aplo, dtype_of<LV>(), st)) return rc;
-        HIP_TRY((lpass<V, LV, 6, 1, 9, true, false>(
-            p, make_pass(p, work, LW_C5, {x, y, pv, ax, ay, aplo}, {ap},
-                         {{0, 3}, {0, 4}, {0, 6}, {1, 3}, {1, 4}, {1, 6}, {2, 3}, {2, 4}, {2, 6}}),
-            chunks, st)));
+        HIP_TRY((lpass<V, LV, 6, 1, Dots<0x03, 0x04, 0x06, 0x13, 0x14, 0x16, 0x23, 0x24, 0x26>, true, false>(
+            p, make_pass(p, work, LW_C5, {x, y, pv, ax, ay, aplo}, {ap}), chunks, st)));
     } else {
-        HIP_TRY((lpass<V, LV, 3, 1, 0, false, false>(p, make_pass(p, work, LW_C4, {pv, x, y}, {pv}, {}), chunks, st)));
+        HIP_TRY((lpass<V, LV, 3, 1, Dots<>, false, false>(p, make_pass(p, work, LW_C4, {pv, x, y}, {pv}), chunks, st)));
         if (int rc = rqx_matvec(p, pv, ap, dtype_of<V>(), st)) return rc;
-        HIP_TRY((lpass<V, LV, 6, 0, 9, false, false>(
-            p, make_pass(p, work, 0, {x, y, pv, ax, ay, ap}, {},
-                         {{0, 3}, {0, 4}, {0, 5}, {1, 3}, {1, 4}, {1, 5}, {2, 3}, {2, 4}, {2, 5}}),
-            chunks, st)));
+        HIP_TRY((lpass<V, LV, 6, 0, Dots<0x03, 0x04, 0x05, 0x13, 0x14, 0x15, 0x23, 0x24, 0x25>, false, false>(
+            p, make_pass(p, work, 0, {x, y, pv, ax, ay, ap}, {}), chunks, st)));
     }
     HIP_TRY(scalar(5, LW_D_RR9, 9));
     // 6: y' = c1 y + c2 p, Ay' likewise, x' = c0 x + y', Ax' likewise; <x', x'>, <x', y'>
-    HIP_TRY((lpass<V, LV, 6, 4, 2, false, false>(
-        p, make_pass(p, work, LW_C6, {x, ax, y, ay, pv, ap}, {y, ay, x, ax}, {{8, 8}, {8, 6}}), chunks, st)));
+    HIP_TRY((lpass<V, LV, 6, 4, Dots<0x88, 0x86>, false, false>(
+        p, make_pass(p, work, LW_C6, {x, ax, y, ay, pv, ap}, {y, ay, x, ax}), chunks, st)));
     HIP_TRY(scalar(6, LW_D_XX, 2));
     // 7: x = x' / |x'|, Ax likewise; y = y' - <x, y'> x, Ay likewise; <y, y>
-    HIP_TRY((lpass<V, LV, 4, 4, 1, false, false>(
-        p, make_pass(p, work, LW_C7, {x, ax, y, ay}, {x, ax, y, ay}, {{6, 6}}), chunks, st)));
+    HIP_TRY((lpass<V, LV, 4, 4, Dots<0x66>, false, false>(
+        p, make_pass(p, work, LW_C7, {x, ax, y, ay}, {x, ax, y, ay}), chunks, st)));
     HIP_TRY(scalar(7, LW_D_YY2, 1));
     // 8: y and Ay normalised
-    HIP_TRY((lpass<V, LV, 2, 2, 0, false, false>(p, make_pass(p, work, LW_C8, {y, ay}, {y, ay}, {}), chunks, st)));
+    HIP_TRY((lpass<V, LV, 2, 2, Dots<>, false, false>(p, make_pass(p, work, LW_C8, {y, ay}, {y, ay}), chunks, st)));
     return RQX_OK;
 }
 }  // namespace
