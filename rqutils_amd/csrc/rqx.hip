@@ -948,8 +948,10 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
             const bool hashed = p.h_off != nullptr;          // the same for the whole plan
             const uint32_t lg = td.lg;
             if (hashed) {
-                const uint32_t ho = td.ho, ns = 2u << lg;
-                for (uint32_t k = tid; k < ns; k += TILE) s_keys[k] = p.h_tab[ho + k];
+                // 16-byte pieces: a table has 2 << lg >= 4 words and starts at a multiple of 4
+                const uint32_t ho = td.ho, ns4 = 1u << (lg - 1);
+                const uint4 *src = reinterpret_cast<const uint4 *>(p.h_tab + ho);
+                for (uint32_t k = tid; k < ns4; k += TILE) reinterpret_cast<uint4 *>(s_keys)[k] = src[k];
             } else {
                 for (uint32_t k = tid; k < tn; k += TILE) s_keys[k] = p.lo[ta + k];
             }
