@@ -251,6 +251,14 @@ class BatchLOCG:
         rn = torch.sqrt(self.plan.dots([(aq, aq)])[:, 0].real)
         return th, rn / torch.maximum(th.abs(), self.scale).clamp_min(1e-300)
 
+    def to_low_precision(self):
+        """Back to the bulk precision (after an escape from an excited-state plateau): x, y
+        and their FP64 images rounded into the low-precision storage."""
+        if self.lo_storage:
+            for lo_v, hi_v in ((self.x_lo, self.x), (self.ax_lo, self.ax), (self.y_lo, self.y), (self.ay_lo, self.ay)):
+                lo_v.copy_(hi_v)
+        self.phase = 'lo'
+
     def to_high_precision(self):
         """Switch to the vector dtype: x and y widened and re-orthonormalised in FP64, images recomputed."""
         if self.lo_storage and self.phase == 'lo':
@@ -280,6 +288,11 @@ def batch_ground_locg(plan: Plan, x0, tol=1e-7, switch_tol=1e-6, maxiter=1000, c
     stall near an excited eigenvector, starts a new best; a stall above
     ``floor_gate`` is the iteration's, not the precision's).  The FP64 phase then polls every ``check_every_hi`` steps (its
     start vector is converged to the FP32 floor, so a few steps finish it).
+
+    Near an excited eigenvector the residual can fall to the switch level and then
+    rise again as the ground state's component grows (an excited-state plateau of
+    LOCG): a problem whose FP64-phase residual rises 100x above its best and above
+    ``floor_gate`` sends the solve back to the bulk phase (at most four times).
 
     A problem stops (its vectors freeze) at relative residual ``tol``, or, with
     ``energy_tol``, in the FP64 phase once its theta has settled -- the last
@@ -337,6 +350,8 @@ def batch_ground_locg(plan: Plan, x0, tol=1e-7, switch_tol=1e-6, maxiter=1000, c
     probes = []
     calm = torch.zeros(K, dtype=torch.int64)
     best = torch.full((K,), float('inf'), dtype=torch.float64)
+    best_hi = torch.full((K,), float('inf'), dtype=torch.float64)
+    back = 0
     since = torch.zeros(K, dtype=torch.int64)
     while it < maxiter:
         for _ in range(check_every if switch_at is None else check_every_hi):
@@ -364,6 +379,18 @@ def batch_ground_locg(plan: Plan, x0, tol=1e-7, switch_tol=1e-6, maxiter=1000, c
                 solver.to_high_precision()
                 solver.active.fill_(True)
                 switch_at = it
+            continue
+        best_hi = torch.minimum(best_hi, torch.where(act, rel, torch.full_like(rel, float('inf'))))
+        escaped = act & (rel > 100 * best_hi) & (rel > floor_gate)
+        if mixed and back < 4 and bool(escaped.any()):
+            solver.to_low_precision()
+            back += 1
+            switch_at = None
+            since.zero_()
+            best = rel.clone()
+            best_hi.fill_(float('inf'))
+            thetas.clear()
+            calm.zero_()
             continue
         if energy_tol is None:
             if active == 0:
@@ -393,7 +420,7 @@ def batch_ground_locg(plan: Plan, x0, tol=1e-7, switch_tol=1e-6, maxiter=1000, c
     if not bool(converged.all()):
         warnings.warn(f'batch_ground_locg: {int((~converged).sum())} of {solver.K} problems not converged '
                       f'after {it} iterations', RuntimeWarning, stacklevel=2)
-    info = {'rel_residual': solver.rel.clone(), 'switch_iteration': switch_at, 'iterations': it,
+    info = {'rel_residual': solver.rel.clone(), 'switch_iteration': switch_at, 'back_to_bulk': back, 'iterations': it,
             'seconds': time.perf_counter() - t0, 'history': history, 'graph': use_graph,
             'converged': converged, 'probes': probes}
     return solver.theta.clone(), solver.x, it, info
