@@ -397,6 +397,7 @@ def line_for(w, plan, leg, dt_name, steps, warmup, traffic):
             'padding_bytes_per_matvec': leg['padding_bytes'],
             'stored_GBs': round(leg['stored_bytes'] / (ms * 1e-3) / 1e9, 1),
             'cache_bytes': cinfo['bytes'], 'cache_entries_incl_padding': cinfo['entries'],
+            'cache_info': {k: v for k, v in cinfo.items() if k != 'built'},
             'cache_build_s': round(leg['cache_build_s'], 4), 'identical_to_matrix_free': leg['identical'],
         },
         'roofline': {

@@ -58,19 +58,21 @@ int fail(int code, const std::string &msg) {
 enum { MODE_MATVEC = 0, MODE_LOOKUP = 1, MODE_COUNT = 2, MODE_GRAD = 3, MODE_ROWCOUNT = 4, MODE_FILL = 5, MODE_COO = 6,
        MODE_FILLCOL = 7 };
 
-// hit cache: sliced ELL, 64 rows per slice (one wave), two sections per slice.
-// Section A holds the hits of groups with no flips in the low split bits (the
-// source has the row's own low key in another segment, so the rows of a slice
-// usually all hit the same such groups and their gathers coalesce); section B
-// the other hits.  Entries are stored in packets of PACK consecutive entries of
-// one row, read with 16-byte loads: entry k of row r (k counted over section
-// A, then B; both widths multiples of PACK) sits at
+// hit cache: 64 rows per slice (one wave), two sections per slice.  Section A
+// holds the hits of groups that flip no bit below the problem's section split
+// (in a product subspace: the source is the row's own low string in another
+// high block, so every row of a slice hits the same such groups and the level
+// is a row offset with, mostly, one value); section B the other hits, stored as
+// SELL-8-64 (below).  The column pass first writes every hit into a full layout
+// of packets of PACK consecutive entries of one row: entry k of row r (k
+// counted over section A, then B; both widths multiples of PACK) at
 //   off[s] + (k / PACK) 64 PACK + (q / e) 64 e + (r % 64) e + q % e,
 // q = k % PACK, s = r / 64, e = the entries of the array's type in 16 bytes (4
 // for columns and FP32 values, 2 for FP64 or complex FP32, 1 for complex FP64),
-// so each 16-byte load of a wave reads 1 KB contiguously.  Rows are padded to
-// the slice's widths with (r, 0).  A row's y is (diagonal + B hits) + A hits,
-// each part accumulated in group order, in the matrix-free kernel too.
+// so each 16-byte access of a wave covers 1 KB contiguously; the stored layout
+// keeps from it the columns that are not row offsets.  A row's y is (diagonal
+// + B hits) + A hits, each part accumulated in group order, in the matrix-free
+// kernel too.
 constexpr int SLICE = 64;
 constexpr int PACK = 4;
 
