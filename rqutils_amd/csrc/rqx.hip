@@ -366,10 +366,11 @@ __host__ __device__ inline uint32_t hash_b1(uint32_t k, uint32_t lg) {
     return ((k & 0xFFFFFFu) * 0x9E3779u) >> (32u - lg);
 }
 __host__ __device__ inline uint32_t hash_b2(uint32_t k, uint32_t lg) {
-    return (((k ^ 0x5BD1E9u) & 0xFFFFFFu) * 0x85EBCBu) >> (32u - lg);
+    return ((k & 0xFFFFFFu) * 0x85EBCBu) >> (32u - lg);
 }
 
-// look t up in the staged table; on a hit, rank = its position in the segment.
+// look t up in the staged table; on a hit, rank = its position in the segment
+// (on a miss rank is meaningless: every caller uses it only for hits).
 // entry ^ (t << HASH_RANK_BITS) is the entry's rank when the entry holds t and at
 // least 2^HASH_RANK_BITS otherwise (empty entries too: keys have at most 18 bits),
 // so the smallest of the four is the answer.
@@ -378,9 +379,8 @@ __device__ inline bool hash_find(const uint32_t *tab, uint32_t lg, uint32_t t, u
     const uint2 b = reinterpret_cast<const uint2 *>(tab)[hash_b2(t, lg)];
     const uint32_t tk = t << HASH_RANK_BITS;
     const uint32_t m = min(min(a.x ^ tk, a.y ^ tk), min(b.x ^ tk, b.y ^ tk));
-    const bool f = m < (1u << HASH_RANK_BITS);
-    rank = f ? m : 0u;
-    return f;
+    rank = m;
+    return m < (1u << HASH_RANK_BITS);
 }
 
 __device__ inline int find_u64(const uint64_t *__restrict__ a, int n, uint64_t t) {
@@ -606,22 +606,17 @@ __device__ inline void group_value_wave(const void *coef_v, const KParams &p, ui
     }
 }
 
-// Matvec hit queues (one per wave, in the dynamic LDS after the staged target):
-// the hits of the groups a lane can evaluate alone are queued -- group, row lane,
-// rank in the target -- and evaluated 64 at a time, one per lane (a hit rate of a
-// few percent leaves most lanes idle when a wave evaluates one group at a time);
-// each row then adds its own hits' terms in group order.  HQ_CAP entries per
-// wave (flushed above HQ_CAP - 64), HQ_LANE per row.
-constexpr int HQ_CAP = 128, HQ_LANE = 8;
+// Matvec hit lists (one per wave, in the dynamic LDS after the staged target):
+// each lane lists its own hits of the groups a lane can evaluate alone -- group
+// and rank in the target, in group order -- and evaluates them when a lane's
+// list is full or the class ends, one round per list position, adding each to
+// its row in group order.  A hit costs one LDS store where it is found; the
+// per-group work of the wave does not grow with the number of hitting lanes.
+// Entry k of lane l at k * 64 + l (consecutive lanes, consecutive words).
+constexpr int HQ_LANE = 8;
 template <typename R, bool CPLX, bool REALC> struct HQLayout {
-    using V = typename VT<R, CPLX>::T;
-    static constexpr size_t meta = 0;                                  // uint2: group, lane << 16 | rank
-    static constexpr size_t vx = meta + HQ_CAP * 8;                    // R: the hit's coefficient sum (real part)
-    static constexpr size_t vy = vx + HQ_CAP * sizeof(R);              // R: imaginary part (complex coefficients)
-    static constexpr size_t xv = vy + (REALC ? 0 : HQ_CAP * sizeof(R));// V: x at the hit's column
-    static constexpr size_t srow = xv + HQ_CAP * sizeof(V);            // uint64: the wave's row states
-    static constexpr size_t lq = srow + 64 * 8;                        // uint8: each lane's queued entries
-    static constexpr size_t bytes = lq + 64 * HQ_LANE;                 // per wave
+    static constexpr size_t meta = 0;                                  // uint2: group, rank
+    static constexpr size_t bytes = 64 * HQ_LANE * 8;                  // per wave
 };
 constexpr size_t HQ_LDS_MAX = (TILE / 64) * HQLayout<double, true, false>::bytes;   // the largest, per workgroup
 
@@ -840,12 +835,11 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
     constexpr bool HQ = MODE == MODE_MATVEC && TP;
     using HL = HQLayout<R, CPLX, REALC>;
     const int lane = tid & 63;
-    char *hq = nullptr;
-    uint32_t hq_n = 0, my_n = 0;   // queued entries of the wave (uniform) and of this lane
-    if constexpr (HQ) {
-        hq = reinterpret_cast<char *>(s_keys) + p.hq_off + static_cast<size_t>(tid >> 6) * HL::bytes;
-        reinterpret_cast<uint64_t *>(hq + HL::srow)[lane] = s;
-    }
+    uint2 *hq = nullptr;
+    uint32_t my_n = 0;             // listed hits of this lane
+    if constexpr (HQ)
+        hq = reinterpret_cast<uint2 *>(reinterpret_cast<char *>(s_keys) + p.hq_off +
+                                       static_cast<size_t>(tid >> 6) * HL::bytes + HL::meta);
 
     V acc = vzero(static_cast<V *>(nullptr));       // diagonal + section-B groups
     V acc_a = vzero(static_cast<V *>(nullptr));     // section-A groups
@@ -1022,56 +1016,37 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
                     }
                 }
             };
-            // evaluate the queued hits, one per lane, then let each row add its hits
-            // (entries of a row are queued in group order); empties the queue
+            // each lane evaluates its listed hits in order and adds them to its row;
+            // empties the lists
             auto hq_flush = [&]() {
                 if constexpr (HQ) {
-                    if (hq_n == 0) return;
-                    for (uint32_t e0 = 0; e0 < hq_n; e0 += 64) {
-                        const uint32_t e = e0 + static_cast<uint32_t>(lane);
-                        if (e < hq_n) {
-                            const uint2 mt = reinterpret_cast<const uint2 *>(hq + HL::meta)[e];
-                            const uint64_t sl = reinterpret_cast<const uint64_t *>(hq + HL::srow)[mt.y >> 16];
-                            R vx, vy;
-                            lane_group_value<R, REALC>(p, p.gdesc[mt.x], sl, vx, vy);
-                            reinterpret_cast<R *>(hq + HL::vx)[e] = vx;
-                            if constexpr (!REALC) reinterpret_cast<R *>(hq + HL::vy)[e] = vy;
-                            reinterpret_cast<V *>(hq + HL::xv)[e] = x[ta + (mt.y & 0xFFFFu)];
-                        }
-                    }
                     for (uint32_t k = 0; __ballot(k < my_n) != 0; ++k) {
                         if (k < my_n) {
-                            const uint32_t e = reinterpret_cast<const uint8_t *>(hq + HL::lq)[lane * HQ_LANE + k];
-                            const V xv = reinterpret_cast<const V *>(hq + HL::xv)[e];
+                            const uint2 mt = hq[k * 64 + static_cast<uint32_t>(lane)];
+                            const V xv = x[ta + mt.y];
+                            R vx, vy;
+                            lane_group_value<R, REALC>(p, p.gdesc[mt.x], s, vx, vy);
                             if constexpr (REALC) {
-                                fma_acc(acc, reinterpret_cast<const R *>(hq + HL::vx)[e], xv);
+                                fma_acc(acc, vx, xv);
                             } else {
                                 CC c;
-                                c.x = reinterpret_cast<const R *>(hq + HL::vx)[e];
-                                c.y = reinterpret_cast<const R *>(hq + HL::vy)[e];
+                                c.x = vx;
+                                c.y = vy;
                                 fma_acc(acc, c, xv);
                             }
                         }
                     }
-                    hq_n = 0;
                     my_n = 0;
                 }
             };
-            // queue this group's hits (a group the lanes evaluate alone)
+            // list this group's hit (a group the lanes evaluate alone)
             auto hq_push = [&](uint32_t g, uint32_t base, bool hit) {
                 if constexpr (HQ) {
-                    const uint64_t m = __ballot(hit);
-                    if (m == 0) return;
                     if (hit) {
-                        const uint32_t pos = hq_n + __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(m >> 32),
-                                                                              __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0u));
-                        reinterpret_cast<uint2 *>(hq + HL::meta)[pos] =
-                            make_uint2(g, (static_cast<uint32_t>(lane) << 16) | base);
-                        reinterpret_cast<uint8_t *>(hq + HL::lq)[lane * HQ_LANE + my_n] = static_cast<uint8_t>(pos);
+                        hq[my_n * 64 + static_cast<uint32_t>(lane)] = make_uint2(g, base);
                         ++my_n;
                     }
-                    hq_n += static_cast<uint32_t>(__popcll(m));
-                    if (hq_n > static_cast<uint32_t>(HQ_CAP - 64) || __ballot(my_n == HQ_LANE) != 0) hq_flush();
+                    if (__ballot(my_n == HQ_LANE) != 0) hq_flush();
                 }
             };
             auto handle = [&](uint32_t g, uint32_t base, bool found, bool sec_a) {
