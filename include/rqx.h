@@ -82,7 +82,9 @@ typedef struct {
     int32_t section_bits;   /* hit-cache section split: groups whose X mask has no bit below it
                                form the first section (>= split_bits; larger when the states are a
                                product of low and high strings at that bit) */
-    int32_t reserved;
+    int32_t alpha_rows;     /* > 0: the matvec takes the hits of the groups that flip only bits
+                               below section_bits, and whose Z masks hold only such bits, from
+                               per-low-string lists of the product's alpha_rows low strings */
 } rqx_problem_info;
 
 /* Library version (major*10000 + minor*100 + patch). */

@@ -95,7 +95,7 @@ class ProblemInfo(ctypes.Structure):
         ('is_real', ctypes.c_int32),
         ('cost', ctypes.c_double),
         ('section_bits', ctypes.c_int32),
-        ('reserved', ctypes.c_int32),
+        ('alpha_rows', ctypes.c_int32),
     ]
 
     def as_dict(self):

@@ -197,6 +197,10 @@ struct ProbDesc {
     uint32_t lomask;                // low-key mask 2^L - 1 of the problem's split
     int32_t has_diag;
     int32_t a_bits;                 // section split: groups with x & (2^a_bits - 1) == 0 are section A (a_bits >= L)
+    uint32_t row0;                  // the problem's first row
+    uint32_t al_nlo;                // alpha lists: low strings n_lo of the product (rows b * n_lo + a), or 0
+    uint32_t al_a0;                 //   the problem's first alpha index in the list arrays (a multiple of 64)
+    uint32_t pad0;
 };
 
 __device__ inline uint64_t section_mask(int32_t a_bits) { return a_bits >= 64 ? ~0ull : ((1ull << a_bits) - 1ull); }
@@ -211,6 +215,24 @@ struct __attribute__((aligned(32))) GroupDesc {
     int32_t tab;         // its sign table, or -1
     uint32_t xpos;       // with a sign table: the positions of x's bits, 6 bits each from
                          // the lowest, and their count in bits 24..31
+};
+
+// Alpha lists, the matvec of an exact product subspace (rows b * n_lo + a with
+// key(b * n_lo + a) = beta_b << a_bits | alpha_a): the groups that flip only
+// alpha bits and whose terms' Z masks hold only alpha bits hit row i = b * n_lo + a
+// at b * n_lo + a' with a value that depends on alpha_a alone.  Per alpha index a,
+// its hits of those groups in group order -- group, a', and the value in the
+// matvec's precision -- replace the searches of all those groups for every row.
+// Entry k of alpha index a sits at boff[a >> 6] + 64 k + (a & 63): the lanes of a
+// wave (consecutive alpha indices) read consecutive entries.
+template <typename R> struct AlEnt;
+template <> struct __attribute__((aligned(16))) AlEnt<float> {
+    uint32_t g, col;
+    float2 v;
+};
+template <> struct __attribute__((aligned(16))) AlEnt<double> {
+    uint32_t g, col, pad0, pad1;
+    double2 v;
 };
 
 // One (class, target segment) pair of a segment's target list, with what the
@@ -229,7 +251,8 @@ struct KParams {
     const uint32_t *seg_off;
     const uint64_t *cls_hi;
     const uint32_t *cls_goff;
-    const uint32_t *cls_lane;      // per class: 1 when every group lane_evaluable takes (the matvec's fast loop)
+    const uint32_t *cls_lane;      // per class: bit 0 when every group lane_evaluable takes (the matvec's
+                                   //   fast loop), bit 1 when its groups flip only alpha bits (alpha lists)
     const uint32_t *st_off;        // per segment: its first (class, target segment) pair
     const uint32_t *h_off;         // hashed segments (NULL: binary search): first table entry,
     const uint32_t *h_log;         //   log2 of the bucket count,
@@ -295,6 +318,14 @@ struct KParams {
     int64_t n_total;
     uint32_t n_tiles;
     uint32_t hq_off;               // matvec: byte offset of the per-wave hit queues in the dynamic LDS
+    // alpha lists (matvec): entries per alpha index, first entry of each block of
+    // 64 alpha indices, the entries in the matvec's precision (AlEnt<R>), and per
+    // class the groups of an alpha class outside the lists ([al_pg_off[c], al_pg_off[c + 1]))
+    const uint32_t *al_len;
+    const int64_t *al_boff;
+    const void *al_ent;
+    const uint32_t *al_pg_off;
+    const uint32_t *al_pg;
 };
 
 template <typename R> struct C2;
@@ -843,6 +874,48 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
 
     V acc = vzero(static_cast<V *>(nullptr));       // diagonal + section-B groups
     V acc_a = vzero(static_cast<V *>(nullptr));     // section-A groups
+    // alpha lists: this lane's alpha index a (row i = al_rb + a), its entries
+    // [0, al_n) and the next one (group al_g, column al_rb + al_col, value al_v;
+    // al_g = UINT32_MAX when none is left)
+    const bool al_on = HQ && pd.al_nlo != 0;
+    uint32_t al_rb = 0, al_k = 0, al_n = 0, al_g = UINT32_MAX, al_col = 0;
+    int64_t al_pos = 0;
+    CC al_v{};
+    auto al_load = [&]() {
+        if (al_k < al_n) {
+            const AlEnt<R> en = static_cast<const AlEnt<R> *>(p.al_ent)[al_pos + 64 * static_cast<int64_t>(al_k)];
+            al_g = en.g;
+            al_col = en.col;
+            al_v = en.v;
+        } else {
+            al_g = UINT32_MAX;
+        }
+    };
+    if constexpr (HQ) {
+        if (al_on && active) {
+            const uint32_t a = (i - pd.row0) % pd.al_nlo, ag = pd.al_a0 + a;
+            al_rb = i - a;
+            al_n = p.al_len[ag];
+            al_pos = p.al_boff[ag >> 6] + (ag & 63u);
+            al_load();
+        }
+    }
+    // add this lane's listed hits of the groups before gend, in group order (the
+    // next entry's load and this one's x load in flight together)
+    auto al_emit = [&](uint32_t gend) {
+        if constexpr (HQ) {
+            while (__ballot(al_g < gend) != 0) {
+                if (al_g < gend) {
+                    const V xv = x[al_rb + al_col];
+                    const CC v = al_v;
+                    ++al_k;
+                    al_load();
+                    if constexpr (REALC) fma_acc(acc, v.x, xv);
+                    else fma_acc(acc, v, xv);
+                }
+            }
+        }
+    };
     unsigned long long n_hit = 0, n_probe = 0, n_look = 0, n_hit_a = 0;
     using FV = typename std::conditional<CPLX, CC, R>::type;          // fill: stored value
     using FL = typename std::conditional<CPLX, float2, float>::type;  // fill: FP32 value of a dual fill
@@ -943,15 +1016,30 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
             const uint32_t ta = td.ta, tn = td.tn;
             const bool hashed = p.h_off != nullptr;          // the same for the whole plan
             const uint32_t lg = td.lg;
-            if (hashed) {
-                // 16-byte pieces: a table has 2 << lg >= 4 words and starts at a multiple of 4
-                const uint32_t ho = td.ho, ns4 = 1u << (lg - 1);
-                const uint4 *src = reinterpret_cast<const uint4 *>(p.h_tab + ho);
-                for (uint32_t k = tid; k < ns4; k += TILE) reinterpret_cast<uint4 *>(s_keys)[k] = src[k];
-            } else {
-                for (uint32_t k = tid; k < tn; k += TILE) s_keys[k] = p.lo[ta + k];
+            // an alpha class: its listed groups come from the alpha lists, its
+            // other groups [pq0, pq1) of al_pg are searched (the target is staged
+            // only for them)
+            bool al_cls = false;
+            uint32_t pq0 = 0, pq1 = 0;
+            if constexpr (HQ) {
+                if (al_on && (ldc(p.cls_lane + td.cls) & 2u) != 0) {
+                    al_cls = true;
+                    pq0 = ldc(p.al_pg_off + td.cls);
+                    pq1 = ldc(p.al_pg_off + td.cls + 1);
+                }
             }
-            __syncthreads();
+            const bool staged = !al_cls || pq0 < pq1;        // the same for the whole workgroup
+            if (staged) {
+                if (hashed) {
+                    // 16-byte pieces: a table has 2 << lg >= 4 words and starts at a multiple of 4
+                    const uint32_t ho = td.ho, ns4 = 1u << (lg - 1);
+                    const uint4 *src = reinterpret_cast<const uint4 *>(p.h_tab + ho);
+                    for (uint32_t k = tid; k < ns4; k += TILE) reinterpret_cast<uint4 *>(s_keys)[k] = src[k];
+                } else {
+                    for (uint32_t k = tid; k < tn; k += TILE) s_keys[k] = p.lo[ta + k];
+                }
+                __syncthreads();
+            }
             // position of t in the target segment and whether it is there
             auto search = [&](uint32_t t, uint32_t &base) -> bool {
                 if (hashed) return hash_find(s_keys, lg, t, base);
@@ -1100,7 +1188,7 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
             // groups are sorted by X mask, so a class's group without low flips is
             // its first one; it is section A when it also flips no bit below the
             // problem's section split (its bits [L, a_bits) are zero), else section B
-            if (g < g1 && ldc(p.g_xlo + g) == 0) {
+            if (!al_cls && g < g1 && ldc(p.g_xlo + g) == 0) {
                 const bool sec_a = (ldc(p.g_x + g) & section_mask(pd.a_bits)) == 0;
                 uint32_t base;
                 const bool f = search(slo, base);
@@ -1136,10 +1224,63 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
                     f3 = s_keys[b3] == t3;
                 }
             };
+            // an alpha class: its groups outside the lists searched four at a time,
+            // each evaluated where it comes in group order, this lane's listed hits
+            // added before it; the hit queue stays empty
+            if constexpr (HQ) {
+                if (al_cls) {
+                    auto other = [&](const GroupDesc &d, uint32_t gq, uint32_t b, bool f) {
+                        al_emit(gq);
+                        const bool h = active && f;
+                        if (lane_evaluable(d)) {
+                            if (h) {
+                                R vx, vy;
+                                lane_group_value<R, REALC>(p, d, s, vx, vy);
+                                const V xv = x[ta + b];
+                                if constexpr (REALC) {
+                                    fma_acc(acc, vx, xv);
+                                } else {
+                                    CC c;
+                                    c.x = vx;
+                                    c.y = vy;
+                                    fma_acc(acc, c, xv);
+                                }
+                            }
+                        } else {
+                            consume(gq, b, h, false, value(d, b, h));
+                        }
+                    };
+                    uint32_t q = pq0;
+                    for (; q + 4 <= pq1; q += 4) {
+                        const uint32_t gq0 = ldc(p.al_pg + q), gq1 = ldc(p.al_pg + q + 1),
+                                       gq2 = ldc(p.al_pg + q + 2), gq3 = ldc(p.al_pg + q + 3);
+                        const GroupDesc d0 = ldc(p.gdesc + gq0), d1 = ldc(p.gdesc + gq1), d2 = ldc(p.gdesc + gq2),
+                                        d3 = ldc(p.gdesc + gq3);
+                        uint32_t b0, b1, b2, b3;
+                        bool f0, f1, f2, f3;
+                        search4(slo ^ (static_cast<uint32_t>(d0.x) & pd.lomask), slo ^ (static_cast<uint32_t>(d1.x) & pd.lomask),
+                                slo ^ (static_cast<uint32_t>(d2.x) & pd.lomask), slo ^ (static_cast<uint32_t>(d3.x) & pd.lomask),
+                                b0, b1, b2, b3, f0, f1, f2, f3);
+                        other(d0, gq0, b0, f0);
+                        other(d1, gq1, b1, f1);
+                        other(d2, gq2, b2, f2);
+                        other(d3, gq3, b3, f3);
+                    }
+                    for (; q < pq1; ++q) {
+                        const uint32_t gq = ldc(p.al_pg + q);
+                        const GroupDesc d = ldc(p.gdesc + gq);
+                        uint32_t b;
+                        const bool f = search(slo ^ (static_cast<uint32_t>(d.x) & pd.lomask), b);
+                        other(d, gq, b, f);
+                    }
+                    al_emit(g1);
+                    g = g1;
+                }
+            }
             // the matvec's loop for a class whose groups the lanes all evaluate
             // alone: the searches need only the low X masks, every hit is queued
             if constexpr (HQ) {
-                if (ldc(p.cls_lane + td.cls) != 0) {
+                if ((ldc(p.cls_lane + td.cls) & 1u) != 0) {
                     for (; g + 4 <= g1; g += 4) {
                         uint32_t b0, b1, b2, b3;
                         bool f0, f1, f2, f3;
@@ -1214,7 +1355,7 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
                 const bool f = search(slo ^ ldc(p.g_xlo + g), base);
                 handle(g, base, f, false);
             }
-            __syncthreads();          // before the next target overwrites s_keys
+            if (staged) __syncthreads();   // before the next target overwrites s_keys
         }
     }
 
@@ -1412,6 +1553,73 @@ __global__ void k_group_tables(const int32_t *__restrict__ g_tab, const uint64_t
         t32re[o] = f.x;
         t32im[o] = f.y;
     }
+}
+
+// Alpha lists of one problem: workgroup a takes alpha index a (row a of the
+// first beta block, keys[a]) and the problem's listed groups lg[0..nlg) in
+// order: group g hits when alpha_a ^ (x_g & amask) is an alpha string, at its
+// index a' among keys[0..nlo) (ascending low parts).  Count pass (boff NULL):
+// len[a0 + a] = the hits.  Fill pass: entry k of the hits at
+// boff[(a0 + a) >> 6] + 64 k + ((a0 + a) & 63), with C_g(s) as the matvec's lanes
+// evaluate it (lane_group_value) in FP64 and in FP32.
+__global__ __launch_bounds__(256) void k_alpha_lists(const KParams p64, const KParams p32,
+                                                     const uint64_t *__restrict__ keys, uint32_t nlo, uint64_t amask,
+                                                     const uint32_t *__restrict__ lg, uint32_t nlg, uint32_t a0,
+                                                     uint32_t *__restrict__ len, const int64_t *__restrict__ boff,
+                                                     AlEnt<float> *__restrict__ e32, AlEnt<double> *__restrict__ e64) {
+    __shared__ uint32_t wn[4];
+    const uint32_t a = blockIdx.x, tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
+    const uint64_t s = keys[a], sa = s & amask;
+    const uint32_t ag = a0 + a;
+    uint32_t base = 0;
+    for (uint32_t j0 = 0; j0 < nlg; j0 += 256) {
+        const uint32_t j = j0 + tid;
+        bool f = false;
+        uint32_t g = 0, col = 0;
+        if (j < nlg) {
+            g = lg[j];
+            const uint64_t t = sa ^ (p64.g_x[g] & amask);
+            uint32_t lo = 0, n = nlo;          // lower bound of t
+            while (n > 0) {
+                const uint32_t half = n >> 1;
+                if ((keys[lo + half] & amask) < t) {
+                    lo += half + 1;
+                    n -= half + 1;
+                } else {
+                    n = half;
+                }
+            }
+            f = lo < nlo && (keys[lo] & amask) == t;
+            col = lo;
+        }
+        const uint64_t m = __ballot(f);
+        if (lane == 0) wn[w] = static_cast<uint32_t>(__popcll(m));
+        __syncthreads();
+        uint32_t before = base;
+        for (uint32_t u = 0; u < w; ++u) before += wn[u];
+        const uint32_t total = wn[0] + wn[1] + wn[2] + wn[3];
+        if (boff && f) {
+            const uint32_t k = before + static_cast<uint32_t>(__popcll(m & ((1ull << lane) - 1ull)));
+            const int64_t pos = boff[ag >> 6] + 64 * static_cast<int64_t>(k) + (ag & 63u);
+            double hx, hy;
+            float lx, ly;
+            lane_group_value<double, false>(p64, p64.gdesc[g], s, hx, hy);
+            lane_group_value<float, false>(p32, p32.gdesc[g], s, lx, ly);
+            AlEnt<double> eh{};
+            eh.g = g;
+            eh.col = col;
+            eh.v = make_double2(hx, hy);
+            e64[pos] = eh;
+            AlEnt<float> el{};
+            el.g = g;
+            el.col = col;
+            el.v = make_float2(lx, ly);
+            e32[pos] = el;
+        }
+        base += total;
+        __syncthreads();          // wn is rewritten by the next chunk
+    }
+    if (!boff && tid == 0) len[ag] = base;
 }
 
 __global__ void k_narrow(const double2 *__restrict__ in, float2 *__restrict__ out, int64_t n) {
@@ -2535,8 +2743,19 @@ void build_segments(Segmentation &sg, const uint64_t *keys, int64_t n, const uin
     segment_classes(sg, xmask, J);
 }
 
-// Estimated per-thread work of the tile kernel, from a deterministic sample of tiles.
-double estimate_cost(const Segmentation &sg, bool has_diag) {
+// What the cost model needs of a product subspace's alpha lists: the product
+// split (0: none), the groups the lists would hold, and their entries per row
+struct AlphaModel {
+    int32_t a_bits = 0;
+    std::vector<uint8_t> listed;
+    double hits_per_row = 0.0;
+};
+
+// Estimated per-thread work of the tile kernel, from a deterministic sample of
+// tiles.  Below the product split, an alpha class searches only its groups
+// outside the lists (none: no staging either) and every row adds its list
+// entries (a load and a multiply-add each).
+double estimate_cost(const Segmentation &sg, bool has_diag, const AlphaModel &am) {
     const int M = static_cast<int>(sg.seg_hi.size());
     const int C = static_cast<int>(sg.cls_hi.size());
     std::vector<uint32_t> tile_seg;
@@ -2547,6 +2766,11 @@ double estimate_cost(const Segmentation &sg, bool has_diag) {
     if (tile_seg.empty()) return 0.0;
     const size_t n_sample = std::min<size_t>(tile_seg.size(), 128);
     const int lm = ceil_log2(static_cast<uint64_t>(M));
+    const bool lists = am.a_bits > sg.L && !am.listed.empty();
+    std::vector<uint32_t> cls_listed(static_cast<size_t>(C), 0);   // listed groups of each alpha class
+    if (lists)
+        for (int c = 0; c < C && (sg.cls_hi[c] >> (am.a_bits - sg.L)) == 0; ++c)
+            for (uint32_t j = sg.cls_goff[c]; j < sg.cls_goff[c + 1]; ++j) cls_listed[c] += am.listed[j];
     double total = 0.0;
     for (size_t si = 0; si < n_sample; ++si) {
         const uint32_t m = tile_seg[(si * tile_seg.size()) / n_sample];
@@ -2559,8 +2783,13 @@ double estimate_cost(const Segmentation &sg, bool has_diag) {
             const uint32_t tn = sg.seg_off[ts + 1] - sg.seg_off[ts];
             uint32_t groups = sg.cls_goff[c + 1] - sg.cls_goff[c];
             if (has_diag && sg.cls_goff[c] == 0) groups -= 1;
+            if (cls_listed[c] > 0) {
+                groups -= cls_listed[c];
+                if (groups == 0) continue;
+            }
             cost += (tn + TILE - 1) / TILE + 12.0 + groups * (2.0 * ceil_log2(tn) + 4.0);
         }
+        if (lists) cost += am.hits_per_row * 6.0;
         total += cost;
     }
     return total / n_sample * tile_seg.size();
@@ -2580,6 +2809,10 @@ struct ProbHost {            // host record of one problem of a plan
     uint32_t diag_terms = 0;   // terms of the x == 0 group
     int32_t is_real = 1;
     double cost = 0.0;         // cost-model estimate of one matvec
+    int64_t a_rows = 0;        // low strings of the product at the section split (0: no product)
+    uint32_t al_nlo = 0;       // alpha lists: low strings (0: none), first alpha index,
+    uint32_t al_a0 = 0;        //   and the problem's listed groups [al_lg0, al_lg0 + al_lgn) of the plan's list
+    uint32_t al_lg0 = 0, al_lgn = 0;
 };
 
 struct HitCache {            // sliced-ELL cache of the hits (column, C_j(s_i)) in one precision
@@ -2676,6 +2909,19 @@ struct rqx_plan {
     double *tab64im = nullptr;
     float *tab32re = nullptr;
     float *tab32im = nullptr;
+    // alpha lists (own allocations): per alpha index its entries, the listed
+    // groups of every problem, per class the searched groups of an alpha class;
+    // then the first entry of each block of 64 alpha indices and the entries
+    void *al_block = nullptr;
+    void *al_ent_block = nullptr;
+    uint32_t *al_len = nullptr;
+    uint32_t *al_lg = nullptr;
+    uint32_t *al_pg_off = nullptr;
+    uint32_t *al_pg = nullptr;
+    int64_t *al_boff = nullptr;
+    AlEnt<float> *al_e32 = nullptr;
+    AlEnt<double> *al_e64 = nullptr;
+    int64_t al_slots = 0;           // entries per precision, padding included
 };
 
 namespace {
@@ -2707,6 +2953,49 @@ bool has_sign_table(const ProblemIn &q, int32_t j) {
     for (int64_t t = k0 + 1; t < k1; ++t)
         if ((q.zmask[t] ^ q.zmask[k0]) & ~x) return false;
     return true;
+}
+
+// The cost model's view of the alpha lists of q when its states are a product
+// at bit a_bits with n_lo low strings (first: the first n_lo states, ascending
+// low parts): the groups the plan build lists (flip only bits below a_bits, a
+// lane evaluates them alone, Z masks below a_bits) and their hits per row,
+// counted for up to 256 low strings spread over the first block.  Empty when
+// the states are no product, nothing would be listed, or the plan would launch
+// the plain kernels (no sign table, no long group), which do not read lists.
+AlphaModel alpha_model(const ProblemIn &q, const uint64_t *first, int64_t n_lo, int32_t a_bits) {
+    AlphaModel am;
+    if (n_lo <= 0 || a_bits <= 0 || a_bits >= 64) return am;
+    const uint64_t amask = (1ull << a_bits) - 1ull;
+    std::vector<uint8_t> listed(static_cast<size_t>(q.n_groups), 0);
+    std::vector<uint64_t> lx;     // X masks of the listed groups
+    bool tp = false;
+    for (int32_t j = 0; j < q.n_groups; ++j) {
+        const uint64_t x = q.xmask[j];
+        if (x == 0) continue;
+        const bool tab = has_sign_table(q, j);
+        const int64_t k = q.grp_off[j + 1] - q.grp_off[j];
+        tp = tp || tab || k >= TP_MIN_TERMS;
+        if ((x & ~amask) != 0 || !(tab || k < TP_MIN_TERMS)) continue;
+        uint64_t zor = 0;
+        for (int64_t t = q.grp_off[j]; t < q.grp_off[j + 1]; ++t) zor |= q.zmask[t];
+        if ((zor & ~amask) == 0) {
+            listed[static_cast<size_t>(j)] = 1;
+            lx.push_back(x);
+        }
+    }
+    if (!tp || lx.empty()) return am;
+    std::vector<uint64_t> al(static_cast<size_t>(n_lo));
+    for (int64_t i = 0; i < n_lo; ++i) al[static_cast<size_t>(i)] = first[i] & amask;
+    const int64_t ns = std::min<int64_t>(n_lo, 256);
+    int64_t hits = 0;
+    for (int64_t u = 0; u < ns; ++u) {
+        const uint64_t a = al[static_cast<size_t>((u * (n_lo - 1)) / std::max<int64_t>(ns - 1, 1))];
+        for (uint64_t x : lx) hits += std::binary_search(al.begin(), al.end(), a ^ x) ? 1 : 0;
+    }
+    am.a_bits = a_bits;
+    am.listed.swap(listed);
+    am.hits_per_row = static_cast<double>(hits) / static_cast<double>(ns);
+    return am;
 }
 
 // two-choice cuckoo table of one segment's sorted low keys (2^lg buckets of two
@@ -2882,7 +3171,93 @@ KParams make_params(const rqx_plan *p, bool fp64) {
     k.queue = p->queue;
     k.n_total = p->n_total;
     k.n_tiles = static_cast<uint32_t>(p->n_tiles);
+    k.al_len = p->al_len;
+    k.al_boff = p->al_boff;
+    k.al_ent = fp64 ? static_cast<const void *>(p->al_e64) : static_cast<const void *>(p->al_e32);
+    k.al_pg_off = p->al_pg_off;
+    k.al_pg = p->al_pg;
     return k;
+}
+
+// the alpha lists' count pass (fill = false: entries per alpha index) or fill
+// pass (entries and values; also after a coefficient update)
+hipError_t launch_alpha_lists(const rqx_plan *p, bool fill, hipStream_t st) {
+    const KParams p64 = make_params(p, true), p32 = make_params(p, false);
+    for (const ProbHost &ph : p->probs) {
+        if (ph.al_nlo == 0) continue;
+        hipLaunchKernelGGL(k_alpha_lists, dim3(ph.al_nlo), dim3(256), 0, st, p64, p32, p->keys + ph.row_base,
+                           ph.al_nlo, (1ull << ph.a_bits) - 1ull, p->al_lg + ph.al_lg0, ph.al_lgn, ph.al_a0,
+                           p->al_len, fill ? p->al_boff : nullptr, p->al_e32, p->al_e64);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+// at most this many bytes of alpha-list entries (both precisions) per plan; a
+// plan beyond it searches every group
+constexpr int64_t ALPHA_LIST_MAX_BYTES = int64_t(1) << 30;
+
+// Build the alpha lists of the problems that have them (ph.al_nlo > 0): counts,
+// the block layout on the host, then the entries.  Without room for them the
+// problems' descriptors are rewritten without lists.
+hipError_t build_alpha_lists(rqx_plan *p, uint32_t n_alpha, const std::vector<uint32_t> &lg,
+                             const std::vector<uint32_t> &pg_off, const std::vector<uint32_t> &pg,
+                             std::vector<ProbDesc> &desc, hipStream_t st) {
+    if (n_alpha == 0 || lg.empty()) return hipSuccess;
+    // every problem searches all its groups: no lists built, or (plain kernels
+    // only: no sign table and no long group in the plan) none read
+    auto none = [&]() {
+        for (size_t k = 0; k < p->probs.size(); ++k) {
+            p->probs[k].al_nlo = 0;
+            desc[k].al_nlo = 0;
+        }
+        hipError_t e = hipMemcpyAsync(p->prob, desc.data(), desc.size() * sizeof(ProbDesc), hipMemcpyHostToDevice, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);   // desc is the caller's host array
+        return e;
+    };
+    if (!p->long_groups) return none();
+    const size_t o_len = 0, o_lg = align_up(size_t(n_alpha) * 4), o_pgo = o_lg + align_up(lg.size() * 4),
+                 o_pg = o_pgo + align_up(pg_off.size() * 4), bytes = o_pg + align_up(std::max<size_t>(pg.size(), 1) * 4);
+    hipError_t e = hipMalloc(&p->al_block, bytes);
+    if (e != hipSuccess) return e;
+    char *b = static_cast<char *>(p->al_block);
+    p->al_len = reinterpret_cast<uint32_t *>(b + o_len);
+    p->al_lg = reinterpret_cast<uint32_t *>(b + o_lg);
+    p->al_pg_off = reinterpret_cast<uint32_t *>(b + o_pgo);
+    p->al_pg = reinterpret_cast<uint32_t *>(b + o_pg);
+    e = fill_async(p->al_len, int64_t(0), static_cast<int64_t>(n_alpha), 0u, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(p->al_lg, lg.data(), lg.size() * 4, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(p->al_pg_off, pg_off.data(), pg_off.size() * 4, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess && !pg.empty()) e = hipMemcpyAsync(p->al_pg, pg.data(), pg.size() * 4, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = launch_alpha_lists(p, false, st);
+    std::vector<uint32_t> len(n_alpha);
+    if (e == hipSuccess) e = hipMemcpyAsync(len.data(), p->al_len, size_t(n_alpha) * 4, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return e;
+    const uint32_t n_blk = n_alpha / 64;          // each problem's range is padded to a multiple of 64
+    std::vector<int64_t> boff(n_blk);
+    int64_t slots = 0;
+    for (uint32_t k = 0; k < n_blk; ++k) {
+        uint32_t w = 0;
+        for (uint32_t u = 0; u < 64; ++u) w = std::max(w, len[64 * k + u]);
+        boff[k] = slots;
+        slots += 64 * static_cast<int64_t>(w);
+    }
+    const int64_t ent_bytes = slots * static_cast<int64_t>(sizeof(AlEnt<float>) + sizeof(AlEnt<double>));
+    if (slots == 0 || ent_bytes > ALPHA_LIST_MAX_BYTES) return none();   // no hits at all, or no room
+    const size_t o_e32 = align_up(size_t(n_blk) * 8), o_e64 = o_e32 + align_up(size_t(slots) * sizeof(AlEnt<float>));
+    e = hipMalloc(&p->al_ent_block, o_e64 + size_t(slots) * sizeof(AlEnt<double>));
+    if (e != hipSuccess) return e;
+    char *eb = static_cast<char *>(p->al_ent_block);
+    p->al_boff = reinterpret_cast<int64_t *>(eb);
+    p->al_e32 = reinterpret_cast<AlEnt<float> *>(eb + o_e32);
+    p->al_e64 = reinterpret_cast<AlEnt<double> *>(eb + o_e64);
+    p->al_slots = slots;
+    e = hipMemcpyAsync(p->al_boff, boff.data(), boff.size() * 8, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = launch_alpha_lists(p, true, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);   // boff is released on return
+    return e;
 }
 
 int check_plan(const rqx_plan *p) {
@@ -3052,7 +3427,8 @@ int max_segment_par(const std::vector<int8_t> &hd, int L, unsigned nt) {
 // low strings, e.g. beta x alpha determinants) -- else L.  Groups flipping no bit
 // below b then map every row of a block to the same row of another block, so their
 // cache levels are row offsets.
-int32_t section_split(const uint64_t *st, int64_t n, int32_t L, int32_t n_qubits) {
+int32_t section_split(const uint64_t *st, int64_t n, int32_t L, int32_t n_qubits, int64_t &rows_lo) {
+    rows_lo = 0;
     if (n < 2) return L;
     for (int32_t b = n_qubits - 1; b > L; --b) {
         const uint64_t lm = (b >= 64) ? ~0ull : ((1ull << b) - 1ull);
@@ -3071,9 +3447,23 @@ int32_t section_split(const uint64_t *st, int64_t n, int32_t L, int32_t n_qubits
                     break;
                 }
         }
-        if (ok) return b;
+        if (ok) {
+            rows_lo = m;
+            return b;
+        }
     }
     return L;
+}
+
+// the splits the cost model compares: the nine widest that keep segments within
+// SEG_CAP rows (lo_l the widest), then, for a product at a_bits with lists, the
+// four below a_bits (the lists need L < a_bits), in descending order
+std::vector<int> split_candidates(int lo_l, const AlphaModel &am) {
+    std::vector<int> Ls;
+    for (int L = lo_l; L >= std::max(0, lo_l - 8); --L) Ls.push_back(L);
+    if (!am.listed.empty())
+        for (int L = std::min(am.a_bits - 1, lo_l - 9); L >= std::max(0, am.a_bits - 4); --L) Ls.push_back(L);
+    return Ls;
 }
 
 int choose_split(const ProblemIn &q, int32_t split_bits, bool has_diag, Segmentation &best, int k) {
@@ -3085,13 +3475,17 @@ int choose_split(const ProblemIn &q, int32_t split_bits, bool has_diag, Segmenta
     });
     const int Lmax_bits = std::min(32, q.n_qubits);
     const std::string tag = "problem " + std::to_string(k) + ": ";
+    // the product split of the states, for the alpha lists of the splits below it
+    int64_t n_lo = 0;
+    const int32_t a_bits = section_split(q.states, q.n_states, 0, q.n_qubits, n_lo);
+    const AlphaModel am = alpha_model(q, q.states, n_lo, a_bits);
     if (split_bits >= 0) {
         if (split_bits > Lmax_bits) return fail(RQX_ERR_ARG, tag + "split_bits must be <= min(32, n_qubits)");
         if (max_segment_par(hd, split_bits, nt) > SEG_CAP)
             return fail(RQX_ERR_LIMIT, tag + "split_bits gives a segment larger than " + std::to_string(SEG_CAP) + " rows");
         best.L = split_bits;
         build_segments(best, q.states, q.n_states, q.xmask, q.n_groups);
-        best.cost = estimate_cost(best, has_diag);
+        best.cost = estimate_cost(best, has_diag, am);
         return RQX_OK;
     }
     int lo_l = 0, hi_l = Lmax_bits;
@@ -3099,14 +3493,15 @@ int choose_split(const ProblemIn &q, int32_t split_bits, bool has_diag, Segmenta
         const int mid = (lo_l + hi_l + 1) / 2;
         if (max_segment_par(hd, mid, nt) <= SEG_CAP) lo_l = mid; else hi_l = mid - 1;
     }
-    // up to nine candidate splits, each segmented and costed on its own host
-    // thread for large subspaces; the first cheapest (largest L) wins
-    const int n_cand = lo_l - std::max(0, lo_l - 8) + 1;
+    // candidate splits, each segmented and costed on its own host thread for large
+    // subspaces; the first cheapest (largest L) wins
+    const std::vector<int> Ls = split_candidates(lo_l, am);
+    const int n_cand = static_cast<int>(Ls.size());
     std::vector<Segmentation> cand(static_cast<size_t>(n_cand));
     auto eval = [&](int c) {
-        cand[c].L = lo_l - c;
+        cand[c].L = Ls[c];
         build_segments(cand[c], q.states, q.n_states, q.xmask, q.n_groups);
-        cand[c].cost = estimate_cost(cand[c], has_diag);
+        cand[c].cost = estimate_cost(cand[c], has_diag, am);
     };
     if (nt > 1) {
         std::vector<std::thread> pool;
@@ -3404,7 +3799,7 @@ int dev_level_stats(DevIngest &dv) {
 // split L's segments from the device states (their starts by a device select),
 // then (cost) the classes and the cost model on the host, as build_segments does
 int dev_segmentation(const DevIngest &dv, int L, const ProblemIn &q, bool has_diag, Segmentation &sg,
-                     bool cost = true) {
+                     const AlphaModel &am, bool cost = true) {
     const int64_t M = dv.n_seg[L];
     if (M < 1 || M > dv.n) return fail(RQX_ERR_HIP, "device ingest: bad segment count");
     const int ni = static_cast<int>(dv.n);
@@ -3435,10 +3830,12 @@ int dev_segmentation(const DevIngest &dv, int L, const ProblemIn &q, bool has_di
     if (num != M) return fail(RQX_ERR_HIP, "device ingest: segment count mismatch");
     if (cost) {
         segment_classes(sg, q.xmask, q.n_groups);
-        sg.cost = estimate_cost(sg, has_diag);
+        sg.cost = estimate_cost(sg, has_diag, am);
     }
     return RQX_OK;
 }
+
+int dev_section_split(const DevIngest &dv, int32_t L, int32_t n_qubits, int32_t &a_bits, int64_t &rows_lo);
 
 // choose_split over device states: the same search and candidates, with the
 // longest segments of every split from one pass
@@ -3446,11 +3843,25 @@ int dev_choose_split(const DevIngest &dv, const ProblemIn &q, int32_t split_bits
                      int k) {
     const int Lmax_bits = std::min(32, q.n_qubits);
     const std::string tag = "problem " + std::to_string(k) + ": ";
+    // the product split of the states (and its first block, to the host), for
+    // the alpha lists of the splits below it
+    AlphaModel am;
+    {
+        int32_t a_bits = 0;
+        int64_t n_lo = 0;
+        if (int rc = dev_section_split(dv, 0, q.n_qubits, a_bits, n_lo)) return rc;
+        if (n_lo > 0) {
+            std::vector<uint64_t> first(static_cast<size_t>(n_lo));
+            ING_TRY(hipMemcpyAsync(first.data(), dv.st, static_cast<size_t>(n_lo) * 8, hipMemcpyDeviceToHost, dv.s));
+            ING_TRY(hipStreamSynchronize(dv.s));
+            am = alpha_model(q, first.data(), n_lo, a_bits);
+        }
+    }
     if (split_bits >= 0) {
         if (split_bits > Lmax_bits) return fail(RQX_ERR_ARG, tag + "split_bits must be <= min(32, n_qubits)");
         if (dv.max_seg[split_bits] > SEG_CAP)
             return fail(RQX_ERR_LIMIT, tag + "split_bits gives a segment larger than " + std::to_string(SEG_CAP) + " rows");
-        return dev_segmentation(dv, split_bits, q, has_diag, best);
+        return dev_segmentation(dv, split_bits, q, has_diag, best, am);
     }
     int lo_l = 0, hi_l = Lmax_bits;
     while (lo_l < hi_l) {
@@ -3460,13 +3871,14 @@ int dev_choose_split(const DevIngest &dv, const ProblemIn &q, int32_t split_bits
     // every candidate's segments from the device, then the classes and the cost
     // model of each on its own host thread (as choose_split); the first cheapest
     // (largest L) wins
-    const int n_cand = lo_l - std::max(0, lo_l - 8) + 1;
+    const std::vector<int> Ls = split_candidates(lo_l, am);
+    const int n_cand = static_cast<int>(Ls.size());
     std::vector<Segmentation> cand(static_cast<size_t>(n_cand));
     for (int c = 0; c < n_cand; ++c)
-        if (int rc = dev_segmentation(dv, lo_l - c, q, has_diag, cand[c], false)) return rc;
+        if (int rc = dev_segmentation(dv, Ls[c], q, has_diag, cand[c], am, false)) return rc;
     auto eval = [&](int c) {
         segment_classes(cand[c], q.xmask, q.n_groups);
-        cand[c].cost = estimate_cost(cand[c], has_diag);
+        cand[c].cost = estimate_cost(cand[c], has_diag, am);
     };
     if (index_threads(q.n_states) > 1) {
         std::vector<std::thread> pool;
@@ -3485,8 +3897,9 @@ int dev_choose_split(const DevIngest &dv, const ProblemIn &q, int32_t split_bits
 
 // section_split over device states: the candidate bits whose first block
 // divides the rows, checked in one pass (the largest that holds wins)
-int dev_section_split(const DevIngest &dv, int32_t L, int32_t n_qubits, int32_t &a_bits) {
+int dev_section_split(const DevIngest &dv, int32_t L, int32_t n_qubits, int32_t &a_bits, int64_t &rows_lo) {
     a_bits = L;
+    rows_lo = 0;
     if (dv.n < 2) return RQX_OK;
     SplitCands c{};
     for (int32_t b = n_qubits - 1; b > L; --b) {
@@ -3510,6 +3923,7 @@ int dev_section_split(const DevIngest &dv, int32_t L, int32_t n_qubits, int32_t 
     for (int j = 0; j < c.n; ++j)
         if (!((failed >> j) & 1ull)) {
             a_bits = c.b[j];
+            rows_lo = c.m[j];
             break;
         }
     return RQX_OK;
@@ -3694,6 +4108,8 @@ int build_plan(rqx_plan **out, const std::vector<ProblemIn> &in, int64_t n_total
     std::vector<int32_t> g_tab;    // per group: first sign-table entry, or -1
     std::vector<uint64_t> g_zor;   // per group: union of its terms' Z masks
     std::vector<GroupDesc> gdesc;
+    std::vector<uint32_t> al_lg, al_pg, al_pg_off;   // alpha lists: listed groups; per class, its other groups
+    uint32_t al_alpha = 0;                            // alpha indices of all problems (each from a multiple of 64)
     int64_t n_tab = 0;
     int64_t row_base = 0, term_base = 0;
     int32_t grp_base = 0;
@@ -3725,10 +4141,10 @@ int build_plan(rqx_plan **out, const std::vector<ProblemIn> &in, int64_t n_total
             dv.s = st;
             if (int rc = dev_level_stats(dv)) return rc;
             if (int rc = dev_choose_split(dv, q, split_bits, ph.has_diag, sg, k)) return rc;
-            if (int rc = dev_section_split(dv, sg.L, q.n_qubits, ph.a_bits)) return rc;
+            if (int rc = dev_section_split(dv, sg.L, q.n_qubits, ph.a_bits, ph.a_rows)) return rc;
         } else {
             if (int rc = choose_split(q, split_bits, ph.has_diag, sg, k)) return rc;
-            ph.a_bits = section_split(q.states, q.n_states, sg.L, q.n_qubits);
+            ph.a_bits = section_split(q.states, q.n_states, sg.L, q.n_qubits, ph.a_rows);
         }
         ph.L = sg.L;
         ph.max_seg = sg.max_seg;
@@ -3792,6 +4208,41 @@ int build_plan(rqx_plan **out, const std::vector<ProblemIn> &in, int64_t n_total
                     all = 0;
             cls_lane_all.push_back(all);
         }
+        {
+            // alpha lists: in a product at the section split (rows b * n_lo + a) the
+            // classes below 2^(a_bits - L) flip only alpha bits; their groups that a
+            // lane evaluates alone and whose Z masks hold only alpha bits are listed
+            // per alpha index, their other groups searched (al_pg)
+            const bool product = ph.a_bits > sg.L && ph.a_bits < 64 && ph.a_rows > 0 && ph.a_rows < q.n_states;
+            const uint64_t amask = product ? (1ull << ph.a_bits) - 1ull : 0ull;
+            auto listed = [&](uint32_t j) {
+                const size_t gj = static_cast<size_t>(grp_base) + j;
+                return q.xmask[j] != 0 && (g_tab[gj] >= 0 || q.grp_off[j + 1] - q.grp_off[j] < TP_MIN_TERMS) &&
+                       (g_zor[gj] & ~amask) == 0;
+            };
+            int32_t n_acls = 0;
+            const size_t lg0 = al_lg.size();
+            if (product)
+                for (; n_acls < ph.n_classes && (sg.cls_hi[n_acls] >> (ph.a_bits - sg.L)) == 0; ++n_acls)
+                    for (uint32_t j = sg.cls_goff[n_acls]; j < sg.cls_goff[n_acls + 1]; ++j)
+                        if (listed(j)) al_lg.push_back(static_cast<uint32_t>(grp_base) + j);
+            if (al_lg.size() == lg0) {
+                n_acls = 0;
+            } else {
+                ph.al_nlo = static_cast<uint32_t>(ph.a_rows);
+                ph.al_a0 = al_alpha;
+                ph.al_lg0 = static_cast<uint32_t>(lg0);
+                ph.al_lgn = static_cast<uint32_t>(al_lg.size() - lg0);
+                al_alpha += (ph.al_nlo + 63u) & ~63u;
+            }
+            for (int32_t c = 0; c < ph.n_classes; ++c) {
+                al_pg_off.push_back(static_cast<uint32_t>(al_pg.size()));
+                if (c >= n_acls) continue;
+                cls_lane_all[static_cast<size_t>(ph.cls_base) + c] |= 2u;
+                for (uint32_t j = sg.cls_goff[c]; j < sg.cls_goff[c + 1]; ++j)
+                    if (q.xmask[j] != 0 && !listed(j)) al_pg.push_back(static_cast<uint32_t>(grp_base) + j);
+            }
+        }
         for (int64_t t = 0; t < q.n_terms; ++t) {
             zall.push_back(q.zmask[t]);
             c64.push_back(q.coef[2 * t]);
@@ -3802,6 +4253,9 @@ int build_plan(rqx_plan **out, const std::vector<ProblemIn> &in, int64_t n_total
         desc[k] = ProbDesc{static_cast<uint32_t>(ph.seg_base), static_cast<uint32_t>(ph.n_segs),
                            static_cast<uint32_t>(ph.cls_base), static_cast<uint32_t>(ph.n_classes),
                            static_cast<uint32_t>(ph.grp_base), lomask, ph.has_diag, ph.a_bits};
+        desc[k].row0 = static_cast<uint32_t>(row_base);
+        desc[k].al_nlo = ph.al_nlo;
+        desc[k].al_a0 = ph.al_a0;
         p->max_seg = std::max(p->max_seg, ph.max_seg);
         p->is_real = p->is_real && ph.is_real;
         if (ph.max_offdiag_terms >= TP_MIN_TERMS) p->long_groups = 1;
@@ -3814,6 +4268,7 @@ int build_plan(rqx_plan **out, const std::vector<ProblemIn> &in, int64_t n_total
     // segment and class ranges end with their sentinels
     seg_off_all.push_back(static_cast<uint32_t>(N));
     cls_goff_all.push_back(static_cast<uint32_t>(J));
+    al_pg_off.push_back(static_cast<uint32_t>(al_pg.size()));
     g_toff.push_back(static_cast<uint32_t>(T));
     p->n_segs = static_cast<int32_t>(seg_hi_all.size());
     p->n_classes = static_cast<int32_t>(cls_hi_all.size());
@@ -3929,6 +4384,8 @@ int build_plan(rqx_plan **out, const std::vector<ProblemIn> &in, int64_t n_total
         (void)hipFree(raw->block);
         (void)hipFree(raw->st_block);
         (void)hipFree(raw->hash_block);
+        (void)hipFree(raw->al_block);
+        (void)hipFree(raw->al_ent_block);
     };
 
     auto up = [&](void *dst, const void *src, size_t bytes) -> hipError_t {
@@ -3989,6 +4446,7 @@ int build_plan(rqx_plan **out, const std::vector<ProblemIn> &in, int64_t n_total
     launch_group_tables(raw, st);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = build_targets(raw, st);
+    if (e == hipSuccess) e = build_alpha_lists(raw, al_alpha, al_lg, al_pg_off, al_pg, desc, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e == hipSuccess) {
         int occ = 0;
@@ -4251,6 +4709,10 @@ int rqx_plan_destroy(rqx_plan *plan) {
     if (e == hipSuccess) e = es;
     const hipError_t eh = hipFree(plan->hash_block);
     if (e == hipSuccess) e = eh;
+    const hipError_t ea = hipFree(plan->al_block);
+    if (e == hipSuccess) e = ea;
+    const hipError_t ee = hipFree(plan->al_ent_block);
+    if (e == hipSuccess) e = ee;
     for (HitCache &c : plan->cache) {
         const hipError_t ec = free_cache(c);
         if (e == hipSuccess) e = ec;
@@ -4308,6 +4770,7 @@ int rqx_plan_get_problem(const rqx_plan *p, int32_t k, rqx_problem_info *out) {
     out->is_real = ph.is_real;
     out->cost = ph.cost;
     out->section_bits = ph.a_bits;
+    out->alpha_rows = static_cast<int32_t>(ph.al_nlo);
     return RQX_OK;
 }
 
@@ -4505,6 +4968,7 @@ int rqx_plan_update_coef(rqx_plan *p, const double *coef, int32_t is_real, rqx_s
     HIP_TRY(hipGetLastError());
     launch_group_tables(p, st);
     HIP_TRY(hipGetLastError());
+    if (p->al_ent_block) HIP_TRY(launch_alpha_lists(p, true, st));   // values from the new tables
     for (const ProbHost &ph : p->probs) {
         if (ph.n_states == 0) continue;
         const unsigned blocks = static_cast<unsigned>((ph.n_states + 255) / 256);

@@ -122,6 +122,46 @@ def test_lookup_and_matvec_jw_small(jw_small):
     assert np.all(d[keys.shape[0]:] == 0)
 
 
+def test_alpha_lists_product_subspace_match_oracle():
+    """A product subspace (60 alpha x 50 beta strings): the matvec takes the hits of the groups that flip
+    only alpha bits from per-alpha-string lists.  Every dtype equals the oracle, also after complex
+    coefficients replace the real ones (the lists' values are rewritten); one state less is no product
+    and has no lists."""
+    _say('alpha lists')
+    from rqutils_amd import _capi
+    w = workloads.jw32(n_alpha=60, n_beta=50)
+    t, keys = w.tables, w.states
+    plan = Plan(t, keys, split_bits=10)          # segments below the alpha bits (a tiny problem's default is wider)
+    assert plan.problems[0]['section_bits'] == 16
+    assert plan.problems[0]['alpha_rows'] == 60
+    rng = np.random.default_rng(51)
+    v = rng.standard_normal(plan.n_total) + 1j * rng.standard_normal(plan.n_total)
+    want = sqdref.matvec_u64(t.xmask, _groups(t), keys, v)
+    assert _rel(plan.matvec(torch.from_numpy(v).to(DEV)).cpu().numpy(), want) < RTOL64
+    assert _rel(plan.matvec(torch.from_numpy(v.astype(np.complex64)).to(DEV)).cpu().numpy(), want) < RTOL32
+    want_re = sqdref.matvec_u64(t.xmask, _groups(t), keys, v.real)
+    assert _rel(plan.matvec(torch.from_numpy(v.real.copy()).to(DEV)).cpu().numpy(), want_re.real) < RTOL64
+    assert _rel(plan.matvec(torch.from_numpy(v.real.astype(np.float32)).to(DEV)).cpu().numpy(), want_re.real) < RTOL32
+    coef = rng.standard_normal(plan.info['n_terms']) + 1j * rng.standard_normal(plan.info['n_terms'])
+    ct = torch.from_numpy(coef).to(DEV).contiguous()
+    _capi.check(_capi.lib().rqx_plan_update_coef(plan._handle, torch.view_as_real(ct).data_ptr(), 0,
+                                                 _capi.current_stream(plan.device)))
+    plan.is_real = False
+    t2 = TermTables(t.xmask, t.grp_off, t.zmask, coef, t.num_qubits)
+    want2 = sqdref.matvec_u64(t2.xmask, _groups(t2), keys, v)
+    assert _rel(plan.matvec(torch.from_numpy(v).to(DEV)).cpu().numpy(), want2) < RTOL64
+    assert _rel(plan.matvec(torch.from_numpy(v.astype(np.complex64)).to(DEV)).cpu().numpy(), want2) < RTOL32
+    # the hit cache sums the same values in the same order: bit for bit the matrix-free result
+    plan.build_cache(0)
+    for dt in (torch.complex128, torch.complex64):
+        x = torch.from_numpy(v).to(DEV).to(dt)
+        plan.use_cache(False)
+        y_free = plan.matvec(x)
+        plan.use_cache(True)
+        assert torch.equal(plan.matvec(x), y_free), dt
+    assert Plan(t, keys[1:], split_bits=10).problems[0]['alpha_rows'] == 0
+
+
 @pytest.mark.parametrize('name', ['small_10q', 'filler_12q'])
 def test_golden_fixtures(name):
     _say(f'golden {name}')
