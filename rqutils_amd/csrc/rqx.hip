@@ -326,6 +326,7 @@ struct KParams {
     const void *al_ent;
     const uint32_t *al_pg_off;
     const uint32_t *al_pg;
+    const AlEnt<float> *al_ent_lo;  // the FP32 entries (a cache fill of both precisions)
 };
 
 template <typename R> struct C2;
@@ -874,46 +875,30 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
 
     V acc = vzero(static_cast<V *>(nullptr));       // diagonal + section-B groups
     V acc_a = vzero(static_cast<V *>(nullptr));     // section-A groups
-    // alpha lists: this lane's alpha index a (row i = al_rb + a), its entries
-    // [0, al_n) and the next one (group al_g, column al_rb + al_col, value al_v;
-    // al_g = UINT32_MAX when none is left)
-    const bool al_on = HQ && pd.al_nlo != 0;
+    // alpha lists (the matvec, the hit count and the cache passes; a value pass
+    // of section A only has no use for them): this lane's alpha index a (row
+    // i = al_rb + a), its entries [0, al_n) and the next one (group al_g, column
+    // al_rb + al_col, value al_v and, in a fill of both precisions, al_l; al_g =
+    // UINT32_MAX when none is left)
+    constexpr bool AL_FILL = MODE == MODE_FILL || MODE == MODE_FILLCOL;
+    constexpr bool AL = HQ || MODE == MODE_ROWCOUNT || AL_FILL;
+    const bool al_on = AL && pd.al_nlo != 0 && !(MODE == MODE_FILL && p.c_a_only);
     uint32_t al_rb = 0, al_k = 0, al_n = 0, al_g = UINT32_MAX, al_col = 0;
     int64_t al_pos = 0;
     CC al_v{};
+    float2 al_l{};
     auto al_load = [&]() {
         if (al_k < al_n) {
-            const AlEnt<R> en = static_cast<const AlEnt<R> *>(p.al_ent)[al_pos + 64 * static_cast<int64_t>(al_k)];
+            const int64_t o = al_pos + 64 * static_cast<int64_t>(al_k);
+            const AlEnt<R> en = static_cast<const AlEnt<R> *>(p.al_ent)[o];
             al_g = en.g;
             al_col = en.col;
             al_v = en.v;
+            if constexpr (AL_FILL) {
+                if (p.c_val_lo) al_l = p.al_ent_lo[o].v;
+            }
         } else {
             al_g = UINT32_MAX;
-        }
-    };
-    if constexpr (HQ) {
-        if (al_on && active) {
-            const uint32_t a = (i - pd.row0) % pd.al_nlo, ag = pd.al_a0 + a;
-            al_rb = i - a;
-            al_n = p.al_len[ag];
-            al_pos = p.al_boff[ag >> 6] + (ag & 63u);
-            al_load();
-        }
-    }
-    // add this lane's listed hits of the groups before gend, in group order (the
-    // next entry's load and this one's x load in flight together)
-    auto al_emit = [&](uint32_t gend) {
-        if constexpr (HQ) {
-            while (__ballot(al_g < gend) != 0) {
-                if (al_g < gend) {
-                    const V xv = x[al_rb + al_col];
-                    const CC v = al_v;
-                    ++al_k;
-                    al_load();
-                    if constexpr (REALC) fma_acc(acc, v.x, xv);
-                    else fma_acc(acc, v, xv);
-                }
-            }
         }
     };
     unsigned long long n_hit = 0, n_probe = 0, n_look = 0, n_hit_a = 0;
@@ -981,6 +966,45 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
             if (p.c_val_lo) store_packet_sell<FL>(p.c_val_lo, fill_bo, fill_W, kb / PACK, fill_pos, b.l);
         }
     };
+    if constexpr (AL) {
+        if (al_on && active) {
+            const uint32_t a = (i - pd.row0) % pd.al_nlo, ag = pd.al_a0 + a;
+            al_rb = i - a;
+            al_n = p.al_len[ag];
+            al_pos = p.al_boff[ag >> 6] + (ag & 63u);
+            if constexpr (MODE == MODE_ROWCOUNT) {
+                n_hit += al_n;         // counted, not visited
+                al_n = 0;
+            }
+            al_load();
+        }
+    }
+    // this lane's listed hits of the groups before gend, in group order: added
+    // to the row (the next entry's load in flight with this one's x load), or
+    // appended to the row's section B
+    auto al_emit = [&](uint32_t gend) {
+        if constexpr (AL) {
+            while (__ballot(al_g < gend) != 0) {
+                if (al_g < gend) {
+                    const uint32_t col = al_rb + al_col;
+                    const CC v = al_v;
+                    const float2 l = al_l;
+                    ++al_k;
+                    if constexpr (MODE == MODE_MATVEC) {
+                        const V xv = x[col];
+                        al_load();
+                        if constexpr (REALC) fma_acc(acc, v.x, xv);
+                        else fma_acc(acc, v, xv);
+                    } else if constexpr (AL_FILL) {
+                        al_load();
+                        if constexpr (CPLX) fill_push(fill_b, static_cast<uint32_t>(n_hit), static_cast<int32_t>(col), v, l);
+                        else fill_push(fill_b, static_cast<uint32_t>(n_hit), static_cast<int32_t>(col), v.x, l.x);
+                        n_hit += 1;
+                    }
+                }
+            }
+        }
+    };
     V gv = vzero(static_cast<V *>(nullptr));
     if constexpr (MODE == MODE_GRAD) {
         if (active) gv = static_cast<const V *>(p.gy)[i];
@@ -1021,7 +1045,7 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
             // only for them)
             bool al_cls = false;
             uint32_t pq0 = 0, pq1 = 0;
-            if constexpr (HQ) {
+            if constexpr (AL) {
                 if (al_on && (ldc(p.cls_lane + td.cls) & 2u) != 0) {
                     al_cls = true;
                     pq0 = ldc(p.al_pg_off + td.cls);
@@ -1224,30 +1248,43 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
                     f3 = s_keys[b3] == t3;
                 }
             };
-            // an alpha class: its groups outside the lists searched four at a time,
-            // each evaluated where it comes in group order, this lane's listed hits
-            // added before it; the hit queue stays empty
+            // an alpha class: its groups outside the lists searched in group order,
+            // this lane's listed hits taken before each and at the end.  The passes
+            // that append to section B search one group at a time (the emitter and
+            // its packet stores inlined once); the matvec four at a time, its hit
+            // queue left empty.
+            if constexpr (AL && !HQ) {
+                if (al_cls) {
+                    for (uint32_t q = pq0;; ++q) {
+                        const uint32_t gq = q < pq1 ? ldc(p.al_pg + q) : g1;
+                        al_emit(gq);
+                        if (q >= pq1) break;
+                        uint32_t b;
+                        const bool f = search(slo ^ ldc(p.g_xlo + gq), b);
+                        handle(gq, b, f, false);
+                    }
+                    g = g1;
+                }
+            }
             if constexpr (HQ) {
                 if (al_cls) {
                     auto other = [&](const GroupDesc &d, uint32_t gq, uint32_t b, bool f) {
                         al_emit(gq);
                         const bool h = active && f;
-                        if (lane_evaluable(d)) {
-                            if (h) {
-                                R vx, vy;
-                                lane_group_value<R, REALC>(p, d, s, vx, vy);
-                                const V xv = x[ta + b];
-                                if constexpr (REALC) {
-                                    fma_acc(acc, vx, xv);
-                                } else {
-                                    CC c;
-                                    c.x = vx;
-                                    c.y = vy;
-                                    fma_acc(acc, c, xv);
-                                }
-                            }
-                        } else {
+                        if (!lane_evaluable(d)) {
                             consume(gq, b, h, false, value(d, b, h));
+                        } else if (h) {
+                            R vx, vy;
+                            lane_group_value<R, REALC>(p, d, s, vx, vy);
+                            const V xv = x[ta + b];
+                            if constexpr (REALC) {
+                                fma_acc(acc, vx, xv);
+                            } else {
+                                CC c;
+                                c.x = vx;
+                                c.y = vy;
+                                fma_acc(acc, c, xv);
+                            }
                         }
                     };
                     uint32_t q = pq0;
@@ -3176,6 +3213,7 @@ KParams make_params(const rqx_plan *p, bool fp64) {
     k.al_ent = fp64 ? static_cast<const void *>(p->al_e64) : static_cast<const void *>(p->al_e32);
     k.al_pg_off = p->al_pg_off;
     k.al_pg = p->al_pg;
+    k.al_ent_lo = p->al_e32;
     return k;
 }
 
