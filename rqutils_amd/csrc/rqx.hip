@@ -200,7 +200,7 @@ struct ProbDesc {
     uint32_t row0;                  // the problem's first row
     uint32_t al_nlo;                // alpha lists: low strings n_lo of the product (rows b * n_lo + a), or 0
     uint32_t al_a0;                 //   the problem's first alpha index in the list arrays (a multiple of 64)
-    uint32_t pad0;
+    uint32_t prod;                  // 1: the states are a product at a_bits (a_bits > L)
 };
 
 __device__ inline uint64_t section_mask(int32_t a_bits) { return a_bits >= 64 ? ~0ull : ((1ull << a_bits) - 1ull); }
@@ -859,6 +859,7 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
     const ProbDesc pd = ldc(p.prob + ldc(p.tile_prob + tile));
     const uint32_t seg = ldc(p.tile_seg + tile);
     const uint32_t i = ldc(p.tile_row0 + tile) + tid;
+    const uint32_t seg0 = ldc(p.seg_off + seg);       // the segment's first row
     const bool active = i < ldc(p.seg_off + seg + 1);
     const uint64_t s = active ? p.keys[i] : 0ull;
     const uint32_t slo = static_cast<uint32_t>(s) & pd.lomask;
@@ -1052,7 +1053,12 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
                     pq1 = ldc(p.al_pg_off + td.cls + 1);
                 }
             }
-            const bool staged = !al_cls || pq0 < pq1;        // the same for the whole workgroup
+            // a section-A group of a product (it flips high bits only; the class's
+            // first group) hits every row, at the row's own rank in the target
+            // segment: no search.  A class of that group alone stages nothing.
+            const bool a_direct = pd.prod && ldc(p.g_xlo + td.g0) == 0 &&
+                                  (ldc(p.g_x + td.g0) & section_mask(pd.a_bits)) == 0;
+            const bool staged = (!al_cls || pq0 < pq1) && !(a_direct && td.g1 == td.g0 + 1);   // uniform
             if (staged) {
                 if (hashed) {
                     // 16-byte pieces: a table has 2 << lg >= 4 words and starts at a multiple of 4
@@ -1214,8 +1220,8 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
             // problem's section split (its bits [L, a_bits) are zero), else section B
             if (!al_cls && g < g1 && ldc(p.g_xlo + g) == 0) {
                 const bool sec_a = (ldc(p.g_x + g) & section_mask(pd.a_bits)) == 0;
-                uint32_t base;
-                const bool f = search(slo, base);
+                uint32_t base = i - seg0;
+                const bool f = a_direct || search(slo, base);
                 if (!(MODE == MODE_FILL && p.c_a_only && !sec_a))   // section B was filled by the column pass
                     handle(g, base, f, sec_a);
                 ++g;
@@ -4294,6 +4300,7 @@ int build_plan(rqx_plan **out, const std::vector<ProblemIn> &in, int64_t n_total
         desc[k].row0 = static_cast<uint32_t>(row_base);
         desc[k].al_nlo = ph.al_nlo;
         desc[k].al_a0 = ph.al_a0;
+        desc[k].prod = ph.a_bits > ph.L ? 1u : 0u;
         p->max_seg = std::max(p->max_seg, ph.max_seg);
         p->is_real = p->is_real && ph.is_real;
         if (ph.max_offdiag_terms >= TP_MIN_TERMS) p->long_groups = 1;
