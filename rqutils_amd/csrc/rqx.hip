@@ -201,6 +201,8 @@ struct ProbDesc {
     uint32_t al_nlo;                // alpha lists: low strings n_lo of the product (rows b * n_lo + a), or 0
     uint32_t al_a0;                 //   the problem's first alpha index in the list arrays (a multiple of 64)
     uint32_t prod;                  // 1: the states are a product at a_bits (a_bits > L)
+    uint32_t al_nhi;                // alpha lists: high strings of the product (rows / n_lo)
+    uint32_t pad0, pad1, pad2;
 };
 
 __device__ inline uint64_t section_mask(int32_t a_bits) { return a_bits >= 64 ? ~0ull : ((1ull << a_bits) - 1ull); }
@@ -233,6 +235,22 @@ template <> struct __attribute__((aligned(16))) AlEnt<float> {
 template <> struct __attribute__((aligned(16))) AlEnt<double> {
     uint32_t g, col, pad0, pad1;
     double2 v;
+};
+// An entry whose column carries AL_SPLIT is a group whose terms also hold beta Z
+// bits, split as C(s) = F(alpha) + sum_j (-1)^popc(s & zeta_j) G_j(beta): zeta_j
+// the distinct alpha parts of its terms with beta Z bits, G_j(beta) the sum of
+// those terms' coefficients times (-1)^popc(s_beta & z_k), F the sum over its
+// terms without beta Z bits (each sum in term order).  The entry's value is F;
+// the group's SplitDesc gives the zeta_j and its rows of the G tables (one row of
+// n_hi entries per j, in the matvec's precision).
+constexpr uint32_t AL_SPLIT = 0x80000000u;
+constexpr int SPLIT_MAX = 2;
+struct __attribute__((aligned(16))) SplitDesc {
+    uint64_t zeta[SPLIT_MAX];
+    uint32_t grow;                  // first row of the G tables
+    uint32_t n_p;                   // patterns (<= SPLIT_MAX; 0: F is the whole sum)
+    uint32_t split;                 // 1: the group's entries are split, 0: plain values
+    uint32_t pad0;
 };
 
 // One (class, target segment) pair of a segment's target list, with what the
@@ -327,6 +345,9 @@ struct KParams {
     const uint32_t *al_pg_off;
     const uint32_t *al_pg;
     const AlEnt<float> *al_ent_lo;  // the FP32 entries (a cache fill of both precisions)
+    const SplitDesc *al_split;      // per group (alpha-listed groups only)
+    const void *al_G;               // G tables in the precision of coef (double2 or float2)
+    const float2 *al_G_lo;          // FP32 G tables (a cache fill of both precisions)
 };
 
 template <typename R> struct C2;
@@ -339,6 +360,22 @@ template <typename R> struct VT<R, true> { using T = typename C2<R>::T; };
 template <typename R> __device__ inline R vzero(R *) { return R(0); }
 __device__ inline double2 vzero(double2 *) { return make_double2(0.0, 0.0); }
 __device__ inline float2 vzero(float2 *) { return make_float2(0.f, 0.f); }
+
+// The value of a split alpha-list entry: F (the entry's value) plus the group's G
+// terms of beta block b, in pattern order
+template <typename R>
+__device__ inline typename C2<R>::T split_value(typename C2<R>::T v, const SplitDesc &sd, uint64_t s,
+                                                const typename C2<R>::T *G, uint32_t nhi, uint32_t b) {
+#pragma unroll
+    for (int j = 0; j < SPLIT_MAX; ++j)
+        if (static_cast<uint32_t>(j) < sd.n_p) {
+            const typename C2<R>::T gv = G[static_cast<size_t>(sd.grow) + static_cast<size_t>(j) * nhi + b];
+            const bool neg = __popcll(s & sd.zeta[j]) & 1;
+            v.x += neg ? -gv.x : gv.x;
+            v.y += neg ? -gv.y : gv.y;
+        }
+    return v;
+}
 
 // acc += c * v  (c complex coefficient sum; v real or complex vector entry)
 __device__ inline void fma_acc(double &acc, double2 c, double v) { acc = fma(c.x, v, acc); }
@@ -884,7 +921,7 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
     constexpr bool AL_FILL = MODE == MODE_FILL || MODE == MODE_FILLCOL;
     constexpr bool AL = HQ || MODE == MODE_ROWCOUNT || AL_FILL;
     const bool al_on = AL && pd.al_nlo != 0 && !(MODE == MODE_FILL && p.c_a_only);
-    uint32_t al_rb = 0, al_k = 0, al_n = 0, al_g = UINT32_MAX, al_col = 0;
+    uint32_t al_rb = 0, al_k = 0, al_n = 0, al_g = UINT32_MAX, al_col = 0, al_b = 0;   // al_b: the beta block
     int64_t al_pos = 0;
     CC al_v{};
     float2 al_l{};
@@ -969,7 +1006,8 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
     };
     if constexpr (AL) {
         if (al_on && active) {
-            const uint32_t a = (i - pd.row0) % pd.al_nlo, ag = pd.al_a0 + a;
+            al_b = (i - pd.row0) / pd.al_nlo;
+            const uint32_t a = (i - pd.row0) - al_b * pd.al_nlo, ag = pd.al_a0 + a;
             al_rb = i - a;
             al_n = p.al_len[ag];
             al_pos = p.al_boff[ag >> 6] + (ag & 63u);
@@ -987,12 +1025,20 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
         if constexpr (AL) {
             while (__ballot(al_g < gend) != 0) {
                 if (al_g < gend) {
-                    const uint32_t col = al_rb + al_col;
-                    const CC v = al_v;
-                    const float2 l = al_l;
+                    const uint32_t col = al_rb + (al_col & ~AL_SPLIT);
+                    V xv{};
+                    if constexpr (MODE == MODE_MATVEC) xv = x[col];
+                    CC v = al_v;
+                    float2 l = al_l;
+                    if (al_col & AL_SPLIT) {
+                        const SplitDesc sd = p.al_split[al_g];
+                        v = split_value<R>(v, sd, s, static_cast<const CC *>(p.al_G), pd.al_nhi, al_b);
+                        if constexpr (AL_FILL) {
+                            if (p.c_val_lo) l = split_value<float>(l, sd, s, p.al_G_lo, pd.al_nhi, al_b);
+                        }
+                    }
                     ++al_k;
                     if constexpr (MODE == MODE_MATVEC) {
-                        const V xv = x[col];
                         al_load();
                         if constexpr (REALC) fma_acc(acc, v.x, xv);
                         else fma_acc(acc, v, xv);
@@ -1598,6 +1644,57 @@ __global__ void k_group_tables(const int32_t *__restrict__ g_tab, const uint64_t
     }
 }
 
+// F of a split group at s: its terms without beta Z bits, in term order
+template <typename R>
+__device__ inline typename C2<R>::T split_f(const KParams &p, const GroupDesc &d, uint64_t s, uint64_t amask) {
+    using CC = typename C2<R>::T;
+    const CC *coef = static_cast<const CC *>(p.coef);
+    CC v;
+    v.x = R(0);
+    v.y = R(0);
+    for (uint32_t k = d.k0; k < d.k1; ++k) {
+        const uint64_t z = p.z[k];
+        if ((z & ~amask) != 0) continue;
+        const CC c = coef[k];
+        const bool odd = __popcll(s & z) & 1;
+        v.x += odd ? -c.x : c.x;
+        v.y += odd ? -c.y : c.y;
+    }
+    return v;
+}
+
+// G tables of one problem's split groups: row r (group rg[r], alpha pattern
+// rz[r]) at entry b = the sum over the group's terms with beta Z bits and alpha
+// part rz[r] of c_k (-1)^popc(s_b & z_k), s_b the beta bits of beta block b's
+// first state; at G[roff[r] + b], FP64 and FP32 (each in term order)
+__global__ __launch_bounds__(256) void k_alpha_g(const KParams p64, const KParams p32, const uint64_t *__restrict__ keys,
+                                                 uint32_t nlo, uint32_t nhi, uint64_t amask,
+                                                 const uint32_t *__restrict__ rg, const uint64_t *__restrict__ rz,
+                                                 const uint32_t *__restrict__ roff, double2 *__restrict__ G64,
+                                                 float2 *__restrict__ G32) {
+    const uint32_t r = blockIdx.y, b = blockIdx.x * 256 + threadIdx.x;
+    if (b >= nhi) return;
+    const uint64_t sb = keys[static_cast<size_t>(b) * nlo] & ~amask, zeta = rz[r];
+    const GroupDesc d = p64.gdesc[rg[r]];
+    const double2 *c64 = static_cast<const double2 *>(p64.coef);
+    const float2 *c32 = static_cast<const float2 *>(p32.coef);
+    double2 h = make_double2(0.0, 0.0);
+    float2 l = make_float2(0.f, 0.f);
+    for (uint32_t k = d.k0; k < d.k1; ++k) {
+        const uint64_t z = p64.z[k];
+        if ((z & ~amask) == 0 || (z & amask) != zeta) continue;
+        const bool odd = __popcll(sb & z) & 1;
+        const double2 ch = c64[k];
+        const float2 cl = c32[k];
+        h.x += odd ? -ch.x : ch.x;
+        h.y += odd ? -ch.y : ch.y;
+        l.x += odd ? -cl.x : cl.x;
+        l.y += odd ? -cl.y : cl.y;
+    }
+    G64[roff[r] + b] = h;
+    G32[roff[r] + b] = l;
+}
+
 // Alpha lists of one problem: workgroup a takes alpha index a (row a of the
 // first beta block, keys[a]) and the problem's listed groups lg[0..nlg) in
 // order: group g hits when alpha_a ^ (x_g & amask) is an alpha string, at its
@@ -1644,19 +1741,24 @@ __global__ __launch_bounds__(256) void k_alpha_lists(const KParams p64, const KP
         if (boff && f) {
             const uint32_t k = before + static_cast<uint32_t>(__popcll(m & ((1ull << lane) - 1ull)));
             const int64_t pos = boff[ag >> 6] + 64 * static_cast<int64_t>(k) + (ag & 63u);
-            double hx, hy;
-            float lx, ly;
-            lane_group_value<double, false>(p64, p64.gdesc[g], s, hx, hy);
-            lane_group_value<float, false>(p32, p32.gdesc[g], s, lx, ly);
+            const GroupDesc gd = p64.gdesc[g];
             AlEnt<double> eh{};
-            eh.g = g;
-            eh.col = col;
-            eh.v = make_double2(hx, hy);
-            e64[pos] = eh;
             AlEnt<float> el{};
-            el.g = g;
-            el.col = col;
-            el.v = make_float2(lx, ly);
+            eh.g = el.g = g;
+            if (p64.al_split[g].split) {
+                eh.col = el.col = col | AL_SPLIT;
+                eh.v = split_f<double>(p64, gd, s, amask);
+                el.v = split_f<float>(p32, gd, s, amask);
+            } else {
+                double hx, hy;
+                float lx, ly;
+                lane_group_value<double, false>(p64, gd, s, hx, hy);
+                lane_group_value<float, false>(p32, gd, s, lx, ly);
+                eh.col = el.col = col;
+                eh.v = make_double2(hx, hy);
+                el.v = make_float2(lx, ly);
+            }
+            e64[pos] = eh;
             e32[pos] = el;
         }
         base += total;
@@ -2856,6 +2958,8 @@ struct ProbHost {            // host record of one problem of a plan
     uint32_t al_nlo = 0;       // alpha lists: low strings (0: none), first alpha index,
     uint32_t al_a0 = 0;        //   and the problem's listed groups [al_lg0, al_lg0 + al_lgn) of the plan's list
     uint32_t al_lg0 = 0, al_lgn = 0;
+    uint32_t al_nhi = 0;       // high strings of the product
+    uint32_t al_r0 = 0, al_rn = 0;   // the problem's G-table rows [al_r0, al_r0 + al_rn) of the plan's list
 };
 
 struct HitCache {            // sliced-ELL cache of the hits (column, C_j(s_i)) in one precision
@@ -2965,6 +3069,13 @@ struct rqx_plan {
     AlEnt<float> *al_e32 = nullptr;
     AlEnt<double> *al_e64 = nullptr;
     int64_t al_slots = 0;           // entries per precision, padding included
+    SplitDesc *al_split = nullptr;  // per group: split values (alpha-listed groups)
+    uint32_t *al_rg = nullptr;      // G-table rows: group, first element, alpha pattern
+    uint32_t *al_roff = nullptr;
+    uint64_t *al_rz = nullptr;
+    double2 *al_G64 = nullptr;      // G tables, FP64 and FP32
+    float2 *al_G32 = nullptr;
+    int64_t al_gelems = 0;
 };
 
 namespace {
@@ -2998,6 +3109,20 @@ bool has_sign_table(const ProblemIn &q, int32_t j) {
     return true;
 }
 
+// The distinct alpha parts (z & amask, first-seen order) of group j's terms that
+// hold beta Z bits (z & ~amask != 0) into P; false when there are more than SPLIT_MAX
+bool split_patterns(const ProblemIn &q, int32_t j, uint64_t amask, std::vector<uint64_t> &P) {
+    P.clear();
+    for (int64_t t = q.grp_off[j]; t < q.grp_off[j + 1]; ++t) {
+        const uint64_t z = q.zmask[t];
+        if ((z & ~amask) == 0) continue;
+        if (std::find(P.begin(), P.end(), z & amask) != P.end()) continue;
+        if (P.size() == static_cast<size_t>(SPLIT_MAX)) return false;
+        P.push_back(z & amask);
+    }
+    return true;
+}
+
 // The cost model's view of the alpha lists of q when its states are a product
 // at bit a_bits with n_lo low strings (first: the first n_lo states, ascending
 // low parts): the groups the plan build lists (flip only bits below a_bits, a
@@ -3010,7 +3135,7 @@ AlphaModel alpha_model(const ProblemIn &q, const uint64_t *first, int64_t n_lo, 
     if (n_lo <= 0 || a_bits <= 0 || a_bits >= 64) return am;
     const uint64_t amask = (1ull << a_bits) - 1ull;
     std::vector<uint8_t> listed(static_cast<size_t>(q.n_groups), 0);
-    std::vector<uint64_t> lx;     // X masks of the listed groups
+    std::vector<uint64_t> lx, pat;   // X masks of the listed groups
     bool tp = false;
     for (int32_t j = 0; j < q.n_groups; ++j) {
         const uint64_t x = q.xmask[j];
@@ -3018,10 +3143,10 @@ AlphaModel alpha_model(const ProblemIn &q, const uint64_t *first, int64_t n_lo, 
         const bool tab = has_sign_table(q, j);
         const int64_t k = q.grp_off[j + 1] - q.grp_off[j];
         tp = tp || tab || k >= TP_MIN_TERMS;
-        if ((x & ~amask) != 0 || !(tab || k < TP_MIN_TERMS)) continue;
+        if ((x & ~amask) != 0) continue;
         uint64_t zor = 0;
         for (int64_t t = q.grp_off[j]; t < q.grp_off[j + 1]; ++t) zor |= q.zmask[t];
-        if ((zor & ~amask) == 0) {
+        if (((tab || k < TP_MIN_TERMS) && (zor & ~amask) == 0) || split_patterns(q, j, amask, pat)) {
             listed[static_cast<size_t>(j)] = 1;
             lx.push_back(x);
         }
@@ -3220,6 +3345,9 @@ KParams make_params(const rqx_plan *p, bool fp64) {
     k.al_pg_off = p->al_pg_off;
     k.al_pg = p->al_pg;
     k.al_ent_lo = p->al_e32;
+    k.al_split = p->al_split;
+    k.al_G = fp64 ? static_cast<const void *>(p->al_G64) : static_cast<const void *>(p->al_G32);
+    k.al_G_lo = p->al_G32;
     return k;
 }
 
@@ -3229,6 +3357,16 @@ hipError_t launch_alpha_lists(const rqx_plan *p, bool fill, hipStream_t st) {
     const KParams p64 = make_params(p, true), p32 = make_params(p, false);
     for (const ProbHost &ph : p->probs) {
         if (ph.al_nlo == 0) continue;
+        // the G tables of the split groups (fill pass), at most 65535 rows a launch
+        for (uint32_t r = 0; fill && r < ph.al_rn; r += 65535u) {
+            const uint32_t nr = std::min(ph.al_rn - r, 65535u);
+            hipLaunchKernelGGL(k_alpha_g, dim3((ph.al_nhi + 255) / 256, nr), dim3(256), 0, st, p64, p32,
+                               p->keys + ph.row_base, ph.al_nlo, ph.al_nhi, (1ull << ph.a_bits) - 1ull,
+                               p->al_rg + ph.al_r0 + r, p->al_rz + ph.al_r0 + r, p->al_roff + ph.al_r0 + r, p->al_G64,
+                               p->al_G32);
+            const hipError_t e = hipGetLastError();
+            if (e != hipSuccess) return e;
+        }
         hipLaunchKernelGGL(k_alpha_lists, dim3(ph.al_nlo), dim3(256), 0, st, p64, p32, p->keys + ph.row_base,
                            ph.al_nlo, (1ull << ph.a_bits) - 1ull, p->al_lg + ph.al_lg0, ph.al_lgn, ph.al_a0,
                            p->al_len, fill ? p->al_boff : nullptr, p->al_e32, p->al_e64);
@@ -3247,6 +3385,8 @@ constexpr int64_t ALPHA_LIST_MAX_BYTES = int64_t(1) << 30;
 // problems' descriptors are rewritten without lists.
 hipError_t build_alpha_lists(rqx_plan *p, uint32_t n_alpha, const std::vector<uint32_t> &lg,
                              const std::vector<uint32_t> &pg_off, const std::vector<uint32_t> &pg,
+                             const std::vector<SplitDesc> &split, const std::vector<uint32_t> &rg,
+                             const std::vector<uint64_t> &rz, const std::vector<uint32_t> &roff, int64_t gelems,
                              std::vector<ProbDesc> &desc, hipStream_t st) {
     if (n_alpha == 0 || lg.empty()) return hipSuccess;
     // every problem searches all its groups: no lists built, or (plain kernels
@@ -3261,8 +3401,11 @@ hipError_t build_alpha_lists(rqx_plan *p, uint32_t n_alpha, const std::vector<ui
         return e;
     };
     if (!p->long_groups) return none();
+    const size_t nr = rg.size();
     const size_t o_len = 0, o_lg = align_up(size_t(n_alpha) * 4), o_pgo = o_lg + align_up(lg.size() * 4),
-                 o_pg = o_pgo + align_up(pg_off.size() * 4), bytes = o_pg + align_up(std::max<size_t>(pg.size(), 1) * 4);
+                 o_pg = o_pgo + align_up(pg_off.size() * 4), o_split = o_pg + align_up(std::max<size_t>(pg.size(), 1) * 4),
+                 o_rg = o_split + align_up(split.size() * sizeof(SplitDesc)), o_roff = o_rg + align_up(std::max<size_t>(nr, 1) * 4),
+                 o_rz = o_roff + align_up(std::max<size_t>(nr, 1) * 4), bytes = o_rz + align_up(std::max<size_t>(nr, 1) * 8);
     hipError_t e = hipMalloc(&p->al_block, bytes);
     if (e != hipSuccess) return e;
     char *b = static_cast<char *>(p->al_block);
@@ -3270,10 +3413,19 @@ hipError_t build_alpha_lists(rqx_plan *p, uint32_t n_alpha, const std::vector<ui
     p->al_lg = reinterpret_cast<uint32_t *>(b + o_lg);
     p->al_pg_off = reinterpret_cast<uint32_t *>(b + o_pgo);
     p->al_pg = reinterpret_cast<uint32_t *>(b + o_pg);
+    p->al_split = reinterpret_cast<SplitDesc *>(b + o_split);
+    p->al_rg = reinterpret_cast<uint32_t *>(b + o_rg);
+    p->al_roff = reinterpret_cast<uint32_t *>(b + o_roff);
+    p->al_rz = reinterpret_cast<uint64_t *>(b + o_rz);
     e = fill_async(p->al_len, int64_t(0), static_cast<int64_t>(n_alpha), 0u, st);
     if (e == hipSuccess) e = hipMemcpyAsync(p->al_lg, lg.data(), lg.size() * 4, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipMemcpyAsync(p->al_pg_off, pg_off.data(), pg_off.size() * 4, hipMemcpyHostToDevice, st);
     if (e == hipSuccess && !pg.empty()) e = hipMemcpyAsync(p->al_pg, pg.data(), pg.size() * 4, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess && !split.empty())
+        e = hipMemcpyAsync(p->al_split, split.data(), split.size() * sizeof(SplitDesc), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess && nr) e = hipMemcpyAsync(p->al_rg, rg.data(), nr * 4, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess && nr) e = hipMemcpyAsync(p->al_roff, roff.data(), nr * 4, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess && nr) e = hipMemcpyAsync(p->al_rz, rz.data(), nr * 8, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = launch_alpha_lists(p, false, st);
     std::vector<uint32_t> len(n_alpha);
     if (e == hipSuccess) e = hipMemcpyAsync(len.data(), p->al_len, size_t(n_alpha) * 4, hipMemcpyDeviceToHost, st);
@@ -3288,16 +3440,22 @@ hipError_t build_alpha_lists(rqx_plan *p, uint32_t n_alpha, const std::vector<ui
         boff[k] = slots;
         slots += 64 * static_cast<int64_t>(w);
     }
-    const int64_t ent_bytes = slots * static_cast<int64_t>(sizeof(AlEnt<float>) + sizeof(AlEnt<double>));
+    const int64_t ent_bytes = slots * static_cast<int64_t>(sizeof(AlEnt<float>) + sizeof(AlEnt<double>)) +
+                              gelems * static_cast<int64_t>(sizeof(double2) + sizeof(float2));
     if (slots == 0 || ent_bytes > ALPHA_LIST_MAX_BYTES) return none();   // no hits at all, or no room
-    const size_t o_e32 = align_up(size_t(n_blk) * 8), o_e64 = o_e32 + align_up(size_t(slots) * sizeof(AlEnt<float>));
-    e = hipMalloc(&p->al_ent_block, o_e64 + size_t(slots) * sizeof(AlEnt<double>));
+    const size_t o_e32 = align_up(size_t(n_blk) * 8), o_e64 = o_e32 + align_up(size_t(slots) * sizeof(AlEnt<float>)),
+                 o_g64 = o_e64 + align_up(size_t(slots) * sizeof(AlEnt<double>)),
+                 o_g32 = o_g64 + align_up(size_t(std::max<int64_t>(gelems, 1)) * sizeof(double2));
+    e = hipMalloc(&p->al_ent_block, o_g32 + size_t(std::max<int64_t>(gelems, 1)) * sizeof(float2));
     if (e != hipSuccess) return e;
     char *eb = static_cast<char *>(p->al_ent_block);
     p->al_boff = reinterpret_cast<int64_t *>(eb);
     p->al_e32 = reinterpret_cast<AlEnt<float> *>(eb + o_e32);
     p->al_e64 = reinterpret_cast<AlEnt<double> *>(eb + o_e64);
+    p->al_G64 = reinterpret_cast<double2 *>(eb + o_g64);
+    p->al_G32 = reinterpret_cast<float2 *>(eb + o_g32);
     p->al_slots = slots;
+    p->al_gelems = gelems;
     e = hipMemcpyAsync(p->al_boff, boff.data(), boff.size() * 8, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = launch_alpha_lists(p, true, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);   // boff is released on return
@@ -4154,6 +4312,10 @@ int build_plan(rqx_plan **out, const std::vector<ProblemIn> &in, int64_t n_total
     std::vector<GroupDesc> gdesc;
     std::vector<uint32_t> al_lg, al_pg, al_pg_off;   // alpha lists: listed groups; per class, its other groups
     uint32_t al_alpha = 0;                            // alpha indices of all problems (each from a multiple of 64)
+    std::vector<SplitDesc> al_split;                  // per group: split values
+    std::vector<uint32_t> al_rg, al_roff;             // G-table rows: group, first element
+    std::vector<uint64_t> al_rz;                      //   and alpha pattern
+    int64_t al_gelems = 0;                            // G-table elements
     int64_t n_tab = 0;
     int64_t row_base = 0, term_base = 0;
     int32_t grp_base = 0;
@@ -4254,37 +4416,72 @@ int build_plan(rqx_plan **out, const std::vector<ProblemIn> &in, int64_t n_total
         }
         {
             // alpha lists: in a product at the section split (rows b * n_lo + a) the
-            // classes below 2^(a_bits - L) flip only alpha bits; their groups that a
+            // classes below 2^(a_bits - L) flip only alpha bits.  Their groups that a
             // lane evaluates alone and whose Z masks hold only alpha bits are listed
-            // per alpha index, their other groups searched (al_pg)
-            const bool product = ph.a_bits > sg.L && ph.a_bits < 64 && ph.a_rows > 0 && ph.a_rows < q.n_states;
+            // per alpha index with their values; those whose terms with beta Z bits
+            // have at most SPLIT_MAX alpha parts are listed with split values; the
+            // others are searched (al_pg)
+            const bool product = ph.a_bits > sg.L && ph.a_bits < 64 && ph.a_rows > 0 && ph.a_rows < q.n_states &&
+                                 q.n_states / ph.a_rows <= 0xFFFFFFFFLL;
             const uint64_t amask = product ? (1ull << ph.a_bits) - 1ull : 0ull;
             auto listed = [&](uint32_t j) {
                 const size_t gj = static_cast<size_t>(grp_base) + j;
                 return q.xmask[j] != 0 && (g_tab[gj] >= 0 || q.grp_off[j + 1] - q.grp_off[j] < TP_MIN_TERMS) &&
                        (g_zor[gj] & ~amask) == 0;
             };
+            std::vector<uint64_t> pat;
+            auto split = [&](uint32_t j) { return q.xmask[j] != 0 && !listed(j) && split_patterns(q, j, amask, pat); };
             int32_t n_acls = 0;
-            const size_t lg0 = al_lg.size();
+            const size_t lg0 = al_lg.size(), r0 = al_rg.size();
+            al_split.resize(al_split.size() + static_cast<size_t>(q.n_groups), SplitDesc{});
+            const uint32_t nhi = product ? static_cast<uint32_t>(q.n_states / ph.a_rows) : 0u;
+            int64_t gel = al_gelems;
             if (product)
                 for (; n_acls < ph.n_classes && (sg.cls_hi[n_acls] >> (ph.a_bits - sg.L)) == 0; ++n_acls)
-                    for (uint32_t j = sg.cls_goff[n_acls]; j < sg.cls_goff[n_acls + 1]; ++j)
-                        if (listed(j)) al_lg.push_back(static_cast<uint32_t>(grp_base) + j);
-            if (al_lg.size() == lg0) {
+                    for (uint32_t j = sg.cls_goff[n_acls]; j < sg.cls_goff[n_acls + 1]; ++j) {
+                        const uint32_t gj = static_cast<uint32_t>(grp_base) + j;
+                        if (listed(j)) {
+                            al_lg.push_back(gj);
+                        } else if (split(j)) {
+                            al_lg.push_back(gj);
+                            SplitDesc &sd = al_split[gj];
+                            sd.split = 1;
+                            sd.n_p = static_cast<uint32_t>(pat.size());
+                            sd.grow = static_cast<uint32_t>(gel);
+                            for (size_t u = 0; u < pat.size(); ++u) {
+                                sd.zeta[u] = pat[u];
+                                al_rg.push_back(gj);
+                                al_rz.push_back(pat[u]);
+                                al_roff.push_back(static_cast<uint32_t>(gel));
+                                gel += nhi;
+                            }
+                        }
+                    }
+            if (al_lg.size() == lg0 || gel > 0xFFFFFFFFLL) {
                 n_acls = 0;
+                al_lg.resize(lg0);
+                al_rg.resize(r0);
+                al_rz.resize(r0);
+                al_roff.resize(r0);
+                for (int32_t j = 0; j < q.n_groups; ++j) al_split[static_cast<size_t>(grp_base) + j] = SplitDesc{};
             } else {
                 ph.al_nlo = static_cast<uint32_t>(ph.a_rows);
+                ph.al_nhi = nhi;
                 ph.al_a0 = al_alpha;
                 ph.al_lg0 = static_cast<uint32_t>(lg0);
                 ph.al_lgn = static_cast<uint32_t>(al_lg.size() - lg0);
+                ph.al_r0 = static_cast<uint32_t>(r0);
+                ph.al_rn = static_cast<uint32_t>(al_rg.size() - r0);
                 al_alpha += (ph.al_nlo + 63u) & ~63u;
+                al_gelems = gel;
             }
             for (int32_t c = 0; c < ph.n_classes; ++c) {
                 al_pg_off.push_back(static_cast<uint32_t>(al_pg.size()));
                 if (c >= n_acls) continue;
                 cls_lane_all[static_cast<size_t>(ph.cls_base) + c] |= 2u;
                 for (uint32_t j = sg.cls_goff[c]; j < sg.cls_goff[c + 1]; ++j)
-                    if (q.xmask[j] != 0 && !listed(j)) al_pg.push_back(static_cast<uint32_t>(grp_base) + j);
+                    if (q.xmask[j] != 0 && !listed(j) && !al_split[static_cast<size_t>(grp_base) + j].split)
+                        al_pg.push_back(static_cast<uint32_t>(grp_base) + j);
             }
         }
         for (int64_t t = 0; t < q.n_terms; ++t) {
@@ -4300,6 +4497,7 @@ int build_plan(rqx_plan **out, const std::vector<ProblemIn> &in, int64_t n_total
         desc[k].row0 = static_cast<uint32_t>(row_base);
         desc[k].al_nlo = ph.al_nlo;
         desc[k].al_a0 = ph.al_a0;
+        desc[k].al_nhi = ph.al_nhi;
         desc[k].prod = ph.a_bits > ph.L ? 1u : 0u;
         p->max_seg = std::max(p->max_seg, ph.max_seg);
         p->is_real = p->is_real && ph.is_real;
@@ -4491,7 +4689,8 @@ int build_plan(rqx_plan **out, const std::vector<ProblemIn> &in, int64_t n_total
     launch_group_tables(raw, st);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = build_targets(raw, st);
-    if (e == hipSuccess) e = build_alpha_lists(raw, al_alpha, al_lg, al_pg_off, al_pg, desc, st);
+    if (e == hipSuccess)
+        e = build_alpha_lists(raw, al_alpha, al_lg, al_pg_off, al_pg, al_split, al_rg, al_rz, al_roff, al_gelems, desc, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e == hipSuccess) {
         int occ = 0;
