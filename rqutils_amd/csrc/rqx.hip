@@ -202,7 +202,12 @@ struct ProbDesc {
     uint32_t al_a0;                 //   the problem's first alpha index in the list arrays (a multiple of 64)
     uint32_t prod;                  // 1: the states are a product at a_bits (a_bits > L)
     uint32_t al_nhi;                // alpha lists: high strings of the product (rows / n_lo)
-    uint32_t pad0, pad1, pad2;
+    uint32_t pm_base;               // partner map: first element (UINT32_MAX: none), its row length
+    uint32_t pm_na;                 //   (alpha patterns), the problem's first beta index of the
+    uint32_t bp_b0;                 //   beta-partner offsets
+    uint32_t bl_b0;                 // beta lists: the problem's first beta index of their offsets
+    uint32_t full;                  // 1: lists cover every group (no class is searched)
+    uint32_t pad0, pad1;
 };
 
 __device__ inline uint64_t section_mask(int32_t a_bits) { return a_bits >= 64 ? ~0ull : ((1ull << a_bits) - 1ull); }
@@ -348,6 +353,23 @@ struct KParams {
     const SplitDesc *al_split;      // per group (alpha-listed groups only)
     const void *al_G;               // G tables in the precision of coef (double2 or float2)
     const float2 *al_G_lo;          // FP32 G tables (a cache fill of both precisions)
+    // partner lists (product subspaces): per alpha index its partners (alpha
+    // pattern, alpha pattern index, a', 0) in the blocked layout of the alpha
+    // lists; per beta index its partners (beta pattern index, b') at
+    // [al_bpoff[b], al_bpoff[b + 1]); the partner map: the group of each (beta
+    // pattern, alpha pattern) pair, or -1
+    const uint32_t *al_plen;
+    const int64_t *al_pboff;
+    const uint4 *al_pent;
+    const uint32_t *al_bpoff;
+    const uint2 *al_bpent;
+    const int32_t *al_pmap;
+    // beta lists (product subspaces whose lists cover every group): per beta index
+    // its hits of the groups that flip only beta bits, [bl_off[b], bl_off[b + 1]),
+    // as (group, b' | AL_SPLIT, value) in the precision of coef (and FP32)
+    const uint32_t *bl_off;
+    const void *bl_ent;
+    const AlEnt<float> *bl_ent_lo;
 };
 
 template <typename R> struct C2;
@@ -723,6 +745,32 @@ __device__ inline void lane_group_value(const KParams &p, const GroupDesc &d, ui
     }
 }
 
+// lane_group_value in FP32 from the FP32 coefficients and tables of a fill of
+// both precisions (coef_lo, tab_lo_re, tab_lo_im): what the FP32 matvec gives
+template <bool RE>
+__device__ inline void lane_group_value_lo(const KParams &p, const GroupDesc &d, uint64_t s, float &vx, float &vy) {
+    vx = 0.f;
+    vy = 0.f;
+    if (d.tab >= 0) {
+        const uint32_t u = tab_entry(d, s, p.n_tab);
+        const bool neg = __popcll(s & d.z0) & 1;
+        const float re = p.tab_lo_re[u];
+        vx = neg ? -re : re;
+        if constexpr (!RE) {
+            const float im = p.tab_lo_im[u];
+            vy = neg ? -im : im;
+        }
+        return;
+    }
+    const float2 *coef = static_cast<const float2 *>(p.coef_lo);
+    for (uint32_t k = d.k0; k < d.k1; ++k) {
+        const float2 c = coef[k];
+        const bool odd = __popcll(s & p.z[k]) & 1;
+        vx += odd ? -c.x : c.x;
+        if constexpr (!RE) vy += odd ? -c.y : c.y;
+    }
+}
+
 // the same in FP64 and FP32 at once (the cache fill of both precisions): each
 // precision summed exactly as group_value_wave sums it
 template <bool RE>
@@ -923,6 +971,8 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
     const bool al_on = AL && pd.al_nlo != 0 && !(MODE == MODE_FILL && p.c_a_only);
     uint32_t al_rb = 0, al_k = 0, al_n = 0, al_g = UINT32_MAX, al_col = 0, al_b = 0;   // al_b: the beta block
     int64_t al_pos = 0;
+    uint32_t pl_n = 0;             // partners of this lane's alpha string, the first at pl_pos
+    int64_t pl_pos = 0;
     CC al_v{};
     float2 al_l{};
     auto al_load = [&]() {
@@ -1004,6 +1054,30 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
             if (p.c_val_lo) store_packet_sell<FL>(p.c_val_lo, fill_bo, fill_W, kb / PACK, fill_pos, b.l);
         }
     };
+    // a section-B value of a group a lane evaluates alone, as the matvecs of the
+    // cache's precisions evaluate it (lanes may hold different groups)
+    auto lane_fill_value = [&](const GroupDesc &d, FV &v, FL &l) {
+        if constexpr (std::is_same<R, double>::value) {
+            if (p.c_val_lo) {
+                double hx, hy;
+                float lx, ly;
+                lane_group_value<double, !CPLX>(p, d, s, hx, hy);
+                lane_group_value_lo<!CPLX>(p, d, s, lx, ly);
+                if constexpr (CPLX) {
+                    v = CC{hx, hy};
+                    l = make_float2(lx, ly);
+                } else {
+                    v = hx;
+                    l = lx;
+                }
+                return;
+            }
+        }
+        R vx, vy;
+        lane_group_value<R, !CPLX>(p, d, s, vx, vy);
+        if constexpr (CPLX) v = CC{vx, vy};
+        else v = vx;
+    };
     if constexpr (AL) {
         if (al_on && active) {
             al_b = (i - pd.row0) / pd.al_nlo;
@@ -1011,6 +1085,10 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
             al_rb = i - a;
             al_n = p.al_len[ag];
             al_pos = p.al_boff[ag >> 6] + (ag & 63u);
+            if (pd.pm_base != UINT32_MAX) {
+                pl_n = p.al_plen[ag];
+                pl_pos = p.al_pboff[ag >> 6] + (ag & 63u);
+            }
             if constexpr (MODE == MODE_ROWCOUNT) {
                 n_hit += al_n;         // counted, not visited
                 al_n = 0;
@@ -1079,8 +1157,10 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
 
     // the classes whose target segment exists for this tile's segment, in class
     // order, with that segment (listed once per plan by k_seg_targets)
-    const uint32_t e1 = ldc(p.st_off + seg + 1);
-    for (uint32_t e = ldc(p.st_off + seg); e < e1; ++e) {
+    // (a fully listed problem searches no class: its groups come from the lists)
+    const bool full_on = AL && pd.full != 0;
+    const uint32_t e1 = full_on ? 0u : ldc(p.st_off + seg + 1);
+    for (uint32_t e = full_on ? 0u : ldc(p.st_off + seg); e < e1; ++e) {
         {
             const TargetDesc td = ldc(p.st_list + e);
             if constexpr (MODE == MODE_COUNT) n_look += (tid == 0);
@@ -1104,7 +1184,13 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
             // segment: no search.  A class of that group alone stages nothing.
             const bool a_direct = pd.prod && ldc(p.g_xlo + td.g0) == 0 &&
                                   (ldc(p.g_x + td.g0) & section_mask(pd.a_bits)) == 0;
-            const bool staged = (!al_cls || pq0 < pq1) && !(a_direct && td.g1 == td.g0 + 1);   // uniform
+            // a mapped class: its groups that flip alpha bits come from the partner
+            // loop after the classes; only a first group of high bits is taken here
+            bool mapped = false;
+            if constexpr (AL) {
+                mapped = al_on && pd.pm_base != UINT32_MAX && (ldc(p.cls_lane + td.cls) & 4u) != 0;
+            }
+            const bool staged = (!al_cls || pq0 < pq1) && !(a_direct && td.g1 == td.g0 + 1) && !mapped;   // uniform
             if (staged) {
                 if (hashed) {
                     // 16-byte pieces: a table has 2 << lg >= 4 words and starts at a multiple of 4
@@ -1264,7 +1350,7 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
             // groups are sorted by X mask, so a class's group without low flips is
             // its first one; it is section A when it also flips no bit below the
             // problem's section split (its bits [L, a_bits) are zero), else section B
-            if (!al_cls && g < g1 && ldc(p.g_xlo + g) == 0) {
+            if (!al_cls && g < g1 && ldc(p.g_xlo + g) == 0 && (!mapped || a_direct)) {
                 const bool sec_a = (ldc(p.g_x + g) & section_mask(pd.a_bits)) == 0;
                 uint32_t base = i - seg0;
                 const bool f = a_direct || search(slo, base);
@@ -1300,6 +1386,7 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
                     f3 = s_keys[b3] == t3;
                 }
             };
+            if (mapped) g = g1;
             // an alpha class: its groups outside the lists searched in group order,
             // this lane's listed hits taken before each and at the end.  The passes
             // that append to section B search one group at a time (the emitter and
@@ -1445,6 +1532,99 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
                 handle(g, base, f, false);
             }
             if (staged) __syncthreads();   // before the next target overwrites s_keys
+        }
+    }
+
+    // the mapped classes' groups that flip alpha bits: for each partner of the
+    // tile's beta string (the same for every row of a tile; beta patterns in
+    // order) and each partner of the row's alpha string (alpha patterns in order),
+    // the map's group hits at (b', a')
+    if constexpr (AL) {
+        if (full_on) al_emit(UINT32_MAX);          // every alpha-list entry (no class took them)
+        if (al_on && pd.pm_base != UINT32_MAX) {
+            const uint32_t jb = (ldc(p.tile_row0 + tile) - pd.row0) / pd.al_nlo;
+            const uint32_t q1 = ldc(p.al_bpoff + pd.bp_b0 + jb + 1);
+            for (uint32_t q = ldc(p.al_bpoff + pd.bp_b0 + jb); q < q1; ++q) {
+                const uint2 bq = ldc(p.al_bpent + q);          // beta pattern index, b'
+                const int32_t *mrow = p.al_pmap + pd.pm_base + static_cast<size_t>(bq.x) * pd.pm_na;
+                const uint32_t tb = pd.row0 + bq.y * pd.al_nlo;
+                for (uint32_t k = 0; __ballot(k < pl_n) != 0; ++k) {
+                    if (k < pl_n) {
+                        const uint4 pe = p.al_pent[pl_pos + 64 * static_cast<int64_t>(k)];
+                        const int32_t gg = mrow[pe.y];
+                        if (gg >= 0) {
+                            const uint32_t col = tb + pe.z;
+                            const GroupDesc d = p.gdesc[gg];
+                            if constexpr (MODE == MODE_MATVEC) {
+                                const V xv = x[col];
+                                R vx, vy;
+                                lane_group_value<R, REALC>(p, d, s, vx, vy);
+                                if constexpr (REALC) {
+                                    fma_acc(acc, vx, xv);
+                                } else {
+                                    CC c;
+                                    c.x = vx;
+                                    c.y = vy;
+                                    fma_acc(acc, c, xv);
+                                }
+                            } else if constexpr (MODE == MODE_ROWCOUNT) {
+                                n_hit += 1;
+                            } else if constexpr (AL_FILL) {
+                                FV fv;
+                                FL fl{};
+                                lane_fill_value(d, fv, fl);
+                                fill_push(fill_b, static_cast<uint32_t>(n_hit), static_cast<int32_t>(col), fv, fl);
+                                n_hit += 1;
+                            }
+                        }
+                    }
+                }
+            }
+        }
+    }
+
+    // a fully listed problem's groups of high bits only: the tile's beta string's
+    // list (the same for every row), each hit at (b', the row's alpha index)
+    if constexpr (AL) {
+        if (full_on) {
+            const uint32_t jb = (ldc(p.tile_row0 + tile) - pd.row0) / pd.al_nlo;
+            const uint32_t a_l = i - pd.row0 - jb * pd.al_nlo;      // (active rows)
+            const uint32_t q0 = ldc(p.bl_off + pd.bl_b0 + jb), q1 = ldc(p.bl_off + pd.bl_b0 + jb + 1);
+            if constexpr (MODE == MODE_ROWCOUNT) {
+                if (active) n_hit_a += q1 - q0;
+            } else {
+                for (uint32_t q = q0; q < q1; ++q) {
+                    const AlEnt<R> en = ldc(static_cast<const AlEnt<R> *>(p.bl_ent) + q);
+                    const uint32_t col = pd.row0 + (en.col & ~AL_SPLIT) * pd.al_nlo + a_l;
+                    CC v = en.v;
+                    float2 l{};
+                    if constexpr (AL_FILL) {
+                        if (p.c_val_lo) l = ldc(p.bl_ent_lo + q).v;
+                    }
+                    if (en.col & AL_SPLIT) {
+                        const SplitDesc sd = ldc(p.al_split + en.g);
+                        if (active) {
+                            v = split_value<R>(v, sd, s, static_cast<const CC *>(p.al_G), pd.al_nlo, a_l);
+                            if constexpr (AL_FILL) {
+                                if (p.c_val_lo) l = split_value<float>(l, sd, s, p.al_G_lo, pd.al_nlo, a_l);
+                            }
+                        }
+                    }
+                    if (active) {
+                        if constexpr (MODE == MODE_MATVEC) {
+                            if constexpr (REALC) fma_acc(acc_a, v.x, x[col]);
+                            else fma_acc(acc_a, v, x[col]);
+                        } else if constexpr (MODE == MODE_FILLCOL) {
+                            fill_put_col(static_cast<uint32_t>(n_hit_a), static_cast<int32_t>(col), static_cast<int32_t>(en.g));
+                            n_hit_a += 1;
+                        } else if constexpr (MODE == MODE_FILL) {
+                            if constexpr (CPLX) fill_put(static_cast<uint32_t>(n_hit_a), v, l);
+                            else fill_put(static_cast<uint32_t>(n_hit_a), v.x, l.x);
+                            n_hit_a += 1;
+                        }
+                    }
+                }
+            }
         }
     }
 
@@ -1644,9 +1824,10 @@ __global__ void k_group_tables(const int32_t *__restrict__ g_tab, const uint64_t
     }
 }
 
-// F of a split group at s: its terms without beta Z bits, in term order
+// F of a split group at s: its terms whose Z masks lie inside keep (the group's
+// own side), in term order
 template <typename R>
-__device__ inline typename C2<R>::T split_f(const KParams &p, const GroupDesc &d, uint64_t s, uint64_t amask) {
+__device__ inline typename C2<R>::T split_f(const KParams &p, const GroupDesc &d, uint64_t s, uint64_t keep) {
     using CC = typename C2<R>::T;
     const CC *coef = static_cast<const CC *>(p.coef);
     CC v;
@@ -1654,7 +1835,7 @@ __device__ inline typename C2<R>::T split_f(const KParams &p, const GroupDesc &d
     v.y = R(0);
     for (uint32_t k = d.k0; k < d.k1; ++k) {
         const uint64_t z = p.z[k];
-        if ((z & ~amask) != 0) continue;
+        if ((z & ~keep) != 0) continue;
         const CC c = coef[k];
         const bool odd = __popcll(s & z) & 1;
         v.x += odd ? -c.x : c.x;
@@ -1663,18 +1844,20 @@ __device__ inline typename C2<R>::T split_f(const KParams &p, const GroupDesc &d
     return v;
 }
 
-// G tables of one problem's split groups: row r (group rg[r], alpha pattern
-// rz[r]) at entry b = the sum over the group's terms with beta Z bits and alpha
-// part rz[r] of c_k (-1)^popc(s_b & z_k), s_b the beta bits of beta block b's
-// first state; at G[roff[r] + b], FP64 and FP32 (each in term order)
-__global__ __launch_bounds__(256) void k_alpha_g(const KParams p64, const KParams p32, const uint64_t *__restrict__ keys,
-                                                 uint32_t nlo, uint32_t nhi, uint64_t amask,
+// G tables of one problem's split groups: row r (group rg[r], pattern rz[r]) at
+// entry e = the sum over the group's terms with Z bits in smask (the other side's
+// bits) and pattern rz[r] outside smask of c_k (-1)^popc(s_e & z_k), s_e = keys[e *
+// stride] & smask; at G[roff[r] + e], FP64 and FP32 (each in term order).  Alpha
+// groups: smask = the beta bits, entries per beta block (stride n_lo); beta groups:
+// smask = the alpha bits, entries per alpha string (stride 1).
+__global__ __launch_bounds__(256) void k_split_g(const KParams p64, const KParams p32, const uint64_t *__restrict__ keys,
+                                                 uint32_t stride, uint32_t n, uint64_t smask,
                                                  const uint32_t *__restrict__ rg, const uint64_t *__restrict__ rz,
                                                  const uint32_t *__restrict__ roff, double2 *__restrict__ G64,
                                                  float2 *__restrict__ G32) {
-    const uint32_t r = blockIdx.y, b = blockIdx.x * 256 + threadIdx.x;
-    if (b >= nhi) return;
-    const uint64_t sb = keys[static_cast<size_t>(b) * nlo] & ~amask, zeta = rz[r];
+    const uint32_t r = blockIdx.y, e = blockIdx.x * 256 + threadIdx.x;
+    if (e >= n) return;
+    const uint64_t se = keys[static_cast<size_t>(e) * stride] & smask, zeta = rz[r];
     const GroupDesc d = p64.gdesc[rg[r]];
     const double2 *c64 = static_cast<const double2 *>(p64.coef);
     const float2 *c32 = static_cast<const float2 *>(p32.coef);
@@ -1682,8 +1865,8 @@ __global__ __launch_bounds__(256) void k_alpha_g(const KParams p64, const KParam
     float2 l = make_float2(0.f, 0.f);
     for (uint32_t k = d.k0; k < d.k1; ++k) {
         const uint64_t z = p64.z[k];
-        if ((z & ~amask) == 0 || (z & amask) != zeta) continue;
-        const bool odd = __popcll(sb & z) & 1;
+        if ((z & smask) == 0 || (z & ~smask) != zeta) continue;
+        const bool odd = __popcll(se & z) & 1;
         const double2 ch = c64[k];
         const float2 cl = c32[k];
         h.x += odd ? -ch.x : ch.x;
@@ -1691,8 +1874,172 @@ __global__ __launch_bounds__(256) void k_alpha_g(const KParams p64, const KParam
         l.x += odd ? -cl.x : cl.x;
         l.y += odd ? -cl.y : cl.y;
     }
-    G64[roff[r] + b] = h;
-    G32[roff[r] + b] = l;
+    G64[roff[r] + e] = h;
+    G32[roff[r] + e] = l;
+}
+
+// Beta lists of one problem: workgroup b takes beta index b (state keys[b * nlo])
+// and the problem's beta groups lg[0..nlg) (they flip only beta bits) in order:
+// group g hits when beta_b ^ x_g is a beta string, at index b'.  Count pass (off
+// NULL): cnt[b] = the hits.  Fill pass: hit k at off[b] + k as (g, b' | AL_SPLIT
+// for a split group, value at keys[b * nlo]: lane_group_value, or F of a split
+// group -- its terms without alpha Z bits), FP64 and FP32.
+__global__ __launch_bounds__(256) void k_beta_lists(const KParams p64, const KParams p32, const uint64_t *__restrict__ keys,
+                                                    uint32_t nlo, uint32_t nhi, uint64_t amask,
+                                                    const uint32_t *__restrict__ lg, uint32_t nlg,
+                                                    uint32_t *__restrict__ cnt, const uint32_t *__restrict__ off,
+                                                    AlEnt<float> *__restrict__ e32, AlEnt<double> *__restrict__ e64) {
+    __shared__ uint32_t wn[4];
+    const uint32_t b = blockIdx.x, tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
+    const uint64_t s = keys[static_cast<size_t>(b) * nlo], sb = s & ~amask;
+    uint32_t base = 0;
+    for (uint32_t j0 = 0; j0 < nlg; j0 += 256) {
+        const uint32_t j = j0 + tid;
+        bool f = false;
+        uint32_t g = 0, col = 0;
+        if (j < nlg) {
+            g = lg[j];
+            const uint64_t t = sb ^ p64.g_x[g];
+            uint32_t lo = 0, n = nhi;          // lower bound of t among the beta strings
+            while (n > 0) {
+                const uint32_t half = n >> 1;
+                if ((keys[static_cast<size_t>(lo + half) * nlo] & ~amask) < t) {
+                    lo += half + 1;
+                    n -= half + 1;
+                } else {
+                    n = half;
+                }
+            }
+            f = lo < nhi && (keys[static_cast<size_t>(lo) * nlo] & ~amask) == t;
+            col = lo;
+        }
+        const uint64_t mk = __ballot(f);
+        if (lane == 0) wn[w] = static_cast<uint32_t>(__popcll(mk));
+        __syncthreads();
+        uint32_t before = base;
+        for (uint32_t u = 0; u < w; ++u) before += wn[u];
+        const uint32_t total = wn[0] + wn[1] + wn[2] + wn[3];
+        if (off && f) {
+            const uint32_t pos = off[b] + before + static_cast<uint32_t>(__popcll(mk & ((1ull << lane) - 1ull)));
+            const GroupDesc gd = p64.gdesc[g];
+            AlEnt<double> eh{};
+            AlEnt<float> el{};
+            eh.g = el.g = g;
+            if (p64.al_split[g].split) {
+                eh.col = el.col = col | AL_SPLIT;
+                eh.v = split_f<double>(p64, gd, s, ~amask);
+                el.v = split_f<float>(p32, gd, s, ~amask);
+            } else {
+                double hx, hy;
+                float lx, ly;
+                lane_group_value<double, false>(p64, gd, s, hx, hy);
+                lane_group_value<float, false>(p32, gd, s, lx, ly);
+                eh.col = el.col = col;
+                eh.v = make_double2(hx, hy);
+                el.v = make_float2(lx, ly);
+            }
+            e64[pos] = eh;
+            e32[pos] = el;
+        }
+        base += total;
+        __syncthreads();          // wn is rewritten by the next chunk
+    }
+    if (!off && tid == 0) cnt[b] = base;
+}
+
+// Partner lists of one problem: workgroup a takes alpha index a and the sorted
+// alpha parts pat[0..npat) of the groups that flip both alpha and beta bits;
+// pattern m is a partner when alpha_a ^ pat[m] is an alpha string, at index a'.
+// Count pass (boff NULL): len[a0 + a] = the partners.  Fill pass: partner k at
+// boff[(a0 + a) >> 6] + 64 k + ((a0 + a) & 63) as (pat[m], m, a', 0), ascending m.
+__global__ __launch_bounds__(256) void k_alpha_partners(const uint64_t *__restrict__ keys, uint32_t nlo, uint64_t amask,
+                                                        const uint64_t *__restrict__ pat, uint32_t npat, uint32_t a0,
+                                                        uint32_t *__restrict__ len, const int64_t *__restrict__ boff,
+                                                        uint4 *__restrict__ ent) {
+    __shared__ uint32_t wn[4];
+    const uint32_t a = blockIdx.x, tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
+    const uint64_t sa = keys[a] & amask;
+    const uint32_t ag = a0 + a;
+    uint32_t base = 0;
+    for (uint32_t m0 = 0; m0 < npat; m0 += 256) {
+        const uint32_t m = m0 + tid;
+        bool f = false;
+        uint32_t col = 0;
+        if (m < npat) {
+            const uint64_t t = sa ^ pat[m];
+            uint32_t lo = 0, n = nlo;          // lower bound of t
+            while (n > 0) {
+                const uint32_t half = n >> 1;
+                if ((keys[lo + half] & amask) < t) {
+                    lo += half + 1;
+                    n -= half + 1;
+                } else {
+                    n = half;
+                }
+            }
+            f = lo < nlo && (keys[lo] & amask) == t;
+            col = lo;
+        }
+        const uint64_t mk = __ballot(f);
+        if (lane == 0) wn[w] = static_cast<uint32_t>(__popcll(mk));
+        __syncthreads();
+        uint32_t before = base;
+        for (uint32_t u = 0; u < w; ++u) before += wn[u];
+        const uint32_t total = wn[0] + wn[1] + wn[2] + wn[3];
+        if (boff && f) {
+            const uint32_t k = before + static_cast<uint32_t>(__popcll(mk & ((1ull << lane) - 1ull)));
+            ent[boff[ag >> 6] + 64 * static_cast<int64_t>(k) + (ag & 63u)] =
+                make_uint4(static_cast<uint32_t>(pat[m]), m, col, 0u);
+        }
+        base += total;
+        __syncthreads();          // wn is rewritten by the next chunk
+    }
+    if (!boff && tid == 0) len[ag] = base;
+}
+
+// Beta partners of one problem: workgroup b takes beta index b (state keys[b *
+// nlo]) and the sorted beta parts bpat[0..nbpat) of the mapped groups; pattern m
+// is a partner when its beta string ^ bpat[m] is a beta string, at index b'.
+// Count pass (off NULL): cnt[b] = the partners.  Fill pass: partner k at
+// off[b] + k as (m, b'), ascending m.
+__global__ __launch_bounds__(256) void k_beta_partners(const uint64_t *__restrict__ keys, uint32_t nlo, uint32_t nhi,
+                                                       uint64_t amask, const uint64_t *__restrict__ bpat, uint32_t nbpat,
+                                                       uint32_t *__restrict__ cnt, const uint32_t *__restrict__ off,
+                                                       uint2 *__restrict__ ent) {
+    __shared__ uint32_t wn[4];
+    const uint32_t b = blockIdx.x, tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
+    const uint64_t sb = keys[static_cast<size_t>(b) * nlo] & ~amask;
+    uint32_t base = 0;
+    for (uint32_t m0 = 0; m0 < nbpat; m0 += 256) {
+        const uint32_t m = m0 + tid;
+        bool f = false;
+        uint32_t col = 0;
+        if (m < nbpat) {
+            const uint64_t t = sb ^ bpat[m];
+            uint32_t lo = 0, n = nhi;          // lower bound of t among the beta strings
+            while (n > 0) {
+                const uint32_t half = n >> 1;
+                if ((keys[static_cast<size_t>(lo + half) * nlo] & ~amask) < t) {
+                    lo += half + 1;
+                    n -= half + 1;
+                } else {
+                    n = half;
+                }
+            }
+            f = lo < nhi && (keys[static_cast<size_t>(lo) * nlo] & ~amask) == t;
+            col = lo;
+        }
+        const uint64_t mk = __ballot(f);
+        if (lane == 0) wn[w] = static_cast<uint32_t>(__popcll(mk));
+        __syncthreads();
+        uint32_t before = base;
+        for (uint32_t u = 0; u < w; ++u) before += wn[u];
+        const uint32_t total = wn[0] + wn[1] + wn[2] + wn[3];
+        if (off && f) ent[off[b] + before + static_cast<uint32_t>(__popcll(mk & ((1ull << lane) - 1ull)))] = make_uint2(m, col);
+        base += total;
+        __syncthreads();          // wn is rewritten by the next chunk
+    }
+    if (!off && tid == 0) cnt[b] = base;
 }
 
 // Alpha lists of one problem: workgroup a takes alpha index a (row a of the
@@ -2960,6 +3307,14 @@ struct ProbHost {            // host record of one problem of a plan
     uint32_t al_lg0 = 0, al_lgn = 0;
     uint32_t al_nhi = 0;       // high strings of the product
     uint32_t al_r0 = 0, al_rn = 0;   // the problem's G-table rows [al_r0, al_r0 + al_rn) of the plan's list
+    uint32_t al_pat0 = 0, al_npat = 0;   // its alpha patterns [al_pat0, al_pat0 + al_npat) of the plan's list,
+    uint32_t al_bpat0 = 0, al_nbpat = 0; //   its beta patterns of the plan's list,
+    uint32_t pm_base = UINT32_MAX;       //   its partner map's first element
+    uint32_t bp_b0 = 0;                  //   and its first beta index of the beta-partner offsets
+    uint32_t full = 0;                   // beta lists: every group listed (no class searched),
+    uint32_t bl_b0 = 0;                  //   the first beta index of their offsets,
+    uint32_t bl_lg0 = 0, bl_lgn = 0;     //   the problem's beta groups of the plan's list,
+    uint32_t bl_r0 = 0, bl_rn = 0;       //   and its G-table rows (entries per alpha string)
 };
 
 struct HitCache {            // sliced-ELL cache of the hits (column, C_j(s_i)) in one precision
@@ -3076,6 +3431,24 @@ struct rqx_plan {
     double2 *al_G64 = nullptr;      // G tables, FP64 and FP32
     float2 *al_G32 = nullptr;
     int64_t al_gelems = 0;
+    void *al_pblock = nullptr;      // partner lists and class maps (own allocations)
+    void *al_pent_block = nullptr;
+    uint32_t *al_plen = nullptr;
+    int64_t *al_pboff = nullptr;
+    uint4 *al_pent = nullptr;
+    uint32_t *al_bpoff = nullptr;
+    uint2 *al_bpent = nullptr;
+    int32_t *al_pmap = nullptr;
+    void *bl_block = nullptr;       // beta lists (own allocation): offsets, listed groups, G-table rows, entries
+    uint32_t *bl_off = nullptr;
+    uint32_t *bl_lg = nullptr;
+    uint32_t *bl_rg = nullptr;
+    uint32_t *bl_roff = nullptr;
+    uint64_t *bl_rz = nullptr;
+    AlEnt<float> *bl_e32 = nullptr;
+    AlEnt<double> *bl_e64 = nullptr;
+    void *bl_ent_block = nullptr;   // their entries (own allocation)
+    int64_t al_pslots = 0;
 };
 
 namespace {
@@ -3348,6 +3721,15 @@ KParams make_params(const rqx_plan *p, bool fp64) {
     k.al_split = p->al_split;
     k.al_G = fp64 ? static_cast<const void *>(p->al_G64) : static_cast<const void *>(p->al_G32);
     k.al_G_lo = p->al_G32;
+    k.al_plen = p->al_plen;
+    k.al_pboff = p->al_pboff;
+    k.al_pent = p->al_pent;
+    k.al_bpoff = p->al_bpoff;
+    k.al_bpent = p->al_bpent;
+    k.al_pmap = p->al_pmap;
+    k.bl_off = p->bl_off;
+    k.bl_ent = fp64 ? static_cast<const void *>(p->bl_e64) : static_cast<const void *>(p->bl_e32);
+    k.bl_ent_lo = p->bl_e32;
     return k;
 }
 
@@ -3360,8 +3742,8 @@ hipError_t launch_alpha_lists(const rqx_plan *p, bool fill, hipStream_t st) {
         // the G tables of the split groups (fill pass), at most 65535 rows a launch
         for (uint32_t r = 0; fill && r < ph.al_rn; r += 65535u) {
             const uint32_t nr = std::min(ph.al_rn - r, 65535u);
-            hipLaunchKernelGGL(k_alpha_g, dim3((ph.al_nhi + 255) / 256, nr), dim3(256), 0, st, p64, p32,
-                               p->keys + ph.row_base, ph.al_nlo, ph.al_nhi, (1ull << ph.a_bits) - 1ull,
+            hipLaunchKernelGGL(k_split_g, dim3((ph.al_nhi + 255) / 256, nr), dim3(256), 0, st, p64, p32,
+                               p->keys + ph.row_base, ph.al_nlo, ph.al_nhi, ~((1ull << ph.a_bits) - 1ull),
                                p->al_rg + ph.al_r0 + r, p->al_rz + ph.al_r0 + r, p->al_roff + ph.al_r0 + r, p->al_G64,
                                p->al_G32);
             const hipError_t e = hipGetLastError();
@@ -3394,7 +3776,9 @@ hipError_t build_alpha_lists(rqx_plan *p, uint32_t n_alpha, const std::vector<ui
     auto none = [&]() {
         for (size_t k = 0; k < p->probs.size(); ++k) {
             p->probs[k].al_nlo = 0;
+            p->probs[k].full = 0;
             desc[k].al_nlo = 0;
+            desc[k].full = 0;
         }
         hipError_t e = hipMemcpyAsync(p->prob, desc.data(), desc.size() * sizeof(ProbDesc), hipMemcpyHostToDevice, st);
         if (e == hipSuccess) e = hipStreamSynchronize(st);   // desc is the caller's host array
@@ -3459,6 +3843,197 @@ hipError_t build_alpha_lists(rqx_plan *p, uint32_t n_alpha, const std::vector<ui
     e = hipMemcpyAsync(p->al_boff, boff.data(), boff.size() * 8, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = launch_alpha_lists(p, true, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);   // boff is released on return
+    return e;
+}
+
+// at most this many class-map elements (int32) per plan
+constexpr int64_t CMAP_MAX_ELEMS = int64_t(1) << 24;
+
+// Partner lists of the problems with a partner map (ph.pm_base set): alpha
+// partners in the blocked layout of the alpha lists, beta partners in offsets per
+// beta index; then the map.  Without them (no alpha lists, no room) the problems'
+// descriptors are rewritten without a map and their classes are searched.
+hipError_t build_partners(rqx_plan *p, uint32_t n_alpha, uint32_t n_boff, const std::vector<uint64_t> &apats,
+                          const std::vector<uint64_t> &bpats, const std::vector<int32_t> &pmap,
+                          std::vector<ProbDesc> &desc, hipStream_t st) {
+    if (pmap.empty()) return hipSuccess;
+    auto none = [&]() {
+        for (size_t k = 0; k < p->probs.size(); ++k) {
+            p->probs[k].pm_base = UINT32_MAX;
+            p->probs[k].full = 0;
+            desc[k].pm_base = UINT32_MAX;
+            desc[k].full = 0;
+        }
+        hipError_t e = hipMemcpyAsync(p->prob, desc.data(), desc.size() * sizeof(ProbDesc), hipMemcpyHostToDevice, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);   // desc is the caller's host array
+        return e;
+    };
+    if (!p->al_ent_block || n_alpha == 0 || apats.empty() || bpats.empty()) return none();
+    const size_t o_len = 0, o_ap = align_up(size_t(n_alpha) * 4), o_bp = o_ap + align_up(apats.size() * 8),
+                 o_map = o_bp + align_up(bpats.size() * 8), o_boff = o_map + align_up(pmap.size() * 4),
+                 bytes = o_boff + align_up(size_t(n_boff) * 4);
+    hipError_t e = hipMalloc(&p->al_pblock, bytes);
+    if (e != hipSuccess) return e;
+    char *b = static_cast<char *>(p->al_pblock);
+    uint32_t *plen = reinterpret_cast<uint32_t *>(b + o_len), *bpoff = reinterpret_cast<uint32_t *>(b + o_boff);
+    uint64_t *ap = reinterpret_cast<uint64_t *>(b + o_ap), *bp = reinterpret_cast<uint64_t *>(b + o_bp);
+    int32_t *map = reinterpret_cast<int32_t *>(b + o_map);
+    e = fill_async(plen, int64_t(0), static_cast<int64_t>(n_alpha), 0u, st);
+    if (e == hipSuccess) e = fill_async(bpoff, int64_t(0), static_cast<int64_t>(n_boff), 0u, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(ap, apats.data(), apats.size() * 8, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(bp, bpats.data(), bpats.size() * 8, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(map, pmap.data(), pmap.size() * 4, hipMemcpyHostToDevice, st);
+    // count (boff NULL) or fill passes of both partner kinds
+    auto launch = [&](const int64_t *aboff, uint4 *aent, const uint32_t *boff, uint2 *bent) -> hipError_t {
+        for (const ProbHost &ph : p->probs) {
+            if (ph.pm_base == UINT32_MAX) continue;
+            const uint64_t amask = (1ull << ph.a_bits) - 1ull;
+            hipLaunchKernelGGL(k_alpha_partners, dim3(ph.al_nlo), dim3(256), 0, st, p->keys + ph.row_base, ph.al_nlo,
+                               amask, ap + ph.al_pat0, ph.al_npat, ph.al_a0, plen, aboff, aent);
+            hipError_t le = hipGetLastError();
+            if (le != hipSuccess) return le;
+            hipLaunchKernelGGL(k_beta_partners, dim3(ph.al_nhi), dim3(256), 0, st, p->keys + ph.row_base, ph.al_nlo,
+                               ph.al_nhi, amask, bp + ph.al_bpat0, ph.al_nbpat, bpoff + ph.bp_b0,
+                               boff ? boff + ph.bp_b0 : nullptr, bent);
+            le = hipGetLastError();
+            if (le != hipSuccess) return le;
+        }
+        return hipSuccess;
+    };
+    if (e == hipSuccess) e = launch(nullptr, nullptr, nullptr, nullptr);
+    std::vector<uint32_t> len(n_alpha), bcnt(n_boff);
+    if (e == hipSuccess) e = hipMemcpyAsync(len.data(), plen, size_t(n_alpha) * 4, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(bcnt.data(), bpoff, size_t(n_boff) * 4, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return e;
+    const uint32_t n_blk = n_alpha / 64;
+    std::vector<int64_t> aoff(n_blk);
+    int64_t slots = 0;
+    for (uint32_t k = 0; k < n_blk; ++k) {
+        uint32_t w = 0;
+        for (uint32_t u = 0; u < 64; ++u) w = std::max(w, len[64 * k + u]);
+        aoff[k] = slots;
+        slots += 64 * static_cast<int64_t>(w);
+    }
+    // beta offsets: each problem's nhi counts become nhi + 1 offsets (exclusive scan)
+    std::vector<uint32_t> boffs(n_boff, 0);
+    int64_t bents = 0;
+    for (const ProbHost &ph : p->probs) {
+        if (ph.pm_base == UINT32_MAX) continue;
+        for (uint32_t j = 0; j <= ph.al_nhi; ++j) {
+            boffs[ph.bp_b0 + j] = static_cast<uint32_t>(bents);
+            if (j < ph.al_nhi) bents += bcnt[ph.bp_b0 + j];
+        }
+    }
+    if (slots * 16 + bents * 8 > ALPHA_LIST_MAX_BYTES || bents > 0xFFFFFFFFLL) return none();   // no room
+    const size_t o_aent = align_up(size_t(n_blk) * 8), o_bent = o_aent + align_up(size_t(std::max<int64_t>(slots, 1)) * 16);
+    e = hipMalloc(&p->al_pent_block, o_bent + size_t(std::max<int64_t>(bents, 1)) * 8);
+    if (e != hipSuccess) return e;
+    char *eb = static_cast<char *>(p->al_pent_block);
+    int64_t *pboff = reinterpret_cast<int64_t *>(eb);
+    uint4 *pent = reinterpret_cast<uint4 *>(eb + o_aent);
+    uint2 *bpent = reinterpret_cast<uint2 *>(eb + o_bent);
+    e = hipMemcpyAsync(pboff, aoff.data(), aoff.size() * 8, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(bpoff, boffs.data(), boffs.size() * 4, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = launch(pboff, pent, bpoff, bpent);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);    // the host arrays are released on return
+    if (e != hipSuccess) return e;
+    p->al_plen = plen;
+    p->al_pboff = pboff;
+    p->al_pent = pent;
+    p->al_bpoff = bpoff;
+    p->al_bpent = bpent;
+    p->al_pmap = map;
+    p->al_pslots = slots;
+    return hipSuccess;
+}
+
+// the beta lists' G-table rows and entries (fill = true; also after a
+// coefficient update), or their counts per beta index into bl_off (fill = false)
+hipError_t launch_beta_lists(const rqx_plan *p, bool fill, hipStream_t st) {
+    const KParams p64 = make_params(p, true), p32 = make_params(p, false);
+    for (const ProbHost &ph : p->probs) {
+        if (!ph.full) continue;
+        const uint64_t amask = (1ull << ph.a_bits) - 1ull;
+        for (uint32_t r = 0; fill && r < ph.bl_rn; r += 65535u) {
+            const uint32_t nr = std::min(ph.bl_rn - r, 65535u);
+            hipLaunchKernelGGL(k_split_g, dim3((ph.al_nlo + 255) / 256, nr), dim3(256), 0, st, p64, p32,
+                               p->keys + ph.row_base, 1u, ph.al_nlo, amask, p->bl_rg + ph.bl_r0 + r,
+                               p->bl_rz + ph.bl_r0 + r, p->bl_roff + ph.bl_r0 + r, p->al_G64, p->al_G32);
+            const hipError_t e = hipGetLastError();
+            if (e != hipSuccess) return e;
+        }
+        hipLaunchKernelGGL(k_beta_lists, dim3(ph.al_nhi), dim3(256), 0, st, p64, p32, p->keys + ph.row_base, ph.al_nlo,
+                           ph.al_nhi, amask, p->bl_lg + ph.bl_lg0, ph.bl_lgn, p->bl_off + ph.bl_b0,
+                           fill ? p->bl_off + ph.bl_b0 : nullptr, p->bl_e32, p->bl_e64);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+// Beta lists of the problems in the fully listed mode (ph.full): counts per beta
+// index, their offsets on the host, the G rows and the entries.  Without them the
+// problems' descriptors are rewritten out of that mode (their classes searched).
+hipError_t build_beta_lists(rqx_plan *p, uint32_t n_boff, const std::vector<uint32_t> &lg,
+                            const std::vector<uint32_t> &rg, const std::vector<uint64_t> &rz,
+                            const std::vector<uint32_t> &roff, std::vector<ProbDesc> &desc, hipStream_t st) {
+    if (lg.empty()) return hipSuccess;
+    bool any = false;
+    for (const ProbHost &ph : p->probs) any = any || ph.full;
+    if (!any) return hipSuccess;
+    auto none = [&]() {
+        for (size_t k = 0; k < p->probs.size(); ++k) {
+            p->probs[k].full = 0;
+            desc[k].full = 0;
+        }
+        hipError_t e = hipMemcpyAsync(p->prob, desc.data(), desc.size() * sizeof(ProbDesc), hipMemcpyHostToDevice, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);   // desc is the caller's host array
+        return e;
+    };
+    if (!p->al_ent_block) return none();
+    const size_t nr = rg.size();
+    const size_t o_off = 0, o_lg = align_up(size_t(n_boff) * 4), o_rg = o_lg + align_up(lg.size() * 4),
+                 o_roff = o_rg + align_up(std::max<size_t>(nr, 1) * 4), o_rz = o_roff + align_up(std::max<size_t>(nr, 1) * 4),
+                 bytes = o_rz + align_up(std::max<size_t>(nr, 1) * 8);
+    hipError_t e = hipMalloc(&p->bl_block, bytes);
+    if (e != hipSuccess) return e;
+    char *b = static_cast<char *>(p->bl_block);
+    p->bl_off = reinterpret_cast<uint32_t *>(b + o_off);
+    p->bl_lg = reinterpret_cast<uint32_t *>(b + o_lg);
+    p->bl_rg = reinterpret_cast<uint32_t *>(b + o_rg);
+    p->bl_roff = reinterpret_cast<uint32_t *>(b + o_roff);
+    p->bl_rz = reinterpret_cast<uint64_t *>(b + o_rz);
+    e = fill_async(p->bl_off, int64_t(0), static_cast<int64_t>(n_boff), 0u, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(p->bl_lg, lg.data(), lg.size() * 4, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess && nr) e = hipMemcpyAsync(p->bl_rg, rg.data(), nr * 4, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess && nr) e = hipMemcpyAsync(p->bl_roff, roff.data(), nr * 4, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess && nr) e = hipMemcpyAsync(p->bl_rz, rz.data(), nr * 8, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = launch_beta_lists(p, false, st);
+    std::vector<uint32_t> cnt(n_boff);
+    if (e == hipSuccess) e = hipMemcpyAsync(cnt.data(), p->bl_off, size_t(n_boff) * 4, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return e;
+    std::vector<uint32_t> off(n_boff, 0);
+    int64_t total = 0;
+    for (const ProbHost &ph : p->probs) {
+        if (!ph.full) continue;
+        for (uint32_t j = 0; j <= ph.al_nhi; ++j) {
+            off[ph.bl_b0 + j] = static_cast<uint32_t>(total);
+            if (j < ph.al_nhi) total += cnt[ph.bl_b0 + j];
+        }
+    }
+    if (total * static_cast<int64_t>(sizeof(AlEnt<float>) + sizeof(AlEnt<double>)) > ALPHA_LIST_MAX_BYTES ||
+        total > 0xFFFFFFFFLL)
+        return none();
+    const size_t o_e64 = align_up(size_t(std::max<int64_t>(total, 1)) * sizeof(AlEnt<float>));
+    e = hipMalloc(&p->bl_ent_block, o_e64 + size_t(std::max<int64_t>(total, 1)) * sizeof(AlEnt<double>));
+    if (e != hipSuccess) return e;
+    p->bl_e32 = static_cast<AlEnt<float> *>(p->bl_ent_block);
+    p->bl_e64 = reinterpret_cast<AlEnt<double> *>(static_cast<char *>(p->bl_ent_block) + o_e64);
+    e = hipMemcpyAsync(p->bl_off, off.data(), off.size() * 4, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = launch_beta_lists(p, true, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);    // off is released on return
     return e;
 }
 
@@ -4316,6 +4891,12 @@ int build_plan(rqx_plan **out, const std::vector<ProblemIn> &in, int64_t n_total
     std::vector<uint32_t> al_rg, al_roff;             // G-table rows: group, first element
     std::vector<uint64_t> al_rz;                      //   and alpha pattern
     int64_t al_gelems = 0;                            // G-table elements
+    std::vector<uint32_t> bl_lg, bl_rg, bl_roff;      // beta lists: listed groups; G-table rows: group, first element
+    std::vector<uint64_t> bl_rz;                      //   and beta pattern
+    uint32_t bl_nboff = 0;                            //   and offsets
+    std::vector<int32_t> al_pmap;                     // partner maps of every problem
+    std::vector<uint64_t> al_apats, al_bpats;         //   their alpha and beta patterns
+    uint32_t al_nboff = 0;                            //   and beta-partner offsets
     int64_t n_tab = 0;
     int64_t row_base = 0, term_base = 0;
     int32_t grp_base = 0;
@@ -4484,6 +5065,115 @@ int build_plan(rqx_plan **out, const std::vector<ProblemIn> &in, int64_t n_total
                         al_pg.push_back(static_cast<uint32_t>(grp_base) + j);
             }
         }
+        if (ph.al_nlo > 0 && ph.a_bits <= 32) {
+            // the partner map: a class above the alpha classes whose groups a lane
+            // evaluates alone (mapped) takes its hits of the groups that flip alpha
+            // bits from the row's alpha partners (alpha parts X_a of those groups
+            // with alpha ^ x_a an alpha string) and the tile's beta partners (beta
+            // parts X_b, likewise), the map giving the group of each (x_b, x_a)
+            const uint64_t amask = (1ull << ph.a_bits) - 1ull;
+            std::vector<uint64_t> Xa, Xb;
+            std::vector<int32_t> mcls;
+            for (int32_t c = 0; c < ph.n_classes; ++c) {
+                if ((sg.cls_hi[c] >> (ph.a_bits - sg.L)) == 0) continue;                       // an alpha class
+                // (a group of high bits only, the class's first, is taken on its own)
+                bool lane = true, any = false;
+                for (uint32_t j = sg.cls_goff[c]; j < sg.cls_goff[c + 1] && lane; ++j)
+                    if ((q.xmask[j] & amask) != 0) {
+                        any = true;
+                        lane = g_tab[static_cast<size_t>(grp_base) + j] >= 0 ||
+                               q.grp_off[j + 1] - q.grp_off[j] < TP_MIN_TERMS;
+                    }
+                if (!any || !lane) continue;
+                for (uint32_t j = sg.cls_goff[c]; j < sg.cls_goff[c + 1]; ++j)
+                    if ((q.xmask[j] & amask) != 0) {
+                        Xa.push_back(q.xmask[j] & amask);
+                        Xb.push_back(q.xmask[j] & ~amask);
+                    }
+                mcls.push_back(c);
+            }
+            for (std::vector<uint64_t> *X : {&Xa, &Xb}) {
+                std::sort(X->begin(), X->end());
+                X->erase(std::unique(X->begin(), X->end()), X->end());
+            }
+            const int64_t elems = static_cast<int64_t>(Xa.size()) * static_cast<int64_t>(Xb.size());
+            if (!mcls.empty() && static_cast<int64_t>(al_pmap.size()) + elems <= CMAP_MAX_ELEMS) {
+                ph.pm_base = static_cast<uint32_t>(al_pmap.size());
+                ph.al_pat0 = static_cast<uint32_t>(al_apats.size());
+                ph.al_npat = static_cast<uint32_t>(Xa.size());
+                ph.al_bpat0 = static_cast<uint32_t>(al_bpats.size());
+                ph.al_nbpat = static_cast<uint32_t>(Xb.size());
+                ph.bp_b0 = al_nboff;
+                al_nboff += ph.al_nhi + 1;
+                al_apats.insert(al_apats.end(), Xa.begin(), Xa.end());
+                al_bpats.insert(al_bpats.end(), Xb.begin(), Xb.end());
+                al_pmap.resize(al_pmap.size() + static_cast<size_t>(elems), -1);
+                for (int32_t c : mcls) {
+                    cls_lane_all[static_cast<size_t>(ph.cls_base) + c] |= 4u;
+                    for (uint32_t j = sg.cls_goff[c]; j < sg.cls_goff[c + 1]; ++j) {
+                        const uint64_t xa = q.xmask[j] & amask;
+                        if (xa == 0) continue;
+                        const size_t ia = static_cast<size_t>(std::lower_bound(Xa.begin(), Xa.end(), xa) - Xa.begin());
+                        const size_t ib = static_cast<size_t>(
+                            std::lower_bound(Xb.begin(), Xb.end(), q.xmask[j] & ~amask) - Xb.begin());
+                        al_pmap[ph.pm_base + ib * Xa.size() + ia] = static_cast<int32_t>(grp_base) + static_cast<int32_t>(j);
+                    }
+                }
+            }
+        }
+        if (ph.al_nlo > 0 && ph.a_bits <= 32) {
+            // beta lists: when the alpha classes search nothing and every other class
+            // is mapped or holds only a group of high bits, those groups come from
+            // per-beta lists (values, or split values with G tables per alpha string)
+            // and the tile kernel searches nothing at all
+            const uint64_t amask = (1ull << ph.a_bits) - 1ull;
+            bool full = al_pg.size() == al_pg_off[static_cast<size_t>(ph.cls_base)];
+            std::vector<uint32_t> blg;
+            std::vector<std::vector<uint64_t>> bpat;
+            std::vector<uint64_t> pat;
+            for (int32_t c = 0; c < ph.n_classes && full; ++c) {
+                if ((sg.cls_hi[c] >> (ph.a_bits - sg.L)) == 0) continue;                       // an alpha class
+                uint32_t j = sg.cls_goff[c];
+                if ((q.xmask[j] & amask) == 0) {         // the class's group of high bits only
+                    if (!split_patterns(q, static_cast<int32_t>(j), ~amask, pat)) full = false;
+                    blg.push_back(static_cast<uint32_t>(grp_base) + j);
+                    bpat.push_back(pat);
+                    ++j;
+                }
+                if (j < sg.cls_goff[c + 1] && !(cls_lane_all[static_cast<size_t>(ph.cls_base) + c] & 4u)) full = false;
+            }
+            int64_t gel = al_gelems;
+            for (const std::vector<uint64_t> &P : bpat) gel += static_cast<int64_t>(P.size()) * ph.al_nlo;
+            if (full && !blg.empty() && gel <= 0xFFFFFFFFLL) {
+                ph.full = 1;
+                ph.bl_b0 = bl_nboff;
+                bl_nboff += ph.al_nhi + 1;
+                ph.bl_lg0 = static_cast<uint32_t>(bl_lg.size());
+                ph.bl_lgn = static_cast<uint32_t>(blg.size());
+                ph.bl_r0 = static_cast<uint32_t>(bl_rg.size());
+                for (size_t u = 0; u < blg.size(); ++u) {
+                    bl_lg.push_back(blg[u]);
+                    // split when some term holds alpha Z bits, or when the group is too long
+                    // for one lane (F is then its whole sum); else its value as listed
+                    const bool lane = g_tab[blg[u]] >= 0 ||
+                                      q.grp_off[blg[u] - grp_base + 1] - q.grp_off[blg[u] - grp_base] < TP_MIN_TERMS;
+                    if (!bpat[u].empty() || !lane) {        // split (F, and G rows when it has patterns)
+                        SplitDesc &sd = al_split[blg[u]];
+                        sd.split = 1;
+                        sd.n_p = static_cast<uint32_t>(bpat[u].size());
+                        sd.grow = static_cast<uint32_t>(al_gelems);
+                        for (size_t v = 0; v < bpat[u].size(); ++v) {
+                            sd.zeta[v] = bpat[u][v];
+                            bl_rg.push_back(blg[u]);
+                            bl_rz.push_back(bpat[u][v]);
+                            bl_roff.push_back(static_cast<uint32_t>(al_gelems));
+                            al_gelems += ph.al_nlo;
+                        }
+                    }
+                }
+                ph.bl_rn = static_cast<uint32_t>(bl_rg.size() - ph.bl_r0);
+            }
+        }
         for (int64_t t = 0; t < q.n_terms; ++t) {
             zall.push_back(q.zmask[t]);
             c64.push_back(q.coef[2 * t]);
@@ -4498,6 +5188,11 @@ int build_plan(rqx_plan **out, const std::vector<ProblemIn> &in, int64_t n_total
         desc[k].al_nlo = ph.al_nlo;
         desc[k].al_a0 = ph.al_a0;
         desc[k].al_nhi = ph.al_nhi;
+        desc[k].pm_base = ph.pm_base;
+        desc[k].pm_na = ph.al_npat;
+        desc[k].bp_b0 = ph.bp_b0;
+        desc[k].bl_b0 = ph.bl_b0;
+        desc[k].full = ph.full;
         desc[k].prod = ph.a_bits > ph.L ? 1u : 0u;
         p->max_seg = std::max(p->max_seg, ph.max_seg);
         p->is_real = p->is_real && ph.is_real;
@@ -4629,6 +5324,10 @@ int build_plan(rqx_plan **out, const std::vector<ProblemIn> &in, int64_t n_total
         (void)hipFree(raw->hash_block);
         (void)hipFree(raw->al_block);
         (void)hipFree(raw->al_ent_block);
+        (void)hipFree(raw->al_pblock);
+        (void)hipFree(raw->al_pent_block);
+        (void)hipFree(raw->bl_block);
+        (void)hipFree(raw->bl_ent_block);
     };
 
     auto up = [&](void *dst, const void *src, size_t bytes) -> hipError_t {
@@ -4691,6 +5390,8 @@ int build_plan(rqx_plan **out, const std::vector<ProblemIn> &in, int64_t n_total
     if (e == hipSuccess) e = build_targets(raw, st);
     if (e == hipSuccess)
         e = build_alpha_lists(raw, al_alpha, al_lg, al_pg_off, al_pg, al_split, al_rg, al_rz, al_roff, al_gelems, desc, st);
+    if (e == hipSuccess) e = build_partners(raw, al_alpha, al_nboff, al_apats, al_bpats, al_pmap, desc, st);
+    if (e == hipSuccess) e = build_beta_lists(raw, bl_nboff, bl_lg, bl_rg, bl_rz, bl_roff, desc, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e == hipSuccess) {
         int occ = 0;
@@ -4957,6 +5658,14 @@ int rqx_plan_destroy(rqx_plan *plan) {
     if (e == hipSuccess) e = ea;
     const hipError_t ee = hipFree(plan->al_ent_block);
     if (e == hipSuccess) e = ee;
+    const hipError_t ep = hipFree(plan->al_pblock);
+    if (e == hipSuccess) e = ep;
+    const hipError_t eq = hipFree(plan->al_pent_block);
+    if (e == hipSuccess) e = eq;
+    const hipError_t eb = hipFree(plan->bl_block);
+    if (e == hipSuccess) e = eb;
+    const hipError_t ebe = hipFree(plan->bl_ent_block);
+    if (e == hipSuccess) e = ebe;
     for (HitCache &c : plan->cache) {
         const hipError_t ec = free_cache(c);
         if (e == hipSuccess) e = ec;
@@ -5213,6 +5922,7 @@ int rqx_plan_update_coef(rqx_plan *p, const double *coef, int32_t is_real, rqx_s
     launch_group_tables(p, st);
     HIP_TRY(hipGetLastError());
     if (p->al_ent_block) HIP_TRY(launch_alpha_lists(p, true, st));   // values from the new tables
+    if (p->bl_ent_block) HIP_TRY(launch_beta_lists(p, true, st));
     for (const ProbHost &ph : p->probs) {
         if (ph.n_states == 0) continue;
         const unsigned blocks = static_cast<unsigned>((ph.n_states + 255) / 256);
