@@ -380,6 +380,7 @@ def line_for(w, plan, leg, dt_name, steps, warmup, traffic):
     """The JSON line's fields for one workload's matvec leg."""
     useful, ms = leg['useful_bytes'], leg['ms']
     achieved = useful / (ms * 1e-3) / 1e9
+    listed = all(pr['list_mode'] == 2 for pr in plan.problems)
     ach_mf = leg['alg_bytes'] / (leg['ms_free'] * 1e-3) / 1e9
     cinfo = leg['cache']
     t_cached = traffic.get('cached') if isinstance(traffic, dict) else None
@@ -408,10 +409,14 @@ def line_for(w, plan, leg, dt_name, steps, warmup, traffic):
         'matrix_free': {
             'ms_per_matvec': round(leg['ms_free'], 4),
             'model_bytes_per_matvec': leg['alg_bytes'],
-            'roofline': {'bound': 'lds', 'achieved': round(ach_mf, 2), 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
-                         'frac': round(ach_mf / HBM_PEAK_GBS, 4),
+            'list_mode': [pr['list_mode'] for pr in plan.problems],
+            'roofline': {'bound': 'hbm' if listed else 'lds', 'achieved': round(ach_mf, 2), 'peak': HBM_PEAK_GBS,
+                         'unit': 'GB/s', 'frac': round(ach_mf / HBM_PEAK_GBS, 4),
                          'traffic': t_free['hbm_bytes_per_launch'] if t_free else None,
-                         'note': 'model bytes count index probes that LDS/L2 serve; measured traffic is HBM bytes'},
+                         'note': ('every hit from the product-subspace lists (no index probe): model bytes are '
+                                  'keys, gathered x entries, y and term tables; measured traffic is HBM bytes')
+                         if listed else
+                         'model bytes count index probes that LDS/L2 serve; measured traffic is HBM bytes'},
         },
     }
 

@@ -82,9 +82,14 @@ typedef struct {
     int32_t section_bits;   /* hit-cache section split: groups whose X mask has no bit below it
                                form the first section (>= split_bits; larger when the states are a
                                product of low and high strings at that bit) */
-    int32_t alpha_rows;     /* > 0: the matvec takes the hits of the groups that flip only bits
-                               below section_bits, and whose Z masks hold only such bits, from
-                               per-low-string lists of the product's alpha_rows low strings */
+    int32_t alpha_rows;     /* > 0: the states are the product of alpha_rows low strings (below
+                               section_bits) and rows / alpha_rows high strings, and the matvec
+                               takes the hits of the groups that flip only low bits from
+                               per-low-string lists */
+    int32_t list_mode;      /* 0: every class searched; 1: low-bit-only groups listed; 2: every
+                               group listed (low-bit-only, high-bit-only and mixed groups through
+                               per-string lists and a pattern map), no search at all */
+    int32_t reserved;
 } rqx_problem_info;
 
 /* Library version (major*10000 + minor*100 + patch). */

@@ -96,6 +96,8 @@ class ProblemInfo(ctypes.Structure):
         ('cost', ctypes.c_double),
         ('section_bits', ctypes.c_int32),
         ('alpha_rows', ctypes.c_int32),
+        ('list_mode', ctypes.c_int32),
+        ('reserved', ctypes.c_int32),
     ]
 
     def as_dict(self):

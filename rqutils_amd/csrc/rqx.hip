@@ -5724,6 +5724,8 @@ int rqx_plan_get_problem(const rqx_plan *p, int32_t k, rqx_problem_info *out) {
     out->cost = ph.cost;
     out->section_bits = ph.a_bits;
     out->alpha_rows = static_cast<int32_t>(ph.al_nlo);
+    out->list_mode = ph.full ? 2 : ph.al_nlo ? 1 : 0;
+    out->reserved = 0;
     return RQX_OK;
 }
 

@@ -294,18 +294,21 @@ class Plan:
 
         ``8 N + 8 P + s_v H + s_v N + T (8 + s_c) + 12 J`` with ``P = N J'``
         (every (row, non-diagonal group) pair is resolved through the two-level
-        index), ``H`` structural nonzeros incl. the diagonal, ``s_v`` the vector
-        element bytes and ``s_c`` the coefficient bytes read per term.
+        index) -- or ``P = 0`` when every group of the problem comes from the
+        product-subspace lists (``list_mode`` 2: no pair is searched) --, ``H``
+        structural nonzeros incl. the diagonal, ``s_v`` the vector element bytes
+        and ``s_c`` the coefficient bytes read per term.
         """
         s_v = torch.empty(0, dtype=dtype).element_size()
         s_c = 16 if dtype in (torch.float64, torch.complex128) else 8
-        n = self.n_states
-        J = self.info['n_groups']
-        Jp = J - (1 if self.info['has_diagonal'] else 0)
-        T = self.info['n_terms']
         if hits is None:
             hits = self.count()['hits']
-        return 8 * n + 8 * n * Jp + s_v * hits + s_v * n + T * (8 + s_c) + 12 * J
+        total = s_v * hits
+        for p in self.problems:
+            n, J, T = p['n_states'], p['n_groups'], p['n_terms']
+            jp = 0 if p['list_mode'] == 2 else J - (1 if p['has_diagonal'] else 0)
+            total += 8 * n + 8 * n * jp + s_v * n + T * (8 + s_c) + 12 * J
+        return total
 
     # ------------------------------------------------------------------ hit cache
     def build_cache(self, precision=64, max_bytes=0):
@@ -471,19 +474,6 @@ class BatchPlan(Plan):
 
     def lookup(self, values=False):
         raise NotImplementedError('lookup tables are per problem: build a Plan for one problem')
-
-    def algorithmic_bytes(self, dtype, hits=None):
-        """Sum over problems of the per-matvec byte model (see ``Plan.algorithmic_bytes``)."""
-        s_v = torch.empty(0, dtype=dtype).element_size()
-        s_c = 16 if dtype in (torch.float64, torch.complex128) else 8
-        if hits is None:
-            hits = self.count()['hits']
-        total = s_v * hits
-        for p in self.problems:
-            n, J, T = p['n_states'], p['n_groups'], p['n_terms']
-            jp = J - (1 if p['has_diagonal'] else 0)
-            total += 8 * n + 8 * n * jp + s_v * n + T * (8 + s_c) + 12 * J
-        return total
 
 def eigenpair_batch(mats: torch.Tensor):
     """Lowest eigenpairs of ``(K, n, n)`` Hermitian device matrices -> (theta (K,), kappa (K, n))."""

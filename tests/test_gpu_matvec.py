@@ -123,10 +123,11 @@ def test_lookup_and_matvec_jw_small(jw_small):
 
 
 def test_alpha_lists_product_subspace_match_oracle():
-    """A product subspace (60 alpha x 50 beta strings): the matvec takes the hits of the groups that flip
-    only alpha bits from per-alpha-string lists.  Every dtype equals the oracle, also after complex
-    coefficients replace the real ones (the lists' values are rewritten); one state less is no product
-    and has no lists."""
+    """A product subspace (60 alpha x 50 beta strings): the matvec takes every hit from lists -- per alpha
+    string for the groups that flip only alpha bits (split values for those whose terms hold beta Z
+    bits), per beta string for the groups that flip only beta bits, and alpha x beta partners through a
+    pattern map for the rest.  Every dtype equals the oracle, also after complex coefficients replace
+    the real ones (values and G tables rewritten); one state less is no product and has no lists."""
     _say('alpha lists')
     from rqutils_amd import _capi
     w = workloads.jw32(n_alpha=60, n_beta=50)
@@ -134,6 +135,7 @@ def test_alpha_lists_product_subspace_match_oracle():
     plan = Plan(t, keys, split_bits=10)          # segments below the alpha bits (a tiny problem's default is wider)
     assert plan.problems[0]['section_bits'] == 16
     assert plan.problems[0]['alpha_rows'] == 60
+    assert plan.problems[0]['list_mode'] == 2          # every group from the lists: no search
     rng = np.random.default_rng(51)
     v = rng.standard_normal(plan.n_total) + 1j * rng.standard_normal(plan.n_total)
     want = sqdref.matvec_u64(t.xmask, _groups(t), keys, v)
@@ -159,7 +161,8 @@ def test_alpha_lists_product_subspace_match_oracle():
         y_free = plan.matvec(x)
         plan.use_cache(True)
         assert torch.equal(plan.matvec(x), y_free), dt
-    assert Plan(t, keys[1:], split_bits=10).problems[0]['alpha_rows'] == 0
+    no_product = Plan(t, keys[1:], split_bits=10).problems[0]
+    assert no_product['alpha_rows'] == 0 and no_product['list_mode'] == 0
 
 
 @pytest.mark.parametrize('name', ['small_10q', 'filler_12q'])
