@@ -76,6 +76,33 @@ def test_batch_matvec_equals_single_plans(problems):
     assert torch.equal(bp.matvec(x), y_q)
 
 
+def test_batch_of_listed_products_equals_single_plans_and_oracle():
+    """Two product subspaces (fully listed at split 10) between random problems in one batched plan:
+    every problem's rows equal its own plan bit for bit and the oracle; the cache gives the same bits."""
+    _say('batch of listed products')
+    w1, w2 = workloads.jw32(n_alpha=40, n_beta=30), workloads.jw32(n_alpha=25, n_beta=45)
+    probs = [_random_problem(11, 12, 100, 900), (w1.tables, w1.states), _random_problem(12, 10, 40, 300),
+             (w2.tables, w2.states)]
+    bp = BatchPlan(probs, split_bits=10)
+    assert [p['list_mode'] for p in bp.problems][1::2] == [2, 2]
+    assert [p['alpha_rows'] for p in bp.problems][1::2] == [40, 25]
+    rng = np.random.default_rng(17)
+    x = torch.from_numpy(rng.standard_normal(bp.n_total) + 1j * rng.standard_normal(bp.n_total)).to(DEV)
+    for dt in (torch.complex128, torch.complex64):
+        y = bp.matvec(x.to(dt))
+        for k, (t, s) in enumerate(probs):
+            a, n = bp.problems[k]['row_offset'], bp.problems[k]['n_states']
+            yk = Plan(t, s, split_bits=10).matvec(x[a:a + n].to(dt).contiguous())
+            assert torch.equal(y[a:a + n], yk), k
+            if dt == torch.complex128:
+                want = _csr(t, s) @ x[a:a + n].cpu().numpy()
+                got = y[a:a + n].cpu().numpy()
+                assert np.max(np.abs(got - want)) <= 1e-12 * np.max(np.abs(want)), k
+    y_free = bp.matvec(x)
+    bp.build_cache(64)
+    assert torch.equal(bp.matvec(x), y_free)
+
+
 def test_persistent_workgroups_take_several_tiles(problems):
     """More tiles than persistent workgroups: every mode equals the one-tile-per-workgroup launch."""
     _say('persistent multi-tile')

@@ -163,6 +163,12 @@ def test_alpha_lists_product_subspace_match_oracle():
         assert torch.equal(plan.matvec(x), y_free), dt
     no_product = Plan(t, keys[1:], split_bits=10).problems[0]
     assert no_product['alpha_rows'] == 0 and no_product['list_mode'] == 0
+    # the plan built from device-resident states finds the same product and the same lists
+    host = Plan(t, keys, split_bits=10)
+    dev = Plan(t, torch.from_numpy(keys.view(np.int64)).to(DEV), split_bits=10)
+    assert dev.problems[0]['list_mode'] == 2 and dev.problems[0]['alpha_rows'] == 60
+    xv = torch.from_numpy(v).to(DEV)
+    assert torch.equal(dev.matvec(xv), host.matvec(xv))
 
 
 @pytest.mark.parametrize('name', ['small_10q', 'filler_12q'])
