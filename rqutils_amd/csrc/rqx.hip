@@ -1667,6 +1667,148 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
     }
 }
 
+// The matvec of a fully listed product problem (list_mode 2): one workgroup of
+// PROD_WG threads per chunk of a beta block, a thread per row.  x of the block is
+// staged in LDS first: every alpha-list gather stays inside the row's block, and
+// scattered 8-byte reads from LDS cost a few cycles where the texture path takes a
+// cycle per cache line.  Per row the sums of the tile kernel's fully listed path,
+// in its order (diagonal, alpha-list entries, partner hits; beta-list entries;
+// their sum) and with the same values, so the results are the same bits (the cache
+// passes run the tile kernel).
+constexpr int PROD_LDS_BYTES = 64 * 1024;
+
+// (WG threads: 1024 for beta blocks of 2048 rows or more, whose staging the 16 waves
+// share; 256 for smaller ones, so that small problems still spread over every CU)
+// (eight waves per SIMD, two 1024-thread workgroups per CU, with 64 VGPRs; the FP64
+// complex-vector variants would spill there and ask for six)
+template <typename R, bool CPLX> constexpr int PROD_WAVES = (sizeof(R) == 8 && CPLX) ? 6 : 8;
+
+template <typename R, bool CPLX, bool REALC, int PROD_WG>
+__global__ __launch_bounds__(PROD_WG) __attribute__((amdgpu_waves_per_eu(PROD_WAVES<R, CPLX>))) void k_product(
+    const KParams p, uint32_t prob, uint32_t chunks,
+                                                     const void *__restrict__ xin, void *__restrict__ yout) {
+    using V = typename VT<R, CPLX>::T;
+    using CC = typename C2<R>::T;
+    extern __shared__ int4 prod_lds[];
+    V *xs = reinterpret_cast<V *>(prod_lds);
+    const ProbDesc pd = ldc(p.prob + prob);
+    const uint32_t w = xcd_remap(blockIdx.x, gridDim.x);
+    const uint32_t b = w / chunks, a = (w % chunks) * PROD_WG + threadIdx.x;
+    const V *x = static_cast<const V *>(xin);
+    const uint32_t blk = pd.row0 + b * pd.al_nlo;          // the beta block's first row
+    for (uint32_t k = threadIdx.x; k < pd.al_nlo; k += PROD_WG) xs[k] = x[blk + k];
+    __syncthreads();
+    if (a >= pd.al_nlo) return;                            // (after the only barrier)
+    const uint32_t i = blk + a;
+    const uint64_t s = p.keys[i];
+    V acc = vzero(static_cast<V *>(nullptr)), acc_a = vzero(static_cast<V *>(nullptr));
+    auto add = [&](V &to, const CC &v, const V &xv) {
+        if constexpr (REALC) fma_acc(to, v.x, xv);
+        else fma_acc(to, v, xv);
+    };
+    if (pd.has_diag) fma_acc(acc, static_cast<const CC *>(p.diag)[i], xs[a]);
+    const uint32_t ag = pd.al_a0 + a;
+    {   // alpha lists, AL_BATCH entries in flight, then their multiply-adds in order
+        constexpr int AL_BATCH = sizeof(R) == 4 ? 8 : 4;
+        const AlEnt<R> *al = static_cast<const AlEnt<R> *>(p.al_ent) + p.al_boff[ag >> 6] + (ag & 63u);
+        const uint32_t n = p.al_len[ag];
+        uint32_t k = 0;
+        for (; k + AL_BATCH <= n; k += AL_BATCH) {
+            AlEnt<R> en[AL_BATCH];
+#pragma unroll
+            for (int u = 0; u < AL_BATCH; ++u) en[u] = al[64 * static_cast<int64_t>(k + u)];
+#pragma unroll
+            for (int u = 0; u < AL_BATCH; ++u) {
+                CC v = en[u].v;
+                if (en[u].col & AL_SPLIT)
+                    v = split_value<R>(v, p.al_split[en[u].g], s, static_cast<const CC *>(p.al_G), pd.al_nhi, b);
+                add(acc, v, xs[en[u].col & ~AL_SPLIT]);
+            }
+        }
+        for (; k < n; ++k) {
+            const AlEnt<R> en = al[64 * static_cast<int64_t>(k)];
+            CC v = en.v;
+            if (en.col & AL_SPLIT)
+                v = split_value<R>(v, p.al_split[en.g], s, static_cast<const CC *>(p.al_G), pd.al_nhi, b);
+            add(acc, v, xs[en.col & ~AL_SPLIT]);
+        }
+    }
+    if (pd.pm_base != UINT32_MAX) {   // partners through the map
+        // the row's first PK partners (pattern index, a') held in registers for every
+        // beta partner; per beta partner their map entries first, then the hits in order
+        constexpr int PK = 4;
+        const uint32_t pn = p.al_plen[ag];
+        const uint4 *pl = p.al_pent + p.al_pboff[ag >> 6] + (ag & 63u);
+        uint32_t pm[PK], pa[PK];
+#pragma unroll
+        for (int u = 0; u < PK; ++u)
+            if (static_cast<uint32_t>(u) < pn) {
+                const uint4 pe = pl[64 * u];
+                pm[u] = pe.y;
+                pa[u] = pe.z;
+            }
+        const uint32_t q1 = ldc(p.al_bpoff + pd.bp_b0 + b + 1);
+        for (uint32_t q = ldc(p.al_bpoff + pd.bp_b0 + b); q < q1; ++q) {
+            const uint2 bq = ldc(p.al_bpent + q);
+            const int32_t *mrow = p.al_pmap + pd.pm_base + static_cast<size_t>(bq.x) * pd.pm_na;
+            const uint32_t tb = pd.row0 + bq.y * pd.al_nlo;
+            int32_t gg[PK];
+#pragma unroll
+            for (int u = 0; u < PK; ++u) gg[u] = static_cast<uint32_t>(u) < pn ? mrow[pm[u]] : -1;
+#pragma unroll
+            for (int u = 0; u < PK; ++u)
+                if (gg[u] >= 0) {
+                    const GroupDesc d = p.gdesc[gg[u]];
+                    const V xv = x[tb + pa[u]];
+                    CC c;
+                    lane_group_value<R, REALC>(p, d, s, c.x, c.y);
+                    add(acc, c, xv);
+                }
+            for (uint32_t k = PK; k < pn; ++k) {
+                const uint4 pe = pl[64 * static_cast<int64_t>(k)];
+                const int32_t g1 = mrow[pe.y];
+                if (g1 >= 0) {
+                    const GroupDesc d = p.gdesc[g1];
+                    const V xv = x[tb + pe.z];
+                    CC c;
+                    lane_group_value<R, REALC>(p, d, s, c.x, c.y);
+                    add(acc, c, xv);
+                }
+            }
+        }
+    }
+    {   // beta lists: BL_BATCH entries (scalar loads) and their x loads in flight, then
+        // their multiply-adds in order
+        constexpr int BL_BATCH = 8;
+        const AlEnt<R> *bl = static_cast<const AlEnt<R> *>(p.bl_ent);
+        const uint32_t q1 = ldc(p.bl_off + pd.bl_b0 + b + 1);
+        uint32_t q = ldc(p.bl_off + pd.bl_b0 + b);
+        for (; q + BL_BATCH <= q1; q += BL_BATCH) {
+            AlEnt<R> en[BL_BATCH];
+            V xv[BL_BATCH];
+#pragma unroll
+            for (int u = 0; u < BL_BATCH; ++u) en[u] = ldc(bl + q + u);
+#pragma unroll
+            for (int u = 0; u < BL_BATCH; ++u) xv[u] = x[pd.row0 + (en[u].col & ~AL_SPLIT) * pd.al_nlo + a];
+#pragma unroll
+            for (int u = 0; u < BL_BATCH; ++u) {
+                CC v = en[u].v;
+                if (en[u].col & AL_SPLIT)
+                    v = split_value<R>(v, ldc(p.al_split + en[u].g), s, static_cast<const CC *>(p.al_G), pd.al_nlo, a);
+                add(acc_a, v, xv[u]);
+            }
+        }
+        for (; q < q1; ++q) {
+            const AlEnt<R> en = ldc(bl + q);
+            CC v = en.v;
+            if (en.col & AL_SPLIT)
+                v = split_value<R>(v, ldc(p.al_split + en.g), s, static_cast<const CC *>(p.al_G), pd.al_nlo, a);
+            add(acc_a, v, x[pd.row0 + (en.col & ~AL_SPLIT) * pd.al_nlo + a]);
+        }
+    }
+    static_cast<V *>(yout)[i] = vadd(acc, acc_a);
+}
+
 // Occupancy hints.  The cache passes with real values are register-bound (both
 // precisions' values, packet buffers): six waves per SIMD (a few registers
 // spill) measured 6% faster than five; complex values would spill too much.
@@ -3360,6 +3502,7 @@ struct rqx_plan {
     int device = 0;
     int is_real = 1;
     int long_groups = 0;       // some group has a sign table, or x != 0 and TP_MIN_TERMS or more terms
+    int all_full = 0;          // every problem fully listed (the matvec runs k_product)
     int max_seg = 0;
     int persistent = 0;        // launch the work-queue kernel
     int blocks_per_cu = 0;     // persistent grid: blocks per CU (0 = occupancy limit)
@@ -4085,6 +4228,30 @@ hipError_t launch_tiles(const rqx_plan *p, const KParams &k_in, const void *x, v
 template <int MODE, typename R, bool CPLX, bool REALC>
 hipError_t launch_tp(const rqx_plan *p, const KParams &k, const void *x, void *y, hipStream_t st) {
     if (MODE == MODE_MATVEC && !p->long_groups) return launch_tiles<MODE, R, CPLX, REALC, MODE != MODE_MATVEC>(p, k, x, y, st);
+    if constexpr (MODE == MODE_MATVEC) {
+        // every problem fully listed and its beta block within the staging budget:
+        // the product kernel, one launch per problem
+        using V = typename VT<R, CPLX>::T;
+        bool prod = p->all_full;
+        for (const ProbHost &ph : p->probs) prod = prod && int64_t(ph.al_nlo) * int64_t(sizeof(V)) <= PROD_LDS_BYTES;
+        if (prod) {
+            for (size_t j = 0; j < p->probs.size(); ++j) {
+                const ProbHost &ph = p->probs[j];
+                const bool big = ph.al_nlo >= 2048;
+                const uint32_t wgs = big ? 1024u : 256u, chunks = (ph.al_nlo + wgs - 1) / wgs;
+                const size_t lds = static_cast<size_t>(ph.al_nlo) * sizeof(V);
+                if (big)
+                    hipLaunchKernelGGL((k_product<R, CPLX, REALC, 1024>), dim3(ph.al_nhi * chunks), dim3(1024), lds, st, k,
+                                       static_cast<uint32_t>(j), chunks, x, y);
+                else
+                    hipLaunchKernelGGL((k_product<R, CPLX, REALC, 256>), dim3(ph.al_nhi * chunks), dim3(256), lds, st, k,
+                                       static_cast<uint32_t>(j), chunks, x, y);
+                const hipError_t e = hipGetLastError();
+                if (e != hipSuccess) return e;
+            }
+            return hipSuccess;
+        }
+    }
     return launch_tiles<MODE, R, CPLX, REALC>(p, k, x, y, st);
 }
 
@@ -5392,6 +5559,10 @@ int build_plan(rqx_plan **out, const std::vector<ProblemIn> &in, int64_t n_total
         e = build_alpha_lists(raw, al_alpha, al_lg, al_pg_off, al_pg, al_split, al_rg, al_rz, al_roff, al_gelems, desc, st);
     if (e == hipSuccess) e = build_partners(raw, al_alpha, al_nboff, al_apats, al_bpats, al_pmap, desc, st);
     if (e == hipSuccess) e = build_beta_lists(raw, bl_nboff, bl_lg, bl_rg, bl_rz, bl_roff, desc, st);
+    if (e == hipSuccess) {
+        raw->all_full = !raw->probs.empty();
+        for (const ProbHost &ph : raw->probs) raw->all_full = raw->all_full && ph.full && ph.n_states > 0;
+    }
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e == hipSuccess) {
         int occ = 0;
