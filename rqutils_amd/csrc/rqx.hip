@@ -1692,23 +1692,26 @@ __global__ __launch_bounds__(PROD_WG) __attribute__((amdgpu_waves_per_eu(PROD_WA
     extern __shared__ int4 prod_lds[];
     V *xs = reinterpret_cast<V *>(prod_lds);
     const ProbDesc pd = ldc(p.prob + prob);
-    const uint32_t w = xcd_remap(blockIdx.x, gridDim.x);
-    const uint32_t b = w / chunks, a = (w % chunks) * PROD_WG + threadIdx.x;
+    // chunk-major: the workgroups of one XCD take one chunk of alpha strings over
+    // consecutive beta blocks, so that chunk's list entries stay in its L2
+    const uint32_t w = xcd_remap(blockIdx.x, gridDim.x), nhi = gridDim.x / chunks;
+    const uint32_t b = w % nhi, a = (w / nhi) * PROD_WG + threadIdx.x;
     const V *x = static_cast<const V *>(xin);
     const uint32_t blk = pd.row0 + b * pd.al_nlo;          // the beta block's first row
     for (uint32_t k = threadIdx.x; k < pd.al_nlo; k += PROD_WG) xs[k] = x[blk + k];
     __syncthreads();
-    if (a >= pd.al_nlo) return;                            // (after the only barrier)
-    const uint32_t i = blk + a;
+    // threads past the block's last row take part in the stagings and barriers only
+    const bool act = a < pd.al_nlo;
+    const uint32_t i = blk + (act ? a : 0u);
     const uint64_t s = p.keys[i];
     V acc = vzero(static_cast<V *>(nullptr)), acc_a = vzero(static_cast<V *>(nullptr));
     auto add = [&](V &to, const CC &v, const V &xv) {
         if constexpr (REALC) fma_acc(to, v.x, xv);
         else fma_acc(to, v, xv);
     };
-    if (pd.has_diag) fma_acc(acc, static_cast<const CC *>(p.diag)[i], xs[a]);
-    const uint32_t ag = pd.al_a0 + a;
-    {   // alpha lists, AL_BATCH entries in flight, then their multiply-adds in order
+    if (act && pd.has_diag) fma_acc(acc, static_cast<const CC *>(p.diag)[i], xs[a]);
+    const uint32_t ag = pd.al_a0 + (act ? a : 0u);
+    if (act) {   // alpha lists, AL_BATCH entries in flight, then their multiply-adds in order
         constexpr int AL_BATCH = sizeof(R) == 4 ? 8 : 4;
         const AlEnt<R> *al = static_cast<const AlEnt<R> *>(p.al_ent) + p.al_boff[ag >> 6] + (ag & 63u);
         const uint32_t n = p.al_len[ag];
@@ -1735,9 +1738,10 @@ __global__ __launch_bounds__(PROD_WG) __attribute__((amdgpu_waves_per_eu(PROD_WA
     }
     if (pd.pm_base != UINT32_MAX) {   // partners through the map
         // the row's first PK partners (pattern index, a') held in registers for every
-        // beta partner; per beta partner their map entries first, then the hits in order
+        // beta partner; per beta partner its block of x staged in LDS (its gathers are
+        // scattered over the block), the map entries first, then the hits in order
         constexpr int PK = 4;
-        const uint32_t pn = p.al_plen[ag];
+        const uint32_t pn = act ? p.al_plen[ag] : 0u;
         const uint4 *pl = p.al_pent + p.al_pboff[ag >> 6] + (ag & 63u);
         uint32_t pm[PK], pa[PK];
 #pragma unroll
@@ -1755,28 +1759,30 @@ __global__ __launch_bounds__(PROD_WG) __attribute__((amdgpu_waves_per_eu(PROD_WA
             int32_t gg[PK];
 #pragma unroll
             for (int u = 0; u < PK; ++u) gg[u] = static_cast<uint32_t>(u) < pn ? mrow[pm[u]] : -1;
+            __syncthreads();                                 // the previous block is read
+            for (uint32_t k = threadIdx.x; k < pd.al_nlo; k += PROD_WG) xs[k] = x[tb + k];
+            __syncthreads();
 #pragma unroll
             for (int u = 0; u < PK; ++u)
                 if (gg[u] >= 0) {
                     const GroupDesc d = p.gdesc[gg[u]];
-                    const V xv = x[tb + pa[u]];
                     CC c;
                     lane_group_value<R, REALC>(p, d, s, c.x, c.y);
-                    add(acc, c, xv);
+                    add(acc, c, xs[pa[u]]);
                 }
             for (uint32_t k = PK; k < pn; ++k) {
                 const uint4 pe = pl[64 * static_cast<int64_t>(k)];
                 const int32_t g1 = mrow[pe.y];
                 if (g1 >= 0) {
                     const GroupDesc d = p.gdesc[g1];
-                    const V xv = x[tb + pe.z];
                     CC c;
                     lane_group_value<R, REALC>(p, d, s, c.x, c.y);
-                    add(acc, c, xv);
+                    add(acc, c, xs[pe.z]);
                 }
             }
         }
     }
+    if (!act) return;                                      // (after the last barrier)
     {   // beta lists: BL_BATCH entries (scalar loads) and their x loads in flight, then
         // their multiply-adds in order
         constexpr int BL_BATCH = 8;
