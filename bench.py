@@ -107,7 +107,8 @@ def event_ms(fn, steps, warmup):
 
 # ---------------------------------------------------------------------------- HBM traffic (rocprofv3 --pmc)
 
-KERNELS = {'cached': 'k_spmv_sell<', 'matrix_free': 'k_tiles<0,'}
+# the matrix-free matvec runs k_product on fully listed product subspaces, k_tiles otherwise
+KERNELS = {'cached': ('k_spmv_sell<',), 'matrix_free': ('k_tiles<0,', 'k_product<')}
 
 
 def pmc_child(args):
@@ -155,9 +156,10 @@ def _rocprof_counter(counter, child_args, tmp):
 
 
 def _kernel_avg(table, prefix):
-    hits = [(v, n) for k, (v, n) in table.items() if prefix in k]
+    names = (prefix,) if isinstance(prefix, str) else prefix
+    hits = [(v, n) for k, (v, n) in table.items() if any(q in k for q in names)]
     if not hits:
-        raise RuntimeError(f'no dispatch of {prefix} in the counter file')
+        raise RuntimeError(f'no dispatch of {" or ".join(names)} in the counter file')
     total = sum(v * n for v, n in hits)
     n = sum(n for _, n in hits)
     return total / n, n
