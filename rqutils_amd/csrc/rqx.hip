@@ -207,7 +207,7 @@ struct ProbDesc {
     uint32_t bp_b0;                 //   beta-partner offsets
     uint32_t bl_b0;                 // beta lists: the problem's first beta index of their offsets
     uint32_t full;                  // 1: lists cover every group (no class is searched)
-    uint32_t pad0, pad1;
+    uint32_t al_g0, al_gnr;         // alpha lists' G tables: first element, values per beta block
 };
 
 __device__ inline uint64_t section_mask(int32_t a_bits) { return a_bits >= 64 ? ~0ull : ((1ull << a_bits) - 1ull); }
@@ -228,31 +228,49 @@ struct __attribute__((aligned(32))) GroupDesc {
 // key(b * n_lo + a) = beta_b << a_bits | alpha_a): the groups that flip only
 // alpha bits and whose terms' Z masks hold only alpha bits hit row i = b * n_lo + a
 // at b * n_lo + a' with a value that depends on alpha_a alone.  Per alpha index a,
-// its hits of those groups in group order -- group, a', and the value in the
-// matvec's precision -- replace the searches of all those groups for every row.
-// Entry k of alpha index a sits at boff[a >> 6] + 64 k + (a & 63): the lanes of a
-// wave (consecutive alpha indices) read consecutive entries.
+// its hits of those groups in group order -- a', the value in the matvec's
+// precision and (a side array) the group -- replace the searches of all those
+// groups for every row.  Entry k of alpha index a sits at boff[a >> 6] + 64 k +
+// (a & 63): the lanes of a wave (consecutive alpha indices) read consecutive
+// entries.  The beta lists use the same entries with the group in place of the
+// split word.
 template <typename R> struct AlEnt;
 template <> struct __attribute__((aligned(16))) AlEnt<float> {
-    uint32_t g, col;
+    union {
+        uint32_t sw;     // alpha lists: the split word (0: a plain value)
+        uint32_t g;      // beta lists: the group
+    };
+    uint32_t col;
     float2 v;
 };
 template <> struct __attribute__((aligned(16))) AlEnt<double> {
-    uint32_t g, col, pad0, pad1;
+    union {
+        uint32_t sw;
+        uint32_t g;
+    };
+    uint32_t col, pad0, pad1;
     double2 v;
 };
-// An entry whose column carries AL_SPLIT is a group whose terms also hold beta Z
-// bits, split as C(s) = F(alpha) + sum_j (-1)^popc(s & zeta_j) G_j(beta): zeta_j
-// the distinct alpha parts of its terms with beta Z bits, G_j(beta) the sum of
-// those terms' coefficients times (-1)^popc(s_beta & z_k), F the sum over its
-// terms without beta Z bits (each sum in term order).  The entry's value is F;
-// the group's SplitDesc gives the zeta_j and its rows of the G tables (one row of
-// n_hi entries per j, in the matvec's precision).
+// An entry whose column carries AL_SPLIT is a group whose terms also hold Z bits
+// of the other side, split as C(s) = F + sum_j (-1)^popc(s & zeta_j) G_j: zeta_j
+// the distinct parts on the entry's side of its terms with the other side's Z
+// bits, G_j (a function of the other side's string) the sum of those terms'
+// coefficients times (-1)^popc(s & z_k) over the other side's bits, F the sum over
+// its terms without the other side's Z bits (each sum in term order).  The
+// entry's value is F.
+//  - alpha lists: the problem's G tables hold, per beta block b, one row of
+//    al_gnr values at al_g0 + b * al_gnr (a beta block's G values are contiguous,
+//    and the product kernel stages them in LDS); the split word holds the
+//    group's first row in bits 0..27, its pattern count in bits 28..29 and the
+//    signs (-1)^popc(alpha_a & zeta_j) of the entry's alpha string in bit 30 + j
+//  - beta lists: the group's SplitDesc gives the zeta_j and its rows of the G
+//    tables (one row of n_lo values per j, over the alpha strings)
 constexpr uint32_t AL_SPLIT = 0x80000000u;
 constexpr int SPLIT_MAX = 2;
+constexpr uint32_t SW_ROW_BITS = 28;
 struct __attribute__((aligned(16))) SplitDesc {
     uint64_t zeta[SPLIT_MAX];
-    uint32_t grow;                  // first row of the G tables
+    uint32_t grow;                  // first row of the G tables (alpha groups: of the problem's rows)
     uint32_t n_p;                   // patterns (<= SPLIT_MAX; 0: F is the whole sum)
     uint32_t split;                 // 1: the group's entries are split, 0: plain values
     uint32_t pad0;
@@ -350,7 +368,8 @@ struct KParams {
     const uint32_t *al_pg_off;
     const uint32_t *al_pg;
     const AlEnt<float> *al_ent_lo;  // the FP32 entries (a cache fill of both precisions)
-    const SplitDesc *al_split;      // per group (alpha-listed groups only)
+    const uint32_t *al_eg;          // per alpha-list entry: its group
+    const SplitDesc *al_split;      // per group (split groups only)
     const void *al_G;               // G tables in the precision of coef (double2 or float2)
     const float2 *al_G_lo;          // FP32 G tables (a cache fill of both precisions)
     // partner lists (product subspaces): per alpha index its partners (alpha
@@ -383,16 +402,31 @@ template <typename R> __device__ inline R vzero(R *) { return R(0); }
 __device__ inline double2 vzero(double2 *) { return make_double2(0.0, 0.0); }
 __device__ inline float2 vzero(float2 *) { return make_float2(0.f, 0.f); }
 
-// The value of a split alpha-list entry: F (the entry's value) plus the group's G
-// terms of beta block b, in pattern order
+// The value of a split beta-list entry at alpha string a: F (the entry's value)
+// plus the group's G terms of a (G rows of n values), in pattern order
 template <typename R>
 __device__ inline typename C2<R>::T split_value(typename C2<R>::T v, const SplitDesc &sd, uint64_t s,
-                                                const typename C2<R>::T *G, uint32_t nhi, uint32_t b) {
+                                                const typename C2<R>::T *G, uint32_t n, uint32_t a) {
 #pragma unroll
     for (int j = 0; j < SPLIT_MAX; ++j)
         if (static_cast<uint32_t>(j) < sd.n_p) {
-            const typename C2<R>::T gv = G[static_cast<size_t>(sd.grow) + static_cast<size_t>(j) * nhi + b];
+            const typename C2<R>::T gv = G[static_cast<size_t>(sd.grow) + static_cast<size_t>(j) * n + a];
             const bool neg = __popcll(s & sd.zeta[j]) & 1;
+            v.x += neg ? -gv.x : gv.x;
+            v.y += neg ? -gv.y : gv.y;
+        }
+    return v;
+}
+
+// The value of a split alpha-list entry of beta block b: F (the entry's value)
+// plus the group's G terms, Gb = the block's G values, in pattern order
+template <typename T> __device__ inline T split_alpha(T v, uint32_t sw, const T *Gb) {
+    const uint32_t r = sw & ((1u << SW_ROW_BITS) - 1u), np = (sw >> SW_ROW_BITS) & 3u;
+#pragma unroll
+    for (int j = 0; j < SPLIT_MAX; ++j)
+        if (static_cast<uint32_t>(j) < np) {
+            const T gv = Gb[r + j];
+            const bool neg = (sw >> (30 + j)) & 1u;
             v.x += neg ? -gv.x : gv.x;
             v.y += neg ? -gv.y : gv.y;
         }
@@ -964,12 +998,12 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
     // alpha lists (the matvec, the hit count and the cache passes; a value pass
     // of section A only has no use for them): this lane's alpha index a (row
     // i = al_rb + a), its entries [0, al_n) and the next one (group al_g, column
-    // al_rb + al_col, value al_v and, in a fill of both precisions, al_l; al_g =
-    // UINT32_MAX when none is left)
+    // al_rb + al_col, split word al_sw, value al_v and, in a fill of both
+    // precisions, al_l; al_g = UINT32_MAX when none is left)
     constexpr bool AL_FILL = MODE == MODE_FILL || MODE == MODE_FILLCOL;
     constexpr bool AL = HQ || MODE == MODE_ROWCOUNT || AL_FILL;
     const bool al_on = AL && pd.al_nlo != 0 && !(MODE == MODE_FILL && p.c_a_only);
-    uint32_t al_rb = 0, al_k = 0, al_n = 0, al_g = UINT32_MAX, al_col = 0, al_b = 0;   // al_b: the beta block
+    uint32_t al_rb = 0, al_k = 0, al_n = 0, al_g = UINT32_MAX, al_col = 0, al_sw = 0, al_b = 0;   // al_b: the beta block
     int64_t al_pos = 0;
     uint32_t pl_n = 0;             // partners of this lane's alpha string, the first at pl_pos
     int64_t pl_pos = 0;
@@ -979,7 +1013,8 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
         if (al_k < al_n) {
             const int64_t o = al_pos + 64 * static_cast<int64_t>(al_k);
             const AlEnt<R> en = static_cast<const AlEnt<R> *>(p.al_ent)[o];
-            al_g = en.g;
+            al_g = p.al_eg[o];
+            al_sw = en.sw;
             al_col = en.col;
             al_v = en.v;
             if constexpr (AL_FILL) {
@@ -1109,10 +1144,10 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
                     CC v = al_v;
                     float2 l = al_l;
                     if (al_col & AL_SPLIT) {
-                        const SplitDesc sd = p.al_split[al_g];
-                        v = split_value<R>(v, sd, s, static_cast<const CC *>(p.al_G), pd.al_nhi, al_b);
+                        const size_t gb = pd.al_g0 + static_cast<size_t>(al_b) * pd.al_gnr;
+                        v = split_alpha(v, al_sw, static_cast<const CC *>(p.al_G) + gb);
                         if constexpr (AL_FILL) {
-                            if (p.c_val_lo) l = split_value<float>(l, sd, s, p.al_G_lo, pd.al_nhi, al_b);
+                            if (p.c_val_lo) l = split_alpha(l, al_sw, p.al_G_lo + gb);
                         }
                     }
                     ++al_k;
@@ -1668,10 +1703,10 @@ __device__ inline void process_tile(const KParams &p, uint32_t tile, uint32_t *s
 }
 
 // The matvec of a fully listed product problem (list_mode 2): one workgroup of
-// PROD_WG threads per chunk of a beta block, a thread per row.  x of the block is
-// staged in LDS first: every alpha-list gather stays inside the row's block, and
-// scattered 8-byte reads from LDS cost a few cycles where the texture path takes a
-// cycle per cache line.  Per row the sums of the tile kernel's fully listed path,
+// PROD_WG threads per chunk of a beta block, a thread per row.  x of the block
+// and its G values (split alpha-list entries) are staged in LDS first: every
+// alpha-list gather stays inside the row's block, and scattered 8-byte reads from
+// LDS cost a few cycles where the texture path takes a cycle per cache line.  Per row the sums of the tile kernel's fully listed path,
 // in its order (diagonal, alpha-list entries, partner hits; beta-list entries;
 // their sum) and with the same values, so the results are the same bits (the cache
 // passes run the tile kernel).
@@ -1692,6 +1727,8 @@ __global__ __launch_bounds__(PROD_WG) __attribute__((amdgpu_waves_per_eu(PROD_WA
     extern __shared__ int4 prod_lds[];
     V *xs = reinterpret_cast<V *>(prod_lds);
     const ProbDesc pd = ldc(p.prob + prob);
+    // the beta block's G values of the split alpha-list entries after the block's x
+    CC *gs = reinterpret_cast<CC *>(prod_lds + (static_cast<size_t>(pd.al_nlo) * sizeof(V) + 15) / 16);
     // chunk-major: the workgroups of one XCD take one chunk of alpha strings over
     // consecutive beta blocks, so that chunk's list entries stay in its L2
     const uint32_t w = xcd_remap(blockIdx.x, gridDim.x), nhi = gridDim.x / chunks;
@@ -1699,6 +1736,10 @@ __global__ __launch_bounds__(PROD_WG) __attribute__((amdgpu_waves_per_eu(PROD_WA
     const V *x = static_cast<const V *>(xin);
     const uint32_t blk = pd.row0 + b * pd.al_nlo;          // the beta block's first row
     for (uint32_t k = threadIdx.x; k < pd.al_nlo; k += PROD_WG) xs[k] = x[blk + k];
+    {
+        const CC *G = static_cast<const CC *>(p.al_G) + pd.al_g0 + static_cast<size_t>(b) * pd.al_gnr;
+        for (uint32_t k = threadIdx.x; k < pd.al_gnr; k += PROD_WG) gs[k] = G[k];
+    }
     __syncthreads();
     // threads past the block's last row take part in the stagings and barriers only
     const bool act = a < pd.al_nlo;
@@ -1723,16 +1764,14 @@ __global__ __launch_bounds__(PROD_WG) __attribute__((amdgpu_waves_per_eu(PROD_WA
 #pragma unroll
             for (int u = 0; u < AL_BATCH; ++u) {
                 CC v = en[u].v;
-                if (en[u].col & AL_SPLIT)
-                    v = split_value<R>(v, p.al_split[en[u].g], s, static_cast<const CC *>(p.al_G), pd.al_nhi, b);
+                if (en[u].col & AL_SPLIT) v = split_alpha(v, en[u].sw, gs);
                 add(acc, v, xs[en[u].col & ~AL_SPLIT]);
             }
         }
         for (; k < n; ++k) {
             const AlEnt<R> en = al[64 * static_cast<int64_t>(k)];
             CC v = en.v;
-            if (en.col & AL_SPLIT)
-                v = split_value<R>(v, p.al_split[en.g], s, static_cast<const CC *>(p.al_G), pd.al_nhi, b);
+            if (en.col & AL_SPLIT) v = split_alpha(v, en.sw, gs);
             add(acc, v, xs[en.col & ~AL_SPLIT]);
         }
     }
@@ -1995,11 +2034,13 @@ __device__ inline typename C2<R>::T split_f(const KParams &p, const GroupDesc &d
 // G tables of one problem's split groups: row r (group rg[r], pattern rz[r]) at
 // entry e = the sum over the group's terms with Z bits in smask (the other side's
 // bits) and pattern rz[r] outside smask of c_k (-1)^popc(s_e & z_k), s_e = keys[e *
-// stride] & smask; at G[roff[r] + e], FP64 and FP32 (each in term order).  Alpha
-// groups: smask = the beta bits, entries per beta block (stride n_lo); beta groups:
-// smask = the alpha bits, entries per alpha string (stride 1).
+// stride] & smask; at G[roff[r] + e * estride], FP64 and FP32 (each in term
+// order).  Alpha groups: smask = the beta bits, entries per beta block (stride
+// n_lo), a block's rows contiguous (estride = the problem's rows); beta groups:
+// smask = the alpha bits, entries per alpha string (stride 1), a row's entries
+// contiguous (estride 1).
 __global__ __launch_bounds__(256) void k_split_g(const KParams p64, const KParams p32, const uint64_t *__restrict__ keys,
-                                                 uint32_t stride, uint32_t n, uint64_t smask,
+                                                 uint32_t stride, uint32_t n, uint32_t estride, uint64_t smask,
                                                  const uint32_t *__restrict__ rg, const uint64_t *__restrict__ rz,
                                                  const uint32_t *__restrict__ roff, double2 *__restrict__ G64,
                                                  float2 *__restrict__ G32) {
@@ -2022,8 +2063,9 @@ __global__ __launch_bounds__(256) void k_split_g(const KParams p64, const KParam
         l.x += odd ? -cl.x : cl.x;
         l.y += odd ? -cl.y : cl.y;
     }
-    G64[roff[r] + e] = h;
-    G32[roff[r] + e] = l;
+    const size_t o = roff[r] + static_cast<size_t>(e) * estride;
+    G64[o] = h;
+    G32[o] = l;
 }
 
 // Beta lists of one problem: workgroup b takes beta index b (state keys[b * nlo])
@@ -2196,12 +2238,15 @@ __global__ __launch_bounds__(256) void k_beta_partners(const uint64_t *__restric
 // index a' among keys[0..nlo) (ascending low parts).  Count pass (boff NULL):
 // len[a0 + a] = the hits.  Fill pass: entry k of the hits at
 // boff[(a0 + a) >> 6] + 64 k + ((a0 + a) & 63), with C_g(s) as the matvec's lanes
-// evaluate it (lane_group_value) in FP64 and in FP32.
+// evaluate it (lane_group_value) in FP64 and in FP32 -- or, for a split group, F
+// and the split word (its first G row, pattern count and alpha_a's signs) --
+// and g at the same position of eg.
 __global__ __launch_bounds__(256) void k_alpha_lists(const KParams p64, const KParams p32,
                                                      const uint64_t *__restrict__ keys, uint32_t nlo, uint64_t amask,
                                                      const uint32_t *__restrict__ lg, uint32_t nlg, uint32_t a0,
                                                      uint32_t *__restrict__ len, const int64_t *__restrict__ boff,
-                                                     AlEnt<float> *__restrict__ e32, AlEnt<double> *__restrict__ e64) {
+                                                     AlEnt<float> *__restrict__ e32, AlEnt<double> *__restrict__ e64,
+                                                     uint32_t *__restrict__ eg) {
     __shared__ uint32_t wn[4];
     const uint32_t a = blockIdx.x, tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
     const uint64_t s = keys[a], sa = s & amask;
@@ -2239,9 +2284,15 @@ __global__ __launch_bounds__(256) void k_alpha_lists(const KParams p64, const KP
             const GroupDesc gd = p64.gdesc[g];
             AlEnt<double> eh{};
             AlEnt<float> el{};
-            eh.g = el.g = g;
-            if (p64.al_split[g].split) {
-                eh.col = el.col = col | AL_SPLIT;
+            const SplitDesc sd = p64.al_split[g];
+            if (sd.split) {
+                uint32_t sw = 0;
+                if (sd.n_p) {
+                    sw = sd.grow | sd.n_p << SW_ROW_BITS;
+                    for (uint32_t j = 0; j < sd.n_p; ++j) sw |= (__popcll(s & sd.zeta[j]) & 1u) << (30 + j);
+                }
+                eh.sw = el.sw = sw;
+                eh.col = el.col = sd.n_p ? col | AL_SPLIT : col;
                 eh.v = split_f<double>(p64, gd, s, amask);
                 el.v = split_f<float>(p32, gd, s, amask);
             } else {
@@ -2255,6 +2306,7 @@ __global__ __launch_bounds__(256) void k_alpha_lists(const KParams p64, const KP
             }
             e64[pos] = eh;
             e32[pos] = el;
+            eg[pos] = g;
         }
         base += total;
         __syncthreads();          // wn is rewritten by the next chunk
@@ -3454,6 +3506,7 @@ struct ProbHost {            // host record of one problem of a plan
     uint32_t al_a0 = 0;        //   and the problem's listed groups [al_lg0, al_lg0 + al_lgn) of the plan's list
     uint32_t al_lg0 = 0, al_lgn = 0;
     uint32_t al_nhi = 0;       // high strings of the product
+    uint32_t al_g0 = 0;        // its G tables' first element (al_rn values per beta block)
     uint32_t al_r0 = 0, al_rn = 0;   // the problem's G-table rows [al_r0, al_r0 + al_rn) of the plan's list
     uint32_t al_pat0 = 0, al_npat = 0;   // its alpha patterns [al_pat0, al_pat0 + al_npat) of the plan's list,
     uint32_t al_bpat0 = 0, al_nbpat = 0; //   its beta patterns of the plan's list,
@@ -3572,8 +3625,9 @@ struct rqx_plan {
     int64_t *al_boff = nullptr;
     AlEnt<float> *al_e32 = nullptr;
     AlEnt<double> *al_e64 = nullptr;
+    uint32_t *al_eg = nullptr;      // per entry: its group
     int64_t al_slots = 0;           // entries per precision, padding included
-    SplitDesc *al_split = nullptr;  // per group: split values (alpha-listed groups)
+    SplitDesc *al_split = nullptr;  // per group: split values (split groups)
     uint32_t *al_rg = nullptr;      // G-table rows: group, first element, alpha pattern
     uint32_t *al_roff = nullptr;
     uint64_t *al_rz = nullptr;
@@ -3867,6 +3921,7 @@ KParams make_params(const rqx_plan *p, bool fp64) {
     k.al_pg_off = p->al_pg_off;
     k.al_pg = p->al_pg;
     k.al_ent_lo = p->al_e32;
+    k.al_eg = p->al_eg;
     k.al_split = p->al_split;
     k.al_G = fp64 ? static_cast<const void *>(p->al_G64) : static_cast<const void *>(p->al_G32);
     k.al_G_lo = p->al_G32;
@@ -3892,7 +3947,7 @@ hipError_t launch_alpha_lists(const rqx_plan *p, bool fill, hipStream_t st) {
         for (uint32_t r = 0; fill && r < ph.al_rn; r += 65535u) {
             const uint32_t nr = std::min(ph.al_rn - r, 65535u);
             hipLaunchKernelGGL(k_split_g, dim3((ph.al_nhi + 255) / 256, nr), dim3(256), 0, st, p64, p32,
-                               p->keys + ph.row_base, ph.al_nlo, ph.al_nhi, ~((1ull << ph.a_bits) - 1ull),
+                               p->keys + ph.row_base, ph.al_nlo, ph.al_nhi, ph.al_rn, ~((1ull << ph.a_bits) - 1ull),
                                p->al_rg + ph.al_r0 + r, p->al_rz + ph.al_r0 + r, p->al_roff + ph.al_r0 + r, p->al_G64,
                                p->al_G32);
             const hipError_t e = hipGetLastError();
@@ -3900,7 +3955,7 @@ hipError_t launch_alpha_lists(const rqx_plan *p, bool fill, hipStream_t st) {
         }
         hipLaunchKernelGGL(k_alpha_lists, dim3(ph.al_nlo), dim3(256), 0, st, p64, p32, p->keys + ph.row_base,
                            ph.al_nlo, (1ull << ph.a_bits) - 1ull, p->al_lg + ph.al_lg0, ph.al_lgn, ph.al_a0,
-                           p->al_len, fill ? p->al_boff : nullptr, p->al_e32, p->al_e64);
+                           p->al_len, fill ? p->al_boff : nullptr, p->al_e32, p->al_e64, p->al_eg);
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
@@ -3973,11 +4028,12 @@ hipError_t build_alpha_lists(rqx_plan *p, uint32_t n_alpha, const std::vector<ui
         boff[k] = slots;
         slots += 64 * static_cast<int64_t>(w);
     }
-    const int64_t ent_bytes = slots * static_cast<int64_t>(sizeof(AlEnt<float>) + sizeof(AlEnt<double>)) +
+    const int64_t ent_bytes = slots * static_cast<int64_t>(sizeof(AlEnt<float>) + sizeof(AlEnt<double>) + 4) +
                               gelems * static_cast<int64_t>(sizeof(double2) + sizeof(float2));
     if (slots == 0 || ent_bytes > ALPHA_LIST_MAX_BYTES) return none();   // no hits at all, or no room
     const size_t o_e32 = align_up(size_t(n_blk) * 8), o_e64 = o_e32 + align_up(size_t(slots) * sizeof(AlEnt<float>)),
-                 o_g64 = o_e64 + align_up(size_t(slots) * sizeof(AlEnt<double>)),
+                 o_eg = o_e64 + align_up(size_t(slots) * sizeof(AlEnt<double>)),
+                 o_g64 = o_eg + align_up(size_t(slots) * 4),
                  o_g32 = o_g64 + align_up(size_t(std::max<int64_t>(gelems, 1)) * sizeof(double2));
     e = hipMalloc(&p->al_ent_block, o_g32 + size_t(std::max<int64_t>(gelems, 1)) * sizeof(float2));
     if (e != hipSuccess) return e;
@@ -3985,6 +4041,7 @@ hipError_t build_alpha_lists(rqx_plan *p, uint32_t n_alpha, const std::vector<ui
     p->al_boff = reinterpret_cast<int64_t *>(eb);
     p->al_e32 = reinterpret_cast<AlEnt<float> *>(eb + o_e32);
     p->al_e64 = reinterpret_cast<AlEnt<double> *>(eb + o_e64);
+    p->al_eg = reinterpret_cast<uint32_t *>(eb + o_eg);
     p->al_G64 = reinterpret_cast<double2 *>(eb + o_g64);
     p->al_G32 = reinterpret_cast<float2 *>(eb + o_g32);
     p->al_slots = slots;
@@ -4107,7 +4164,7 @@ hipError_t launch_beta_lists(const rqx_plan *p, bool fill, hipStream_t st) {
         for (uint32_t r = 0; fill && r < ph.bl_rn; r += 65535u) {
             const uint32_t nr = std::min(ph.bl_rn - r, 65535u);
             hipLaunchKernelGGL(k_split_g, dim3((ph.al_nlo + 255) / 256, nr), dim3(256), 0, st, p64, p32,
-                               p->keys + ph.row_base, 1u, ph.al_nlo, amask, p->bl_rg + ph.bl_r0 + r,
+                               p->keys + ph.row_base, 1u, ph.al_nlo, 1u, amask, p->bl_rg + ph.bl_r0 + r,
                                p->bl_rz + ph.bl_r0 + r, p->bl_roff + ph.bl_r0 + r, p->al_G64, p->al_G32);
             const hipError_t e = hipGetLastError();
             if (e != hipSuccess) return e;
@@ -4235,17 +4292,21 @@ template <int MODE, typename R, bool CPLX, bool REALC>
 hipError_t launch_tp(const rqx_plan *p, const KParams &k, const void *x, void *y, hipStream_t st) {
     if (MODE == MODE_MATVEC && !p->long_groups) return launch_tiles<MODE, R, CPLX, REALC, MODE != MODE_MATVEC>(p, k, x, y, st);
     if constexpr (MODE == MODE_MATVEC) {
-        // every problem fully listed and its beta block within the staging budget:
-        // the product kernel, one launch per problem
+        // every problem fully listed and its beta block (x and G values) within the
+        // staging budget: the product kernel, one launch per problem
         using V = typename VT<R, CPLX>::T;
+        auto lds_bytes = [](const ProbHost &ph) {
+            return ((static_cast<size_t>(ph.al_nlo) * sizeof(V) + 15) & ~size_t(15)) +
+                   static_cast<size_t>(ph.al_rn) * sizeof(typename C2<R>::T);
+        };
         bool prod = p->all_full;
-        for (const ProbHost &ph : p->probs) prod = prod && int64_t(ph.al_nlo) * int64_t(sizeof(V)) <= PROD_LDS_BYTES;
+        for (const ProbHost &ph : p->probs) prod = prod && lds_bytes(ph) <= static_cast<size_t>(PROD_LDS_BYTES);
         if (prod) {
             for (size_t j = 0; j < p->probs.size(); ++j) {
                 const ProbHost &ph = p->probs[j];
                 const bool big = ph.al_nlo >= 2048;
                 const uint32_t wgs = big ? 1024u : 256u, chunks = (ph.al_nlo + wgs - 1) / wgs;
-                const size_t lds = static_cast<size_t>(ph.al_nlo) * sizeof(V);
+                const size_t lds = lds_bytes(ph);
                 if (big)
                     hipLaunchKernelGGL((k_product<R, CPLX, REALC, 1024>), dim3(ph.al_nhi * chunks), dim3(1024), lds, st, k,
                                        static_cast<uint32_t>(j), chunks, x, y);
@@ -5189,7 +5250,8 @@ int build_plan(rqx_plan **out, const std::vector<ProblemIn> &in, int64_t n_total
             const size_t lg0 = al_lg.size(), r0 = al_rg.size();
             al_split.resize(al_split.size() + static_cast<size_t>(q.n_groups), SplitDesc{});
             const uint32_t nhi = product ? static_cast<uint32_t>(q.n_states / ph.a_rows) : 0u;
-            int64_t gel = al_gelems;
+            // the problem's G rows, numbered from 0 (the split words' row field),
+            // at al_gelems + b * rows + r for beta block b
             if (product)
                 for (; n_acls < ph.n_classes && (sg.cls_hi[n_acls] >> (ph.a_bits - sg.L)) == 0; ++n_acls)
                     for (uint32_t j = sg.cls_goff[n_acls]; j < sg.cls_goff[n_acls + 1]; ++j) {
@@ -5201,17 +5263,17 @@ int build_plan(rqx_plan **out, const std::vector<ProblemIn> &in, int64_t n_total
                             SplitDesc &sd = al_split[gj];
                             sd.split = 1;
                             sd.n_p = static_cast<uint32_t>(pat.size());
-                            sd.grow = static_cast<uint32_t>(gel);
+                            sd.grow = static_cast<uint32_t>(al_rg.size() - r0);
                             for (size_t u = 0; u < pat.size(); ++u) {
                                 sd.zeta[u] = pat[u];
+                                al_roff.push_back(static_cast<uint32_t>(al_gelems + static_cast<int64_t>(al_rg.size() - r0)));
                                 al_rg.push_back(gj);
                                 al_rz.push_back(pat[u]);
-                                al_roff.push_back(static_cast<uint32_t>(gel));
-                                gel += nhi;
                             }
                         }
                     }
-            if (al_lg.size() == lg0 || gel > 0xFFFFFFFFLL) {
+            const int64_t nrow = static_cast<int64_t>(al_rg.size() - r0), gel = al_gelems + nrow * nhi;
+            if (al_lg.size() == lg0 || gel > 0xFFFFFFFFLL || nrow >= (int64_t(1) << SW_ROW_BITS)) {
                 n_acls = 0;
                 al_lg.resize(lg0);
                 al_rg.resize(r0);
@@ -5225,7 +5287,8 @@ int build_plan(rqx_plan **out, const std::vector<ProblemIn> &in, int64_t n_total
                 ph.al_lg0 = static_cast<uint32_t>(lg0);
                 ph.al_lgn = static_cast<uint32_t>(al_lg.size() - lg0);
                 ph.al_r0 = static_cast<uint32_t>(r0);
-                ph.al_rn = static_cast<uint32_t>(al_rg.size() - r0);
+                ph.al_rn = static_cast<uint32_t>(nrow);
+                ph.al_g0 = static_cast<uint32_t>(al_gelems);
                 al_alpha += (ph.al_nlo + 63u) & ~63u;
                 al_gelems = gel;
             }
@@ -5366,6 +5429,8 @@ int build_plan(rqx_plan **out, const std::vector<ProblemIn> &in, int64_t n_total
         desc[k].bp_b0 = ph.bp_b0;
         desc[k].bl_b0 = ph.bl_b0;
         desc[k].full = ph.full;
+        desc[k].al_g0 = ph.al_g0;
+        desc[k].al_gnr = ph.al_rn;
         desc[k].prod = ph.a_bits > ph.L ? 1u : 0u;
         p->max_seg = std::max(p->max_seg, ph.max_seg);
         p->is_real = p->is_real && ph.is_real;
