@@ -208,6 +208,9 @@ struct ProbDesc {
     uint32_t bl_b0;                 // beta lists: the problem's first beta index of their offsets
     uint32_t full;                  // 1: lists cover every group (no class is searched)
     uint32_t al_g0, al_gnr;         // alpha lists' G tables: first element, values per beta block
+    uint32_t al_pat0;               // partner map: the problem's first alpha pattern in the plan's list,
+    uint32_t al_hsl;                //   log2 of the table values per pattern the product kernel stages
+    uint32_t pad0, pad1;
 };
 
 __device__ inline uint64_t section_mask(int32_t a_bits) { return a_bits >= 64 ? ~0ull : ((1ull << a_bits) - 1ull); }
@@ -383,6 +386,8 @@ struct KParams {
     const uint32_t *al_bpoff;
     const uint2 *al_bpent;
     const int32_t *al_pmap;
+    const uint64_t *al_ap;          // per alpha pattern (the plan's list): the pattern and its
+    const uint64_t *al_pz;          //   reference Z part (the partners' alpha signs)
     // beta lists (product subspaces whose lists cover every group): per beta index
     // its hits of the groups that flip only beta bits, [bl_off[b], bl_off[b + 1]),
     // as (group, b' | AL_SPLIT, value) in the precision of coef (and FP32)
@@ -1776,48 +1781,95 @@ __global__ __launch_bounds__(PROD_WG) __attribute__((amdgpu_waves_per_eu(PROD_WA
         }
     }
     if (pd.pm_base != UINT32_MAX) {   // partners through the map
-        // the row's first PK partners (pattern index, a') held in registers for every
-        // beta partner; per beta partner its block of x staged in LDS (its gathers are
-        // scattered over the block), the map entries first, then the hits in order
+        // per beta partner: its block of x staged in LDS (the rows' partner hits
+        // gather from all over it) and, per alpha pattern u, the map's group for
+        // the beta partner's pattern: hk[u] = -1 (none), -2 (values in hv), or
+        // the group (evaluated per lane).  A group with a sign table whose first
+        // term's alpha part is the pattern's reference pz_u but for bits zz at the
+        // pattern's X bits has C(s) = (-1)^popc(s & z0) tab[u(s)], u(s) = the alpha
+        // bits ua | the beta bits of the block's string, and (-1)^popc(s & z0) =
+        // (-1)^popc(s_beta & z0) (-1)^popc(ua & zz) (-1)^popc(s_alpha & pz_u):
+        // hv[u][ua] holds the first two signs times tab[u(s)], the row's partner
+        // entry gives ua and the third sign -- the same value as the row's own
+        // evaluation, bit for bit (negation is exact).  Then the row's hits in
+        // alpha-pattern order, its first PK partners held in registers
         constexpr int PK = 4;
+        const uint32_t hs = 1u << pd.al_hsl;
+        CC *hv = reinterpret_cast<CC *>(prod_lds + (static_cast<size_t>(pd.al_nlo) * sizeof(V) + 15) / 16 +
+                                        (static_cast<size_t>(pd.al_gnr) * sizeof(CC) + 15) / 16);
+        int32_t *hk = reinterpret_cast<int32_t *>(hv + static_cast<size_t>(pd.pm_na) * hs);
+        const uint64_t amask = section_mask(pd.a_bits), sb = p.keys[blk] & ~amask;
         const uint32_t pn = act ? p.al_plen[ag] : 0u;
         const uint4 *pl = p.al_pent + p.al_pboff[ag >> 6] + (ag & 63u);
-        uint32_t pm[PK], pa[PK];
+        uint32_t pm[PK], pa[PK], pw[PK];
 #pragma unroll
         for (int u = 0; u < PK; ++u)
             if (static_cast<uint32_t>(u) < pn) {
                 const uint4 pe = pl[64 * u];
                 pm[u] = pe.y;
                 pa[u] = pe.z;
+                pw[u] = pe.w;
             }
+        auto hit = [&](uint32_t m, uint32_t col, uint32_t w) {
+            const int32_t kind = hk[m];
+            if (kind == -2) {
+                CC v = hv[m * hs + (w & 15u)];
+                if (w & 16u) {
+                    v.x = -v.x;
+                    v.y = -v.y;
+                }
+                add(acc, v, xs[col]);
+            } else if (kind >= 0) {
+                const GroupDesc d = p.gdesc[kind];
+                CC c;
+                lane_group_value<R, REALC>(p, d, s, c.x, c.y);
+                add(acc, c, xs[col]);
+            }
+        };
         const uint32_t q1 = ldc(p.al_bpoff + pd.bp_b0 + b + 1);
         for (uint32_t q = ldc(p.al_bpoff + pd.bp_b0 + b); q < q1; ++q) {
             const uint2 bq = ldc(p.al_bpent + q);
             const int32_t *mrow = p.al_pmap + pd.pm_base + static_cast<size_t>(bq.x) * pd.pm_na;
             const uint32_t tb = pd.row0 + bq.y * pd.al_nlo;
-            int32_t gg[PK];
-#pragma unroll
-            for (int u = 0; u < PK; ++u) gg[u] = static_cast<uint32_t>(u) < pn ? mrow[pm[u]] : -1;
             __syncthreads();                                 // the previous block is read
             for (uint32_t k = threadIdx.x; k < pd.al_nlo; k += PROD_WG) xs[k] = x[tb + k];
+            for (uint32_t u = threadIdx.x; u < pd.pm_na; u += PROD_WG) {
+                const int32_t g = mrow[u];
+                int32_t kind = g;
+                if (g >= 0) {
+                    const GroupDesc d = p.gdesc[g];
+                    const uint64_t xa = p.al_ap[pd.al_pat0 + u], zz = (d.z0 & amask) ^ p.al_pz[pd.al_pat0 + u];
+                    const uint32_t na = static_cast<uint32_t>(__popcll(xa));
+                    if (d.tab >= 0 && na <= pd.al_hsl && (zz & ~xa) == 0) {
+                        uint32_t mz = 0;                     // zz over the table index's alpha bits
+                        uint64_t t = xa;
+                        for (uint32_t j = 0; j < na; ++j, t &= t - 1ull)
+                            mz |= static_cast<uint32_t>((zz >> __builtin_ctzll(t)) & 1ull) << j;
+                        const uint32_t ub = tab_index_packed(d.xpos, sb);
+                        const bool nb = __popcll(sb & d.z0) & 1;
+                        for (uint32_t ua = 0; ua < (1u << na); ++ua) {
+                            const uint32_t e = ((ub | ua) & 0xFFu) * (p.n_tab & 0xFFFFFFu) + static_cast<uint32_t>(d.tab);
+                            const R re = static_cast<const R *>(p.tab_re)[e];
+                            R im = R(0);
+                            if constexpr (!REALC) im = static_cast<const R *>(p.tab_im)[e];
+                            const bool neg = nb ^ (__popc(ua & mz) & 1);
+                            CC v;
+                            v.x = neg ? -re : re;
+                            v.y = neg ? -im : im;
+                            hv[u * hs + ua] = v;
+                        }
+                        kind = -2;
+                    }
+                }
+                hk[u] = kind;
+            }
             __syncthreads();
 #pragma unroll
             for (int u = 0; u < PK; ++u)
-                if (gg[u] >= 0) {
-                    const GroupDesc d = p.gdesc[gg[u]];
-                    CC c;
-                    lane_group_value<R, REALC>(p, d, s, c.x, c.y);
-                    add(acc, c, xs[pa[u]]);
-                }
+                if (static_cast<uint32_t>(u) < pn) hit(pm[u], pa[u], pw[u]);
             for (uint32_t k = PK; k < pn; ++k) {
                 const uint4 pe = pl[64 * static_cast<int64_t>(k)];
-                const int32_t g1 = mrow[pe.y];
-                if (g1 >= 0) {
-                    const GroupDesc d = p.gdesc[g1];
-                    CC c;
-                    lane_group_value<R, REALC>(p, d, s, c.x, c.y);
-                    add(acc, c, xs[pe.z]);
-                }
+                hit(pe.y, pe.z, pe.w);
             }
         }
     }
@@ -2141,9 +2193,12 @@ __global__ __launch_bounds__(256) void k_beta_lists(const KParams p64, const KPa
 // alpha parts pat[0..npat) of the groups that flip both alpha and beta bits;
 // pattern m is a partner when alpha_a ^ pat[m] is an alpha string, at index a'.
 // Count pass (boff NULL): len[a0 + a] = the partners.  Fill pass: partner k at
-// boff[(a0 + a) >> 6] + 64 k + ((a0 + a) & 63) as (pat[m], m, a', 0), ascending m.
+// boff[(a0 + a) >> 6] + 64 k + ((a0 + a) & 63) as (pat[m], m, a', w), ascending m,
+// w = alpha_a's bits at pat[m]'s positions (the first 4, ascending: the alpha bits
+// of a sign-table index) | popc(alpha_a & pz[m]) mod 2 in bit 4.
 __global__ __launch_bounds__(256) void k_alpha_partners(const uint64_t *__restrict__ keys, uint32_t nlo, uint64_t amask,
-                                                        const uint64_t *__restrict__ pat, uint32_t npat, uint32_t a0,
+                                                        const uint64_t *__restrict__ pat,
+                                                        const uint64_t *__restrict__ pz, uint32_t npat, uint32_t a0,
                                                         uint32_t *__restrict__ len, const int64_t *__restrict__ boff,
                                                         uint4 *__restrict__ ent) {
     __shared__ uint32_t wn[4];
@@ -2178,8 +2233,13 @@ __global__ __launch_bounds__(256) void k_alpha_partners(const uint64_t *__restri
         const uint32_t total = wn[0] + wn[1] + wn[2] + wn[3];
         if (boff && f) {
             const uint32_t k = before + static_cast<uint32_t>(__popcll(mk & ((1ull << lane) - 1ull)));
+            uint64_t t = pat[m];
+            uint32_t w4 = 0;
+            for (int j = 0; j < TAB_MAX_BITS && t; ++j, t &= t - 1ull)
+                w4 |= static_cast<uint32_t>((sa >> __builtin_ctzll(t)) & 1ull) << j;
+            w4 |= (static_cast<uint32_t>(__popcll(sa & pz[m])) & 1u) << 4;
             ent[boff[ag >> 6] + 64 * static_cast<int64_t>(k) + (ag & 63u)] =
-                make_uint4(static_cast<uint32_t>(pat[m]), m, col, 0u);
+                make_uint4(static_cast<uint32_t>(pat[m]), m, col, w4);
         }
         base += total;
         __syncthreads();          // wn is rewritten by the next chunk
@@ -3508,6 +3568,7 @@ struct ProbHost {            // host record of one problem of a plan
     uint32_t al_nhi = 0;       // high strings of the product
     uint32_t al_g0 = 0;        // its G tables' first element (al_rn values per beta block)
     uint32_t al_r0 = 0, al_rn = 0;   // the problem's G-table rows [al_r0, al_r0 + al_rn) of the plan's list
+    uint32_t al_hsl = 0;       // log2 of the table values per alpha pattern the product kernel stages
     uint32_t al_pat0 = 0, al_npat = 0;   // its alpha patterns [al_pat0, al_pat0 + al_npat) of the plan's list,
     uint32_t al_bpat0 = 0, al_nbpat = 0; //   its beta patterns of the plan's list,
     uint32_t pm_base = UINT32_MAX;       //   its partner map's first element
@@ -3642,6 +3703,8 @@ struct rqx_plan {
     uint32_t *al_bpoff = nullptr;
     uint2 *al_bpent = nullptr;
     int32_t *al_pmap = nullptr;
+    uint64_t *al_ap = nullptr;      // alpha patterns and their reference Z parts
+    uint64_t *al_pz = nullptr;
     void *bl_block = nullptr;       // beta lists (own allocation): offsets, listed groups, G-table rows, entries
     uint32_t *bl_off = nullptr;
     uint32_t *bl_lg = nullptr;
@@ -3931,6 +3994,8 @@ KParams make_params(const rqx_plan *p, bool fp64) {
     k.al_bpoff = p->al_bpoff;
     k.al_bpent = p->al_bpent;
     k.al_pmap = p->al_pmap;
+    k.al_ap = p->al_ap;
+    k.al_pz = p->al_pz;
     k.bl_off = p->bl_off;
     k.bl_ent = fp64 ? static_cast<const void *>(p->bl_e64) : static_cast<const void *>(p->bl_e32);
     k.bl_ent_lo = p->bl_e32;
@@ -4060,8 +4125,8 @@ constexpr int64_t CMAP_MAX_ELEMS = int64_t(1) << 24;
 // beta index; then the map.  Without them (no alpha lists, no room) the problems'
 // descriptors are rewritten without a map and their classes are searched.
 hipError_t build_partners(rqx_plan *p, uint32_t n_alpha, uint32_t n_boff, const std::vector<uint64_t> &apats,
-                          const std::vector<uint64_t> &bpats, const std::vector<int32_t> &pmap,
-                          std::vector<ProbDesc> &desc, hipStream_t st) {
+                          const std::vector<uint64_t> &apz, const std::vector<uint64_t> &bpats,
+                          const std::vector<int32_t> &pmap, std::vector<ProbDesc> &desc, hipStream_t st) {
     if (pmap.empty()) return hipSuccess;
     auto none = [&]() {
         for (size_t k = 0; k < p->probs.size(); ++k) {
@@ -4075,18 +4140,21 @@ hipError_t build_partners(rqx_plan *p, uint32_t n_alpha, uint32_t n_boff, const 
         return e;
     };
     if (!p->al_ent_block || n_alpha == 0 || apats.empty() || bpats.empty()) return none();
-    const size_t o_len = 0, o_ap = align_up(size_t(n_alpha) * 4), o_bp = o_ap + align_up(apats.size() * 8),
-                 o_map = o_bp + align_up(bpats.size() * 8), o_boff = o_map + align_up(pmap.size() * 4),
+    const size_t o_len = 0, o_ap = align_up(size_t(n_alpha) * 4), o_pz = o_ap + align_up(apats.size() * 8),
+                 o_bp = o_pz + align_up(apz.size() * 8), o_map = o_bp + align_up(bpats.size() * 8),
+                 o_boff = o_map + align_up(pmap.size() * 4),
                  bytes = o_boff + align_up(size_t(n_boff) * 4);
     hipError_t e = hipMalloc(&p->al_pblock, bytes);
     if (e != hipSuccess) return e;
     char *b = static_cast<char *>(p->al_pblock);
     uint32_t *plen = reinterpret_cast<uint32_t *>(b + o_len), *bpoff = reinterpret_cast<uint32_t *>(b + o_boff);
-    uint64_t *ap = reinterpret_cast<uint64_t *>(b + o_ap), *bp = reinterpret_cast<uint64_t *>(b + o_bp);
+    uint64_t *ap = reinterpret_cast<uint64_t *>(b + o_ap), *pz = reinterpret_cast<uint64_t *>(b + o_pz),
+             *bp = reinterpret_cast<uint64_t *>(b + o_bp);
     int32_t *map = reinterpret_cast<int32_t *>(b + o_map);
     e = fill_async(plen, int64_t(0), static_cast<int64_t>(n_alpha), 0u, st);
     if (e == hipSuccess) e = fill_async(bpoff, int64_t(0), static_cast<int64_t>(n_boff), 0u, st);
     if (e == hipSuccess) e = hipMemcpyAsync(ap, apats.data(), apats.size() * 8, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(pz, apz.data(), apz.size() * 8, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipMemcpyAsync(bp, bpats.data(), bpats.size() * 8, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipMemcpyAsync(map, pmap.data(), pmap.size() * 4, hipMemcpyHostToDevice, st);
     // count (boff NULL) or fill passes of both partner kinds
@@ -4095,7 +4163,7 @@ hipError_t build_partners(rqx_plan *p, uint32_t n_alpha, uint32_t n_boff, const 
             if (ph.pm_base == UINT32_MAX) continue;
             const uint64_t amask = (1ull << ph.a_bits) - 1ull;
             hipLaunchKernelGGL(k_alpha_partners, dim3(ph.al_nlo), dim3(256), 0, st, p->keys + ph.row_base, ph.al_nlo,
-                               amask, ap + ph.al_pat0, ph.al_npat, ph.al_a0, plen, aboff, aent);
+                               amask, ap + ph.al_pat0, pz + ph.al_pat0, ph.al_npat, ph.al_a0, plen, aboff, aent);
             hipError_t le = hipGetLastError();
             if (le != hipSuccess) return le;
             hipLaunchKernelGGL(k_beta_partners, dim3(ph.al_nhi), dim3(256), 0, st, p->keys + ph.row_base, ph.al_nlo,
@@ -4150,6 +4218,8 @@ hipError_t build_partners(rqx_plan *p, uint32_t n_alpha, uint32_t n_boff, const 
     p->al_bpoff = bpoff;
     p->al_bpent = bpent;
     p->al_pmap = map;
+    p->al_ap = ap;
+    p->al_pz = pz;
     p->al_pslots = slots;
     return hipSuccess;
 }
@@ -4292,12 +4362,16 @@ template <int MODE, typename R, bool CPLX, bool REALC>
 hipError_t launch_tp(const rqx_plan *p, const KParams &k, const void *x, void *y, hipStream_t st) {
     if (MODE == MODE_MATVEC && !p->long_groups) return launch_tiles<MODE, R, CPLX, REALC, MODE != MODE_MATVEC>(p, k, x, y, st);
     if constexpr (MODE == MODE_MATVEC) {
-        // every problem fully listed and its beta block (x and G values) within the
-        // staging budget: the product kernel, one launch per problem
+        // every problem fully listed and its beta block (x, G values, the partner
+        // values per alpha pattern) within the staging budget: the product kernel,
+        // one launch per problem
         using V = typename VT<R, CPLX>::T;
+        using CC = typename C2<R>::T;
         auto lds_bytes = [](const ProbHost &ph) {
-            return ((static_cast<size_t>(ph.al_nlo) * sizeof(V) + 15) & ~size_t(15)) +
-                   static_cast<size_t>(ph.al_rn) * sizeof(typename C2<R>::T);
+            size_t b = ((static_cast<size_t>(ph.al_nlo) * sizeof(V) + 15) & ~size_t(15)) +
+                       ((static_cast<size_t>(ph.al_rn) * sizeof(CC) + 15) & ~size_t(15));
+            if (ph.pm_base != UINT32_MAX) b += static_cast<size_t>(ph.al_npat) * ((size_t(1) << ph.al_hsl) * sizeof(CC) + 4);
+            return b;
         };
         bool prod = p->all_full;
         for (const ProbHost &ph : p->probs) prod = prod && lds_bytes(ph) <= static_cast<size_t>(PROD_LDS_BYTES);
@@ -5130,6 +5204,7 @@ int build_plan(rqx_plan **out, const std::vector<ProblemIn> &in, int64_t n_total
     uint32_t bl_nboff = 0;                            //   and offsets
     std::vector<int32_t> al_pmap;                     // partner maps of every problem
     std::vector<uint64_t> al_apats, al_bpats;         //   their alpha and beta patterns
+    std::vector<uint64_t> al_apz;                     //   the alpha patterns' reference Z parts
     uint32_t al_nboff = 0;                            //   and beta-partner offsets
     int64_t n_tab = 0;
     int64_t row_base = 0, term_base = 0;
@@ -5344,6 +5419,15 @@ int build_plan(rqx_plan **out, const std::vector<ProblemIn> &in, int64_t n_total
                 al_apats.insert(al_apats.end(), Xa.begin(), Xa.end());
                 al_bpats.insert(al_bpats.end(), Xb.begin(), Xb.end());
                 al_pmap.resize(al_pmap.size() + static_cast<size_t>(elems), -1);
+                // per alpha pattern a reference Z part: the alpha part of the first
+                // term of its first group with a sign table.  The product kernel
+                // stages, per beta partner, the values of the pattern's table groups
+                // whose first terms' alpha parts differ from it only at the pattern's
+                // X bits, over those bits (2^popc entries; al_hsl = the largest popc
+                // below TAB_MAX_BITS)
+                std::vector<uint64_t> pz(Xa.size(), 0);
+                std::vector<uint8_t> pz_set(Xa.size(), 0);
+                ph.al_hsl = 0;
                 for (int32_t c : mcls) {
                     cls_lane_all[static_cast<size_t>(ph.cls_base) + c] |= 4u;
                     for (uint32_t j = sg.cls_goff[c]; j < sg.cls_goff[c + 1]; ++j) {
@@ -5353,8 +5437,17 @@ int build_plan(rqx_plan **out, const std::vector<ProblemIn> &in, int64_t n_total
                         const size_t ib = static_cast<size_t>(
                             std::lower_bound(Xb.begin(), Xb.end(), q.xmask[j] & ~amask) - Xb.begin());
                         al_pmap[ph.pm_base + ib * Xa.size() + ia] = static_cast<int32_t>(grp_base) + static_cast<int32_t>(j);
+                        const uint32_t na = static_cast<uint32_t>(__builtin_popcountll(xa));
+                        if (g_tab[static_cast<size_t>(grp_base) + j] >= 0 && na < static_cast<uint32_t>(TAB_MAX_BITS)) {
+                            if (!pz_set[ia]) {
+                                pz[ia] = q.zmask[q.grp_off[j]] & amask;
+                                pz_set[ia] = 1;
+                            }
+                            ph.al_hsl = std::max(ph.al_hsl, na);
+                        }
                     }
                 }
+                al_apz.insert(al_apz.end(), pz.begin(), pz.end());
             }
         }
         if (ph.al_nlo > 0 && ph.a_bits <= 32) {
@@ -5431,6 +5524,8 @@ int build_plan(rqx_plan **out, const std::vector<ProblemIn> &in, int64_t n_total
         desc[k].full = ph.full;
         desc[k].al_g0 = ph.al_g0;
         desc[k].al_gnr = ph.al_rn;
+        desc[k].al_pat0 = ph.al_pat0;
+        desc[k].al_hsl = ph.al_hsl;
         desc[k].prod = ph.a_bits > ph.L ? 1u : 0u;
         p->max_seg = std::max(p->max_seg, ph.max_seg);
         p->is_real = p->is_real && ph.is_real;
@@ -5628,7 +5723,7 @@ int build_plan(rqx_plan **out, const std::vector<ProblemIn> &in, int64_t n_total
     if (e == hipSuccess) e = build_targets(raw, st);
     if (e == hipSuccess)
         e = build_alpha_lists(raw, al_alpha, al_lg, al_pg_off, al_pg, al_split, al_rg, al_rz, al_roff, al_gelems, desc, st);
-    if (e == hipSuccess) e = build_partners(raw, al_alpha, al_nboff, al_apats, al_bpats, al_pmap, desc, st);
+    if (e == hipSuccess) e = build_partners(raw, al_alpha, al_nboff, al_apats, al_apz, al_bpats, al_pmap, desc, st);
     if (e == hipSuccess) e = build_beta_lists(raw, bl_nboff, bl_lg, bl_rg, bl_rz, bl_roff, desc, st);
     if (e == hipSuccess) {
         raw->all_full = !raw->probs.empty();

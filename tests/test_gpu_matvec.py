@@ -171,6 +171,60 @@ def test_alpha_lists_product_subspace_match_oracle():
     assert torch.equal(dev.matvec(xv), host.matvec(xv))
 
 
+def test_product_partners_mixed_evaluation_match_oracle():
+    """Partner hits of a product subspace whose map holds groups of three kinds: sign-table groups whose
+    first term matches the alpha pattern's reference Z part but for the pattern's X bits (values staged
+    per beta partner), table groups with an extra alpha Z bit outside the pattern (evaluated per row),
+    and groups of one or two terms without a table (evaluated per row).  Every dtype equals the oracle,
+    and the hit cache equals the matrix-free result bit for bit."""
+    _say('product partners, mixed evaluation')
+    w = workloads.jw32(n_alpha=60, n_beta=50)
+    t, keys = w.tables, w.states
+    amask = np.uint64((1 << 16) - 1)
+    rng = np.random.default_rng(77)
+    xm, zm, cf, off = [], [], [], [0]
+    n_thin = n_moved = 0
+    for j in range(t.num_groups):
+        x = np.uint64(t.xmask[j])
+        z = t.zmask[t.grp_off[j]:t.grp_off[j + 1]].astype(np.uint64).copy()
+        c = t.coef[t.grp_off[j]:t.grp_off[j + 1]].copy()
+        mixed = (x & amask) != 0 and (x & ~amask) != 0
+        if mixed and z.size >= 3:
+            r = rng.random()
+            if r < 0.3:                    # one or two terms: no sign table
+                k = int(rng.integers(1, 3))
+                z, c = z[:k], c[:k]
+                n_thin += 1
+            elif r < 0.5:                  # an alpha Z bit outside the pattern on every term
+                free = [b for b in range(16) if not (int(x) >> b) & 1]
+                z = z ^ np.uint64(1 << int(rng.choice(free)))
+                n_moved += 1
+        xm.append(x)
+        zm.extend(z.tolist())
+        cf.extend(c.tolist())
+        off.append(len(zm))
+    assert n_thin > 0 and n_moved > 0
+    t3 = TermTables(np.array(xm, dtype=np.uint64), np.array(off, dtype=np.int64), np.array(zm, dtype=np.uint64),
+                    np.array(cf), t.num_qubits)
+    plan = Plan(t3, keys, split_bits=10)
+    assert plan.problems[0]['list_mode'] == 2
+    rng = np.random.default_rng(78)
+    v = rng.standard_normal(plan.n_total) + 1j * rng.standard_normal(plan.n_total)
+    want = sqdref.matvec_u64(t3.xmask, _groups(t3), keys, v)
+    assert _rel(plan.matvec(torch.from_numpy(v).to(DEV)).cpu().numpy(), want) < RTOL64
+    assert _rel(plan.matvec(torch.from_numpy(v.astype(np.complex64)).to(DEV)).cpu().numpy(), want) < RTOL32
+    want_re = sqdref.matvec_u64(t3.xmask, _groups(t3), keys, v.real)
+    assert _rel(plan.matvec(torch.from_numpy(v.real.copy()).to(DEV)).cpu().numpy(), want_re.real) < RTOL64
+    assert _rel(plan.matvec(torch.from_numpy(v.real.astype(np.float32)).to(DEV)).cpu().numpy(), want_re.real) < RTOL32
+    plan.build_cache(0)
+    for dt in (torch.complex128, torch.complex64, torch.float64, torch.float32):
+        x = torch.from_numpy(v.real.copy() if not dt.is_complex else v).to(DEV).to(dt)
+        plan.use_cache(False)
+        y_free = plan.matvec(x)
+        plan.use_cache(True)
+        assert torch.equal(plan.matvec(x), y_free), dt
+
+
 @pytest.mark.parametrize('name', ['small_10q', 'filler_12q'])
 def test_golden_fixtures(name):
     _say(f'golden {name}')
