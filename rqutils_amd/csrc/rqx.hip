@@ -271,6 +271,21 @@ template <> struct __attribute__((aligned(16))) AlEnt<double> {
 constexpr uint32_t AL_SPLIT = 0x80000000u;
 constexpr int SPLIT_MAX = 2;
 constexpr uint32_t SW_ROW_BITS = 28;
+// The alpha-list entries once more for real coefficients (what the product kernel
+// reads then): one word -- a' in bits 0..13, the split word's pattern count in
+// bits 14..15, its signs in bits 16..17, its first G row in bits 18..31 -- and the
+// real part of the value.  (The product kernel runs only where a beta block's x and
+// G values fit in 64 KiB of LDS, so a' and the row stay below 2^14 there.)
+constexpr uint32_t CE_BITS = 14;
+template <typename R> struct AlEntC;
+template <> struct __attribute__((aligned(8))) AlEntC<float> {
+    uint32_t w;
+    float v;
+};
+template <> struct __attribute__((aligned(16))) AlEntC<double> {
+    uint32_t w, pad;
+    double v;
+};
 struct __attribute__((aligned(16))) SplitDesc {
     uint64_t zeta[SPLIT_MAX];
     uint32_t grow;                  // first row of the G tables (alpha groups: of the problem's rows)
@@ -372,6 +387,7 @@ struct KParams {
     const uint32_t *al_pg;
     const AlEnt<float> *al_ent_lo;  // the FP32 entries (a cache fill of both precisions)
     const uint32_t *al_eg;          // per alpha-list entry: its group
+    const void *al_ec;              // the entries for real coefficients (AlEntC, precision of coef)
     const SplitDesc *al_split;      // per group (split groups only)
     const void *al_G;               // G tables in the precision of coef (double2 or float2)
     const float2 *al_G_lo;          // FP32 G tables (a cache fill of both precisions)
@@ -1757,7 +1773,34 @@ __global__ __launch_bounds__(PROD_WG) __attribute__((amdgpu_waves_per_eu(PROD_WA
     };
     if (act && pd.has_diag) fma_acc(acc, static_cast<const CC *>(p.diag)[i], xs[a]);
     const uint32_t ag = pd.al_a0 + (act ? a : 0u);
-    if (act) {   // alpha lists, AL_BATCH entries in flight, then their multiply-adds in order
+    if (act && REALC) {   // alpha lists (real coefficients: the one-word entries), as below
+        constexpr int AL_BATCH = 8;
+        const AlEntC<R> *al = static_cast<const AlEntC<R> *>(p.al_ec) + p.al_boff[ag >> 6] + (ag & 63u);
+        const uint32_t n = p.al_len[ag];
+        auto value = [&](const AlEntC<R> &en) {
+            R v = en.v;
+            const uint32_t np = (en.w >> CE_BITS) & 3u, r = en.w >> (CE_BITS + 4);
+#pragma unroll
+            for (int j = 0; j < SPLIT_MAX; ++j)    // split_alpha's real part
+                if (static_cast<uint32_t>(j) < np) {
+                    const R gv = gs[r + j].x;
+                    v += ((en.w >> (CE_BITS + 2 + j)) & 1u) ? -gv : gv;
+                }
+            return v;
+        };
+        uint32_t k = 0;
+        for (; k + AL_BATCH <= n; k += AL_BATCH) {
+            AlEntC<R> en[AL_BATCH];
+#pragma unroll
+            for (int u = 0; u < AL_BATCH; ++u) en[u] = al[64 * static_cast<int64_t>(k + u)];
+#pragma unroll
+            for (int u = 0; u < AL_BATCH; ++u) fma_acc(acc, value(en[u]), xs[en[u].w & ((1u << CE_BITS) - 1u)]);
+        }
+        for (; k < n; ++k) {
+            const AlEntC<R> en = al[64 * static_cast<int64_t>(k)];
+            fma_acc(acc, value(en), xs[en.w & ((1u << CE_BITS) - 1u)]);
+        }
+    } else if (act) {   // alpha lists, AL_BATCH entries in flight, then their multiply-adds in order
         constexpr int AL_BATCH = sizeof(R) == 4 ? 8 : 4;
         const AlEnt<R> *al = static_cast<const AlEnt<R> *>(p.al_ent) + p.al_boff[ag >> 6] + (ag & 63u);
         const uint32_t n = p.al_len[ag];
@@ -2372,6 +2415,22 @@ __global__ __launch_bounds__(256) void k_alpha_lists(const KParams p64, const KP
         __syncthreads();          // wn is rewritten by the next chunk
     }
     if (!boff && tid == 0) len[ag] = base;
+}
+
+// The real-coefficient form of alpha-list slots [0, n) (padding slots included:
+// never read)
+__global__ void k_compact_alpha(const AlEnt<float> *__restrict__ e32, const AlEnt<double> *__restrict__ e64,
+                                AlEntC<float> *__restrict__ c32, AlEntC<double> *__restrict__ c64, int64_t n) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const AlEnt<float> el = e32[i];
+    const AlEnt<double> eh = e64[i];
+    uint32_t w = (el.col & ~AL_SPLIT) & ((1u << CE_BITS) - 1u);
+    if (el.col & AL_SPLIT)
+        w |= ((el.sw >> SW_ROW_BITS) & 3u) << CE_BITS | (el.sw >> 30) << (CE_BITS + 2) |
+             (el.sw & ((1u << SW_ROW_BITS) - 1u)) << (CE_BITS + 4);
+    c32[i] = AlEntC<float>{w, el.v.x};
+    c64[i] = AlEntC<double>{w, 0u, eh.v.x};
 }
 
 __global__ void k_narrow(const double2 *__restrict__ in, float2 *__restrict__ out, int64_t n) {
@@ -3687,6 +3746,8 @@ struct rqx_plan {
     AlEnt<float> *al_e32 = nullptr;
     AlEnt<double> *al_e64 = nullptr;
     uint32_t *al_eg = nullptr;      // per entry: its group
+    AlEntC<float> *al_c32 = nullptr;   // the entries for real coefficients
+    AlEntC<double> *al_c64 = nullptr;
     int64_t al_slots = 0;           // entries per precision, padding included
     SplitDesc *al_split = nullptr;  // per group: split values (split groups)
     uint32_t *al_rg = nullptr;      // G-table rows: group, first element, alpha pattern
@@ -3985,6 +4046,7 @@ KParams make_params(const rqx_plan *p, bool fp64) {
     k.al_pg = p->al_pg;
     k.al_ent_lo = p->al_e32;
     k.al_eg = p->al_eg;
+    k.al_ec = fp64 ? static_cast<const void *>(p->al_c64) : static_cast<const void *>(p->al_c32);
     k.al_split = p->al_split;
     k.al_G = fp64 ? static_cast<const void *>(p->al_G64) : static_cast<const void *>(p->al_G32);
     k.al_G_lo = p->al_G32;
@@ -4023,6 +4085,11 @@ hipError_t launch_alpha_lists(const rqx_plan *p, bool fill, hipStream_t st) {
                            p->al_len, fill ? p->al_boff : nullptr, p->al_e32, p->al_e64, p->al_eg);
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
+    }
+    if (fill && p->al_slots > 0) {
+        hipLaunchKernelGGL(k_compact_alpha, dim3(static_cast<uint32_t>((p->al_slots + 255) / 256)), dim3(256), 0, st,
+                           p->al_e32, p->al_e64, p->al_c32, p->al_c64, p->al_slots);
+        return hipGetLastError();
     }
     return hipSuccess;
 }
@@ -4093,12 +4160,14 @@ hipError_t build_alpha_lists(rqx_plan *p, uint32_t n_alpha, const std::vector<ui
         boff[k] = slots;
         slots += 64 * static_cast<int64_t>(w);
     }
-    const int64_t ent_bytes = slots * static_cast<int64_t>(sizeof(AlEnt<float>) + sizeof(AlEnt<double>) + 4) +
+    const int64_t ent_bytes = slots * static_cast<int64_t>(sizeof(AlEnt<float>) + sizeof(AlEnt<double>) + 4 +
+                                                           sizeof(AlEntC<float>) + sizeof(AlEntC<double>)) +
                               gelems * static_cast<int64_t>(sizeof(double2) + sizeof(float2));
     if (slots == 0 || ent_bytes > ALPHA_LIST_MAX_BYTES) return none();   // no hits at all, or no room
     const size_t o_e32 = align_up(size_t(n_blk) * 8), o_e64 = o_e32 + align_up(size_t(slots) * sizeof(AlEnt<float>)),
                  o_eg = o_e64 + align_up(size_t(slots) * sizeof(AlEnt<double>)),
-                 o_g64 = o_eg + align_up(size_t(slots) * 4),
+                 o_c32 = o_eg + align_up(size_t(slots) * 4), o_c64 = o_c32 + align_up(size_t(slots) * sizeof(AlEntC<float>)),
+                 o_g64 = o_c64 + align_up(size_t(slots) * sizeof(AlEntC<double>)),
                  o_g32 = o_g64 + align_up(size_t(std::max<int64_t>(gelems, 1)) * sizeof(double2));
     e = hipMalloc(&p->al_ent_block, o_g32 + size_t(std::max<int64_t>(gelems, 1)) * sizeof(float2));
     if (e != hipSuccess) return e;
@@ -4107,6 +4176,8 @@ hipError_t build_alpha_lists(rqx_plan *p, uint32_t n_alpha, const std::vector<ui
     p->al_e32 = reinterpret_cast<AlEnt<float> *>(eb + o_e32);
     p->al_e64 = reinterpret_cast<AlEnt<double> *>(eb + o_e64);
     p->al_eg = reinterpret_cast<uint32_t *>(eb + o_eg);
+    p->al_c32 = reinterpret_cast<AlEntC<float> *>(eb + o_c32);
+    p->al_c64 = reinterpret_cast<AlEntC<double> *>(eb + o_c64);
     p->al_G64 = reinterpret_cast<double2 *>(eb + o_g64);
     p->al_G32 = reinterpret_cast<float2 *>(eb + o_g32);
     p->al_slots = slots;
