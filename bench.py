@@ -123,6 +123,7 @@ def pmc_child(args):
     for _ in range(3):
         plan.matvec(x, out=y)
     plan.build_cache(64 if dt in (torch.float64, torch.complex128) else 32)
+    plan.use_cache(True)                    # the cached kernel (the build timed one launch of each path too)
     for _ in range(3):
         plan.matvec(x, out=y)
     torch.cuda.synchronize()
@@ -299,6 +300,7 @@ def locg_legs(plan, e_oracle, maxiter=1000, energy_tol=1e-9, target=1e-8, matrix
 
     plan.drop_cache()
     e_plain, niter, total, trace = plain('cached')
+    plain_path = plan.matvec_path(torch.complex128)
     e_ref = e_oracle if e_oracle is not None else e_plain
     t_plain = to_target(trace, e_ref, total)
     plan.drop_cache()
@@ -308,7 +310,7 @@ def locg_legs(plan, e_oracle, maxiter=1000, energy_tol=1e-9, target=1e-8, matrix
                                else 'plain FP64 LOCG on the GPU (no oracle at this size)'}
     out['plain_cached'] = {'dtype': 'c128', 'energy': e_plain, 'rel_err': abs(e_plain - e_ref) / abs(e_ref),
                            'iterations': niter, 'matvecs': 4 * niter + 3, 'time_to_target_s': round(t_plain, 4),
-                           'total_s': round(total, 4)}
+                           'total_s': round(total, 4), 'matvec_path': plain_path}
     torch.cuda.empty_cache()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -316,6 +318,7 @@ def locg_legs(plan, e_oracle, maxiter=1000, energy_tol=1e-9, target=1e-8, matrix
     torch.cuda.synchronize()
     t_build = time.perf_counter() - t0
     g = graph_mixed(plan, x0, maxiter, energy_tol, e_ref, target=target)
+    g['matvec_paths'] = {'c64': plan.matvec_path(torch.complex64), 'c128': plan.matvec_path(torch.complex128)}
     g['time_to_target_s'] = round(g['time_to_target_s'] + t_build, 4)
     g['time_s'] = round(g['time_s'] + t_build, 4)
     g['cache_build_s'] = round(t_build, 4)
@@ -364,7 +367,10 @@ def matvec_leg(plan, dt, steps, warmup, seed=1002):
     cinfo = plan.build_cache(precision)
     torch.cuda.synchronize()
     t_cache = time.perf_counter() - t0
+    auto_path = plan.matvec_path(dt)          # the default mode's pick (build_cache timed both)
+    plan.use_cache(True)                      # the cached kernel itself
     ms = event_ms(lambda: plan.matvec(x, out=y), steps, warmup)
+    plan.use_cache('auto')
     identical = bool(torch.equal(y, y_free))
     del y_free
     stored = plan.cached_bytes(dt)
@@ -372,10 +378,11 @@ def matvec_leg(plan, dt, steps, warmup, seed=1002):
     useful = stored - padding
     log(f'cached matvec: {ms:.4f} ms, cache {cinfo["bytes"] / 1e9:.2f} GB built in {t_cache:.3f} s; '
         f'{useful / 1e9:.3f} GB non-padding of {stored / 1e9:.3f} GB stored -> {useful / ms / 1e6:.0f} GB/s; '
-        f'identical to matrix-free: {identical}')
+        f'identical to matrix-free: {identical}; default path with the cache built: {auto_path}')
     plan.drop_cache()
     return {'counts': counts, 'alg_bytes': alg, 'ms_free': ms_mf, 'ms': ms, 'cache': cinfo, 'cache_build_s': t_cache,
-            'stored_bytes': stored, 'padding_bytes': padding, 'useful_bytes': useful, 'identical': identical}
+            'stored_bytes': stored, 'padding_bytes': padding, 'useful_bytes': useful, 'identical': identical,
+            'auto_path': auto_path, 'free_path': plan.matvec_path(dt)}
 
 
 def line_for(w, plan, leg, dt_name, steps, warmup, traffic):
@@ -387,6 +394,7 @@ def line_for(w, plan, leg, dt_name, steps, warmup, traffic):
     cinfo = leg['cache']
     t_cached = traffic.get('cached') if isinstance(traffic, dict) else None
     t_free = traffic.get('matrix_free') if isinstance(traffic, dict) else None
+    hbm_mf = t_free['hbm_bytes_per_launch'] / (leg['ms_free'] * 1e-3) / 1e9 if t_free else None
     return {
         'value': round(achieved, 2),
         'ms_per_step': round(ms, 4),
@@ -402,6 +410,7 @@ def line_for(w, plan, leg, dt_name, steps, warmup, traffic):
             'cache_bytes': cinfo['bytes'], 'cache_entries_incl_padding': cinfo['entries'],
             'cache_info': {k: v for k, v in cinfo.items() if k != 'built'},
             'cache_build_s': round(leg['cache_build_s'], 4), 'identical_to_matrix_free': leg['identical'],
+            'default_path_with_cache': leg['auto_path'],
         },
         'roofline': {
             'bound': 'hbm', 'achieved': round(achieved, 2), 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
@@ -409,16 +418,22 @@ def line_for(w, plan, leg, dt_name, steps, warmup, traffic):
             'traffic': t_cached['hbm_bytes_per_launch'] if t_cached else None,
         },
         'matrix_free': {
+            'kernel': leg['free_path'],
             'ms_per_matvec': round(leg['ms_free'], 4),
             'model_bytes_per_matvec': leg['alg_bytes'],
             'list_mode': [pr['list_mode'] for pr in plan.problems],
-            'roofline': {'bound': 'hbm' if listed else 'lds', 'achieved': round(ach_mf, 2), 'peak': HBM_PEAK_GBS,
-                         'unit': 'GB/s', 'frac': round(ach_mf / HBM_PEAK_GBS, 4),
+            'roofline': {'bound': 'hbm' if listed else 'lds',
+                         'achieved': round(hbm_mf, 2) if hbm_mf is not None else None, 'peak': HBM_PEAK_GBS,
+                         'unit': 'GB/s', 'frac': round(hbm_mf / HBM_PEAK_GBS, 4) if hbm_mf is not None else None,
                          'traffic': t_free['hbm_bytes_per_launch'] if t_free else None,
-                         'note': ('every hit from the product-subspace lists (no index probe): model bytes are '
-                                  'keys, gathered x entries, y and term tables; measured traffic is HBM bytes')
+                         'model_GBs': round(ach_mf, 2), 'model_frac': round(ach_mf / HBM_PEAK_GBS, 4),
+                         'note': ('achieved = measured HBM bytes per launch / kernel time.  Model bytes (every hit '
+                                  'from the product-subspace lists, no index probe: keys, one gathered x entry per '
+                                  'hit, y, term tables) count the x gathers that the kernel serves from LDS, '
+                                  'so model_frac is no HBM bound')
                          if listed else
-                         'model bytes count index probes that LDS/L2 serve; measured traffic is HBM bytes'},
+                         'achieved = measured HBM bytes per launch / kernel time; the model bytes count index '
+                         'probes that LDS/L2 serve'},
         },
     }
 

@@ -318,8 +318,23 @@ int rqx_plan_drop_cache(rqx_plan *plan, int32_t precision);
  */
 int rqx_plan_cache_info(const rqx_plan *plan, int32_t precision, int64_t *out);
 
-/* Whether rqx_matvec uses a built cache (default 1); 0 forces the matrix-free kernel. */
+/*
+ * Whether rqx_matvec uses a built cache: 0 never (the matrix-free kernels), 1
+ * always, 2 (a new plan's default) the faster path: rqx_plan_build_cache times
+ * the cached and the matrix-free matvec once for each complex dtype its cache
+ * serves whose matrix-free matvec runs the product kernel (every problem fully
+ * listed), and keeps the matrix-free kernel where it was faster (real dtypes keep
+ * the cache).  Both paths give the same bits.  No reference counterpart: cache_strategy picks the cache in
+ * rqutils/sqd.py:343-407 and this refines that choice.
+ */
 int rqx_plan_use_cache(rqx_plan *plan, int32_t enable);
+
+/*
+ * The kernel rqx_matvec runs for dtype now: *path = 2 the hit cache, 1 the
+ * product kernel (matrix-free, every hit from the lists), 0 the matrix-free tile
+ * kernels.  No reference counterpart (reporting only).
+ */
+int rqx_plan_matvec_path(const rqx_plan *plan, int32_t dtype, int32_t *path);
 
 /* doubles of per-problem work space of rqx_locg_step, and the slot of it that the
  * step reads but never writes: the scale of the relative residual (below) */

@@ -34,7 +34,7 @@ EXPORTS = (
     'rqx_eigenpair_small_host', 'rqx_sort_unique_u64', 'rqx_grad_coef', 'rqx_expect_terms',
     'rqx_plan_update_coef', 'rqx_plan_create_batch', 'rqx_plan_get_problem', 'rqx_plan_set_launch',
     'rqx_eigenpair_batch', 'rqx_batch_dots', 'rqx_batch_combine', 'rqx_plan_build_cache',
-    'rqx_plan_drop_cache', 'rqx_plan_cache_info', 'rqx_plan_use_cache', 'rqx_locg_step',
+    'rqx_plan_drop_cache', 'rqx_plan_cache_info', 'rqx_plan_use_cache', 'rqx_plan_matvec_path', 'rqx_locg_step',
     'rqx_plan_hits', 'rqx_locg_step_static', 'rqx_full_apply', 'rqx_plan_create_device', 'rqx_pack_to_u64',
     'rqx_plan_digest',
 )
@@ -139,6 +139,7 @@ _SIGS = {
     'rqx_plan_drop_cache': (_i32, [_vp, _i32]),
     'rqx_plan_cache_info': (_i32, [_vp, _i32, _vp]),
     'rqx_plan_use_cache': (_i32, [_vp, _i32]),
+    'rqx_plan_matvec_path': (_i32, [_vp, _i32, _vp]),
     'rqx_locg_step': (_i32, [_vp, _i32, _i32, _vp, _vp, _vp, _vp, _vp, ctypes.c_double, _vp]),
     'rqx_locg_step_static': (_i32, [_vp, _i32, _i32, _vp, _vp, _vp, _vp, _vp, ctypes.c_double, _vp]),
     'rqx_full_apply': (_i32, [_vp, _vp, _vp, _i64, _i32, _vp]),

@@ -3677,7 +3677,8 @@ inline int vector_chunks(int64_t max_rows) {
 
 struct rqx_plan {
     HitCache cache[2];         // [0]: FP32 values, [1]: FP64 values
-    int use_cache = 1;
+    int use_cache = 2;         // 0: never, 1: a built cache always, 2: the faster path (cache_slower)
+    int8_t cache_slower[4] = {0, 0, 0, 0};   // per dtype: the product kernel measured faster than the cache
     int device = 0;
     int is_real = 1;
     int long_groups = 0;       // some group has a sign table, or x != 0 and TP_MIN_TERMS or more terms
@@ -4427,6 +4428,31 @@ hipError_t launch_tiles(const rqx_plan *p, const KParams &k_in, const void *x, v
     return hipGetLastError();
 }
 
+// The product kernel's LDS per workgroup for one problem (vector and coefficient
+// element bytes): the beta block's x, its G values, the partner values per alpha
+// pattern and their kinds
+size_t prod_lds_bytes(const ProbHost &ph, size_t v_bytes, size_t c_bytes) {
+    size_t b = ((static_cast<size_t>(ph.al_nlo) * v_bytes + 15) & ~size_t(15)) +
+               ((static_cast<size_t>(ph.al_rn) * c_bytes + 15) & ~size_t(15));
+    if (ph.pm_base != UINT32_MAX) b += static_cast<size_t>(ph.al_npat) * ((size_t(1) << ph.al_hsl) * c_bytes + 4);
+    return b;
+}
+
+// whether the matvec runs the product kernel: every problem fully listed and its
+// beta block within the staging budget (and the plan not on the plain kernels)
+bool product_plan(const rqx_plan *p, size_t v_bytes, size_t c_bytes) {
+    if (!p->long_groups || !p->all_full || p->probs.empty()) return false;
+    for (const ProbHost &ph : p->probs)
+        if (prod_lds_bytes(ph, v_bytes, c_bytes) > static_cast<size_t>(PROD_LDS_BYTES)) return false;
+    return true;
+}
+bool is_fp64(int32_t dtype);
+bool product_plan(const rqx_plan *p, int32_t dtype) {
+    const size_t c = is_fp64(dtype) ? 16 : 8;
+    const size_t v = dtype == RQX_C128 ? 16 : (dtype == RQX_F64 || dtype == RQX_C64) ? 8 : 4;
+    return product_plan(p, v, c);
+}
+
 // matvec kernels with the plain evaluator where no group has a sign table or
 // is long enough for term-parallel sums (other modes always carry the full one)
 template <int MODE, typename R, bool CPLX, bool REALC>
@@ -4438,15 +4464,8 @@ hipError_t launch_tp(const rqx_plan *p, const KParams &k, const void *x, void *y
         // one launch per problem
         using V = typename VT<R, CPLX>::T;
         using CC = typename C2<R>::T;
-        auto lds_bytes = [](const ProbHost &ph) {
-            size_t b = ((static_cast<size_t>(ph.al_nlo) * sizeof(V) + 15) & ~size_t(15)) +
-                       ((static_cast<size_t>(ph.al_rn) * sizeof(CC) + 15) & ~size_t(15));
-            if (ph.pm_base != UINT32_MAX) b += static_cast<size_t>(ph.al_npat) * ((size_t(1) << ph.al_hsl) * sizeof(CC) + 4);
-            return b;
-        };
-        bool prod = p->all_full;
-        for (const ProbHost &ph : p->probs) prod = prod && lds_bytes(ph) <= static_cast<size_t>(PROD_LDS_BYTES);
-        if (prod) {
+        auto lds_bytes = [](const ProbHost &ph) { return prod_lds_bytes(ph, sizeof(V), sizeof(CC)); };
+        if (product_plan(p, sizeof(V), sizeof(CC))) {
             for (size_t j = 0; j < p->probs.size(); ++j) {
                 const ProbHost &ph = p->probs[j];
                 const bool big = ph.al_nlo >= 2048;
@@ -5843,6 +5862,11 @@ hipError_t free_cache(HitCache &c) {
 
 // release the cache of one slot; caches built together are released together
 hipError_t release_cache(rqx_plan *p, int slot) {
+    for (int s = 0; s < 2; ++s)
+        if (s == slot || p->cache[slot].shared) {
+            p->cache_slower[s ? RQX_F64 : RQX_F32] = 0;
+            p->cache_slower[s ? RQX_C128 : RQX_C64] = 0;
+        }
     if (!p->cache[slot].shared) return free_cache(p->cache[slot]);
     const hipError_t e = free_cache(p->cache[1]);
     p->cache[0] = HitCache{};
@@ -6229,6 +6253,64 @@ int rqx_full_apply(const rqx_plan *p, const void *x, void *y, int64_t n_cols, in
     return RQX_OK;
 }
 
+// the matvec's kernels over the plan's rows: the cached kernel (a built cache of
+// dtype's precision) or the matrix-free kernels
+hipError_t matvec_kernels(const rqx_plan *p, const void *x, void *y, int32_t dtype, bool cached, hipStream_t st) {
+    const int slot = is_fp64(dtype) ? 1 : 0;
+    if (cached) {
+        const HitCache &c = p->cache[slot];
+        bool any_diag = false;
+        for (const ProbHost &ph : p->probs) any_diag = any_diag || ph.has_diag;
+        const void *diag = !any_diag ? nullptr
+                         : slot == 1 ? static_cast<const void *>(p->diag64)
+                                     : static_cast<const void *>(p->diag32);
+        switch (dtype) {
+            case RQX_F64: return spmv_launch<double, false>(p, c, diag, x, y, st);
+            case RQX_C128: return spmv_launch<double, true>(p, c, diag, x, y, st);
+            case RQX_F32: return spmv_launch<float, false>(p, c, diag, x, y, st);
+            default: return spmv_launch<float, true>(p, c, diag, x, y, st);
+        }
+    }
+    const KParams k = make_params(p, is_fp64(dtype));
+    return launch_dtype<MODE_MATVEC>(p, k, x, y, dtype, st);
+}
+
+// After a cache build: for each complex dtype the new caches serve whose matvec
+// would run the product kernel, both paths timed on zeros (one run after one
+// warm-up each: a few matvecs; both give the same bits) and cache_slower set where
+// the product kernel is faster.  Real dtypes keep the cache.
+hipError_t choose_paths(rqx_plan *p, bool fp32, bool fp64, hipStream_t st) {
+    std::vector<int32_t> dts;
+    if (fp32 && product_plan(p, RQX_C64)) dts.push_back(RQX_C64);
+    if (fp64 && product_plan(p, RQX_C128)) dts.push_back(RQX_C128);
+    if (dts.empty() || p->n_total == 0) return hipSuccess;
+    void *buf = nullptr;
+    const size_t bytes = static_cast<size_t>(p->n_total) * 16;
+    hipError_t e = hipMalloc(&buf, 2 * bytes);
+    if (e != hipSuccess) return e;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    e = hipEventCreate(&ev[0]);
+    if (e == hipSuccess) e = hipEventCreate(&ev[1]);
+    if (e == hipSuccess) e = fill_async(static_cast<uint32_t *>(buf), int64_t(0), static_cast<int64_t>(bytes / 4), 0u, st);
+    void *x = buf, *y = static_cast<char *>(buf) + bytes;
+    for (size_t i = 0; i < dts.size() && e == hipSuccess; ++i) {
+        float ms[2] = {0.f, 0.f};
+        for (int cached = 0; cached < 2 && e == hipSuccess; ++cached) {
+            e = matvec_kernels(p, x, y, dts[i], cached == 1, st);
+            if (e == hipSuccess) e = hipEventRecord(ev[0], st);
+            if (e == hipSuccess) e = matvec_kernels(p, x, y, dts[i], cached == 1, st);
+            if (e == hipSuccess) e = hipEventRecord(ev[1], st);
+            if (e == hipSuccess) e = hipEventSynchronize(ev[1]);
+            if (e == hipSuccess) e = hipEventElapsedTime(&ms[cached], ev[0], ev[1]);
+        }
+        if (e == hipSuccess) p->cache_slower[dts[i]] = ms[0] < ms[1] ? 1 : 0;
+    }
+    for (hipEvent_t v : ev)
+        if (v) (void)hipEventDestroy(v);
+    const hipError_t ef = hipFree(buf);
+    return e != hipSuccess ? e : ef;
+}
+
 int rqx_matvec(const rqx_plan *p, const void *x, void *y, int32_t dtype, rqx_stream_t stream) {
     g_err.clear();
     if (int rc = check_plan(p)) return rc;
@@ -6237,24 +6319,9 @@ int rqx_matvec(const rqx_plan *p, const void *x, void *y, int32_t dtype, rqx_str
     if (x == y) return fail(RQX_ERR_ARG, "x and y must not alias");
     if (int rc = check_dtype(p, dtype)) return rc;
     hipStream_t st = static_cast<hipStream_t>(stream);
-    const int slot = is_fp64(dtype) ? 1 : 0;
-    const HitCache &c = p->cache[slot];
-    if (p->use_cache && c.built && p->n_states > 0) {
-        bool any_diag = false;
-        for (const ProbHost &ph : p->probs) any_diag = any_diag || ph.has_diag;
-        const void *diag = !any_diag ? nullptr
-                         : slot == 1 ? static_cast<const void *>(p->diag64)
-                                     : static_cast<const void *>(p->diag32);
-        switch (dtype) {
-            case RQX_F64: HIP_TRY((spmv_launch<double, false>(p, c, diag, x, y, st))); break;
-            case RQX_C128: HIP_TRY((spmv_launch<double, true>(p, c, diag, x, y, st))); break;
-            case RQX_F32: HIP_TRY((spmv_launch<float, false>(p, c, diag, x, y, st))); break;
-            default: HIP_TRY((spmv_launch<float, true>(p, c, diag, x, y, st))); break;
-        }
-    } else {
-        const KParams k = make_params(p, is_fp64(dtype));
-        HIP_TRY(launch_dtype<MODE_MATVEC>(p, k, x, y, dtype, st));
-    }
+    const bool cached = p->use_cache && p->cache[is_fp64(dtype) ? 1 : 0].built && p->n_states > 0 &&
+                        !(p->use_cache == 2 && p->cache_slower[dtype]);
+    HIP_TRY(matvec_kernels(p, x, y, dtype, cached, st));
     if (p->n_total > p->n_states) {
         const int64_t a = p->n_states, b = p->n_total;
         switch (dtype) {
@@ -6597,6 +6664,8 @@ int rqx_plan_build_cache(rqx_plan *p, int32_t precision, int64_t max_bytes, rqx_
     release_scratch();
     p->cache[slot].built = true;
     if (both) p->cache[0].built = true;
+    e = choose_paths(p, slot == 0 || both, slot == 1, st);
+    if (e != hipSuccess) return fail(RQX_ERR_HIP, std::string("cache path choice: ") + hipGetErrorString(e));
     return RQX_OK;
 }
 
@@ -6636,7 +6705,19 @@ int rqx_plan_cache_info(const rqx_plan *p, int32_t precision, int64_t *out) {
 int rqx_plan_use_cache(rqx_plan *p, int32_t enable) {
     g_err.clear();
     if (!p) return fail(RQX_ERR_ARG, "plan is NULL");
-    p->use_cache = enable ? 1 : 0;
+    if (enable < 0 || enable > 2) return fail(RQX_ERR_ARG, "enable must be 0, 1 or 2");
+    p->use_cache = enable;
+    return RQX_OK;
+}
+
+int rqx_plan_matvec_path(const rqx_plan *p, int32_t dtype, int32_t *path) {
+    g_err.clear();
+    if (int rc = check_plan(p)) return rc;
+    if (!path) return fail(RQX_ERR_ARG, "path is NULL");
+    if (dtype < RQX_F64 || dtype > RQX_C64) return fail(RQX_ERR_ARG, "unknown dtype");
+    const bool cached = p->use_cache && p->cache[is_fp64(dtype) ? 1 : 0].built && p->n_states > 0 &&
+                        !(p->use_cache == 2 && p->cache_slower[dtype]);
+    *path = cached ? 2 : product_plan(p, dtype) ? 1 : 0;
     return RQX_OK;
 }
 

@@ -347,8 +347,20 @@ class Plan:
         return info
 
     def use_cache(self, enable=True):
-        """Let ``matvec`` read a built cache (default) or force the matrix-free kernel."""
-        _capi.check(_capi.lib().rqx_plan_use_cache(self._handle, int(bool(enable))))
+        """Which path ``matvec`` takes once a cache is built: ``True`` the cache always,
+        ``False`` the matrix-free kernels, ``'auto'`` (a new plan's default) the faster
+        of the two -- ``build_cache`` times both for each complex dtype the cache serves
+        whose matrix-free matvec runs the product kernel (real dtypes keep the cache).
+        Both paths give the same bits."""
+        mode = 2 if enable == 'auto' else int(bool(enable))
+        _capi.check(_capi.lib().rqx_plan_use_cache(self._handle, mode))
+
+    def matvec_path(self, dtype):
+        """The kernel ``matvec`` runs for ``dtype`` now: ``'cache'``, ``'product'`` (matrix-free,
+        every hit from the product-subspace lists) or ``'tiles'`` (matrix-free tile kernels)."""
+        out = np.zeros(1, dtype=np.int32)
+        _capi.check(_capi.lib().rqx_plan_matvec_path(self._handle, _DTYPE_CODES[dtype], _capi.ptr(out)))
+        return ('tiles', 'product', 'cache')[int(out[0])]
 
     def cached_bytes(self, dtype):
         """Bytes one cached matvec must move: the cache, the diagonal, ``x`` once and ``y``.

@@ -153,14 +153,22 @@ def test_alpha_lists_product_subspace_match_oracle():
     want2 = sqdref.matvec_u64(t2.xmask, _groups(t2), keys, v)
     assert _rel(plan.matvec(torch.from_numpy(v).to(DEV)).cpu().numpy(), want2) < RTOL64
     assert _rel(plan.matvec(torch.from_numpy(v.astype(np.complex64)).to(DEV)).cpu().numpy(), want2) < RTOL32
-    # the hit cache sums the same values in the same order: bit for bit the matrix-free result
+    # the hit cache sums the same values in the same order: bit for bit the matrix-free result;
+    # the default path ('auto') is whichever build_cache timed faster, the same bits again
     plan.build_cache(0)
     for dt in (torch.complex128, torch.complex64):
         x = torch.from_numpy(v).to(DEV).to(dt)
         plan.use_cache(False)
+        assert plan.matvec_path(dt) == 'product'
         y_free = plan.matvec(x)
         plan.use_cache(True)
+        assert plan.matvec_path(dt) == 'cache'
         assert torch.equal(plan.matvec(x), y_free), dt
+        plan.use_cache('auto')
+        assert plan.matvec_path(dt) in ('cache', 'product')
+        assert torch.equal(plan.matvec(x), y_free), dt
+    plan.drop_cache()
+    assert plan.matvec_path(torch.complex64) == 'product'
     no_product = Plan(t, keys[1:], split_bits=10).problems[0]
     assert no_product['alpha_rows'] == 0 and no_product['list_mode'] == 0
     # the plan built from device-resident states finds the same product and the same lists

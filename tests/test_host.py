@@ -212,6 +212,7 @@ def test_capi_argument_errors_without_gpu():
     assert lib.rqx_plan_drop_cache(None, 0) == _capi.RQX_ERR_ARG
     assert lib.rqx_plan_cache_info(None, 64, None) == _capi.RQX_ERR_ARG
     assert lib.rqx_plan_use_cache(None, 1) == _capi.RQX_ERR_ARG
+    assert lib.rqx_plan_matvec_path(None, _capi.RQX_C64, None) == _capi.RQX_ERR_ARG
     assert lib.rqx_plan_hits(None, None, None, None, 0, None, None) == _capi.RQX_ERR_ARG
     assert lib.rqx_locg_step(None, _capi.RQX_C128, 1, vec, None, None, None, None, 1e-8, None) == _capi.RQX_ERR_ARG
     assert lib.rqx_batch_dots(None, 1, None, None, None, _capi.RQX_C128, None) == _capi.RQX_ERR_ARG
