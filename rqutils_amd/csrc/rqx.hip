@@ -3689,6 +3689,7 @@ struct rqx_plan {
     int occupancy = 1;         // resident tile workgroups per CU (FP64 complex matvec kernel)
     mutable std::vector<std::pair<const void *, int>> occ_cache;   // per tile-kernel instance
     int n_cu = 1;
+    int lds_max = 65536;       // the device's LDS per workgroup
     int64_t n_states = 0, n_total = 0, n_terms = 0;
     int32_t n_groups = 0, n_classes = 0, n_segs = 0, n_tiles = 0;
     std::vector<ProbHost> probs;
@@ -4439,11 +4440,14 @@ size_t prod_lds_bytes(const ProbHost &ph, size_t v_bytes, size_t c_bytes) {
 }
 
 // whether the matvec runs the product kernel: every problem fully listed and its
-// beta block within the staging budget (and the plan not on the plain kernels)
+// beta block within the device's LDS per workgroup (and the plan not on the
+// plain kernels).  Blocks within PROD_LDS_BYTES leave room for two 1024-thread
+// workgroups per CU; larger ones (complex128 blocks of 5000 rows) run one.
 bool product_plan(const rqx_plan *p, size_t v_bytes, size_t c_bytes) {
     if (!p->long_groups || !p->all_full || p->probs.empty()) return false;
+    const size_t budget = static_cast<size_t>(std::max(p->lds_max, PROD_LDS_BYTES));
     for (const ProbHost &ph : p->probs)
-        if (prod_lds_bytes(ph, v_bytes, c_bytes) > static_cast<size_t>(PROD_LDS_BYTES)) return false;
+        if (prod_lds_bytes(ph, v_bytes, c_bytes) > budget) return false;
     return true;
 }
 bool is_fp64(int32_t dtype);
@@ -5658,6 +5662,7 @@ int build_plan(rqx_plan **out, const std::vector<ProblemIn> &in, int64_t n_total
     for (const ProbHost &ph : p->probs) hash_fits = hash_fits && ph.L <= HASH_MAX_KEY_BITS;
     int lds_max = 65536;
     HIP_TRY(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, p->device));
+    p->lds_max = lds_max;
     // the matvec's hit queues share the workgroup's LDS with the staged table
     const size_t hash_max_words = static_cast<size_t>(std::max(lds_max - 256 - static_cast<int>(HQ_LDS_MAX), 0)) / 4;
     if (dev) {

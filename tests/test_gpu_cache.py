@@ -21,6 +21,20 @@ def _say(msg):
     sys.stdout.flush()
 
 
+@pytest.fixture(autouse=True)
+def _cache_always(monkeypatch):
+    """These tests exercise the cached kernel itself: after a build, matvec reads the cache
+    always (the default mode may take the product kernel where the build timed it faster)."""
+    build = Plan.build_cache
+
+    def build_and_use(self, *args, **kwargs):
+        info = build(self, *args, **kwargs)
+        self.use_cache(True)
+        return info
+
+    monkeypatch.setattr(Plan, 'build_cache', build_and_use)
+
+
 def _random_problem(seed, n, t, ns, probs=(0.55, 0.15, 0.15, 0.15), drop_diagonal=False):
     rng = np.random.default_rng(seed)
     codes = workloads.random_pauli_codes(n, t, probs, rng)
