@@ -233,6 +233,30 @@ def test_product_partners_mixed_evaluation_match_oracle():
         assert torch.equal(plan.matvec(x), y_free), dt
 
 
+def test_product_kernel_blocks_beyond_64k_lds():
+    """4200 alpha x 3 beta strings: a complex128 beta block (4200 x 16 bytes of x plus its staged
+    values) needs more than 64 KiB of LDS, so the product kernel runs one workgroup per CU with the
+    device's LDS; complex64 blocks fit 64 KiB.  Both equal the oracle and the cache bit for bit."""
+    _say('product kernel, large blocks')
+    w = workloads.jw32(n_alpha=4200, n_beta=3)
+    t, keys = w.tables, w.states
+    plan = Plan(t, keys, split_bits=10)          # segments below the alpha bits
+    assert plan.problems[0]['list_mode'] == 2 and plan.problems[0]['alpha_rows'] == 4200
+    rng = np.random.default_rng(91)
+    v = rng.standard_normal(plan.n_total) + 1j * rng.standard_normal(plan.n_total)
+    want = sqdref.matvec_u64(t.xmask, _groups(t), keys, v)
+    free = {}
+    for dt, tol in ((torch.complex128, RTOL64), (torch.complex64, RTOL32)):
+        assert plan.matvec_path(dt) == 'product'
+        free[dt] = plan.matvec(torch.from_numpy(v).to(DEV).to(dt))
+        assert _rel(free[dt].cpu().numpy(), want) < tol, dt
+    plan.build_cache(0)
+    plan.use_cache(True)
+    for dt, y in free.items():
+        assert torch.equal(plan.matvec(torch.from_numpy(v).to(DEV).to(dt)), y), dt
+    plan.use_cache('auto')
+
+
 @pytest.mark.parametrize('name', ['small_10q', 'filler_12q'])
 def test_golden_fixtures(name):
     _say(f'golden {name}')
