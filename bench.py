@@ -400,7 +400,9 @@ def line_for(w, plan, leg, dt_name, steps, warmup, traffic):
         'ms_per_step': round(ms, 4),
         'config': {
             'workload': w.name,
-            'path': 'hit cache (cache_strategy (1, *), the run_sqd default); matrix-free kernel in matrix_free',
+            'path': ('hit cache kernel (cache_strategy (1, *), the run_sqd default; timed with the cache forced: '
+                     'the default mode runs the kernel named by default_path_with_cache, the faster one the '
+                     'cache build timed, same bits); matrix-free kernel in matrix_free'),
             'n_states': plan.n_states, 'n_groups': plan.info['n_groups'], 'n_terms': plan.info['n_terms'],
             'hits_per_matvec': leg['counts']['hits'], 'segment_probes_per_matvec': leg['counts']['probes'],
             'split_bits': plan.info['split_bits'],
