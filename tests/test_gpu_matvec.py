@@ -233,6 +233,39 @@ def test_product_partners_mixed_evaluation_match_oracle():
         assert torch.equal(plan.matvec(x), y_free), dt
 
 
+def test_product_kernel_real_coefficient_update_matches_oracle():
+    """A real coefficient update on a fully listed product plan rewrites the alpha-list entries,
+    their one-word form for real coefficients (what the product kernel reads), the G tables and
+    the beta lists: the matrix-free matvec equals the oracle with the new coefficients, and the
+    cache built afterwards equals it bit for bit."""
+    _say('product kernel, real coefficient update')
+    from rqutils_amd import _capi
+    w = workloads.jw32(n_alpha=60, n_beta=50)
+    t, keys = w.tables, w.states
+    plan = Plan(t, keys, split_bits=10)
+    assert plan.problems[0]['list_mode'] == 2 and plan.is_real
+    rng = np.random.default_rng(95)
+    coef = rng.standard_normal(plan.info['n_terms']).astype(np.complex128)
+    ct = torch.from_numpy(coef).to(DEV).contiguous()
+    _capi.check(_capi.lib().rqx_plan_update_coef(plan._handle, torch.view_as_real(ct).data_ptr(), 1,
+                                                 _capi.current_stream(plan.device)))
+    t2 = TermTables(t.xmask, t.grp_off, t.zmask, coef, t.num_qubits)
+    v = rng.standard_normal(plan.n_total) + 1j * rng.standard_normal(plan.n_total)
+    want = sqdref.matvec_u64(t2.xmask, _groups(t2), keys, v)
+    free = {}
+    for dt, tol in ((torch.complex128, RTOL64), (torch.complex64, RTOL32)):
+        assert plan.matvec_path(dt) == 'product'
+        free[dt] = plan.matvec(torch.from_numpy(v).to(DEV).to(dt))
+        assert _rel(free[dt].cpu().numpy(), want) < tol, dt
+    want_re = sqdref.matvec_u64(t2.xmask, _groups(t2), keys, v.real)
+    assert _rel(plan.matvec(torch.from_numpy(v.real.astype(np.float32)).to(DEV)).cpu().numpy(), want_re.real) < RTOL32
+    plan.build_cache(0)
+    plan.use_cache(True)
+    for dt, y in free.items():
+        assert torch.equal(plan.matvec(torch.from_numpy(v).to(DEV).to(dt)), y), dt
+    plan.use_cache('auto')
+
+
 def test_product_kernel_blocks_beyond_64k_lds():
     """4200 alpha x 3 beta strings: a complex128 beta block (4200 x 16 bytes of x plus its staged
     values) needs more than 64 KiB of LDS, so the product kernel runs one workgroup per CU with the
